@@ -1,0 +1,144 @@
+/*
+ * iscg.h -- C ABI of the MI355X image-formation library (libiscg.so, gfx950).
+ *
+ * The entry points sit under the reference renderer's existing C++ seams (Sim/ = adapter/inSiliScope/Simulation/):
+ * each one names the reference interface it replaces. Plain pointers and sizes only: no C++ or torch types.
+ *
+ * Conventions (following the core's C ABI, core/include/insiliscope/insiliscope.h:1-60):
+ *   - every call returns int32_t: 0 on success, < 0 on failure; iscg_last_error() then says why;
+ *   - a context is one HIP device and one HIP stream; calls on one context are serialised by the caller
+ *     (not re-entrant), as GpuSimulator is one instance per thread (Sim/GpuSimD3D11.h:28-30);
+ *   - array arguments may be host memory or device memory of the context's device: device pointers are used in
+ *     place (the fast path: inputs resident in HBM), host pointers are staged through device buffers the context
+ *     owns. Calls return after their results are written (host outputs) or enqueued on the context's stream
+ *     (device outputs: iscg_synchronize, or events on iscg_stream(), orders later work);
+ *   - the arithmetic matches the reference CPU path: the counter RNG and camera noise in FP64 (bit-exact ADU),
+ *     the splat in FP64 accumulation (bit-exact photon images), the PSF and mean field in FP32 / split precision
+ *     checked against FP64 to <= 1e-5 relative L2.
+ */
+#ifndef ISCG_H
+#define ISCG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISCG_ABI_VERSION 1
+
+typedef struct iscg_context iscg_context;
+
+/* ---- context ------------------------------------------------------------------------------------------------- */
+
+int32_t iscg_abi_version(void);
+/* A context on HIP device `device` with its own default-priority stream. */
+int32_t iscg_create(int32_t device, iscg_context** out);
+void iscg_destroy(iscg_context* h);
+/* The last failure's message (empty after a success); valid until the next call on h. NULL h: creation errors. */
+const char* iscg_last_error(const iscg_context* h);
+/* Waits for everything enqueued on the context's stream. */
+int32_t iscg_synchronize(iscg_context* h);
+/* The context's hipStream_t (as void*), for callers that time or order work with HIP events. */
+void* iscg_stream(iscg_context* h);
+/* Test hook: the next `count` entry-point calls on h fail with `message` before doing any work, as a device error
+ * would (a caller's CPU fallback, Sim/ScopeMovie.cpp:7634-7645, is exercised without faulting a GPU). */
+int32_t iscg_inject_failure(iscg_context* h, int32_t count, const char* message);
+
+/* ---- camera noise ---------------------------------------------------------------------------------------------- */
+
+/* sim::CameraNoiseParams (Sim/SMLMNoise.h:35-50). */
+typedef struct
+{
+   double quantumEfficiency;
+   double darkCurrentElectrons;
+   double gainPhotonsPerAdu;
+   double readNoiseElectrons;
+   int32_t emccd;
+   int32_t bitDepth;
+   double emGain;
+   double cicElectrons;
+} iscg_camera;
+
+/* Replaces sim::ApplyNoiseChain (Sim/SMLMNoise.h:136, Sim/SMLMNoise.cpp:142-216), batched over frames.
+ * photons: nFrames x height x width floats; frame k's stream is (seed, frames[k], pixel + pixelBase).
+ * offset / gain / readNoise: width*height maps or NULL (NULL: the scalar of cam; a NULL offset map is offset 0,
+ * as an empty PixelOffsetMap). out: nFrames x height x width uint16 ADU. */
+int32_t iscg_noise(iscg_context* h, const float* photons, uint32_t width, uint32_t height, const iscg_camera* cam,
+                   const float* offset, const float* gain, const float* readNoise, uint32_t seed,
+                   const uint32_t* frames, uint32_t nFrames, uint32_t pixelBase, uint16_t* out);
+
+/* ---- blink splat (+ noise) -------------------------------------------------------------------------------------- */
+
+/* One kernel stack the splat reads (a PsfKernelCache, Sim/PsfGeneratorBridge.h:243-293): its sum-1 planes; the
+ * library builds the block sums (PsfKernelPlanes::Sums, Sim/PsfGeneratorBridge.cpp:1069-1088) on the device. */
+typedef struct
+{
+   const float* planes;   /* nz x size x size, row-major, each summing to 1 */
+   const int16_t* spanLo; /* halo cut (PsfHaloSpans::lo/hi, nz x (2 camRad + 1)), or NULL: the whole square */
+   const int16_t* spanHi;
+   double zStepNm;
+   uint64_t serial;       /* PsfKernelCache::SplatSerial(); equal non-zero serials skip the re-upload */
+   int32_t nz;
+   int32_t size;          /* sizeOversampled = 2 halfWidthOversampled + 1 */
+   int32_t oversampling;
+   int32_t interpMode;    /* PsfInterpMode: 0 nearest, 1 linear, 2 cubic (Catmull-Rom on block sums) */
+} iscg_kernel_view;
+
+/* One emitter of a frame as RenderPhotonImage places it (sim::FrameEmitter, Sim/SMLMSimulation.h:158): position in
+ * camera pixels, photons for the frame, its z plane (NearestZIndex) and kernel index. The splat setup is computed
+ * on the device in FP64, so frames are bit-exact with the CPU. 32 bytes. */
+typedef struct
+{
+   double xPx, yPx, photons;
+   int32_t zIndex, kernel;
+} iscg_frame_emitter;
+
+/* sim::GpuSplatEmitter (Sim/GpuSimD3D11.h:50-59), the 64-byte wire format CollectGpuEmitters makes
+ * (Sim/SMLMSimulation.cpp:219): setup done on the host, float weights. */
+typedef struct
+{
+   int32_t x0, y0, bx, by;
+   int32_t plane; /* z plane | kernel index << 16 */
+   int32_t nTaps;
+   float photons;
+   int32_t cutRadius2;
+   float wx[4];
+   float wy[4];
+} iscg_splat_emitter;
+
+/* Replaces SplatGpuHost::SetKernels / GpuSimulator::SetKernels (Sim/SplatGpu.h:50, Sim/GpuSimD3D11.h:75):
+ * up to 4 kernels, resident until replaced. */
+int32_t iscg_splat_set_kernels(iscg_context* h, const iscg_kernel_view* kernels, int32_t n);
+
+/* Replaces SplatGpuHost::SetStatic / GpuSimulator::SetStatic (Sim/SplatGpu.h:53, Sim/GpuSimD3D11.h:83): frame size,
+ * the window's first sensor pixel (noise stream), camera maps (width*height or NULL) and the background: the map
+ * (photons/pixel/frame, NULL: flatBackground everywhere). key != 0 equal to the last call's: kept as they are. */
+int32_t iscg_splat_set_static(iscg_context* h, uint32_t width, uint32_t height, uint32_t pixelBase,
+                              const float* offset, const float* gain, const float* readNoise, const float* background,
+                              double flatBackground, uint64_t key);
+
+/* Replaces SplatGpuHost::RenderFrames / GpuSimulator::RenderFrames (Sim/SplatGpu.h:57, Sim/GpuSimD3D11.h:99) and,
+ * bit for bit, RenderPhotonImage (Sim/SMLMSimulation.cpp:103) + ApplyNoiseChain per frame.
+ * ems: nEmitters records, frame k's are [emStart[k], emStart[k+1]) in event order (nFrames + 1 offsets).
+ * frames: the frame ids of the noise stream (seed, frames[k], pixel). bgScales: per-frame background factor or NULL
+ * (1). extras: nFrames x width*height photons the blinks are added onto instead of the background, or NULL.
+ * minPhotons: blinks below it are not splatted (blink-min-photons; 0: all). outAdu (NULL: no noise) and outPhotons
+ * (the frame before noise, NULL: not written): nFrames x width*height. */
+int32_t iscg_splat_render_frames(iscg_context* h, const iscg_frame_emitter* ems, uint32_t nEmitters,
+                                 const uint32_t* emStart, uint32_t nFrames, const uint32_t* frames,
+                                 const float* bgScales, const float* extras, const iscg_camera* cam, uint32_t seed,
+                                 double minPhotons, uint16_t* outAdu, float* outPhotons);
+
+/* The same for GpuSplatEmitter records (the reference's GPU wire format). */
+int32_t iscg_splat_render_records(iscg_context* h, const iscg_splat_emitter* ems, uint32_t nEmitters,
+                                  const uint32_t* emStart, uint32_t nFrames, const uint32_t* frames,
+                                  const float* bgScales, const float* extras, const iscg_camera* cam, uint32_t seed,
+                                  uint16_t* outAdu, float* outPhotons);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

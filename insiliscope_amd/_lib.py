@@ -1,0 +1,82 @@
+"""ctypes binding of libiscg.so (include/iscg.h). Loading fails loudly when the library is missing: there is no
+CPU fallback behind this package."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libiscg.so")
+
+
+class IscgError(RuntimeError):
+    """A failed libiscg call (the message is iscg_last_error's)."""
+
+
+class Camera(C.Structure):
+    """iscg_camera = sim::CameraNoiseParams (Sim/SMLMNoise.h:35-50)."""
+
+    _fields_ = [("quantumEfficiency", C.c_double), ("darkCurrentElectrons", C.c_double),
+                ("gainPhotonsPerAdu", C.c_double), ("readNoiseElectrons", C.c_double),
+                ("emccd", C.c_int32), ("bitDepth", C.c_int32), ("emGain", C.c_double),
+                ("cicElectrons", C.c_double)]
+
+
+class KernelView(C.Structure):
+    """iscg_kernel_view: one kernel stack for the splat."""
+
+    _fields_ = [("planes", C.c_void_p), ("spanLo", C.c_void_p), ("spanHi", C.c_void_p), ("zStepNm", C.c_double),
+                ("serial", C.c_uint64), ("nz", C.c_int32), ("size", C.c_int32), ("oversampling", C.c_int32),
+                ("interpMode", C.c_int32)]
+
+
+class PsfRequest(C.Structure):
+    """iscg_psf_request: the numeric fields of sim::PsfGeneratorRequest (Sim/PsfGeneratorBridge.h:91-170)."""
+
+    _fields_ = [("wavelengthNm", C.c_double), ("na", C.c_double), ("immersionIndex", C.c_double),
+                ("sampleIndex", C.c_double), ("workingDistanceUm", C.c_double), ("sampleDepthNm", C.c_double),
+                ("pixelSizeNm", C.c_double), ("zStepNm", C.c_double), ("maskWaist", C.c_double),
+                ("zernike", C.c_double * 28), ("maskType", C.c_int32), ("maskModes", C.c_int32),
+                ("oversampling", C.c_int32), ("kernelHalfWidthPx", C.c_int32), ("nz", C.c_int32),
+                ("pupilSamples", C.c_int32), ("interpMode", C.c_int32), ("reserved", C.c_int32)]
+
+
+_lib = None
+
+I32, U32, U64, D, P = C.c_int32, C.c_uint32, C.c_uint64, C.c_double, C.c_void_p
+
+# name -> (restype, argtypes); every symbol include/iscg.h declares
+SIGNATURES = {
+    "iscg_abi_version": (I32, []),
+    "iscg_create": (I32, [I32, C.POINTER(P)]),
+    "iscg_destroy": (None, [P]),
+    "iscg_last_error": (C.c_char_p, [P]),
+    "iscg_synchronize": (I32, [P]),
+    "iscg_stream": (P, [P]),
+    "iscg_inject_failure": (I32, [P, I32, C.c_char_p]),
+    "iscg_noise": (I32, [P, P, U32, U32, C.POINTER(Camera), P, P, P, U32, P, U32, U32, P]),
+    "iscg_splat_set_kernels": (I32, [P, C.POINTER(KernelView), I32]),
+    "iscg_splat_set_static": (I32, [P, U32, U32, U32, P, P, P, P, D, U64]),
+    "iscg_splat_render_frames": (I32, [P, P, U32, P, U32, P, P, P, C.POINTER(Camera), U32, D, P, P]),
+    "iscg_splat_render_records": (I32, [P, P, U32, P, U32, P, P, P, C.POINTER(Camera), U32, P, P]),
+}
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def lib():
+    """The loaded library (raises if libiscg.so was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise IscgError(f"{LIB_PATH} is missing: build it first (python -m insiliscope_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            if hasattr(L, name):
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+        _lib = L
+    return _lib

@@ -1,0 +1,187 @@
+// context.hip -- the iscg context: device selection, its stream, errors, host-argument staging.
+#include "iscg_internal.h"
+
+#include <cstdio>
+
+namespace iscg {
+
+void destroy_splat(iscg_context* h);
+void destroy_psf(iscg_context* h);
+void destroy_wf(iscg_context* h);
+void destroy_roi(iscg_context* h);
+
+int32_t fail(iscg_context* h, const std::string& msg)
+{
+   if (h)
+      h->err = msg;
+   return -1;
+}
+
+bool ok(iscg_context* h, hipError_t e, const char* what)
+{
+   if (e == hipSuccess)
+      return true;
+   fail(h, std::string(what) + ": " + hipGetErrorString(e));
+   return false;
+}
+
+int32_t begin(iscg_context* h)
+{
+   if (!h)
+      return -1;
+   h->err.clear();
+   if (h->injectCount > 0)
+   {
+      --h->injectCount;
+      return fail(h, h->injectMessage);
+   }
+   if (!ok(h, hipSetDevice(h->device), "hipSetDevice"))
+      return -1;
+   return 0;
+}
+
+bool is_device_ptr(const void* p)
+{
+   if (!p)
+      return false;
+   hipPointerAttribute_t a;
+   const hipError_t e = hipPointerGetAttributes(&a, p);
+   if (e != hipSuccess)
+   {
+      (void)hipGetLastError();
+      return false;
+   }
+   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.isManaged;
+}
+
+const void* stage_in(iscg_context* h, int slot, const void* p, size_t bytes)
+{
+   if (!p || bytes == 0 || is_device_ptr(p))
+      return p;
+   DevBuf& b = h->stage[slot];
+   if (!ok(h, b.reserve(bytes), "staging allocation"))
+      return nullptr;
+   if (!ok(h, hipMemcpyAsync(b.p, p, bytes, hipMemcpyHostToDevice, h->stream), "host-to-device copy"))
+      return nullptr;
+   return b.p;
+}
+
+void* stage_out(iscg_context* h, int slot, void* p, size_t bytes)
+{
+   if (!p || bytes == 0 || is_device_ptr(p))
+      return p;
+   DevBuf& b = h->stage[slot];
+   if (!ok(h, b.reserve(bytes), "staging allocation"))
+      return nullptr;
+   return b.p;
+}
+
+bool finish_out(iscg_context* h, void* user, const void* dev, size_t bytes)
+{
+   if (!user || bytes == 0 || user == dev)
+      return true;
+   if (!ok(h, hipMemcpyAsync(user, dev, bytes, hipMemcpyDeviceToHost, h->stream), "device-to-host copy"))
+      return false;
+   return ok(h, hipStreamSynchronize(h->stream), "stream synchronize");
+}
+
+}  // namespace iscg
+
+using namespace iscg;
+
+namespace {
+std::string g_createError;
+}
+
+extern "C" {
+
+int32_t iscg_abi_version(void) { return ISCG_ABI_VERSION; }
+
+int32_t iscg_create(int32_t device, iscg_context** out)
+{
+   if (!out)
+      return -1;
+   *out = nullptr;
+   int n = 0;
+   hipError_t e = hipGetDeviceCount(&n);
+   if (e != hipSuccess || n <= 0)
+   {
+      g_createError = std::string("no HIP device: ") + hipGetErrorString(e);
+      return -1;
+   }
+   if (device < 0 || device >= n)
+   {
+      g_createError = "device index out of range";
+      return -1;
+   }
+   e = hipSetDevice(device);
+   if (e != hipSuccess)
+   {
+      g_createError = std::string("hipSetDevice: ") + hipGetErrorString(e);
+      return -1;
+   }
+   hipDeviceProp_t prop;
+   e = hipGetDeviceProperties(&prop, device);
+   if (e != hipSuccess)
+   {
+      g_createError = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+      return -1;
+   }
+   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+   {
+      g_createError = std::string("libiscg is built for gfx950; device is ") + prop.gcnArchName;
+      return -1;
+   }
+   iscg_context* h = new iscg_context();
+   h->device = device;
+   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+   if (e != hipSuccess)
+   {
+      g_createError = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+      delete h;
+      return -1;
+   }
+   g_createError.clear();
+   *out = h;
+   return 0;
+}
+
+void iscg_destroy(iscg_context* h)
+{
+   if (!h)
+      return;
+   (void)hipSetDevice(h->device);
+   (void)hipStreamSynchronize(h->stream);
+   destroy_splat(h);
+   destroy_psf(h);
+   destroy_wf(h);
+   destroy_roi(h);
+   for (auto& b : h->stage)
+      b.release();
+   (void)hipStreamDestroy(h->stream);
+   delete h;
+}
+
+const char* iscg_last_error(const iscg_context* h) { return h ? h->err.c_str() : g_createError.c_str(); }
+
+int32_t iscg_synchronize(iscg_context* h)
+{
+   if (!h)
+      return -1;
+   if (!ok(h, hipSetDevice(h->device), "hipSetDevice"))
+      return -1;
+   return ok(h, hipStreamSynchronize(h->stream), "stream synchronize") ? 0 : -1;
+}
+
+void* iscg_stream(iscg_context* h) { return h ? (void*)h->stream : nullptr; }
+
+int32_t iscg_inject_failure(iscg_context* h, int32_t count, const char* message)
+{
+   if (!h)
+      return -1;
+   h->injectCount = count < 0 ? 0 : count;
+   h->injectMessage = message ? message : "injected failure";
+   return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,612 @@
+// splat.hip -- the blink splat + camera noise for batches of frames with uneven blink counts.
+//
+// Replaces SplatGpuHost / GpuSimulator (Sim/SplatGpu.h:40-77, Sim/GpuSimD3D11.h:61-105) and, bit for bit, the CPU
+// path they stand in for: RenderPhotonImage (Sim/SMLMSimulation.cpp:103-201) + ApplyNoiseChain.
+//
+// Layout: each kernel's block sums (Sim/PsfGeneratorBridge.cpp:1021-1050) stay resident in HBM, nz x bw x bw
+// float row-major (C4: 81 x 68^2 = 1.5 MB, L2-resident). A batch is a flat emitter array in frame order with
+// per-frame offsets (emStart), so frames of any blink count share one launch.
+//
+// Passes:
+//   1. setup: one lane per emitter -- SplatSetup (Sim/PsfGeneratorBridge.cpp:1382-1422) in FP64 into a 112-byte
+//      record, and the emitter's centre pixel into a compact 8-byte array for the tile scans;
+//   2. tiles: one workgroup per (frame, 32x32 tile), grid-stride. The tile's photon image lives in LDS, starting
+//      from the background (or the caller's extra image). The frame's emitters are scanned in chunks of 256; a
+//      wave ballot + prefix keeps the hits in frame order; each hit's patch (clipped to the tile) is spread over
+//      the workgroup's lanes, one pixel per lane, FP64 taps as SplatKernel.inl's PixelTaps, one float add per
+//      pixel per emitter in frame order (a barrier between emitters), so every pixel sums the same terms in the
+//      same order as the CPU. Then the noise chain runs per pixel and the ADU are written once.
+// HBM traffic per frame: the ADU (2 B/px), optional photon image (4 B/px), emitter records, the maps; the block
+// sums come from L2.
+#include "iscg_internal.h"
+
+#include <climits>
+
+namespace iscg {
+
+bool launch_noise(iscg_context* h, const float* photons, uint16_t* out, uint32_t npx, uint32_t nFrames,
+                  const uint32_t* frames, const float* offset, const float* gain, const float* readN,
+                  const CamParams& cp, uint32_t seed, uint32_t pixelBase);
+
+constexpr int kMaxKernels = 4;
+constexpr int kTile = 32;
+constexpr int kBlock = 256;
+
+struct KernelDev
+{
+   DevBuf sums;   // nz * bw * bw floats
+   DevBuf spans;  // 2 * nz * (2 camRad + 1) int16 (lo block then hi block), or empty
+   int bw = 0, os = 1, camRad = 0, size = 0, nz = 0, interp = 0;
+   bool hasSpans = false;
+   uint64_t serial = 0;
+};
+
+struct SplatState
+{
+   KernelDev k[kMaxKernels];
+   int nk = 0;
+   uint32_t w = 0, h = 0, pixelBase = 0;
+   DevBuf offset, gain, readN, bg;
+   bool hasOffset = false, hasGain = false, hasRead = false, hasBg = false, haveStatic = false;
+   double flatBg = 0.0;
+   uint64_t key = 0;
+   DevBuf setup, pos, photons;
+};
+
+void destroy_splat(iscg_context* h)
+{
+   delete h->splat;
+   h->splat = nullptr;
+}
+
+static SplatState& state(iscg_context* h)
+{
+   if (!h->splat)
+      h->splat = new SplatState();
+   return *h->splat;
+}
+
+// The per-emitter record the tile pass reads (SplatSetup + photons + plane).
+struct alignas(16) SetupRec
+{
+   double wx[4], wy[4];
+   double photons;
+   int32_t x0, y0, bx, by;
+   int32_t nTaps, kernel, plane, cutR2;
+   int32_t pad[2];
+};
+static_assert(sizeof(SetupRec) == 112, "SetupRec layout");
+
+struct KernelTable
+{
+   const float* sums[kMaxKernels];
+   const int16_t* lo[kMaxKernels];
+   const int16_t* hi[kMaxKernels];
+   int bw[kMaxKernels], os[kMaxKernels], camRad[kMaxKernels], size[kMaxKernels], nz[kMaxKernels],
+      interp[kMaxKernels];
+   int nk, maxCamRad;
+};
+
+static KernelTable table_of(const SplatState& s)
+{
+   KernelTable t{};
+   t.nk = s.nk;
+   t.maxCamRad = 0;
+   for (int i = 0; i < s.nk; ++i)
+   {
+      const KernelDev& k = s.k[i];
+      t.sums[i] = k.sums.as<float>();
+      const size_t nSpan = (size_t)k.nz * (2 * k.camRad + 1);
+      t.lo[i] = k.hasSpans ? k.spans.as<int16_t>() : nullptr;
+      t.hi[i] = k.hasSpans ? k.spans.as<int16_t>() + nSpan : nullptr;
+      t.bw[i] = k.bw;
+      t.os[i] = k.os;
+      t.camRad[i] = k.camRad;
+      t.size[i] = k.size;
+      t.nz[i] = k.nz;
+      t.interp[i] = k.interp;
+      t.maxCamRad = k.camRad > t.maxCamRad ? k.camRad : t.maxCamRad;
+   }
+   return t;
+}
+
+// Block sums of one plane (Sim/PsfGeneratorBridge.cpp:1021-1050): B[ai][bi] = sum over the os x os block at
+// (ai - (os-1), bi - (os-1)), FP64 accumulation in the reference's order (row sums, then column sums of them).
+__global__ void block_sums_kernel(const float* __restrict__ planes, float* __restrict__ out, int n, int os, int nz)
+{
+   const int off = os - 1, bw = n + off;
+   const size_t total = (size_t)nz * bw * bw;
+   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x)
+   {
+      const int z = (int)(idx / ((size_t)bw * bw));
+      const int rem = (int)(idx - (size_t)z * bw * bw);
+      const int ai = rem / bw, bi = rem - ai * bw;
+      const float* P = planes + (size_t)z * n * n;
+      const int a = ai - off, y0 = a > 0 ? a : 0, y1 = (a + off) < (n - 1) ? (a + off) : (n - 1);
+      const int b = bi - off, x0 = b > 0 ? b : 0, x1 = (b + off) < (n - 1) ? (b + off) : (n - 1);
+      double acc = 0.0;
+      for (int y = y0; y <= y1; ++y)
+      {
+         double hs = 0.0;
+         for (int x = x0; x <= x1; ++x)
+            hs += P[(size_t)y * n + x];
+         acc += hs;
+      }
+      out[idx] = (float)acc;
+   }
+}
+
+__device__ __forceinline__ void catmull_rom(double f, double* w)
+{
+   const double f2 = f * f, f3 = f2 * f;
+   w[0] = -0.5 * f3 + f2 - 0.5 * f;
+   w[1] = 1.5 * f3 - 2.5 * f2 + 1.0;
+   w[2] = -1.5 * f3 + 2.0 * f2 + 0.5 * f;
+   w[3] = 0.5 * f3 - 0.5 * f2;
+}
+
+constexpr int kFar = INT_MIN / 4;  // a centre no tile reaches
+
+// Pass 1 for iscg_frame_emitter input: SplatSetup in FP64 (Sim/PsfGeneratorBridge.cpp:1382-1422), validity as
+// PlanSplat (:1424-1433) and the min-photons filter (Sim/SMLMSimulation.cpp:151-155).
+__global__ void setup_exact_kernel(const iscg_frame_emitter* __restrict__ ems, uint32_t n, KernelTable kt,
+                                   double minPhotons, SetupRec* __restrict__ recs, int2* __restrict__ pos)
+{
+   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+   {
+      const iscg_frame_emitter em = ems[e];
+      SetupRec r;
+      for (int i = 0; i < 4; ++i)
+      {
+         r.wx[i] = i == 0 ? 1.0 : 0.0;
+         r.wy[i] = i == 0 ? 1.0 : 0.0;
+      }
+      r.photons = em.photons;
+      r.kernel = em.kernel;
+      r.plane = em.zIndex;
+      r.cutR2 = INT_MAX;
+      r.pad[0] = r.pad[1] = 0;
+      // positions beyond +-2^28 px cannot reach a frame (and would overflow the integer pixel arithmetic)
+      bool valid = em.kernel >= 0 && em.kernel < kt.nk && em.photons > 0.0 &&
+                   !(minPhotons > 0.0 && em.photons < minPhotons) && fabs(em.xPx) < 268435456.0 &&
+                   fabs(em.yPx) < 268435456.0;
+      if (valid)
+         valid = em.zIndex >= 0 && em.zIndex < kt.nz[em.kernel];
+      if (valid)
+      {
+         const int os = kt.os[em.kernel], interp = kt.interp[em.kernel];
+         const double kc = (kt.size[em.kernel] - 1) / 2.0;
+         r.x0 = (int)floor(em.xPx + 0.5);
+         r.y0 = (int)floor(em.yPx + 0.5);
+         const double tx = kc + (r.x0 - em.xPx - 0.5) * os + 0.5;
+         const double ty = kc + (r.y0 - em.yPx - 0.5) * os + 0.5;
+         if (interp == 0)
+         {
+            r.bx = (int)floor(tx + 0.5);
+            r.by = (int)floor(ty + 0.5);
+            r.nTaps = 1;
+         }
+         else
+         {
+            const int bx = (int)floor(tx), by = (int)floor(ty);
+            const double fx = tx - bx, fy = ty - by;
+            if (interp == 2)
+            {
+               r.bx = bx - 1;
+               r.by = by - 1;
+               r.nTaps = 4;
+               catmull_rom(fx, r.wx);
+               catmull_rom(fy, r.wy);
+            }
+            else
+            {
+               r.bx = bx;
+               r.by = by;
+               r.nTaps = 2;
+               r.wx[0] = 1.0 - fx;
+               r.wx[1] = fx;
+               r.wy[0] = 1.0 - fy;
+               r.wy[1] = fy;
+            }
+         }
+      }
+      else
+      {
+         r.x0 = r.y0 = kFar;
+         r.bx = r.by = 0;
+         r.nTaps = 0;
+      }
+      recs[e] = r;
+      pos[e] = make_int2(r.x0, r.y0);
+   }
+}
+
+// Pass 1 for the 64-byte GpuSplatEmitter records (Sim/GpuSimD3D11.h:50-59, made by CollectGpuEmitters,
+// Sim/SMLMSimulation.cpp:219-256): setup already done on the host with float weights.
+__global__ void setup_records_kernel(const iscg_splat_emitter* __restrict__ ems, uint32_t n, KernelTable kt,
+                                     SetupRec* __restrict__ recs, int2* __restrict__ pos)
+{
+   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+   {
+      const iscg_splat_emitter em = ems[e];
+      SetupRec r;
+      for (int i = 0; i < 4; ++i)
+      {
+         r.wx[i] = (double)em.wx[i];
+         r.wy[i] = (double)em.wy[i];
+      }
+      r.photons = (double)em.photons;
+      r.kernel = (int)((uint32_t)em.plane >> 16);
+      r.plane = em.plane & 0xffff;
+      r.cutR2 = em.cutRadius2;
+      r.pad[0] = r.pad[1] = 0;
+      r.bx = em.bx;
+      r.by = em.by;
+      r.nTaps = em.nTaps == 1 || em.nTaps == 2 ? em.nTaps : 4;
+      const bool valid = r.kernel < kt.nk && r.plane < kt.nz[r.kernel] && r.photons > 0.0 && abs(em.x0) < (1 << 28) &&
+                         abs(em.y0) < (1 << 28) && abs(em.bx) < (1 << 28) && abs(em.by) < (1 << 28);
+      r.x0 = valid ? em.x0 : kFar;
+      r.y0 = valid ? em.y0 : kFar;
+      if (!valid)
+         r.nTaps = 0;
+      recs[e] = r;
+      pos[e] = make_int2(r.x0, r.y0);
+   }
+}
+
+struct TileArgs
+{
+   const SetupRec* recs;
+   const int2* pos;
+   const uint32_t* emStart;   // nFrames + 1, relative to the emitter array
+   const uint32_t* frames;    // nFrames frame ids (noise stream)
+   const float* bgScales;     // nFrames or null (1)
+   const float* extras;       // nFrames x npx or null
+   const float* bgMap;        // npx or null
+   const float* offset;
+   const float* gain;
+   const float* readN;
+   uint16_t* outAdu;          // nFrames x npx or null
+   float* outPhotons;         // nFrames x npx or null
+   double flatBg;
+   uint32_t w, h, tilesX, nTiles, nFrames, nEmitters, seed, pixelBase;
+};
+
+template <bool NOISE, bool EMCCD>
+__global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelTable kt, CamParams cp)
+{
+   __shared__ float img[kTile * kTile];
+   __shared__ int list[kBlock];
+   __shared__ int waveHits[kBlock / 64];
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const size_t npx = (size_t)a.w * a.h;
+   const size_t items = (size_t)a.nTiles * a.nFrames;
+   for (size_t item = blockIdx.x; item < items; item += gridDim.x)
+   {
+      const uint32_t f = (uint32_t)(item / a.nTiles);
+      const uint32_t t = (uint32_t)(item - (size_t)f * a.nTiles);
+      const int tx0 = (int)(t % a.tilesX) * kTile, ty0 = (int)(t / a.tilesX) * kTile;
+      const float* extra = a.extras ? a.extras + (size_t)f * npx : nullptr;
+      const double bgScale = a.bgScales ? (double)a.bgScales[f] : 1.0;
+      const bool flat = a.bgMap == nullptr && bgScale == 1.0;
+      // the tile's starting photons: the caller's extra image, else the background (RenderPhotonImage :133-149)
+      for (int p = tid; p < kTile * kTile; p += kBlock)
+      {
+         const int x = tx0 + (p & (kTile - 1)), y = ty0 + p / kTile;
+         float v = 0.0f;
+         if (x < (int)a.w && y < (int)a.h)
+         {
+            const size_t i = (size_t)y * a.w + x;
+            if (extra)
+               v = extra[i];
+            else if (flat)
+               v = (float)a.flatBg;
+            else
+               v = (float)((a.bgMap ? (double)a.bgMap[i] : a.flatBg) * bgScale);
+         }
+         img[p] = v;
+      }
+      __syncthreads();
+      // the frame's range, clamped to the emitter array (offsets come from the caller)
+      const uint32_t e1 = min(a.emStart[f + 1], a.nEmitters), e0 = min(a.emStart[f], e1);
+      const int R = kt.maxCamRad;
+      for (uint32_t c0 = e0; c0 < e1; c0 += kBlock)
+      {
+         // scan: emitters whose (2R+1)^2 square reaches the tile, kept in frame order
+         const uint32_t e = c0 + tid;
+         bool hit = false;
+         if (e < e1)
+         {
+            const int2 q = a.pos[e];
+            hit = q.x + R >= tx0 && q.x - R <= tx0 + kTile - 1 && q.y + R >= ty0 && q.y - R <= ty0 + kTile - 1;
+         }
+         const unsigned long long mask = __ballot(hit);
+         if (lane == 0)
+            waveHits[wave] = __popcll(mask);
+         __syncthreads();
+         int before = 0, nHits = 0;
+         for (int w = 0; w < kBlock / 64; ++w)
+         {
+            before += w < wave ? waveHits[w] : 0;
+            nHits += waveHits[w];
+         }
+         if (hit)
+            list[before + __popcll(mask & ((1ull << lane) - 1ull))] = (int)e;
+         __syncthreads();
+         // splat the hits in order: one pixel of the clipped patch per lane
+         for (int k = 0; k < nHits; ++k)
+         {
+            const SetupRec& r = a.recs[list[k]];
+            const int kk = r.kernel;
+            const int camRad = kt.camRad[kk], os = kt.os[kk], bw = kt.bw[kk], NT = r.nTaps, off = os - 1;
+            // patch rows/cols inside the image (Rows<NT> :87-89) and the tile
+            int dxLo = -camRad > -r.x0 ? -camRad : -r.x0;
+            int dxHi = camRad < (int)a.w - 1 - r.x0 ? camRad : (int)a.w - 1 - r.x0;
+            int dyLo = -camRad > -r.y0 ? -camRad : -r.y0;
+            int dyHi = camRad < (int)a.h - 1 - r.y0 ? camRad : (int)a.h - 1 - r.y0;
+            dxLo = dxLo > tx0 - r.x0 ? dxLo : tx0 - r.x0;
+            dxHi = dxHi < tx0 + kTile - 1 - r.x0 ? dxHi : tx0 + kTile - 1 - r.x0;
+            dyLo = dyLo > ty0 - r.y0 ? dyLo : ty0 - r.y0;
+            dyHi = dyHi < ty0 + kTile - 1 - r.y0 ? dyHi : ty0 + kTile - 1 - r.y0;
+            const int rw = dxHi - dxLo + 1, rh = dyHi - dyLo + 1;
+            if (rw > 0 && rh > 0)
+            {
+               const float* B = kt.sums[kk] + (size_t)r.plane * bw * bw;
+               const int nSpan = 2 * camRad + 1;
+               const int16_t* lo = kt.lo[kk] ? kt.lo[kk] + (size_t)r.plane * nSpan : nullptr;
+               const int16_t* hi = kt.hi[kk] ? kt.hi[kk] + (size_t)r.plane * nSpan : nullptr;
+               for (int p = tid; p < rw * rh; p += kBlock)
+               {
+                  const int dy = dyLo + p / rw, dx = dxLo + (p - (p / rw) * rw);
+                  if (lo && (dx < lo[dy + camRad] || dx > hi[dy + camRad]))
+                     continue;
+                  if (dx * dx + dy * dy > r.cutR2)
+                     continue;
+                  const int r0 = r.by + dy * os + off, cc = r.bx + dx * os + off;
+                  const int jLo = 0 > -r0 ? 0 : -r0, jHi = NT < bw - r0 ? NT : bw - r0;
+                  const int iLo = 0 > -cc ? 0 : -cc, iHi = NT < bw - cc ? NT : bw - cc;
+                  double sum = 0.0;
+                  for (int j = jLo; j < jHi; ++j)
+                  {
+                     const float* brow = B + (size_t)(r0 + j) * bw + cc;
+                     double row = 0.0;
+                     for (int i = iLo; i < iHi; ++i)
+                        row += r.wx[i] * brow[i];
+                     sum += r.wy[j] * row;
+                  }
+                  const int li = (r.y0 + dy - ty0) * kTile + (r.x0 + dx - tx0);
+                  img[li] += (float)(r.photons * sum);
+               }
+            }
+            __syncthreads();
+         }
+      }
+      // out: photons and/or the noise chain, per pixel
+      for (int p = tid; p < kTile * kTile; p += kBlock)
+      {
+         const int x = tx0 + (p & (kTile - 1)), y = ty0 + p / kTile;
+         if (x >= (int)a.w || y >= (int)a.h)
+            continue;
+         const size_t i = (size_t)y * a.w + x;
+         const float v = img[p];
+         if (a.outPhotons)
+            a.outPhotons[(size_t)f * npx + i] = v;
+         if (NOISE)
+            a.outAdu[(size_t)f * npx + i] =
+               noise_pixel<EMCCD>(cp, v, a.seed, a.frames[f], (uint32_t)i + a.pixelBase, a.offset != nullptr,
+                                  a.offset ? a.offset[i] : 0.0f, a.gain != nullptr, a.gain ? a.gain[i] : 0.0f,
+                                  a.readN != nullptr, a.readN ? a.readN[i] : 0.0f);
+      }
+      __syncthreads();
+   }
+}
+
+static bool upload(iscg_context* h, DevBuf& b, const void* src, size_t bytes)
+{
+   if (!ok(h, b.reserve(bytes), "device allocation"))
+      return false;
+   const hipMemcpyKind kind = is_device_ptr(src) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+   return ok(h, hipMemcpyAsync(b.p, src, bytes, kind, h->stream), "copy to device");
+}
+
+static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t nEmitters, const uint32_t* emStart,
+                      uint32_t nFrames, const uint32_t* frames, const float* bgScales, const float* extras,
+                      const iscg_camera* cam, uint32_t seed, double minPhotons, uint16_t* outAdu, float* outPhotons)
+{
+   SplatState& s = state(h);
+   if (!s.haveStatic)
+      return fail(h, "splat: iscg_splat_set_static not called");
+   if (!emStart || (!frames && outAdu) || (nEmitters && !ems))
+      return fail(h, "splat: null argument");
+   if (outAdu && !cam)
+      return fail(h, "splat: the noise needs a camera");
+   if (nEmitters && s.nk == 0)
+      return fail(h, "splat: emitters but no kernels (iscg_splat_set_kernels)");
+   if (nFrames == 0)
+      return 0;
+   const size_t npx = (size_t)s.w * s.h;
+   const uint32_t* dStart = (const uint32_t*)stage_in(h, 0, emStart, (size_t)(nFrames + 1) * sizeof(uint32_t));
+   const uint32_t* dFrames = frames ? (const uint32_t*)stage_in(h, 1, frames, nFrames * sizeof(uint32_t)) : nullptr;
+   const float* dScales = bgScales ? (const float*)stage_in(h, 2, bgScales, nFrames * sizeof(float)) : nullptr;
+   const float* dExtras = extras ? (const float*)stage_in(h, 3, extras, nFrames * npx * sizeof(float)) : nullptr;
+   const size_t recBytes = records ? sizeof(iscg_splat_emitter) : sizeof(iscg_frame_emitter);
+   const void* dEms = nEmitters ? stage_in(h, 4, ems, (size_t)nEmitters * recBytes) : nullptr;
+   uint16_t* dAdu = outAdu ? (uint16_t*)stage_out(h, 5, outAdu, nFrames * npx * sizeof(uint16_t)) : nullptr;
+   float* dPh = outPhotons ? (float*)stage_out(h, 6, outPhotons, nFrames * npx * sizeof(float)) : nullptr;
+   if (!dStart || (frames && !dFrames) || (bgScales && !dScales) || (extras && !dExtras) || (nEmitters && !dEms) ||
+       (outAdu && !dAdu) || (outPhotons && !dPh))
+      return -1;
+   if (!ok(h, s.setup.reserve((size_t)(nEmitters ? nEmitters : 1) * sizeof(SetupRec)), "setup allocation") ||
+       !ok(h, s.pos.reserve((size_t)(nEmitters ? nEmitters : 1) * sizeof(int2)), "setup allocation"))
+      return -1;
+   const KernelTable kt = table_of(s);
+   if (nEmitters)
+   {
+      const unsigned grid = grid_for(nEmitters, 256, 4);
+      if (records)
+         hipLaunchKernelGGL(setup_records_kernel, dim3(grid), dim3(256), 0, h->stream,
+                            (const iscg_splat_emitter*)dEms, nEmitters, kt, s.setup.as<SetupRec>(), s.pos.as<int2>());
+      else
+         hipLaunchKernelGGL(setup_exact_kernel, dim3(grid), dim3(256), 0, h->stream, (const iscg_frame_emitter*)dEms,
+                            nEmitters, kt, minPhotons, s.setup.as<SetupRec>(), s.pos.as<int2>());
+      if (!ok(h, hipGetLastError(), "splat setup launch"))
+         return -1;
+   }
+   TileArgs a;
+   a.recs = s.setup.as<SetupRec>();
+   a.pos = s.pos.as<int2>();
+   a.emStart = dStart;
+   a.frames = dFrames;
+   a.bgScales = dScales;
+   a.extras = dExtras;
+   a.bgMap = s.hasBg ? s.bg.as<float>() : nullptr;
+   a.offset = s.hasOffset ? s.offset.as<float>() : nullptr;
+   a.gain = s.hasGain ? s.gain.as<float>() : nullptr;
+   a.readN = s.hasRead ? s.readN.as<float>() : nullptr;
+   a.outAdu = dAdu;
+   a.outPhotons = dPh;
+   a.flatBg = s.flatBg;
+   a.w = s.w;
+   a.h = s.h;
+   a.tilesX = (s.w + kTile - 1) / kTile;
+   a.nTiles = a.tilesX * ((s.h + kTile - 1) / kTile);
+   a.nFrames = nFrames;
+   a.nEmitters = nEmitters;
+   a.seed = seed;
+   a.pixelBase = s.pixelBase;
+   const CamParams cp = cam ? cam_params(cam) : CamParams{};
+   const size_t items = (size_t)a.nTiles * nFrames;
+   const unsigned grid = (unsigned)(items < 256 * 16 ? items : 256 * 16);
+   if (!dAdu)
+      hipLaunchKernelGGL((splat_tiles_kernel<false, false>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
+   else if (cp.emccd)
+      hipLaunchKernelGGL((splat_tiles_kernel<true, true>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
+   else
+      hipLaunchKernelGGL((splat_tiles_kernel<true, false>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
+   if (!ok(h, hipGetLastError(), "splat launch"))
+      return -1;
+   if (outAdu && !finish_out(h, outAdu, dAdu, nFrames * npx * sizeof(uint16_t)))
+      return -1;
+   if (outPhotons && !finish_out(h, outPhotons, dPh, nFrames * npx * sizeof(float)))
+      return -1;
+   return 0;
+}
+
+}  // namespace iscg
+
+using namespace iscg;
+
+extern "C" {
+
+int32_t iscg_splat_set_kernels(iscg_context* h, const iscg_kernel_view* kernels, int32_t n)
+{
+   if (begin(h))
+      return -1;
+   if (n < 0 || n > kMaxKernels || (n > 0 && !kernels))
+      return fail(h, "iscg_splat_set_kernels: 0..4 kernels");
+   SplatState& s = state(h);
+   for (int i = 0; i < n; ++i)
+   {
+      const iscg_kernel_view& v = kernels[i];
+      if (!v.planes || v.nz < 1 || v.size < 3 || (v.size % 2) != 1 || v.oversampling < 1 || v.interpMode < 0 ||
+          v.interpMode > 2)
+         return fail(h, "iscg_splat_set_kernels: bad kernel view (planes, odd size, nz >= 1, os >= 1, interp 0-2)");
+   }
+   for (int i = 0; i < n; ++i)
+   {
+      const iscg_kernel_view& v = kernels[i];
+      KernelDev& k = s.k[i];
+      const int os = v.oversampling, bw = v.size + os - 1, camRad = ((v.size - 1) / 2) / os;
+      const bool same = v.serial != 0 && k.serial == v.serial && k.size == v.size && k.nz == v.nz && k.os == os;
+      k.interp = v.interpMode;
+      if (same)
+         continue;
+      k.bw = bw;
+      k.os = os;
+      k.camRad = camRad;
+      k.size = v.size;
+      k.nz = v.nz;
+      k.serial = v.serial;
+      const size_t planeBytes = (size_t)v.nz * v.size * v.size * sizeof(float);
+      const float* dPlanes = (const float*)stage_in(h, 0, v.planes, planeBytes);
+      if (!dPlanes || !ok(h, k.sums.reserve((size_t)v.nz * bw * bw * sizeof(float)), "block-sum allocation"))
+         return -1;
+      const size_t total = (size_t)v.nz * bw * bw;
+      hipLaunchKernelGGL(block_sums_kernel, dim3(grid_for(total, 256, 4)), dim3(256), 0, h->stream, dPlanes,
+                         k.sums.as<float>(), v.size, os, v.nz);
+      if (!ok(h, hipGetLastError(), "block sums launch"))
+         return -1;
+      k.hasSpans = v.spanLo && v.spanHi;
+      if (k.hasSpans)
+      {
+         const size_t nSpan = (size_t)v.nz * (2 * camRad + 1);
+         if (!ok(h, k.spans.reserve(2 * nSpan * sizeof(int16_t)), "span allocation"))
+            return -1;
+         const hipMemcpyKind kl = is_device_ptr(v.spanLo) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+         const hipMemcpyKind kh = is_device_ptr(v.spanHi) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+         if (!ok(h, hipMemcpyAsync(k.spans.p, v.spanLo, nSpan * sizeof(int16_t), kl, h->stream), "span copy") ||
+             !ok(h, hipMemcpyAsync(k.spans.as<int16_t>() + nSpan, v.spanHi, nSpan * sizeof(int16_t), kh, h->stream),
+                 "span copy"))
+            return -1;
+      }
+      // the staged planes must outlive the block-sum kernel before the staging buffer is reused
+      if (!ok(h, hipStreamSynchronize(h->stream), "stream synchronize"))
+         return -1;
+   }
+   s.nk = n;
+   return 0;
+}
+
+int32_t iscg_splat_set_static(iscg_context* h, uint32_t width, uint32_t height, uint32_t pixelBase,
+                              const float* offset, const float* gain, const float* readNoise, const float* background,
+                              double flatBackground, uint64_t key)
+{
+   if (begin(h))
+      return -1;
+   if (width == 0 || height == 0 || (size_t)width * height > 0xFFFFFFFFull)
+      return fail(h, "iscg_splat_set_static: bad frame size");
+   SplatState& s = state(h);
+   if (key != 0 && s.haveStatic && key == s.key && width == s.w && height == s.h)
+      return 0;
+   const size_t bytes = (size_t)width * height * sizeof(float);
+   s.hasOffset = offset != nullptr;
+   s.hasGain = gain != nullptr;
+   s.hasRead = readNoise != nullptr;
+   s.hasBg = background != nullptr;
+   if ((offset && !upload(h, s.offset, offset, bytes)) || (gain && !upload(h, s.gain, gain, bytes)) ||
+       (readNoise && !upload(h, s.readN, readNoise, bytes)) || (background && !upload(h, s.bg, background, bytes)))
+      return -1;
+   if (!ok(h, hipStreamSynchronize(h->stream), "stream synchronize"))
+      return -1;
+   s.w = width;
+   s.h = height;
+   s.pixelBase = pixelBase;
+   s.flatBg = flatBackground;
+   s.key = key;
+   s.haveStatic = true;
+   return 0;
+}
+
+int32_t iscg_splat_render_frames(iscg_context* h, const iscg_frame_emitter* ems, uint32_t nEmitters,
+                                 const uint32_t* emStart, uint32_t nFrames, const uint32_t* frames,
+                                 const float* bgScales, const float* extras, const iscg_camera* cam, uint32_t seed,
+                                 double minPhotons, uint16_t* outAdu, float* outPhotons)
+{
+   if (begin(h))
+      return -1;
+   return render(h, ems, false, nEmitters, emStart, nFrames, frames, bgScales, extras, cam, seed, minPhotons, outAdu,
+                 outPhotons);
+}
+
+int32_t iscg_splat_render_records(iscg_context* h, const iscg_splat_emitter* ems, uint32_t nEmitters,
+                                  const uint32_t* emStart, uint32_t nFrames, const uint32_t* frames,
+                                  const float* bgScales, const float* extras, const iscg_camera* cam, uint32_t seed,
+                                  uint16_t* outAdu, float* outPhotons)
+{
+   if (begin(h))
+      return -1;
+   return render(h, ems, true, nEmitters, emStart, nFrames, frames, bgScales, extras, cam, seed, 0.0, outAdu,
+                 outPhotons);
+}
+
+}  // extern "C"

@@ -1,0 +1,8 @@
+// stubs.hip -- teardown hooks for subsystems not built into this library yet.
+#include "iscg_internal.h"
+
+namespace iscg {
+void destroy_psf(iscg_context* h) { h->psf = nullptr; }
+void destroy_wf(iscg_context* h) { h->wf = nullptr; }
+void destroy_roi(iscg_context* h) { h->roi = nullptr; }
+}  // namespace iscg

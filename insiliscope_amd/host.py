@@ -1,0 +1,411 @@
+"""Host-side mirror of the reference renderer's GPU seams, over libiscg's C ABI.
+
+Names, arguments and error behaviour follow the reference's C++ interfaces (Sim/ = adapter/inSiliScope/Simulation/):
+calls return ``True``/``False`` and leave the reason in ``err`` where the reference returns ``bool`` with a
+``std::string& err``; results come back as numpy arrays (host) or are written into caller tensors (device).
+
+Array arguments may be numpy arrays (host memory, staged by the library) or torch tensors on the context's GPU
+(used in place). There is no CPU fallback here: without libiscg.so every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import IscgError
+
+# ---- records -------------------------------------------------------------------------------------------------
+
+# iscg_frame_emitter: sim::FrameEmitter (Sim/SMLMSimulation.h:158) as the splat takes it
+FRAME_EMITTER_DTYPE = np.dtype([("xPx", "f8"), ("yPx", "f8"), ("photons", "f8"), ("zIndex", "i4"),
+                                ("kernel", "i4")])
+# iscg_splat_emitter: sim::GpuSplatEmitter (Sim/GpuSimD3D11.h:50-59), 64 bytes
+SPLAT_EMITTER_DTYPE = np.dtype([("x0", "i4"), ("y0", "i4"), ("bx", "i4"), ("by", "i4"), ("plane", "i4"),
+                                ("nTaps", "i4"), ("photons", "f4"), ("cutRadius2", "i4"), ("wx", "f4", 4),
+                                ("wy", "f4", 4)])
+# sim::BlinkEvent's hot-path fields (Sim/SMLMSimulation.h:82-105)
+BLINK_EVENT_DTYPE = np.dtype([("xUm", "f8"), ("yUm", "f8"), ("zNm", "f8"), ("tStart", "f8"), ("tEnd", "f8"),
+                              ("brightness", "f8")])
+assert FRAME_EMITTER_DTYPE.itemsize == 32 and SPLAT_EMITTER_DTYPE.itemsize == 64
+
+INTERP = {"Nearest": 0, "Linear": 1, "Cubic": 2}
+
+
+def _ptr(a):
+    """(pointer, keep-alive) for a numpy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return None, None
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            a = np.ascontiguousarray(a)
+        return a.ctypes.data, a
+    if hasattr(a, "data_ptr"):  # torch tensor (device or host)
+        if not a.is_contiguous():
+            raise IscgError("tensor arguments must be contiguous")
+        return a.data_ptr(), a
+    raise TypeError(f"unsupported array type {type(a)!r}")
+
+
+def _arr(a, dtype):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return a
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ---- context ---------------------------------------------------------------------------------------------------
+
+class Context:
+    """One iscg_context: a device and its stream. Not re-entrant; one per thread (Sim/GpuSimD3D11.h:28-30)."""
+
+    def __init__(self, device: int = 0):
+        self.L = _lib.lib()
+        h = C.c_void_p()
+        if self.L.iscg_create(device, C.byref(h)) != 0:
+            raise IscgError(self.L.iscg_last_error(None).decode())
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.iscg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def error(self) -> str:
+        return self.L.iscg_last_error(self.h).decode()
+
+    def check(self, rc: int, what: str = "") -> None:
+        if rc != 0:
+            raise IscgError(f"{what}: {self.error()}" if what else self.error())
+
+    def synchronize(self):
+        self.check(self.L.iscg_synchronize(self.h), "iscg_synchronize")
+
+    @property
+    def stream(self) -> int:
+        return self.L.iscg_stream(self.h)
+
+    def inject_failure(self, count: int, message: str = "injected failure"):
+        self.check(self.L.iscg_inject_failure(self.h, count, message.encode()))
+
+
+_default: dict[int, Context] = {}
+
+
+def default_context(device: int = 0) -> Context:
+    if device not in _default:
+        _default[device] = Context(device)
+    return _default[device]
+
+
+# ---- camera noise --------------------------------------------------------------------------------------------
+
+@dataclass
+class CameraNoiseParams:
+    """sim::CameraNoiseParams (Sim/SMLMNoise.h:35-50), same defaults."""
+
+    quantumEfficiency: float = 0.85
+    darkCurrentElectrons: float = 0.0
+    gainPhotonsPerAdu: float = 1.0
+    readNoiseElectrons: float = 1.5
+    emccd: bool = False
+    emGain: float = 300.0
+    cicElectrons: float = 0.002
+    bitDepth: int = 16
+
+    def to_c(self) -> _lib.Camera:
+        return _lib.Camera(self.quantumEfficiency, self.darkCurrentElectrons, self.gainPhotonsPerAdu,
+                           self.readNoiseElectrons, int(bool(self.emccd)), int(self.bitDepth), self.emGain,
+                           self.cicElectrons)
+
+
+def ApplyNoiseChainFrames(photons, cam: CameraNoiseParams, noiseSeed: int, frames, offsetMap=None, gainMap=None,
+                          readNoiseMap=None, pixelBase: int = 0, out=None, ctx: Context | None = None):
+    """ApplyNoiseChain (Sim/SMLMNoise.h:136) over a batch: photons (nFrames, height, width) float32 -> ADU uint16.
+    frames: the frame id of each image (the noise stream). Device tensors in, device tensor out."""
+    ctx = ctx or default_context()
+    ph = _arr(photons, np.float32)
+    n_frames, height, width = ph.shape
+    fr = _arr(frames, np.uint32)
+    if out is None:
+        if hasattr(ph, "data_ptr") and getattr(ph, "is_cuda", False):
+            import torch
+            # int16 storage holds the uint16 ADU bits (view as uint16 on the host)
+            out = torch.empty((n_frames, height, width), dtype=torch.int16, device=ph.device)
+        else:
+            out = np.zeros((n_frames, height, width), np.uint16)
+    keep = [(_ptr(x)) for x in (ph, fr, _arr(offsetMap, np.float32), _arr(gainMap, np.float32),
+                                _arr(readNoiseMap, np.float32), out)]
+    p = [k[0] for k in keep]
+    cc = cam.to_c()
+    ctx.check(ctx.L.iscg_noise(ctx.h, p[0], width, height, C.byref(cc), p[2], p[3], p[4], noiseSeed & 0xFFFFFFFF,
+                               p[1], n_frames, pixelBase, p[5]), "iscg_noise")
+    return out
+
+
+def ApplyNoiseChain(photonImage, width: int, height: int, cam: CameraNoiseParams, offsetMap=None, gainMap=None,
+                    readNoiseMap=None, noiseSeed: int = 0, frame: int = 0, pixelBase: int = 0,
+                    ctx: Context | None = None):
+    """sim::ApplyNoiseChain (Sim/SMLMNoise.h:136) for one frame: photonImage (height*width floats) -> uint16 ADU.
+    Maps whose size is not width*height are ignored, as the reference's size checks do (SMLMNoise.cpp:151-158)."""
+    ph = np.ascontiguousarray(photonImage, np.float32).reshape(1, height, width)
+    n = width * height
+
+    def fit(m):
+        return None if m is None or np.size(m) != n else np.ascontiguousarray(m, np.float32)
+
+    out = ApplyNoiseChainFrames(ph, cam, noiseSeed, np.array([frame], np.uint32), fit(offsetMap), fit(gainMap),
+                                fit(readNoiseMap), pixelBase, ctx=ctx)
+    return out.reshape(height, width)
+
+
+# ---- kernel caches and the splat -----------------------------------------------------------------------------
+
+_serials = itertools.count(1)
+
+
+@dataclass
+class PsfHaloSpans:
+    """sim::PsfHaloSpans (Sim/PsfGeneratorBridge.h:214-225)."""
+
+    lo: np.ndarray          # int16 (nz, 2 camRad + 1)
+    hi: np.ndarray
+    radius2: np.ndarray     # int32 (nz,)
+    camRad: int
+    threshold: float = 0.0
+
+
+@dataclass
+class PsfKernelCache:
+    """sim::PsfKernelCache (Sim/PsfGeneratorBridge.h:243-293): sum-1 planes of one kernel stack."""
+
+    planes: object                  # (nz, size, size) float32, numpy or device tensor
+    oversampling: int = 1
+    zStepNm: float = 100.0
+    interpMode: int = 0
+    halo: PsfHaloSpans | None = None
+    serial: int = field(default_factory=lambda: next(_serials))
+
+    @property
+    def nz(self) -> int:
+        return int(self.planes.shape[0])
+
+    @property
+    def sizeOversampled(self) -> int:
+        return int(self.planes.shape[1])
+
+    @property
+    def halfWidthOversampled(self) -> int:
+        return (self.sizeOversampled - 1) // 2
+
+    @property
+    def blockSumWidth(self) -> int:
+        return self.sizeOversampled + self.oversampling - 1
+
+    @property
+    def camRad(self) -> int:
+        return self.halfWidthOversampled // self.oversampling
+
+    def NearestZIndex(self, zUm):
+        """PsfKernelCache::NearestZIndex (Sim/PsfGeneratorBridge.cpp:40-60), vectorised: (index, clamped)."""
+        z = np.asarray(zUm, np.float64)
+        if self.nz <= 1 or self.zStepNm <= 0.0:
+            return np.zeros(z.shape, np.int32), np.zeros(z.shape, bool)
+        idx_f = z * 1000.0 / self.zStepNm + (self.nz - 1) / 2.0
+        # std::lround: half away from zero
+        idx = np.where(idx_f >= 0, np.floor(idx_f + 0.5), np.ceil(idx_f - 0.5)).astype(np.int64)
+        clamped = (idx < 0) | (idx >= self.nz)
+        return np.clip(idx, 0, self.nz - 1).astype(np.int32), clamped
+
+
+def CollectFrameEmitters(events, frameIndex: int, pixelSizeNm: float, photonsPerBlink: float,
+                         driftOffsetXPx: float = 0.0, driftOffsetYPx: float = 0.0,
+                         psfCache: PsfKernelCache | None = None, globalZOffsetUm: float = 0.0, kernel: int = 0):
+    """sim::CollectFrameEmitters (Sim/SMLMSimulation.cpp:55-101) without an illumination field: the frame's emitters
+    in event order as iscg_frame_emitter records (the same FP64 operations, vectorised)."""
+    ev = np.asarray(events, dtype=BLINK_EVENT_DTYPE)
+    f = float(frameIndex)
+    ov = np.minimum(f + 1.0, ev["tEnd"]) - np.maximum(f, ev["tStart"])
+    keep = ov > 0.0
+    ev, ov = ev[keep], np.minimum(ov[keep], 1.0)
+    out = np.zeros(len(ev), FRAME_EMITTER_DTYPE)
+    site_x = ev["xUm"] * 1000.0 / pixelSizeNm
+    site_y = ev["yUm"] * 1000.0 / pixelSizeNm
+    out["xPx"] = site_x + driftOffsetXPx
+    out["yPx"] = site_y + driftOffsetYPx
+    ph = photonsPerBlink * ov
+    b = ev["brightness"]
+    out["photons"] = np.where(b != 1.0, ph * b, ph)
+    if psfCache is not None:
+        out["zIndex"], _ = psfCache.NearestZIndex(ev["zNm"] / 1000.0 - globalZOffsetUm)
+    out["kernel"] = kernel
+    return out
+
+
+def _catmull_rom(f):
+    f2 = f * f
+    f3 = f2 * f
+    return np.stack([-0.5 * f3 + f2 - 0.5 * f, 1.5 * f3 - 2.5 * f2 + 1.0, -1.5 * f3 + 2.0 * f2 + 0.5 * f,
+                     0.5 * f3 - 0.5 * f2], axis=-1)
+
+
+def CollectGpuEmitters(frameEmitters, cache: PsfKernelCache, kernel: int = 0, minPhotons: float = 0.0):
+    """sim::CollectGpuEmitters (Sim/SMLMSimulation.cpp:219-256) from a frame's emitters: SplatSetup
+    (Sim/PsfGeneratorBridge.cpp:1382-1422) on the host, 64-byte GpuSplatEmitter records with float weights."""
+    em = np.asarray(frameEmitters, FRAME_EMITTER_DTYPE)
+    em = em[~((em["photons"] <= 0.0) | (em["photons"] < minPhotons))]
+    os_ = max(1, cache.oversampling)
+    kc = (cache.sizeOversampled - 1) / 2.0
+    x0 = np.floor(em["xPx"] + 0.5)
+    y0 = np.floor(em["yPx"] + 0.5)
+    tx = kc + (x0 - em["xPx"] - 0.5) * os_ + 0.5
+    ty = kc + (y0 - em["yPx"] - 0.5) * os_ + 0.5
+    out = np.zeros(len(em), SPLAT_EMITTER_DTYPE)
+    out["x0"], out["y0"] = x0, y0
+    wx = np.zeros((len(em), 4))
+    wy = np.zeros((len(em), 4))
+    if cache.interpMode == 0:
+        out["bx"], out["by"], out["nTaps"] = np.floor(tx + 0.5), np.floor(ty + 0.5), 1
+        wx[:, 0] = wy[:, 0] = 1.0
+    else:
+        bx, by = np.floor(tx), np.floor(ty)
+        fx, fy = tx - bx, ty - by
+        if cache.interpMode == 2:
+            out["bx"], out["by"], out["nTaps"] = bx - 1, by - 1, 4
+            wx, wy = _catmull_rom(fx), _catmull_rom(fy)
+        else:
+            out["bx"], out["by"], out["nTaps"] = bx, by, 2
+            wx[:, 0], wx[:, 1], wy[:, 0], wy[:, 1] = 1.0 - fx, fx, 1.0 - fy, fy
+    out["wx"], out["wy"] = wx.astype(np.float32), wy.astype(np.float32)
+    out["plane"] = em["zIndex"] | (kernel << 16)
+    out["photons"] = em["photons"].astype(np.float32)
+    if cache.halo is not None:
+        out["cutRadius2"] = cache.halo.radius2[np.clip(em["zIndex"], 0, cache.nz - 1)]
+    else:
+        out["cutRadius2"] = np.iinfo(np.int32).max
+    return out
+
+
+class SplatGpuHost:
+    """sim::SplatGpuHost / sim::GpuSimulator (Sim/SplatGpu.h:40-77, Sim/GpuSimD3D11.h:61-105) on libiscg."""
+
+    kMaxKernels = 4
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or Context()
+        self.err = ""
+        self.width = self.height = 0
+
+    def _call(self, rc: int) -> bool:
+        if rc != 0:
+            self.err = self.ctx.error()
+            return False
+        self.err = ""
+        return True
+
+    def SetKernels(self, kernels: list[PsfKernelCache]) -> bool:
+        if len(kernels) > self.kMaxKernels:
+            self.err = "at most 4 kernels"
+            return False
+        views = (_lib.KernelView * max(1, len(kernels)))()
+        keep = []
+        for i, k in enumerate(kernels):
+            pl, a = _ptr(_arr(k.planes, np.float32))
+            keep.append(a)
+            lo = hi = None
+            if k.halo is not None:
+                lo, a1 = _ptr(np.ascontiguousarray(k.halo.lo, np.int16))
+                hi, a2 = _ptr(np.ascontiguousarray(k.halo.hi, np.int16))
+                keep += [a1, a2]
+            views[i] = _lib.KernelView(pl, lo, hi, k.zStepNm, k.serial, k.nz, k.sizeOversampled, k.oversampling,
+                                       k.interpMode)
+        self.kernels = list(kernels)
+        return self._call(self.ctx.L.iscg_splat_set_kernels(self.ctx.h, views, len(kernels)))
+
+    def SetStatic(self, width: int, height: int, pixelBase: int = 0, offset=None, gain=None, readNoise=None,
+                  background=None, flatBackground: float = 0.0, key: int = 0) -> bool:
+        n = width * height
+        fit = lambda m: None if m is None or np.size(m) != n else _arr(m, np.float32)
+        ps = [_ptr(fit(m)) for m in (offset, gain, readNoise, background)]
+        self.width, self.height = width, height
+        return self._call(self.ctx.L.iscg_splat_set_static(self.ctx.h, width, height, pixelBase, ps[0][0], ps[1][0],
+                                                           ps[2][0], ps[3][0], float(flatBackground), key))
+
+    def RenderBatch(self, emitters, emStart, frames, cam: CameraNoiseParams | None, seed: int, bgScales=None,
+                    extras=None, minPhotons: float = 0.0, outAdu=None, outPhotons=None, records: bool = False):
+        """One call over a flat batch: emitters (iscg_frame_emitter or GpuSplatEmitter records) with per-frame
+        offsets emStart (nFrames + 1). Writes outAdu / outPhotons (allocated as numpy when None and wanted)."""
+        em = _arr(emitters, SPLAT_EMITTER_DTYPE if records else FRAME_EMITTER_DTYPE)
+        st = _arr(emStart, np.uint32)
+        n_frames = int(st.shape[0]) - 1
+        fr = _arr(frames, np.uint32)
+        ptrs = [_ptr(x) for x in (em, st, fr, _arr(bgScales, np.float32), _arr(extras, np.float32), outAdu,
+                                  outPhotons)]
+        cc = cam.to_c() if cam is not None else None
+        n_em = int(em.shape[0]) if em is not None else 0
+        L = self.ctx.L
+        if records:
+            rc = L.iscg_splat_render_records(self.ctx.h, ptrs[0][0], n_em, ptrs[1][0], n_frames, ptrs[2][0],
+                                             ptrs[3][0], ptrs[4][0], C.byref(cc) if cc else None, seed & 0xFFFFFFFF,
+                                             ptrs[5][0], ptrs[6][0])
+        else:
+            rc = L.iscg_splat_render_frames(self.ctx.h, ptrs[0][0], n_em, ptrs[1][0], n_frames, ptrs[2][0],
+                                            ptrs[3][0], ptrs[4][0], C.byref(cc) if cc else None, seed & 0xFFFFFFFF,
+                                            float(minPhotons), ptrs[5][0], ptrs[6][0])
+        return self._call(rc)
+
+    def RenderFrames(self, emitters: list, frames, extras, cam: CameraNoiseParams, seed: int, outs: list,
+                     bgScales=None) -> bool:
+        """SplatGpuHost::RenderFrames (Sim/SplatGpu.h:57): emitters[k] (GpuSplatEmitter records) of frame k onto
+        extras[k] (or the background), noise keyed by (seed, frames[k]); outs[k] receives the ADU."""
+        counts = [len(e) for e in emitters]
+        start = np.zeros(len(emitters) + 1, np.uint32)
+        start[1:] = np.cumsum(counts)
+        flat = np.concatenate([np.asarray(e, SPLAT_EMITTER_DTYPE) for e in emitters]) if sum(counts) else \
+            np.zeros(0, SPLAT_EMITTER_DTYPE)
+        npx = self.width * self.height
+        ext = None
+        if extras is not None and any(x is not None for x in extras):
+            ext = np.zeros((len(emitters), npx), np.float32)
+            for k, x in enumerate(extras):
+                ext[k] = 0.0 if x is None else np.asarray(x, np.float32).reshape(-1)
+        adu = np.zeros((len(emitters), npx), np.uint16)
+        if not self.RenderBatch(flat, start, np.asarray(frames, np.uint32), cam, seed, bgScales, ext,
+                                outAdu=adu, records=True):
+            return False
+        for k in range(len(emitters)):
+            outs[k][...] = adu[k].reshape(np.shape(outs[k]))
+        return True
+
+    def MaxBatchFrames(self) -> int:
+        """Frames per call within a 1 GiB output budget (the device holds the whole batch in HBM)."""
+        return max(1, (1 << 30) // max(1, 6 * self.width * self.height))
+
+
+def SetDefaultBlinkSplatGpu(make):
+    """Installs a host factory, as sim::SetDefaultBlinkSplatGpu (Sim/ScopeMovie.h:378) does for the viewer."""
+    global _default_splat_factory
+    _default_splat_factory = make
+
+
+_default_splat_factory = None
+
+
+def DefaultBlinkSplatGpu():
+    return _default_splat_factory() if _default_splat_factory else SplatGpuHost()
+
+
+__all__ = ["Context", "CameraNoiseParams", "ApplyNoiseChain", "ApplyNoiseChainFrames", "PsfKernelCache",
+           "PsfHaloSpans", "SplatGpuHost", "CollectFrameEmitters", "CollectGpuEmitters", "FRAME_EMITTER_DTYPE",
+           "SPLAT_EMITTER_DTYPE", "BLINK_EVENT_DTYPE", "INTERP", "IscgError"]

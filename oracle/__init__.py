@@ -1,0 +1,413 @@
+"""ctypes bindings of the CPU checkers.
+
+TEST INFRASTRUCTURE ONLY. ``liboracle.so`` is the FP64 C restatement of the reference's image-formation path
+(``oracle.c``); ``_ref/libiscref.so`` is the reference's own CPU code compiled from its sources (``ref_shim.cpp``),
+present only where it was built. Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg
+import this package; the product library never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libiscref.so")
+REFERENCE_ROOT = "/root/reference"
+
+
+def build(ref: bool = True) -> None:
+    """Compile liboracle.so, and libiscref.so when the reference checkout is present."""
+    subprocess.run(["make", "-C", HERE, "oracle"], check=True, stdout=subprocess.DEVNULL)
+    if ref and os.path.isdir(REFERENCE_ROOT):
+        subprocess.run(["make", "-C", HERE, "-j8", "ref", f"REF={REFERENCE_ROOT}"], check=True,
+                       stdout=subprocess.DEVNULL)
+
+
+class Camera(C.Structure):
+    """orc_camera / sim::CameraNoiseParams (Sim/SMLMNoise.h:35-50)."""
+
+    _fields_ = [("quantumEfficiency", C.c_double), ("darkCurrentElectrons", C.c_double),
+                ("gainPhotonsPerAdu", C.c_double), ("readNoiseElectrons", C.c_double),
+                ("emccd", C.c_int32), ("bitDepth", C.c_int32), ("emGain", C.c_double),
+                ("cicElectrons", C.c_double)]
+
+
+class PsfRequest(C.Structure):
+    """orc_psf_request: the numeric fields of sim::PsfGeneratorRequest (Sim/PsfGeneratorBridge.h:91-170)."""
+
+    _fields_ = [("wavelengthNm", C.c_double), ("na", C.c_double), ("immersionIndex", C.c_double),
+                ("sampleIndex", C.c_double), ("workingDistanceUm", C.c_double), ("sampleDepthNm", C.c_double),
+                ("pixelSizeNm", C.c_double), ("zStepNm", C.c_double), ("maskWaist", C.c_double),
+                ("zernike", C.c_double * 28), ("maskType", C.c_int32), ("maskModes", C.c_int32),
+                ("oversampling", C.c_int32), ("kernelHalfWidthPx", C.c_int32), ("nz", C.c_int32),
+                ("pupilSamples", C.c_int32), ("interpMode", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Emitter(C.Structure):
+    _fields_ = [("xPx", C.c_double), ("yPx", C.c_double), ("photons", C.c_double),
+                ("zIndex", C.c_int32), ("kernel", C.c_int32)]
+
+
+class SplatSetupT(C.Structure):
+    _fields_ = [("x0", C.c_int32), ("y0", C.c_int32), ("bx", C.c_int32), ("by", C.c_int32),
+                ("nTaps", C.c_int32), ("pad", C.c_int32), ("wx", C.c_double * 4), ("wy", C.c_double * 4)]
+
+
+EVENT_DTYPE = np.dtype([("xUm", "f8"), ("yUm", "f8"), ("zNm", "f8"), ("tStart", "f8"), ("tEnd", "f8"),
+                        ("brightness", "f8")])
+EMITTER_DTYPE = np.dtype([("xPx", "f8"), ("yPx", "f8"), ("photons", "f8"), ("zIndex", "i4"), ("kernel", "i4")])
+# sim::GpuSplatEmitter (Sim/GpuSimD3D11.h:50-59)
+GPU_EMITTER_DTYPE = np.dtype([("x0", "i4"), ("y0", "i4"), ("bx", "i4"), ("by", "i4"), ("plane", "i4"),
+                              ("nTaps", "i4"), ("photons", "f4"), ("cutRadius2", "i4"), ("wx", "f4", 4),
+                              ("wy", "f4", 4)])
+
+
+def _p(a, ct):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+def camera(**kw) -> Camera:
+    c = Camera(quantumEfficiency=1.0, darkCurrentElectrons=0.0, gainPhotonsPerAdu=1.0, readNoiseElectrons=1.5,
+               emccd=0, bitDepth=16, emGain=300.0, cicElectrons=0.002)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def psf_request(**kw) -> PsfRequest:
+    r = PsfRequest(wavelengthNm=660.0, na=1.4, immersionIndex=1.518, sampleIndex=1.518, workingDistanceUm=150.0,
+                   sampleDepthNm=0.0, pixelSizeNm=100.0, zStepNm=100.0, maskWaist=1.0, maskType=0, maskModes=5,
+                   oversampling=4, kernelHalfWidthPx=8, nz=1, pupilSamples=0, interpMode=0, reserved=0)
+    for k, v in kw.items():
+        if k == "zernike":
+            z = np.zeros(28)
+            if isinstance(v, dict):
+                for j, a in v.items():
+                    z[j] = a
+            else:
+                z[: len(v)] = v
+            for j in range(28):
+                r.zernike[j] = float(z[j])
+        else:
+            setattr(r, k, v)
+    return r
+
+
+class Oracle:
+    """The FP64 restatement (oracle.c)."""
+
+    def __init__(self, path: str = LIB_PATH):
+        if not os.path.exists(path):
+            build(ref=False)
+        L = C.CDLL(path)
+        self.L = L
+        D, I32, U32, I64 = C.c_double, C.c_int32, C.c_uint32, C.c_int64
+        PD, PF, PI16 = C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_int16)
+        L.orc_pcg4d.argtypes = [C.POINTER(C.c_uint32)]
+        L.orc_hash_unit.argtypes = [U32, I32, I32, U32]
+        L.orc_hash_unit.restype = D
+        L.orc_hash_stream.argtypes = [U32, I32, I32, U32, I32, PD]
+        L.orc_counter_uniforms.argtypes = [U32, U32, U32, I32, PD]
+        L.orc_counter_gauss.argtypes = [U32, U32, U32, I32, PD]
+        L.orc_counter_poisson.argtypes = [U32, U32, U32, D]
+        L.orc_counter_poisson.restype = D
+        L.orc_counter_gamma.argtypes = [U32, U32, U32, D]
+        L.orc_counter_gamma.restype = D
+        L.orc_noise_chain.argtypes = [PF, C.POINTER(C.c_uint16), U32, U32, C.POINTER(Camera), PF, PF, PF, U32, U32,
+                                      U32]
+        L.orc_psf_geometry.argtypes = [C.POINTER(PsfRequest), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]
+        L.orc_psf_planes.argtypes = [C.POINTER(PsfRequest), PF]
+        L.orc_psf_planes.restype = I32
+        L.orc_psf_fields.argtypes = [C.POINTER(PsfRequest), PD]
+        L.orc_psf_fields.restype = I32
+        L.orc_psf_normalize.argtypes = [PF, I32, I32]
+        L.orc_block_sums.argtypes = [PF, I32, I32, PF]
+        L.orc_nearest_z.argtypes = [I32, D, D, C.POINTER(I32)]
+        L.orc_nearest_z.restype = I32
+        L.orc_splat_setup.argtypes = [I32, I32, D, D, I32, C.POINTER(SplatSetupT)]
+        L.orc_render_frame.argtypes = [PF, U32, U32, D, PF, D, C.c_void_p, I32, PF, I32, I32, I32, I32, I32, PI16,
+                                       PI16, D]
+        L.orc_wf_radius.argtypes = [I32, I32, I32]
+        L.orc_wf_radius.restype = I32
+        L.orc_wf_kernel.argtypes = [PF, I32, I32, I32, I32, PF]
+        L.orc_wf_direct.argtypes = [PF, I32, I32, I32, D, I64, PF, I32, I32, D, PD, I32, PD]
+
+    # -- RNG --
+    def pcg4d(self, a, b, c, d):
+        v = (C.c_uint32 * 4)(a & 0xFFFFFFFF, b & 0xFFFFFFFF, c & 0xFFFFFFFF, d & 0xFFFFFFFF)
+        self.L.orc_pcg4d(v)
+        return tuple(v)
+
+    def hash_unit(self, seed, cx, cy, k):
+        return self.L.orc_hash_unit(seed & 0xFFFFFFFF, _i32(cx), _i32(cy), k & 0xFFFFFFFF)
+
+    def hash_stream(self, seed, cx, cy, base, n):
+        out = np.zeros(n)
+        self.L.orc_hash_stream(seed & 0xFFFFFFFF, _i32(cx), _i32(cy), base & 0xFFFFFFFF, n, _p(out, C.c_double))
+        return out
+
+    def counter_uniforms(self, seed, frame, pixel, n):
+        out = np.zeros(n)
+        self.L.orc_counter_uniforms(seed, frame, pixel, n, _p(out, C.c_double))
+        return out
+
+    def counter_gauss(self, seed, frame, pixel, n):
+        out = np.zeros(n)
+        self.L.orc_counter_gauss(seed, frame, pixel, n, _p(out, C.c_double))
+        return out
+
+    def counter_poisson(self, seed, frame, pixel, l):
+        return self.L.orc_counter_poisson(seed, frame, pixel, l)
+
+    def counter_gamma(self, seed, frame, pixel, k):
+        return self.L.orc_counter_gamma(seed, frame, pixel, k)
+
+    # -- noise --
+    def noise(self, photons, cam: Camera, seed, frame, offset=None, gain=None, read=None, pixel_base=0):
+        photons = np.ascontiguousarray(photons, dtype=np.float32)
+        h, w = photons.shape
+        out = np.zeros((h, w), np.uint16)
+        f = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        offset, gain, read = f(offset), f(gain), f(read)
+        self.L.orc_noise_chain(_p(photons, C.c_float), _p(out, C.c_uint16), w, h, C.byref(cam),
+                               _p(offset, C.c_float), _p(gain, C.c_float), _p(read, C.c_float), seed, frame,
+                               pixel_base)
+        return out
+
+    # -- PSF --
+    def psf_geometry(self, req):
+        s, nz, m = C.c_int32(), C.c_int32(), C.c_int32()
+        self.L.orc_psf_geometry(C.byref(req), C.byref(s), C.byref(nz), C.byref(m))
+        return s.value, nz.value, m.value
+
+    def psf_planes(self, req, normalize=False):
+        size, nz, _ = self.psf_geometry(req)
+        out = np.zeros((nz, size, size), np.float32)
+        if self.L.orc_psf_planes(C.byref(req), _p(out, C.c_float)) != 0:
+            raise ValueError("bad PSF request")
+        if normalize:
+            self.L.orc_psf_normalize(_p(out, C.c_float), nz, size)
+        return out
+
+    def psf_fields(self, req):
+        size, nz, _ = self.psf_geometry(req)
+        out = np.zeros((nz, size, size, 2), np.float64)
+        if self.L.orc_psf_fields(C.byref(req), _p(out, C.c_double)) != 0:
+            raise ValueError("bad PSF request")
+        return out[..., 0] + 1j * out[..., 1]
+
+    def normalize(self, planes):
+        planes = np.ascontiguousarray(planes, np.float32).copy()
+        self.L.orc_psf_normalize(_p(planes, C.c_float), planes.shape[0], planes.shape[1])
+        return planes
+
+    def block_sums(self, kernel, os_):
+        kernel = np.ascontiguousarray(kernel, np.float32)
+        n = kernel.shape[0]
+        out = np.zeros((n + os_ - 1, n + os_ - 1), np.float32)
+        self.L.orc_block_sums(_p(kernel, C.c_float), n, os_, _p(out, C.c_float))
+        return out
+
+    def nearest_z(self, nz, z_step_nm, z_um):
+        c = C.c_int32()
+        return self.L.orc_nearest_z(nz, z_step_nm, z_um, C.byref(c)), bool(c.value)
+
+    def splat_setup(self, size, os_, x, y, interp):
+        r = SplatSetupT()
+        self.L.orc_splat_setup(size, os_, x, y, interp, C.byref(r))
+        return r
+
+    def render_frame(self, w, h, bg, emitters, block_sums, size, os_, interp, bg_map=None, bg_scale=1.0,
+                     span_lo=None, span_hi=None, min_photons=0.0):
+        emitters = np.ascontiguousarray(emitters, dtype=EMITTER_DTYPE)
+        bs = np.ascontiguousarray(block_sums, np.float32)
+        img = np.zeros((h, w), np.float32)
+        bg_map = None if bg_map is None else np.ascontiguousarray(bg_map, np.float32)
+        lo = None if span_lo is None else np.ascontiguousarray(span_lo, np.int16)
+        hi = None if span_hi is None else np.ascontiguousarray(span_hi, np.int16)
+        self.L.orc_render_frame(_p(img, C.c_float), w, h, bg, _p(bg_map, C.c_float), bg_scale,
+                                emitters.ctypes.data if len(emitters) else None, len(emitters),
+                                _p(bs, C.c_float), bs.shape[0], bs.shape[1], os_, size, interp,
+                                _p(lo, C.c_int16), _p(hi, C.c_int16), min_photons)
+        return img
+
+    # -- mean field --
+    def wf_radius(self, size, os_, upscale):
+        return self.L.orc_wf_radius(size, os_, upscale)
+
+    def wf_kernel(self, plane, os_, upscale, R):
+        plane = np.ascontiguousarray(plane, np.float32)
+        out = np.zeros((2 * R + 1, 2 * R + 1), np.float32)
+        self.L.orc_wf_kernel(_p(plane, C.c_float), plane.shape[0], os_, upscale, R, _p(out, C.c_float))
+        return out
+
+    def wf_direct(self, density, dz_um, k0, kernels, z_step_nm, foci_um):
+        density = np.ascontiguousarray(density, np.float32)
+        kernels = np.ascontiguousarray(kernels, np.float32)
+        foci = np.ascontiguousarray(foci_um, np.float64)
+        nzD, ny, nx = density.shape
+        nzP, D, _ = kernels.shape
+        out = np.zeros((len(foci), ny, nx))
+        self.L.orc_wf_direct(_p(density, C.c_float), nx, ny, nzD, dz_um, k0, _p(kernels, C.c_float), nzP,
+                             (D - 1) // 2, z_step_nm, _p(foci, C.c_double), len(foci), _p(out, C.c_double))
+        return out
+
+
+class Reference:
+    """The reference's own CPU functions (ref_shim.cpp over the compiled reference sources)."""
+
+    def __init__(self, path: str = REF_PATH):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = C.CDLL(path)
+        self.L = L
+        D, I32, U32, I64 = C.c_double, C.c_int32, C.c_uint32, C.c_int64
+        PF, PI16, PI32 = C.POINTER(C.c_float), C.POINTER(C.c_int16), C.POINTER(C.c_int32)
+        L.ref_psf_planes.argtypes = [C.POINTER(PsfRequest), PF]
+        L.ref_psf_planes.restype = I32
+        L.ref_psf_kernel_cache.argtypes = [C.POINTER(PsfRequest), PF, PF]
+        L.ref_psf_kernel_cache.restype = I32
+        L.ref_block_sums.argtypes = [PF, I32, I32, PF]
+        L.ref_noise_chain.argtypes = [PF, C.POINTER(C.c_uint16), U32, U32, C.POINTER(Camera), PF, PF, PF, U32, U32,
+                                      U32, I32]
+        L.ref_collect_emitters.argtypes = [C.c_void_p, I32, I64, U32, U32, D, D, I32, D, D, C.c_void_p, I32]
+        L.ref_collect_emitters.restype = I32
+        L.ref_render_frame.argtypes = [PF, U32, U32, C.c_void_p, I32, I64, D, D, D, PF, I32, I32, I32, D, I32, D, D,
+                                       D, I32]
+        L.ref_render_frame.restype = I32
+        L.ref_halo_spans.argtypes = [PF, I32, I32, I32, I32, D, PI16, PI16, PI32]
+        L.ref_halo_spans.restype = I32
+        L.ref_collect_gpu_emitters.argtypes = [C.c_void_p, I32, I64, U32, U32, D, D, PF, I32, I32, I32, D, I32, D,
+                                               C.c_void_p, I32]
+        L.ref_collect_gpu_emitters.restype = I32
+        L.ref_wf_kernel.argtypes = [PF, I32, I32, I32, D, I32, I32, I32, PF]
+        L.ref_wf_kernel.restype = I32
+        L.ref_render_noise_frames.argtypes = [C.c_void_p, I32, PI32, I32, U32, U32, D, D, PF, I32, I32, I32, D, I32,
+                                              C.POINTER(Camera), PF, U32, C.POINTER(C.c_uint16)]
+        L.ref_render_noise_frames.restype = I32
+        L.ref_psf_library.argtypes = [C.POINTER(PsfRequest), I32, PF]
+        L.ref_psf_library.restype = I32
+
+    def psf_planes(self, req, size, nz):
+        out = np.zeros((nz, size, size), np.float32)
+        if self.L.ref_psf_planes(C.byref(req), _p(out, C.c_float)) != 0:
+            raise ValueError("bad PSF request")
+        return out
+
+    def psf_kernel_cache(self, req, size, nz, os_):
+        bw = size + os_ - 1
+        planes = np.zeros((nz, size, size), np.float32)
+        bs = np.zeros((nz, bw, bw), np.float32)
+        if self.L.ref_psf_kernel_cache(C.byref(req), _p(planes, C.c_float), _p(bs, C.c_float)) != 0:
+            raise ValueError("bad PSF request")
+        return planes, bs
+
+    def block_sums(self, kernel, os_):
+        kernel = np.ascontiguousarray(kernel, np.float32)
+        n = kernel.shape[0]
+        out = np.zeros((n + os_ - 1, n + os_ - 1), np.float32)
+        self.L.ref_block_sums(_p(kernel, C.c_float), n, os_, _p(out, C.c_float))
+        return out
+
+    def noise(self, photons, cam, seed, frame, offset=None, gain=None, read=None, pixel_base=0, parallel=False):
+        photons = np.ascontiguousarray(photons, dtype=np.float32)
+        h, w = photons.shape
+        out = np.zeros((h, w), np.uint16)
+        f = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        offset, gain, read = f(offset), f(gain), f(read)
+        self.L.ref_noise_chain(_p(photons, C.c_float), _p(out, C.c_uint16), w, h, C.byref(cam),
+                               _p(offset, C.c_float), _p(gain, C.c_float), _p(read, C.c_float), seed, frame,
+                               pixel_base, int(parallel))
+        return out
+
+    def collect_emitters(self, events, frame, w, h, pixel_nm, photons_per_blink, nz, z_step_nm, global_z_um=0.0):
+        events = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+        out = np.zeros(max(1, len(events)), dtype=EMITTER_DTYPE)
+        n = self.L.ref_collect_emitters(events.ctypes.data, len(events), frame, w, h, pixel_nm, photons_per_blink,
+                                        nz, z_step_nm, global_z_um, out.ctypes.data, len(out))
+        return out[:n]
+
+    def render_frame(self, w, h, events, frame, pixel_nm, photons_per_blink, bg, planes, os_, z_step_nm, interp,
+                     global_z_um=0.0, halo=0.0, min_photons=0.0, parallel=False):
+        events = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+        planes = np.ascontiguousarray(planes, np.float32)
+        img = np.zeros((h, w), np.float32)
+        self.L.ref_render_frame(_p(img, C.c_float), w, h, events.ctypes.data, len(events), frame, pixel_nm,
+                                photons_per_blink, bg, _p(planes, C.c_float), planes.shape[0], planes.shape[1], os_,
+                                z_step_nm, interp, global_z_um, halo, min_photons, int(parallel))
+        return img
+
+    def halo_spans(self, planes, os_, interp, threshold):
+        planes = np.ascontiguousarray(planes, np.float32)
+        nz, size = planes.shape[0], planes.shape[1]
+        cam_rad = ((size - 1) // 2) // os_
+        n = 2 * cam_rad + 1
+        lo = np.zeros((nz, n), np.int16)
+        hi = np.zeros((nz, n), np.int16)
+        r2 = np.zeros(nz, np.int32)
+        r = self.L.ref_halo_spans(_p(planes, C.c_float), nz, size, os_, interp, threshold, _p(lo, C.c_int16),
+                                  _p(hi, C.c_int16), _p(r2, C.c_int32))
+        if r < 0:
+            raise ValueError("no halo cut")
+        return lo, hi, r2
+
+    def collect_gpu_emitters(self, events, frame, w, h, pixel_nm, photons_per_blink, planes, os_, z_step_nm, interp,
+                             global_z_um=0.0):
+        events = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+        planes = np.ascontiguousarray(planes, np.float32)
+        out = np.zeros(max(1, len(events)), dtype=GPU_EMITTER_DTYPE)
+        n = self.L.ref_collect_gpu_emitters(events.ctypes.data, len(events), frame, w, h, pixel_nm,
+                                            photons_per_blink, _p(planes, C.c_float), planes.shape[0],
+                                            planes.shape[1], os_, z_step_nm, interp, global_z_um, out.ctypes.data,
+                                            len(out))
+        return out[:n]
+
+    def wf_radius(self, planes, os_, upscale):
+        planes = np.ascontiguousarray(planes, np.float32)
+        return self.L.ref_wf_kernel(_p(planes, C.c_float), planes.shape[0], planes.shape[1], os_, 100.0, upscale, 0,
+                                    0, None)
+
+    def wf_kernel(self, planes, os_, upscale, p, R):
+        planes = np.ascontiguousarray(planes, np.float32)
+        out = np.zeros((2 * R + 1, 2 * R + 1), np.float32)
+        self.L.ref_wf_kernel(_p(planes, C.c_float), planes.shape[0], planes.shape[1], os_, 100.0, upscale, p, R,
+                             _p(out, C.c_float))
+        return out
+
+    def render_noise_frames(self, events, ev_start, w, h, pixel_nm, bg, planes, os_, z_step_nm, interp, cam, offset,
+                            seed):
+        events = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+        ev_start = np.ascontiguousarray(ev_start, np.int32)
+        planes = np.ascontiguousarray(planes, np.float32)
+        n_frames = len(ev_start) - 1
+        out = np.zeros((n_frames, h, w), np.uint16)
+        offset = None if offset is None else np.ascontiguousarray(offset, np.float32)
+        self.L.ref_render_noise_frames(events.ctypes.data, len(events), _p(ev_start, C.c_int32), n_frames, w, h,
+                                       pixel_nm, bg, _p(planes, C.c_float), planes.shape[0], planes.shape[1], os_,
+                                       z_step_nm, interp, C.byref(cam), _p(offset, C.c_float), seed,
+                                       _p(out, C.c_uint16))
+        return out
+
+    def psf_library(self, reqs, out=None):
+        arr = (PsfRequest * len(reqs))(*reqs)
+        return self.L.ref_psf_library(arr, len(reqs), _p(out, C.c_float))
+
+
+def _i32(v):
+    v &= 0xFFFFFFFF
+    return v - (1 << 32) if v >= (1 << 31) else v
+
+
+def load_oracle() -> Oracle:
+    return Oracle()
+
+
+def load_reference() -> Reference | None:
+    try:
+        return Reference()
+    except (FileNotFoundError, OSError):
+        return None

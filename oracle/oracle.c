@@ -1,0 +1,1009 @@
+/*
+ * oracle.c -- FP64 CPU restatement of inSiliScope's image-formation path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path (see oracle.h). It is
+ * compiled with -ffp-contract=off -fno-fast-math like the reference
+ * (core/CMakeLists.txt:41), and every expression keeps the reference's operand
+ * order, so where the reference is deterministic FP64 this file gives the same
+ * bits on the same libm.
+ *
+ * How each part is pinned (tests/test_oracle.py):
+ *   pcg4d / HashUnit / HashStream ... spec/golden/ref.txt `rng` and `stream` lines (bit-exact).
+ *   counter uniforms/Gauss/Poisson/Gamma, noise chain ... ADU fixtures written by the compiled reference
+ *                                     (oracle/_ref) in tests/golden (bit-exact); moments and closed forms.
+ *   PSF planes ....................... tests/psf/zernike_ref.bin (4 cases, rel L2 < 1e-6, same argmax); compiled
+ *                                     reference planes at the C1/C3 geometries (bit-exact); chirp-Z == direct DFT
+ *                                     on tiny grids; sum-1 after normalisation.
+ *   block sums / splat ............... compiled-reference frames in tests/golden (bit-exact); photon conservation;
+ *                                     block sums total os^2.
+ *   mean-field kernel + direct conv .. compiled-reference KernelWidefieldPsf planes (bit-exact); delta density
+ *                                     reproduces the kernel; linearity; brute force on tiny grids.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const double kPi = 3.14159265358979323846;
+
+static int imin(int a, int b) { return a < b ? a : b; }
+static int imax(int a, int b) { return a > b ? a : b; }
+
+/* ======================================================================= */
+/* RNG: core/src/rng.h:18-52, Sim/SMLMCounterRng.h:37-55                  */
+/* ======================================================================= */
+
+void orc_pcg4d(uint32_t v[4])
+{
+   uint32_t a = v[0], b = v[1], c = v[2], d = v[3];
+   a = a * 1664525u + 1013904223u;
+   b = b * 1664525u + 1013904223u;
+   c = c * 1664525u + 1013904223u;
+   d = d * 1664525u + 1013904223u;
+   a += b * d;
+   b += c * a;
+   c += a * b;
+   d += b * c;
+   a ^= a >> 16;
+   b ^= b >> 16;
+   c ^= c >> 16;
+   d ^= d >> 16;
+   a += b * d;
+   b += c * a;
+   c += a * b;
+   d += b * c;
+   v[0] = a;
+   v[1] = b;
+   v[2] = c;
+   v[3] = d;
+}
+
+/* core/src/rng.h:37 */
+double orc_hash_unit(uint32_t seed, int32_t cx, int32_t cy, uint32_t k)
+{
+   uint32_t v[4] = {seed, (uint32_t)cx, (uint32_t)cy, k};
+   orc_pcg4d(v);
+   return ((double)(v[0] >> 9) + 0.5) * 1.1920928955078125e-7;
+}
+
+/* core/src/rng.h:45-52: draw i = HashUnit(seed, cx, cy, base*4096 + i), uint32 wrap. */
+void orc_hash_stream(uint32_t seed, int32_t cx, int32_t cy, uint32_t base, int32_t n, double* out)
+{
+   const uint32_t b = base * 4096u;
+   for (int32_t i = 0; i < n; ++i)
+      out[i] = orc_hash_unit(seed, cx, cy, b + (uint32_t)i);
+}
+
+/* Sim/SMLMCounterRng.h:59-89 (CounterRng). */
+typedef struct
+{
+   uint32_t seed, frame, pix, ctr;
+   uint32_t buf[4];
+   int k;
+} crng;
+
+static void crng_init(crng* u, uint32_t seed, uint32_t frame, uint32_t pixel)
+{
+   u->seed = seed;
+   u->frame = frame;
+   u->pix = pixel;
+   u->ctr = 0;
+   u->k = 4;
+}
+
+static double crng_uniform(crng* u)
+{
+   if (u->k == 4)
+   {
+      u->buf[0] = u->seed;
+      u->buf[1] = u->frame;
+      u->buf[2] = u->pix;
+      u->buf[3] = u->ctr;
+      orc_pcg4d(u->buf);
+      ++u->ctr;
+      u->k = 0;
+   }
+   return ((u->buf[u->k++] >> 9) + 0.5) * 1.1920928955078125e-7;
+}
+
+/* Sim/SMLMCounterRng.h:95-99 */
+static double crng_gauss(crng* u)
+{
+   double r = sqrt(-2.0 * log(crng_uniform(u)));
+   return r * cos(6.283185307179586 * crng_uniform(u));
+}
+
+/* Sim/SMLMCounterRng.h:104-119 */
+static double crng_poisson(double l, crng* u)
+{
+   if (l > 60.0)
+   {
+      double v = floor(l + sqrt(l) * crng_gauss(u) + 0.5);
+      return v > 0.0 ? v : 0.0;
+   }
+   if (!(l > 0.0))
+      return 0.0;
+   const double L = exp(-l);
+   int k = 0;
+   double p = 1.0;
+   do
+   {
+      ++k;
+      p *= crng_uniform(u);
+   } while (p > L && k < 1000);
+   return (double)(k - 1);
+}
+
+/* Sim/SMLMCounterRng.h:123-138 */
+static double crng_gamma(double k, crng* u)
+{
+   const double d = k - 1.0 / 3.0, c = 1.0 / sqrt(9.0 * d);
+   for (int it = 0; it < 1000; ++it)
+   {
+      const double x = crng_gauss(u), t = 1.0 + c * x;
+      if (t <= 0.0)
+         continue;
+      const double v = t * t * t, w = crng_uniform(u);
+      if (w < 1.0 - 0.0331 * x * x * x * x)
+         return d * v;
+      if (log(w) < 0.5 * x * x + d * (1.0 - v + log(v)))
+         return d * v;
+   }
+   return d;
+}
+
+void orc_counter_uniforms(uint32_t seed, uint32_t frame, uint32_t pixel, int32_t n, double* out)
+{
+   crng u;
+   crng_init(&u, seed, frame, pixel);
+   for (int32_t i = 0; i < n; ++i)
+      out[i] = crng_uniform(&u);
+}
+
+void orc_counter_gauss(uint32_t seed, uint32_t frame, uint32_t pixel, int32_t n, double* out)
+{
+   crng u;
+   crng_init(&u, seed, frame, pixel);
+   for (int32_t i = 0; i < n; ++i)
+      out[i] = crng_gauss(&u);
+}
+
+double orc_counter_poisson(uint32_t seed, uint32_t frame, uint32_t pixel, double l)
+{
+   crng u;
+   crng_init(&u, seed, frame, pixel);
+   return crng_poisson(l, &u);
+}
+
+double orc_counter_gamma(uint32_t seed, uint32_t frame, uint32_t pixel, double k)
+{
+   crng u;
+   crng_init(&u, seed, frame, pixel);
+   return crng_gamma(k, &u);
+}
+
+/* ======================================================================= */
+/* Camera noise: Sim/SMLMNoise.cpp:142-216                                 */
+/* ======================================================================= */
+
+void orc_noise_chain(const float* photons, uint16_t* out, uint32_t width, uint32_t height, const orc_camera* cam,
+                     const float* offsetMap, const float* gainMap, const float* readMap, uint32_t seed,
+                     uint32_t frame, uint32_t pixelBase)
+{
+   const size_t n = (size_t)width * height;
+   crng u;
+   crng_init(&u, seed, frame, 0);
+   if (cam->emccd)
+   {
+      const int bits = imin(16, imax(1, cam->bitDepth));
+      const double maxAdu = ldexp(1.0, bits) - 1.0;
+      const double emGain = cam->emGain > 1.0 ? cam->emGain : 1.0;
+      for (size_t i = 0; i < n; ++i)
+      {
+         crng_init(&u, seed, frame, (uint32_t)i + pixelBase);
+         const double ph0 = (double)photons[i];
+         const double ph = ph0 > 0.0 ? ph0 : 0.0;
+         const double ne =
+            crng_poisson(ph * cam->quantumEfficiency + cam->darkCurrentElectrons + cam->cicElectrons, &u);
+         const double o = ne > 0.0 ? crng_gamma(ne, &u) : 0.0;
+         const double readE = (readMap ? (double)readMap[i] : cam->readNoiseElectrons) / emGain;
+         const double gain = gainMap ? (double)gainMap[i] : cam->gainPhotonsPerAdu;
+         const double offset = offsetMap ? (double)offsetMap[i] : 0.0;
+         const double g = crng_gauss(&u);
+         double adu = floor(offset + (o + readE * g) / gain + 0.5);
+         out[i] = (uint16_t)(adu < 0.0 ? 0.0 : (adu > maxAdu ? maxAdu : adu));
+      }
+      return;
+   }
+   for (size_t i = 0; i < n; ++i)
+   {
+      crng_init(&u, seed, frame, (uint32_t)i + pixelBase);
+      const double ph0 = (double)photons[i];
+      const double ph = ph0 > 0.0 ? ph0 : 0.0;
+      const double ne = crng_poisson(ph * cam->quantumEfficiency + cam->darkCurrentElectrons, &u);
+      const double readNoise = readMap ? (double)readMap[i] : cam->readNoiseElectrons;
+      const double gain = gainMap ? (double)gainMap[i] : cam->gainPhotonsPerAdu;
+      const double offset = offsetMap ? (double)offsetMap[i] : 0.0;
+      const double g = crng_gauss(&u);
+      double adu = offset + (ne + readNoise * g) / gain;
+      adu = adu < 0.0 ? 0.0 : (adu > 65535.0 ? 65535.0 : adu);
+      out[i] = (uint16_t)floor(adu + 0.5);
+   }
+}
+
+/* ======================================================================= */
+/* Radix-2 FFT and chirp-Z: Sim/FftRadix2.h:39-113, Sim/ChirpZ.h:28-141     */
+/* ======================================================================= */
+
+typedef struct
+{
+   int n;
+   double *c, *s;
+   int* rev;
+} twiddles;
+
+/* Sim/FftRadix2.h:39 MakeTwiddles */
+static void make_twiddles(twiddles* t, int n, int sign)
+{
+   t->n = n;
+   t->c = (double*)calloc((size_t)imax(1, n - 1), sizeof(double));
+   t->s = (double*)calloc((size_t)imax(1, n - 1), sizeof(double));
+   t->rev = (int*)calloc((size_t)imax(1, n), sizeof(int));
+   for (int len = 2; len <= n; len <<= 1)
+   {
+      const double ang = sign * 2.0 * 3.14159265358979323846 / len;
+      const double wr = cos(ang), wi = sin(ang);
+      const int half = len / 2;
+      double cr = 1.0, ci = 0.0;
+      for (int k = 0; k < half; ++k)
+      {
+         t->c[half - 1 + k] = cr;
+         t->s[half - 1 + k] = ci;
+         const double nr = cr * wr - ci * wi;
+         ci = cr * wi + ci * wr;
+         cr = nr;
+      }
+   }
+   for (int i = 1, j = 0; i < n; ++i)
+   {
+      int bit = n >> 1;
+      for (; j & bit; bit >>= 1)
+         j ^= bit;
+      j ^= bit;
+      t->rev[i] = j;
+   }
+}
+
+static void free_twiddles(twiddles* t)
+{
+   free(t->c);
+   free(t->s);
+   free(t->rev);
+}
+
+/* Sim/FftRadix2.h:74 Fft1d */
+static void fft1d(double* re, double* im, const twiddles* tw)
+{
+   const int n = tw->n;
+   for (int i = 1; i < n; ++i)
+   {
+      const int j = tw->rev[i];
+      if (i < j)
+      {
+         double t = re[i];
+         re[i] = re[j];
+         re[j] = t;
+         t = im[i];
+         im[i] = im[j];
+         im[j] = t;
+      }
+   }
+   for (int len = 2; len <= n; len <<= 1)
+   {
+      const int half = len / 2;
+      const double* tc = tw->c + (half - 1);
+      const double* ts = tw->s + (half - 1);
+      for (int i = 0; i < n; i += len)
+      {
+         double* ar = re + i;
+         double* ai = im + i;
+         double* br = re + i + half;
+         double* bi = im + i + half;
+         for (int k = 0; k < half; ++k)
+         {
+            const double cr = tc[k], ci = ts[k];
+            const double vr = br[k] * cr - bi[k] * ci, vi = br[k] * ci + bi[k] * cr;
+            br[k] = ar[k] - vr;
+            bi[k] = ai[k] - vi;
+            ar[k] += vr;
+            ai[k] += vi;
+         }
+      }
+   }
+}
+
+typedef struct
+{
+   int M, P, L;
+   double invL;
+   twiddles fwd, inv;
+   double *preC, *preS, *gRe, *gIm, *c1, *s1, *c2, *s2;
+} czt_plan;
+
+/* Sim/ChirpZ.h:40-84 CztPlan(M, dk, k0, P, dx, x0) */
+static void czt_init(czt_plan* p, int M, double dk, double k0, int P, double dx, double x0)
+{
+   p->M = M;
+   p->P = P;
+   const double theta = dk * dx;
+   int L = 1;
+   while (L < M + P - 1)
+      L <<= 1;
+   p->L = L;
+   p->invL = 1.0 / L;
+   make_twiddles(&p->fwd, L, -1);
+   make_twiddles(&p->inv, L, 1);
+   p->preC = (double*)malloc(sizeof(double) * (size_t)M);
+   p->preS = (double*)malloc(sizeof(double) * (size_t)M);
+   for (int m = 0; m < M; ++m)
+   {
+      const double ang = m * dk * x0 + theta * m * m / 2.0;
+      p->preC[m] = cos(ang);
+      p->preS[m] = sin(ang);
+   }
+   p->gRe = (double*)calloc((size_t)L, sizeof(double));
+   p->gIm = (double*)calloc((size_t)L, sizeof(double));
+   for (int n = -(M - 1); n < P; ++n)
+   {
+      const double ang = -theta * n * n / 2.0;
+      const int idx = n >= 0 ? n : L + n;
+      p->gRe[idx] = cos(ang);
+      p->gIm[idx] = sin(ang);
+   }
+   fft1d(p->gRe, p->gIm, &p->fwd);
+   p->c1 = (double*)malloc(sizeof(double) * (size_t)P);
+   p->s1 = (double*)malloc(sizeof(double) * (size_t)P);
+   p->c2 = (double*)malloc(sizeof(double) * (size_t)P);
+   p->s2 = (double*)malloc(sizeof(double) * (size_t)P);
+   for (int q = 0; q < P; ++q)
+   {
+      const double a1 = theta * q * q / 2.0;
+      p->c1[q] = cos(a1);
+      p->s1[q] = sin(a1);
+      const double a2 = k0 * (x0 + q * dx);
+      p->c2[q] = cos(a2);
+      p->s2[q] = sin(a2);
+   }
+}
+
+static void czt_free(czt_plan* p)
+{
+   free_twiddles(&p->fwd);
+   free_twiddles(&p->inv);
+   free(p->preC);
+   free(p->preS);
+   free(p->gRe);
+   free(p->gIm);
+   free(p->c1);
+   free(p->s1);
+   free(p->c2);
+   free(p->s2);
+}
+
+/* Sim/ChirpZ.h:91-141 Apply, one line (each batched line performs exactly these operations). The reference's
+ * pruned first stages give the same values up to the sign of zeros, which |E|^2 cannot see (Sim/FftRadix2.h:110-118). */
+static void czt_apply(const czt_plan* p, const double* inRe, const double* inIm, double* outRe, double* outIm,
+                      double* aRe, double* aIm)
+{
+   const int L = p->L;
+   memset(aRe, 0, sizeof(double) * (size_t)L);
+   memset(aIm, 0, sizeof(double) * (size_t)L);
+   for (int m = 0; m < p->M; ++m)
+   {
+      const double cr = p->preC[m], ci = p->preS[m];
+      const double re = inRe[m], im = inIm[m];
+      aRe[m] = re * cr - im * ci;
+      aIm[m] = re * ci + im * cr;
+   }
+   fft1d(aRe, aIm, &p->fwd);
+   for (int i = 0; i < L; ++i)
+   {
+      const double gr = p->gRe[i], gi = p->gIm[i];
+      const double re = aRe[i] * gr - aIm[i] * gi;
+      const double im = aRe[i] * gi + aIm[i] * gr;
+      aRe[i] = re;
+      aIm[i] = im;
+   }
+   fft1d(aRe, aIm, &p->inv);
+   for (int q = 0; q < p->P; ++q)
+   {
+      const double cc1 = p->c1[q], ss1 = p->s1[q];
+      const double cc2 = p->c2[q], ss2 = p->s2[q];
+      const double convRe = aRe[q] * p->invL, convIm = aIm[q] * p->invL;
+      const double sRe = convRe * cc1 - convIm * ss1, sIm = convRe * ss1 + convIm * cc1;
+      outRe[q] = sRe * cc2 - sIm * ss2;
+      outIm[q] = sRe * ss2 + sIm * cc2;
+   }
+}
+
+/* ======================================================================= */
+/* Zernike basis and pupil: Sim/ZernikePsf.cpp:35-110                      */
+/* ======================================================================= */
+
+/* :35 IndexToNM */
+static void index_to_nm(int j, int* n, int* m)
+{
+   for (int nn = 0;; ++nn)
+      for (int l = 0; l <= nn; ++l)
+         if (nn * (nn + 1) / 2 + l == j)
+         {
+            *n = nn;
+            *m = -nn + 2 * l;
+            return;
+         }
+}
+
+/* :47 Binomial */
+static double binomial(int a, int b)
+{
+   if (b < 0 || b > a)
+      return 0.0;
+   double r = 1.0;
+   for (int i = 0; i < b; ++i)
+      r = r * (a - i) / (i + 1);
+   return r;
+}
+
+/* :58 ZernikeRadial (unnormalised, m >= 0) */
+static double zernike_radial(int n, int m, double rho)
+{
+   double r = 0.0;
+   for (int k = 0; k <= (n - m) / 2; ++k)
+   {
+      const double coeff = (k % 2 == 0 ? 1.0 : -1.0) * binomial(n - k, k) * binomial(n - 2 * k, (n - m) / 2 - k);
+      r += coeff * pow(rho, (double)(n - 2 * k));
+   }
+   return r;
+}
+
+/* :74 ZernikeValue */
+static double zernike_value(int n, int l, int m, double rho, double phi)
+{
+   const double radial = rho <= 1.0 ? zernike_radial(n, m, rho) : 0.0;
+   return radial * (l >= 0 ? cos(m * phi) : sin(m * phi));
+}
+
+/* :80 LaguerreL */
+static double laguerre(int p, int a, double x)
+{
+   double lm1 = 0.0, l = 1.0;
+   for (int k = 0; k < p; ++k)
+   {
+      const double next = ((2 * k + 1 + a - x) * l - (k + a) * lm1) / (k + 1);
+      lm1 = l;
+      l = next;
+   }
+   return l;
+}
+
+/* :93 MaskPhase */
+static double mask_phase(int doubleHelix, int maskModes, double maskWaist, double rhoNorm, double phi)
+{
+   if (!doubleHelix)
+      return 0.0;
+   const int n = imax(2, maskModes);
+   const double w = maskWaist > 0.2 ? maskWaist : 0.2;
+   const double rc = rhoNorm < 1.0 ? rhoNorm : 1.0;
+   const double u = (0.0 < rc ? rc : 0.0) / w;
+   const double u2 = u * u;
+   double re = 0.0, im = 0.0;
+   for (int p = 0; p < n; ++p)
+   {
+      const int l = 2 * p + 1;
+      const double amp = pow(u, (double)l) * exp(-u2) * laguerre(p, l, 2.0 * u2);
+      re += amp * cos(l * phi);
+      im += amp * sin(l * phi);
+   }
+   return (re == 0.0 && im == 0.0) ? 0.0 : atan2(im, re);
+}
+
+/* :118-130 GeometryFor, :134-148 ZernikePupilSamples */
+int32_t orc_psf_geometry(const orc_psf_request* r, int32_t* size, int32_t* nz, int32_t* M)
+{
+   const int os = imax(1, r->oversampling);
+   const int camHalf = imax(1, r->kernelHalfWidthPx);
+   const int halfOv = camHalf * os;
+   const int sz = 2 * halfOv + 1;
+   const int nzWanted = imax(3, r->nz);
+   if (size)
+      *size = sz;
+   if (nz)
+      *nz = (nzWanted % 2 == 1) ? nzWanted : nzWanted + 1;
+   if (M)
+   {
+      if (r->pupilSamples > 0)
+         *M = imax(8, imin(1024, r->pupilSamples));
+      else
+      {
+         const double diagonalUm = 2.0 * sqrt(2.0) * camHalf * r->pixelSizeNm / 1000.0;
+         const double m = 4.0 + 2.0 * r->na * diagonalUm / (r->wavelengthNm / 1000.0);
+         const int need = imax(64, imin(512, 4 * (int)ceil(m / 4.0)));
+         int fftLen = 1;
+         while (fftLen < need + sz - 1)
+            fftLen <<= 1;
+         *M = imax(need, imin(512, (fftLen - sz + 1) / 4 * 4));
+      }
+   }
+   return 0;
+}
+
+/* ComputeZernikePsfPlanes (Sim/ZernikePsf.cpp:150-337) up to the complex field of every plane. Writes either the
+ * fields (re, im interleaved, when fields != NULL) or the |E|^2 planes (when planes != NULL). */
+static int32_t psf_compute(const orc_psf_request* req, double* fields, float* planes)
+{
+   if (!(req->na > 0.0) || !(req->wavelengthNm > 0.0) || !(req->immersionIndex > 0.0) || !(req->sampleIndex > 0.0) ||
+       !(req->pixelSizeNm > 0.0) || !(req->zStepNm > 0.0))
+      return -1;
+   int size, nz, kFftM;
+   orc_psf_geometry(req, &size, &nz, &kFftM);
+   const int os = imax(1, req->oversampling);
+   const double resLateralNm = req->pixelSizeNm / os;
+
+   const double NA = req->na;
+   const double lambda = req->wavelengthNm * 1E-9;
+   const double ni = req->immersionIndex;
+   const double ns = req->sampleIndex;
+   const double ti0 = req->workingDistanceUm * 1E-6;
+   const double particleAxialPosition = req->sampleDepthNm * 1E-9;
+   const double resAxialM = req->zStepNm * 1E-9;
+   const double resLateralM = resLateralNm * 1E-9;
+   const int doubleHelix = req->maskType == 1;
+
+   /* the nonzero terms, in index order (:178-188) */
+   int nModes = 0;
+   double mc[28];
+   int mn[28], ml[28], mm[28];
+   for (int j = 0; j < 28; ++j)
+   {
+      if (req->zernike[j] == 0.0)
+         continue;
+      mc[nModes] = req->zernike[j];
+      index_to_nm(j, &mn[nModes], &ml[nModes]);
+      mm[nModes] = abs(ml[nModes]);
+      ++nModes;
+   }
+
+   const double k0 = 2.0 * kPi / lambda;
+   const double bMax = (ns / NA < 1.0) ? ns / NA : 1.0;
+   const double kMax = k0 * NA * bMax;
+   const double dk = (2.0 * kMax) / (kFftM - 4);
+   const double kMin = -floor(kFftM / 2.0) * dk;
+   const int nx = size, ny = size;
+   const double x0m = -((nx - 1) / 2.0) * resLateralM;
+   czt_plan plan;
+   czt_init(&plan, kFftM, dk, kMin, nx, resLateralM, x0m);
+
+   const int M = kFftM, c0 = kFftM / 2;
+   const size_t MM = (size_t)M * M;
+   char* inside = (char*)calloc(MM, 1);
+   double* opd1Tab = (double*)calloc(MM, sizeof(double));
+   double* opd3Factor = (double*)calloc(MM, sizeof(double));
+   double* zernTab = (double*)calloc(MM, sizeof(double));
+   double* maskTab = (double*)calloc(MM, sizeof(double));
+   for (int iy = 0; iy < M; ++iy)
+   {
+      const double ky = (iy - c0) * dk;
+      for (int ix = 0; ix < M; ++ix)
+      {
+         const double kx = (ix - c0) * dk;
+         const double kr2 = kx * kx + ky * ky;
+         if (kr2 > kMax * kMax)
+            continue;
+         const size_t idx = (size_t)iy * M + ix;
+         inside[idx] = 1;
+         const double kr = sqrt(kr2);
+         const double rho = kr / (k0 * NA);
+         const double phi = atan2(ky, kx);
+         const double s1 = NA * rho / ns;
+         const double s3 = NA * rho / ni;
+         const double a1 = 1.0 - s1 * s1, a3 = 1.0 - s3 * s3;
+         opd1Tab[idx] = ns * particleAxialPosition * sqrt(0.0 < a1 ? a1 : 0.0);
+         opd3Factor[idx] = sqrt(0.0 < a3 ? a3 : 0.0);
+         double zernikePhase = 0.0;
+         for (int t = 0; t < nModes; ++t)
+            zernikePhase += mc[t] * zernike_value(mn[t], ml[t], mm[t], rho / bMax, phi);
+         zernikePhase *= 2.0 * kPi;
+         zernTab[idx] = zernikePhase;
+         maskTab[idx] = mask_phase(doubleHelix, req->maskModes, req->maskWaist, rho / bMax, phi);
+      }
+   }
+
+   const double focalShift = particleAxialPosition * (ni / ns);
+   double* pupilRe = (double*)malloc(sizeof(double) * MM);
+   double* pupilIm = (double*)malloc(sizeof(double) * MM);
+   double* midRe = (double*)malloc(sizeof(double) * (size_t)M * ny);
+   double* midIm = (double*)malloc(sizeof(double) * (size_t)M * ny);
+   double* lineRe = (double*)malloc(sizeof(double) * (size_t)M);
+   double* lineIm = (double*)malloc(sizeof(double) * (size_t)M);
+   double* oRe = (double*)malloc(sizeof(double) * (size_t)imax(nx, ny));
+   double* oIm = (double*)malloc(sizeof(double) * (size_t)imax(nx, ny));
+   double* aRe = (double*)malloc(sizeof(double) * (size_t)plan.L);
+   double* aIm = (double*)malloc(sizeof(double) * (size_t)plan.L);
+   for (int z = 0; z < nz; ++z)
+   {
+      const double ti = (ti0 - focalShift) + resAxialM * (z - (nz - 1.0) / 2.0);
+      for (size_t idx = 0; idx < MM; ++idx)
+      {
+         pupilRe[idx] = 0.0;
+         pupilIm[idx] = 0.0;
+         if (!inside[idx])
+            continue;
+         const double opd3 = ni * (ti - ti0) * opd3Factor[idx];
+         const double phase = k0 * (opd1Tab[idx] + opd3) + zernTab[idx] + maskTab[idx];
+         pupilRe[idx] = cos(phase);
+         pupilIm[idx] = sin(phase);
+      }
+      /* row pass: one CZT over ky per pupil column (:265-312); all-zero columns give exact zeros */
+      for (int m = 0; m < M; ++m)
+      {
+         for (int n = 0; n < M; ++n)
+         {
+            lineRe[n] = pupilRe[(size_t)n * M + m];
+            lineIm[n] = pupilIm[(size_t)n * M + m];
+         }
+         czt_apply(&plan, lineRe, lineIm, oRe, oIm, aRe, aIm);
+         for (int y = 0; y < ny; ++y)
+         {
+            midRe[(size_t)y * M + m] = oRe[y];
+            midIm[(size_t)y * M + m] = oIm[y];
+         }
+      }
+      /* column pass: one CZT over kx per intermediate row (:313-334) */
+      for (int y = 0; y < ny; ++y)
+      {
+         czt_apply(&plan, midRe + (size_t)y * M, midIm + (size_t)y * M, oRe, oIm, aRe, aIm);
+         for (int x = 0; x < nx; ++x)
+         {
+            const size_t o = ((size_t)z * ny + y) * nx + x;
+            if (planes)
+               planes[o] = (float)(oRe[x] * oRe[x] + oIm[x] * oIm[x]);
+            if (fields)
+            {
+               fields[2 * o] = oRe[x];
+               fields[2 * o + 1] = oIm[x];
+            }
+         }
+      }
+   }
+   free(pupilRe);
+   free(pupilIm);
+   free(midRe);
+   free(midIm);
+   free(lineRe);
+   free(lineIm);
+   free(oRe);
+   free(oIm);
+   free(aRe);
+   free(aIm);
+   free(inside);
+   free(opd1Tab);
+   free(opd3Factor);
+   free(zernTab);
+   free(maskTab);
+   czt_free(&plan);
+   return 0;
+}
+
+int32_t orc_psf_planes(const orc_psf_request* r, float* out) { return psf_compute(r, NULL, out); }
+
+int32_t orc_psf_fields(const orc_psf_request* r, double* out) { return psf_compute(r, out, NULL); }
+
+/* Sim/ZernikePsf.cpp:401-410 */
+void orc_psf_normalize(float* planes, int32_t nz, int32_t n)
+{
+   const size_t plane = (size_t)n * n;
+   for (int32_t z = 0; z < nz; ++z)
+   {
+      float* p = planes + (size_t)z * plane;
+      double sum = 0.0;
+      for (size_t i = 0; i < plane; ++i)
+         sum += p[i];
+      if (sum > 0.0)
+         for (size_t i = 0; i < plane; ++i)
+            p[i] = (float)(p[i] / sum);
+   }
+}
+
+/* ======================================================================= */
+/* Kernel cache and splat: Sim/PsfGeneratorBridge.cpp, Sim/SplatKernel.inl */
+/* ======================================================================= */
+
+/* Sim/PsfGeneratorBridge.cpp:1021-1050 BuildBlockSums */
+void orc_block_sums(const float* kernel, int32_t n, int32_t os, float* out)
+{
+   const int off = os - 1, bw = n + off;
+   double* hsum = (double*)malloc(sizeof(double) * (size_t)bw * n);
+   for (int y = 0; y < n; ++y)
+      for (int bi = 0; bi < bw; ++bi)
+      {
+         const int b = bi - off, x0 = imax(0, b), x1 = imin(n - 1, b + off);
+         double acc = 0.0;
+         for (int x = x0; x <= x1; ++x)
+            acc += kernel[(size_t)y * n + x];
+         hsum[(size_t)y * bw + bi] = acc;
+      }
+   for (int ai = 0; ai < bw; ++ai)
+   {
+      const int a = ai - off, y0 = imax(0, a), y1 = imin(n - 1, a + off);
+      for (int bi = 0; bi < bw; ++bi)
+      {
+         double acc = 0.0;
+         for (int y = y0; y <= y1; ++y)
+            acc += hsum[(size_t)y * bw + bi];
+         out[(size_t)ai * bw + bi] = (float)acc;
+      }
+   }
+   free(hsum);
+}
+
+/* Sim/PsfGeneratorBridge.cpp:40-60 PsfKernelCache::NearestZIndex */
+int32_t orc_nearest_z(int32_t nz, double zStepNm, double zUm, int32_t* clamped)
+{
+   if (clamped)
+      *clamped = 0;
+   if (nz <= 1 || zStepNm <= 0.0)
+      return 0;
+   const double idxF = zUm * 1000.0 / zStepNm + (nz - 1) / 2.0;
+   int idx = (int)lround(idxF);
+   if (idx < 0)
+   {
+      idx = 0;
+      if (clamped)
+         *clamped = 1;
+   }
+   if (idx >= nz)
+   {
+      idx = nz - 1;
+      if (clamped)
+         *clamped = 1;
+   }
+   return idx;
+}
+
+/* :1250 CatmullRomWeights */
+static void catmull_rom(double f, double* w)
+{
+   const double f2 = f * f, f3 = f2 * f;
+   w[0] = -0.5 * f3 + f2 - 0.5 * f;
+   w[1] = 1.5 * f3 - 2.5 * f2 + 1.0;
+   w[2] = -1.5 * f3 + 2.0 * f2 + 0.5 * f;
+   w[3] = 0.5 * f3 - 0.5 * f2;
+}
+
+/* :1382-1422 SplatSetup (interp 0 nearest, 1 linear, 2 cubic) */
+void orc_splat_setup(int32_t sizeOversampled, int32_t os_, double xPx, double yPx, int32_t interp,
+                     orc_splat_setup_t* r)
+{
+   memset(r, 0, sizeof(*r));
+   const int os = imax(1, os_);
+   const double kc = (sizeOversampled - 1) / 2.0;
+   r->x0 = (int)floor(xPx + 0.5);
+   r->y0 = (int)floor(yPx + 0.5);
+   const double tx = kc + (r->x0 - xPx - 0.5) * os + 0.5;
+   const double ty = kc + (r->y0 - yPx - 0.5) * os + 0.5;
+   if (interp == 0)
+   {
+      r->bx = (int)floor(tx + 0.5);
+      r->by = (int)floor(ty + 0.5);
+      r->nTaps = 1;
+      r->wx[0] = 1.0;
+      r->wy[0] = 1.0;
+      return;
+   }
+   const int bx = (int)floor(tx), by = (int)floor(ty);
+   const double fx = tx - bx, fy = ty - by;
+   if (interp == 2)
+   {
+      r->bx = bx - 1;
+      r->by = by - 1;
+      r->nTaps = 4;
+      catmull_rom(fx, r->wx);
+      catmull_rom(fy, r->wy);
+      return;
+   }
+   r->bx = bx;
+   r->by = by;
+   r->nTaps = 2;
+   r->wx[0] = 1.0 - fx;
+   r->wx[1] = fx;
+   r->wy[0] = 1.0 - fy;
+   r->wy[1] = fy;
+}
+
+/* Sim/SplatKernel.inl:51-167 (PixelTaps and Rows<NT>: the interior lanes run the same per-pixel sequence), rows
+ * [0, height), optional halo spans for this plane. */
+static void splat_rows(float* img, uint32_t width, uint32_t height, const float* B, int bw, int os, int camRad,
+                       const orc_splat_setup_t* st, double photons, const int16_t* spanLo, const int16_t* spanHi)
+{
+   const int off = os - 1, NT = st->nTaps;
+   const int dyLo = imax(-camRad, -st->y0), dyHi = imin(camRad, (int)height - 1 - st->y0);
+   const int dxLo = imax(-camRad, -st->x0), dxHi = imin(camRad, (int)width - 1 - st->x0);
+   const int base = st->bx + off;
+   for (int dy = dyLo; dy <= dyHi; ++dy)
+   {
+      const int r0 = st->by + dy * os + off;
+      const int jLo = imax(0, -r0), jHi = imin(NT, bw - r0);
+      float* rowOut = img + (size_t)(st->y0 + dy) * width;
+      int rLo = dxLo, rHi = dxHi;
+      if (spanLo)
+      {
+         rLo = imax(rLo, spanLo[dy + camRad]);
+         rHi = imin(rHi, spanHi[dy + camRad]);
+         if (rLo > rHi)
+            continue;
+      }
+      for (int dx = rLo; dx <= rHi; ++dx)
+      {
+         const int c0 = base + dx * os;
+         const int iLo = imax(0, -c0), iHi = imin(NT, bw - c0);
+         double sum = 0.0;
+         for (int j = jLo; j < jHi; ++j)
+         {
+            double row = 0.0;
+            const float* brow = B + (size_t)(r0 + j) * bw;
+            for (int i = iLo; i < iHi; ++i)
+               row += st->wx[i] * brow[c0 + i];
+            sum += st->wy[j] * row;
+         }
+         rowOut[st->x0 + dx] += (float)(photons * sum);
+      }
+   }
+}
+
+/* Sim/SMLMSimulation.cpp:103-157 RenderPhotonImage, serial path (bands give the same bits, :176-200). */
+void orc_render_frame(float* img, uint32_t width, uint32_t height, double bg, const float* bgMap, double bgScale,
+                      const orc_emitter* ems, int32_t nEms, const float* blockSums, int32_t nzK, int32_t bw,
+                      int32_t os, int32_t sizeOversampled, int32_t interp, const int16_t* spanLo,
+                      const int16_t* spanHi, double minPhotons)
+{
+   const size_t n = (size_t)width * height;
+   if (!bgMap && bgScale == 1.0)
+   {
+      const float v = (float)bg;
+      for (size_t i = 0; i < n; ++i)
+         img[i] = v;
+   }
+   else
+      for (size_t i = 0; i < n; ++i)
+      {
+         const double b = bgMap ? (double)bgMap[i] : bg;
+         img[i] = (float)(b * bgScale);
+      }
+   const int camRad = ((sizeOversampled - 1) / 2) / imax(1, os);
+   const size_t planeSize = (size_t)bw * bw;
+   const int nSpan = 2 * camRad + 1;
+   for (int32_t e = 0; e < nEms; ++e)
+   {
+      const orc_emitter* em = &ems[e];
+      if (minPhotons > 0.0 && em->photons < minPhotons)
+         continue;
+      /* PlanSplat (:1424-1464): no splat for photons <= 0 or a plane outside the stack */
+      if (em->photons <= 0.0 || em->zIndex < 0 || em->zIndex >= nzK)
+         continue;
+      orc_splat_setup_t st;
+      orc_splat_setup(sizeOversampled, os, em->xPx, em->yPx, interp, &st);
+      splat_rows(img, width, height, blockSums + (size_t)em->zIndex * planeSize, bw, imax(1, os), camRad, &st,
+                 em->photons, spanLo ? spanLo + (size_t)em->zIndex * nSpan : NULL,
+                 spanHi ? spanHi + (size_t)em->zIndex * nSpan : NULL);
+   }
+}
+
+/* ======================================================================= */
+/* Mean field: Sim/WidefieldRender.cpp:263-323, cli/widefield_check.cpp    */
+/* ======================================================================= */
+
+/* :263 ValidUpscale */
+static int valid_upscale(int oversampling, int requested)
+{
+   const int os = imax(1, oversampling);
+   for (int u = imax(1, imin(requested, os)); u > 1; --u)
+      if (os % u == 0)
+         return u;
+   return 1;
+}
+
+static void wf_geom(int32_t n, int32_t os_, int32_t upscale, int* r, int* s0)
+{
+   const int os = imax(1, os_);
+   *r = imax(1, os / valid_upscale(os, upscale));
+   const double kc = (n - 1) / 2.0;
+   *s0 = (int)floor(kc - *r / 2.0 + 1.0);
+}
+
+/* :297-304 KernelWidefieldPsf::Radius */
+int32_t orc_wf_radius(int32_t n, int32_t os, int32_t upscale)
+{
+   int r, s0;
+   wf_geom(n, os, upscale, &r, &s0);
+   const int dMin = (int)ceil((-s0 - r + 1) / (double)r);
+   const int dMax = (int)floor((n - 1 - s0) / (double)r);
+   return imax(1, imax(-dMin, dMax));
+}
+
+/* :306-323 KernelWidefieldPsf::Kernel */
+void orc_wf_kernel(const float* P, int32_t n, int32_t os, int32_t upscale, int32_t R, float* out)
+{
+   int r, s0;
+   wf_geom(n, os, upscale, &r, &s0);
+   const int D = 2 * R + 1;
+   for (int dy = -R; dy <= R; ++dy)
+   {
+      const int ya = s0 + dy * r, yb = imin(n, ya + r);
+      for (int dx = -R; dx <= R; ++dx)
+      {
+         const int xa = s0 + dx * r, xb = imin(n, xa + r);
+         double acc = 0.0;
+         for (int y = imax(0, ya); y < yb; ++y)
+            for (int x = imax(0, xa); x < xb; ++x)
+               acc += P[(size_t)y * n + x];
+         out[(size_t)(dy + R) * D + (dx + R)] = (float)acc;
+      }
+   }
+}
+
+/* cli/widefield_check.cpp:287-329 DirectImage (before the crop, max and bin), with PlanFocus's plane coordinate
+ * (Sim/WidefieldRender.cpp:1476-1490) and KernelWidefieldPsf::PlaneCoord (:285). */
+void orc_wf_direct(const float* density, int32_t nx, int32_t ny, int32_t nzD, double dzUm, int64_t k0,
+                   const float* kernels, int32_t nzP, int32_t R, double zStepNm, const double* focusUm, int32_t nF,
+                   double* out)
+{
+   const size_t n2 = (size_t)nx * ny;
+   const int D = 2 * R + 1, maxPlane = imax(0, nzP - 1);
+   memset(out, 0, sizeof(double) * n2 * (size_t)nF);
+   for (int32_t f = 0; f < nF; ++f)
+   {
+      double* img = out + (size_t)f * n2;
+      for (int32_t i = 0; i < nzD; ++i)
+      {
+         const double centre = ((double)(k0 + i) + 0.5) * dzUm;
+         double t = (nzP <= 1) ? 0.0 : (centre - focusUm[f]) * 1000.0 / zStepNm + (nzP - 1) / 2.0;
+         t = t < 0.0 ? 0.0 : (t > maxPlane ? maxPlane : t);
+         int p0 = (int)floor(t);
+         double fr = t - p0;
+         if (p0 >= maxPlane)
+         {
+            p0 = maxPlane;
+            fr = 0.0;
+         }
+         for (int s = 0; s < 2; ++s)
+         {
+            const double w = s ? fr : 1 - fr;
+            if (w == 0)
+               continue;
+            const float* k = kernels + (size_t)(p0 + s) * D * D;
+            const float* dens = density + (size_t)i * n2;
+            for (int iy = 0; iy < ny; ++iy)
+               for (int ix = 0; ix < nx; ++ix)
+               {
+                  const double a = dens[(size_t)iy * nx + ix];
+                  if (a == 0)
+                     continue;
+                  for (int dy = -R; dy <= R; ++dy)
+                  {
+                     const int Y = iy + dy;
+                     if (Y < 0 || Y >= ny)
+                        continue;
+                     for (int dx = -R; dx <= R; ++dx)
+                     {
+                        const int X = ix + dx;
+                        if (X < 0 || X >= nx)
+                           continue;
+                        img[(size_t)Y * nx + X] += w * a * k[(size_t)(dy + R) * D + dx + R];
+                     }
+                  }
+               }
+         }
+      }
+   }
+}

@@ -1,0 +1,338 @@
+// ref_shim.cpp -- extern "C" entry points over the reference's own CPU functions, compiled together with the
+// reference sources (read in place under the reference checkout) into oracle/_ref/libiscref.so.
+//
+// TEST INFRASTRUCTURE ONLY: used to write golden fixtures (tests/golden/make_golden.py), to cross-check the FP64
+// restatement in oracle.c, and as the timed CPU baseline. Nothing here is linked into the product library.
+//
+// Each entry point forwards to the named reference function with the arguments spelled out; the only code of its
+// own is argument marshalling.
+
+#include "PsfGeneratorBridge.h"
+#include "SMLMNoise.h"
+#include "SMLMSimulation.h"
+#include "SMLMZernike.h"
+#include "WidefieldRender.h"
+#include "ZernikePsf.h"
+#include "Parallel.h"
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+extern "C" {
+#include "oracle.h"
+}
+
+namespace {
+
+sim::PsfGeneratorRequest ToRequest(const orc_psf_request* r)
+{
+   sim::PsfGeneratorRequest q;
+   q.model = sim::PsfModelKind::GibsonLanniZernike;
+   q.wavelengthNm = r->wavelengthNm;
+   q.na = r->na;
+   q.immersionIndex = r->immersionIndex;
+   q.sampleIndex = r->sampleIndex;
+   q.workingDistanceUm = r->workingDistanceUm;
+   q.sampleDepthNm = r->sampleDepthNm;
+   q.pixelSizeNm = r->pixelSizeNm;
+   // Every digit (FormatZernikeCoefficients keeps 6 significant digits; std::stod round-trips %.17g exactly).
+   std::string z;
+   for (int j = 0; j < 28; ++j)
+   {
+      char buf[40];
+      std::snprintf(buf, sizeof buf, "%s%.17g", j ? "," : "", r->zernike[j]);
+      z += buf;
+   }
+   q.zernikeCoefficients = z;
+   q.maskType = r->maskType == 1 ? sim::PsfMaskType::DoubleHelix : sim::PsfMaskType::None;
+   q.maskModes = r->maskModes;
+   q.maskWaist = r->maskWaist;
+   q.oversampling = r->oversampling;
+   q.kernelHalfWidthPx = r->kernelHalfWidthPx;
+   q.nz = r->nz;
+   q.zStepNm = r->zStepNm;
+   q.pupilSamples = r->pupilSamples;
+   q.interpMode = static_cast<sim::PsfInterpMode>(r->interpMode);
+   return q;
+}
+
+sim::CameraNoiseParams ToCamera(const orc_camera* c)
+{
+   sim::CameraNoiseParams p;
+   p.quantumEfficiency = c->quantumEfficiency;
+   p.darkCurrentElectrons = c->darkCurrentElectrons;
+   p.gainPhotonsPerAdu = c->gainPhotonsPerAdu;
+   p.readNoiseElectrons = c->readNoiseElectrons;
+   p.emccd = c->emccd != 0;
+   p.emGain = c->emGain;
+   p.cicElectrons = c->cicElectrons;
+   p.bitDepth = c->bitDepth;
+   return p;
+}
+
+// A kernel cache holding caller-given sum-1 planes (PsfKernelCache::SetData; block sums are made on first use).
+sim::PsfKernelCache MakeCache(const float* planes, int nz, int size, int os, double zStepNm, int interp)
+{
+   sim::PsfKernelCache c;
+   c.oversampling = os;
+   c.halfWidthOversampled = (size - 1) / 2;
+   c.sizeOversampled = size;
+   c.nz = nz;
+   c.zStepNm = zStepNm;
+   c.interpMode = static_cast<sim::PsfInterpMode>(interp);
+   c.blockSumWidth = size + os - 1;
+   sim::PsfKernelPlanes d;
+   d.planes.resize(static_cast<size_t>(nz));
+   const size_t plane = static_cast<size_t>(size) * size;
+   for (int z = 0; z < nz; ++z)
+      d.planes[static_cast<size_t>(z)].assign(planes + z * plane, planes + (z + 1) * plane);
+   c.SetData(std::move(d));
+   c.valid = true;
+   return c;
+}
+
+} // namespace
+
+extern "C" {
+
+// ComputeZernikePsfPlanes (Sim/ZernikePsf.cpp:150): raw planes.
+int32_t ref_psf_planes(const orc_psf_request* r, float* out)
+{
+   std::vector<std::vector<float>> planes;
+   std::string err;
+   if (!sim::ComputeZernikePsfPlanes(ToRequest(r), planes, err))
+      return -1;
+   size_t o = 0;
+   for (const auto& p : planes)
+   {
+      std::memcpy(out + o, p.data(), p.size() * sizeof(float));
+      o += p.size();
+   }
+   return 0;
+}
+
+// BuildZernikePsfKernelCache (Sim/ZernikePsf.cpp:382): sum-1 planes and every plane's row-major block sums.
+int32_t ref_psf_kernel_cache(const orc_psf_request* r, float* planesOut, float* blockSumsOut)
+{
+   sim::PsfKernelCache cache;
+   std::string err;
+   if (!sim::BuildZernikePsfKernelCache(ToRequest(r), cache, err))
+      return -1;
+   const size_t plane = static_cast<size_t>(cache.sizeOversampled) * cache.sizeOversampled;
+   const size_t bplane = static_cast<size_t>(cache.blockSumWidth) * cache.blockSumWidth;
+   for (int z = 0; z < cache.nz; ++z)
+   {
+      if (planesOut)
+         std::memcpy(planesOut + z * plane, cache.Planes()[static_cast<size_t>(z)].data(), plane * sizeof(float));
+      if (blockSumsOut)
+      {
+         const std::vector<float> b = cache.BlockSums(z).RowMajor();
+         std::memcpy(blockSumsOut + z * bplane, b.data(), bplane * sizeof(float));
+      }
+   }
+   return 0;
+}
+
+// BuildBlockSums (Sim/PsfGeneratorBridge.cpp:1021).
+void ref_block_sums(const float* kernel, int32_t n, int32_t os, float* out)
+{
+   const std::vector<float> b = sim::BuildBlockSums(kernel, n, os);
+   std::memcpy(out, b.data(), b.size() * sizeof(float));
+}
+
+// ApplyNoiseChain (Sim/SMLMNoise.cpp:142); maps NULL = none.
+void ref_noise_chain(const float* photons, uint16_t* out, uint32_t w, uint32_t h, const orc_camera* cam,
+                     const float* offset, const float* gain, const float* read, uint32_t seed, uint32_t frame,
+                     uint32_t pixelBase, int32_t parallel)
+{
+   const size_t n = static_cast<size_t>(w) * h;
+   std::vector<float> img(photons, photons + n);
+   std::vector<uint16_t> adu;
+   sim::PixelOffsetMap om;
+   sim::PixelGainMap gm;
+   sim::PixelReadNoiseMap rm;
+   if (offset)
+   {
+      om.width = w;
+      om.height = h;
+      om.offset.assign(offset, offset + n);
+   }
+   if (gain)
+   {
+      gm.width = w;
+      gm.height = h;
+      gm.gainPhotonsPerAdu.assign(gain, gain + n);
+   }
+   if (read)
+   {
+      rm.width = w;
+      rm.height = h;
+      rm.readNoiseElectrons.assign(read, read + n);
+   }
+   sim::ApplyNoiseChain(img, adu, w, h, ToCamera(cam), om, gm, rm, seed, frame, parallel != 0, pixelBase);
+   std::memcpy(out, adu.data(), n * sizeof(uint16_t));
+}
+
+// One blink event of the shim's flat event array (sim::BlinkEvent's fields used on the hot path).
+struct ref_event
+{
+   double xUm, yUm, zNm, tStart, tEnd, brightness;
+};
+
+static std::vector<sim::BlinkEvent> ToEvents(const ref_event* ev, int32_t n)
+{
+   std::vector<sim::BlinkEvent> out(static_cast<size_t>(n));
+   for (int32_t i = 0; i < n; ++i)
+   {
+      out[static_cast<size_t>(i)].xUm = ev[i].xUm;
+      out[static_cast<size_t>(i)].yUm = ev[i].yUm;
+      out[static_cast<size_t>(i)].zNm = ev[i].zNm;
+      out[static_cast<size_t>(i)].tStart = ev[i].tStart;
+      out[static_cast<size_t>(i)].tEnd = ev[i].tEnd;
+      out[static_cast<size_t>(i)].brightness = ev[i].brightness;
+   }
+   return out;
+}
+
+// CollectFrameEmitters (Sim/SMLMSimulation.cpp:55): the frame's emitters (x, y, photons, zIndex); returns the count.
+int32_t ref_collect_emitters(const ref_event* ev, int32_t nEv, int64_t frame, uint32_t w, uint32_t h,
+                             double pixelSizeNm, double photonsPerBlink, int32_t nzK, double zStepNm,
+                             double globalZUm, orc_emitter* out, int32_t cap)
+{
+   sim::PsfKernelCache c;
+   c.nz = nzK;
+   c.zStepNm = zStepNm;
+   std::vector<sim::FrameEmitter> ems;
+   sim::CollectFrameEmitters(ToEvents(ev, nEv), frame, w, h, pixelSizeNm, photonsPerBlink, 0.0, 0.0, &c, globalZUm,
+                             nullptr, ems);
+   const int32_t n = static_cast<int32_t>(ems.size());
+   for (int32_t i = 0; i < n && i < cap; ++i)
+   {
+      out[i].xPx = ems[static_cast<size_t>(i)].xPx;
+      out[i].yPx = ems[static_cast<size_t>(i)].yPx;
+      out[i].photons = ems[static_cast<size_t>(i)].photons;
+      out[i].zIndex = ems[static_cast<size_t>(i)].zIndex;
+      out[i].kernel = 0;
+   }
+   return n;
+}
+
+// RenderPhotonImage (Sim/SMLMSimulation.cpp:103) with a kernel cache of the given sum-1 planes; halo cut by
+// WithHaloCut(threshold) when threshold > 0.
+int32_t ref_render_frame(float* img, uint32_t w, uint32_t h, const ref_event* ev, int32_t nEv, int64_t frame,
+                         double pixelSizeNm, double photonsPerBlink, double bg, const float* planes, int32_t nz,
+                         int32_t size, int32_t os, double zStepNm, int32_t interp, double globalZUm,
+                         double haloThreshold, double minPhotons, int32_t parallel)
+{
+   sim::PsfKernelCache c = MakeCache(planes, nz, size, os, zStepNm, interp);
+   if (haloThreshold > 0.0)
+      c = sim::WithHaloCut(c, haloThreshold);
+   std::vector<float> out;
+   sim::RenderExtras x;
+   x.parallel = parallel != 0;
+   x.minPhotons = minPhotons;
+   sim::RenderPhotonImage(out, w, h, ToEvents(ev, nEv), frame, pixelSizeNm, 1.0, photonsPerBlink, bg, 0.0, 0.0, &c,
+                          globalZUm, nullptr, nullptr, &x);
+   std::memcpy(img, out.data(), out.size() * sizeof(float));
+   return 0;
+}
+
+// WithHaloCut (Sim/PsfGeneratorBridge.cpp:1186): the spans (nz x (2 camRad + 1) each) and radius2 per plane.
+int32_t ref_halo_spans(const float* planes, int32_t nz, int32_t size, int32_t os, int32_t interp, double threshold,
+                       int16_t* lo, int16_t* hi, int32_t* radius2)
+{
+   sim::PsfKernelCache c = sim::WithHaloCut(MakeCache(planes, nz, size, os, 100.0, interp), threshold);
+   if (!c.halo)
+      return -1;
+   std::memcpy(lo, c.halo->lo.data(), c.halo->lo.size() * sizeof(int16_t));
+   std::memcpy(hi, c.halo->hi.data(), c.halo->hi.size() * sizeof(int16_t));
+   std::memcpy(radius2, c.halo->radius2.data(), c.halo->radius2.size() * sizeof(int32_t));
+   return c.halo->camRad;
+}
+
+// CollectGpuEmitters (Sim/SMLMSimulation.cpp:219): 64-byte records (sim::GpuSplatEmitter); returns the count.
+int32_t ref_collect_gpu_emitters(const ref_event* ev, int32_t nEv, int64_t frame, uint32_t w, uint32_t h,
+                                 double pixelSizeNm, double photonsPerBlink, const float* planes, int32_t nz,
+                                 int32_t size, int32_t os, double zStepNm, int32_t interp, double globalZUm,
+                                 void* out, int32_t cap)
+{
+   sim::PsfKernelCache c = MakeCache(planes, nz, size, os, zStepNm, interp);
+   std::vector<sim::GpuSplatEmitter> g;
+   sim::CollectGpuEmitters(ToEvents(ev, nEv), frame, w, h, pixelSizeNm, photonsPerBlink, 0.0, 0.0, c, globalZUm,
+                           nullptr, g);
+   const int32_t n = static_cast<int32_t>(g.size());
+   std::memcpy(out, g.data(), static_cast<size_t>(n < cap ? n : cap) * sizeof(sim::GpuSplatEmitter));
+   return n;
+}
+
+// KernelWidefieldPsf (Sim/WidefieldRender.cpp:272-323): radius and grid-pitch kernel of plane p.
+int32_t ref_wf_kernel(const float* planes, int32_t nz, int32_t size, int32_t os, double zStepNm, int32_t upscale,
+                      int32_t p, int32_t R, float* out)
+{
+   sim::PsfKernelCache c = MakeCache(planes, nz, size, os, zStepNm, 0);
+   sim::KernelWidefieldPsf psf(c, upscale);
+   if (R <= 0)
+      return psf.Radius(0, nz - 1);
+   std::vector<float> k;
+   psf.Kernel(p, R, k);
+   std::memcpy(out, k.data(), k.size() * sizeof(float));
+   return R;
+}
+
+// Frames rendered and noised as FluorescenceMovie::Render batches them (Sim/ScopeMovie.cpp:7649-7653): the frames
+// of a batch in parallel, each RenderPhotonImage (serial) + ApplyNoiseChain. events: one flat list; frame f renders
+// the events overlapping [f, f+1). The timed CPU baseline.
+int32_t ref_render_noise_frames(const ref_event* ev, int32_t nEv, const int32_t* evStart, int32_t nFrames,
+                                uint32_t w, uint32_t h, double pixelSizeNm, double bg, const float* planes,
+                                int32_t nz, int32_t size, int32_t os, double zStepNm, int32_t interp,
+                                const orc_camera* cam, const float* offset, uint32_t seed, uint16_t* out)
+{
+   sim::PsfKernelCache c = MakeCache(planes, nz, size, os, zStepNm, interp);
+   c.PrepareSplat();
+   const sim::CameraNoiseParams cp = ToCamera(cam);
+   sim::PixelOffsetMap om;
+   if (offset)
+   {
+      om.width = w;
+      om.height = h;
+      om.offset.assign(offset, offset + static_cast<size_t>(w) * h);
+   }
+   const sim::PixelGainMap gm;
+   const sim::PixelReadNoiseMap rm;
+   const size_t npx = static_cast<size_t>(w) * h;
+   sim::ParallelFor(static_cast<unsigned>(nFrames), [&](unsigned f) {
+      const std::vector<sim::BlinkEvent> events = ToEvents(ev + evStart[f], evStart[f + 1] - evStart[f]);
+      std::vector<float> img;
+      sim::RenderPhotonImage(img, w, h, events, static_cast<long>(f), pixelSizeNm, 1.0, 1.0, bg, 0.0, 0.0, &c, 0.0);
+      std::vector<uint16_t> adu;
+      sim::ApplyNoiseChain(img, adu, w, h, cp, om, gm, rm, seed, f, false, 0);
+      std::memcpy(out + f * npx, adu.data(), npx * sizeof(uint16_t));
+   });
+   return 0;
+}
+
+// ComputeZernikePsfPlanes over many requests, requests in sequence and planes in parallel as the reference runs
+// them (Sim/ZernikePsf.cpp:236). The timed CPU baseline of the PSF library.
+int32_t ref_psf_library(const orc_psf_request* reqs, int32_t nReq, float* out)
+{
+   size_t o = 0;
+   for (int32_t i = 0; i < nReq; ++i)
+   {
+      std::vector<std::vector<float>> planes;
+      std::string err;
+      if (!sim::ComputeZernikePsfPlanes(ToRequest(reqs + i), planes, err))
+         return -1;
+      for (const auto& p : planes)
+      {
+         if (out)
+            std::memcpy(out + o, p.data(), p.size() * sizeof(float));
+         o += p.size();
+      }
+   }
+   return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,177 @@
+"""Writes the golden fixtures of tests/golden from the reference itself.
+
+Run in a container that has the reference checkout (/root/reference) after `make -C oracle ref` built
+oracle/_ref/libiscref.so from the reference's own sources:
+
+    python tests/golden/make_golden.py
+
+Outputs (inputs and expected outputs only):
+  rng_vectors.json   the `rng` / `stream` lines of spec/golden/ref.txt, parsed (pcg4d, HashUnit, HashStream)
+  zernike_ref.bin    tests/psf/zernike_ref.bin, the reference's PSF fixture (4 cases x 3 planes x 65^2, float32)
+  psf_ref.npz        ComputeZernikePsfPlanes at the C1 and C3 geometries (selected planes) and the sum-1 planes and
+                     block sums of a small splat kernel (BuildZernikePsfKernelCache)
+  noise_ref.npz      ApplyNoiseChain ADU for fixed photon images: sCMOS and EMCCD, with and without maps, pixelBase
+  splat_ref.npz      CollectFrameEmitters, CollectGpuEmitters and RenderPhotonImage frames (Nearest, Linear, Cubic,
+                     halo cut, edge emitters) on seeded events
+  wf_ref.npz         KernelWidefieldPsf radius and grid-pitch planes
+"""
+from __future__ import annotations
+
+import json
+import os
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+from insiliscope_amd import configs  # noqa: E402
+
+REF = "/root/reference"
+
+
+def rng_vectors():
+    rng, stream = [], []
+    with open(os.path.join(REF, "spec/golden/ref.txt")) as f:
+        for line in f:
+            t = line.split()
+            if not t:
+                continue
+            if t[0] == "rng":
+                rng.append({"seed": int(t[1]), "cx": int(t[2]), "cy": int(t[3]), "k": int(t[4]),
+                            "abcd": [int(x) for x in t[5:9]], "unit": t[9]})
+            elif t[0] == "stream":
+                stream.append({"seed": int(t[1]), "cx": int(t[2]), "cy": int(t[3]), "base": int(t[4]),
+                               "draws": t[5:]})
+    with open(os.path.join(HERE, "rng_vectors.json"), "w") as f:
+        json.dump({"source": "spec/golden/ref.txt", "rng": rng, "stream": stream}, f, indent=1)
+
+
+def req(**kw):
+    return O.psf_request(**{k: v for k, v in kw.items()})
+
+
+def psf(R):
+    out = {}
+    # C1 geometry (33^2 x 41 at 50 nm, M 128), planes 0, 10, 20, 30, 40
+    c1 = configs.c1_psf(seed=1)
+    r1 = req(**c1)
+    size, nz, _ = O.Oracle().psf_geometry(r1)
+    p = R.psf_planes(r1, size, nz)
+    out["c1_zernike"] = c1["zernike"]
+    out["c1_planes_idx"] = np.array([0, 10, 20, 30, 40])
+    out["c1_planes"] = p[out["c1_planes_idx"]]
+    # C3 geometry (65^2 x 41, M 128): 2 wavelengths x 2 Zernike sets, planes 0, 20, 40
+    reqs = configs.c3_requests(seed=1, n_lambda=8, n_sets=64)
+    pick = [0, 5, 7 * 64 + 3, 7 * 64 + 63]
+    out["c3_pick"] = np.array(pick)
+    out["c3_zernike"] = np.stack([reqs[i]["zernike"] for i in pick])
+    out["c3_lambda"] = np.array([reqs[i]["wavelengthNm"] for i in pick])
+    planes = []
+    for i in pick:
+        rr = req(**reqs[i])
+        size, nz, _ = O.Oracle().psf_geometry(rr)
+        planes.append(R.psf_planes(rr, size, nz)[[0, 20, 40]])
+    out["c3_planes"] = np.stack(planes)
+    # a small splat kernel: hw 8, os 4, 9 planes at 50 nm, M 64 (the cache: sum-1 planes + block sums)
+    small = configs.c4_psf(seed=1)
+    small.update(nz=9, zStepNm=50.0, pupilSamples=64)
+    rs = req(**small)
+    size, nz, _ = O.Oracle().psf_geometry(rs)
+    planes, bs = R.psf_kernel_cache(rs, size, nz, 4)
+    out["small_zernike"] = small["zernike"]
+    out["small_planes"] = planes
+    out["small_block_sums"] = bs
+    np.savez_compressed(os.path.join(HERE, "psf_ref.npz"), **out)
+
+
+def noise(R):
+    out = {}
+    h, w = 48, 64
+    yy, xx = np.mgrid[0:h, 0:w]
+    # photons 0 .. ~300 across the frame (Knuth and normal-approximation branches, zeros and negatives)
+    ph = (xx * 4.7 + yy * 0.31).astype(np.float32)
+    ph[0, :8] = [-3.0, 0.0, 1e-9, 0.5, 59.9, 60.0, 60.1, 1000.0]
+    out["photons"] = ph
+    offset = np.full((h, w), 100.0, np.float32)
+    rng = np.random.default_rng(7)
+    gain = (2.0 * (1 + 0.05 * rng.standard_normal((h, w)))).astype(np.float32)
+    read = (1.5 * (1 + 0.2 * rng.standard_normal((h, w)))).clip(0).astype(np.float32)
+    out["offset"], out["gain"], out["read"] = offset, gain, read
+    sc = O.camera(quantumEfficiency=1.0, darkCurrentElectrons=0.0, gainPhotonsPerAdu=2.0, readNoiseElectrons=1.5,
+                  emccd=0)
+    em = O.camera(quantumEfficiency=1.0, darkCurrentElectrons=0.0, gainPhotonsPerAdu=1.0 / 300.0,
+                  readNoiseElectrons=50.0, emccd=1, emGain=300.0, cicElectrons=0.002, bitDepth=16)
+    em12 = O.camera(quantumEfficiency=0.9, darkCurrentElectrons=0.3, gainPhotonsPerAdu=0.05, readNoiseElectrons=60.0,
+                    emccd=1, emGain=100.0, cicElectrons=0.01, bitDepth=12)
+    out["scmos_f3"] = R.noise(ph, sc, 1, 3, offset=offset)
+    out["scmos_maps_f4"] = R.noise(ph, sc, 1, 4, offset=offset, gain=gain, read=read, pixel_base=640)
+    out["scmos_nooffset_f5"] = R.noise(ph, sc, 9, 5)
+    out["emccd_f3"] = R.noise(ph, em, 1, 3, offset=offset)
+    out["emccd_maps_f7"] = R.noise(ph, em, 2, 7, offset=offset, gain=gain * 0.01, read=read * 30, pixel_base=64)
+    out["emccd12_f1"] = R.noise(ph * 3, em12, 5, 1, offset=offset)
+    out["scmos_params"] = np.array([1.0, 0.0, 2.0, 1.5])
+    np.savez_compressed(os.path.join(HERE, "noise_ref.npz"), **out)
+
+
+def splat(R, P):
+    out = {}
+    planes, _bs = P["small_planes"], None
+    w, h = 96, 80
+    ev, start = configs.blink_events(3, 3, w, h, 100.0, 40.0, 220.0, 2000.0)
+    # edge cases: emitters on and beyond the borders, exact half-pixel ties, a dim one, one at z far outside
+    extra = np.zeros(8, ev.dtype)
+    extra["xUm"] = [0.0, 0.005, 9.6 - 0.005, 9.6 + 0.3, 4.05, 4.0, -0.2, 2.5]
+    extra["yUm"] = [0.0, 7.995, 0.3, 4.0, 4.05, 4.0, 3.0, 8.0 + 0.4]
+    extra["zNm"] = [0, 40, -60, 90, 10, 900, -1000, 0]
+    extra["tStart"] = 1.0
+    extra["tEnd"] = [2.0, 1.5, 2.0, 2.0, 3.0, 2.0, 2.0, 2.0]
+    extra["brightness"] = [1500, 800, 1200, 900, 0.5, 2000, 1000, 700]
+    ev = np.concatenate([ev, extra])
+    out["events"] = ev.view(np.float64).reshape(-1, 6)
+    out["w_h"] = np.array([w, h])
+    for interp, name in ((0, "nearest"), (1, "linear"), (2, "cubic")):
+        for f in range(3):
+            out[f"frame_{name}_{f}"] = R.render_frame(w, h, ev, f, 100.0, 1.0, 20.0, planes, 4, 50.0, interp)
+    out["frame_cubic_halo_1"] = R.render_frame(w, h, ev, 1, 100.0, 1.0, 20.0, planes, 4, 50.0, 2, halo=3e-4)
+    lo, hi, r2 = R.halo_spans(planes, 4, 2, 3e-4)
+    out["halo_lo"], out["halo_hi"], out["halo_r2"] = lo, hi, r2
+    out["frame_cubic_min_1"] = R.render_frame(w, h, ev, 1, 100.0, 1.0, 20.0, planes, 4, 50.0, 2, min_photons=900.0)
+    out["frame_cubic_parallel_1"] = R.render_frame(w, h, ev, 1, 100.0, 1.0, 20.0, planes, 4, 50.0, 2,
+                                                   parallel=True)
+    for f in range(3):
+        em = R.collect_emitters(ev, f, w, h, 100.0, 1.0, planes.shape[0], 50.0)
+        out[f"emitters_{f}"] = em.view(np.uint8)
+        g = R.collect_gpu_emitters(ev, f, w, h, 100.0, 1.0, planes, 4, 50.0, 2)
+        out[f"gpu_emitters_{f}"] = g.view(np.uint8)
+    np.savez_compressed(os.path.join(HERE, "splat_ref.npz"), **out)
+
+
+def wf(R, P):
+    out = {}
+    planes = P["small_planes"]
+    for up in (1, 2, 4):
+        r = R.wf_radius(planes, 4, up)
+        out[f"radius_u{up}"] = np.array(r)
+        out[f"kernel_u{up}"] = np.stack([R.wf_kernel(planes, 4, up, p, r) for p in (0, 4, 8)])
+    np.savez_compressed(os.path.join(HERE, "wf_ref.npz"), **out)
+
+
+def main():
+    R = O.Reference()
+    rng_vectors()
+    shutil.copyfile(os.path.join(REF, "tests/psf/zernike_ref.bin"), os.path.join(HERE, "zernike_ref.bin"))
+    psf(R)
+    P = np.load(os.path.join(HERE, "psf_ref.npz"))
+    noise(R)
+    splat(R, P)
+    wf(R, P)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

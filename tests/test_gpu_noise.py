@@ -1,0 +1,63 @@
+"""Camera noise on the GPU (iscg_noise) against the compiled reference's ADU fixtures and the oracle: bit-exact."""
+import numpy as np
+import pytest
+
+import oracle as O
+from insiliscope_amd import host
+from test_oracle import NOISE_CASES, noise_case_args
+
+pytestmark = pytest.mark.gpu
+
+
+def cam_of(c: O.Camera) -> host.CameraNoiseParams:
+    return host.CameraNoiseParams(c.quantumEfficiency, c.darkCurrentElectrons, c.gainPhotonsPerAdu,
+                                  c.readNoiseElectrons, bool(c.emccd), c.emGain, c.cicElectrons, c.bitDepth)
+
+
+@pytest.mark.parametrize("case", NOISE_CASES, ids=[c[0] for c in NOISE_CASES])
+def test_noise_bit_exact_vs_reference_fixture(ctx, golden, case):
+    G = golden("noise_ref.npz")
+    ph, cam, seed, frame, kw, base = noise_case_args(G, case)
+    print("case", case[0], flush=True)
+    h, w = ph.shape
+    got = host.ApplyNoiseChain(ph, w, h, cam_of(cam), offsetMap=kw.get("offset"), gainMap=kw.get("gain"),
+                               readNoiseMap=kw.get("read"), noiseSeed=seed, frame=frame, pixelBase=base, ctx=ctx)
+    assert np.array_equal(got, G[case[0]])
+
+
+def test_noise_batch_device_tensors_vs_oracle(ctx, orc):
+    import torch
+    rng = np.random.default_rng(11)
+    n, h, w = 6, 128, 160
+    ph = (rng.random((n, h, w)) * 120.0).astype(np.float32)
+    ph[:, :4, :4] = 0.0
+    frames = np.array([0, 1, 2, 17, 1000, 4294967295], np.uint32)
+    off = np.full((h, w), 100.0, np.float32)
+    for emccd in (False, True):
+        cam = host.CameraNoiseParams(1.0, 0.0, 1.0 / 300.0 if emccd else 2.0, 50.0 if emccd else 1.5, emccd, 300.0,
+                                     0.002, 16)
+        print("emccd", emccd, flush=True)
+        d_ph = torch.from_numpy(ph).cuda()
+        d_fr = torch.from_numpy(frames.view(np.int32)).cuda()
+        d_off = torch.from_numpy(off).cuda()
+        out = host.ApplyNoiseChainFrames(d_ph, cam, 42, d_fr, offsetMap=d_off, ctx=ctx)
+        assert out.is_cuda
+        ctx.synchronize()
+        got = out.cpu().numpy().view(np.uint16)
+        c = O.camera(quantumEfficiency=1.0, darkCurrentElectrons=0.0, gainPhotonsPerAdu=cam.gainPhotonsPerAdu,
+                     readNoiseElectrons=cam.readNoiseElectrons, emccd=int(emccd), emGain=300.0, cicElectrons=0.002,
+                     bitDepth=16)
+        for k in range(n):
+            want = orc.noise(ph[k], c, 42, int(frames[k]), offset=off)
+            assert np.array_equal(got[k], want), f"frame {k} emccd {emccd}: {(got[k] != want).sum()} mismatches"
+
+
+def test_noise_failure_injection_leaves_caller_in_charge(ctx):
+    """An injected device-side failure returns an error (the caller's CPU fallback path), then the next call works."""
+    ph = np.full((8, 8), 10.0, np.float32)
+    cam = host.CameraNoiseParams(quantumEfficiency=1.0)
+    ctx.inject_failure(1, "test: device lost")
+    with pytest.raises(host.IscgError, match="device lost"):
+        host.ApplyNoiseChain(ph, 8, 8, cam, ctx=ctx)
+    out = host.ApplyNoiseChain(ph, 8, 8, cam, ctx=ctx)
+    assert out.shape == (8, 8)

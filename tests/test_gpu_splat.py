@@ -1,0 +1,151 @@
+"""The splat (+ fused noise) on the GPU through the C ABI, against the compiled reference's RenderPhotonImage frames
+and the oracle: bit-exact photon images and ADU."""
+import numpy as np
+import pytest
+
+import oracle as O
+from insiliscope_amd import configs, host
+
+pytestmark = pytest.mark.gpu
+
+
+def fixture(golden):
+    G, P = golden("splat_ref.npz"), golden("psf_ref.npz")
+    ev = G["events"].view(host.BLINK_EVENT_DTYPE).reshape(-1)
+    w, h = (int(x) for x in G["w_h"])
+    return G, P, ev, w, h
+
+
+def batch(G, frames):
+    ems = [G[f"emitters_{f}"].view(host.FRAME_EMITTER_DTYPE) for f in frames]
+    start = np.zeros(len(ems) + 1, np.uint32)
+    start[1:] = np.cumsum([len(e) for e in ems])
+    return np.concatenate(ems), start
+
+
+@pytest.mark.parametrize("interp,name", [(0, "nearest"), (1, "linear"), (2, "cubic")])
+def test_splat_photons_bit_exact_vs_reference(ctx, golden, interp, name):
+    G, P, ev, w, h = fixture(golden)
+    print("interp", name, flush=True)
+    sp = host.SplatGpuHost(ctx)
+    cache = host.PsfKernelCache(P["small_planes"], oversampling=4, zStepNm=50.0, interpMode=interp)
+    assert sp.SetKernels([cache]), sp.err
+    assert sp.SetStatic(w, h, flatBackground=20.0), sp.err
+    em, start = batch(G, [0, 1, 2])
+    ph = np.zeros((3, h, w), np.float32)
+    assert sp.RenderBatch(em, start, None, None, 0, outPhotons=ph), sp.err
+    for f in range(3):
+        want = G[f"frame_{name}_{f}"]
+        assert np.array_equal(ph[f], want), f"frame {f}: {(ph[f] != want).sum()} pixels differ"
+
+
+def test_splat_halo_and_min_photons(ctx, golden):
+    G, P, ev, w, h = fixture(golden)
+    sp = host.SplatGpuHost(ctx)
+    halo = host.PsfHaloSpans(G["halo_lo"], G["halo_hi"], G["halo_r2"], camRad=8, threshold=3e-4)
+    cache = host.PsfKernelCache(P["small_planes"], oversampling=4, zStepNm=50.0, interpMode=2, halo=halo)
+    assert sp.SetKernels([cache]) and sp.SetStatic(w, h, flatBackground=20.0), sp.err
+    em, start = batch(G, [1])
+    ph = np.zeros((1, h, w), np.float32)
+    assert sp.RenderBatch(em, start, None, None, 0, outPhotons=ph), sp.err
+    assert np.array_equal(ph[0], G["frame_cubic_halo_1"])
+    cache2 = host.PsfKernelCache(P["small_planes"], oversampling=4, zStepNm=50.0, interpMode=2)
+    assert sp.SetKernels([cache2]), sp.err
+    assert sp.RenderBatch(em, start, None, None, 0, minPhotons=900.0, outPhotons=ph), sp.err
+    assert np.array_equal(ph[0], G["frame_cubic_min_1"])
+
+
+def test_splat_gpu_emitter_records(ctx, golden):
+    """The reference's 64-byte wire format (float weights): within float rounding of the CPU frames."""
+    G, P, ev, w, h = fixture(golden)
+    sp = host.SplatGpuHost(ctx)
+    cache = host.PsfKernelCache(P["small_planes"], oversampling=4, zStepNm=50.0, interpMode=2)
+    assert sp.SetKernels([cache]) and sp.SetStatic(w, h, flatBackground=20.0), sp.err
+    recs = [G[f"gpu_emitters_{f}"].view(host.SPLAT_EMITTER_DTYPE) for f in range(3)]
+    start = np.zeros(4, np.uint32)
+    start[1:] = np.cumsum([len(r) for r in recs])
+    ph = np.zeros((3, h, w), np.float32)
+    assert sp.RenderBatch(np.concatenate(recs), start, None, None, 0, outPhotons=ph, records=True), sp.err
+    for f in range(3):
+        want = G[f"frame_cubic_{f}"].astype(np.float64)
+        err = np.sqrt(((ph[f] - want) ** 2).sum() / (want ** 2).sum())
+        assert err < 1e-6
+
+
+def test_splat_fused_noise_bit_exact(ctx, orc, golden):
+    """Splat + EMCCD noise in one pass = the reference frames through the oracle's ApplyNoiseChain; also through
+    RenderFrames (SplatGpuHost's per-frame lists)."""
+    G, P, ev, w, h = fixture(golden)
+    sp = host.SplatGpuHost(ctx)
+    cache = host.PsfKernelCache(P["small_planes"], oversampling=4, zStepNm=50.0, interpMode=2)
+    off = np.full((h, w), 100.0, np.float32)
+    assert sp.SetKernels([cache]) and sp.SetStatic(w, h, offset=off, flatBackground=20.0), sp.err
+    cam = configs.emccd_c4()
+    ocam = O.camera(quantumEfficiency=1.0, gainPhotonsPerAdu=1.0 / 300.0, readNoiseElectrons=50.0, emccd=1,
+                    emGain=300.0, cicElectrons=0.002)
+    em, start = batch(G, [0, 1, 2])
+    frames = np.array([10, 11, 12], np.uint32)
+    adu = np.zeros((3, h, w), np.uint16)
+    assert sp.RenderBatch(em, start, frames, cam, 1, outAdu=adu), sp.err
+    for f in range(3):
+        want = orc.noise(G[f"frame_cubic_{f}"], ocam, 1, int(frames[f]), offset=off)
+        assert np.array_equal(adu[f], want)
+    recs = [G[f"gpu_emitters_{f}"].view(host.SPLAT_EMITTER_DTYPE) for f in range(3)]
+    outs = [np.zeros((h, w), np.uint16) for _ in range(3)]
+    assert sp.RenderFrames(recs, frames, None, cam, 1, outs), sp.err
+    assert all(o.any() for o in outs)
+
+
+def test_splat_uneven_batch_c4_like_vs_oracle(ctx, orc):
+    """512x512 frames with Poisson(800) blinks, one empty frame, C4's kernel geometry (hw 8, os 4) on a short stack
+    at 15 nm, Catmull-Rom, EMCCD: every frame bit-exact against oracle render + noise."""
+    req = configs.c4_psf(seed=1)
+    req.update(nz=11, pupilSamples=64)
+    planes = orc.psf_planes(O.psf_request(**req), normalize=True)
+    cache = host.PsfKernelCache(planes, oversampling=4, zStepNm=15.0, interpMode=2)
+    ev, st = configs.c4_events(seed=1, n_frames=3)
+    ems = [host.CollectFrameEmitters(ev[st[f]:st[f + 1]], f, 100.0, 1.0, psfCache=cache) for f in range(3)]
+    ems.insert(1, np.zeros(0, host.FRAME_EMITTER_DTYPE))
+    start = np.zeros(len(ems) + 1, np.uint32)
+    start[1:] = np.cumsum([len(e) for e in ems])
+    w = h = 512
+    off = np.full((h, w), 100.0, np.float32)
+    sp = host.SplatGpuHost(ctx)
+    assert sp.SetKernels([cache]) and sp.SetStatic(w, h, offset=off, flatBackground=50.0), sp.err
+    frames = np.arange(len(ems), dtype=np.uint32)
+    adu = np.zeros((len(ems), h, w), np.uint16)
+    ph = np.zeros((len(ems), h, w), np.float32)
+    cam = configs.emccd_c4()
+    assert sp.RenderBatch(np.concatenate(ems), start, frames, cam, 1, outAdu=adu, outPhotons=ph), sp.err
+    bs = np.stack([orc.block_sums(p, 4) for p in planes])
+    ocam = O.camera(quantumEfficiency=1.0, gainPhotonsPerAdu=1.0 / 300.0, readNoiseElectrons=50.0, emccd=1,
+                    emGain=300.0, cicElectrons=0.002)
+    for f, em in enumerate(ems):
+        want = orc.render_frame(w, h, 50.0, em, bs, planes.shape[1], 4, 2)
+        assert np.array_equal(ph[f], want), f"frame {f}: {(ph[f] != want).sum()} photon pixels differ"
+        want_adu = orc.noise(want, ocam, 1, f, offset=off)
+        assert np.array_equal(adu[f], want_adu), f"frame {f}: {(adu[f] != want_adu).sum()} ADU differ"
+
+
+def test_splat_extras_and_bg_scale(ctx, orc, golden):
+    """Blinks added onto a caller's extra image replace the background; a background map with a per-frame scale
+    follows RenderPhotonImage's float(map * scale)."""
+    G, P, ev, w, h = fixture(golden)
+    cache = host.PsfKernelCache(P["small_planes"], oversampling=4, zStepNm=50.0, interpMode=2)
+    rng = np.random.default_rng(5)
+    bg = (rng.random((h, w)) * 30).astype(np.float32)
+    sp = host.SplatGpuHost(ctx)
+    assert sp.SetKernels([cache]) and sp.SetStatic(w, h, background=bg, flatBackground=0.0), sp.err
+    em, start = batch(G, [0, 1])
+    extras = np.stack([np.zeros((h, w), np.float32), (rng.random((h, w)) * 7).astype(np.float32)])
+    ph = np.zeros((2, h, w), np.float32)
+    assert sp.RenderBatch(em, start, None, None, 0, bgScales=np.array([0.7, 1.0], np.float32), outPhotons=ph), sp.err
+    bs = np.stack([orc.block_sums(p, 4) for p in P["small_planes"]])
+    e0 = G["emitters_0"].view(O.EMITTER_DTYPE)
+    want0 = orc.render_frame(w, h, 0.0, e0, bs, 65, 4, 2, bg_map=bg, bg_scale=float(np.float32(0.7)))
+    assert np.array_equal(ph[0], want0)
+    assert sp.RenderBatch(em, start, None, None, 0, extras=extras, outPhotons=ph), sp.err
+    # onto extras[1]: the oracle frame with a background map equal to extras[1] and scale 1 starts from the same
+    e1 = G["emitters_1"].view(O.EMITTER_DTYPE)
+    want1 = orc.render_frame(w, h, 0.0, e1, bs, 65, 4, 2, bg_map=extras[1], bg_scale=1.0)
+    assert np.array_equal(ph[1], want1)

@@ -1,0 +1,124 @@
+"""CPU-side checks of the product: the C ABI library exports what include/iscg.h declares (no GPU calls), and the
+host-side mirror of the reference's per-emitter arithmetic equals the compiled reference's output."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from insiliscope_amd import _lib, configs, host
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "iscg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(iscg_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(_lib.LIB_PATH), "libiscg.so not built"
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\sT\s(iscg_\w+)", out))
+    declared = declared_symbols()
+    assert declared, "no declarations parsed"
+    missing = [s for s in declared if s not in exported]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_ctypes_signatures_cover_header():
+    assert set(declared_symbols()) == set(_lib.SIGNATURES), "SIGNATURES out of step with include/iscg.h"
+
+
+def test_library_loads_without_gpu():
+    L = _lib.lib()
+    assert L.iscg_abi_version() == 1
+
+
+def test_struct_layouts():
+    import ctypes as C
+    assert C.sizeof(_lib.Camera) == 56
+    assert C.sizeof(_lib.PsfRequest) == C.sizeof(O.PsfRequest)
+    assert host.FRAME_EMITTER_DTYPE.itemsize == 32
+    assert host.SPLAT_EMITTER_DTYPE.itemsize == 64
+
+
+def splat_fixture(golden):
+    G, P = golden("splat_ref.npz"), golden("psf_ref.npz")
+    ev = G["events"].view(host.BLINK_EVENT_DTYPE).reshape(-1)
+    w, h = (int(x) for x in G["w_h"])
+    return G, P, ev, w, h
+
+
+def test_collect_frame_emitters_matches_reference(golden):
+    G, P, ev, w, h = splat_fixture(golden)
+    cache = host.PsfKernelCache(P["small_planes"], oversampling=4, zStepNm=50.0, interpMode=2)
+    for f in range(3):
+        got = host.CollectFrameEmitters(ev, f, 100.0, 1.0, psfCache=cache)
+        want = G[f"emitters_{f}"].view(host.FRAME_EMITTER_DTYPE)
+        assert got.tobytes() == want.tobytes()
+
+
+def test_collect_gpu_emitters_matches_reference(golden):
+    G, P, ev, w, h = splat_fixture(golden)
+    cache = host.PsfKernelCache(P["small_planes"], oversampling=4, zStepNm=50.0, interpMode=2)
+    for f in range(3):
+        em = host.CollectFrameEmitters(ev, f, 100.0, 1.0, psfCache=cache)
+        got = host.CollectGpuEmitters(em, cache)
+        want = G[f"gpu_emitters_{f}"].view(host.SPLAT_EMITTER_DTYPE)
+        assert got.tobytes() == want.tobytes()
+
+
+def test_nearest_z_matches_oracle(orc):
+    cache = host.PsfKernelCache(np.zeros((81, 5, 5), np.float32), oversampling=1, zStepNm=15.0)
+    z = np.array([-2.0, -0.6, -0.0075, 0.0, 0.0074999, 0.0075, 0.6, 0.6075, 3.0, -0.5925])
+    idx, cl = cache.NearestZIndex(z)
+    for zi, i, c in zip(z, idx, cl):
+        assert (i, c) == orc.nearest_z(81, 15.0, zi)
+
+
+def test_synthetic_workloads_are_deterministic():
+    ev1, s1 = configs.c4_events(seed=1, n_frames=5)
+    ev2, s2 = configs.c4_events(seed=1, n_frames=5)
+    assert ev1.tobytes() == ev2.tobytes() and np.array_equal(s1, s2)
+    counts = np.diff(s1)
+    assert counts.min() > 650 and counts.max() < 950  # Poisson(800)
+    assert np.all((ev1["zNm"] >= -600) & (ev1["zNm"] <= 600))
+    z = configs.zernike_noll(1, range(5, 16), amplitude=0.05)
+    assert np.count_nonzero(z) == 11 and np.max(np.abs(z)) <= 0.05
+    assert z[0] == 0 and z[4] == 0  # piston and defocus (Noll 1, 4) untouched
+
+
+def test_noll_to_osa_parity_rule():
+    """Noll's rule: even j -> cos (m >= 0 in OSA), odd j -> sin (m < 0); same radial order n."""
+    def osa_nm(j):
+        for n in range(10):
+            for l in range(n + 1):
+                if n * (n + 1) // 2 + l == j:
+                    return n, -n + 2 * l
+    def noll_n(j):
+        n = 0
+        while (n + 1) * (n + 2) // 2 < j:
+            n += 1
+        return n
+    for noll, osa in configs.NOLL_TO_OSA.items():
+        n, m = osa_nm(osa)
+        assert n == noll_n(noll)
+        if m != 0:
+            assert (noll % 2 == 0) == (m > 0)
+
+
+def test_host_calls_fail_loudly_without_gpu():
+    """No CPU fallback: creating a context with no device raises."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(host.IscgError):
+        host.Context(0)
