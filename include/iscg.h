@@ -137,6 +137,40 @@ int32_t iscg_splat_render_records(iscg_context* h, const iscg_splat_emitter* ems
                                   const float* bgScales, const float* extras, const iscg_camera* cam, uint32_t seed,
                                   uint16_t* outAdu, float* outPhotons);
 
+/* ---- mean field: all foci of a z stack by one 3D FFT convolution ----------------------------------------------- */
+
+/* The geometry of a dye grid and its focus series (WidefieldGridSpec / WidefieldDyeGrid, Sim/WidefieldRender.h:87-115;
+ * WidefieldScene::FocusSeries, Sim/WidefieldRender.h:588). Dye plane i is centred at (k0 + i + 0.5) dzUm; image f is
+ * at focus focus0Um + f dzUm. The images are the grid cells [fovX0, fovX0 + cw) x [fovY0, fovY0 + ch) (0: to the grid
+ * edge). NX, NY, NZ: FFT sizes (powers of two; 0: the smallest with NX >= nx + R, NY >= ny + R, NZ >= nzD + nF - 1,
+ * the linear-convolution bound). upscale: grid cells per camera pixel (1 only). */
+typedef struct
+{
+   double dzUm;
+   double focus0Um;
+   int64_t k0;
+   uint32_t nx, ny, nzD, nF;
+   uint32_t fovX0, fovY0, cw, ch;
+   uint32_t NX, NY, NZ;
+   int32_t upscale;
+} iscg_wf_desc;
+
+/* Plans the convolution for one PSF (replaces the kernel side of WidefieldScene: KernelWidefieldPsf,
+ * Sim/WidefieldRender.cpp:263-323; MakeKernel :1408; PlanFocus's blend and end-plane clamp :1460-1524).
+ * psfPlanes: nzP x size x size sum-1 oversampled planes (a PsfKernelCache's), zStepNm apart, plane (nzP-1)/2 in
+ * focus. kernelRadius: grid cells (0: KernelWidefieldPsf::Radius). Keeps the kernel spectrum resident. */
+int32_t iscg_wf_conv3d_plan(iscg_context* h, const iscg_wf_desc* desc, const float* psfPlanes, int32_t nzP,
+                            int32_t size, int32_t oversampling, double zStepNm, int32_t kernelRadius);
+
+/* The planned focus series of one density (replaces WidefieldScene::FocusSeries + WidefieldImages::Render +
+ * ApplyNoiseChain for a persistent population, Sim/WidefieldRender.cpp:547-580,2845-2900).
+ * density: nzD x ny x nx weighted dye counts. Frame f = max(0, image_f) + background; outImages (nF x ch x cw floats,
+ * or NULL) receives it; outAdu (or NULL) its camera ADU, noise stream (seed, frameBase + f, pixel + pixelBase),
+ * offset map (ch x cw or NULL: 0). */
+int32_t iscg_wf_conv3d(iscg_context* h, const float* density, const iscg_camera* cam, const float* offset,
+                       double background, uint32_t seed, uint32_t frameBase, uint32_t pixelBase, float* outImages,
+                       uint16_t* outAdu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,15 @@ class KernelView(C.Structure):
                 ("interpMode", C.c_int32)]
 
 
+class WfDesc(C.Structure):
+    """iscg_wf_desc: a dye grid and its focus series."""
+
+    _fields_ = [("dzUm", C.c_double), ("focus0Um", C.c_double), ("k0", C.c_int64), ("nx", C.c_uint32),
+                ("ny", C.c_uint32), ("nzD", C.c_uint32), ("nF", C.c_uint32), ("fovX0", C.c_uint32),
+                ("fovY0", C.c_uint32), ("cw", C.c_uint32), ("ch", C.c_uint32), ("NX", C.c_uint32),
+                ("NY", C.c_uint32), ("NZ", C.c_uint32), ("upscale", C.c_int32)]
+
+
 class PsfRequest(C.Structure):
     """iscg_psf_request: the numeric fields of sim::PsfGeneratorRequest (Sim/PsfGeneratorBridge.h:91-170)."""
 
@@ -59,6 +68,8 @@ SIGNATURES = {
     "iscg_splat_set_static": (I32, [P, U32, U32, U32, P, P, P, P, D, U64]),
     "iscg_splat_render_frames": (I32, [P, P, U32, P, U32, P, P, P, C.POINTER(Camera), U32, D, P, P]),
     "iscg_splat_render_records": (I32, [P, P, U32, P, U32, P, P, P, C.POINTER(Camera), U32, P, P]),
+    "iscg_wf_conv3d_plan": (I32, [P, C.POINTER(WfDesc), P, I32, I32, I32, D, I32]),
+    "iscg_wf_conv3d": (I32, [P, P, C.POINTER(Camera), P, D, U32, U32, U32, P, P]),
 }
 
 
@@ -66,10 +77,22 @@ def library_path() -> str:
     return LIB_PATH
 
 
+def _share_hip_runtime():
+    """Loads torch's HIP runtime first when torch is installed. torch's wheel carries its own libamdhip64 /
+    libhsa-runtime64 with the same SONAMEs as ROCm's; the loader then resolves libiscg.so's dependencies to the copies
+    already loaded, so torch tensors and libiscg share one runtime (two runtimes in one process cannot both open
+    the GPU)."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """The loaded library (raises if libiscg.so was not built)."""
     global _lib
     if _lib is None:
+        _share_hip_runtime()
         if not os.path.exists(LIB_PATH):
             raise IscgError(f"{LIB_PATH} is missing: build it first (python -m insiliscope_amd.build)")
         L = C.CDLL(LIB_PATH)

@@ -3,6 +3,6 @@
 
 namespace iscg {
 void destroy_psf(iscg_context* h) { h->psf = nullptr; }
-void destroy_wf(iscg_context* h) { h->wf = nullptr; }
+
 void destroy_roi(iscg_context* h) { h->roi = nullptr; }
 }  // namespace iscg

@@ -393,6 +393,51 @@ class SplatGpuHost:
         return max(1, (1 << 30) // max(1, 6 * self.width * self.height))
 
 
+# ---- mean field --------------------------------------------------------------------------------------------------
+
+class WidefieldStack:
+    """A focus series of a dye density as one 3D FFT convolution on the GPU: the per-focus work of
+    sim::WidefieldScene (FocusSeries, Sim/WidefieldRender.h:588) for foci spaced by the dye-plane thickness, with
+    the PSF as a KernelWidefieldPsf of a PsfKernelCache (Sim/WidefieldRender.h:219-236)."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or Context()
+        self.err = ""
+        self.desc = None
+
+    def _call(self, rc: int) -> bool:
+        self.err = "" if rc == 0 else self.ctx.error()
+        return rc == 0
+
+    def Plan(self, cache: PsfKernelCache, nx: int, ny: int, nzD: int, dzUm: float, k0: int, focus0Um: float,
+             nF: int, fov=(0, 0, 0, 0), fft=(0, 0, 0), kernelRadius: int = 0) -> bool:
+        d = _lib.WfDesc(dzUm=dzUm, focus0Um=focus0Um, k0=k0, nx=nx, ny=ny, nzD=nzD, nF=nF, fovX0=fov[0],
+                        fovY0=fov[1], cw=fov[2], ch=fov[3], NX=fft[0], NY=fft[1], NZ=fft[2], upscale=1)
+        pl, keep = _ptr(_arr(cache.planes, np.float32))
+        ok = self._call(self.ctx.L.iscg_wf_conv3d_plan(self.ctx.h, C.byref(d), pl, cache.nz, cache.sizeOversampled,
+                                                       cache.oversampling, cache.zStepNm, kernelRadius))
+        del keep
+        if ok:
+            cw = fov[2] or nx - fov[0]
+            ch = fov[3] or ny - fov[1]
+            self.desc = (nx, ny, nzD, nF, cw, ch)
+        return ok
+
+    @property
+    def image_shape(self):
+        nx, ny, nzD, nF, cw, ch = self.desc
+        return (nF, ch, cw)
+
+    def FocusSeries(self, density, cam: CameraNoiseParams | None = None, offset=None, background: float = 0.0,
+                    seed: int = 0, frameBase: int = 0, pixelBase: int = 0, outImages=None, outAdu=None) -> bool:
+        """density (nzD, ny, nx) -> outImages (nF, ch, cw) float32 frames and/or outAdu uint16 (camera noise)."""
+        ptrs = [_ptr(x) for x in (_arr(density, np.float32), _arr(offset, np.float32), outImages, outAdu)]
+        cc = cam.to_c() if cam is not None else None
+        return self._call(self.ctx.L.iscg_wf_conv3d(self.ctx.h, ptrs[0][0], C.byref(cc) if cc else None,
+                                                    ptrs[1][0], float(background), seed & 0xFFFFFFFF, frameBase,
+                                                    pixelBase, ptrs[2][0], ptrs[3][0]))
+
+
 def SetDefaultBlinkSplatGpu(make):
     """Installs a host factory, as sim::SetDefaultBlinkSplatGpu (Sim/ScopeMovie.h:378) does for the viewer."""
     global _default_splat_factory
@@ -407,5 +452,5 @@ def DefaultBlinkSplatGpu():
 
 
 __all__ = ["Context", "CameraNoiseParams", "ApplyNoiseChain", "ApplyNoiseChainFrames", "PsfKernelCache",
-           "PsfHaloSpans", "SplatGpuHost", "CollectFrameEmitters", "CollectGpuEmitters", "FRAME_EMITTER_DTYPE",
+           "PsfHaloSpans", "SplatGpuHost", "WidefieldStack", "CollectFrameEmitters", "CollectGpuEmitters", "FRAME_EMITTER_DTYPE",
            "SPLAT_EMITTER_DTYPE", "BLINK_EVENT_DTYPE", "INTERP", "IscgError"]
