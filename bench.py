@@ -1,0 +1,390 @@
+"""Throughput of the image-formation path on one MI355X, against the reference CPU path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload wf|smlm|psf] [--no-cpu] [--no-secondary]
+
+Default workload (`wf`): the widefield mean field of a 256 x 256 x 64 dye density (100 x 100 x 50 nm voxels, seeded)
+convolved with a 127-plane Gibson-Lanni PSF (257 x 257 at 100 nm, 50 nm apart; NA 1.45, ni 1.518, ns 1.33, 670 nm,
+Noll 5-15 +-0.05 waves) by a zero-padded 512 x 512 x 128 FP32 R2C/C2R 3D FFT, sampled at the 64 dye-plane foci, then
+sCMOS noise (read 1.5 e-, gain 2 e-/ADU, offset 100) -> 64 uint16 frames. A step is one such focus stack; the metric
+is frames (focus images) per second with the density already in HBM.
+
+Secondary lines in the same JSON: the SMLM DNA-PAINT movie (512 x 512, Poisson(800) blinks/frame, Catmull-Rom splat
+from a 65 x 65 x 81 table, EMCCD noise) in frames/s, and the PSF library (512 PSFs x 41 planes of 65 x 65, M 128) in
+PSFs/s.
+
+roofline: the dominant kernel's algorithmic bytes per launch / its mean launch time (HIP events on the library's
+stream during the timed steps) against 8 TB/s HBM. cpu_baseline: the reference's own CPU code (compiled from its
+sources into oracle/_ref) on a bounded sample, all host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "frames/s (SMLM 512² & WF 256²×64z) + HBM GB/s; fwd+adjoint at ≤1e-5 rel-L2"
+HBM_PEAK_GBS = 8000.0
+# host threads for the CPU baseline: the GPU box's CPU share per GPU (16), or fewer where the host has fewer
+CORES = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "16")), os.cpu_count() or 1))
+MFMA_FP16_PEAK_TFS = 2500.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Timer:
+    """HIP events on the library context's stream."""
+
+    def __init__(self, ctx):
+        import torch
+        self.torch = torch
+        self.stream = torch.cuda.ExternalStream(ctx.stream)
+        self.ctx = ctx
+
+    def __enter__(self):
+        self.ctx.synchronize()
+        self.a = self.torch.cuda.Event(enable_timing=True)
+        self.b = self.torch.cuda.Event(enable_timing=True)
+        self.a.record(self.stream)
+        return self
+
+    def __exit__(self, *exc):
+        self.b.record(self.stream)
+        self.b.synchronize()
+        self.ms = self.a.elapsed_time(self.b)
+
+
+def profile(ctx, L):
+    import ctypes as C
+    cap = 64
+    names = C.create_string_buffer(64 * cap)
+    launches = (C.c_int64 * cap)()
+    total = (C.c_double * cap)()
+    n = L.iscg_profile_read(ctx.h, names, launches, total, cap, 1)
+    out = {}
+    for i in range(min(n, cap)):
+        nm = names.raw[64 * i:64 * (i + 1)].split(b"\0")[0].decode()
+        out[nm] = (int(launches[i]), float(total[i]))
+    return out
+
+
+def traffic_for(kernel):
+    p = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+# ---- widefield mean field (C2) -----------------------------------------------------------------------------
+
+def wf_psf_request():
+    from insiliscope_amd import configs
+    return configs.psf_request_fields(wavelengthNm=670.0, na=1.45, immersionIndex=1.518, sampleIndex=1.33,
+                                      workingDistanceUm=170.0, pixelSizeNm=100.0, zStepNm=50.0, oversampling=1,
+                                      kernelHalfWidthPx=128, nz=127, pupilSamples=256,
+                                      zernike=configs.zernike_noll(1, range(5, 16), amplitude=0.05))
+
+
+def wf_bytes(nx, ny, nzD, nF, NX, NY, NZ):
+    """Algorithmic HBM bytes per launch of each pass: each pass reads its input and writes its output once (half
+    spectra of NX/2 + 1 complex64 columns; zero padding never read)."""
+    h = NX // 2 + 1
+    c = 8
+    return {
+        "wf_rows_r2c": nzD * ny * nx * 4 + nzD * ny * h * c,
+        "wf_cols_fwd": nzD * ny * h * c + nzD * NY * h * c,
+        "wf_zconv": nzD * NY * h * c + NZ * NY * h * c + nF * NY * h * c,
+        "wf_cols_inv": nF * NY * h * c + nF * ny * h * c,
+        "wf_rows_c2r_noise": nF * ny * h * c + nF * ny * nx * 2,
+    }
+
+
+def run_wf(ctx, steps, warmup):
+    import torch
+    from insiliscope_amd import configs, host
+    L = ctx.L
+    nx = ny = 256
+    nzD = nF = 64
+    t0 = time.time()
+    psf = host.ComputePsfPlanesBatch([wf_psf_request()], normalize=True,
+                                     out=torch.empty((1, 127, 257, 257), dtype=torch.float32, device="cuda"),
+                                     ctx=ctx)[0]
+    ctx.synchronize()
+    cache = host.PsfKernelCache(psf, oversampling=1, zStepNm=50.0)
+    wf = host.WidefieldStack(ctx)
+    if not wf.Plan(cache, nx, ny, nzD, 0.05, 0, 0.025, nF, fft=(512, 512, 128)):
+        raise RuntimeError(wf.err)
+    dens = torch.from_numpy(configs.c2_density(seed=1)).cuda()
+    off = torch.full((ny, nx), 100.0, dtype=torch.float32, device="cuda")
+    adu = torch.empty((nF, ny, nx), dtype=torch.int16, device="cuda")
+    cam = configs.scmos_c1()
+    ctx.synchronize()
+    log(f"wf setup {time.time() - t0:.1f} s (PSF 127 x 257^2 on the GPU, plan)")
+
+    def step(k):
+        if not wf.FocusSeries(dens, cam=cam, offset=off, seed=1, frameBase=k * nF, outAdu=adu):
+            raise RuntimeError(wf.err)
+
+    for k in range(warmup):
+        step(k)
+    L.iscg_profile_enable(ctx.h, 1)
+    profile(ctx, L)
+    with Timer(ctx) as t:
+        for k in range(steps):
+            step(warmup + k)
+    prof = profile(ctx, L)
+    L.iscg_profile_enable(ctx.h, 0)
+    ms = t.ms / steps
+    nbytes = wf_bytes(nx, ny, nzD, nF, 512, 512, 128)
+    dom = max(prof, key=lambda k: prof[k][1])
+    avg_ms = prof[dom][1] / prof[dom][0]
+    achieved = nbytes[dom] / (avg_ms * 1e-3) / 1e9
+    total_bytes = sum(nbytes.values())
+    kernels = {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 5),
+                   "GB_s": round(nbytes.get(k, 0) / (v[1] / v[0] * 1e-3) / 1e9, 1)} for k, v in prof.items()}
+    traffic = traffic_for(dom)
+    return {
+        "value": nF * 1000.0 / ms, "ms_per_step": ms, "dominant": dom, "_psf": psf,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": dom, "algorithmic_bytes": nbytes[dom]},
+        "stack_GB_s": round(total_bytes / (ms * 1e-3) / 1e9, 1),
+        "kernels": kernels,
+    }
+
+
+def cpu_wf(planes):
+    """The reference's WidefieldScene (UpdateFromGrid + FocusSeries: per dye plane a 2D real FFT, per focus the
+    spectra re-paired with the PSF plane spectra and one inverse FFT) on the same density and PSF planes, timed on
+    the host cores (ParallelFor, ISC_THREADS = all cores): one whole 64-focus stack after a warm-up stack."""
+    import oracle as O
+    from insiliscope_amd import configs
+    ref = O.load_reference()
+    if ref is None:
+        return None
+    dens = configs.c2_density(seed=1)
+    foci = (np.arange(64) + 0.5) * 0.05
+    ref.wf_focus_series(dens, 50.0, planes, 1, 50.0, foci[:2], kernel_cap_um=12.8)
+    t0 = time.time()
+    ref.wf_focus_series(dens, 50.0, planes, 1, 50.0, foci, kernel_cap_um=12.8)
+    el = time.time() - t0
+    return {"value": round(len(foci) / el, 2), "unit": "frames/s", "cores": CORES, "kind": "reference",
+            "sample": f"WidefieldScene::UpdateFromGrid + FocusSeries of the 256x256x64 density at all 64 foci "
+                      f"({el:.2f} s, kernel radius 128 cells; camera noise not included)"}
+
+
+# ---- SMLM DNA-PAINT (C4) ---------------------------------------------------------------------------------------
+
+def run_smlm(ctx, steps, warmup, n_frames=2000):
+    import torch
+    from insiliscope_amd import configs, host
+    L = ctx.L
+    t0 = time.time()
+    planes = host.ComputePsfPlanesBatch([configs.c4_psf(seed=1)], normalize=True,
+                                        out=torch.empty((1, 81, 65, 65), dtype=torch.float32, device="cuda"),
+                                        ctx=ctx)[0]
+    cache = host.PsfKernelCache(planes, oversampling=4, zStepNm=15.0, interpMode=2)
+    ev, st = configs.c4_events(seed=1, n_frames=n_frames)
+    ems = np.zeros(len(ev), host.FRAME_EMITTER_DTYPE)
+    for f in range(n_frames):
+        ems[st[f]:st[f + 1]] = host.CollectFrameEmitters(ev[st[f]:st[f + 1]], f, 100.0, 1.0, psfCache=cache)
+    d_ems = torch.from_numpy(ems.view(np.uint8).reshape(len(ems), host.FRAME_EMITTER_DTYPE.itemsize)).cuda()
+    d_start = torch.from_numpy(st.astype(np.int32)).cuda()
+    d_frames = torch.arange(n_frames, dtype=torch.int32, device="cuda")
+    w = h = 512
+    off = np.full((h, w), 100.0, np.float32)
+    sp = host.SplatGpuHost(ctx)
+    if not (sp.SetKernels([cache]) and sp.SetStatic(w, h, offset=off, flatBackground=50.0)):
+        raise RuntimeError(sp.err)
+    adu = torch.empty((n_frames, h, w), dtype=torch.int16, device="cuda")
+    cam = configs.emccd_c4()
+    ctx.synchronize()
+    log(f"smlm setup {time.time() - t0:.1f} s ({len(ev)} blinks over {n_frames} frames)")
+
+    def step():
+        if not sp.RenderBatch(d_ems, d_start, d_frames, cam, 1, outAdu=adu):
+            raise RuntimeError(sp.err)
+
+    for _ in range(warmup):
+        step()
+    L.iscg_profile_enable(ctx.h, 1)
+    profile(ctx, L)
+    with Timer(ctx) as t:
+        for _ in range(steps):
+            step()
+    prof = profile(ctx, L)
+    L.iscg_profile_enable(ctx.h, 0)
+    ms = t.ms / steps
+    # algorithmic bytes of the splat+noise pass: ADU out, emitter records and setup in, offset map
+    tiles = prof.get("splat_tiles_noise", (1, ms))
+    b = n_frames * w * h * 2 + len(ev) * (112 + 8) + w * h * 4
+    t_ms = tiles[1] / tiles[0]
+    return {"value": n_frames * 1000.0 / ms, "unit": "frames/s", "ms_per_step": ms,
+            "config": {"workload": "SMLM DNA-PAINT 512x512 x 2000 frames, Poisson(800) blinks/frame, Catmull-Rom "
+                                   "splat from 65x65x81 (os 4, 15 nm), EMCCD gain 300 read 50 e-",
+                       "blinks": int(len(ev))},
+            "splat_noise_kernel": {"avg_ms": round(t_ms, 4), "GB_s": round(b / (t_ms * 1e-3) / 1e9, 1),
+                                   "bound": "fp64 valu (counter RNG + Knuth/Gamma noise)"},
+            "_events": (ev, st, np.asarray(planes.cpu()))}
+
+
+def cpu_smlm(ev, st, planes, frames=96):
+    import oracle as O
+    from insiliscope_amd import configs
+    ref = O.load_reference()
+    if ref is None:
+        return None
+    cam = configs.emccd_c4()
+    oc = O.camera(quantumEfficiency=1.0, gainPhotonsPerAdu=cam.gainPhotonsPerAdu, readNoiseElectrons=50.0,
+                  emccd=1, emGain=300.0, cicElectrons=0.002)
+    off = np.full((512, 512), 100.0, np.float32)
+    sub_ev = ev[: st[frames]]
+    t0 = time.time()
+    ref.render_noise_frames(sub_ev, st[: frames + 1], 512, 512, 100.0, 50.0, planes, 4, 15.0, 2, oc, off, 1)
+    el = time.time() - t0
+    return {"value": round(frames / el, 2), "unit": "frames/s", "cores": CORES, "kind": "reference",
+            "sample": f"RenderPhotonImage (Cubic) + ApplyNoiseChain (EMCCD), {frames} of the 2000 frames, frames in "
+                      f"parallel as FluorescenceMovie batches them"}
+
+
+# ---- PSF library (C3) -------------------------------------------------------------------------------------------
+
+def run_psf(ctx, steps, warmup):
+    import torch
+    from insiliscope_amd import configs, host
+    L = ctx.L
+    reqs = configs.c3_requests(seed=1)
+    out = torch.empty((len(reqs), 41, 65, 65), dtype=torch.float32, device="cuda")
+
+    def step():
+        host.ComputePsfPlanesBatch(reqs, out=out, ctx=ctx)
+
+    for _ in range(warmup):
+        step()
+    L.iscg_profile_enable(ctx.h, 1)
+    profile(ctx, L)
+    with Timer(ctx) as t:
+        for _ in range(steps):
+            step()
+    prof = profile(ctx, L)
+    L.iscg_profile_enable(ctx.h, 0)
+    ms = t.ms / steps
+    planes = len(reqs) * 41
+    flops = planes * 8.0 * 65 * 128 * (128 + 65)
+    k = prof.get("psf_planes_mfma", (1, ms))
+    kms = k[1] / k[0]
+    tf = flops / (kms * 1e-3) / 1e12
+    return {"value": len(reqs) * 1000.0 / ms, "unit": "PSFs/s", "ms_per_step": ms,
+            "planes_per_s": planes * 1000.0 / ms,
+            "config": {"workload": "PSF library: 8 wavelengths x 64 Zernike sets (Noll 4-21, sigma 0.04), 41 planes "
+                                   "of 65x65, M 128"},
+            "mfma_kernel": {"avg_ms": round(kms, 4), "TFLOP_s_complex_fp32_equiv": round(tf, 1),
+                            "TFLOP_s_executed_fp16": round(3 * tf, 1),
+                            "frac_of_fp16_dense_peak": round(3 * tf / MFMA_FP16_PEAK_TFS, 4)},
+            "kernels": {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4)} for k, v in prof.items()}}
+
+
+def cpu_psf(n=64):
+    import oracle as O
+    from insiliscope_amd import configs
+    ref = O.load_reference()
+    if ref is None:
+        return None
+    reqs = [O.psf_request(**r) for r in configs.c3_requests(seed=1)[:n]]
+    t0 = time.time()
+    ref.psf_library(reqs)
+    el = time.time() - t0
+    return {"value": round(n / el, 2), "unit": "PSFs/s", "cores": CORES, "kind": "reference",
+            "sample": f"ComputeZernikePsfPlanes x {n} requests (41 planes each, planes in parallel)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="wf", choices=["wf", "smlm", "psf"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the image-formation path has no multi-GPU mode in this build "
+              "(frames and PSFs shard across GPUs without exchange; run one process per GPU).", file=sys.stderr)
+        sys.exit(2)
+    os.environ["ISC_THREADS"] = str(CORES)
+    import torch
+    from insiliscope_amd import host
+    if not torch.cuda.is_available():
+        print("no GPU", file=sys.stderr)
+        sys.exit(1)
+    ctx = host.Context(0)
+    secondary = {}
+    if args.workload == "wf":
+        r = run_wf(ctx, args.steps, args.warmup)
+        config = {"workload": "WF mean field 256x256x64 density (100x100x50 nm), 127-plane Gibson-Lanni PSF 257x257, "
+                              "512x512x128 FP32 R2C/C2R 3D FFT, 64 foci, sCMOS noise",
+                  "frames_per_step": 64, "fft": [512, 512, 128], "kernels": r["kernels"],
+                  "stack_GB_s_algorithmic": r["stack_GB_s"]}
+        unit, dtype = "frames/s", "f32"
+    elif args.workload == "smlm":
+        r = run_smlm(ctx, args.steps, args.warmup)
+        r.pop("_events")
+        config, unit, dtype = r["config"], "frames/s", "f64"
+        r["roofline"] = None
+    else:
+        r = run_psf(ctx, args.steps, args.warmup)
+        config, unit, dtype = r["config"], "PSFs/s", "f16x3/f32"
+        r["roofline"] = None
+    cpu = None
+    if args.workload == "wf" and not args.no_secondary:
+        try:
+            s = run_smlm(ctx, 2, 1)
+            ev, st, planes = s.pop("_events")
+            secondary["smlm_c4"] = s
+            if not args.no_cpu:
+                c = cpu_smlm(ev, st, planes)
+                if c:
+                    s["cpu_baseline"] = c
+                    s["speedup_vs_cpu"] = round(s["value"] / c["value"], 1)
+        except Exception as e:  # a secondary line must not hide the primary one
+            secondary["smlm_c4"] = {"error": str(e)}
+        try:
+            p = run_psf(ctx, 3, 1)
+            secondary["psf_library_c3"] = p
+            if not args.no_cpu:
+                c = cpu_psf()
+                if c:
+                    p["cpu_baseline"] = c
+                    p["speedup_vs_cpu"] = round(p["value"] / c["value"], 1)
+        except Exception as e:
+            secondary["psf_library_c3"] = {"error": str(e)}
+    if not args.no_cpu and args.workload == "wf":
+        try:
+            cpu = cpu_wf(np.ascontiguousarray(r["_psf"].cpu().numpy()))
+        except Exception as e:
+            cpu = {"error": str(e)}
+    r.pop("_psf", None)
+    line = {
+        "metric": METRIC, "value": round(r["value"], 2), "unit": unit, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(r["ms_per_step"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic (pcg4d-seeded)",
+        "config": config, "roofline": r.get("roofline"), "cpu_baseline": cpu,
+    }
+    if cpu and cpu.get("value"):
+        line["speedup_vs_cpu"] = round(r["value"] / cpu["value"], 1)
+    if secondary:
+        line["secondary"] = secondary
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

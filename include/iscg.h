@@ -46,6 +46,13 @@ void* iscg_stream(iscg_context* h);
  * would (a caller's CPU fallback, Sim/ScopeMovie.cpp:7634-7645, is exercised without faulting a GPU). */
 int32_t iscg_inject_failure(iscg_context* h, int32_t count, const char* message);
 
+/* Per-kernel timing on the context's stream (the ISC_PROFILE analogue, Sim/Timing.h:1-60): with profiling on, each
+ * kernel launch is bracketed by HIP events. iscg_profile_read waits for the stream and reports, per kernel name,
+ * launches and total milliseconds (names: cap x 64 chars); returns the number of kernels. reset != 0 clears. */
+int32_t iscg_profile_enable(iscg_context* h, int32_t on);
+int32_t iscg_profile_read(iscg_context* h, char* names, int64_t* launches, double* totalMs, int32_t cap,
+                          int32_t reset);
+
 /* ---- camera noise ---------------------------------------------------------------------------------------------- */
 
 /* sim::CameraNoiseParams (Sim/SMLMNoise.h:35-50). */
@@ -136,6 +143,42 @@ int32_t iscg_splat_render_records(iscg_context* h, const iscg_splat_emitter* ems
                                   const uint32_t* emStart, uint32_t nFrames, const uint32_t* frames,
                                   const float* bgScales, const float* extras, const iscg_camera* cam, uint32_t seed,
                                   uint16_t* outAdu, float* outPhotons);
+
+/* ---- PSF planes ------------------------------------------------------------------------------------------------ */
+
+/* The numeric fields of sim::PsfGeneratorRequest (Sim/PsfGeneratorBridge.h:91-170) for the GibsonLanniZernike model;
+ * the Zernike coefficients as the 28 OSA/ANSI values in waves instead of the formatted string. */
+typedef struct
+{
+   double wavelengthNm;
+   double na;
+   double immersionIndex;
+   double sampleIndex;
+   double workingDistanceUm;
+   double sampleDepthNm;
+   double pixelSizeNm;
+   double zStepNm;
+   double maskWaist;
+   double zernike[28];
+   int32_t maskType; /* PsfMaskType: 0 none, 1 double helix */
+   int32_t maskModes;
+   int32_t oversampling;
+   int32_t kernelHalfWidthPx;
+   int32_t nz;
+   int32_t pupilSamples; /* 0: ZernikePupilSamples's choice */
+   int32_t interpMode;
+   int32_t reserved;
+} iscg_psf_request;
+
+/* GeometryFor + ZernikePupilSamples (Sim/ZernikePsf.cpp:118-148): plane size 2 hw os + 1, nz (odd, >= 3), pupil
+ * samples per axis. Host only. */
+int32_t iscg_psf_geometry(const iscg_psf_request* r, int32_t* size, int32_t* nz, int32_t* pupilSamples);
+
+/* Replaces ComputeZernikePsfPlanes (Sim/ZernikePsf.h:51; normalize != 0: the sum-1 planes of
+ * BuildZernikePsfKernelCache, Sim/ZernikePsf.cpp:382-417) for nReq requests sharing size, nz and pupil samples
+ * (<= 256). out: nReq x nz x size x size floats (outCap floats available); plane k at defocus (k - (nz-1)/2) zStep. */
+int32_t iscg_psf_planes(iscg_context* h, const iscg_psf_request* reqs, int32_t nReq, int32_t normalize, float* out,
+                        size_t outCap);
 
 /* ---- mean field: all foci of a z stack by one 3D FFT convolution ----------------------------------------------- */
 

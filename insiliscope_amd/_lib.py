@@ -63,11 +63,15 @@ SIGNATURES = {
     "iscg_synchronize": (I32, [P]),
     "iscg_stream": (P, [P]),
     "iscg_inject_failure": (I32, [P, I32, C.c_char_p]),
+    "iscg_profile_enable": (I32, [P, I32]),
+    "iscg_profile_read": (I32, [P, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), I32, I32]),
     "iscg_noise": (I32, [P, P, U32, U32, C.POINTER(Camera), P, P, P, U32, P, U32, U32, P]),
     "iscg_splat_set_kernels": (I32, [P, C.POINTER(KernelView), I32]),
     "iscg_splat_set_static": (I32, [P, U32, U32, U32, P, P, P, P, D, U64]),
     "iscg_splat_render_frames": (I32, [P, P, U32, P, U32, P, P, P, C.POINTER(Camera), U32, D, P, P]),
     "iscg_splat_render_records": (I32, [P, P, U32, P, U32, P, P, P, C.POINTER(Camera), U32, P, P]),
+    "iscg_psf_geometry": (I32, [C.POINTER(PsfRequest), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
+    "iscg_psf_planes": (I32, [P, C.POINTER(PsfRequest), I32, I32, P, C.c_size_t]),
     "iscg_wf_conv3d_plan": (I32, [P, C.POINTER(WfDesc), P, I32, I32, I32, D, I32]),
     "iscg_wf_conv3d": (I32, [P, P, C.POINTER(Camera), P, D, U32, U32, U32, P, P]),
 }

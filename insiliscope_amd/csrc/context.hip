@@ -85,6 +85,39 @@ bool finish_out(iscg_context* h, void* user, const void* dev, size_t bytes)
    return ok(h, hipStreamSynchronize(h->stream), "stream synchronize");
 }
 
+static hipEvent_t take_event(iscg_context* h)
+{
+   if (!h->eventPool.empty())
+   {
+      hipEvent_t e = h->eventPool.back();
+      h->eventPool.pop_back();
+      return e;
+   }
+   hipEvent_t e = nullptr;
+   if (hipEventCreate(&e) != hipSuccess)
+      return nullptr;
+   return e;
+}
+
+ProfScope::ProfScope(iscg_context* hh, const char* nm) : h(hh), name(nm)
+{
+   if (h->profiling)
+   {
+      a = take_event(h);
+      if (a && hipEventRecord(a, h->stream) != hipSuccess)
+         a = nullptr;
+   }
+}
+
+ProfScope::~ProfScope()
+{
+   if (!a)
+      return;
+   hipEvent_t b = take_event(h);
+   if (b && hipEventRecord(b, h->stream) == hipSuccess)
+      h->pending.push_back({name, a, b});
+}
+
 }  // namespace iscg
 
 using namespace iscg;
@@ -158,6 +191,13 @@ void iscg_destroy(iscg_context* h)
    destroy_roi(h);
    for (auto& b : h->stage)
       b.release();
+   for (auto& p : h->pending)
+   {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+   }
+   for (auto e : h->eventPool)
+      (void)hipEventDestroy(e);
    (void)hipStreamDestroy(h->stream);
    delete h;
 }
@@ -174,6 +214,56 @@ int32_t iscg_synchronize(iscg_context* h)
 }
 
 void* iscg_stream(iscg_context* h) { return h ? (void*)h->stream : nullptr; }
+
+int32_t iscg_profile_enable(iscg_context* h, int32_t on)
+{
+   if (!h)
+      return -1;
+   h->profiling = on != 0;
+   return 0;
+}
+
+int32_t iscg_profile_read(iscg_context* h, char* names, int64_t* launches, double* totalMs, int32_t cap,
+                          int32_t reset)
+{
+   if (!h)
+      return -1;
+   if (!ok(h, hipSetDevice(h->device), "hipSetDevice") || !ok(h, hipStreamSynchronize(h->stream), "synchronize"))
+      return -1;
+   for (auto& p : h->pending)
+   {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess)
+      {
+         auto& t = h->profTotals[p.name];
+         t.first += 1;
+         t.second += ms;
+      }
+      h->eventPool.push_back(p.a);
+      h->eventPool.push_back(p.b);
+   }
+   h->pending.clear();
+   int32_t i = 0;
+   for (const auto& kv : h->profTotals)
+   {
+      if (i < cap)
+      {
+         if (names)
+         {
+            std::strncpy(names + 64 * i, kv.first.c_str(), 63);
+            names[64 * i + 63] = 0;
+         }
+         if (launches)
+            launches[i] = kv.second.first;
+         if (totalMs)
+            totalMs[i] = kv.second.second;
+      }
+      ++i;
+   }
+   if (reset)
+      h->profTotals.clear();
+   return i;
+}
 
 int32_t iscg_inject_failure(iscg_context* h, int32_t count, const char* message)
 {

@@ -8,6 +8,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -66,6 +67,16 @@ struct iscg_context
    int device = 0;
    hipStream_t stream = nullptr;
    std::string err;
+   // per-kernel timing (iscg_profile_*): event pairs around each launch on the context's stream
+   struct ProfPending
+   {
+      const char* name;
+      hipEvent_t a, b;
+   };
+   bool profiling = false;
+   std::vector<ProfPending> pending;
+   std::vector<hipEvent_t> eventPool;
+   std::map<std::string, std::pair<long long, double>> profTotals;
    int injectCount = 0;
    std::string injectMessage;
    // staging for host arguments, one per argument slot of a call
@@ -246,6 +257,17 @@ __device__ __forceinline__ uint16_t noise_pixel(const CamParams& cp, float photo
    adu = adu < 0.0 ? 0.0 : (adu > 65535.0 ? 65535.0 : adu);
    return (uint16_t)floor(adu + 0.5);
 }
+
+// Times the launches made in its scope when profiling is on: events recorded on the context's stream before and
+// after, resolved by iscg_profile_read.
+struct ProfScope
+{
+   iscg_context* h;
+   const char* name;
+   hipEvent_t a = nullptr;
+   ProfScope(iscg_context* hh, const char* nm);
+   ~ProfScope();
+};
 
 // Grid size for a grid-stride launch over n items with `block` threads: enough workgroups to fill all 256 CUs
 // several times over, never more than the items need.

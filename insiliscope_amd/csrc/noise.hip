@@ -34,6 +34,7 @@ bool launch_noise(iscg_context* h, const float* photons, uint16_t* out, uint32_t
    if (total == 0)
       return true;
    const unsigned block = 256, grid = grid_for(total, block, 16);
+   ProfScope ps(h, "noise");
    if (cp.emccd)
       hipLaunchKernelGGL(noise_kernel<true>, dim3(grid), dim3(block), 0, h->stream, photons, out, npx, nFrames, frames,
                          offset, gain, readN, cp, seed, pixelBase);

@@ -1,0 +1,566 @@
+// psf.hip -- the Gibson-Lanni + Zernike PSF planes (ComputeZernikePsfPlanes, Sim/ZernikePsf.cpp:150-337) for a
+// batch of requests x z planes.
+//
+// The reference takes the pupil to the image plane by a separable Bluestein chirp-Z transform in FP64
+// (Sim/ChirpZ.h:28-141): out[p] = sum_m in[m] exp(i (kMin + m dk)(x0 + p dx)). That is a dense P x M complex
+// matrix W, so a plane is E = W Pup W^T: two complex matrix products per plane, the shape MFMA is built for.
+//
+//   tables (per request, FP64, Sim/ZernikePsf.cpp:199-231): the z-independent phase A = k0 opd1 + 2 pi Zernike + mask
+//     and slope B = k0 ni opd3Factor on the M x M pupil grid (NaN outside the aperture);
+//   W (per request): exp(i (kMin + m dk)(x0 + p dx)), the argument in FP64 and wrapped to [-pi, pi] before the FP32
+//     sincos, stored as fp16 hi + lo parts (hi = fp16(v), lo = fp16(v - hi): ~22 significant bits);
+//   planes: one workgroup per (request, plane, block of output rows). Per 32-row chunk of the pupil the FP64
+//     phase A + (ti - ti0) B is wrapped and turned into the split pupil in LDS; mid = W_rows Pup and then
+//     E = mid W^T run on v_mfma_f32_16x16x32_f16 as hi*hi + hi*lo + lo*hi products accumulated in FP32
+//     (12 MFMAs per complex 16x16x32 step); |E|^2 is written as FP32.
+// Work per plane 8 P M (M + P) real FLOP counted as complex FP32 (C3: 12.85 MFLOP); executed MFMA work is 3x that.
+#include "iscg_internal.h"
+
+#include <cmath>
+#include <string>
+#include <type_traits>
+
+namespace iscg {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kKP = 40;  // LDS row of a 32-long k slice, padded (halves)
+
+struct PsfReqDev
+{
+   double k0, NA, ni, ns, bMax, kMax, dk, kMin, x0, dx, depth, zStepM, ti0, focalShift, maskWaist;
+   double coeff[28];
+   int n[28], l[28], m[28];
+   int M, size, nz, nModes, doubleHelix, maskModes, pad[2];
+};
+
+struct PsfState
+{
+   DevBuf reqs, tabA, tabB, W, tz, sums;
+};
+
+void destroy_psf(iscg_context* h)
+{
+   delete h->psf;
+   h->psf = nullptr;
+}
+
+// ---- tables -------------------------------------------------------------------------------------------------
+
+__device__ double binomial_d(int a, int b)
+{
+   if (b < 0 || b > a)
+      return 0.0;
+   double r = 1.0;
+   for (int i = 0; i < b; ++i)
+      r = r * (a - i) / (i + 1);
+   return r;
+}
+
+// Sim/ZernikePsf.cpp:58-78 (unnormalised radial polynomial, cos for l >= 0, sin for l < 0, zero beyond rho 1)
+__device__ double zernike_value_d(int n, int l, int m, double rho, double phi)
+{
+   double radial = 0.0;
+   if (rho <= 1.0)
+      for (int k = 0; k <= (n - m) / 2; ++k)
+      {
+         const double coeff = (k % 2 == 0 ? 1.0 : -1.0) * binomial_d(n - k, k) * binomial_d(n - 2 * k, (n - m) / 2 - k);
+         radial += coeff * pow(rho, (double)(n - 2 * k));
+      }
+   return radial * (l >= 0 ? cos(m * phi) : sin(m * phi));
+}
+
+// Sim/ZernikePsf.cpp:80-110 (Gauss-Laguerre double-helix mask phase)
+__device__ double mask_phase_d(int maskModes, double maskWaist, double rhoNorm, double phi)
+{
+   const int n = maskModes > 2 ? maskModes : 2;
+   const double w = 0.2 < maskWaist ? maskWaist : 0.2;
+   const double rc = rhoNorm < 1.0 ? rhoNorm : 1.0;
+   const double u = (0.0 < rc ? rc : 0.0) / w;
+   const double u2 = u * u;
+   double re = 0.0, im = 0.0;
+   for (int p = 0; p < n; ++p)
+   {
+      const int l = 2 * p + 1;
+      double lm1 = 0.0, L = 1.0;
+      const double x = 2.0 * u2;
+      for (int k = 0; k < p; ++k)
+      {
+         const double next = ((2 * k + 1 + l - x) * L - (k + l) * lm1) / (k + 1);
+         lm1 = L;
+         L = next;
+      }
+      const double amp = pow(u, (double)l) * exp(-u2) * L;
+      re += amp * cos(l * phi);
+      im += amp * sin(l * phi);
+   }
+   return (re == 0.0 && im == 0.0) ? 0.0 : atan2(im, re);
+}
+
+// A (base phase) and B (phase per metre of ti - ti0) on the Mp x Mp grid; NaN outside the aperture or the M grid.
+__global__ void psf_tables_kernel(const PsfReqDev* __restrict__ reqs, int nReq, int Mp, double* __restrict__ A,
+                                  double* __restrict__ B)
+{
+   const size_t per = (size_t)Mp * Mp, total = per * nReq;
+   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x)
+   {
+      const int r = (int)(idx / per);
+      const int rem = (int)(idx - (size_t)r * per);
+      const int iy = rem / Mp, ix = rem % Mp;
+      const PsfReqDev& q = reqs[r];
+      double a = __builtin_nan(""), b = 0.0;
+      const int c0 = q.M / 2;
+      if (iy < q.M && ix < q.M)
+      {
+         const double ky = (iy - c0) * q.dk, kx = (ix - c0) * q.dk;
+         const double kr2 = kx * kx + ky * ky;
+         if (!(kr2 > q.kMax * q.kMax))
+         {
+            const double kr = sqrt(kr2);
+            const double rho = kr / (q.k0 * q.NA);
+            const double phi = atan2(ky, kx);
+            const double s1 = q.NA * rho / q.ns, s3 = q.NA * rho / q.ni;
+            const double a1 = 1.0 - s1 * s1, a3 = 1.0 - s3 * s3;
+            const double opd1 = q.ns * q.depth * sqrt(0.0 < a1 ? a1 : 0.0);
+            const double opd3Factor = sqrt(0.0 < a3 ? a3 : 0.0);
+            double zp = 0.0;
+            for (int t = 0; t < q.nModes; ++t)
+               zp += q.coeff[t] * zernike_value_d(q.n[t], q.l[t], q.m[t], rho / q.bMax, phi);
+            zp *= 2.0 * kPi;
+            const double mask = q.doubleHelix ? mask_phase_d(q.maskModes, q.maskWaist, rho / q.bMax, phi) : 0.0;
+            a = q.k0 * opd1 + zp + mask;
+            b = q.k0 * (q.ni * opd3Factor);
+         }
+      }
+      A[idx] = a;
+      B[idx] = b;
+   }
+}
+
+__device__ __forceinline__ void split_store(float v, _Float16* hi, _Float16* lo)
+{
+   const _Float16 h = (_Float16)v;
+   *hi = h;
+   *lo = (_Float16)(v - (float)h);
+}
+
+// Wraps an FP64 angle to [-pi, pi] in FP64, then FP32 sincos of the remainder.
+__device__ __forceinline__ void wrapped_sincos(double ang, float* s, float* c)
+{
+   const double r = ang - 6.283185307179586476925 * rint(ang * 0.15915494309189533577);
+   sincosf((float)r, s, c);
+}
+
+// W planes per request: [4][Pp][Mp] halves (re hi, re lo, im hi, im lo); zero beyond P rows / M columns.
+__global__ void psf_w_kernel(const PsfReqDev* __restrict__ reqs, int nReq, int Pp, int Mp, _Float16* __restrict__ W)
+{
+   const size_t per = (size_t)Pp * Mp, total = per * nReq;
+   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x)
+   {
+      const int r = (int)(idx / per);
+      const int rem = (int)(idx - (size_t)r * per);
+      const int p = rem / Mp, m = rem % Mp;
+      const PsfReqDev& q = reqs[r];
+      float s = 0.0f, c = 0.0f;
+      if (p < q.size && m < q.M)
+         wrapped_sincos((q.kMin + m * q.dk) * (q.x0 + p * q.dx), &s, &c);
+      _Float16* base = W + (size_t)r * 4 * per + rem;
+      split_store(c, base, base + per);
+      split_store(s, base + 2 * per, base + 3 * per);
+   }
+}
+
+// ---- planes ---------------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ half8 ld8(const _Float16* p) { return *(const half8*)p; }
+
+__device__ __forceinline__ float4v mfma(half8 a, half8 b, float4v c)
+{
+   return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// acc += (ahi + alo)(bhi + blo) without the lo*lo term
+__device__ __forceinline__ float4v mfma3(half8 ahi, half8 alo, half8 bhi, half8 blo, float4v acc)
+{
+   acc = mfma(ahi, bhi, acc);
+   acc = mfma(ahi, blo, acc);
+   return mfma(alo, bhi, acc);
+}
+
+// One workgroup (4 waves) per (request, plane, block of YB*16 output rows). MP: padded pupil samples (multiple of
+// 32), PP: padded output size (multiple of 16).
+template <int MP, int YB>
+__global__ __launch_bounds__(256) void psf_planes_kernel(const PsfReqDev* __restrict__ reqs, const double* __restrict__ A,
+                                                         const double* __restrict__ B, const _Float16* __restrict__ W,
+                                                         const double* __restrict__ tz, int nz, int Pp, int nYBlocks,
+                                                         float* __restrict__ out)
+{
+   constexpr int CT1 = MP / 16;              // column tiles of mid
+   constexpr int T1 = YB * CT1;              // tiles of GEMM 1
+   constexpr int PER1 = (T1 + 3) / 4;        // per wave
+   constexpr int MS = MP + 8;                // LDS row of mid (halves)
+   __shared__ __attribute__((aligned(16))) _Float16 pup[4][MP][kKP];      // pupil chunk, [part][m][k]
+   __shared__ __attribute__((aligned(16))) _Float16 mid[4][YB * 16][MS];  // mid rows, [part][y][m]
+
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int item = blockIdx.x;
+   const int yb = item % nYBlocks;
+   const int zi = (item / nYBlocks) % nz;
+   const int r = item / (nYBlocks * nz);
+   const PsfReqDev& q = reqs[r];
+   const int P = q.size;
+   const int y0 = yb * YB * 16;
+   const size_t wPer = (size_t)Pp * MP;
+   const _Float16* Wr = W + (size_t)r * 4 * wPer;
+   const double* Ar = A + (size_t)r * MP * MP;
+   const double* Br = B + (size_t)r * MP * MP;
+   const double t = tz[(size_t)r * nz + zi];
+   const int lr = lane & 15, lk = 8 * (lane >> 4);
+
+   // ---- GEMM 1: mid[y][m] = sum_n W[y0 + y][n] Pup[n][m] ----
+   float4v accR[PER1], accI[PER1];
+#pragma unroll
+   for (int s = 0; s < PER1; ++s)
+      accR[s] = accI[s] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+   for (int n0 = 0; n0 < MP; n0 += 32)
+   {
+      // the pupil rows [n0, n0 + 32): FP64 phase, wrapped, FP32 sincos, split
+      for (int i = tid; i < 32 * MP; i += 256)
+      {
+         const int k = i / MP, m = i % MP;
+         const size_t o = (size_t)(n0 + k) * MP + m;
+         const double a = Ar[o];
+         float s = 0.0f, c = 0.0f;
+         if (a == a)  // inside the aperture
+            wrapped_sincos(a + t * Br[o], &s, &c);
+         split_store(c, &pup[0][m][k], &pup[1][m][k]);
+         split_store(s, &pup[2][m][k], &pup[3][m][k]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < PER1; ++s)
+      {
+         const int ti = wave + 4 * s;
+         // rows at or beyond Pp (the last block's padding) stay zero: W holds Pp rows only
+         if (ti < T1 && y0 + (ti / CT1) * 16 < Pp)
+         {
+            const int rb = ti / CT1, cb = ti % CT1;
+            const size_t wo = (size_t)(y0 + rb * 16 + lr) * MP + n0 + lk;
+            const half8 arh = ld8(Wr + wo), arl = ld8(Wr + wPer + wo);
+            const half8 aih = ld8(Wr + 2 * wPer + wo), ail = ld8(Wr + 3 * wPer + wo);
+            const int mc = cb * 16 + lr;
+            const half8 brh = ld8(&pup[0][mc][lk]), brl = ld8(&pup[1][mc][lk]);
+            const half8 bih = ld8(&pup[2][mc][lk]), bil = ld8(&pup[3][mc][lk]);
+            accR[s] = mfma3(arh, arl, brh, brl, accR[s]);
+            accR[s] = mfma3(-aih, -ail, bih, bil, accR[s]);
+            accI[s] = mfma3(arh, arl, bih, bil, accI[s]);
+            accI[s] = mfma3(aih, ail, brh, brl, accI[s]);
+         }
+      }
+      __syncthreads();
+   }
+   // mid -> LDS as split halves
+#pragma unroll
+   for (int s = 0; s < PER1; ++s)
+   {
+      const int ti = wave + 4 * s;
+      if (ti < T1)
+      {
+         const int rb = ti / CT1, cb = ti % CT1;
+#pragma unroll
+         for (int i = 0; i < 4; ++i)
+         {
+            const int y = rb * 16 + 4 * (lane >> 4) + i, m = cb * 16 + lr;
+            split_store(accR[s][i], &mid[0][y][m], &mid[1][y][m]);
+            split_store(accI[s][i], &mid[2][y][m], &mid[3][y][m]);
+         }
+      }
+   }
+   __syncthreads();
+
+   // ---- GEMM 2: E[y][x] = sum_m mid[y][m] W[x][m]; |E|^2 out ----
+   const int CT2 = Pp / 16, T2 = YB * CT2;
+   float* plane = out + ((size_t)r * nz + zi) * P * P;
+   for (int ti = wave; ti < T2; ti += 4)
+   {
+      const int rb = ti / CT2, cb = ti % CT2;
+      if (y0 + rb * 16 >= P)
+         continue;
+      float4v eR = {0.0f, 0.0f, 0.0f, 0.0f}, eI = {0.0f, 0.0f, 0.0f, 0.0f};
+      const int yr = rb * 16 + lr;
+      const size_t wx = (size_t)(cb * 16 + lr) * MP + lk;
+#pragma unroll 2
+      for (int m0 = 0; m0 < MP; m0 += 32)
+      {
+         const half8 arh = ld8(&mid[0][yr][m0 + lk]), arl = ld8(&mid[1][yr][m0 + lk]);
+         const half8 aih = ld8(&mid[2][yr][m0 + lk]), ail = ld8(&mid[3][yr][m0 + lk]);
+         const half8 brh = ld8(Wr + wx + m0), brl = ld8(Wr + wPer + wx + m0);
+         const half8 bih = ld8(Wr + 2 * wPer + wx + m0), bil = ld8(Wr + 3 * wPer + wx + m0);
+         eR = mfma3(arh, arl, brh, brl, eR);
+         eR = mfma3(-aih, -ail, bih, bil, eR);
+         eI = mfma3(arh, arl, bih, bil, eI);
+         eI = mfma3(aih, ail, brh, brl, eI);
+      }
+      const int x = cb * 16 + lr;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+      {
+         const int y = y0 + rb * 16 + 4 * (lane >> 4) + i;
+         if (y < P && x < P)
+            plane[(size_t)y * P + x] = eR[i] * eR[i] + eI[i] * eI[i];
+      }
+   }
+}
+
+// Sum-1 normalisation per plane (Sim/ZernikePsf.cpp:401-410): FP64 sum, v = float(v / sum).
+__global__ __launch_bounds__(256) void psf_normalize_kernel(float* __restrict__ planes, int n2)
+{
+   __shared__ double part[256];
+   float* p = planes + (size_t)blockIdx.x * n2;
+   double s = 0.0;
+   for (int i = threadIdx.x; i < n2; i += 256)
+      s += p[i];
+   part[threadIdx.x] = s;
+   __syncthreads();
+   for (int w = 128; w > 0; w >>= 1)
+   {
+      if ((int)threadIdx.x < w)
+         part[threadIdx.x] += part[threadIdx.x + w];
+      __syncthreads();
+   }
+   const double sum = part[0];
+   if (sum > 0.0)
+      for (int i = threadIdx.x; i < n2; i += 256)
+         p[i] = (float)(p[i] / sum);
+}
+
+// ---- host ---------------------------------------------------------------------------------------------------------
+
+// GeometryFor + ZernikePupilSamples (Sim/ZernikePsf.cpp:118-148)
+static void geometry(const iscg_psf_request* r, int* size, int* nz, int* M)
+{
+   const int os = std::max(1, r->oversampling);
+   const int camHalf = std::max(1, r->kernelHalfWidthPx);
+   const int sz = 2 * camHalf * os + 1;
+   const int nzW = std::max(3, r->nz);
+   *size = sz;
+   *nz = (nzW % 2 == 1) ? nzW : nzW + 1;
+   if (r->pupilSamples > 0)
+      *M = std::max(8, std::min(1024, r->pupilSamples));
+   else
+   {
+      const double diagonalUm = 2.0 * std::sqrt(2.0) * camHalf * r->pixelSizeNm / 1000.0;
+      const double m = 4.0 + 2.0 * r->na * diagonalUm / (r->wavelengthNm / 1000.0);
+      const int need = std::max(64, std::min(512, 4 * (int)std::ceil(m / 4.0)));
+      int fftLen = 1;
+      while (fftLen < need + sz - 1)
+         fftLen <<= 1;
+      *M = std::max(need, std::min(512, (fftLen - sz + 1) / 4 * 4));
+   }
+}
+
+static void index_to_nm(int j, int* n, int* m)
+{
+   for (int nn = 0;; ++nn)
+      for (int l = 0; l <= nn; ++l)
+         if (nn * (nn + 1) / 2 + l == j)
+         {
+            *n = nn;
+            *m = -nn + 2 * l;
+            return;
+         }
+}
+
+// Sim/ZernikePsf.cpp:150-197 (the request's SI scalars, the chirp-Z grid) into the device record.
+static PsfReqDev request_dev(const iscg_psf_request* rq, int size, int nz, int M)
+{
+   PsfReqDev d{};
+   const int os = std::max(1, rq->oversampling);
+   const double resLateralM = (rq->pixelSizeNm / os) * 1E-9;
+   d.NA = rq->na;
+   const double lambda = rq->wavelengthNm * 1E-9;
+   d.ni = rq->immersionIndex;
+   d.ns = rq->sampleIndex;
+   d.ti0 = rq->workingDistanceUm * 1E-6;
+   d.depth = rq->sampleDepthNm * 1E-9;
+   d.zStepM = rq->zStepNm * 1E-9;
+   d.k0 = 2.0 * kPi / lambda;
+   d.bMax = std::min(1.0, d.ns / d.NA);
+   d.kMax = d.k0 * d.NA * d.bMax;
+   d.dk = (2.0 * d.kMax) / (M - 4);
+   d.kMin = -std::floor(M / 2.0) * d.dk;
+   d.x0 = -((size - 1) / 2.0) * resLateralM;
+   d.dx = resLateralM;
+   d.focalShift = d.depth * (d.ni / d.ns);
+   d.M = M;
+   d.size = size;
+   d.nz = nz;
+   d.doubleHelix = rq->maskType == 1;
+   d.maskModes = rq->maskModes;
+   d.maskWaist = rq->maskWaist;
+   int nm = 0;
+   for (int j = 0; j < 28; ++j)
+   {
+      if (rq->zernike[j] == 0.0)
+         continue;
+      d.coeff[nm] = rq->zernike[j];
+      index_to_nm(j, &d.n[nm], &d.l[nm]);
+      d.m[nm] = std::abs(d.l[nm]);
+      ++nm;
+   }
+   d.nModes = nm;
+   return d;
+}
+
+template <int MP, int YB>
+static void launch_planes(hipStream_t s, int items, const PsfReqDev* reqs, const double* A, const double* B,
+                          const _Float16* W, const double* tz, int nz, int Pp, int nYB, float* out)
+{
+   hipLaunchKernelGGL((psf_planes_kernel<MP, YB>), dim3(items), dim3(256), 0, s, reqs, A, B, W, tz, nz, Pp, nYB, out);
+}
+
+// Output-row blocks of YB tiles keep the pupil chunk (MP x 40 halves x 4) and the mid rows (YB 16 x (MP + 8) x 4)
+// within ~150 KB of LDS.
+static bool psf_planes_launch(iscg_context* h, int nReq, int nz, int size, int Mp, int Pp, float* out)
+{
+   PsfState& s = *h->psf;
+   auto go = [&](auto mp, auto yb) {
+      constexpr int MP = decltype(mp)::value, YB = decltype(yb)::value;
+      const int nYB = (Pp + YB * 16 - 1) / (YB * 16);
+      launch_planes<MP, YB>(h->stream, nReq * nz * nYB, s.reqs.as<const PsfReqDev>(), s.tabA.as<const double>(),
+                            s.tabB.as<const double>(), s.W.as<const _Float16>(), s.tz.as<const double>(), nz, Pp, nYB,
+                            out);
+   };
+   using I32 = std::integral_constant<int, 32>;
+   switch (Mp)
+   {
+   case 32: go(I32{}, std::integral_constant<int, 5>{}); break;
+   case 64: go(std::integral_constant<int, 64>{}, std::integral_constant<int, 5>{}); break;
+   case 96: go(std::integral_constant<int, 96>{}, std::integral_constant<int, 5>{}); break;
+   case 128: go(std::integral_constant<int, 128>{}, std::integral_constant<int, 5>{}); break;
+   case 160: go(std::integral_constant<int, 160>{}, std::integral_constant<int, 4>{}); break;
+   case 192: go(std::integral_constant<int, 192>{}, std::integral_constant<int, 3>{}); break;
+   case 224: go(std::integral_constant<int, 224>{}, std::integral_constant<int, 2>{}); break;
+   case 256: go(std::integral_constant<int, 256>{}, std::integral_constant<int, 2>{}); break;
+   default: return fail(h, "iscg_psf_planes: unsupported pupil size") == 0;
+   }
+   (void)size;
+   return ok(h, hipGetLastError(), "PSF planes launch");
+}
+
+}  // namespace iscg
+
+using namespace iscg;
+
+extern "C" {
+
+int32_t iscg_psf_geometry(const iscg_psf_request* r, int32_t* size, int32_t* nz, int32_t* pupilSamples)
+{
+   if (!r)
+      return -1;
+   int s, z, m;
+   geometry(r, &s, &z, &m);
+   if (size)
+      *size = s;
+   if (nz)
+      *nz = z;
+   if (pupilSamples)
+      *pupilSamples = m;
+   return 0;
+}
+
+int32_t iscg_psf_planes(iscg_context* h, const iscg_psf_request* reqs, int32_t nReq, int32_t normalize, float* out,
+                        size_t outCap)
+{
+   if (begin(h))
+      return -1;
+   if (!reqs || nReq <= 0 || !out)
+      return fail(h, "iscg_psf_planes: null argument or no request");
+   int size = 0, nz = 0, M = 0;
+   std::vector<PsfReqDev> dev((size_t)nReq);
+   for (int i = 0; i < nReq; ++i)
+   {
+      const iscg_psf_request* r = reqs + i;
+      if (!(r->na > 0.0) || !(r->wavelengthNm > 0.0) || !(r->immersionIndex > 0.0) || !(r->sampleIndex > 0.0) ||
+          !(r->pixelSizeNm > 0.0) || !(r->zStepNm > 0.0))
+         return fail(h, "GibsonLanniZernike PSF: NA, wavelength, refractive indices, pixel size and z step must be > 0.");
+      int s, z, m;
+      geometry(r, &s, &z, &m);
+      if (i == 0)
+      {
+         size = s;
+         nz = z;
+         M = m;
+      }
+      else if (s != size || z != nz || m != M)
+         return fail(h, "iscg_psf_planes: the requests of one call must share the kernel size, plane count and "
+                        "pupil samples");
+      dev[(size_t)i] = request_dev(r, s, z, m);
+   }
+   const int Mp = (M + 31) / 32 * 32;
+   if (Mp > 256)
+      return fail(h, "iscg_psf_planes: pupil samples above 256 are not supported on this path");
+   const int Pp = (size + 15) / 16 * 16;
+   const size_t planeFloats = (size_t)size * size;
+   const size_t need = (size_t)nReq * nz * planeFloats;
+   if (outCap < need)
+      return fail(h, "iscg_psf_planes: output too small (" + std::to_string(need) + " floats needed)");
+   if (!h->psf)
+      h->psf = new PsfState();
+   PsfState& s = *h->psf;
+   // per (request, plane): ti - ti0, as the reference forms it (Sim/ZernikePsf.cpp:238-244)
+   std::vector<double> tz((size_t)nReq * nz);
+   for (int i = 0; i < nReq; ++i)
+      for (int z = 0; z < nz; ++z)
+      {
+         const PsfReqDev& d = dev[(size_t)i];
+         const double ti = (d.ti0 - d.focalShift) + d.zStepM * (z - (nz - 1.0) / 2.0);
+         tz[(size_t)i * nz + z] = ti - d.ti0;
+      }
+   const size_t wHalves = (size_t)nReq * 4 * Pp * Mp;
+   if (!ok(h, s.reqs.reserve(dev.size() * sizeof(PsfReqDev)), "allocation") ||
+       !ok(h, s.tabA.reserve((size_t)nReq * Mp * Mp * sizeof(double)), "allocation") ||
+       !ok(h, s.tabB.reserve((size_t)nReq * Mp * Mp * sizeof(double)), "allocation") ||
+       !ok(h, s.W.reserve(wHalves * sizeof(_Float16)), "allocation") ||
+       !ok(h, s.tz.reserve(tz.size() * sizeof(double)), "allocation"))
+      return -1;
+   if (!ok(h, hipMemcpyAsync(s.reqs.p, dev.data(), dev.size() * sizeof(PsfReqDev), hipMemcpyHostToDevice, h->stream),
+           "copy") ||
+       !ok(h, hipMemcpyAsync(s.tz.p, tz.data(), tz.size() * sizeof(double), hipMemcpyHostToDevice, h->stream), "copy"))
+      return -1;
+   float* dOut = (float*)stage_out(h, 0, out, need * sizeof(float));
+   if (!dOut)
+      return -1;
+   const PsfReqDev* dReq = s.reqs.as<const PsfReqDev>();
+   const size_t nTab = (size_t)nReq * Mp * Mp;
+   {
+   ProfScope ps(h, "psf_tables");
+   hipLaunchKernelGGL(psf_tables_kernel, dim3(grid_for(nTab, 256, 16)), dim3(256), 0, h->stream, dReq, nReq, Mp,
+                      s.tabA.as<double>(), s.tabB.as<double>());
+   }
+   const size_t nW = (size_t)nReq * Pp * Mp;
+   hipLaunchKernelGGL(psf_w_kernel, dim3(grid_for(nW, 256, 16)), dim3(256), 0, h->stream, dReq, nReq, Pp, Mp,
+                      s.W.as<_Float16>());
+   if (!ok(h, hipGetLastError(), "PSF table kernels"))
+      return -1;
+   {
+      ProfScope ps(h, "psf_planes_mfma");
+      if (!psf_planes_launch(h, nReq, nz, size, Mp, Pp, dOut))
+         return -1;
+   }
+   if (normalize)
+   {
+      ProfScope ps(h, "psf_normalize");
+      hipLaunchKernelGGL(psf_normalize_kernel, dim3((unsigned)(nReq * nz)), dim3(256), 0, h->stream, dOut,
+                         (int)planeFloats);
+      if (!ok(h, hipGetLastError(), "PSF normalisation"))
+         return -1;
+   }
+   if (!finish_out(h, out, dOut, need * sizeof(float)))
+      return -1;
+   return 0;
+}
+
+}  // extern "C"

@@ -443,6 +443,7 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
    if (nEmitters)
    {
       const unsigned grid = grid_for(nEmitters, 256, 4);
+      ProfScope ps(h, "splat_setup");
       if (records)
          hipLaunchKernelGGL(setup_records_kernel, dim3(grid), dim3(256), 0, h->stream,
                             (const iscg_splat_emitter*)dEms, nEmitters, kt, s.setup.as<SetupRec>(), s.pos.as<int2>());
@@ -477,6 +478,7 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
    const CamParams cp = cam ? cam_params(cam) : CamParams{};
    const size_t items = (size_t)a.nTiles * nFrames;
    const unsigned grid = (unsigned)(items < 256 * 16 ? items : 256 * 16);
+   ProfScope ps(h, "splat_tiles_noise");
    if (!dAdu)
       hipLaunchKernelGGL((splat_tiles_kernel<false, false>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
    else if (cp.emccd)

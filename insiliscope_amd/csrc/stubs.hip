@@ -2,7 +2,6 @@
 #include "iscg_internal.h"
 
 namespace iscg {
-void destroy_psf(iscg_context* h) { h->psf = nullptr; }
 
 void destroy_roi(iscg_context* h) { h->roi = nullptr; }
 }  // namespace iscg

@@ -422,9 +422,13 @@ static bool forward_xy(iscg_context* h, WfState& s, const float* in, int nzIn, i
 {
    const iscg_wf_desc& d = s.d;
    const int S = (int)s.S;
-   if (!dispatch_n<RowsR2C, 1024>((int)d.NX, h->stream, in, (size_t)nxIn, nzIn * nyIn, nxIn, x1, S,
-                            s.twX.as<const float2>()))
-      return false;
+   {
+      ProfScope ps(h, "wf_rows_r2c");
+      if (!dispatch_n<RowsR2C, 1024>((int)d.NX, h->stream, in, (size_t)nxIn, nzIn * nyIn, nxIn, x1, S,
+                                     s.twX.as<const float2>()))
+         return false;
+   }
+   ProfScope ps(h, "wf_cols_fwd");
    return dispatch_n<ColsF>((int)d.NY, h->stream, false, (const float2*)x1, nyIn, (size_t)nyIn * S, x2, 0,
                             (int)d.NY, (size_t)d.NY * S, nzIn, S, (int)(d.NX / 2 + 1), s.twY.as<const float2>());
 }
@@ -587,11 +591,17 @@ int32_t iscg_wf_conv3d(iscg_context* h, const float* density, const iscg_camera*
    if (!forward_xy(h, s, dIn, (int)d.nzD, (int)d.ny, (int)d.nx, x1, x2))
       return fail(h, "iscg_wf_conv3d: unsupported FFT size");
    // C: x2 -> x1 [nF][NY][S]
+   {
+   ProfScope ps(h, "wf_zconv");
    dispatch_n<ZConv>((int)d.NZ, h->stream, false, (const float2*)x2, (int)d.nzD, x1, (int)d.nF,
                      s.ghat.as<const float2>(), (int)d.NY, S, (int)(d.NX / 2 + 1), 1.0f, s.twZ.as<const float2>());
+   }
    // D: x1 [nF][NY][S] -> x2 [nF][ch][S] (rows fovY0 ..)
+   {
+   ProfScope ps(h, "wf_cols_inv");
    dispatch_n<ColsF>((int)d.NY, h->stream, true, (const float2*)x1, (int)d.NY, (size_t)d.NY * S, x2, (int)d.fovY0,
                      (int)d.ch, (size_t)d.ch * S, (int)d.nF, S, (int)(d.NX / 2 + 1), s.twY.as<const float2>());
+   }
    // E: rows -> images / ADU
    C2rEpilogue e;
    e.img = dImg;
@@ -605,7 +615,11 @@ int32_t iscg_wf_conv3d(iscg_context* h, const float* density, const iscg_camera*
    e.ch = (int)d.ch;
    e.cw = (int)d.cw;
    e.x0 = (int)d.fovX0;
-   dispatch_n<RowsC2R, 1024>((int)d.NX, h->stream, (const float2*)x2, S, (int)(d.nF * d.ch), e, s.twX.as<const float2>());
+   {
+      ProfScope ps(h, "wf_rows_c2r_noise");
+      dispatch_n<RowsC2R, 1024>((int)d.NX, h->stream, (const float2*)x2, S, (int)(d.nF * d.ch), e,
+                                s.twX.as<const float2>());
+   }
    if (!ok(h, hipGetLastError(), "iscg_wf_conv3d launch"))
       return -1;
    if (outImages && !finish_out(h, outImages, dImg, nOut * sizeof(float)))

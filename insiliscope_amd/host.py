@@ -393,6 +393,66 @@ class SplatGpuHost:
         return max(1, (1 << 30) // max(1, 6 * self.width * self.height))
 
 
+# ---- PSF planes --------------------------------------------------------------------------------------------------
+
+def psf_request(fields) -> _lib.PsfRequest:
+    """An iscg_psf_request from a dict of sim::PsfGeneratorRequest's numeric fields (zernike: 28 OSA values or a
+    {index: waves} dict); missing fields take the reference's defaults (Sim/PsfGeneratorBridge.h:91-170)."""
+    if isinstance(fields, _lib.PsfRequest):
+        return fields
+    d = dict(wavelengthNm=660.0, na=1.4, immersionIndex=1.518, sampleIndex=1.518, workingDistanceUm=150.0,
+             sampleDepthNm=0.0, pixelSizeNm=100.0, zStepNm=100.0, maskWaist=1.0, maskType=0, maskModes=5,
+             oversampling=4, kernelHalfWidthPx=8, nz=1, pupilSamples=0, interpMode=0)
+    d.update(fields)
+    z = np.zeros(28)
+    zern = d.pop("zernike", None)
+    if isinstance(zern, dict):
+        for j, v in zern.items():
+            z[j] = v
+    elif zern is not None:
+        z[: len(zern)] = zern
+    r = _lib.PsfRequest(**{k: v for k, v in d.items() if k in dict(_lib.PsfRequest._fields_)})
+    for j in range(28):
+        r.zernike[j] = float(z[j])
+    return r
+
+
+def PsfGeometry(fields):
+    """(size, nz, pupilSamples) of a request (GeometryFor + ZernikePupilSamples, Sim/ZernikePsf.cpp:118-148)."""
+    r = psf_request(fields)
+    s, z, m = C.c_int32(), C.c_int32(), C.c_int32()
+    _lib.lib().iscg_psf_geometry(C.byref(r), C.byref(s), C.byref(z), C.byref(m))
+    return s.value, z.value, m.value
+
+
+def ComputePsfPlanesBatch(requests, normalize: bool = False, out=None, ctx: Context | None = None):
+    """ComputeZernikePsfPlanes (Sim/ZernikePsf.h:51) for many requests sharing size, nz and pupil samples, in one
+    launch: (nReq, nz, size, size) float32 (sum-1 planes when normalize, as BuildZernikePsfKernelCache)."""
+    ctx = ctx or default_context()
+    reqs = [psf_request(r) for r in requests]
+    size, nz, _ = PsfGeometry(reqs[0])
+    arr = (_lib.PsfRequest * len(reqs))(*reqs)
+    if out is None:
+        out = np.zeros((len(reqs), nz, size, size), np.float32)
+    p, keep = _ptr(out)
+    cap = int(np.prod(out.shape)) if hasattr(out, "shape") else 0
+    ctx.check(ctx.L.iscg_psf_planes(ctx.h, arr, len(reqs), int(bool(normalize)), p, cap), "iscg_psf_planes")
+    del keep
+    return out
+
+
+def ComputeZernikePsfPlanes(fields, ctx: Context | None = None):
+    """sim::ComputeZernikePsfPlanes for one request: (nz, size, size) raw |E|^2 planes."""
+    return ComputePsfPlanesBatch([fields], ctx=ctx)[0]
+
+
+def BuildZernikePsfKernelCache(fields, ctx: Context | None = None) -> PsfKernelCache:
+    """sim::BuildZernikePsfKernelCache (Sim/ZernikePsf.cpp:382): the sum-1 planes as a kernel cache."""
+    r = psf_request(fields)
+    planes = ComputePsfPlanesBatch([r], normalize=True, ctx=ctx)[0]
+    return PsfKernelCache(planes, oversampling=max(1, r.oversampling), zStepNm=r.zStepNm, interpMode=r.interpMode)
+
+
 # ---- mean field --------------------------------------------------------------------------------------------------
 
 class WidefieldStack:
@@ -452,5 +512,6 @@ def DefaultBlinkSplatGpu():
 
 
 __all__ = ["Context", "CameraNoiseParams", "ApplyNoiseChain", "ApplyNoiseChainFrames", "PsfKernelCache",
-           "PsfHaloSpans", "SplatGpuHost", "WidefieldStack", "CollectFrameEmitters", "CollectGpuEmitters", "FRAME_EMITTER_DTYPE",
+           "PsfHaloSpans", "SplatGpuHost", "WidefieldStack", "ComputePsfPlanesBatch", "ComputeZernikePsfPlanes",
+           "BuildZernikePsfKernelCache", "PsfGeometry", "psf_request", "CollectFrameEmitters", "CollectGpuEmitters", "FRAME_EMITTER_DTYPE",
            "SPLAT_EMITTER_DTYPE", "BLINK_EVENT_DTYPE", "INTERP", "IscgError"]

@@ -291,6 +291,8 @@ class Reference:
         L.ref_render_noise_frames.restype = I32
         L.ref_psf_library.argtypes = [C.POINTER(PsfRequest), I32, PF]
         L.ref_psf_library.restype = I32
+        L.ref_wf_focus_series.argtypes = [PF, I32, I32, I32, D, PF, I32, I32, I32, D, D, C.POINTER(D), I32, PF]
+        L.ref_wf_focus_series.restype = I32
 
     def psf_planes(self, req, size, nz):
         out = np.zeros((nz, size, size), np.float32)
@@ -390,6 +392,21 @@ class Reference:
                                        pixel_nm, bg, _p(planes, C.c_float), planes.shape[0], planes.shape[1], os_,
                                        z_step_nm, interp, C.byref(cam), _p(offset, C.c_float), seed,
                                        _p(out, C.c_uint16))
+        return out
+
+    def wf_focus_series(self, density, dz_nm, planes, os_, z_step_nm, foci_um, kernel_cap_um=12.8):
+        """WidefieldScene::UpdateFromGrid + FocusSeries on a persistent population, flat illumination over the FOV,
+        unit dose and efficiency: (nF, ny, nx) images."""
+        density = np.ascontiguousarray(density, np.float32)
+        planes = np.ascontiguousarray(planes, np.float32)
+        foci = np.ascontiguousarray(foci_um, np.float64)
+        nz, ny, nx = density.shape
+        out = np.zeros((len(foci), ny, nx), np.float32)
+        r = self.L.ref_wf_focus_series(_p(density, C.c_float), nx, ny, nz, dz_nm, _p(planes, C.c_float),
+                                       planes.shape[0], planes.shape[1], os_, z_step_nm, kernel_cap_um,
+                                       _p(foci, C.c_double), len(foci), _p(out, C.c_float))
+        if r < 0:
+            raise RuntimeError("WidefieldScene failed")
         return out
 
     def psf_library(self, reqs, out=None):

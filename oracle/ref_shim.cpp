@@ -13,6 +13,7 @@
 #include "SMLMZernike.h"
 #include "WidefieldRender.h"
 #include "ZernikePsf.h"
+#include "Illumination.h"
 #include "Parallel.h"
 
 #include <cstdint>
@@ -333,6 +334,69 @@ int32_t ref_psf_library(const orc_psf_request* reqs, int32_t nReq, float* out)
       }
    }
    return 0;
+}
+
+// WidefieldScene::UpdateFromGrid + FocusSeries (Sim/WidefieldRender.h:458,588) for a persistent population on a
+// nx x ny x nzD grid at 100 nm x 100 nm x dzNm, flat illumination over exactly the FOV (no margin), unit dose and
+// collection efficiency (the spec's mean-field convention, Sim/WidefieldRender.h:258-263), every dye plane in every
+// focus (no slab), the kernel radius cap at kernelCapUm. Foci: the nF world heights focusUm. out: nF x ny x nx, the
+// images of the FOV cells (WidefieldImages::persistent). Returns the FFT grid edge the scene chose (>0) or -1.
+int32_t ref_wf_focus_series(const float* density, int32_t nx, int32_t ny, int32_t nzD, double dzNm,
+                            const float* planes, int32_t nzP, int32_t size, int32_t os, double zStepNm,
+                            double kernelCapUm, const double* focusUm, int32_t nF, float* out)
+{
+   sim::PsfKernelCache c = MakeCache(planes, nzP, size, os, zStepNm, 0);
+   sim::KernelWidefieldPsf psf(c, 1);
+   sim::WidefieldSceneSpec spec;
+   spec.pixelUm = 0.1;
+   spec.originXUm = 0.05;   // pixel (0, 0)'s centre: the FOV starts on a cell edge (frac 0)
+   spec.originYUm = 0.05;
+   spec.width = static_cast<unsigned>(nx);
+   spec.height = static_cast<unsigned>(ny);
+   spec.focusWorldUm = focusUm[0];
+   spec.slabHalfUm = 0.0;
+   spec.grid.upscale = 1;
+   spec.grid.zPlaneNm = dzNm;
+   spec.marginUm = 0.0;
+   spec.kernelCapUm = kernelCapUm;
+   spec.eta = 1.0;
+   spec.unitDose = 1.0;
+   sim::FlatIllumination ill(nx * 0.1, ny * 0.1);
+   unsigned fx = 0, fy = 0;
+   double frx = 0, fry = 0;
+   const sim::WidefieldGridSpec g = sim::WidefieldScene::GridSpecFor(ill, spec, &fx, &fy, &frx, &fry);
+   sim::WidefieldDyeGrid G;
+   G.spec = g;
+   G.k0 = 0;
+   G.nz = static_cast<unsigned>(nzD);
+   const size_t gp = static_cast<size_t>(g.nx) * g.ny;
+   G.persistent.assign(gp * nzD, 0.0f);
+   double total = 0.0;
+   for (int z = 0; z < nzD; ++z)
+      for (int y = 0; y < ny; ++y)
+         for (int x = 0; x < nx; ++x)
+         {
+            const float v = density[(static_cast<size_t>(z) * ny + y) * nx + x];
+            G.persistent[z * gp + static_cast<size_t>(fy + y) * g.nx + fx + x] = v;
+            total += v;
+         }
+   G.nPersistent = static_cast<long>(total);
+   sim::WidefieldScene sc;
+   std::string err;
+   if (!sc.UpdateFromGrid(G, ill, spec, psf, err))
+      return -1;
+   std::vector<double> foci(focusUm, focusUm + nF);
+   std::vector<sim::WidefieldImages> images;
+   if (!sc.FocusSeries(foci, images, err))
+      return -1;
+   for (int f = 0; f < nF; ++f)
+   {
+      const sim::WidefieldImages& im = images[static_cast<size_t>(f)];
+      if (im.persistent.size() != static_cast<size_t>(nx) * ny)
+         return -1;
+      std::memcpy(out + static_cast<size_t>(f) * nx * ny, im.persistent.data(), im.persistent.size() * sizeof(float));
+   }
+   return 1;
 }
 
 } // extern "C"

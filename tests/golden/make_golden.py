@@ -13,7 +13,7 @@ Outputs (inputs and expected outputs only):
   noise_ref.npz      ApplyNoiseChain ADU for fixed photon images: sCMOS and EMCCD, with and without maps, pixelBase
   splat_ref.npz      CollectFrameEmitters, CollectGpuEmitters and RenderPhotonImage frames (Nearest, Linear, Cubic,
                      halo cut, edge emitters) on seeded events
-  wf_ref.npz         KernelWidefieldPsf radius and grid-pitch planes
+  wf_ref.npz         KernelWidefieldPsf radius and grid-pitch planes; WidefieldScene::FocusSeries images
 """
 from __future__ import annotations
 
@@ -158,6 +158,13 @@ def wf(R, P):
         r = R.wf_radius(planes, 4, up)
         out[f"radius_u{up}"] = np.array(r)
         out[f"kernel_u{up}"] = np.stack([R.wf_kernel(planes, 4, up, p, r) for p in (0, 4, 8)])
+    # WidefieldScene::FocusSeries of a small persistent population (48 x 40 x 6 planes at 50 nm), 7 foci on the
+    # plane centres and 3 between them
+    rng = np.random.default_rng(3)
+    dens = np.where(rng.random((6, 40, 48)) < 0.05, 1.0 + np.floor(rng.random((6, 40, 48)) * 3), 0.0)
+    out["scene_density"] = dens.astype(np.float32)
+    out["scene_foci"] = np.array([(k + 0.5) * 0.05 for k in range(7)] + [0.01, 0.13, 0.31])
+    out["scene_images"] = R.wf_focus_series(out["scene_density"], 50.0, planes, 4, 50.0, out["scene_foci"])
     np.savez_compressed(os.path.join(HERE, "wf_ref.npz"), **out)
 
 

@@ -1,0 +1,84 @@
+"""PSF planes on the GPU (iscg_psf_planes: FP64 phase, split-FP16 MFMA matrix DFT) against the reference's fixture
+(tests/psf/zernike_ref.bin), the compiled reference's planes at the C1/C3 geometries and the FP64 oracle:
+relative L2 <= 1e-5 per plane."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from insiliscope_amd import configs, host
+from test_oracle import ZREF_CASES, GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # relative L2 per plane, split-precision MFMA path against FP64
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def test_psf_vs_zernike_ref_bin(ctx):
+    b = open(os.path.join(GOLDEN, "zernike_ref.bin"), "rb").read()
+    cases, nx, ny, nz = np.frombuffer(b[4:20], np.int32)
+    ref = np.frombuffer(b[20:], np.float32).reshape(cases, nz, ny, nx)
+    reqs = []
+    for name, ns, depth, zern, mask in ZREF_CASES:
+        reqs.append(dict(wavelengthNm=660.0, na=1.4, immersionIndex=1.518, sampleIndex=ns, workingDistanceUm=150.0,
+                         sampleDepthNm=depth, pixelSizeNm=100.0, oversampling=4, kernelHalfWidthPx=8, nz=3,
+                         zStepNm=100.0, maskModes=5, maskWaist=1.0, pupilSamples=64, zernike=zern, maskType=mask))
+    got = host.ComputePsfPlanesBatch(reqs, ctx=ctx)
+    for c in range(cases):
+        for z in range(nz):
+            e = rel_l2(got[c, z], ref[c, z])
+            print(ZREF_CASES[c][0], z, e, flush=True)
+            assert e <= TOL and np.argmax(got[c, z]) == np.argmax(ref[c, z])
+
+
+def test_psf_c1_all_planes_vs_oracle(ctx, orc, golden):
+    G = golden("psf_ref.npz")
+    c1 = configs.c1_psf(seed=1)
+    got = host.ComputeZernikePsfPlanes(c1, ctx=ctx)
+    want = orc.psf_planes(O.psf_request(**c1))
+    assert got.shape == want.shape == (41, 33, 33)
+    errs = [rel_l2(got[z], want[z]) for z in range(41)]
+    print("C1 max rel L2", max(errs), flush=True)
+    assert max(errs) <= TOL
+    for j, z in enumerate(G["c1_planes_idx"]):
+        assert rel_l2(got[z], G["c1_planes"][j]) <= TOL
+
+
+def test_psf_c3_batch_vs_reference(ctx, golden):
+    G = golden("psf_ref.npz")
+    reqs = configs.c3_requests(seed=1)
+    pick = [int(i) for i in G["c3_pick"]]
+    got = host.ComputePsfPlanesBatch([reqs[i] for i in pick], ctx=ctx)
+    for j in range(len(pick)):
+        for k, z in enumerate((0, 20, 40)):
+            e = rel_l2(got[j, z], G["c3_planes"][j, k])
+            assert e <= TOL, (j, z, e)
+
+
+def test_psf_normalized_and_wide_kernel(ctx, orc):
+    """M 256 and a 129-wide kernel (several output-row blocks per plane); sum-1 planes."""
+    req = dict(wavelengthNm=600.0, na=1.3, immersionIndex=1.518, sampleIndex=1.45, sampleDepthNm=800.0,
+               pixelSizeNm=100.0, oversampling=1, kernelHalfWidthPx=64, nz=5, zStepNm=250.0, pupilSamples=256,
+               zernike={3: 0.04, 7: -0.03, 12: 0.05})
+    got = host.ComputePsfPlanesBatch([req], normalize=True, ctx=ctx)[0]
+    want = orc.psf_planes(O.psf_request(**req), normalize=True)
+    assert got.shape == want.shape == (5, 129, 129)
+    for z in range(5):
+        assert rel_l2(got[z], want[z]) <= TOL
+        assert abs(got[z].astype(np.float64).sum() - 1.0) < 1e-5
+
+
+def test_psf_mixed_batch_rejected(ctx):
+    a = configs.c1_psf(seed=1)
+    b = dict(a, kernelHalfWidthPx=8)
+    with pytest.raises(host.IscgError, match="share"):
+        host.ComputePsfPlanesBatch([a, b], ctx=ctx)
+    with pytest.raises(host.IscgError, match="must be > 0"):
+        host.ComputePsfPlanesBatch([dict(a, na=0.0)], ctx=ctx)

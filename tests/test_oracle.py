@@ -306,6 +306,19 @@ def test_wf_kernel_bit_exact_vs_reference(orc, golden):
             assert np.array_equal(orc.wf_kernel(planes[p], 4, up, R), G[f"kernel_u{up}"][j])
 
 
+def test_wf_direct_vs_reference_scene(orc, golden):
+    """The direct per-focus convolution equals the reference's WidefieldScene::FocusSeries (FP32 FFT re-pairing,
+    cli/widefield_check.cpp's tolerance is 1e-4 max rel; here 1e-6 relative L2), foci on and between plane
+    centres (the linear z blend)."""
+    G, P = golden("wf_ref.npz"), golden("psf_ref.npz")
+    planes = P["small_planes"]
+    R = orc.wf_radius(65, 4, 1)
+    K = np.stack([orc.wf_kernel(p, 4, 1, R) for p in planes])
+    want = orc.wf_direct(G["scene_density"], 0.05, 0, K, 50.0, G["scene_foci"])
+    for f in range(len(G["scene_foci"])):
+        assert rel_l2(G["scene_images"][f], np.maximum(want[f], 0.0)) < 1e-6
+
+
 def test_wf_direct_delta_reproduces_kernel(orc):
     """A single dye on a plane centre, focus at that plane: the image is the kernel plane at zero defocus."""
     rng = np.random.default_rng(1)
