@@ -214,6 +214,38 @@ int32_t iscg_wf_conv3d(iscg_context* h, const float* density, const iscg_camera*
                        double background, uint32_t seed, uint32_t frameBase, uint32_t pixelBase, float* outImages,
                        uint16_t* outAdu);
 
+/* ---- differentiable ROI model (new: no reference counterpart) ---------------------------------------------------- */
+
+/* The optics of the ROIs (the fields of a PsfGeneratorRequest that the pupil uses) and the ROI sampling. */
+typedef struct
+{
+   double wavelengthNm, na, immersionIndex, sampleIndex, sampleDepthNm, pixelSizeNm;
+   int32_t pupilSamples; /* M, even, <= 64 */
+   int32_t roiSize;      /* R pixels per axis; roiSize * oversampling <= 64 */
+   int32_t oversampling; /* sub-samples per pixel per axis (the pixel integral) */
+   int32_t reserved;
+} iscg_roi_model;
+
+/* One emitter: x, y in pixels from the ROI corner (pixel p spans [p, p + 1)), z in nm of defocus (the PSF stack's
+ * plane coordinate: ti - ti0 = z - focal shift, Sim/ZernikePsf.cpp:238-244), photons, background photons/pixel. */
+typedef struct
+{
+   double x, y, z, photons, background;
+} iscg_roi_params;
+
+/* The expected image mu (nRoi x R x R) of each ROI: the reference's Gibson-Lanni + Zernike pupil (zernike: 28 OSA
+ * values in waves) shifted to (x, y), at defocus z, integrated over the pixels, times photons, plus background;
+ * the PSF integrates to 1 over the plane. No reference counterpart (the nearest is the finite-difference Fisher
+ * matrix of DescribePsfCramerRao, Sim/PsfGeneratorBridge.cpp:1554). */
+int32_t iscg_roi_forward(iscg_context* h, const iscg_roi_model* model, const double* zernike,
+                         const iscg_roi_params* params, int32_t nRoi, float* mu);
+
+/* The Poisson negative log-likelihood sum (mu - d log mu) of data d (nRoi x R x R counts) and its gradient: per ROI
+ * d/d(x, y, z, photons, background) (nRoi x 5) and the shared d/d zernike[28], by the adjoint (pupil recomputed). */
+int32_t iscg_roi_backward(iscg_context* h, const iscg_roi_model* model, const double* zernike,
+                          const iscg_roi_params* params, int32_t nRoi, const float* data, double* gradParams,
+                          double* gradZernike, double* nll);
+
 #ifdef __cplusplus
 }
 #endif

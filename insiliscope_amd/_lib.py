@@ -39,6 +39,15 @@ class WfDesc(C.Structure):
                 ("NY", C.c_uint32), ("NZ", C.c_uint32), ("upscale", C.c_int32)]
 
 
+class RoiModel(C.Structure):
+    """iscg_roi_model."""
+
+    _fields_ = [("wavelengthNm", C.c_double), ("na", C.c_double), ("immersionIndex", C.c_double),
+                ("sampleIndex", C.c_double), ("sampleDepthNm", C.c_double), ("pixelSizeNm", C.c_double),
+                ("pupilSamples", C.c_int32), ("roiSize", C.c_int32), ("oversampling", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 class PsfRequest(C.Structure):
     """iscg_psf_request: the numeric fields of sim::PsfGeneratorRequest (Sim/PsfGeneratorBridge.h:91-170)."""
 
@@ -72,6 +81,8 @@ SIGNATURES = {
     "iscg_splat_render_records": (I32, [P, P, U32, P, U32, P, P, P, C.POINTER(Camera), U32, P, P]),
     "iscg_psf_geometry": (I32, [C.POINTER(PsfRequest), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
     "iscg_psf_planes": (I32, [P, C.POINTER(PsfRequest), I32, I32, P, C.c_size_t]),
+    "iscg_roi_forward": (I32, [P, C.POINTER(RoiModel), C.POINTER(D), P, I32, P]),
+    "iscg_roi_backward": (I32, [P, C.POINTER(RoiModel), C.POINTER(D), P, I32, P, P, C.POINTER(D), C.POINTER(D)]),
     "iscg_wf_conv3d_plan": (I32, [P, C.POINTER(WfDesc), P, I32, I32, I32, D, I32]),
     "iscg_wf_conv3d": (I32, [P, P, C.POINTER(Camera), P, D, U32, U32, U32, P, P]),
 }

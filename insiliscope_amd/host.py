@@ -498,6 +498,81 @@ class WidefieldStack:
                                                     pixelBase, ptrs[2][0], ptrs[3][0]))
 
 
+# ---- differentiable ROI model -------------------------------------------------------------------------------------
+
+ROI_PARAMS_DTYPE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("photons", "f8"), ("background", "f8")])
+
+
+def roi_model(**kw) -> _lib.RoiModel:
+    """iscg_roi_model with the C5 optics as defaults (670 nm, NA 1.45, ni 1.518, ns 1.33, 100 nm pixels, M 64,
+    15 x 15 pixel ROIs, 4 x 4 sub-samples per pixel)."""
+    d = dict(wavelengthNm=670.0, na=1.45, immersionIndex=1.518, sampleIndex=1.33, sampleDepthNm=0.0,
+             pixelSizeNm=100.0, pupilSamples=64, roiSize=15, oversampling=4, reserved=0)
+    d.update(kw)
+    return _lib.RoiModel(**d)
+
+
+def _zern(z):
+    a = np.zeros(28)
+    z = np.asarray(z, np.float64).reshape(-1)
+    a[: len(z)] = z
+    return (C.c_double * 28)(*a)
+
+
+def RoiForward(model, zernike, params, out=None, ctx: Context | None = None):
+    """Expected images mu (nRoi, R, R) float32 of emitter ROIs (iscg_roi_forward)."""
+    ctx = ctx or default_context()
+    prm = _arr(params, ROI_PARAMS_DTYPE)
+    n = int(prm.shape[0])
+    if out is None:
+        out = np.zeros((n, model.roiSize, model.roiSize), np.float32)
+    pp, k1 = _ptr(prm)
+    po, k2 = _ptr(out)
+    ctx.check(ctx.L.iscg_roi_forward(ctx.h, C.byref(model), _zern(zernike), pp, n, po), "iscg_roi_forward")
+    return out
+
+
+def RoiBackward(model, zernike, params, data, gradParams=None, ctx: Context | None = None):
+    """Poisson NLL of data and its gradient (iscg_roi_backward): (nll, d/d(x,y,z,N,b) (nRoi, 5), d/d zernike (28))."""
+    ctx = ctx or default_context()
+    prm = _arr(params, ROI_PARAMS_DTYPE)
+    n = int(prm.shape[0])
+    d = _arr(data, np.float32)
+    if gradParams is None:
+        gradParams = np.zeros((n, 5), np.float64)
+    gz = (C.c_double * 28)()
+    nll = C.c_double()
+    pp, k1 = _ptr(prm)
+    pd, k2 = _ptr(d)
+    pg, k3 = _ptr(gradParams)
+    ctx.check(ctx.L.iscg_roi_backward(ctx.h, C.byref(model), _zern(zernike), pp, n, pd, pg, gz, C.byref(nll)),
+              "iscg_roi_backward")
+    return nll.value, gradParams, np.array(gz[:])
+
+
+def PhaseRetrieval(model, params, data, modes, zernike0=None, iterations: int = 200, ctx: Context | None = None):
+    """Zernike phase retrieval: the coefficients of the OSA slots `modes` (waves) that minimise the Poisson NLL of
+    the data (e.g. a bead z-stack: one ROI per plane with its known defocus), by L-BFGS on the adjoint gradient.
+    Returns (zernike[28], final NLL, iterations used)."""
+    from scipy.optimize import minimize
+    ctx = ctx or default_context()
+    z0 = np.zeros(28) if zernike0 is None else np.array(zernike0, np.float64)
+    modes = list(modes)
+    data = np.ascontiguousarray(data, np.float32)
+
+    def f(v):
+        z = z0.copy()
+        z[modes] = v
+        nll, _, gz = RoiBackward(model, z, params, data, ctx=ctx)
+        return nll, gz[modes]
+
+    res = minimize(f, z0[modes], jac=True, method="L-BFGS-B", options={"maxiter": iterations, "gtol": 1e-10,
+                                                                        "ftol": 1e-15})
+    z = z0.copy()
+    z[modes] = res.x
+    return z, float(res.fun), int(res.nit)
+
+
 def SetDefaultBlinkSplatGpu(make):
     """Installs a host factory, as sim::SetDefaultBlinkSplatGpu (Sim/ScopeMovie.h:378) does for the viewer."""
     global _default_splat_factory
@@ -513,5 +588,6 @@ def DefaultBlinkSplatGpu():
 
 __all__ = ["Context", "CameraNoiseParams", "ApplyNoiseChain", "ApplyNoiseChainFrames", "PsfKernelCache",
            "PsfHaloSpans", "SplatGpuHost", "WidefieldStack", "ComputePsfPlanesBatch", "ComputeZernikePsfPlanes",
-           "BuildZernikePsfKernelCache", "PsfGeometry", "psf_request", "CollectFrameEmitters", "CollectGpuEmitters", "FRAME_EMITTER_DTYPE",
+           "BuildZernikePsfKernelCache", "PsfGeometry", "psf_request", "ROI_PARAMS_DTYPE", "roi_model", "RoiForward",
+           "RoiBackward", "PhaseRetrieval", "CollectFrameEmitters", "CollectGpuEmitters", "FRAME_EMITTER_DTYPE",
            "SPLAT_EMITTER_DTYPE", "BLINK_EVENT_DTYPE", "INTERP", "IscgError"]

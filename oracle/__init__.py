@@ -47,6 +47,16 @@ class PsfRequest(C.Structure):
                 ("pupilSamples", C.c_int32), ("interpMode", C.c_int32), ("reserved", C.c_int32)]
 
 
+class RoiModel(C.Structure):
+    _fields_ = [("wavelengthNm", C.c_double), ("na", C.c_double), ("immersionIndex", C.c_double),
+                ("sampleIndex", C.c_double), ("sampleDepthNm", C.c_double), ("pixelSizeNm", C.c_double),
+                ("pupilSamples", C.c_int32), ("roiSize", C.c_int32), ("oversampling", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+ROI_PARAMS_DTYPE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("photons", "f8"), ("background", "f8")])
+
+
 class Emitter(C.Structure):
     _fields_ = [("xPx", C.c_double), ("yPx", C.c_double), ("photons", C.c_double),
                 ("zIndex", C.c_int32), ("kernel", C.c_int32)]
@@ -135,6 +145,9 @@ class Oracle:
         L.orc_wf_radius.restype = I32
         L.orc_wf_kernel.argtypes = [PF, I32, I32, I32, I32, PF]
         L.orc_wf_direct.argtypes = [PF, I32, I32, I32, D, I64, PF, I32, I32, D, PD, I32, PD]
+        L.orc_roi_forward.argtypes = [C.POINTER(RoiModel), PD, C.c_void_p, I32, PD]
+        L.orc_roi_backward.argtypes = [C.POINTER(RoiModel), PD, C.c_void_p, I32, PD, PD, PD]
+        L.orc_roi_backward.restype = D
 
     # -- RNG --
     def pcg4d(self, a, b, c, d):
@@ -255,6 +268,35 @@ class Oracle:
         self.L.orc_wf_direct(_p(density, C.c_float), nx, ny, nzD, dz_um, k0, _p(kernels, C.c_float), nzP,
                              (D - 1) // 2, z_step_nm, _p(foci, C.c_double), len(foci), _p(out, C.c_double))
         return out
+
+
+    # -- ROI model and adjoint --
+    def roi_forward(self, model, zernike, params):
+        params = np.ascontiguousarray(params, ROI_PARAMS_DTYPE)
+        z = np.ascontiguousarray(zernike, np.float64)
+        R = model.roiSize
+        mu = np.zeros((len(params), R, R))
+        self.L.orc_roi_forward(C.byref(model), _p(z, C.c_double), params.ctypes.data, len(params),
+                               _p(mu, C.c_double))
+        return mu
+
+    def roi_backward(self, model, zernike, params, data):
+        params = np.ascontiguousarray(params, ROI_PARAMS_DTYPE)
+        z = np.ascontiguousarray(zernike, np.float64)
+        d = np.ascontiguousarray(data, np.float64)
+        gp = np.zeros((len(params), 5))
+        gz = np.zeros(28)
+        nll = self.L.orc_roi_backward(C.byref(model), _p(z, C.c_double), params.ctypes.data, len(params),
+                                      _p(d, C.c_double), _p(gp, C.c_double), _p(gz, C.c_double))
+        return nll, gp, gz
+
+
+def roi_model(**kw) -> RoiModel:
+    m = RoiModel(wavelengthNm=670.0, na=1.45, immersionIndex=1.518, sampleIndex=1.33, sampleDepthNm=0.0,
+                 pixelSizeNm=100.0, pupilSamples=64, roiSize=15, oversampling=4, reserved=0)
+    for k, v in kw.items():
+        setattr(m, k, v)
+    return m
 
 
 class Reference:

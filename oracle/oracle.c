@@ -1007,3 +1007,310 @@ void orc_wf_direct(const float* density, int32_t nx, int32_t ny, int32_t nzD, do
       }
    }
 }
+
+/* ======================================================================= */
+/* ROI model and its adjoint (FP64)                                         */
+/* ======================================================================= */
+
+typedef struct
+{
+   int M, R, os, Ps, nIn;
+   double k0, NA, ni, ns, bMax, kMax, dk, kMin, pix, focalShift, norm;
+   double *A, *B, *Z; /* base phase, z slope (per metre), Zernike basis [28][M*M] */
+   char* inside;
+   double *kx;        /* k_m */
+   double *wr, *wi;   /* W[s][m] = exp(i k_m x_s), x_s = (s + 0.5) / os pixels */
+} roi_tables;
+
+static void roi_tables_init(roi_tables* t, const orc_roi_model* m, const double* zernike)
+{
+   t->M = m->pupilSamples;
+   t->R = m->roiSize;
+   t->os = m->oversampling;
+   t->Ps = t->R * t->os;
+   const int M = t->M;
+   const double lambda = m->wavelengthNm * 1e-9;
+   t->NA = m->na;
+   t->ni = m->immersionIndex;
+   t->ns = m->sampleIndex;
+   t->k0 = 2.0 * kPi / lambda;
+   t->bMax = (t->ns / t->NA < 1.0) ? t->ns / t->NA : 1.0;
+   t->kMax = t->k0 * t->NA * t->bMax;
+   t->dk = (2.0 * t->kMax) / (M - 4);
+   t->kMin = -floor(M / 2.0) * t->dk;
+   t->pix = m->pixelSizeNm * 1e-9;
+   const double depth = m->sampleDepthNm * 1e-9;
+   t->focalShift = depth * (t->ni / t->ns);
+   const size_t MM = (size_t)M * M;
+   t->A = (double*)calloc(MM, sizeof(double));
+   t->B = (double*)calloc(MM, sizeof(double));
+   t->Z = (double*)calloc(28 * MM, sizeof(double));
+   t->inside = (char*)calloc(MM, 1);
+   t->kx = (double*)malloc(sizeof(double) * (size_t)M);
+   const int c0 = M / 2;
+   t->nIn = 0;
+   for (int j = 0; j < M; ++j)
+      t->kx[j] = t->kMin + j * t->dk;
+   for (int iy = 0; iy < M; ++iy)
+      for (int ix = 0; ix < M; ++ix)
+      {
+         const double ky = (iy - c0) * t->dk, kx = (ix - c0) * t->dk;
+         const double kr2 = kx * kx + ky * ky;
+         if (kr2 > t->kMax * t->kMax)
+            continue;
+         const size_t idx = (size_t)iy * M + ix;
+         t->inside[idx] = 1;
+         ++t->nIn;
+         const double rho = sqrt(kr2) / (t->k0 * t->NA);
+         const double phi = atan2(ky, kx);
+         const double s1 = t->NA * rho / t->ns, s3 = t->NA * rho / t->ni;
+         const double a1 = 1.0 - s1 * s1, a3 = 1.0 - s3 * s3;
+         const double opd1 = t->ns * depth * sqrt(a1 > 0.0 ? a1 : 0.0);
+         const double opd3 = sqrt(a3 > 0.0 ? a3 : 0.0);
+         double zp = 0.0;
+         for (int j = 0; j < 28; ++j)
+         {
+            int n, l;
+            index_to_nm(j, &n, &l);
+            const double zv = zernike_value(n, l, abs(l), rho / t->bMax, phi);
+            t->Z[(size_t)j * MM + idx] = zv;
+            zp += zernike[j] * zv;
+         }
+         t->A[idx] = t->k0 * opd1 + 2.0 * kPi * zp;
+         t->B[idx] = t->k0 * t->ni * opd3;
+      }
+   t->wr = (double*)malloc(sizeof(double) * (size_t)t->Ps * M);
+   t->wi = (double*)malloc(sizeof(double) * (size_t)t->Ps * M);
+   for (int s = 0; s < t->Ps; ++s)
+   {
+      const double xs = (s + 0.5) / t->os * t->pix;
+      for (int j = 0; j < M; ++j)
+      {
+         t->wr[(size_t)s * M + j] = cos(t->kx[j] * xs);
+         t->wi[(size_t)s * M + j] = sin(t->kx[j] * xs);
+      }
+   }
+   const double q = (t->pix / t->os) * t->dk / (2.0 * kPi);
+   t->norm = q * q / t->nIn;
+}
+
+static void roi_tables_free(roi_tables* t)
+{
+   free(t->A);
+   free(t->B);
+   free(t->Z);
+   free(t->inside);
+   free(t->kx);
+   free(t->wr);
+   free(t->wi);
+}
+
+/* The shifted pupil P'_mn = P_mn(z) exp(-i (k_m x + k_n y)) of one ROI (re, im: M*M). */
+static void roi_pupil(const roi_tables* t, const orc_roi_params* p, double* pr, double* pi)
+{
+   const int M = t->M;
+   const double tz = -t->focalShift + p->z * 1e-9;
+   for (int n = 0; n < M; ++n)
+      for (int m = 0; m < M; ++m)
+      {
+         const size_t idx = (size_t)n * M + m;
+         if (!t->inside[idx])
+         {
+            pr[idx] = pi[idx] = 0.0;
+            continue;
+         }
+         const double ph = t->A[idx] + tz * t->B[idx] - (t->kx[m] * p->x + t->kx[n] * p->y) * t->pix;
+         pr[idx] = cos(ph);
+         pi[idx] = sin(ph);
+      }
+}
+
+/* E[t][s] = sum_n W[t][n] sum_m P'[n][m] W[s][m] (separable sums). */
+static void roi_field(const roi_tables* t, const double* pr, const double* pi, double* er, double* ei, double* tr,
+                      double* ti)
+{
+   const int M = t->M, Ps = t->Ps;
+   for (int n = 0; n < M; ++n)
+      for (int s = 0; s < Ps; ++s)
+      {
+         double ar = 0.0, ai = 0.0;
+         for (int m = 0; m < M; ++m)
+         {
+            const double a = pr[(size_t)n * M + m], b = pi[(size_t)n * M + m];
+            const double c = t->wr[(size_t)s * M + m], d = t->wi[(size_t)s * M + m];
+            ar += a * c - b * d;
+            ai += a * d + b * c;
+         }
+         tr[(size_t)n * Ps + s] = ar;
+         ti[(size_t)n * Ps + s] = ai;
+      }
+   for (int y = 0; y < Ps; ++y)
+      for (int s = 0; s < Ps; ++s)
+      {
+         double ar = 0.0, ai = 0.0;
+         for (int n = 0; n < M; ++n)
+         {
+            const double a = t->wr[(size_t)y * M + n], b = t->wi[(size_t)y * M + n];
+            const double c = tr[(size_t)n * Ps + s], d = ti[(size_t)n * Ps + s];
+            ar += a * c - b * d;
+            ai += a * d + b * c;
+         }
+         er[(size_t)y * Ps + s] = ar;
+         ei[(size_t)y * Ps + s] = ai;
+      }
+}
+
+void orc_roi_forward(const orc_roi_model* m, const double* zernike, const orc_roi_params* p, int32_t nRoi,
+                     double* mu)
+{
+   roi_tables t;
+   roi_tables_init(&t, m, zernike);
+   const int M = t.M, Ps = t.Ps, R = t.R, os = t.os;
+   double* pr = (double*)malloc(sizeof(double) * (size_t)M * M);
+   double* pi = (double*)malloc(sizeof(double) * (size_t)M * M);
+   double* tr = (double*)malloc(sizeof(double) * (size_t)M * Ps);
+   double* ti = (double*)malloc(sizeof(double) * (size_t)M * Ps);
+   double* er = (double*)malloc(sizeof(double) * (size_t)Ps * Ps);
+   double* ei = (double*)malloc(sizeof(double) * (size_t)Ps * Ps);
+   for (int32_t r = 0; r < nRoi; ++r)
+   {
+      roi_pupil(&t, p + r, pr, pi);
+      roi_field(&t, pr, pi, er, ei, tr, ti);
+      double* out = mu + (size_t)r * R * R;
+      for (int py = 0; py < R; ++py)
+         for (int px = 0; px < R; ++px)
+         {
+            double acc = 0.0;
+            for (int sy = 0; sy < os; ++sy)
+               for (int sx = 0; sx < os; ++sx)
+               {
+                  const size_t o = (size_t)(py * os + sy) * Ps + px * os + sx;
+                  acc += er[o] * er[o] + ei[o] * ei[o];
+               }
+            out[py * R + px] = p[r].photons * t.norm * acc + p[r].background;
+         }
+   }
+   free(pr);
+   free(pi);
+   free(tr);
+   free(ti);
+   free(er);
+   free(ei);
+   roi_tables_free(&t);
+}
+
+double orc_roi_backward(const orc_roi_model* m, const double* zernike, const orc_roi_params* p, int32_t nRoi,
+                        const double* data, double* gradParams, double* gradZernike)
+{
+   roi_tables t;
+   roi_tables_init(&t, m, zernike);
+   const int M = t.M, Ps = t.Ps, R = t.R, os = t.os;
+   const size_t MM = (size_t)M * M;
+   double* pr = (double*)malloc(sizeof(double) * MM);
+   double* pi = (double*)malloc(sizeof(double) * MM);
+   double* tr = (double*)malloc(sizeof(double) * (size_t)M * Ps);
+   double* ti = (double*)malloc(sizeof(double) * (size_t)M * Ps);
+   double* er = (double*)malloc(sizeof(double) * (size_t)Ps * Ps);
+   double* ei = (double*)malloc(sizeof(double) * (size_t)Ps * Ps);
+   double* w = (double*)malloc(sizeof(double) * (size_t)R * R);
+   double* psf = (double*)malloc(sizeof(double) * (size_t)R * R);
+   double* br = (double*)malloc(sizeof(double) * MM);
+   double* bi = (double*)malloc(sizeof(double) * MM);
+   double nll = 0.0;
+   for (int j = 0; j < 28; ++j)
+      gradZernike[j] = 0.0;
+   for (int32_t r = 0; r < nRoi; ++r)
+   {
+      const orc_roi_params* q = p + r;
+      roi_pupil(&t, q, pr, pi);
+      roi_field(&t, pr, pi, er, ei, tr, ti);
+      const double* d = data + (size_t)r * R * R;
+      double gN = 0.0, gb = 0.0;
+      for (int py = 0; py < R; ++py)
+         for (int px = 0; px < R; ++px)
+         {
+            double acc = 0.0;
+            for (int sy = 0; sy < os; ++sy)
+               for (int sx = 0; sx < os; ++sx)
+               {
+                  const size_t o = (size_t)(py * os + sy) * Ps + px * os + sx;
+                  acc += er[o] * er[o] + ei[o] * ei[o];
+               }
+            const double f = t.norm * acc;
+            const double mu = q->photons * f + q->background;
+            const double dv = d[py * R + px];
+            nll += mu - dv * log(mu);
+            const double wv = 1.0 - dv / mu;
+            w[py * R + px] = wv;
+            psf[py * R + px] = f;
+            gN += wv * f;
+            gb += wv;
+         }
+      /* G_s = 2 N norm w_p(s) E_s, into er/ei */
+      for (int y = 0; y < Ps; ++y)
+         for (int s = 0; s < Ps; ++s)
+         {
+            const size_t o = (size_t)y * Ps + s;
+            const double g = 2.0 * q->photons * t.norm * w[(y / os) * R + s / os];
+            er[o] *= g;
+            ei[o] *= g;
+         }
+      /* U[n][s] = sum_y conj(W[y][n]) G[y][s] -> tr/ti */
+      for (int n = 0; n < M; ++n)
+         for (int s = 0; s < Ps; ++s)
+         {
+            double ar = 0.0, ai = 0.0;
+            for (int y = 0; y < Ps; ++y)
+            {
+               const double a = t.wr[(size_t)y * M + n], b = -t.wi[(size_t)y * M + n];
+               const double c = er[(size_t)y * Ps + s], dd = ei[(size_t)y * Ps + s];
+               ar += a * c - b * dd;
+               ai += a * dd + b * c;
+            }
+            tr[(size_t)n * Ps + s] = ar;
+            ti[(size_t)n * Ps + s] = ai;
+         }
+      /* B[n][m] = sum_s U[n][s] conj(W[s][m]) */
+      double gx = 0.0, gy = 0.0, gz = 0.0;
+      for (int n = 0; n < M; ++n)
+         for (int mm = 0; mm < M; ++mm)
+         {
+            const size_t idx = (size_t)n * M + mm;
+            if (!t.inside[idx])
+               continue;
+            double ar = 0.0, ai = 0.0;
+            for (int s = 0; s < Ps; ++s)
+            {
+               const double a = tr[(size_t)n * Ps + s], b = ti[(size_t)n * Ps + s];
+               const double c = t.wr[(size_t)s * M + mm], dd = -t.wi[(size_t)s * M + mm];
+               ar += a * c - b * dd;
+               ai += a * dd + b * c;
+            }
+            /* dNLL/dphi = -Im(P' conj(B)) */
+            const double g = -(pi[idx] * ar - pr[idx] * ai);
+            gx += g * (-t.kx[mm]) * t.pix;
+            gy += g * (-t.kx[n]) * t.pix;
+            gz += g * t.B[idx] * 1e-9;
+            for (int j = 0; j < 28; ++j)
+               gradZernike[j] += g * 2.0 * kPi * t.Z[(size_t)j * MM + idx];
+         }
+      double* gp = gradParams + (size_t)r * 5;
+      gp[0] = gx;
+      gp[1] = gy;
+      gp[2] = gz;
+      gp[3] = gN;
+      gp[4] = gb;
+   }
+   free(pr);
+   free(pi);
+   free(tr);
+   free(ti);
+   free(er);
+   free(ei);
+   free(w);
+   free(psf);
+   free(br);
+   free(bi);
+   roi_tables_free(&t);
+   return nll;
+}

@@ -122,6 +122,33 @@ void orc_wf_direct(const float* density, int32_t nx, int32_t ny, int32_t nzD, do
                    const float* kernels, int32_t nzP, int32_t R, double zStepNm, const double* focusUm, int32_t nF,
                    double* out);
 
+/* ---- ROI forward model and Poisson-NLL gradient (new capability; no reference counterpart) ---- */
+/* The reference's pupil (Sim/ZernikePsf.cpp:163-248) evaluated at an emitter: E(x_s) = sum_mn P_mn(z)
+ * exp(i(k_m (x_s - x) + k_n (y_s - y))) on os x os sub-samples of an R x R pixel ROI, mu = N norm sum_sub |E|^2 + b,
+ * norm = (dxs dk / 2 pi)^2 / n_inside (the PSF's integral over one period is 1, Parseval). */
+typedef struct
+{
+   double wavelengthNm, na, immersionIndex, sampleIndex, sampleDepthNm, pixelSizeNm;
+   int32_t pupilSamples; /* M */
+   int32_t roiSize;      /* R pixels per axis */
+   int32_t oversampling; /* sub-samples per pixel per axis */
+   int32_t reserved;
+} orc_roi_model;
+
+typedef struct
+{
+   double x, y; /* pixels from the ROI corner (pixel p spans [p, p + 1)) */
+   double z;    /* nm of defocus (the PSF stack's plane coordinate) */
+   double photons, background;
+} orc_roi_params;
+
+/* mu: nRoi x R x R. */
+void orc_roi_forward(const orc_roi_model* m, const double* zernike, const orc_roi_params* p, int32_t nRoi,
+                     double* mu);
+/* NLL = sum (mu - d log mu); gradients d NLL / d (x, y, z, N, b) per ROI (nRoi x 5) and d NLL / d zernike[28]. */
+double orc_roi_backward(const orc_roi_model* m, const double* zernike, const orc_roi_params* p, int32_t nRoi,
+                        const double* data, double* gradParams, double* gradZernike);
+
 #ifdef __cplusplus
 }
 #endif

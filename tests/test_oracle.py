@@ -374,3 +374,65 @@ def test_planted_mistake_fails_a_pin(orc):
     w, h = (int(x) for x in G["w_h"])
     img = orc.render_frame(w, h, 20.0, em, bs, planes.shape[1], 4, 2)
     assert not np.array_equal(img, G["frame_cubic_0"])
+
+
+# ---- ROI model and its adjoint (new capability: pinned by central finite differences) ----------------------------
+
+def roi_case(seed=1, n=4):
+    rng = np.random.default_rng(seed)
+    z = np.zeros(28)
+    for noll in range(4, 25):
+        z[configs.NOLL_TO_OSA[noll]] = rng.normal(0.0, 0.03)
+    p = np.zeros(n, O.ROI_PARAMS_DTYPE)
+    p["x"] = 7.5 + rng.uniform(-0.5, 0.5, n)
+    p["y"] = 7.5 + rng.uniform(-0.5, 0.5, n)
+    p["z"] = rng.uniform(-500, 500, n)
+    p["photons"] = 2000.0 * np.exp(0.5 * rng.standard_normal(n))
+    p["background"] = rng.uniform(5, 50, n)
+    return z, p, rng
+
+
+def test_roi_adjoint_vs_central_differences(orc):
+    """The FP64 adjoint gradient equals central finite differences of the FP64 forward model (h = 1e-3 in x, y (px),
+    z (nm); 1e-3 relative in photons and background; 1e-4 waves for the Zernike coefficients): <= 1e-4 relative."""
+    m = O.roi_model()
+    z, p, rng = roi_case()
+    mu = orc.roi_forward(m, z, p)
+    # data away from the model (the gradient's terms do not cancel near an optimum)
+    d = rng.poisson(1.2 * mu + 3.0).astype(np.float64)
+    nll, gp, gz = orc.roi_backward(m, z, p, d)
+
+    def NLL(pp, zz):
+        mm = orc.roi_forward(m, zz, pp)
+        return float((mm - d * np.log(mm)).sum())
+
+    assert abs(nll - NLL(p, z)) < 1e-9 * abs(nll)
+    steps = {"x": 1e-3, "y": 1e-3, "z": 1e-3}
+    for i in range(len(p)):
+        for k, name in enumerate(["x", "y", "z", "photons", "background"]):
+            h = steps.get(name, 1e-3 * abs(p[name][i]))
+            a, b = p.copy(), p.copy()
+            a[name][i] += h
+            b[name][i] -= h
+            fd = (NLL(a, z) - NLL(b, z)) / (2 * h)
+            assert abs(gp[i, k] - fd) <= 1e-4 * abs(fd) + 1e-9, (i, name, gp[i, k], fd)
+    for j in (3, 4, 5, 7, 12, 16, 24, 25):
+        h = 1e-4
+        a, b = z.copy(), z.copy()
+        a[j] += h
+        b[j] -= h
+        fd = (NLL(p, a) - NLL(p, b)) / (2 * h)
+        assert abs(gz[j] - fd) <= 1e-4 * abs(fd) + 1e-9, (j, gz[j], fd)
+
+
+def test_roi_forward_normalisation_and_shift(orc):
+    """A centred in-focus emitter's PSF integrates to ~1 over a wide ROI; shifting the emitter by whole pixels shifts
+    the image exactly (the pupil phase ramp)."""
+    m = O.roi_model(roiSize=16, oversampling=2, pupilSamples=64)
+    p = np.zeros(2, O.ROI_PARAMS_DTYPE)
+    p["x"] = [8.0, 9.0]
+    p["y"] = [8.0, 8.0]
+    p["photons"] = 1.0
+    mu = orc.roi_forward(m, np.zeros(28), p)
+    assert 0.85 < mu[0].sum() < 1.0
+    assert np.allclose(mu[1][:, 1:], mu[0][:, :-1], rtol=1e-9, atol=1e-15)
