@@ -46,10 +46,12 @@ class Timer:
     def __init__(self, ctx):
         import torch
         self.torch = torch
-        self.stream = torch.cuda.ExternalStream(ctx.stream)
         self.ctx = ctx
 
     def __enter__(self):
+        # the wrappers enqueue on torch's current stream (Context.follow_torch): time that stream
+        self.ctx.follow_torch()
+        self.stream = self.torch.cuda.current_stream(self.ctx.device)
         self.ctx.synchronize()
         self.a = self.torch.cuda.Event(enable_timing=True)
         self.b = self.torch.cuda.Event(enable_timing=True)

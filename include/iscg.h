@@ -42,6 +42,11 @@ const char* iscg_last_error(const iscg_context* h);
 int32_t iscg_synchronize(iscg_context* h);
 /* The context's hipStream_t (as void*), for callers that time or order work with HIP events. */
 void* iscg_stream(iscg_context* h);
+/* Enqueues the context's later work on `stream`, a hipStream_t of the context's device (NULL: the null stream), so
+ * that a caller's own stream orders it with the caller's kernels; the current stream is synchronized first. The
+ * stream stays the caller's: iscg_destroy does not destroy it. iscg_stream() before any call returns the context's
+ * own stream, which may be passed back. No reference counterpart (the reference's GPU path owns its device queue). */
+int32_t iscg_set_stream(iscg_context* h, void* stream);
 /* Test hook: the next `count` entry-point calls on h fail with `message` before doing any work, as a device error
  * would (a caller's CPU fallback, Sim/ScopeMovie.cpp:7634-7645, is exercised without faulting a GPU). */
 int32_t iscg_inject_failure(iscg_context* h, int32_t count, const char* message);
@@ -67,6 +72,18 @@ typedef struct
    double emGain;
    double cicElectrons;
 } iscg_camera;
+
+/* The evaluation path of the camera noise (iscg_noise, the splat and mean-field noise epilogues). No reference
+ * counterpart: both paths give the ADU of sim::ApplyNoiseChain. exact = 1 evaluates every pixel with the FP64 chain;
+ * exact = 0 (the default) evaluates the Box-Muller normal and the Poisson threshold exp(-l) in FP32, takes each
+ * decision (a count, the final rounding) only where it holds over twice their error bound, and evaluates the other
+ * pixels with the FP64 chain. The bounds are verified on the device once per process, over every input the FP32
+ * functions can receive; a device that fails the check uses the FP64 chain. */
+int32_t iscg_noise_set_path(iscg_context* h, int32_t exact);
+
+/* out[0] = 1 when h uses the FP32 path, else 0; out[1] = the verified bound on |g_fp32 - g| of one normal;
+ * out[2] = the verified relative bound of the FP32 threshold. Runs the verification if it has not run. */
+int32_t iscg_noise_bounds(iscg_context* h, double* out);
 
 /* Replaces sim::ApplyNoiseChain (Sim/SMLMNoise.h:136, Sim/SMLMNoise.cpp:142-216), batched over frames.
  * photons: nFrames x height x width floats; frame k's stream is (seed, frames[k], pixel + pixelBase).

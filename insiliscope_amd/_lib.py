@@ -74,6 +74,9 @@ SIGNATURES = {
     "iscg_inject_failure": (I32, [P, I32, C.c_char_p]),
     "iscg_profile_enable": (I32, [P, I32]),
     "iscg_profile_read": (I32, [P, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), I32, I32]),
+    "iscg_set_stream": (I32, [P, P]),
+    "iscg_noise_set_path": (I32, [P, I32]),
+    "iscg_noise_bounds": (I32, [P, P]),
     "iscg_noise": (I32, [P, P, U32, U32, C.POINTER(Camera), P, P, P, U32, P, U32, U32, P]),
     "iscg_splat_set_kernels": (I32, [P, C.POINTER(KernelView), I32]),
     "iscg_splat_set_static": (I32, [P, U32, U32, U32, P, P, P, P, D, U64]),

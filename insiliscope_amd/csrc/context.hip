@@ -167,7 +167,8 @@ int32_t iscg_create(int32_t device, iscg_context** out)
    }
    iscg_context* h = new iscg_context();
    h->device = device;
-   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+   e = hipStreamCreateWithFlags(&h->ownStream, hipStreamNonBlocking);
+   h->stream = h->ownStream;
    if (e != hipSuccess)
    {
       g_createError = std::string("hipStreamCreate: ") + hipGetErrorString(e);
@@ -185,6 +186,7 @@ void iscg_destroy(iscg_context* h)
       return;
    (void)hipSetDevice(h->device);
    (void)hipStreamSynchronize(h->stream);
+   (void)hipStreamSynchronize(h->ownStream);
    destroy_splat(h);
    destroy_psf(h);
    destroy_wf(h);
@@ -198,7 +200,7 @@ void iscg_destroy(iscg_context* h)
    }
    for (auto e : h->eventPool)
       (void)hipEventDestroy(e);
-   (void)hipStreamDestroy(h->stream);
+   (void)hipStreamDestroy(h->ownStream);
    delete h;
 }
 
@@ -214,6 +216,20 @@ int32_t iscg_synchronize(iscg_context* h)
 }
 
 void* iscg_stream(iscg_context* h) { return h ? (void*)h->stream : nullptr; }
+
+int32_t iscg_set_stream(iscg_context* h, void* stream)
+{
+   if (begin(h) != 0)
+      return -1;
+   hipStream_t s = (hipStream_t)stream;
+   if (s == h->stream)
+      return 0;
+   // work already enqueued (uploads of kernels, plans, maps) completes before the new stream uses it
+   if (!ok(h, hipStreamSynchronize(h->stream), "iscg_set_stream: synchronize"))
+      return -1;
+   h->stream = s;
+   return 0;
+}
 
 int32_t iscg_profile_enable(iscg_context* h, int32_t on)
 {

@@ -65,7 +65,8 @@ struct RoiState;
 struct iscg_context
 {
    int device = 0;
-   hipStream_t stream = nullptr;
+   hipStream_t stream = nullptr;     // where the context's work is enqueued (iscg_set_stream)
+   hipStream_t ownStream = nullptr;  // the stream iscg_create made
    std::string err;
    // per-kernel timing (iscg_profile_*): event pairs around each launch on the context's stream
    struct ProfPending
@@ -85,6 +86,7 @@ struct iscg_context
    iscg::PsfState* psf = nullptr;
    iscg::WfState* wf = nullptr;
    iscg::RoiState* roi = nullptr;
+   int noiseExact = 0;  // iscg_noise_set_path: 1 evaluates every pixel with the FP64 chain
 };
 
 namespace iscg {
@@ -210,6 +212,8 @@ struct CamParams
 {
    double qe, dark, gain, read, emGain, cic, maxAdu;
    int emccd;
+   int fast;  // 1: noise_pixel_fast
+
 };
 
 inline CamParams cam_params(const iscg_camera* c)
@@ -224,6 +228,7 @@ inline CamParams cam_params(const iscg_camera* c)
    const int bits = c->bitDepth < 1 ? 1 : (c->bitDepth > 16 ? 16 : c->bitDepth);
    p.maxAdu = (double)((1u << bits) - 1u);  // ldexp(1, bits) - 1, exact
    p.emccd = c->emccd != 0;
+   p.fast = 0;
    return p;
 }
 
@@ -268,6 +273,129 @@ struct ProfScope
    ProfScope(iscg_context* hh, const char* nm);
    ~ProfScope();
 };
+
+// ---- filtered fast path -----------------------------------------------------------------------------------------
+// The same chain with the Box-Muller and Knuth-threshold transcendentals in FP32, carrying a bound on their error;
+// every decision that depends on them (a count boundary, the final rounding) is taken only when it holds across the
+// whole error interval, else the pixel is recomputed by noise_pixel, the FP64 chain above. The draws consumed are the
+// same (the same branches are taken), so the ADU are those of noise_pixel.
+//
+// The bounds are measured, not assumed: noise_verify_device evaluates, on the device, every input the fast path can
+// see (u1 and u2 each take the 2^23 values (j + 0.5) 2^-23; the Poisson mean is (float)l with l in (0, 60], every
+// float up to 60) against the FP64 functions of the exact chain, and the fast path is used only when
+//   |g_fast - g| <= max|dr| (1 + max|dc|) + 5.77 max|dc| + 1e-15 <= kGaussErr / 2   (|r| <= sqrt(-2 log 2^-24) = 5.768)
+//   |L_fast / L - 1| <= max rel(expf) + 3.6e-6 <= kExpRel / 2         (|l - (float)l| <= 60 * 2^-24 = 3.58e-6)
+// The margins below use the full kGaussErr and kExpRel, twice the verified bounds.
+constexpr double kGaussErr = 1e-5;
+constexpr double kExpRel = 1.6e-5;
+constexpr double kRMax = 5.77;
+
+__device__ __forceinline__ double gauss_fast(CounterRng& u)
+{
+   const float u1 = (float)u.uniform(), u2 = (float)u.uniform();
+   const float r = sqrtf(-2.0f * logf(u1));
+   return (double)r * (double)cospif(2.0f * u2);  // exact product of two floats
+}
+
+// Poisson(l) with the filtered threshold; ok = false when a decision fell inside the error interval.
+__device__ __forceinline__ double counter_poisson_fast(double l, CounterRng& u, bool& ok)
+{
+   if (l > 60.0)
+   {
+      const double sl = sqrt(l);
+      const double v = l + sl * gauss_fast(u) + 0.5;
+      const double m = sl * kGaussErr + 1e-14 * (1.0 + l + fabs(v));
+      const double a = floor(v - m), b = floor(v + m);
+      ok &= a == b;
+      return 0.0 < a ? a : 0.0;
+   }
+   if (!(l > 0.0))
+      return 0.0;
+   const double L = (double)expf(-(float)l);
+   const double tol = L * kExpRel;
+   int k = 0;
+   double p = 1.0;
+   bool more;
+   do
+   {
+      ++k;
+      p *= u.uniform();
+      ok &= fabs(p - L) > tol;
+      more = p > L;
+   } while (more && k < 1000);
+   return (double)(k - 1);
+}
+
+template <bool EMCCD>
+__device__ __forceinline__ uint16_t noise_pixel_fast(const CamParams& cp, float photon, uint32_t seed, uint32_t frame,
+                                                     uint32_t pixel, bool hasOffset, float offsetV, bool hasGain,
+                                                     float gainV, bool hasRead, float readV)
+{
+   CounterRng u(seed, frame, pixel);
+   bool ok = true;
+   const double ph0 = (double)photon;
+   const double photons = 0.0 < ph0 ? ph0 : 0.0;
+   const double gain = hasGain ? (double)gainV : cp.gain;
+   const double offset = hasOffset ? (double)offsetV : 0.0;
+   double r = 0.0;
+   if (EMCCD)
+   {
+      const double ne = counter_poisson_fast(photons * cp.qe + cp.dark + cp.cic, u, ok);
+      if (ok)
+      {
+         const double out = ne > 0.0 ? counter_gamma(ne, u) : 0.0;  // the gain register stays FP64
+         const double readE = (hasRead ? (double)readV : cp.read) / cp.emGain;
+         const double q = (out + readE * gauss_fast(u)) / gain;
+         const double x = offset + q + 0.5;
+         // monotone in x: floor then clip agree over [x - m, x + m] when the two floors do
+         const double m = fabs(readE / gain) * kGaussErr +
+                          1e-12 * (1.0 + fabs(offset) + (fabs(out) + kRMax * fabs(readE)) / fabs(gain));
+         const double a = floor(x - m), b = floor(x + m);
+         ok &= a == b;
+         r = a < 0.0 ? 0.0 : (a > cp.maxAdu ? cp.maxAdu : a);
+      }
+   }
+   else
+   {
+      const double ne = counter_poisson_fast(photons * cp.qe + cp.dark, u, ok);
+      if (ok)
+      {
+         const double readNoise = hasRead ? (double)readV : cp.read;
+         const double x = offset + (ne + readNoise * gauss_fast(u)) / gain;
+         const double m = fabs(readNoise / gain) * kGaussErr +
+                          1e-12 * (1.0 + fabs(offset) + (fabs(ne) + kRMax * fabs(readNoise)) / fabs(gain));
+         // clip then round half up, both monotone: take both ends of the interval
+         double lo = x - m, hi = x + m;
+         lo = lo < 0.0 ? 0.0 : (lo > 65535.0 ? 65535.0 : lo);
+         hi = hi < 0.0 ? 0.0 : (hi > 65535.0 ? 65535.0 : hi);
+         const double a = floor(lo + 0.5), b = floor(hi + 0.5);
+         ok &= a == b;  // NaN anywhere fails here and takes the FP64 chain
+         r = a;
+      }
+   }
+   if (ok)
+      return (uint16_t)r;
+   return noise_pixel<EMCCD>(cp, photon, seed, frame, pixel, hasOffset, offsetV, hasGain, gainV, hasRead, readV);
+}
+
+// The path a launch takes: cp.fast is set by the host from the context's setting and the device's verification.
+template <bool EMCCD>
+__device__ __forceinline__ uint16_t noise_pixel_any(const CamParams& cp, float photon, uint32_t seed, uint32_t frame,
+                                                    uint32_t pixel, bool hasOffset, float offsetV, bool hasGain,
+                                                    float gainV, bool hasRead, float readV)
+{
+   if (cp.fast)
+      return noise_pixel_fast<EMCCD>(cp, photon, seed, frame, pixel, hasOffset, offsetV, hasGain, gainV, hasRead,
+                                     readV);
+   return noise_pixel<EMCCD>(cp, photon, seed, frame, pixel, hasOffset, offsetV, hasGain, gainV, hasRead, readV);
+}
+
+// Runs the device verification once per device (cached for the process); true when the fast path may be used.
+// bounds (optional, 2 doubles): the verified |g_fast - g| bound and relative threshold bound.
+bool noise_verify_device(iscg_context* h, double* bounds);
+// The CamParams of a launch on h: cam_params plus the path (fast unless h asks for the exact chain or the device
+// failed verification). False (error set) when the verification could not run.
+bool cam_params_for(iscg_context* h, const iscg_camera* c, CamParams& out);
 
 // Grid size for a grid-stride launch over n items with `block` threads: enough workgroups to fill all 256 CUs
 // several times over, never more than the items need.

@@ -5,6 +5,10 @@
 // its lane divergence is the Poisson/Gamma draw counts. Reads 4 B and writes 2 B per pixel (plus maps).
 #include "iscg_internal.h"
 
+#include <cstring>
+#include <map>
+#include <mutex>
+
 namespace iscg {
 
 template <bool EMCCD>
@@ -19,7 +23,7 @@ __global__ __launch_bounds__(256) void noise_kernel(const float* __restrict__ ph
    {
       const uint32_t f = (uint32_t)(idx / npx);
       const uint32_t i = (uint32_t)(idx - (size_t)f * npx);
-      out[idx] = noise_pixel<EMCCD>(cp, photons[idx], seed, frames[f], i + pixelBase, offset != nullptr,
+      out[idx] = noise_pixel_any<EMCCD>(cp, photons[idx], seed, frames[f], i + pixelBase, offset != nullptr,
                                     offset ? offset[i] : 0.0f, gain != nullptr, gain ? gain[i] : 0.0f,
                                     readN != nullptr, readN ? readN[i] : 0.0f);
    }
@@ -44,9 +48,157 @@ bool launch_noise(iscg_context* h, const float* photons, uint16_t* out, uint32_t
    return ok(h, hipGetLastError(), "noise kernel launch");
 }
 
+// ---- verification of the FP32 path's error bounds (see noise_pixel_fast) -----------------------------------------
+
+__device__ __forceinline__ double wave_max(double v)
+{
+   for (int o = 32; o > 0; o >>= 1)
+   {
+      const double w = __shfl_xor(v, o);
+      v = v > w ? v : w;
+   }
+   return v;
+}
+
+// v >= 0 or NaN; NaN and inf count as +inf. Non-negative doubles order as their bit patterns.
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* dst, double v)
+{
+   const unsigned long long bits = v <= 1.7976931348623157e308 ? (unsigned long long)__double_as_longlong(v)
+                                                                : 0x7FF0000000000000ull;
+   atomicMax(dst, bits);
+}
+
+// out[0] = max |r_fp32 - r|, out[1] = max |c_fp32 - c| over the 2^23 uniforms (j + 0.5) 2^-23 of CounterRng.
+__global__ __launch_bounds__(256) void verify_gauss_kernel(unsigned long long* out)
+{
+   double mr = 0.0, mc = 0.0;
+   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < (1u << 23); j += gridDim.x * blockDim.x)
+   {
+      const double u = ((double)j + 0.5) * 1.1920928955078125e-7;
+      const float uf = (float)u;
+      const double dr = fabs((double)sqrtf(-2.0f * logf(uf)) - sqrt(-2.0 * log(u)));
+      const double dc = fabs((double)cospif(2.0f * uf) - cos(6.283185307179586 * u));
+      mr = !(dr <= mr) ? dr : mr;  // NaN propagates
+      mc = !(dc <= mc) ? dc : mc;
+   }
+   mr = wave_max(mr);
+   mc = wave_max(mc);
+   if ((threadIdx.x & 63) == 0)
+   {
+      atomic_max_nonneg(out, mr);
+      atomic_max_nonneg(out + 1, mc);
+   }
+}
+
+// out[2] = max |expf(-f) / exp(-f) - 1| over every float f in [0, 60].
+__global__ __launch_bounds__(256) void verify_exp_kernel(unsigned long long* out)
+{
+   constexpr uint32_t kLast = 0x42700000u;  // 60.0f
+   double m = 0.0;
+   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= kLast; b += gridDim.x * blockDim.x)
+   {
+      const float f = __uint_as_float(b);
+      const double e = exp(-(double)f);
+      const double rel = fabs((double)expf(-f) - e) / e;
+      m = !(rel <= m) ? rel : m;
+   }
+   m = wave_max(m);
+   if ((threadIdx.x & 63) == 0)
+      atomic_max_nonneg(out + 2, m);
+}
+
+namespace {
+std::mutex g_verifyMutex;
+struct Verified
+{
+   bool done = false, fastOk = false;
+   double gaussBound = 0.0, expBound = 0.0;
+};
+std::map<int, Verified> g_verified;
+}  // namespace
+
+bool noise_verify_device(iscg_context* h, double* bounds)
+{
+   std::lock_guard<std::mutex> lock(g_verifyMutex);
+   Verified& v = g_verified[h->device];
+   if (!v.done)
+   {
+      unsigned long long* d = nullptr;
+      unsigned long long r[3] = {0, 0, 0};
+      if (!ok(h, hipMalloc((void**)&d, sizeof(r)), "noise verification alloc"))
+         return false;
+      bool good = ok(h, hipMemsetAsync(d, 0, sizeof(r), h->stream), "noise verification memset");
+      if (good)
+      {
+         hipLaunchKernelGGL(verify_gauss_kernel, dim3(1024), dim3(256), 0, h->stream, d);
+         hipLaunchKernelGGL(verify_exp_kernel, dim3(8192), dim3(256), 0, h->stream, d);
+         good = ok(h, hipGetLastError(), "noise verification launch") &&
+                ok(h, hipMemcpyAsync(r, d, sizeof(r), hipMemcpyDeviceToHost, h->stream), "noise verification copy") &&
+                ok(h, hipStreamSynchronize(h->stream), "noise verification");
+      }
+      (void)hipFree(d);
+      if (!good)
+         return false;
+      double m[3];
+      std::memcpy(m, r, sizeof(m));
+      // |g_fp32 - g| <= max|dr| (1 + max|dc|) + |r|max |dc| (+ the FP64 product's rounding); the threshold also
+      // carries l -> (float)l: |l - (float)l| <= 60 * 2^-24 in the exponent, a factor within 1 +- 3.6e-6.
+      v.gaussBound = m[0] * (1.0 + m[1]) + kRMax * m[1] + 1e-15;
+      v.expBound = m[2] * (1.0 + 3.6e-6) + 3.6e-6 + 1e-15;
+      v.fastOk = v.gaussBound <= 0.5 * kGaussErr && v.expBound <= 0.5 * kExpRel;  // false for NaN too
+      v.done = true;
+   }
+   if (bounds)
+   {
+      bounds[0] = v.gaussBound;
+      bounds[1] = v.expBound;
+   }
+   return v.fastOk;
+}
+
+bool cam_params_for(iscg_context* h, const iscg_camera* c, CamParams& out)
+{
+   out = cam_params(c);
+   if (h->noiseExact)
+      return true;
+   const std::string before = h->err;
+   const bool fast = noise_verify_device(h, nullptr);
+   if (!fast && h->err != before)
+      return false;  // the verification itself failed
+   out.fast = fast ? 1 : 0;
+   return true;
+}
+
 }  // namespace iscg
 
 using namespace iscg;
+
+extern "C" int32_t iscg_noise_set_path(iscg_context* h, int32_t exact)
+{
+   if (begin(h) != 0)
+      return -1;
+   if (exact != 0 && exact != 1)
+      return fail(h, "iscg_noise_set_path: exact must be 0 or 1");
+   h->noiseExact = exact;
+   return 0;
+}
+
+extern "C" int32_t iscg_noise_bounds(iscg_context* h, double* out)
+{
+   if (begin(h) != 0)
+      return -1;
+   if (!out)
+      return fail(h, "iscg_noise_bounds: out is NULL");
+   double b[2] = {0.0, 0.0};
+   const std::string before = h->err;
+   const bool fast = noise_verify_device(h, b);
+   if (h->err != before)
+      return -1;
+   out[0] = fast && !h->noiseExact ? 1.0 : 0.0;
+   out[1] = b[0];
+   out[2] = b[1];
+   return 0;
+}
 
 extern "C" int32_t iscg_noise(iscg_context* h, const float* photons, uint32_t width, uint32_t height,
                               const iscg_camera* cam, const float* offset, const float* gain, const float* readNoise,
@@ -71,8 +223,9 @@ extern "C" int32_t iscg_noise(iscg_context* h, const float* photons, uint32_t wi
    uint16_t* dOut = (uint16_t*)stage_out(h, 5, out, nAll * sizeof(uint16_t));
    if (!dPh || !dFr || (offset && !dOff) || (gain && !dGain) || (readNoise && !dRead) || !dOut)
       return -1;
-   if (!launch_noise(h, dPh, dOut, (uint32_t)npx, nFrames, dFr, dOff, dGain, dRead, cam_params(cam), seed,
-                     pixelBase))
+   CamParams cp;
+   if (!cam_params_for(h, cam, cp) ||
+       !launch_noise(h, dPh, dOut, (uint32_t)npx, nFrames, dFr, dOff, dGain, dRead, cp, seed, pixelBase))
       return -1;
    if (!finish_out(h, out, dOut, nAll * sizeof(uint16_t)))
       return -1;

@@ -393,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelT
             a.outPhotons[(size_t)f * npx + i] = v;
          if (NOISE)
             a.outAdu[(size_t)f * npx + i] =
-               noise_pixel<EMCCD>(cp, v, a.seed, a.frames[f], (uint32_t)i + a.pixelBase, a.offset != nullptr,
+               noise_pixel_any<EMCCD>(cp, v, a.seed, a.frames[f], (uint32_t)i + a.pixelBase, a.offset != nullptr,
                                   a.offset ? a.offset[i] : 0.0f, a.gain != nullptr, a.gain ? a.gain[i] : 0.0f,
                                   a.readN != nullptr, a.readN ? a.readN[i] : 0.0f);
       }
@@ -475,7 +475,9 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
    a.nEmitters = nEmitters;
    a.seed = seed;
    a.pixelBase = s.pixelBase;
-   const CamParams cp = cam ? cam_params(cam) : CamParams{};
+   CamParams cp{};
+   if (cam && !cam_params_for(h, cam, cp))
+      return false;
    const size_t items = (size_t)a.nTiles * nFrames;
    const unsigned grid = (unsigned)(items < 256 * 16 ? items : 256 * 16);
    ProfScope ps(h, "splat_tiles_noise");

@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void rows_c2r_kernel(const float2* __restrict_
          if (NOISE)
          {
             const size_t pix = (size_t)y * e.cw + x;
-            e.adu[o] = noise_pixel<EMCCD>(e.cp, ph, e.seed, e.frameBase + (uint32_t)f, (uint32_t)pix + e.pixelBase,
+            e.adu[o] = noise_pixel_any<EMCCD>(e.cp, ph, e.seed, e.frameBase + (uint32_t)f, (uint32_t)pix + e.pixelBase,
                                           e.offset != nullptr, e.offset ? e.offset[pix] : 0.0f, false, 0.0f, false,
                                           0.0f);
          }
@@ -608,7 +608,9 @@ int32_t iscg_wf_conv3d(iscg_context* h, const float* density, const iscg_camera*
    e.adu = dAdu;
    e.offset = dOff;
    e.background = background;
-   e.cp = cam ? cam_params(cam) : CamParams{};
+   e.cp = CamParams{};
+   if (cam && !cam_params_for(h, cam, e.cp))
+      return -1;
    e.seed = seed;
    e.frameBase = frameBase;
    e.pixelBase = pixelBase;

@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import itertools
+import sys
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -70,6 +71,8 @@ class Context:
             raise IscgError(self.L.iscg_last_error(None).decode())
         self.h = h
         self.device = device
+        self._own_stream = self.L.iscg_stream(h) or 0
+        self._cur_stream = self._own_stream
 
     def close(self):
         if getattr(self, "h", None):
@@ -94,10 +97,35 @@ class Context:
 
     @property
     def stream(self) -> int:
-        return self.L.iscg_stream(self.h)
+        return self.L.iscg_stream(self.h) or 0
+
+    def set_stream(self, stream: int | None):
+        """Enqueue later work on `stream` (a hipStream_t as int; None: the context's own stream)."""
+        s = self._own_stream if stream is None else int(stream)
+        if s != self._cur_stream:
+            self.check(self.L.iscg_set_stream(self.h, s or None), "iscg_set_stream")
+            self._cur_stream = s
+
+    def follow_torch(self):
+        """Once torch has initialised the GPU, enqueue on torch's current stream of this device, so tensors made or
+        read by torch kernels are ordered with the library's work; the context's own stream otherwise."""
+        torch = sys.modules.get("torch")
+        if torch is not None and torch.cuda.is_initialized():
+            self.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+        else:
+            self.set_stream(None)
 
     def inject_failure(self, count: int, message: str = "injected failure"):
         self.check(self.L.iscg_inject_failure(self.h, count, message.encode()))
+
+    def set_noise_path(self, exact: bool):
+        """exact=True: every noise pixel through the FP64 chain; False: the verified FP32 filter (same ADU)."""
+        self.check(self.L.iscg_noise_set_path(self.h, 1 if exact else 0))
+
+    def noise_bounds(self) -> dict:
+        out = (C.c_double * 3)()
+        self.check(self.L.iscg_noise_bounds(self.h, out))
+        return {"fp32_path": bool(out[0]), "gauss_bound": out[1], "threshold_rel_bound": out[2]}
 
 
 _default: dict[int, Context] = {}
@@ -149,6 +177,7 @@ def ApplyNoiseChainFrames(photons, cam: CameraNoiseParams, noiseSeed: int, frame
                                 _arr(readNoiseMap, np.float32), out)]
     p = [k[0] for k in keep]
     cc = cam.to_c()
+    ctx.follow_torch()
     ctx.check(ctx.L.iscg_noise(ctx.h, p[0], width, height, C.byref(cc), p[2], p[3], p[4], noiseSeed & 0xFFFFFFFF,
                                p[1], n_frames, pixelBase, p[5]), "iscg_noise")
     return out
@@ -331,6 +360,7 @@ class SplatGpuHost:
             views[i] = _lib.KernelView(pl, lo, hi, k.zStepNm, k.serial, k.nz, k.sizeOversampled, k.oversampling,
                                        k.interpMode)
         self.kernels = list(kernels)
+        self.ctx.follow_torch()
         return self._call(self.ctx.L.iscg_splat_set_kernels(self.ctx.h, views, len(kernels)))
 
     def SetStatic(self, width: int, height: int, pixelBase: int = 0, offset=None, gain=None, readNoise=None,
@@ -339,6 +369,7 @@ class SplatGpuHost:
         fit = lambda m: None if m is None or np.size(m) != n else _arr(m, np.float32)
         ps = [_ptr(fit(m)) for m in (offset, gain, readNoise, background)]
         self.width, self.height = width, height
+        self.ctx.follow_torch()
         return self._call(self.ctx.L.iscg_splat_set_static(self.ctx.h, width, height, pixelBase, ps[0][0], ps[1][0],
                                                            ps[2][0], ps[3][0], float(flatBackground), key))
 
@@ -355,6 +386,7 @@ class SplatGpuHost:
         cc = cam.to_c() if cam is not None else None
         n_em = int(em.shape[0]) if em is not None else 0
         L = self.ctx.L
+        self.ctx.follow_torch()
         if records:
             rc = L.iscg_splat_render_records(self.ctx.h, ptrs[0][0], n_em, ptrs[1][0], n_frames, ptrs[2][0],
                                              ptrs[3][0], ptrs[4][0], C.byref(cc) if cc else None, seed & 0xFFFFFFFF,
@@ -436,6 +468,7 @@ def ComputePsfPlanesBatch(requests, normalize: bool = False, out=None, ctx: Cont
         out = np.zeros((len(reqs), nz, size, size), np.float32)
     p, keep = _ptr(out)
     cap = int(np.prod(out.shape)) if hasattr(out, "shape") else 0
+    ctx.follow_torch()
     ctx.check(ctx.L.iscg_psf_planes(ctx.h, arr, len(reqs), int(bool(normalize)), p, cap), "iscg_psf_planes")
     del keep
     return out
@@ -474,6 +507,7 @@ class WidefieldStack:
         d = _lib.WfDesc(dzUm=dzUm, focus0Um=focus0Um, k0=k0, nx=nx, ny=ny, nzD=nzD, nF=nF, fovX0=fov[0],
                         fovY0=fov[1], cw=fov[2], ch=fov[3], NX=fft[0], NY=fft[1], NZ=fft[2], upscale=1)
         pl, keep = _ptr(_arr(cache.planes, np.float32))
+        self.ctx.follow_torch()
         ok = self._call(self.ctx.L.iscg_wf_conv3d_plan(self.ctx.h, C.byref(d), pl, cache.nz, cache.sizeOversampled,
                                                        cache.oversampling, cache.zStepNm, kernelRadius))
         del keep
@@ -493,6 +527,7 @@ class WidefieldStack:
         """density (nzD, ny, nx) -> outImages (nF, ch, cw) float32 frames and/or outAdu uint16 (camera noise)."""
         ptrs = [_ptr(x) for x in (_arr(density, np.float32), _arr(offset, np.float32), outImages, outAdu)]
         cc = cam.to_c() if cam is not None else None
+        self.ctx.follow_torch()
         return self._call(self.ctx.L.iscg_wf_conv3d(self.ctx.h, ptrs[0][0], C.byref(cc) if cc else None,
                                                     ptrs[1][0], float(background), seed & 0xFFFFFFFF, frameBase,
                                                     pixelBase, ptrs[2][0], ptrs[3][0]))
@@ -528,6 +563,7 @@ def RoiForward(model, zernike, params, out=None, ctx: Context | None = None):
         out = np.zeros((n, model.roiSize, model.roiSize), np.float32)
     pp, k1 = _ptr(prm)
     po, k2 = _ptr(out)
+    ctx.follow_torch()
     ctx.check(ctx.L.iscg_roi_forward(ctx.h, C.byref(model), _zern(zernike), pp, n, po), "iscg_roi_forward")
     return out
 
@@ -545,6 +581,7 @@ def RoiBackward(model, zernike, params, data, gradParams=None, ctx: Context | No
     pp, k1 = _ptr(prm)
     pd, k2 = _ptr(d)
     pg, k3 = _ptr(gradParams)
+    ctx.follow_torch()
     ctx.check(ctx.L.iscg_roi_backward(ctx.h, C.byref(model), _zern(zernike), pp, n, pd, pg, gz, C.byref(nll)),
               "iscg_roi_backward")
     return nll.value, gradParams, np.array(gz[:])

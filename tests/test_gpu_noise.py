@@ -61,3 +61,50 @@ def test_noise_failure_injection_leaves_caller_in_charge(ctx):
         host.ApplyNoiseChain(ph, 8, 8, cam, ctx=ctx)
     out = host.ApplyNoiseChain(ph, 8, 8, cam, ctx=ctx)
     assert out.shape == (8, 8)
+
+
+def test_noise_fp32_path_verified_on_device(ctx):
+    """The FP32 filter's error bounds, measured on the device over every input it can receive, leave the 2x margin."""
+    b = ctx.noise_bounds()
+    print(b, flush=True)
+    assert b["fp32_path"]
+    assert 0.0 < b["gauss_bound"] <= 5e-6
+    assert 3.6e-6 < b["threshold_rel_bound"] <= 8e-6
+
+
+def _both_paths(ctx, fn):
+    ctx.set_noise_path(True)
+    try:
+        exact = fn()
+    finally:
+        ctx.set_noise_path(False)
+    fast = fn()
+    return exact, fast
+
+
+@pytest.mark.parametrize("emccd", [False, True])
+@pytest.mark.parametrize("read", [1.6, 300.0, 1e4])
+def test_noise_fp32_path_equals_fp64_chain_full_size(ctx, emccd, read):
+    """64 frames of 256 x 256 (the mean-field stack's size) through both paths: identical ADU. Photon levels span the
+    Knuth branch, the normal approximation (l > 60), zero and the 16-bit clip; read = 1e4 puts a tenth of the pixels
+    within the margin, so lanes of one wave take both paths."""
+    import torch
+    print("emccd", emccd, "read", read, flush=True)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    n, h, w = 64, 256, 256
+    lvl = torch.tensor([0.0, 0.3, 3.0, 20.0, 59.0, 61.0, 400.0, 5e4, 2e5], device="cuda")
+    idx = torch.randint(0, len(lvl), (n, h, w), device="cuda", generator=g)
+    ph = (lvl[idx] * (0.5 + torch.rand((n, h, w), device="cuda", generator=g))).contiguous()
+    off = (100.0 + 5.0 * torch.rand((h, w), device="cuda", generator=g)).contiguous()
+    gain = (1.8 + 0.4 * torch.rand((h, w), device="cuda", generator=g)).contiguous()
+    frames = torch.arange(n, dtype=torch.int32, device="cuda") * 3 + 1
+    cam = host.CameraNoiseParams(0.82, 0.05, 1.0 / 200.0 if emccd else 2.0, read, emccd, 200.0, 0.003, 16)
+    exact, fast = _both_paths(
+        ctx, lambda: host.ApplyNoiseChainFrames(ph, cam, 77, frames, offsetMap=off,
+                                                gainMap=None if emccd else gain, ctx=ctx).cpu())
+    assert torch.equal(exact, fast), f"{(exact != fast).sum().item()} of {exact.numel()} differ"
+
+
+def test_noise_set_path_rejects_bad_value(ctx):
+    with pytest.raises(host.IscgError, match="exact must be 0 or 1"):
+        ctx.check(ctx.L.iscg_noise_set_path(ctx.h, 2))
