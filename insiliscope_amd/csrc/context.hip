@@ -200,6 +200,8 @@ void iscg_destroy(iscg_context* h)
    }
    for (auto e : h->eventPool)
       (void)hipEventDestroy(e);
+   if (h->noiseCount)
+      (void)hipFree(h->noiseCount);
    (void)hipStreamDestroy(h->ownStream);
    delete h;
 }

@@ -87,6 +87,8 @@ struct iscg_context
    iscg::WfState* wf = nullptr;
    iscg::RoiState* roi = nullptr;
    int noiseExact = 0;  // iscg_noise_set_path: 1 evaluates every pixel with the FP64 chain
+   iscg::DevBuf noiseItems;            // deferred pixels (NoiseQueue::items)
+   uint32_t* noiseCount = nullptr;     // NoiseQueue::count, 2 words
 };
 
 namespace iscg {
@@ -157,9 +159,50 @@ struct CounterRng
       }
       const uint32_t x = k == 0 ? b0 : (k == 1 ? b1 : (k == 2 ? b2 : b3));
       ++k;
-      return ((double)(x >> 9) + 0.5) * 1.1920928955078125e-7;
+      return unit_of(x);
+   }
+   // ((x >> 9) + 0.5) 2^-23, exact in FP64 (and in FP32)
+   static __device__ __forceinline__ double unit_of(uint32_t x) { return ((double)(x >> 9) + 0.5) * 1.1920928955078125e-7; }
+   // Starts the next group of four draws (the state uniform() would reach after using up the current one).
+   __device__ __forceinline__ void next_group()
+   {
+      b0 = seed;
+      b1 = frame;
+      b2 = pix;
+      b3 = ctr;
+      pcg4d(b0, b1, b2, b3);
+      ++ctr;
+      k = 0;
    }
 };
+
+// Knuth's product loop of CounterPoisson (Sim/SMLMCounterRng.h:110-118) from a fresh stream (the chain's first
+// draws, so groups of four start at draw 0): p_j = p_{j-1} u_j in FP64, stop at the first p_k <= L or at k = 1000;
+// returns k - 1 and leaves u after draw k, as the one-draw-at-a-time loop does. The products strictly decrease
+// (u <= 1 - 2^-24, and no underflow before p <= L since L >= e^-60), so a group holds the stop iff its fourth
+// product is <= L: the loop does four multiplies and one compare per group, and the stop is located once, after it.
+// last = p_k, prev = p_{k-1} (p_0 = 1).
+__device__ __forceinline__ int knuth_fresh(double L, CounterRng& u, double& last, double& prev)
+{
+   double p = 1.0, p1, p2, p3, p4;
+   int g = 0;
+   for (;; ++g)
+   {
+      u.next_group();
+      p1 = p * CounterRng::unit_of(u.b0);
+      p2 = p1 * CounterRng::unit_of(u.b1);
+      p3 = p2 * CounterRng::unit_of(u.b2);
+      p4 = p3 * CounterRng::unit_of(u.b3);
+      if (!(p4 > L) || g == 249)  // 250 groups = the 1000-draw cap
+         break;
+      p = p4;
+   }
+   const int n = (int)(p1 > L) + (int)(p2 > L) + (int)(p3 > L) + (int)(p4 > L);  // products > L in this group
+   last = n == 0 ? p1 : (n == 1 ? p2 : (n == 2 ? p3 : p4));
+   prev = n == 0 ? p : (n == 1 ? p1 : (n == 2 ? p2 : p3));
+   u.k = n < 4 ? n + 1 : 4;
+   return 4 * g + (n < 4 ? n : 3);
+}
 
 // Sim/SMLMCounterRng.h:95-99
 __device__ __forceinline__ double counter_gauss(CounterRng& u)
@@ -168,7 +211,7 @@ __device__ __forceinline__ double counter_gauss(CounterRng& u)
    return r * cos(6.283185307179586 * u.uniform());
 }
 
-// Sim/SMLMCounterRng.h:104-119
+// Sim/SMLMCounterRng.h:104-119, as the chain's first draw (u fresh)
 __device__ __forceinline__ double counter_poisson(double l, CounterRng& u)
 {
    if (l > 60.0)
@@ -178,15 +221,8 @@ __device__ __forceinline__ double counter_poisson(double l, CounterRng& u)
    }
    if (!(l > 0.0))
       return 0.0;
-   const double L = exp(-l);
-   int k = 0;
-   double p = 1.0;
-   do
-   {
-      ++k;
-      p *= u.uniform();
-   } while (p > L && k < 1000);
-   return (double)(k - 1);
+   double last, prev;
+   return (double)knuth_fresh(exp(-l), u, last, prev);
 }
 
 // Sim/SMLMCounterRng.h:123-138
@@ -211,6 +247,7 @@ __device__ __forceinline__ double counter_gamma(double k, CounterRng& u)
 struct CamParams
 {
    double qe, dark, gain, read, emGain, cic, maxAdu;
+   double invGain;  // 1 / gain (the FP32 path's multiply)
    int emccd;
    int fast;  // 1: noise_pixel_fast
 
@@ -227,6 +264,7 @@ inline CamParams cam_params(const iscg_camera* c)
    p.cic = c->cicElectrons;
    const int bits = c->bitDepth < 1 ? 1 : (c->bitDepth > 16 ? 16 : c->bitDepth);
    p.maxAdu = (double)((1u << bits) - 1u);  // ldexp(1, bits) - 1, exact
+   p.invGain = 1.0 / p.gain;
    p.emccd = c->emccd != 0;
    p.fast = 0;
    return p;
@@ -313,29 +351,26 @@ __device__ __forceinline__ double counter_poisson_fast(double l, CounterRng& u, 
       return 0.0;
    const double L = (double)expf(-(float)l);
    const double tol = L * kExpRel;
-   int k = 0;
-   double p = 1.0;
-   bool more;
-   do
-   {
-      ++k;
-      p *= u.uniform();
-      ok &= fabs(p - L) > tol;
-      more = p > L;
-   } while (more && k < 1000);
-   return (double)(k - 1);
+   double last, prev;
+   const int n = knuth_fresh(L, u, last, prev);
+   // the products decrease: only p_k and p_{k-1} can sit on the other side of the exact threshold
+   ok &= fabs(last - L) > tol && (n == 0 || fabs(prev - L) > tol);
+   return (double)n;
 }
 
+// ok = false: some decision was within the error interval; the result is then not used.
 template <bool EMCCD>
 __device__ __forceinline__ uint16_t noise_pixel_fast(const CamParams& cp, float photon, uint32_t seed, uint32_t frame,
                                                      uint32_t pixel, bool hasOffset, float offsetV, bool hasGain,
-                                                     float gainV, bool hasRead, float readV)
+                                                     float gainV, bool hasRead, float readV, bool& ok)
 {
    CounterRng u(seed, frame, pixel);
-   bool ok = true;
+   ok = true;
    const double ph0 = (double)photon;
    const double photons = 0.0 < ph0 ? ph0 : 0.0;
-   const double gain = hasGain ? (double)gainV : cp.gain;
+   // 1/gain: the launch's FP64 reciprocal (relative error 1.1e-16), or a map value's FP32 one (<= 6e-8)
+   const double invGain = hasGain ? (double)(1.0f / gainV) : cp.invGain;
+   const double relErr = hasGain ? 2e-7 : 1e-12;  // with the FP64 roundings of the sum, inflated
    const double offset = hasOffset ? (double)offsetV : 0.0;
    double r = 0.0;
    if (EMCCD)
@@ -345,11 +380,11 @@ __device__ __forceinline__ uint16_t noise_pixel_fast(const CamParams& cp, float 
       {
          const double out = ne > 0.0 ? counter_gamma(ne, u) : 0.0;  // the gain register stays FP64
          const double readE = (hasRead ? (double)readV : cp.read) / cp.emGain;
-         const double q = (out + readE * gauss_fast(u)) / gain;
+         const double q = (out + readE * gauss_fast(u)) * invGain;
          const double x = offset + q + 0.5;
          // monotone in x: floor then clip agree over [x - m, x + m] when the two floors do
-         const double m = fabs(readE / gain) * kGaussErr +
-                          1e-12 * (1.0 + fabs(offset) + (fabs(out) + kRMax * fabs(readE)) / fabs(gain));
+         const double m = fabs(readE * invGain) * kGaussErr + relErr * (1.0 + fabs(offset) + fabs(q)) +
+                          1e-12 * kRMax * fabs(readE * invGain);
          const double a = floor(x - m), b = floor(x + m);
          ok &= a == b;
          r = a < 0.0 ? 0.0 : (a > cp.maxAdu ? cp.maxAdu : a);
@@ -361,9 +396,10 @@ __device__ __forceinline__ uint16_t noise_pixel_fast(const CamParams& cp, float 
       if (ok)
       {
          const double readNoise = hasRead ? (double)readV : cp.read;
-         const double x = offset + (ne + readNoise * gauss_fast(u)) / gain;
-         const double m = fabs(readNoise / gain) * kGaussErr +
-                          1e-12 * (1.0 + fabs(offset) + (fabs(ne) + kRMax * fabs(readNoise)) / fabs(gain));
+         const double q = (ne + readNoise * gauss_fast(u)) * invGain;
+         const double x = offset + q;
+         const double m = fabs(readNoise * invGain) * kGaussErr + relErr * (1.0 + fabs(offset) + fabs(q)) +
+                          1e-12 * kRMax * fabs(readNoise * invGain);
          // clip then round half up, both monotone: take both ends of the interval
          double lo = x - m, hi = x + m;
          lo = lo < 0.0 ? 0.0 : (lo > 65535.0 ? 65535.0 : lo);
@@ -373,22 +409,51 @@ __device__ __forceinline__ uint16_t noise_pixel_fast(const CamParams& cp, float 
          r = a;
       }
    }
-   if (ok)
-      return (uint16_t)r;
-   return noise_pixel<EMCCD>(cp, photon, seed, frame, pixel, hasOffset, offsetV, hasGain, gainV, hasRead, readV);
+   return (uint16_t)r;
 }
 
-// The path a launch takes: cp.fast is set by the host from the context's setting and the device's verification.
-template <bool EMCCD>
-__device__ __forceinline__ uint16_t noise_pixel_any(const CamParams& cp, float photon, uint32_t seed, uint32_t frame,
-                                                    uint32_t pixel, bool hasOffset, float offsetV, bool hasGain,
-                                                    float gainV, bool hasRead, float readV)
+// ---- deferred pixels ----------------------------------------------------------------------------------------------
+// A launch that produces ADU evaluates each pixel with noise_pixel_fast; a pixel it cannot decide (or every pixel,
+// when the context asks for the FP64 chain) is appended here as (output index, photon bits) and finished by
+// noise_fallback_kernel, which runs noise_pixel. Keeping the FP64 chain out of the producing kernels keeps their
+// register use (and occupancy) to that of the FP32 path. Launch geometry shared by all producers: output index
+// idx = f * npx + i, frame id frames[f] (or frameBase + f), stream pixel i + pixelBase, maps indexed by i.
+struct NoiseQueue
 {
-   if (cp.fast)
-      return noise_pixel_fast<EMCCD>(cp, photon, seed, frame, pixel, hasOffset, offsetV, hasGain, gainV, hasRead,
-                                     readV);
-   return noise_pixel<EMCCD>(cp, photon, seed, frame, pixel, hasOffset, offsetV, hasGain, gainV, hasRead, readV);
+   uint2* items;     // [capacity >= pixels of the launch]
+   uint32_t* count;  // [0]: items, [1]: fallback blocks done (both zero between launches)
+};
+
+__device__ __forceinline__ void noise_defer(const NoiseQueue& q, uint32_t idx, float photon)
+{
+   const uint32_t slot = atomicAdd(q.count, 1u);
+   q.items[slot] = make_uint2(idx, __float_as_uint(photon));
 }
+
+template <bool EMCCD>
+__device__ __forceinline__ void noise_store(const CamParams& cp, const NoiseQueue& q, uint16_t* __restrict__ out,
+                                            uint32_t idx, float photon, uint32_t seed, uint32_t frame, uint32_t pixel,
+                                            bool hasOffset, float offsetV, bool hasGain, float gainV, bool hasRead,
+                                            float readV)
+{
+   bool ok = false;
+   uint16_t r = 0;
+   if (cp.fast)
+      r = noise_pixel_fast<EMCCD>(cp, photon, seed, frame, pixel, hasOffset, offsetV, hasGain, gainV, hasRead, readV,
+                                  ok);
+   if (ok)
+      out[idx] = r;
+   else
+      noise_defer(q, idx, photon);
+}
+
+// The queue for a launch of `pixels` ADU (capacity grows; the counters are allocated and zeroed once).
+bool noise_queue(iscg_context* h, size_t pixels, NoiseQueue& q);
+// Finishes the deferred pixels of the launch just enqueued on h's stream and resets the queue's counters.
+// frames: per-frame ids or null (then frameBase + f).
+bool launch_noise_fallback(iscg_context* h, const NoiseQueue& q, uint16_t* out, uint32_t npx, const uint32_t* frames,
+                           uint32_t frameBase, const float* offset, const float* gain, const float* readN,
+                           const CamParams& cp, uint32_t seed, uint32_t pixelBase);
 
 // Runs the device verification once per device (cached for the process); true when the fast path may be used.
 // bounds (optional, 2 doubles): the verified |g_fast - g| bound and relative threshold bound.

@@ -5,6 +5,7 @@
 // its lane divergence is the Poisson/Gamma draw counts. Reads 4 B and writes 2 B per pixel (plus maps).
 #include "iscg_internal.h"
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -16,36 +17,120 @@ __global__ __launch_bounds__(256) void noise_kernel(const float* __restrict__ ph
                                                      uint32_t npx, uint32_t nFrames, const uint32_t* __restrict__ frames,
                                                      const float* __restrict__ offset, const float* __restrict__ gain,
                                                      const float* __restrict__ readN, CamParams cp, uint32_t seed,
-                                                     uint32_t pixelBase)
+                                                     uint32_t pixelBase, NoiseQueue q)
 {
-   const size_t total = (size_t)npx * nFrames;
-   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x)
+   const uint32_t total = npx * nFrames;  // < 2^32 (the host splits larger batches)
+   for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x)
    {
-      const uint32_t f = (uint32_t)(idx / npx);
-      const uint32_t i = (uint32_t)(idx - (size_t)f * npx);
-      out[idx] = noise_pixel_any<EMCCD>(cp, photons[idx], seed, frames[f], i + pixelBase, offset != nullptr,
-                                    offset ? offset[i] : 0.0f, gain != nullptr, gain ? gain[i] : 0.0f,
-                                    readN != nullptr, readN ? readN[i] : 0.0f);
+      const uint32_t f = idx / npx;
+      const uint32_t i = idx - f * npx;
+      noise_store<EMCCD>(cp, q, out, idx, photons[idx], seed, frames[f], i + pixelBase, offset != nullptr,
+                         offset ? offset[i] : 0.0f, gain != nullptr, gain ? gain[i] : 0.0f, readN != nullptr,
+                         readN ? readN[i] : 0.0f);
    }
 }
 
-// Launches the noise kernel on device arrays (shared with the splat path when noise runs as its own pass).
+// The FP64 chain for the deferred pixels of the previous launch on the stream; the last block to finish zeroes the
+// counters for the next launch.
+template <bool EMCCD>
+__global__ __launch_bounds__(256) void noise_fallback_kernel(NoiseQueue q, uint16_t* __restrict__ out, uint32_t npx,
+                                                              const uint32_t* __restrict__ frames, uint32_t frameBase,
+                                                              const float* __restrict__ offset,
+                                                              const float* __restrict__ gain,
+                                                              const float* __restrict__ readN, CamParams cp,
+                                                              uint32_t seed, uint32_t pixelBase)
+{
+   const uint32_t n = __hip_atomic_load(q.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+   {
+      const uint2 it = q.items[j];
+      const uint32_t f = it.x / npx;
+      const uint32_t i = it.x - f * npx;
+      out[it.x] = noise_pixel<EMCCD>(cp, __uint_as_float(it.y), seed, frames ? frames[f] : frameBase + f,
+                                     i + pixelBase, offset != nullptr, offset ? offset[i] : 0.0f, gain != nullptr,
+                                     gain ? gain[i] : 0.0f, readN != nullptr, readN ? readN[i] : 0.0f);
+   }
+   __syncthreads();
+   if (threadIdx.x == 0)
+   {
+      // every block has read n before it counts itself done
+      const uint32_t done = __hip_atomic_fetch_add(q.count + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (done == gridDim.x - 1)
+      {
+         __hip_atomic_store(q.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+         __hip_atomic_store(q.count + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+   }
+}
+
+bool noise_queue(iscg_context* h, size_t pixels, NoiseQueue& q)
+{
+   if (!h->noiseCount)
+   {
+      void* p = nullptr;
+      if (!ok(h, hipMalloc(&p, 2 * sizeof(uint32_t)), "noise queue counters"))
+         return false;
+      h->noiseCount = (uint32_t*)p;
+      if (!ok(h, hipMemsetAsync(p, 0, 2 * sizeof(uint32_t), h->stream), "noise queue counters"))
+         return false;
+   }
+   if (!ok(h, h->noiseItems.reserve(pixels * sizeof(uint2)), "noise queue"))
+      return false;
+   q.items = h->noiseItems.as<uint2>();
+   q.count = h->noiseCount;
+   return true;
+}
+
+bool launch_noise_fallback(iscg_context* h, const NoiseQueue& q, uint16_t* out, uint32_t npx, const uint32_t* frames,
+                           uint32_t frameBase, const float* offset, const float* gain, const float* readN,
+                           const CamParams& cp, uint32_t seed, uint32_t pixelBase)
+{
+   ProfScope ps(h, "noise_fallback");
+   const unsigned grid = cp.fast ? 64 : 2048;  // the FP64 mode defers every pixel
+   if (cp.emccd)
+      hipLaunchKernelGGL(noise_fallback_kernel<true>, dim3(grid), dim3(256), 0, h->stream, q, out, npx, frames,
+                         frameBase, offset, gain, readN, cp, seed, pixelBase);
+   else
+      hipLaunchKernelGGL(noise_fallback_kernel<false>, dim3(grid), dim3(256), 0, h->stream, q, out, npx, frames,
+                         frameBase, offset, gain, readN, cp, seed, pixelBase);
+   return ok(h, hipGetLastError(), "noise fallback launch");
+}
+
+// Launches the noise kernel on device arrays (shared with the splat path when noise runs as its own pass), in
+// batches of whole frames below 2^31 pixels so the launch's pixel index fits 32 bits.
 bool launch_noise(iscg_context* h, const float* photons, uint16_t* out, uint32_t npx, uint32_t nFrames,
                   const uint32_t* frames, const float* offset, const float* gain, const float* readN,
                   const CamParams& cp, uint32_t seed, uint32_t pixelBase)
 {
-   const size_t total = (size_t)npx * nFrames;
-   if (total == 0)
+   if ((size_t)npx * nFrames == 0)
       return true;
-   const unsigned block = 256, grid = grid_for(total, block, 16);
-   ProfScope ps(h, "noise");
-   if (cp.emccd)
-      hipLaunchKernelGGL(noise_kernel<true>, dim3(grid), dim3(block), 0, h->stream, photons, out, npx, nFrames, frames,
-                         offset, gain, readN, cp, seed, pixelBase);
-   else
-      hipLaunchKernelGGL(noise_kernel<false>, dim3(grid), dim3(block), 0, h->stream, photons, out, npx, nFrames,
-                         frames, offset, gain, readN, cp, seed, pixelBase);
-   return ok(h, hipGetLastError(), "noise kernel launch");
+   if (npx > (1u << 31))
+      return fail(h, "noise: more than 2^31 pixels per frame") == 0;
+   const uint32_t perLaunch = (uint32_t)std::min<size_t>(nFrames, ((size_t)1 << 31) / npx);
+   NoiseQueue q;
+   if (!noise_queue(h, (size_t)perLaunch * npx, q))
+      return false;
+   for (uint32_t f0 = 0; f0 < nFrames; f0 += perLaunch)
+   {
+      const uint32_t nf = std::min(perLaunch, nFrames - f0);
+      const float* ph = photons + (size_t)f0 * npx;
+      uint16_t* o = out + (size_t)f0 * npx;
+      const unsigned block = 256, grid = grid_for((size_t)nf * npx, block, 16);
+      {
+         ProfScope ps(h, "noise");
+         if (cp.emccd)
+            hipLaunchKernelGGL(noise_kernel<true>, dim3(grid), dim3(block), 0, h->stream, ph, o, npx, nf, frames + f0,
+                               offset, gain, readN, cp, seed, pixelBase, q);
+         else
+            hipLaunchKernelGGL(noise_kernel<false>, dim3(grid), dim3(block), 0, h->stream, ph, o, npx, nf,
+                               frames + f0, offset, gain, readN, cp, seed, pixelBase, q);
+         if (!ok(h, hipGetLastError(), "noise kernel launch"))
+            return false;
+      }
+      if (!launch_noise_fallback(h, q, o, npx, frames + f0, 0, offset, gain, readN, cp, seed, pixelBase))
+         return false;
+   }
+   return true;
 }
 
 // ---- verification of the FP32 path's error bounds (see noise_pixel_fast) -----------------------------------------

@@ -269,6 +269,7 @@ struct TileArgs
    uint16_t* outAdu;          // nFrames x npx or null
    float* outPhotons;         // nFrames x npx or null
    double flatBg;
+   NoiseQueue q;              // deferred noise pixels (index f * npx + i)
    uint32_t w, h, tilesX, nTiles, nFrames, nEmitters, seed, pixelBase;
 };
 
@@ -392,10 +393,10 @@ __global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelT
          if (a.outPhotons)
             a.outPhotons[(size_t)f * npx + i] = v;
          if (NOISE)
-            a.outAdu[(size_t)f * npx + i] =
-               noise_pixel_any<EMCCD>(cp, v, a.seed, a.frames[f], (uint32_t)i + a.pixelBase, a.offset != nullptr,
-                                  a.offset ? a.offset[i] : 0.0f, a.gain != nullptr, a.gain ? a.gain[i] : 0.0f,
-                                  a.readN != nullptr, a.readN ? a.readN[i] : 0.0f);
+            noise_store<EMCCD>(cp, a.q, a.outAdu, (uint32_t)((size_t)f * npx + i), v, a.seed, a.frames[f],
+                               (uint32_t)i + a.pixelBase, a.offset != nullptr, a.offset ? a.offset[i] : 0.0f,
+                               a.gain != nullptr, a.gain ? a.gain[i] : 0.0f, a.readN != nullptr,
+                               a.readN ? a.readN[i] : 0.0f);
       }
       __syncthreads();
    }
@@ -477,17 +478,30 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
    a.pixelBase = s.pixelBase;
    CamParams cp{};
    if (cam && !cam_params_for(h, cam, cp))
-      return false;
+      return -1;
+   a.q = NoiseQueue{nullptr, nullptr};
+   if (dAdu)
+   {
+      if ((size_t)nFrames * npx > 0xFFFFFFFFull)
+         return fail(h, "splat: a batch with noise holds at most 2^32 - 1 pixels");
+      if (!noise_queue(h, (size_t)nFrames * npx, a.q))
+         return -1;
+   }
    const size_t items = (size_t)a.nTiles * nFrames;
    const unsigned grid = (unsigned)(items < 256 * 16 ? items : 256 * 16);
-   ProfScope ps(h, "splat_tiles_noise");
-   if (!dAdu)
-      hipLaunchKernelGGL((splat_tiles_kernel<false, false>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
-   else if (cp.emccd)
-      hipLaunchKernelGGL((splat_tiles_kernel<true, true>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
-   else
-      hipLaunchKernelGGL((splat_tiles_kernel<true, false>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
-   if (!ok(h, hipGetLastError(), "splat launch"))
+   {
+      ProfScope ps(h, "splat_tiles_noise");
+      if (!dAdu)
+         hipLaunchKernelGGL((splat_tiles_kernel<false, false>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
+      else if (cp.emccd)
+         hipLaunchKernelGGL((splat_tiles_kernel<true, true>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
+      else
+         hipLaunchKernelGGL((splat_tiles_kernel<true, false>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
+      if (!ok(h, hipGetLastError(), "splat launch"))
+         return -1;
+   }
+   if (dAdu && !launch_noise_fallback(h, a.q, dAdu, (uint32_t)npx, dFrames, 0, a.offset, a.gain, a.readN, cp, seed,
+                                      s.pixelBase))
       return -1;
    if (outAdu && !finish_out(h, outAdu, dAdu, nFrames * npx * sizeof(uint16_t)))
       return -1;

@@ -238,6 +238,7 @@ struct C2rEpilogue
    const float* offset;   // [ch][cw] or null
    double background;
    CamParams cp;
+   NoiseQueue q;          // deferred pixels (index o = row * cw + x)
    uint32_t seed, frameBase, pixelBase;
    int ch, cw, x0;
 };
@@ -293,10 +294,9 @@ __global__ __launch_bounds__(256) void rows_c2r_kernel(const float2* __restrict_
             e.img[o] = ph;
          if (NOISE)
          {
-            const size_t pix = (size_t)y * e.cw + x;
-            e.adu[o] = noise_pixel_any<EMCCD>(e.cp, ph, e.seed, e.frameBase + (uint32_t)f, (uint32_t)pix + e.pixelBase,
-                                          e.offset != nullptr, e.offset ? e.offset[pix] : 0.0f, false, 0.0f, false,
-                                          0.0f);
+            const uint32_t pix = (uint32_t)(y * e.cw + x);
+            noise_store<EMCCD>(e.cp, e.q, e.adu, (uint32_t)o, ph, e.seed, e.frameBase + (uint32_t)f, pix + e.pixelBase,
+                               e.offset != nullptr, e.offset ? e.offset[pix] : 0.0f, false, 0.0f, false, 0.0f);
          }
       }
       __syncthreads();
@@ -617,12 +617,20 @@ int32_t iscg_wf_conv3d(iscg_context* h, const float* density, const iscg_camera*
    e.ch = (int)d.ch;
    e.cw = (int)d.cw;
    e.x0 = (int)d.fovX0;
+   e.q = NoiseQueue{nullptr, nullptr};
+   if (dAdu && nOut > 0xFFFFFFFFull)
+      return fail(h, "iscg_wf_conv3d: ADU output holds at most 2^32 - 1 pixels");
+   if (dAdu && !noise_queue(h, nOut, e.q))
+      return -1;
    {
       ProfScope ps(h, "wf_rows_c2r_noise");
       dispatch_n<RowsC2R, 1024>((int)d.NX, h->stream, (const float2*)x2, S, (int)(d.nF * d.ch), e,
                                 s.twX.as<const float2>());
    }
    if (!ok(h, hipGetLastError(), "iscg_wf_conv3d launch"))
+      return -1;
+   if (dAdu && !launch_noise_fallback(h, e.q, dAdu, d.ch * d.cw, nullptr, frameBase, dOff, nullptr, nullptr, e.cp,
+                                      seed, pixelBase))
       return -1;
    if (outImages && !finish_out(h, outImages, dImg, nOut * sizeof(float)))
       return -1;
