@@ -231,6 +231,62 @@ int32_t iscg_wf_conv3d(iscg_context* h, const float* density, const iscg_camera*
                        double background, uint32_t seed, uint32_t frameBase, uint32_t pixelBase, float* outImages,
                        uint16_t* outAdu);
 
+/* ---- mean field, one focus per job: the WidefieldAccelerator seam (Sim/WidefieldRender.h:306-361) ---------- */
+
+/* WidefieldGpuJob::Dep (Sim/WidefieldRender.h:325-331): the channel's spectrum gains the plane's times
+ * (w0 K[p0] + w1 K[p1]) (p1, w1 only when two). */
+typedef struct
+{
+   uint64_t key;
+   int32_t p0, p1;
+   float w0, w1;
+   int32_t two;
+   int32_t reserved;
+} iscg_wf_dep;
+
+/* WidefieldGpuJob (Sim/WidefieldRender.h:306-340) flattened into arrays. The index arrays (kernelP, kernelSpec,
+ * planeKey, planeStart, channelStart, deps) are host memory; the spectra each kernelSpec entry points to, cells and
+ * values may be host or device memory. A plane's cells are distinct (the scene's sparse planes are). */
+typedef struct
+{
+   uint32_t NX, NY;               /* FFT grid: powers of two, NX in [16, 1024], NY in [16, 512] */
+   uint32_t nx, ny;               /* dye grid (<= NX, NY); cell = y * nx + x */
+   uint32_t fovX0, fovY0, cw, ch; /* the FOV cells in the grid (fovX0 + cw <= NX, fovY0 + ch <= NY) */
+   double fracX, fracY;           /* sub-cell shift */
+   uint64_t geometry;             /* keys and kernel planes are valid within one geometry value */
+   int32_t nKernels, nPlanes, nChannels, hasPersistent;
+   const int32_t* kernelP;        /* [nKernels] kernel plane p */
+   const float* const* kernelSpec; /* [nKernels] half spectra: NY rows of NX/2 + 1 complex (re, im) floats */
+   const uint64_t* planeKey;      /* [nPlanes] the planes to transform and keep resident */
+   const uint32_t* planeStart;    /* [nPlanes + 1] offsets into cells / values, planeStart[0] = 0 */
+   const uint32_t* cells;
+   const float* values;           /* count x weight */
+   const uint32_t* channelStart;  /* [nChannels + 1] offsets into deps */
+   const iscg_wf_dep* deps;
+} iscg_wf_job;
+
+/* WidefieldAccelerator::HasPlane (Sim/WidefieldRender.h:349): 1 when the spectrum of key in geometry is resident
+ * (its dyes are then left out of the job), 0 when not, -1 on error. */
+int32_t iscg_wf_has_plane(iscg_context* h, uint64_t geometry, uint64_t key);
+
+/* WidefieldAccelerator::Fits (Sim/WidefieldRender.h:355): 1 when `planes` plane spectra of an NX x NY grid fit in
+ * the plane budget beside the resident spectra of the other geometries the context serves, else 0 (the scene then
+ * renders on the CPU); -1 on error. A 1 counts the geometry as served. */
+int32_t iscg_wf_fits(iscg_context* h, uint64_t geometry, uint32_t NX, uint32_t NY, uint64_t planes);
+
+/* WidefieldAccelerator::Images (Sim/WidefieldRender.h:351; the algorithm of the GPU hosts, restated on the CPU as
+ * JobReferenceHost, cli/widefield_check.cpp:816-902): the job's planes are transformed and kept resident by key
+ * (kernel spectra by p, within the geometry); per channel, in dep order, S += A_key (w0 K_p0 + w1 K_p1) in complex
+ * FP32; the sub-cell phase ramp exp(2 pi i (kx fracX / NX + ky fracY / NY)) (cosine only at the Nyquist bins);
+ * the inverse transform, normalised, cropped to the FOV. out: nChannels x ch x cw floats. A dep whose plane or
+ * kernel is neither carried nor resident fails the call, as the reference's host does. */
+int32_t iscg_wf_images(iscg_context* h, const iscg_wf_job* job, float* out);
+
+/* The device memory the resident plane and kernel spectra of all geometries may take (default: half of the
+ * device's). When a job needs more, the least recently used other geometries are dropped first. No reference
+ * counterpart (the reference's hosts size their pools themselves). */
+int32_t iscg_wf_set_budget(iscg_context* h, uint64_t bytes);
+
 /* ---- differentiable ROI model (new: no reference counterpart) ---------------------------------------------------- */
 
 /* The optics of the ROIs (the fields of a PsfGeneratorRequest that the pupil uses) and the ROI sampling. */

@@ -39,6 +39,25 @@ class WfDesc(C.Structure):
                 ("NY", C.c_uint32), ("NZ", C.c_uint32), ("upscale", C.c_int32)]
 
 
+class WfDep(C.Structure):
+    """iscg_wf_dep: WidefieldGpuJob::Dep (Sim/WidefieldRender.h:325-331)."""
+
+    _fields_ = [("key", C.c_uint64), ("p0", C.c_int32), ("p1", C.c_int32), ("w0", C.c_float), ("w1", C.c_float),
+                ("two", C.c_int32), ("reserved", C.c_int32)]
+
+
+class WfJob(C.Structure):
+    """iscg_wf_job: WidefieldGpuJob (Sim/WidefieldRender.h:306-340) flattened."""
+
+    _fields_ = [("NX", C.c_uint32), ("NY", C.c_uint32), ("nx", C.c_uint32), ("ny", C.c_uint32),
+                ("fovX0", C.c_uint32), ("fovY0", C.c_uint32), ("cw", C.c_uint32), ("ch", C.c_uint32),
+                ("fracX", C.c_double), ("fracY", C.c_double), ("geometry", C.c_uint64), ("nKernels", C.c_int32),
+                ("nPlanes", C.c_int32), ("nChannels", C.c_int32), ("hasPersistent", C.c_int32),
+                ("kernelP", C.c_void_p), ("kernelSpec", C.c_void_p), ("planeKey", C.c_void_p),
+                ("planeStart", C.c_void_p), ("cells", C.c_void_p), ("values", C.c_void_p),
+                ("channelStart", C.c_void_p), ("deps", C.c_void_p)]
+
+
 class RoiModel(C.Structure):
     """iscg_roi_model."""
 
@@ -75,6 +94,10 @@ SIGNATURES = {
     "iscg_profile_enable": (I32, [P, I32]),
     "iscg_profile_read": (I32, [P, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), I32, I32]),
     "iscg_set_stream": (I32, [P, P]),
+    "iscg_wf_has_plane": (I32, [P, C.c_uint64, C.c_uint64]),
+    "iscg_wf_fits": (I32, [P, C.c_uint64, U32, U32, C.c_uint64]),
+    "iscg_wf_images": (I32, [P, C.POINTER(WfJob), P]),
+    "iscg_wf_set_budget": (I32, [P, C.c_uint64]),
     "iscg_noise_set_path": (I32, [P, I32]),
     "iscg_noise_bounds": (I32, [P, P]),
     "iscg_noise": (I32, [P, P, U32, U32, C.POINTER(Camera), P, P, P, U32, P, U32, U32, P]),

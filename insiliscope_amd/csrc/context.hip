@@ -9,6 +9,7 @@ void destroy_splat(iscg_context* h);
 void destroy_psf(iscg_context* h);
 void destroy_wf(iscg_context* h);
 void destroy_roi(iscg_context* h);
+void destroy_wf_accel(iscg_context* h);
 
 int32_t fail(iscg_context* h, const std::string& msg)
 {
@@ -190,6 +191,7 @@ void iscg_destroy(iscg_context* h)
    destroy_splat(h);
    destroy_psf(h);
    destroy_wf(h);
+   destroy_wf_accel(h);
    destroy_roi(h);
    for (auto& b : h->stage)
       b.release();

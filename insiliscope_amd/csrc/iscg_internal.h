@@ -58,6 +58,7 @@ struct DevBuf
 struct SplatState;
 struct PsfState;
 struct WfState;
+struct WfAccelState;
 struct RoiState;
 
 }  // namespace iscg
@@ -85,6 +86,7 @@ struct iscg_context
    iscg::SplatState* splat = nullptr;
    iscg::PsfState* psf = nullptr;
    iscg::WfState* wf = nullptr;
+   iscg::WfAccelState* wfAccel = nullptr;
    iscg::RoiState* roi = nullptr;
    int noiseExact = 0;  // iscg_noise_set_path: 1 evaluates every pixel with the FP64 chain
    iscg::DevBuf noiseItems;            // deferred pixels (NoiseQueue::items)

@@ -533,6 +533,109 @@ class WidefieldStack:
                                                     pixelBase, ptrs[2][0], ptrs[3][0]))
 
 
+@dataclass
+class WidefieldGpuJob:
+    """sim::WidefieldGpuJob (Sim/WidefieldRender.h:306-340): the GPU's share of one focus.
+
+    kernels: [(p, spec)] with spec the (NY, NX/2 + 1) complex64 half spectrum of kernel plane p;
+    planes: [(key, cells, values)] sparse dye planes (cell = y * nx + x) the host does not hold yet;
+    channels: per channel a list of deps (key, p0, p1, w0, w1, two)."""
+
+    NX: int = 0
+    NY: int = 0
+    nx: int = 0
+    ny: int = 0
+    fovX0: int = 0
+    fovY0: int = 0
+    cw: int = 0
+    ch: int = 0
+    fracX: float = 0.0
+    fracY: float = 0.0
+    geometry: int = 0
+    kernels: list = field(default_factory=list)
+    planes: list = field(default_factory=list)
+    channels: list = field(default_factory=list)
+    hasPersistent: bool = False
+    kMaxGpuFft = 2048
+
+
+class WidefieldGpuHost:
+    """A WidefieldAccelerator (Sim/WidefieldRender.h:342-361) on libiscg: plane spectra resident in HBM by key per
+    geometry, kernel spectra per (geometry, p); Images runs the per-focus MAC, phase ramp and cropped inverse on the
+    GPU. Methods return bool with the message in .err, as the reference's do."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or Context()
+        self.err = ""
+
+    def HasPlane(self, geometry: int, key: int) -> bool:
+        rc = self.ctx.L.iscg_wf_has_plane(self.ctx.h, geometry, key)
+        if rc < 0:
+            raise IscgError(self.ctx.error())
+        return rc == 1
+
+    def Fits(self, geometry: int, NX: int, NY: int, planes: int) -> bool:
+        rc = self.ctx.L.iscg_wf_fits(self.ctx.h, geometry, NX, NY, planes)
+        if rc < 0:
+            raise IscgError(self.ctx.error())
+        return rc == 1
+
+    def SetBudget(self, nbytes: int):
+        self.ctx.check(self.ctx.L.iscg_wf_set_budget(self.ctx.h, int(nbytes)))
+
+    def Images(self, job: WidefieldGpuJob, out: list | None = None, outArray=None) -> bool:
+        """One (ch, cw) float32 image per job channel, appended to `out` (numpy), or written to outArray
+        (nChannels, ch, cw) (numpy, or a device tensor left on the device)."""
+        keep = []
+        kp = np.array([k[0] for k in job.kernels], np.int32)
+        specs = []
+        for _, sp in job.kernels:
+            a = sp if hasattr(sp, "data_ptr") else np.ascontiguousarray(sp, np.complex64)
+            p, k = _ptr(a)
+            specs.append(p)
+            keep.append(k)
+        spec_ptrs = (C.c_void_p * max(1, len(specs)))(*specs)
+        keys = np.array([p[0] for p in job.planes], np.uint64)
+        cells = [np.ascontiguousarray(p[1], np.uint32).ravel() for p in job.planes]
+        vals = [np.ascontiguousarray(p[2], np.float32).ravel() for p in job.planes]
+        starts = np.zeros(len(job.planes) + 1, np.uint32)
+        if job.planes:
+            starts[1:] = np.cumsum([len(c) for c in cells])
+        cat_c = np.concatenate(cells) if cells else np.zeros(0, np.uint32)
+        cat_v = np.concatenate(vals) if vals else np.zeros(0, np.float32)
+        deps = [d for ch in job.channels for d in ch]
+        dep_arr = (_lib.WfDep * max(1, len(deps)))()
+        for i, (key, p0, p1, w0, w1, two) in enumerate(deps):
+            dep_arr[i] = _lib.WfDep(int(key), int(p0), int(p1), float(w0), float(w1), int(bool(two)), 0)
+        cstart = np.zeros(len(job.channels) + 1, np.uint32)
+        if job.channels:
+            cstart[1:] = np.cumsum([len(ch) for ch in job.channels])
+        nC = len(job.channels)
+        if outArray is None:
+            outArray = np.zeros((nC, job.ch, job.cw), np.float32)
+        po, ko = _ptr(outArray)
+        J = _lib.WfJob(NX=job.NX, NY=job.NY, nx=job.nx, ny=job.ny, fovX0=job.fovX0, fovY0=job.fovY0, cw=job.cw,
+                       ch=job.ch, fracX=job.fracX, fracY=job.fracY, geometry=job.geometry,
+                       nKernels=len(job.kernels), nPlanes=len(job.planes), nChannels=nC,
+                       hasPersistent=int(bool(job.hasPersistent)), kernelP=kp.ctypes.data,
+                       kernelSpec=C.cast(spec_ptrs, C.c_void_p), planeKey=keys.ctypes.data,
+                       planeStart=starts.ctypes.data, cells=cat_c.ctypes.data, values=cat_v.ctypes.data,
+                       channelStart=cstart.ctypes.data, deps=C.cast(dep_arr, C.c_void_p))
+        self.ctx.follow_torch()
+        rc = self.ctx.L.iscg_wf_images(self.ctx.h, C.byref(J), po)
+        self.err = "" if rc == 0 else self.ctx.error()
+        if rc != 0:
+            return False
+        if out is not None:
+            out.clear()
+            if isinstance(outArray, np.ndarray):
+                out.extend(outArray[c] for c in range(nC))
+            else:
+                out.extend(outArray[c] for c in range(nC))
+        del keep, ko
+        return True
+
+
 # ---- differentiable ROI model -------------------------------------------------------------------------------------
 
 ROI_PARAMS_DTYPE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("photons", "f8"), ("background", "f8")])

@@ -335,6 +335,9 @@ class Reference:
         L.ref_psf_library.restype = I32
         L.ref_wf_focus_series.argtypes = [PF, I32, I32, I32, D, PF, I32, I32, I32, D, D, C.POINTER(D), I32, PF]
         L.ref_wf_focus_series.restype = I32
+        L.ref_wf_capture_jobs.argtypes = [PF, I32, I32, I32, D, PF, I32, I32, I32, D, D, C.POINTER(D), I32, D, D,
+                                          C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), PF]
+        L.ref_wf_capture_jobs.restype = I32
 
     def psf_planes(self, req, size, nz):
         out = np.zeros((nz, size, size), np.float32)
@@ -451,9 +454,180 @@ class Reference:
             raise RuntimeError("WidefieldScene failed")
         return out
 
+    def wf_capture_jobs(self, density, dz_nm, planes, os_, z_step_nm, foci_um, dx_um=0.0, dy_um=0.0,
+                        kernel_cap_um=12.8):
+        """The WidefieldGpuJobs a GPU-mode WidefieldScene sends its accelerator over a focus series (planes resident
+        after the first job), and that scene's own CPU images: (jobs, (nF, ny, nx) images)."""
+        density = np.ascontiguousarray(density, np.float32)
+        planes = np.ascontiguousarray(planes, np.float32)
+        foci = np.ascontiguousarray(foci_um, np.float64)
+        nz, ny, nx = density.shape
+        out = np.zeros((len(foci), ny, nx), np.float32)
+        cap = 1 << 28
+        blob = np.zeros(cap, np.uint8)
+        used = C.c_size_t(0)
+        r = self.L.ref_wf_capture_jobs(_p(density, C.c_float), nx, ny, nz, dz_nm, _p(planes, C.c_float),
+                                       planes.shape[0], planes.shape[1], os_, z_step_nm, kernel_cap_um,
+                                       _p(foci, C.c_double), len(foci), dx_um, dy_um, blob.ctypes.data, cap,
+                                       C.byref(used), _p(out, C.c_float))
+        if r < 0:
+            raise RuntimeError(f"WidefieldScene job capture failed ({r})")
+        return parse_wf_jobs(blob[:used.value].tobytes(), r), out
+
     def psf_library(self, reqs, out=None):
         arr = (PsfRequest * len(reqs))(*reqs)
         return self.L.ref_psf_library(arr, len(reqs), _p(out, C.c_float))
+
+
+# ---- WidefieldGpuJob (Sim/WidefieldRender.h:306-340): parsing captured jobs and the CPU statement of the hosts --
+
+def parse_wf_jobs(blob: bytes, n: int) -> list:
+    """Jobs in ref_wf_capture_jobs's byte layout -> namespaces with WidefieldGpuJob's fields (kernels [(p, spec)],
+    planes [(key, cells, values)], channels [[(key, p0, p1, w0, w1, two)]])."""
+    from types import SimpleNamespace
+    jobs, o = [], 0
+
+    def take(fmt_dtype, count=1):
+        nonlocal o
+        a = np.frombuffer(blob, dtype=fmt_dtype, count=count, offset=o)
+        o += a.nbytes
+        return a
+
+    for _ in range(n):
+        NX, NY, nx, ny, fovX0, fovY0, cw, ch = (int(v) for v in take("<u4", 8))
+        fracX, fracY = (float(v) for v in take("<f8", 2))
+        geometry = int(take("<u8")[0])
+        nK, nP, nC, hasP = (int(v) for v in take("<i4", 4))
+        W = NX // 2 + 1
+        kernels = []
+        for _k in range(nK):
+            p = int(take("<i4")[0])
+            spec = take("<c8", W * NY).reshape(NY, W).copy()
+            kernels.append((p, spec))
+        planes = []
+        for _p_ in range(nP):
+            key = int(take("<u8")[0])
+            cnt = int(take("<u4")[0])
+            cells = take("<u4", cnt).copy()
+            vals = take("<f4", cnt).copy()
+            planes.append((key, cells, vals))
+        channels = []
+        for _c in range(nC):
+            nd = int(take("<u4")[0])
+            deps = []
+            for _d in range(nd):
+                key = int(take("<u8")[0])
+                p0, p1 = (int(v) for v in take("<i4", 2))
+                w0, w1 = (float(v) for v in take("<f4", 2))
+                two = bool(take("<i4")[0])
+                deps.append((key, p0, p1, w0, w1, two))
+            channels.append(deps)
+        jobs.append(SimpleNamespace(NX=NX, NY=NY, nx=nx, ny=ny, fovX0=fovX0, fovY0=fovY0, cw=cw, ch=ch,
+                                    fracX=fracX, fracY=fracY, geometry=geometry, kernels=kernels, planes=planes,
+                                    channels=channels, hasPersistent=bool(hasP)))
+    if o != len(blob):
+        raise ValueError("trailing bytes in the job blob")
+    return jobs
+
+
+def pack_wf_jobs(jobs, prefix: str, out: dict) -> None:
+    """Jobs -> flat arrays under `prefix` in an npz dict (kernel spectra stored once per p: fixed in a geometry)."""
+    out[prefix + "n"] = np.array(len(jobs))
+    specs = {}
+    for i, j in enumerate(jobs):
+        q = f"{prefix}{i}_"
+        out[q + "u32"] = np.array([j.NX, j.NY, j.nx, j.ny, j.fovX0, j.fovY0, j.cw, j.ch], np.uint32)
+        out[q + "f64"] = np.array([j.fracX, j.fracY])
+        out[q + "u64"] = np.array([j.geometry], np.uint64)
+        out[q + "hasPersistent"] = np.array(int(j.hasPersistent))
+        out[q + "kp"] = np.array([k[0] for k in j.kernels], np.int32)
+        for p, spec in j.kernels:
+            if p in specs and not np.array_equal(specs[p], spec):
+                raise ValueError("kernel spectrum changed within a geometry")
+            specs[p] = spec
+        out[q + "keys"] = np.array([pl[0] for pl in j.planes], np.uint64)
+        out[q + "starts"] = np.concatenate([[0], np.cumsum([len(pl[1]) for pl in j.planes])]).astype(np.uint32)
+        out[q + "cells"] = np.concatenate([pl[1] for pl in j.planes] or [np.zeros(0, np.uint32)]).astype(np.uint32)
+        out[q + "values"] = np.concatenate([pl[2] for pl in j.planes] or [np.zeros(0, np.float32)]).astype(np.float32)
+        deps = [d for ch in j.channels for d in ch]
+        out[q + "dep_keys"] = np.array([d[0] for d in deps], np.uint64)
+        out[q + "dep_p"] = np.array([[d[1], d[2], int(d[5])] for d in deps], np.int32).reshape(-1, 3)
+        out[q + "dep_w"] = np.array([[d[3], d[4]] for d in deps], np.float32).reshape(-1, 2)
+        out[q + "chan"] = np.concatenate([[0], np.cumsum([len(ch) for ch in j.channels])]).astype(np.uint32)
+    for p, spec in specs.items():
+        out[f"{prefix}kernel{p}"] = spec
+
+
+def unpack_wf_jobs(npz, prefix: str) -> list:
+    from types import SimpleNamespace
+    jobs = []
+    for i in range(int(npz[prefix + "n"])):
+        q = f"{prefix}{i}_"
+        NX, NY, nx, ny, fovX0, fovY0, cw, ch = (int(v) for v in npz[q + "u32"])
+        fracX, fracY = (float(v) for v in npz[q + "f64"])
+        kernels = [(int(p), npz[f"{prefix}kernel{int(p)}"]) for p in npz[q + "kp"]]
+        st, cells, vals = npz[q + "starts"], npz[q + "cells"], npz[q + "values"]
+        planes = [(int(k), cells[st[a]:st[a + 1]], vals[st[a]:st[a + 1]]) for a, k in enumerate(npz[q + "keys"])]
+        dk, dp, dw, cs = npz[q + "dep_keys"], npz[q + "dep_p"], npz[q + "dep_w"], npz[q + "chan"]
+        deps = [(int(dk[t]), int(dp[t, 0]), int(dp[t, 1]), float(dw[t, 0]), float(dw[t, 1]), bool(dp[t, 2]))
+                for t in range(len(dk))]
+        channels = [deps[cs[c]:cs[c + 1]] for c in range(len(cs) - 1)]
+        jobs.append(SimpleNamespace(NX=NX, NY=NY, nx=nx, ny=ny, fovX0=fovX0, fovY0=fovY0, cw=cw, ch=ch,
+                                    fracX=fracX, fracY=fracY, geometry=int(npz[q + "u64"][0]), kernels=kernels,
+                                    planes=planes, channels=channels,
+                                    hasPersistent=bool(npz[q + "hasPersistent"])))
+    return jobs
+
+
+class JobReferenceHost:
+    """The GPU hosts' per-focus algorithm (restated from JobReferenceHost, cli/widefield_check.cpp:816-902) in FP64:
+    plane spectra kept by key within one geometry; per channel S = sum_d A (w0 K0 + w1 K1); the phase ramp of
+    ApplyShiftRamp (Sim/DriftShift.cpp:24-48), cosine only at the Nyquist bins; the normalised inverse, cropped."""
+
+    def __init__(self):
+        self.geometry = None
+        self.shape = None
+        self.planes = {}
+        self.jobPlanes = 0
+
+    def HasPlane(self, geometry, key) -> bool:
+        return geometry == self.geometry and key in self.planes
+
+    def Images(self, job) -> list:
+        NX, NY = job.NX, job.NY
+        if job.geometry != self.geometry or self.shape != (NX, NY):
+            self.planes = {}
+            self.geometry = job.geometry
+            self.shape = (NX, NY)
+        for key, cells, values in job.planes:
+            img = np.zeros((NY, NX))
+            cells = np.asarray(cells, np.int64)
+            img[cells // job.nx, cells % job.nx] = np.asarray(values, np.float64)
+            self.planes[key] = np.fft.rfft2(img)
+            self.jobPlanes += 1
+        K = {p: np.asarray(spec, np.complex128) for p, spec in job.kernels}
+        W = NX // 2 + 1
+        kx = np.arange(W)
+        ky = np.arange(NY)
+        kys = np.where(ky <= NY // 2, ky, ky - NY).astype(np.float64)
+        ax = 2 * np.pi * kx * job.fracX / NX
+        ay = 2 * np.pi * kys * job.fracY / NY
+        px = np.where(kx == NX // 2, np.cos(ax) + 0j, np.exp(1j * ax))
+        py = np.where(ky == NY // 2, np.cos(ay) + 0j, np.exp(1j * ay))
+        out = []
+        for deps in job.channels:
+            acc = np.zeros((NY, W), np.complex128)
+            for key, p0, p1, w0, w1, two in deps:
+                if key not in self.planes or p0 not in K or (two and p1 not in K):
+                    raise KeyError("job refers to a plane or kernel it does not carry")
+                k = w0 * K[p0]
+                if two:
+                    k = k + w1 * K[p1]
+                acc += self.planes[key] * k
+            acc *= py[:, None] * px[None, :]
+            img = np.fft.irfft2(acc, s=(NY, NX))
+            out.append(img[job.fovY0:job.fovY0 + job.ch, job.fovX0:job.fovX0 + job.cw])
+        return out
 
 
 def _i32(v):

@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -397,6 +398,164 @@ int32_t ref_wf_focus_series(const float* density, int32_t nx, int32_t ny, int32_
       std::memcpy(out + static_cast<size_t>(f) * nx * ny, im.persistent.data(), im.persistent.size() * sizeof(float));
    }
    return 1;
+}
+
+} // extern "C"
+
+namespace {
+
+// A WidefieldAccelerator that records each job the scene sends (in the byte layout documented at
+// ref_wf_capture_jobs) and answers with zero images; planes count as resident once a job has carried them.
+class CaptureHost : public sim::WidefieldAccelerator
+{
+public:
+   bool HasPlane(unsigned long long geometry, unsigned long long key) const override
+   {
+      return geometry == geometry_ && keys_.count(key) != 0;
+   }
+   bool Images(const sim::WidefieldGpuJob& job, std::vector<std::vector<float>>& out, std::string&) override
+   {
+      if (job.geometry != geometry_)
+      {
+         keys_.clear();
+         geometry_ = job.geometry;
+      }
+      const unsigned W = job.NX / 2 + 1;
+      for (uint32_t v : {job.NX, job.NY, job.nx, job.ny, job.fovX0, job.fovY0, job.cw, job.ch})
+         Put(v);
+      Put(job.fracX);
+      Put(job.fracY);
+      Put(static_cast<uint64_t>(job.geometry));
+      Put(static_cast<int32_t>(job.kernels.size()));
+      Put(static_cast<int32_t>(job.planes.size()));
+      Put(static_cast<int32_t>(job.channels.size()));
+      Put(static_cast<int32_t>(job.hasPersistent));
+      for (const auto& k : job.kernels)
+      {
+         Put(static_cast<int32_t>(k.p));
+         if (k.spec->size() != static_cast<size_t>(W) * job.NY)
+            return false;
+         Bytes(k.spec->data(), k.spec->size() * sizeof(sim::cfloat));
+      }
+      for (const auto& p : job.planes)
+      {
+         Put(static_cast<uint64_t>(p.key));
+         Put(static_cast<uint32_t>(p.cells.size()));
+         Bytes(p.cells.data(), p.cells.size() * sizeof(uint32_t));
+         Bytes(p.values.data(), p.values.size() * sizeof(float));
+         keys_.insert(p.key);
+      }
+      for (const auto& ch : job.channels)
+      {
+         Put(static_cast<uint32_t>(ch.size()));
+         for (const auto& d : ch)
+         {
+            Put(static_cast<uint64_t>(d.key));
+            Put(static_cast<int32_t>(d.p0));
+            Put(static_cast<int32_t>(d.p1));
+            Put(d.w0);
+            Put(d.w1);
+            Put(static_cast<int32_t>(d.two));
+         }
+      }
+      out.assign(job.channels.size(), std::vector<float>(static_cast<size_t>(job.cw) * job.ch, 0.0f));
+      ++jobs;
+      return true;
+   }
+   std::vector<uint8_t> blob;
+   int jobs = 0;
+
+private:
+   template <typename T>
+   void Put(T v)
+   {
+      Bytes(&v, sizeof v);
+   }
+   void Bytes(const void* p, size_t n)
+   {
+      const uint8_t* b = static_cast<const uint8_t*>(p);
+      blob.insert(blob.end(), b, b + n);
+   }
+   unsigned long long geometry_ = ~0ull;
+   std::set<unsigned long long> keys_;
+};
+
+} // namespace
+
+extern "C" {
+
+// The jobs a GPU-mode WidefieldScene sends its accelerator for FocusSeries over the foci (one per focus, the
+// planes resident after the first), and the same GPU-mode scene's own CPU images: a persistent population
+// (density [nzD][ny][nx]) under flat illumination, the FOV's origin moved by (dxUm, dyUm) off the cell edges.
+// Job layout (little-endian, packed): u32 NX NY nx ny fovX0 fovY0 cw ch; f64 fracX fracY; u64 geometry;
+// i32 nKernels nPlanes nChannels hasPersistent; per kernel i32 p + (NX/2+1) NY complex f32; per plane u64 key,
+// u32 n, u32 cells[n], f32 values[n]; per channel u32 nDeps, per dep u64 key, i32 p0 p1, f32 w0 w1, i32 two.
+// Returns the number of jobs (-1 on failure, -2 when blobCap is too small); *blobUsed = bytes written.
+int32_t ref_wf_capture_jobs(const float* density, int32_t nx, int32_t ny, int32_t nzD, double dzNm,
+                            const float* planes, int32_t nzP, int32_t size, int32_t os, double zStepNm,
+                            double kernelCapUm, const double* focusUm, int32_t nF, double dxUm, double dyUm,
+                            uint8_t* blob, size_t blobCap, size_t* blobUsed, float* outCpu)
+{
+   sim::PsfKernelCache c = MakeCache(planes, nzP, size, os, zStepNm, 0);
+   sim::KernelWidefieldPsf psf(c, 1);
+   sim::WidefieldSceneSpec spec;
+   spec.pixelUm = 0.1;
+   spec.originXUm = 0.05 + dxUm;
+   spec.originYUm = 0.05 + dyUm;
+   spec.width = static_cast<unsigned>(nx);
+   spec.height = static_cast<unsigned>(ny);
+   spec.focusWorldUm = focusUm[0];
+   spec.slabHalfUm = 0.0;
+   spec.grid.upscale = 1;
+   spec.grid.zPlaneNm = dzNm;
+   spec.marginUm = 0.0;
+   spec.kernelCapUm = kernelCapUm;
+   spec.eta = 1.0;
+   spec.unitDose = 1.0;
+   sim::FlatIllumination ill(nx * 0.1, ny * 0.1);
+   unsigned fx = 0, fy = 0;
+   double frx = 0, fry = 0;
+   const sim::WidefieldGridSpec g = sim::WidefieldScene::GridSpecFor(ill, spec, &fx, &fy, &frx, &fry);
+   sim::WidefieldDyeGrid G;
+   G.spec = g;
+   G.k0 = 0;
+   G.nz = static_cast<unsigned>(nzD);
+   const size_t gp = static_cast<size_t>(g.nx) * g.ny;
+   G.persistent.assign(gp * nzD, 0.0f);
+   double total = 0.0;
+   for (int z = 0; z < nzD; ++z)
+      for (int y = 0; y < ny; ++y)
+         for (int x = 0; x < nx; ++x)
+         {
+            const float v = density[(static_cast<size_t>(z) * ny + y) * nx + x];
+            G.persistent[z * gp + static_cast<size_t>(fy + y) * g.nx + fx + x] = v;
+            total += v;
+         }
+   G.nPersistent = static_cast<long>(total);
+   CaptureHost host;
+   sim::WidefieldScene gs, cs;
+   gs.SetGpuMode(true);
+   cs.SetGpuMode(true);
+   gs.SetAccelerator(&host);
+   std::string err;
+   if (!gs.UpdateFromGrid(G, ill, spec, psf, err) || !cs.UpdateFromGrid(G, ill, spec, psf, err))
+      return -1;
+   std::vector<double> foci(focusUm, focusUm + nF);
+   std::vector<sim::WidefieldImages> gi, ci;
+   if (!gs.FocusSeries(foci, gi, err) || !cs.FocusSeries(foci, ci, err) || !gs.UsingAccelerator())
+      return -1;
+   for (int f = 0; f < nF; ++f)
+   {
+      const sim::WidefieldImages& im = ci[static_cast<size_t>(f)];
+      if (im.persistent.size() != static_cast<size_t>(nx) * ny)
+         return -1;
+      std::memcpy(outCpu + static_cast<size_t>(f) * nx * ny, im.persistent.data(), im.persistent.size() * sizeof(float));
+   }
+   *blobUsed = host.blob.size();
+   if (host.blob.size() > blobCap)
+      return -2;
+   std::memcpy(blob, host.blob.data(), host.blob.size());
+   return host.jobs;
 }
 
 } // extern "C"

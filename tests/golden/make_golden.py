@@ -14,6 +14,8 @@ Outputs (inputs and expected outputs only):
   splat_ref.npz      CollectFrameEmitters, CollectGpuEmitters and RenderPhotonImage frames (Nearest, Linear, Cubic,
                      halo cut, edge emitters) on seeded events
   wf_ref.npz         KernelWidefieldPsf radius and grid-pitch planes; WidefieldScene::FocusSeries images
+  wf_jobs_ref.npz    the WidefieldGpuJobs of GPU-mode scenes (as sent to a WidefieldAccelerator) and the scenes' own
+                     CPU images
 """
 from __future__ import annotations
 
@@ -166,6 +168,20 @@ def wf(R, P):
     out["scene_foci"] = np.array([(k + 0.5) * 0.05 for k in range(7)] + [0.01, 0.13, 0.31])
     out["scene_images"] = R.wf_focus_series(out["scene_density"], 50.0, planes, 4, 50.0, out["scene_foci"])
     np.savez_compressed(os.path.join(HERE, "wf_ref.npz"), **out)
+    # The WidefieldGpuJobs a GPU-mode scene sends its accelerator (the first at the spec's focus, then one per focus
+    # of the series, planes resident after the first) and the same scene's own CPU images, for two scenes: the one
+    # above with the FOV 0.0123 / -0.031 um off the cell edges, and a 100 x 90 x 8 one (a 128-point grid)
+    jobs = {}
+    foci = np.array([0.125, 0.02, 0.21, 0.2003])
+    js, img = R.wf_capture_jobs(out["scene_density"], 50.0, planes, 4, 50.0, foci, dx_um=0.0123, dy_um=-0.031)
+    O.pack_wf_jobs(js, "a_", jobs)
+    jobs["a_foci"], jobs["a_images"] = foci, img
+    dens = np.where(rng.random((8, 90, 100)) < 0.03, 1.0 + np.floor(rng.random((8, 90, 100)) * 3), 0.0)
+    foci = np.array([0.2, 0.0, 0.33, 0.41])
+    js, img = R.wf_capture_jobs(dens.astype(np.float32), 50.0, planes, 4, 50.0, foci, dx_um=-0.047, dy_um=0.022)
+    O.pack_wf_jobs(js, "b_", jobs)
+    jobs["b_foci"], jobs["b_images"] = foci, img
+    np.savez_compressed(os.path.join(HERE, "wf_jobs_ref.npz"), **jobs)
 
 
 def main():

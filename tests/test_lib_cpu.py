@@ -47,6 +47,38 @@ def test_struct_layouts():
     assert host.SPLAT_EMITTER_DTYPE.itemsize == 64
 
 
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes struct has the size and field offsets of its C typedef (gcc compiles a probe of iscg.h)."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not found")
+    pairs = {"iscg_camera": _lib.Camera, "iscg_kernel_view": _lib.KernelView, "iscg_wf_desc": _lib.WfDesc,
+             "iscg_wf_dep": _lib.WfDep, "iscg_wf_job": _lib.WfJob, "iscg_roi_model": _lib.RoiModel,
+             "iscg_psf_request": _lib.PsfRequest}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "iscg.h"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'printf("{cname} <sizeof> %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run([gcc, "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, what, val = line.split()
+        got[(cname, what)] = int(val)
+    for cname, py in pairs.items():
+        assert got[(cname, "<sizeof>")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
 def splat_fixture(golden):
     G, P = golden("splat_ref.npz"), golden("psf_ref.npz")
     ev = G["events"].view(host.BLINK_EVENT_DTYPE).reshape(-1)

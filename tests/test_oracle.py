@@ -362,6 +362,69 @@ def test_wf_direct_is_linear(orc):
     assert np.allclose(ia + ib, iab, rtol=1e-12, atol=1e-12)
 
 
+# ---- the per-focus job seam (WidefieldAccelerator): JobReferenceHost restated ------------------------------------
+
+def test_wf_job_oracle_matches_reference_scene(golden):
+    """The reference's own jobs (captured from GPU-mode WidefieldScenes: sub-cell fracs, two-kernel blends, clamped
+    end planes, planes resident after the first job) through the FP64 restatement equal the scenes' CPU images."""
+    G = golden("wf_jobs_ref.npz")
+    for pre in ("a_", "b_"):
+        jobs = O.unpack_wf_jobs(G, pre)
+        H = O.JobReferenceHost()
+        imgs = [H.Images(j)[0] for j in jobs]
+        assert len(jobs[0].planes) > 0 and all(len(j.planes) == 0 for j in jobs[1:])
+        # job 0 renders the spec's focus (the series' first), then one job per focus
+        for f, want in enumerate(G[pre + "images"]):
+            assert rel_l2(imgs[f + 1], want) < 1e-6, (pre, f)
+        assert rel_l2(imgs[0], G[pre + "images"][0]) < 1e-6
+
+
+def _job(NX, NY, nx, ny, planes, kernels_real, deps, frac=(0.0, 0.0), fov=None):
+    from types import SimpleNamespace
+    fov = fov or (0, 0, nx, ny)
+    kernels = [(p, np.fft.rfft2(k)) for p, k in kernels_real.items()]
+    return SimpleNamespace(NX=NX, NY=NY, nx=nx, ny=ny, fovX0=fov[0], fovY0=fov[1], cw=fov[2], ch=fov[3],
+                           fracX=frac[0], fracY=frac[1], geometry=7, kernels=kernels, planes=planes,
+                           channels=[deps], hasPersistent=True)
+
+
+def test_wf_job_oracle_closed_forms():
+    """frac 0: the circular convolution (brute force); an integer frac: a roll by whole cells; frac 1/2 on a pure
+    cosine: the cosine half a cell on; the Nyquist row with frac 1/2: zero (the reference's cosine-only rule)."""
+    rng = np.random.default_rng(4)
+    NX, NY, nx, ny = 16, 16, 12, 10
+    cells = rng.choice(nx * ny, 9, replace=False).astype(np.uint32)
+    vals = rng.random(9).astype(np.float32) + 0.5
+    k0, k1 = rng.random((NY, NX)), rng.random((NY, NX))
+    deps = [(5, 0, 1, 0.7, 0.3, True)]
+    img0 = O.JobReferenceHost().Images(_job(NX, NY, nx, ny, [(5, cells, vals)], {0: k0, 1: k1}, deps))[0]
+    K = 0.7 * k0 + 0.3 * k1
+    want = np.zeros((NY, NX))
+    for c, v in zip(cells, vals):
+        cy, cx = divmod(int(c), nx)
+        want += v * np.roll(np.roll(K, cy, axis=0), cx, axis=1)
+    assert np.allclose(img0, want[:ny, :nx], rtol=1e-12, atol=1e-12)
+    full = (0, 0, NX, NY)
+    base = O.JobReferenceHost().Images(_job(NX, NY, nx, ny, [(5, cells, vals)], {0: k0, 1: k1}, deps, fov=full))[0]
+    for frac, axis in (((1.0, 0.0), 1), ((0.0, 1.0), 0)):
+        sh = O.JobReferenceHost().Images(_job(NX, NY, nx, ny, [(5, cells, vals)], {0: k0, 1: k1}, deps, frac,
+                                              fov=full))[0]
+        assert np.allclose(sh, np.roll(base, -1, axis=axis), atol=1e-12)
+    # a cosine along y, the kernel a delta at the origin: frac 1/2 gives the cosine half a cell on
+    delta = np.zeros((NY, NX))
+    delta[0, 0] = 1.0
+    yy, xx = np.mgrid[0:NY, 0:NX]
+    cos_cells = np.arange(NX * NY, dtype=np.uint32)
+    cos_vals = np.cos(2 * np.pi * 3 * yy / NY).ravel().astype(np.float64)
+    got = O.JobReferenceHost().Images(_job(NX, NY, NX, NY, [(1, cos_cells, cos_vals)], {0: delta},
+                                           [(1, 0, 0, 1.0, 0.0, False)], (0.0, 0.5), fov=full))[0]
+    assert np.allclose(got, np.cos(2 * np.pi * 3 * (yy + 0.5) / NY), atol=1e-12)
+    nyq = np.cos(np.pi * xx).ravel()
+    got = O.JobReferenceHost().Images(_job(NX, NY, NX, NY, [(1, cos_cells, nyq)], {0: delta},
+                                           [(1, 0, 0, 1.0, 0.0, False)], (0.5, 0.0), fov=full))[0]
+    assert np.allclose(got, 0.0, atol=1e-12)
+
+
 # ---- a planted mistake is caught ----------------------------------------------------------------------------------
 
 def test_planted_mistake_fails_a_pin(orc):
