@@ -64,6 +64,19 @@ class Timer:
         self.ms = self.a.elapsed_time(self.b)
 
 
+def profiled(ctx, step, n):
+    """Per-kernel (launches, total ms) over n more steps with the library's event pairs around each launch (kept out
+    of the timed steps: the events add their own gaps)."""
+    L = ctx.L
+    L.iscg_profile_enable(ctx.h, 1)
+    profile(ctx, L)
+    for k in range(n):
+        step(k)
+    prof = profile(ctx, L)
+    L.iscg_profile_enable(ctx.h, 0)
+    return prof
+
+
 def profile(ctx, L):
     import ctypes as C
     cap = 64
@@ -138,14 +151,11 @@ def run_wf(ctx, steps, warmup):
 
     for k in range(warmup):
         step(k)
-    L.iscg_profile_enable(ctx.h, 1)
-    profile(ctx, L)
     with Timer(ctx) as t:
         for k in range(steps):
             step(warmup + k)
-    prof = profile(ctx, L)
-    L.iscg_profile_enable(ctx.h, 0)
     ms = t.ms / steps
+    prof = profiled(ctx, lambda k: step(warmup + steps + k), min(steps, 10))
     nbytes = wf_bytes(nx, ny, nzD, nF, 512, 512, 128)
     dom = max(prof, key=lambda k: prof[k][1])
     avg_ms = prof[dom][1] / prof[dom][0]
@@ -218,14 +228,11 @@ def run_smlm(ctx, steps, warmup, n_frames=2000):
 
     for _ in range(warmup):
         step()
-    L.iscg_profile_enable(ctx.h, 1)
-    profile(ctx, L)
     with Timer(ctx) as t:
         for _ in range(steps):
             step()
-    prof = profile(ctx, L)
-    L.iscg_profile_enable(ctx.h, 0)
     ms = t.ms / steps
+    prof = profiled(ctx, lambda k: step(), min(steps, 3))
     # algorithmic bytes of the splat+noise pass: ADU out, emitter records and setup in, offset map
     tiles = prof.get("splat_tiles_noise", (1, ms))
     b = n_frames * w * h * 2 + len(ev) * (112 + 8) + w * h * 4
@@ -272,14 +279,11 @@ def run_psf(ctx, steps, warmup):
 
     for _ in range(warmup):
         step()
-    L.iscg_profile_enable(ctx.h, 1)
-    profile(ctx, L)
     with Timer(ctx) as t:
         for _ in range(steps):
             step()
-    prof = profile(ctx, L)
-    L.iscg_profile_enable(ctx.h, 0)
     ms = t.ms / steps
+    prof = profiled(ctx, lambda k: step(), min(steps, 3))
     planes = len(reqs) * 41
     flops = planes * 8.0 * 65 * 128 * (128 + 65)
     k = prof.get("psf_planes_mfma", (1, ms))
@@ -317,6 +321,8 @@ def main():
     ap.add_argument("--workload", default="wf", choices=["wf", "smlm", "psf"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (a kernel of known traffic for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the image-formation path has no multi-GPU mode in this build "
@@ -329,6 +335,15 @@ def main():
         print("no GPU", file=sys.stderr)
         sys.exit(1)
     ctx = host.Context(0)
+    if args.calibrate_copy:
+        n = args.calibrate_copy << 18  # float32 elements
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
     secondary = {}
     if args.workload == "wf":
         r = run_wf(ctx, args.steps, args.warmup)
