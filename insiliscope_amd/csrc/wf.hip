@@ -93,54 +93,72 @@ __global__ void wf_place_kernel(const float* __restrict__ K, int R, int nzP, con
    }
 }
 
-// Pass C: per (ky, 16-column block) the z line: FFT of the nzIn planes (zero beyond), times Ghat, inverse FFT, the
-// first nOut planes written. BUILD: forward only, times `scale`, every plane written (the plan's kernel spectrum).
-template <int N, bool BUILD>
-__global__ __launch_bounds__(kCols * N / 8) void zconv_kernel(const float2* __restrict__ in, int nzIn,
-                                                              float2* __restrict__ out, int nOut,
-                                                              const float2* __restrict__ ghat, int NY, int S,
-                                                              int nColsValid, float scale,
-                                                              const float2* __restrict__ tw)
+// Columns per block of pass C: 16 (128-byte row segments). Measured at NZ = 128: 8 columns and 32 columns are both
+// slower (the pass is bound by the latency of its LDS stages and barriers, which wider blocks stretch).
+template <int N>
+constexpr int zconv_cols()
 {
-   constexpr int CS = kCols + 1;
+   return 16;
+}
+
+// Pass C: per (ky, KC-column block) the z line: FFT of the nzIn planes (zero beyond), times Ghat, inverse FFT, the
+// first nOut planes written. BUILD: forward only, times `scale`, every plane written (the plan's kernel spectrum).
+// Columns at or beyond S (the last block's tail) are neither read nor written.
+template <int N, bool BUILD>
+__global__ __launch_bounds__(zconv_cols<N>() * N / 8) void zconv_kernel(const float2* __restrict__ in, int nzIn,
+                                                                        float2* __restrict__ out, int nOut,
+                                                                        const float2* __restrict__ ghat, int NY,
+                                                                        int S, int nColsValid, float scale,
+                                                                        const float2* __restrict__ tw)
+{
+   constexpr int KC = zconv_cols<N>();
+   constexpr int CS = KC + 1;
    __shared__ float2 buf[N * CS];
+   __shared__ float2 twl[N];  // the twiddles, read by every stage
    const int tid = threadIdx.x;
-   const int colBlocks = (nColsValid + kCols - 1) / kCols;
+   for (int i = tid; i < N; i += blockDim.x)
+      twl[i] = tw[i];
+   __syncthreads();
+   const int colBlocks = (nColsValid + KC - 1) / KC;
    const int items = NY * colBlocks;
    const size_t plane = (size_t)NY * S;
    for (int it = blockIdx.x; it < items; it += gridDim.x)
    {
-      const int ky = it / colBlocks, c0 = (it % colBlocks) * kCols;
+      const int ky = it / colBlocks, c0 = (it % colBlocks) * KC;
       const size_t off = (size_t)ky * S + c0;
-      for (int i = tid; i < N * kCols; i += blockDim.x)
+      const int cw = S - c0 < KC ? S - c0 : KC;  // columns inside the row
+      for (int i = tid; i < N * KC; i += blockDim.x)
       {
-         const int z = i / kCols, c = i % kCols;
-         buf[z * CS + c] = z < nzIn ? in[(size_t)z * plane + off + c] : make_float2(0.0f, 0.0f);
+         const int z = i / KC, c = i % KC;
+         buf[z * CS + c] = z < nzIn && c < cw ? in[(size_t)z * plane + off + c] : make_float2(0.0f, 0.0f);
       }
       __syncthreads();
-      fft_lines<N, false, true>(buf, 1, CS, kCols, tid, tw);
+      fft_lines<N, false, true>(buf, 1, CS, KC, tid, twl);
       if (BUILD)
       {
-         for (int i = tid; i < N * kCols; i += blockDim.x)
+         for (int i = tid; i < N * KC; i += blockDim.x)
          {
-            const int z = i / kCols, c = i % kCols;
+            const int z = i / KC, c = i % KC;
             const float2 v = buf[z * CS + c];
-            out[(size_t)z * plane + off + c] = make_float2(v.x * scale, v.y * scale);
+            if (c < cw)
+               out[(size_t)z * plane + off + c] = make_float2(v.x * scale, v.y * scale);
          }
          __syncthreads();
          continue;
       }
-      for (int i = tid; i < N * kCols; i += blockDim.x)
+      for (int i = tid; i < N * KC; i += blockDim.x)
       {
-         const int z = i / kCols, c = i % kCols;
-         buf[z * CS + c] = cmul(buf[z * CS + c], ghat[(size_t)z * plane + off + c]);
+         const int z = i / KC, c = i % KC;
+         if (c < cw)
+            buf[z * CS + c] = cmul(buf[z * CS + c], ghat[(size_t)z * plane + off + c]);
       }
       __syncthreads();
-      fft_lines<N, true, true>(buf, 1, CS, kCols, tid, tw);
-      for (int i = tid; i < nOut * kCols; i += blockDim.x)
+      fft_lines<N, true, true>(buf, 1, CS, KC, tid, twl);
+      for (int i = tid; i < nOut * KC; i += blockDim.x)
       {
-         const int z = i / kCols, c = i % kCols;
-         out[(size_t)z * plane + off + c] = buf[z * CS + c];
+         const int z = i / KC, c = i % KC;
+         if (c < cw)
+            out[(size_t)z * plane + off + c] = buf[z * CS + c];
       }
       __syncthreads();
    }
@@ -152,12 +170,13 @@ struct ZConv
    static void run(hipStream_t s, bool build, const float2* in, int nzIn, float2* out, int nOut, const float2* ghat,
                    int NY, int S, int nCols, float scale, const float2* tw)
    {
-      const size_t items = (size_t)NY * ((nCols + kCols - 1) / kCols);
+      constexpr int KC = zconv_cols<N>();
+      const size_t items = (size_t)NY * ((nCols + KC - 1) / KC);
       if (build)
-         hipLaunchKernelGGL((zconv_kernel<N, true>), dim3(wgs(items, 256 * 16)), dim3(kCols * N / 8), 0, s, in,
+         hipLaunchKernelGGL((zconv_kernel<N, true>), dim3(wgs(items, 256 * 16)), dim3(KC * N / 8), 0, s, in,
                             nzIn, out, nOut, ghat, NY, S, nCols, scale, tw);
       else
-         hipLaunchKernelGGL((zconv_kernel<N, false>), dim3(wgs(items, 256 * 16)), dim3(kCols * N / 8), 0, s, in,
+         hipLaunchKernelGGL((zconv_kernel<N, false>), dim3(wgs(items, 256 * 16)), dim3(KC * N / 8), 0, s, in,
                             nzIn, out, nOut, ghat, NY, S, nCols, scale, tw);
    }
 };

@@ -26,7 +26,11 @@ __global__ __launch_bounds__(256) void rows_r2c_kernel(const float* __restrict__
 {
    constexpr int T = N / 8, LINES = 256 / T > 0 ? 256 / T : 1, LS = N + 1;
    __shared__ float2 buf[LINES * LS];
+   __shared__ float2 twl[N];  // the twiddles, read by every stage
    const int tid = threadIdx.x;
+   for (int i = tid; i < N; i += blockDim.x)
+      twl[i] = tw[i];
+   __syncthreads();
    const int pairs = (nRows + 1) / 2;
    for (int p0 = blockIdx.x * LINES; p0 < pairs; p0 += gridDim.x * LINES)
    {
@@ -43,7 +47,7 @@ __global__ __launch_bounds__(256) void rows_r2c_kernel(const float* __restrict__
          buf[l * LS + n] = make_float2(a, b);
       }
       __syncthreads();
-      fft_lines<N, false, false>(buf, LS, 1, LINES, tid, tw);
+      fft_lines<N, false, false>(buf, LS, 1, LINES, tid, twl);
       for (int i = tid; i < LINES * (N / 2 + 1); i += blockDim.x)
       {
          const int l = i / (N / 2 + 1), k = i % (N / 2 + 1), pr = p0 + l;
@@ -71,7 +75,11 @@ __global__ __launch_bounds__(kCols * N / 8) void cols_kernel(const float2* __res
 {
    constexpr int CS = kCols + 1;  // padded row of the block in LDS
    __shared__ float2 buf[N * CS];
+   __shared__ float2 twl[N];  // the twiddles, read by every stage
    const int tid = threadIdx.x;
+   for (int i = tid; i < N; i += blockDim.x)
+      twl[i] = tw[i];
+   __syncthreads();
    const int colBlocks = (nColsValid + kCols - 1) / kCols;
    const int items = nPlanes * colBlocks;
    for (int it = blockIdx.x; it < items; it += gridDim.x)
@@ -84,7 +92,7 @@ __global__ __launch_bounds__(kCols * N / 8) void cols_kernel(const float2* __res
          buf[n * CS + c] = n < rowsIn ? src[(size_t)n * S + c0 + c] : make_float2(0.0f, 0.0f);
       }
       __syncthreads();
-      fft_lines<N, INV, true>(buf, 1, CS, kCols, tid, tw);
+      fft_lines<N, INV, true>(buf, 1, CS, kCols, tid, twl);
       float2* dst = out + (size_t)pl * outPlaneStride;
       for (int i = tid; i < rowsOut * kCols; i += blockDim.x)
       {
@@ -117,7 +125,11 @@ __global__ __launch_bounds__(256) void rows_c2r_kernel(const float2* __restrict_
 {
    constexpr int T = N / 8, LINES = 256 / T > 0 ? 256 / T : 1, LS = N + 1;
    __shared__ float2 buf[LINES * LS];
+   __shared__ float2 twl[N];  // the twiddles, read by every stage
    const int tid = threadIdx.x;
+   for (int i = tid; i < N; i += blockDim.x)
+      twl[i] = tw[i];
+   __syncthreads();
    const int pairs = (nRows + 1) / 2;
    for (int p0 = blockIdx.x * LINES; p0 < pairs; p0 += gridDim.x * LINES)
    {
@@ -142,7 +154,7 @@ __global__ __launch_bounds__(256) void rows_c2r_kernel(const float2* __restrict_
             buf[l * LS + N - k] = make_float2(A.x + B.y, B.x - A.y);
       }
       __syncthreads();
-      fft_lines<N, true, false>(buf, LS, 1, LINES, tid, tw);
+      fft_lines<N, true, false>(buf, LS, 1, LINES, tid, twl);
       for (int i = tid; i < LINES * 2 * e.cw; i += blockDim.x)
       {
          const int l = i / (2 * e.cw), rem = i % (2 * e.cw), half = rem / e.cw, x = rem % e.cw, pr = p0 + l;
