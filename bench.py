@@ -34,6 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # host threads for the CPU baseline: the GPU box's CPU share per GPU (16), or fewer where the host has fewer
 CORES = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "16")), os.cpu_count() or 1))
 MFMA_FP16_PEAK_TFS = 2500.0
+FP32_PEAK_TFS = 157.3
 
 
 def log(*a):
@@ -313,6 +314,135 @@ def cpu_psf(n=64):
             "sample": f"ComputeZernikePsfPlanes x {n} requests (41 planes each, planes in parallel)"}
 
 
+# ---- SMLM 128x128 x 50 frames end to end (C1) -------------------------------------------------------------------
+
+def run_c1(ctx, steps, warmup):
+    """One step = the C1 PSF on the GPU (41 planes, 129x129 at os 4, M 128), its splat kernel (block sums) set up on
+    the device, and 50 frames of 128x128 rendered with Catmull-Rom splats, bg 20 and sCMOS noise: the movie's
+    GPU work end to end (the blink events are collected on the host beforehand)."""
+    import torch
+    from insiliscope_amd import configs, host
+    req = configs.c1_psf(seed=1, oversampling=4)
+    nz, size = 41, 129
+    planes = torch.empty((1, nz, size, size), dtype=torch.float32, device="cuda")
+    host.ComputePsfPlanesBatch([req], normalize=True, out=planes, ctx=ctx)
+    cache = host.PsfKernelCache(planes[0], oversampling=4, zStepNm=50.0, interpMode=2)
+    ev, st = configs.c1_events(seed=1, n_frames=50)
+    ems = np.zeros(len(ev), host.FRAME_EMITTER_DTYPE)
+    for f in range(50):
+        ems[st[f]:st[f + 1]] = host.CollectFrameEmitters(ev[st[f]:st[f + 1]], f, 100.0, 1.0, psfCache=cache)
+    d_ems = torch.from_numpy(ems.view(np.uint8).reshape(len(ems), host.FRAME_EMITTER_DTYPE.itemsize)).cuda()
+    d_start = torch.from_numpy(st.astype(np.int32)).cuda()
+    d_frames = torch.arange(50, dtype=torch.int32, device="cuda")
+    w = h = 128
+    off = torch.full((h, w), 100.0, dtype=torch.float32, device="cuda")
+    adu = torch.empty((50, h, w), dtype=torch.int16, device="cuda")
+    cam = configs.scmos_c1()
+    sp = host.SplatGpuHost(ctx)
+    if not sp.SetStatic(w, h, offset=off, flatBackground=20.0):
+        raise RuntimeError(sp.err)
+    serial = [0]
+
+    def step():
+        host.ComputePsfPlanesBatch([req], normalize=True, out=planes, ctx=ctx)
+        serial[0] += 1
+        cache.serial = serial[0]  # a new stack: the block sums are made again
+        if not (sp.SetKernels([cache]) and sp.RenderBatch(d_ems, d_start, d_frames, cam, 1, outAdu=adu)):
+            raise RuntimeError(sp.err)
+
+    for _ in range(warmup):
+        step()
+    with Timer(ctx) as t:
+        for _ in range(steps):
+            step()
+    ms = t.ms / steps
+    prof = profiled(ctx, lambda k: step(), min(steps, 3))
+    return {"value": 50 * 1000.0 / ms, "unit": "frames/s", "ms_per_step": ms,
+            "config": {"workload": "SMLM 128x128 x 50 frames end to end: Gibson-Lanni PSF (41 planes 129x129, os 4, "
+                                   "M 128, Noll 5-15 +-0.05) + Catmull-Rom splats of ~200 blinks/frame + sCMOS",
+                       "blinks": int(len(ev))},
+            "kernels": {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4)} for k, v in prof.items()},
+            "_events": (ev, st)}
+
+
+def cpu_c1(ev, st):
+    """The reference on the host cores: BuildZernikePsfKernelCache (ComputeZernikePsfPlanes, sum-1, block sums) of
+    the C1 request, then RenderPhotonImage (Cubic) + ApplyNoiseChain for the 50 frames (frames in parallel)."""
+    import oracle as O
+    from insiliscope_amd import configs
+    ref = O.load_reference()
+    if ref is None:
+        return None
+    req = O.psf_request(**configs.c1_psf(seed=1, oversampling=4))
+    cam = configs.scmos_c1()
+    oc = O.camera(quantumEfficiency=1.0, gainPhotonsPerAdu=cam.gainPhotonsPerAdu,
+                  readNoiseElectrons=cam.readNoiseElectrons, emccd=0)
+    off = np.full((128, 128), 100.0, np.float32)
+    t0 = time.time()
+    planes, _ = ref.psf_kernel_cache(req, 129, 41, 4)
+    ref.render_noise_frames(ev, st, 128, 128, 100.0, 20.0, planes, 4, 50.0, 2, oc, off, 1)
+    el = time.time() - t0
+    return {"value": round(50 / el, 2), "unit": "frames/s", "cores": CORES, "kind": "reference",
+            "sample": f"the whole 50-frame movie ({el:.2f} s): PSF kernel cache + 50 rendered, noised frames"}
+
+
+# ---- ROI forward + adjoint (C5) ----------------------------------------------------------------------------------
+
+def run_c5(ctx, steps, warmup):
+    """One step = the Poisson NLL and its gradient (x, y, z, photons, background per ROI; the 28 shared Zernike
+    terms) of 4096 ROIs of 15x15 (pupil M 64, 4x4 sub-samples per pixel), data drawn once from the counter stream."""
+    import torch
+    from insiliscope_amd import configs, host
+    z, p = configs.c5_rois(seed=1, n=4096)
+    prm = np.zeros(len(p), host.ROI_PARAMS_DTYPE)
+    for i, k in enumerate(("x", "y", "z", "photons", "background")):
+        prm[k] = p[:, i]
+    m = host.roi_model()
+    mu = torch.from_numpy(host.RoiForward(m, z, prm, ctx=ctx)).cuda()
+    cam = host.CameraNoiseParams(1.0, 0.0, 1.0, 0.0, False)  # ADU = Poisson(mu) counts
+    frames = torch.arange(len(p), dtype=torch.int32, device="cuda")
+    counts = host.ApplyNoiseChainFrames(mu, cam, 5, frames, ctx=ctx)
+    data = counts.to(torch.int32).bitwise_and(0xFFFF).to(torch.float32)
+    d_prm = torch.from_numpy(prm.view(np.uint8).reshape(len(prm), -1)).cuda()
+    grad = torch.empty((len(p), 5), dtype=torch.float64, device="cuda")
+
+    def step():
+        host.RoiBackward(m, z, d_prm, data, gradParams=grad, ctx=ctx)
+
+    for _ in range(warmup):
+        step()
+    with Timer(ctx) as t:
+        for _ in range(steps):
+            step()
+    ms = t.ms / steps
+    prof = profiled(ctx, lambda k: step(), min(steps, 3))
+    P = 15 * 4
+    flops = len(p) * 3 * 8.0 * P * 64 * (64 + P)  # forward 8 P M (M + P), adjoint about twice that
+    k = prof.get("roi_backward", (1, ms))
+    kms = k[1] / k[0]
+    return {"value": len(p) * 1000.0 / ms, "unit": "ROI gradients/s", "ms_per_step": ms,
+            "config": {"workload": "C5 adjoint: 4096 ROIs 15x15, Gibson-Lanni + 21 Zernike (Noll 4-24), pupil M 64, "
+                                   "os 4, Poisson NLL gradient wrt x, y, z, N, b and the shared coefficients"},
+            "roi_kernel": {"avg_ms": round(kms, 4), "TFLOP_s_fp32": round(flops / (kms * 1e-3) / 1e12, 2),
+                           "frac_of_fp32_vector_peak": round(flops / (kms * 1e-3) / 1e12 / FP32_PEAK_TFS, 4)},
+            "kernels": {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4)} for k, v in prof.items()},
+            "_case": (z, prm, data.cpu().numpy())}
+
+
+def cpu_c5(z, prm, data):
+    """The FP64 C restatement of the ROI model and its adjoint (oracle.c, one thread) over the whole batch."""
+    import oracle as O
+    orc = O.load_oracle()
+    p = np.zeros(len(prm), O.ROI_PARAMS_DTYPE)
+    for k in ("x", "y", "z", "photons", "background"):
+        p[k] = prm[k]
+    t0 = time.time()
+    orc.roi_backward(O.roi_model(), z, p, data.astype(np.float64))
+    el = time.time() - t0
+    return {"value": round(len(p) / el, 2), "unit": "ROI gradients/s", "cores": 1, "kind": "port",
+            "sample": f"all {len(p)} ROIs ({el:.2f} s, FP64, one thread; the reference has no adjoint)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -374,6 +504,27 @@ def main():
                     s["speedup_vs_cpu"] = round(s["value"] / c["value"], 1)
         except Exception as e:  # a secondary line must not hide the primary one
             secondary["smlm_c4"] = {"error": str(e)}
+        try:
+            c = run_c1(ctx, 5, 1)
+            ev, st = c.pop("_events")
+            secondary["smlm_c1"] = c
+            if not args.no_cpu:
+                b = cpu_c1(ev, st)
+                if b:
+                    c["cpu_baseline"] = b
+                    c["speedup_vs_cpu"] = round(c["value"] / b["value"], 1)
+        except Exception as e:
+            secondary["smlm_c1"] = {"error": str(e)}
+        try:
+            c = run_c5(ctx, 10, 2)
+            case = c.pop("_case")
+            secondary["adjoint_c5"] = c
+            if not args.no_cpu:
+                b = cpu_c5(*case)
+                c["cpu_baseline"] = b
+                c["speedup_vs_cpu"] = round(c["value"] / b["value"], 1)
+        except Exception as e:
+            secondary["adjoint_c5"] = {"error": str(e)}
         try:
             p = run_psf(ctx, 3, 1)
             secondary["psf_library_c3"] = p

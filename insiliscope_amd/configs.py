@@ -199,6 +199,25 @@ def c4_events(seed=1, n_frames=2000, first_frame=0):
     return blink_events(seed, n_frames, 512, 512, 100.0, 800.0, 600.0, 3000.0, first_frame=first_frame)
 
 
+# ---- ROI adjoint (C5) -------------------------------------------------------------------------------------------
+
+_U_ROI = 0x524F4949
+
+
+def c5_rois(seed=1, n=4096, roi=15):
+    """C5: n ROIs of roi x roi pixels; x, y uniform +-0.5 px about the centre, z uniform +-500 nm, photons
+    2000 exp(0.5 N(0,1)), background uniform [5, 50]; shared Zernike Noll 4-24 from N(0, 0.03^2) (OSA slots).
+    Returns (zernike[28], params) with params in ROI_PARAMS order (x, y, z, photons, background)."""
+    z = zernike_noll(seed, range(4, 25), sigma=0.03)
+    idx = np.arange(n, dtype=np.int64)
+    u = uniforms(seed, _U_ROI, idx, 0)
+    v = uniforms(seed, _U_ROI, idx, 1)
+    c = (roi - 1) / 2.0 + 0.5
+    params = np.stack([c + u[:, 0] - 0.5, c + u[:, 1] - 0.5, (2.0 * u[:, 2] - 1.0) * 500.0,
+                       2000.0 * np.exp(0.5 * normals(v)), 5.0 + 45.0 * u[:, 3]], axis=1)
+    return z, params
+
+
 # ---- mean-field density ----------------------------------------------------------------------------------------
 
 def c2_density(seed=1, nx=256, ny=256, nz=64, occupancy=0.03):
