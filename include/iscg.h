@@ -107,7 +107,9 @@ typedef struct
    int32_t nz;
    int32_t size;          /* sizeOversampled = 2 halfWidthOversampled + 1 */
    int32_t oversampling;
-   int32_t interpMode;    /* PsfInterpMode: 0 nearest, 1 linear, 2 cubic (Catmull-Rom on block sums) */
+   int32_t interpMode;    /* PsfInterpMode: 0 nearest, 1 linear, 2 cubic (Catmull-Rom on block sums), 3 Fft (one
+                           * Fourier shift of the raw plane per emitter, PlanSplat :1443-1456; render_frames only,
+                           * size <= 255) */
 } iscg_kernel_view;
 
 /* One emitter of a frame as RenderPhotonImage places it (sim::FrameEmitter, Sim/SMLMSimulation.h:158): position in

@@ -21,6 +21,7 @@
 #include "iscg_internal.h"
 
 #include <climits>
+#include <cmath>
 
 namespace iscg {
 
@@ -35,9 +36,10 @@ constexpr int kBlock = 256;
 struct KernelDev
 {
    DevBuf sums;   // nz * bw * bw floats
+   DevBuf raw;    // nz * size * size floats, kept for the Fft placement (interp 3)
    DevBuf spans;  // 2 * nz * (2 camRad + 1) int16 (lo block then hi block), or empty
    int bw = 0, os = 1, camRad = 0, size = 0, nz = 0, interp = 0;
-   bool hasSpans = false;
+   bool hasSpans = false, hasRaw = false;
    uint64_t serial = 0;
 };
 
@@ -51,6 +53,9 @@ struct SplatState
    double flatBg = 0.0;
    uint64_t key = 0;
    DevBuf setup, pos, photons;
+   // the Fft placement: per-emitter shifted block sums, per-block scratch, radix-2 tables of length fftN
+   DevBuf fftSums, fftScratch, fftTables;
+   int fftN = 0;
 };
 
 void destroy_splat(iscg_context* h)
@@ -80,6 +85,7 @@ static_assert(sizeof(SetupRec) == 112, "SetupRec layout");
 struct KernelTable
 {
    const float* sums[kMaxKernels];
+   const float* raw[kMaxKernels];
    const int16_t* lo[kMaxKernels];
    const int16_t* hi[kMaxKernels];
    int bw[kMaxKernels], os[kMaxKernels], camRad[kMaxKernels], size[kMaxKernels], nz[kMaxKernels],
@@ -96,6 +102,7 @@ static KernelTable table_of(const SplatState& s)
    {
       const KernelDev& k = s.k[i];
       t.sums[i] = k.sums.as<float>();
+      t.raw[i] = k.hasRaw ? k.raw.as<float>() : nullptr;
       const size_t nSpan = (size_t)k.nz * (2 * k.camRad + 1);
       t.lo[i] = k.hasSpans ? k.spans.as<int16_t>() : nullptr;
       t.hi[i] = k.hasSpans ? k.spans.as<int16_t>() + nSpan : nullptr;
@@ -180,11 +187,19 @@ __global__ void setup_exact_kernel(const iscg_frame_emitter* __restrict__ ems, u
          r.y0 = (int)floor(em.yPx + 0.5);
          const double tx = kc + (r.x0 - em.xPx - 0.5) * os + 0.5;
          const double ty = kc + (r.y0 - em.yPx - 0.5) * os + 0.5;
-         if (interp == 0)
+         if (interp == 0 || interp == 3)
          {
             r.bx = (int)floor(tx + 0.5);
             r.by = (int)floor(ty + 0.5);
             r.nTaps = 1;
+            if (interp == 3)
+            {
+               // PlanSplat's Fft branch (Sim/PsfGeneratorBridge.cpp:1443-1456): the plane moved by the sub-sample
+               // fraction left after rounding (kept in the unused tap weights), read nearest from its own sums
+               r.wx[1] = tx - (double)r.bx;
+               r.wy[1] = ty - (double)r.by;
+               r.pad[0] = 1;
+            }
          }
          else
          {
@@ -254,6 +269,182 @@ __global__ void setup_records_kernel(const iscg_splat_emitter* __restrict__ ems,
    }
 }
 
+// ---- the Fft placement (PsfInterpMode::Fft) ----------------------------------------------------------------------
+// FftShiftKernelTile (Sim/PsfGeneratorBridge.cpp:1293-1380) per emitter, in FP64 with the reference's operations:
+// every row of the raw n x n plane zero-padded to N (a power of two >= 2n), Fft1d forward (Sim/FftRadix2.h:74-105:
+// bit reversal, then radix-2 stages with the tabulated recurrence twiddles of MakeTwiddles, :39-69), times
+// exp(2 pi i kk shift / N), Fft1d inverse, x 1/N, the first n complex values kept; the same along every column of
+// that intermediate, the real part rounded to float; then BuildBlockSums of the shifted plane. kFftLines lines are
+// transformed at once by kFftLane lanes each. The phase cos/sin are the device's FP64 functions where the
+// reference uses the host's: a shifted value differs only if one of them differs by an ulp AND the FP64 result sits
+// that close to a float rounding boundary.
+constexpr int kFftLines = 8, kFftLane = 32, kFftMaxN = 512;
+
+struct FftTables
+{
+   const double *fc, *fs, *ic, *is;  // N - 1 each (stage len's values from len/2 - 1)
+   const int* rev;                   // N
+   int N;
+};
+
+// One radix-2 transform of every line in lr/li (the block's kFftLines lines of N, already bit-reversed).
+__device__ __forceinline__ void fft_radix2_lines(double* lr, double* li, int N, const double* __restrict__ tc0,
+                                                 const double* __restrict__ ts0, int line, int lane)
+{
+   for (int len = 2; len <= N; len <<= 1)
+   {
+      const int half = len >> 1;
+      const double* tc = tc0 + (half - 1);
+      const double* ts = ts0 + (half - 1);
+      double* re = lr + line * N;
+      double* im = li + line * N;
+      for (int b = lane; b < N / 2; b += kFftLane)
+      {
+         const int k = b & (half - 1), a = (b - k) * 2 + k, c = a + half;
+         const double cr = tc[k], ci = ts[k];
+         const double vr = re[c] * cr - im[c] * ci, vi = re[c] * ci + im[c] * cr;
+         re[c] = re[a] - vr;
+         im[c] = im[a] - vi;
+         re[a] = re[a] + vr;
+         im[a] = im[a] + vi;
+      }
+      __syncthreads();
+   }
+}
+
+// FourierShiftLine (Sim/PsfGeneratorBridge.cpp:1264-1281) on the block's lines: forward (input bit-reversed
+// already), phase, the in-place bit reversal of Fft1d, inverse, x 1/N.
+__device__ __forceinline__ void fourier_shift_lines(double* lr, double* li, const FftTables& t, const double* pc,
+                                                    const double* ps, int line, int lane)
+{
+   const int N = t.N;
+   fft_radix2_lines(lr, li, N, t.fc, t.fs, line, lane);
+   double* re = lr + line * N;
+   double* im = li + line * N;
+   for (int k = lane; k < N; k += kFftLane)
+   {
+      const double r = re[k], i = im[k];
+      re[k] = r * pc[k] - i * ps[k];
+      im[k] = r * ps[k] + i * pc[k];
+   }
+   __syncthreads();
+   for (int k = lane; k < N; k += kFftLane)
+   {
+      const int j = t.rev[k];
+      if (k < j)
+      {
+         const double a = re[k], b = im[k];
+         re[k] = re[j];
+         im[k] = im[j];
+         re[j] = a;
+         im[j] = b;
+      }
+   }
+   __syncthreads();
+   fft_radix2_lines(lr, li, N, t.ic, t.is, line, lane);
+   const double norm = 1.0 / N;
+   for (int k = lane; k < N; k += kFftLane)
+   {
+      re[k] *= norm;
+      im[k] *= norm;
+   }
+   __syncthreads();
+}
+
+// Per Fft emitter (record pad[0] = 1): the shifted plane's block sums into sums + e * stride. scratch: per block
+// 2 n^2 doubles (the complex intermediate) + n^2 floats (the shifted plane).
+__global__ __launch_bounds__(kFftLines * kFftLane) void fft_shift_sums_kernel(const SetupRec* __restrict__ recs,
+                                                                             uint32_t n, KernelTable kt, FftTables t,
+                                                                             float* __restrict__ sums, size_t stride,
+                                                                             double* __restrict__ scratch,
+                                                                             size_t scratchStride)
+{
+   __shared__ double lr[kFftLines * kFftMaxN], li[kFftLines * kFftMaxN];
+   __shared__ double phase[4 * kFftMaxN];
+   const int tid = threadIdx.x, line = tid / kFftLane, lane = tid % kFftLane;
+   const int N = t.N, half = N / 2;
+   double* midRe = scratch + (size_t)blockIdx.x * scratchStride;
+   for (uint32_t e = blockIdx.x; e < n; e += gridDim.x)
+   {
+      const SetupRec r = recs[e];
+      if (r.pad[0] != 1 || r.nTaps == 0)
+         continue;
+      const int kk = r.kernel, sz = kt.size[kk], os = kt.os[kk], off = os - 1, bw = sz + off;
+      double* midIm = midRe + (size_t)sz * sz;
+      float* shifted = (float*)(midIm + (size_t)sz * sz);
+      const float* plane = kt.raw[kk] + (size_t)r.plane * sz * sz;
+      double* xc = phase;
+      double* xs = phase + N;
+      double* yc = phase + 2 * N;
+      double* ys = phase + 3 * N;
+      for (int k = tid; k < N; k += blockDim.x)
+      {
+         const int q = k < half ? k : k - N;
+         const double ax = 6.283185307179586 * q * r.wx[1] / N, ay = 6.283185307179586 * q * r.wy[1] / N;
+         xc[k] = cos(ax);
+         xs[k] = sin(ax);
+         yc[k] = cos(ay);
+         ys[k] = sin(ay);
+      }
+      // rows: g(x) = f(x + shiftX) along every row, complex intermediate kept
+      for (int y0 = 0; y0 < sz; y0 += kFftLines)
+      {
+         const int y = y0 + line;
+         for (int k = lane; k < N; k += kFftLane)
+         {
+            const int j = t.rev[k];  // the bit reversal of Fft1d, applied while loading
+            lr[line * N + j] = y < sz && k < sz ? (double)plane[(size_t)y * sz + k] : 0.0;
+            li[line * N + j] = 0.0;
+         }
+         __syncthreads();
+         fourier_shift_lines(lr, li, t, xc, xs, line, lane);
+         if (y < sz)
+            for (int x = lane; x < sz; x += kFftLane)
+            {
+               midRe[(size_t)y * sz + x] = lr[line * N + x];
+               midIm[(size_t)y * sz + x] = li[line * N + x];
+            }
+         __syncthreads();
+      }
+      // columns of the intermediate, the real part as float
+      for (int x0 = 0; x0 < sz; x0 += kFftLines)
+      {
+         const int x = x0 + line;
+         for (int k = lane; k < N; k += kFftLane)
+         {
+            const int j = t.rev[k];
+            const bool in = x < sz && k < sz;
+            lr[line * N + j] = in ? midRe[(size_t)k * sz + x] : 0.0;
+            li[line * N + j] = in ? midIm[(size_t)k * sz + x] : 0.0;
+         }
+         __syncthreads();
+         fourier_shift_lines(lr, li, t, yc, ys, line, lane);
+         if (x < sz)
+            for (int y = lane; y < sz; y += kFftLane)
+               shifted[(size_t)y * sz + x] = (float)lr[line * N + y];
+         __syncthreads();
+      }
+      // BuildBlockSums of the shifted plane (row sums, then sums of them, FP64)
+      float* out = sums + (size_t)e * stride;
+      for (int idx = tid; idx < bw * bw; idx += blockDim.x)
+      {
+         const int ai = idx / bw, bi = idx - ai * bw;
+         const int a = ai - off, ya = a > 0 ? a : 0, yb = (a + off) < (sz - 1) ? (a + off) : (sz - 1);
+         const int b = bi - off, xa = b > 0 ? b : 0, xb = (b + off) < (sz - 1) ? (b + off) : (sz - 1);
+         double acc = 0.0;
+         for (int yy = ya; yy <= yb; ++yy)
+         {
+            double hs = 0.0;
+            for (int xx = xa; xx <= xb; ++xx)
+               hs += shifted[(size_t)yy * sz + xx];
+            acc += hs;
+         }
+         out[idx] = (float)acc;
+      }
+      __syncthreads();
+   }
+}
+
 struct TileArgs
 {
    const SetupRec* recs;
@@ -269,6 +460,8 @@ struct TileArgs
    uint16_t* outAdu;          // nFrames x npx or null
    float* outPhotons;         // nFrames x npx or null
    double flatBg;
+   const float* fftSums;      // per-emitter shifted block sums (the Fft placement), stride fftStride floats
+   size_t fftStride;
    NoiseQueue q;              // deferred noise pixels (index f * npx + i)
    uint32_t w, h, tilesX, nTiles, nFrames, nEmitters, seed, pixelBase;
 };
@@ -352,7 +545,8 @@ __global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelT
             const int rw = dxHi - dxLo + 1, rh = dyHi - dyLo + 1;
             if (rw > 0 && rh > 0)
             {
-               const float* B = kt.sums[kk] + (size_t)r.plane * bw * bw;
+               const float* B = r.pad[0] == 1 ? a.fftSums + (size_t)list[k] * a.fftStride
+                                              : kt.sums[kk] + (size_t)r.plane * bw * bw;
                const int nSpan = 2 * camRad + 1;
                const int16_t* lo = kt.lo[kk] ? kt.lo[kk] + (size_t)r.plane * nSpan : nullptr;
                const int16_t* hi = kt.hi[kk] ? kt.hi[kk] + (size_t)r.plane * nSpan : nullptr;
@@ -402,6 +596,64 @@ __global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelT
    }
 }
 
+// MakeTwiddles (Sim/FftRadix2.h:39-69) for N, both signs, and the bit reversal: [fc | fs | ic | is] (N - 1 doubles
+// each) then N ints -- the host's cos/sin and the same recurrence, so the same bits as the reference's tables.
+static bool fft_tables(iscg_context* h, SplatState& s, int N, FftTables& t)
+{
+   const size_t m = (size_t)(N - 1);
+   if (s.fftN != N)
+   {
+      std::vector<double> tab(4 * m);
+      for (int sign = -1, part = 0; part < 2; sign += 2, ++part)
+      {
+         double* c = tab.data() + (size_t)(2 * part) * m;
+         double* sn = c + m;
+         for (int len = 2; len <= N; len <<= 1)
+         {
+            const double ang = sign * 2.0 * 3.14159265358979323846 / len;
+            const double wr = std::cos(ang), wi = std::sin(ang);
+            const int half = len / 2;
+            double cr = 1.0, ci = 0.0;
+            for (int k = 0; k < half; ++k)
+            {
+               c[half - 1 + k] = cr;
+               sn[half - 1 + k] = ci;
+               const double nr = cr * wr - ci * wi;
+               ci = cr * wi + ci * wr;
+               cr = nr;
+            }
+         }
+      }
+      std::vector<int> rev((size_t)N, 0);
+      for (int i = 1, j = 0; i < N; ++i)
+      {
+         int bit = N >> 1;
+         for (; j & bit; bit >>= 1)
+            j ^= bit;
+         j ^= bit;
+         rev[(size_t)i] = j;
+      }
+      const size_t bytes = 4 * m * sizeof(double) + (size_t)N * sizeof(int);
+      if (!ok(h, s.fftTables.reserve(bytes), "Fft tables") ||
+          !ok(h, hipMemcpyAsync(s.fftTables.p, tab.data(), 4 * m * sizeof(double), hipMemcpyHostToDevice, h->stream),
+              "Fft tables") ||
+          !ok(h, hipMemcpyAsync(s.fftTables.as<double>() + 4 * m, rev.data(), (size_t)N * sizeof(int),
+                                hipMemcpyHostToDevice, h->stream),
+              "Fft tables") ||
+          !ok(h, hipStreamSynchronize(h->stream), "Fft tables"))
+         return false;
+      s.fftN = N;
+   }
+   const double* base = s.fftTables.as<const double>();
+   t.fc = base;
+   t.fs = base + m;
+   t.ic = base + 2 * m;
+   t.is = base + 3 * m;
+   t.rev = (const int*)(base + 4 * m);
+   t.N = N;
+   return true;
+}
+
 static bool upload(iscg_context* h, DevBuf& b, const void* src, size_t bytes)
 {
    if (!ok(h, b.reserve(bytes), "device allocation"))
@@ -441,6 +693,19 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
        !ok(h, s.pos.reserve((size_t)(nEmitters ? nEmitters : 1) * sizeof(int2)), "setup allocation"))
       return -1;
    const KernelTable kt = table_of(s);
+   // the Fft placement: one kernel width for all Fft stacks of the batch, exact records only
+   int fftSize = 0, fftOs = 1;
+   for (int i = 0; i < s.nk; ++i)
+      if (s.k[i].interp == 3)
+      {
+         if (fftSize && (fftSize != s.k[i].size || fftOs != s.k[i].os))
+            return fail(h, "splat: the Fft placement's stacks of one batch must share their size and oversampling");
+         fftSize = s.k[i].size;
+         fftOs = s.k[i].os;
+      }
+   if (fftSize && records)
+      return fail(h, "splat: the Fft placement needs iscg_splat_render_frames (GPU records carry no sub-sample "
+                     "shift; the reference's GPU host does not take Fft stacks either)");
    if (nEmitters)
    {
       const unsigned grid = grid_for(nEmitters, 256, 4);
@@ -455,6 +720,36 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
          return -1;
    }
    TileArgs a;
+   a.fftSums = nullptr;
+   a.fftStride = 0;
+   if (fftSize && nEmitters)
+   {
+      int N = 1;
+      while (N < 2 * fftSize)
+         N <<= 1;
+      if (N > kFftMaxN)
+         return fail(h, "splat: the Fft placement takes kernels up to 256 samples wide");
+      const size_t bw = (size_t)fftSize + fftOs - 1, stride = bw * bw;
+      if ((size_t)nEmitters * stride * sizeof(float) > ((size_t)8 << 30))
+         return fail(h, "splat: Fft placement batch too large (per-emitter block sums over 8 GiB): split the frames");
+      const unsigned blocks = (unsigned)std::min<uint32_t>(nEmitters, 2048);
+      const size_t sz2 = (size_t)fftSize * fftSize, scratchStride = 2 * sz2 + (sz2 + 1) / 2;
+      FftTables t;
+      if (!ok(h, s.fftSums.reserve((size_t)nEmitters * stride * sizeof(float)), "Fft placement sums") ||
+          !ok(h, s.fftScratch.reserve((size_t)blocks * scratchStride * sizeof(double)), "Fft placement scratch") ||
+          !fft_tables(h, s, N, t))
+         return -1;
+      {
+         ProfScope ps(h, "splat_fft_shift");
+         hipLaunchKernelGGL(fft_shift_sums_kernel, dim3(blocks), dim3(kFftLines * kFftLane), 0, h->stream,
+                            (const SetupRec*)s.setup.as<SetupRec>(), nEmitters, kt, t, s.fftSums.as<float>(), stride,
+                            s.fftScratch.as<double>(), scratchStride);
+      }
+      if (!ok(h, hipGetLastError(), "Fft placement launch"))
+         return -1;
+      a.fftSums = s.fftSums.as<const float>();
+      a.fftStride = stride;
+   }
    a.recs = s.setup.as<SetupRec>();
    a.pos = s.pos.as<int2>();
    a.emStart = dStart;
@@ -527,15 +822,16 @@ int32_t iscg_splat_set_kernels(iscg_context* h, const iscg_kernel_view* kernels,
    {
       const iscg_kernel_view& v = kernels[i];
       if (!v.planes || v.nz < 1 || v.size < 3 || (v.size % 2) != 1 || v.oversampling < 1 || v.interpMode < 0 ||
-          v.interpMode > 2)
-         return fail(h, "iscg_splat_set_kernels: bad kernel view (planes, odd size, nz >= 1, os >= 1, interp 0-2)");
+          v.interpMode > 3)
+         return fail(h, "iscg_splat_set_kernels: bad kernel view (planes, odd size, nz >= 1, os >= 1, interp 0-3)");
    }
    for (int i = 0; i < n; ++i)
    {
       const iscg_kernel_view& v = kernels[i];
       KernelDev& k = s.k[i];
       const int os = v.oversampling, bw = v.size + os - 1, camRad = ((v.size - 1) / 2) / os;
-      const bool same = v.serial != 0 && k.serial == v.serial && k.size == v.size && k.nz == v.nz && k.os == os;
+      const bool same = v.serial != 0 && k.serial == v.serial && k.size == v.size && k.nz == v.nz && k.os == os &&
+                        (v.interpMode != 3 || k.hasRaw);
       k.interp = v.interpMode;
       if (same)
          continue;
@@ -553,6 +849,12 @@ int32_t iscg_splat_set_kernels(iscg_context* h, const iscg_kernel_view* kernels,
       hipLaunchKernelGGL(block_sums_kernel, dim3(grid_for(total, 256, 4)), dim3(256), 0, h->stream, dPlanes,
                          k.sums.as<float>(), v.size, os, v.nz);
       if (!ok(h, hipGetLastError(), "block sums launch"))
+         return -1;
+      // the Fft placement shifts the raw planes per emitter
+      k.hasRaw = v.interpMode == 3;
+      if (k.hasRaw && (!ok(h, k.raw.reserve(planeBytes), "kernel plane allocation") ||
+                       !ok(h, hipMemcpyAsync(k.raw.p, dPlanes, planeBytes, hipMemcpyDeviceToDevice, h->stream),
+                           "kernel plane copy")))
          return -1;
       k.hasSpans = v.spanLo && v.spanHi;
       if (k.hasSpans)

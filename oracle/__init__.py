@@ -141,6 +141,9 @@ class Oracle:
         L.orc_splat_setup.argtypes = [I32, I32, D, D, I32, C.POINTER(SplatSetupT)]
         L.orc_render_frame.argtypes = [PF, U32, U32, D, PF, D, C.c_void_p, I32, PF, I32, I32, I32, I32, I32, PI16,
                                        PI16, D]
+        L.orc_fft_shift_tile.argtypes = [PF, I32, D, D, PF]
+        L.orc_render_frame_fft.argtypes = [PF, U32, U32, D, PF, D, C.c_void_p, I32, PF, I32, I32, I32, PI16, PI16,
+                                           D]
         L.orc_wf_radius.argtypes = [I32, I32, I32]
         L.orc_wf_radius.restype = I32
         L.orc_wf_kernel.argtypes = [PF, I32, I32, I32, I32, PF]
@@ -249,6 +252,24 @@ class Oracle:
         return img
 
     # -- mean field --
+    def fft_shift_tile(self, kernel, shift_x, shift_y):
+        k = np.ascontiguousarray(kernel, np.float32)
+        out = np.zeros_like(k)
+        self.L.orc_fft_shift_tile(_p(k, C.c_float), k.shape[0], shift_x, shift_y, _p(out, C.c_float))
+        return out
+
+    def render_frame_fft(self, w, h, bg, emitters, planes, os_, span_lo=None, span_hi=None, min_photons=0.0):
+        emitters = np.ascontiguousarray(emitters, dtype=EMITTER_DTYPE)
+        planes = np.ascontiguousarray(planes, np.float32)
+        img = np.zeros((h, w), np.float32)
+        lo = None if span_lo is None else np.ascontiguousarray(span_lo, np.int16)
+        hi = None if span_hi is None else np.ascontiguousarray(span_hi, np.int16)
+        self.L.orc_render_frame_fft(_p(img, C.c_float), w, h, bg, None, 1.0,
+                                    emitters.ctypes.data if len(emitters) else None, len(emitters),
+                                    _p(planes, C.c_float), planes.shape[0], os_, planes.shape[1],
+                                    _p(lo, C.c_int16), _p(hi, C.c_int16), min_photons)
+        return img
+
     def wf_radius(self, size, os_, upscale):
         return self.L.orc_wf_radius(size, os_, upscale)
 

@@ -899,6 +899,215 @@ void orc_render_frame(float* img, uint32_t width, uint32_t height, double bg, co
    }
 }
 
+/* ---- the Fft placement (PsfInterpMode::Fft) ---------------------------------------------------------------- */
+
+/* Sim/FftRadix2.h:39-69 MakeTwiddles: per stage len the len/2 values of the recurrence from (1, 0); rev = the
+ * bit-reversal permutation. c, s: n - 1 doubles; rev: n ints. */
+static void radix2_twiddles(int n, int sign, double* c, double* s, int* rev)
+{
+   for (int len = 2; len <= n; len <<= 1)
+   {
+      const double ang = sign * 2.0 * 3.14159265358979323846 / len;
+      const double wr = cos(ang), wi = sin(ang);
+      const int half = len / 2;
+      double cr = 1.0, ci = 0.0;
+      for (int k = 0; k < half; ++k)
+      {
+         c[half - 1 + k] = cr;
+         s[half - 1 + k] = ci;
+         const double nr = cr * wr - ci * wi;
+         ci = cr * wi + ci * wr;
+         cr = nr;
+      }
+   }
+   rev[0] = 0;
+   for (int i = 1, j = 0; i < n; ++i)
+   {
+      int bit = n >> 1;
+      for (; j & bit; bit >>= 1)
+         j ^= bit;
+      j ^= bit;
+      rev[i] = j;
+   }
+}
+
+/* Sim/FftRadix2.h:74-105 Fft1d: in-place iterative radix-2, unnormalized */
+static void radix2_fft(double* re, double* im, int n, const double* c, const double* s, const int* rev)
+{
+   for (int i = 1; i < n; ++i)
+   {
+      const int j = rev[i];
+      if (i < j)
+      {
+         double t = re[i];
+         re[i] = re[j];
+         re[j] = t;
+         t = im[i];
+         im[i] = im[j];
+         im[j] = t;
+      }
+   }
+   for (int len = 2; len <= n; len <<= 1)
+   {
+      const int half = len / 2;
+      const double* tc = c + (half - 1);
+      const double* ts = s + (half - 1);
+      for (int i = 0; i < n; i += len)
+      {
+         double* ar = re + i;
+         double* ai = im + i;
+         double* br = re + i + half;
+         double* bi = im + i + half;
+         for (int k = 0; k < half; ++k)
+         {
+            const double cr = tc[k], ci = ts[k];
+            const double vr = br[k] * cr - bi[k] * ci, vi = br[k] * ci + bi[k] * cr;
+            br[k] = ar[k] - vr;
+            bi[k] = ai[k] - vi;
+            ar[k] += vr;
+            ai[k] += vi;
+         }
+      }
+   }
+}
+
+typedef struct
+{
+   int N;
+   double *fc, *fs, *ic, *is;
+   int* rev;
+} radix2_plan;
+
+/* Sim/PsfGeneratorBridge.cpp:1264-1281 FourierShiftLine */
+static void fourier_shift_line(double* re, double* im, const radix2_plan* p, const double* pc, const double* ps)
+{
+   const int N = p->N;
+   radix2_fft(re, im, N, p->fc, p->fs, p->rev);
+   for (int k = 0; k < N; ++k)
+   {
+      const double r = re[k], i = im[k];
+      re[k] = r * pc[k] - i * ps[k];
+      im[k] = r * ps[k] + i * pc[k];
+   }
+   radix2_fft(re, im, N, p->ic, p->is, p->rev);
+   const double norm = 1.0 / N;
+   for (int k = 0; k < N; ++k)
+   {
+      re[k] *= norm;
+      im[k] *= norm;
+   }
+}
+
+/* Sim/PsfGeneratorBridge.cpp:1293-1380 FftShiftKernelTile: g(i) = f(i + shift) by zero-padded (N = pow2 >= 2n)
+ * Fourier shifts along every row, then every column of the complex intermediate; the real part as float. */
+void orc_fft_shift_tile(const float* kernel, int32_t n, double shiftX, double shiftY, float* out)
+{
+   int N = 1;
+   while (N < 2 * n)
+      N <<= 1;
+   const int half = N / 2;
+   radix2_plan p;
+   p.N = N;
+   p.fc = (double*)malloc(sizeof(double) * (size_t)4 * N);
+   p.fs = p.fc + N;
+   p.ic = p.fs + N;
+   p.is = p.ic + N;
+   p.rev = (int*)malloc(sizeof(int) * (size_t)N);
+   radix2_twiddles(N, -1, p.fc, p.fs, p.rev);
+   radix2_twiddles(N, 1, p.ic, p.is, p.rev);
+   double* tab = (double*)malloc(sizeof(double) * (size_t)6 * N);
+   double *xc = tab, *xs = tab + N, *yc = tab + 2 * N, *ys = tab + 3 * N, *re = tab + 4 * N, *im = tab + 5 * N;
+   for (int k = 0; k < N; ++k)
+   {
+      const int kk = k < half ? k : k - N;
+      const double ax = 2.0 * 3.14159265358979323846 * kk * shiftX / N;
+      const double ay = 2.0 * 3.14159265358979323846 * kk * shiftY / N;
+      xc[k] = cos(ax);
+      xs[k] = sin(ax);
+      yc[k] = cos(ay);
+      ys[k] = sin(ay);
+   }
+   double* midRe = (double*)malloc(sizeof(double) * (size_t)2 * n * n);
+   double* midIm = midRe + (size_t)n * n;
+   for (int y = 0; y < n; ++y)
+   {
+      for (int k = 0; k < N; ++k)
+      {
+         re[k] = k < n ? kernel[(size_t)y * n + k] : 0.0;
+         im[k] = 0.0;
+      }
+      fourier_shift_line(re, im, &p, xc, xs);
+      for (int x = 0; x < n; ++x)
+      {
+         midRe[(size_t)y * n + x] = re[x];
+         midIm[(size_t)y * n + x] = im[x];
+      }
+   }
+   for (int x = 0; x < n; ++x)
+   {
+      for (int k = 0; k < N; ++k)
+      {
+         re[k] = k < n ? midRe[(size_t)k * n + x] : 0.0;
+         im[k] = k < n ? midIm[(size_t)k * n + x] : 0.0;
+      }
+      fourier_shift_line(re, im, &p, yc, ys);
+      for (int y = 0; y < n; ++y)
+         out[(size_t)y * n + x] = (float)re[y];
+   }
+   free(midRe);
+   free(tab);
+   free(p.rev);
+   free(p.fc);
+}
+
+/* RenderPhotonImage with the Fft placement: PlanSplat's Fft branch (Sim/PsfGeneratorBridge.cpp:1443-1456) per
+ * emitter -- the nearest setup, ONE Fourier shift of the raw plane by the sub-cell fraction, its block sums read
+ * nearest at the rounded grid position. planes: nz raw n x n planes. */
+void orc_render_frame_fft(float* img, uint32_t width, uint32_t height, double bg, const float* bgMap, double bgScale,
+                          const orc_emitter* ems, int32_t nEms, const float* planes, int32_t nzK, int32_t os_,
+                          int32_t n, const int16_t* spanLo, const int16_t* spanHi, double minPhotons)
+{
+   const int os = imax(1, os_), bw = n + os - 1;
+   const size_t npx = (size_t)width * height;
+   if (!bgMap && bgScale == 1.0)
+   {
+      const float v = (float)bg;
+      for (size_t i = 0; i < npx; ++i)
+         img[i] = v;
+   }
+   else
+      for (size_t i = 0; i < npx; ++i)
+      {
+         const double b = bgMap ? (double)bgMap[i] : bg;
+         img[i] = (float)(b * bgScale);
+      }
+   const int camRad = ((n - 1) / 2) / os;
+   const int nSpan = 2 * camRad + 1;
+   float* shifted = (float*)malloc(sizeof(float) * (size_t)n * n);
+   float* sums = (float*)malloc(sizeof(float) * (size_t)bw * bw);
+   for (int32_t e = 0; e < nEms; ++e)
+   {
+      const orc_emitter* em = &ems[e];
+      if (minPhotons > 0.0 && em->photons < minPhotons)
+         continue;
+      if (em->photons <= 0.0 || em->zIndex < 0 || em->zIndex >= nzK)
+         continue;
+      orc_splat_setup_t st;
+      orc_splat_setup(n, os, em->xPx, em->yPx, 0, &st);
+      const double kc = (n - 1) / 2.0;
+      const double tx = kc + (st.x0 - em->xPx - 0.5) * os + 0.5, ty = kc + (st.y0 - em->yPx - 0.5) * os + 0.5;
+      const double rx = floor(tx + 0.5), ry = floor(ty + 0.5);
+      orc_fft_shift_tile(planes + (size_t)em->zIndex * n * n, n, tx - rx, ty - ry, shifted);
+      orc_block_sums(shifted, n, os, sums);
+      st.bx = (int)rx;
+      st.by = (int)ry;
+      splat_rows(img, width, height, sums, bw, os, camRad, &st, em->photons,
+                 spanLo ? spanLo + (size_t)em->zIndex * nSpan : NULL, spanHi ? spanHi + (size_t)em->zIndex * nSpan : NULL);
+   }
+   free(sums);
+   free(shifted);
+}
+
 /* ======================================================================= */
 /* Mean field: Sim/WidefieldRender.cpp:263-323, cli/widefield_check.cpp    */
 /* ======================================================================= */

@@ -110,6 +110,16 @@ void orc_render_frame(float* img, uint32_t width, uint32_t height, double bg, co
                       int32_t os, int32_t sizeOversampled, int32_t interp, const int16_t* spanLo,
                       const int16_t* spanHi, double minPhotons);
 
+/* FftShiftKernelTile (Sim/PsfGeneratorBridge.cpp:1293-1380): the n x n kernel moved by (shiftX, shiftY) samples,
+ * g(i) = f(i + shift), by zero-padded FP64 radix-2 Fourier shifts along rows then columns. */
+void orc_fft_shift_tile(const float* kernel, int32_t n, double shiftX, double shiftY, float* out);
+
+/* RenderPhotonImage with the Fft placement (PlanSplat, Sim/PsfGeneratorBridge.cpp:1443-1456): per emitter one
+ * Fourier shift of its raw plane (planes: nz x n x n) and a nearest read of the shifted block sums. */
+void orc_render_frame_fft(float* img, uint32_t width, uint32_t height, double bg, const float* bgMap, double bgScale,
+                          const orc_emitter* ems, int32_t nEms, const float* planes, int32_t nzK, int32_t os,
+                          int32_t n, const int16_t* spanLo, const int16_t* spanHi, double minPhotons);
+
 /* ---- mean field (Sim/WidefieldRender.cpp:263-323,1460-1524; cli/widefield_check.cpp:287-343) ---- */
 /* Grid-pitch kernel plane p of an oversampled plane (KernelWidefieldPsf::Kernel): (2R+1)^2 floats. */
 int32_t orc_wf_radius(int32_t sizeOversampled, int32_t os, int32_t upscale);

@@ -12,7 +12,7 @@ Outputs (inputs and expected outputs only):
                      block sums of a small splat kernel (BuildZernikePsfKernelCache)
   noise_ref.npz      ApplyNoiseChain ADU for fixed photon images: sCMOS and EMCCD, with and without maps, pixelBase
   splat_ref.npz      CollectFrameEmitters, CollectGpuEmitters and RenderPhotonImage frames (Nearest, Linear, Cubic,
-                     halo cut, edge emitters) on seeded events
+                     Fft, halo cut, edge emitters) on seeded events
   wf_ref.npz         KernelWidefieldPsf radius and grid-pitch planes; WidefieldScene::FocusSeries images
   wf_jobs_ref.npz    the WidefieldGpuJobs of GPU-mode scenes (as sent to a WidefieldAccelerator) and the scenes' own
                      CPU images
@@ -145,6 +145,13 @@ def splat(R, P):
     out["frame_cubic_min_1"] = R.render_frame(w, h, ev, 1, 100.0, 1.0, 20.0, planes, 4, 50.0, 2, min_photons=900.0)
     out["frame_cubic_parallel_1"] = R.render_frame(w, h, ev, 1, 100.0, 1.0, 20.0, planes, 4, 50.0, 2,
                                                    parallel=True)
+    # the Fft placement (one Fourier shift of the raw plane per emitter), serial and parallel, and with a halo cut
+    for f in range(3):
+        out[f"frame_fft_{f}"] = R.render_frame(w, h, ev, f, 100.0, 1.0, 20.0, planes, 4, 50.0, 3)
+    out["frame_fft_parallel_1"] = R.render_frame(w, h, ev, 1, 100.0, 1.0, 20.0, planes, 4, 50.0, 3, parallel=True)
+    out["frame_fft_halo_1"] = R.render_frame(w, h, ev, 1, 100.0, 1.0, 20.0, planes, 4, 50.0, 3, halo=3e-4)
+    lo, hi, r2 = R.halo_spans(planes, 4, 3, 3e-4)
+    out["halo_fft_lo"], out["halo_fft_hi"] = lo, hi
     for f in range(3):
         em = R.collect_emitters(ev, f, w, h, 100.0, 1.0, planes.shape[0], 50.0)
         out[f"emitters_{f}"] = em.view(np.uint8)

@@ -149,3 +149,59 @@ def test_splat_extras_and_bg_scale(ctx, orc, golden):
     e1 = G["emitters_1"].view(O.EMITTER_DTYPE)
     want1 = orc.render_frame(w, h, 0.0, e1, bs, 65, 4, 2, bg_map=extras[1], bg_scale=1.0)
     assert np.array_equal(ph[1], want1)
+
+
+def test_splat_fft_placement_vs_reference(ctx, golden):
+    """The Fft placement (interp 3): one FP64 Fourier shift of the raw plane per emitter, the reference's radix-2
+    operations; the phase cos/sin are the device's, so a pixel may differ by a float rounding where one of them
+    differs by an ulp (none so far); required: at most 1 pixel in 10^4 and those within 1e-6 relative."""
+    G, P, ev, w, h = fixture(golden)
+    sp = host.SplatGpuHost(ctx)
+    cache = host.PsfKernelCache(P["small_planes"], oversampling=4, zStepNm=50.0, interpMode=3)
+    assert sp.SetKernels([cache]) and sp.SetStatic(w, h, flatBackground=20.0), sp.err
+    em, start = batch(G, [0, 1, 2])
+    ph = np.zeros((3, h, w), np.float32)
+    assert sp.RenderBatch(em, start, None, None, 0, outPhotons=ph), sp.err
+    for f in range(3):
+        want = G[f"frame_fft_{f}"]
+        diff = ph[f] != want
+        print("frame", f, "pixels differing", int(diff.sum()), flush=True)
+        assert diff.sum() <= max(1, want.size // 10000)
+        assert np.allclose(ph[f], want, rtol=1e-6, atol=1e-6)
+    halo = host.PsfHaloSpans(G["halo_fft_lo"], G["halo_fft_hi"], None, camRad=8, threshold=3e-4)
+    cache_h = host.PsfKernelCache(P["small_planes"], oversampling=4, zStepNm=50.0, interpMode=3, halo=halo)
+    assert sp.SetKernels([cache_h]), sp.err
+    em, start = batch(G, [1])
+    ph1 = np.zeros((1, h, w), np.float32)
+    assert sp.RenderBatch(em, start, None, None, 0, outPhotons=ph1), sp.err
+    assert np.allclose(ph1[0], G["frame_fft_halo_1"], rtol=1e-6, atol=1e-6)
+    assert (ph1[0] != G["frame_fft_halo_1"]).sum() <= 1
+
+
+def test_splat_fft_placement_many_emitters_vs_oracle(ctx, orc, golden):
+    """200 emitters over 4 frames (noise fused too), against the FP64 restatement, and the records path refusing
+    Fft stacks as the reference's GPU host does."""
+    P = golden("psf_ref.npz")
+    planes = P["small_planes"]
+    w, h = 64, 56
+    ev, st = configs.blink_events(11, 4, w, h, 100.0, 50.0, 200.0, 1500.0)
+    cache = host.PsfKernelCache(planes, oversampling=4, zStepNm=50.0, interpMode=3)
+    ems = [host.CollectFrameEmitters(ev[st[f]:st[f + 1]], f, 100.0, 1.0, psfCache=cache) for f in range(4)]
+    start = np.zeros(5, np.uint32)
+    start[1:] = np.cumsum([len(e) for e in ems])
+    em = np.concatenate(ems)
+    sp = host.SplatGpuHost(ctx)
+    assert sp.SetKernels([cache]) and sp.SetStatic(w, h, flatBackground=5.0), sp.err
+    ph = np.zeros((4, h, w), np.float32)
+    assert sp.RenderBatch(em, start, None, None, 0, outPhotons=ph), sp.err
+    bad = 0
+    for f in range(4):
+        want = orc.render_frame_fft(w, h, 5.0, ems[f].view(O.EMITTER_DTYPE), planes, 4)
+        assert np.allclose(ph[f], want, rtol=1e-6, atol=1e-6)
+        bad += int((ph[f] != want).sum())
+    print("pixels differing from the FP64 restatement", bad, flush=True)
+    assert bad <= max(1, ph.size // 10000)
+    rec = host.CollectGpuEmitters(ems[0], host.PsfKernelCache(planes, oversampling=4, zStepNm=50.0, interpMode=2))
+    assert not sp.RenderBatch(rec, np.array([0, len(rec)], np.uint32), np.zeros(1, np.uint32), None, 0,
+                              outPhotons=ph[:1], records=True)
+    assert "Fft" in sp.err

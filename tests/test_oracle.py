@@ -270,6 +270,50 @@ def test_render_frame_bit_exact_vs_reference(orc, golden, interp, name):
     assert np.array_equal(G["frame_cubic_parallel_1"], G["frame_cubic_1"])
 
 
+def test_render_frame_fft_placement_bit_exact_vs_reference(orc, golden):
+    """The Fft placement (one zero-padded FP64 Fourier shift of the raw plane per emitter, block sums read nearest)
+    equals the compiled reference's frames bit for bit, with and without the halo cut."""
+    G, P = golden("splat_ref.npz"), golden("psf_ref.npz")
+    ev, w, h, planes = splat_inputs(G, P)
+    for f in range(3):
+        em = G[f"emitters_{f}"].view(O.EMITTER_DTYPE)
+        assert np.array_equal(orc.render_frame_fft(w, h, 20.0, em, planes, 4), G[f"frame_fft_{f}"])
+    em = G["emitters_1"].view(O.EMITTER_DTYPE)
+    img = orc.render_frame_fft(w, h, 20.0, em, planes, 4, span_lo=G["halo_fft_lo"], span_hi=G["halo_fft_hi"])
+    assert np.array_equal(img, G["frame_fft_halo_1"])
+    assert np.array_equal(G["frame_fft_parallel_1"], G["frame_fft_1"])
+
+
+def test_fft_shift_tile_closed_forms(orc):
+    """Shift 0 is the identity; a whole-sample shift moves the tile by one sample with zeros coming in (the padding
+    is >= 2n, so nothing wraps); a half-sample shift of a smooth, band-limited line matches its analytic value."""
+    rng = np.random.default_rng(8)
+    n = 17
+    k = rng.random((n, n)).astype(np.float32)
+    assert np.allclose(orc.fft_shift_tile(k, 0.0, 0.0), k, atol=1e-6)
+    sx = orc.fft_shift_tile(k, 1.0, 0.0)
+    want = np.zeros_like(k)
+    want[:, :-1] = k[:, 1:]
+    assert np.allclose(sx, want, atol=1e-6)
+    sy = orc.fft_shift_tile(k, 0.0, -2.0)
+    want = np.zeros_like(k)
+    want[2:, :] = k[:-2, :]
+    assert np.allclose(sy, want, atol=1e-6)
+    # a cosine of period N along x (N = 64 for n = 17 -> pow2 >= 34 is 64): g(i) = f(i + 1/2) exactly in exact math
+    N = 64
+    x = np.arange(n)
+    line = np.cos(2 * np.pi * 3 * x / N)
+    tile = np.tile(line, (n, 1)).astype(np.float32)
+    got = orc.fft_shift_tile(tile, 0.5, 0.0)
+    # the zero padding cuts the cosine at n, so compare against the shift of the padded (windowed) line
+    padded = np.zeros(N)
+    padded[:n] = tile[0]
+    spec = np.fft.fft(padded)
+    kk = np.where(np.arange(N) < N // 2, np.arange(N), np.arange(N) - N)
+    ref = np.fft.ifft(spec * np.exp(2j * np.pi * kk * 0.5 / N)).real[:n]
+    assert np.allclose(got[0], ref, atol=1e-6)
+
+
 def test_render_frame_halo_and_min_photons(orc, golden):
     G, P = golden("splat_ref.npz"), golden("psf_ref.npz")
     ev, w, h, planes = splat_inputs(G, P)
