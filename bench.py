@@ -272,7 +272,7 @@ def run_psf(ctx, steps, warmup):
     import torch
     from insiliscope_amd import configs, host
     L = ctx.L
-    reqs = configs.c3_requests(seed=1)
+    reqs = host.PsfRequestArray(configs.c3_requests(seed=1))  # the requests as the C structs the seam takes
     out = torch.empty((len(reqs), 41, 65, 65), dtype=torch.float32, device="cuda")
 
     def step():

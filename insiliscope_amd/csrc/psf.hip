@@ -17,6 +17,7 @@
 #include "iscg_internal.h"
 
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <type_traits>
 
@@ -33,6 +34,9 @@ struct PsfReqDev
    double k0, NA, ni, ns, bMax, kMax, dk, kMin, x0, dx, depth, zStepM, ti0, focalShift, maskWaist;
    double coeff[28];
    int n[28], l[28], m[28];
+   // sum_t coeff_t R_t(rho) {cos | sin}(m_t phi) folded into polynomials: zc[0][m][j] multiplies rho^j cos(m phi),
+   // zc[1][m][j] rho^j sin(m phi) (OSA j < 28: n <= 6)
+   double zc[2][7][7];
    int M, size, nz, nModes, doubleHelix, maskModes, pad[2];
 };
 
@@ -48,29 +52,6 @@ void destroy_psf(iscg_context* h)
 }
 
 // ---- tables -------------------------------------------------------------------------------------------------
-
-__device__ double binomial_d(int a, int b)
-{
-   if (b < 0 || b > a)
-      return 0.0;
-   double r = 1.0;
-   for (int i = 0; i < b; ++i)
-      r = r * (a - i) / (i + 1);
-   return r;
-}
-
-// Sim/ZernikePsf.cpp:58-78 (unnormalised radial polynomial, cos for l >= 0, sin for l < 0, zero beyond rho 1)
-__device__ double zernike_value_d(int n, int l, int m, double rho, double phi)
-{
-   double radial = 0.0;
-   if (rho <= 1.0)
-      for (int k = 0; k <= (n - m) / 2; ++k)
-      {
-         const double coeff = (k % 2 == 0 ? 1.0 : -1.0) * binomial_d(n - k, k) * binomial_d(n - 2 * k, (n - m) / 2 - k);
-         radial += coeff * pow(rho, (double)(n - 2 * k));
-      }
-   return radial * (l >= 0 ? cos(m * phi) : sin(m * phi));
-}
 
 // Sim/ZernikePsf.cpp:80-110 (Gauss-Laguerre double-helix mask phase)
 __device__ double mask_phase_d(int maskModes, double maskWaist, double rhoNorm, double phi)
@@ -120,16 +101,39 @@ __global__ void psf_tables_kernel(const PsfReqDev* __restrict__ reqs, int nReq, 
          {
             const double kr = sqrt(kr2);
             const double rho = kr / (q.k0 * q.NA);
-            const double phi = atan2(ky, kx);
             const double s1 = q.NA * rho / q.ns, s3 = q.NA * rho / q.ni;
             const double a1 = 1.0 - s1 * s1, a3 = 1.0 - s3 * s3;
             const double opd1 = q.ns * q.depth * sqrt(0.0 < a1 ? a1 : 0.0);
             const double opd3Factor = sqrt(0.0 < a3 ? a3 : 0.0);
+            // the Zernike sum (ZernikeValue, Sim/ZernikePsf.cpp:58-78; zero for rho / bMax > 1) from the folded
+            // polynomials: rho^j by products, cos / sin (m phi) by angle addition from kx / kr, ky / kr
             double zp = 0.0;
-            for (int t = 0; t < q.nModes; ++t)
-               zp += q.coeff[t] * zernike_value_d(q.n[t], q.l[t], q.m[t], rho / q.bMax, phi);
+            const double rn = rho / q.bMax;
+            if (q.nModes > 0 && rn <= 1.0)
+            {
+               double pw[7];
+               pw[0] = 1.0;
+               for (int j = 1; j < 7; ++j)
+                  pw[j] = pw[j - 1] * rn;
+               const double c1 = kr > 0.0 ? kx / kr : 1.0, s1 = kr > 0.0 ? ky / kr : 0.0;
+               double cm = 1.0, sm = 0.0;
+               for (int m = 0; m < 7; ++m)
+               {
+                  double rc = 0.0, rs = 0.0;
+                  for (int j = m; j < 7; ++j)
+                  {
+                     rc += q.zc[0][m][j] * pw[j];
+                     rs += q.zc[1][m][j] * pw[j];
+                  }
+                  zp += rc * cm + rs * sm;
+                  const double cn = cm * c1 - sm * s1;
+                  sm = sm * c1 + cm * s1;
+                  cm = cn;
+               }
+            }
             zp *= 2.0 * kPi;
-            const double mask = q.doubleHelix ? mask_phase_d(q.maskModes, q.maskWaist, rho / q.bMax, phi) : 0.0;
+            const double mask =
+               q.doubleHelix ? mask_phase_d(q.maskModes, q.maskWaist, rho / q.bMax, atan2(ky, kx)) : 0.0;
             a = q.k0 * opd1 + zp + mask;
             b = q.k0 * (q.ni * opd3Factor);
          }
@@ -189,17 +193,19 @@ __device__ __forceinline__ float4v mfma3(half8 ahi, half8 alo, half8 bhi, half8 
    return mfma(alo, bhi, acc);
 }
 
-// One workgroup (4 waves) per (request, plane, block of YB*16 output rows). MP: padded pupil samples (multiple of
+// One workgroup (kPsfWaves waves) per (request, plane, block of YB*16 output rows). MP: padded pupil samples (multiple of
 // 32), PP: padded output size (multiple of 16).
+constexpr int kPsfWaves = 8, kPsfThreads = 64 * kPsfWaves;
+
 template <int MP, int YB>
-__global__ __launch_bounds__(256) void psf_planes_kernel(const PsfReqDev* __restrict__ reqs, const double* __restrict__ A,
+__global__ __launch_bounds__(kPsfThreads) void psf_planes_kernel(const PsfReqDev* __restrict__ reqs, const double* __restrict__ A,
                                                          const double* __restrict__ B, const _Float16* __restrict__ W,
                                                          const double* __restrict__ tz, int nz, int Pp, int nYBlocks,
                                                          float* __restrict__ out)
 {
    constexpr int CT1 = MP / 16;              // column tiles of mid
    constexpr int T1 = YB * CT1;              // tiles of GEMM 1
-   constexpr int PER1 = (T1 + 3) / 4;        // per wave
+   constexpr int PER1 = (T1 + kPsfWaves - 1) / kPsfWaves;  // per wave
    constexpr int MS = MP + 8;                // LDS row of mid (halves)
    __shared__ __attribute__((aligned(16))) _Float16 pup[4][MP][kKP];      // pupil chunk, [part][m][k]
    __shared__ __attribute__((aligned(16))) _Float16 mid[4][YB * 16][MS];  // mid rows, [part][y][m]
@@ -227,7 +233,7 @@ __global__ __launch_bounds__(256) void psf_planes_kernel(const PsfReqDev* __rest
    for (int n0 = 0; n0 < MP; n0 += 32)
    {
       // the pupil rows [n0, n0 + 32): FP64 phase, wrapped, FP32 sincos, split
-      for (int i = tid; i < 32 * MP; i += 256)
+      for (int i = tid; i < 32 * MP; i += kPsfThreads)
       {
          const int k = i / MP, m = i % MP;
          const size_t o = (size_t)(n0 + k) * MP + m;
@@ -242,7 +248,7 @@ __global__ __launch_bounds__(256) void psf_planes_kernel(const PsfReqDev* __rest
 #pragma unroll
       for (int s = 0; s < PER1; ++s)
       {
-         const int ti = wave + 4 * s;
+         const int ti = wave + kPsfWaves * s;
          // rows at or beyond Pp (the last block's padding) stay zero: W holds Pp rows only
          if (ti < T1 && y0 + (ti / CT1) * 16 < Pp)
          {
@@ -265,7 +271,7 @@ __global__ __launch_bounds__(256) void psf_planes_kernel(const PsfReqDev* __rest
 #pragma unroll
    for (int s = 0; s < PER1; ++s)
    {
-      const int ti = wave + 4 * s;
+      const int ti = wave + kPsfWaves * s;
       if (ti < T1)
       {
          const int rb = ti / CT1, cb = ti % CT1;
@@ -283,7 +289,7 @@ __global__ __launch_bounds__(256) void psf_planes_kernel(const PsfReqDev* __rest
    // ---- GEMM 2: E[y][x] = sum_m mid[y][m] W[x][m]; |E|^2 out ----
    const int CT2 = Pp / 16, T2 = YB * CT2;
    float* plane = out + ((size_t)r * nz + zi) * P * P;
-   for (int ti = wave; ti < T2; ti += 4)
+   for (int ti = wave; ti < T2; ti += kPsfWaves)
    {
       const int rb = ti / CT2, cb = ti % CT2;
       if (y0 + rb * 16 >= P)
@@ -411,6 +417,25 @@ static PsfReqDev request_dev(const iscg_psf_request* rq, int size, int nz, int M
       ++nm;
    }
    d.nModes = nm;
+   // the folded polynomials: R_n^m(rho) = sum_k (-1)^k C(n-k, k) C(n-2k, (n-m)/2 - k) rho^(n-2k)
+   std::memset(d.zc, 0, sizeof(d.zc));
+   auto binom = [](int a, int b) {
+      if (b < 0 || b > a)
+         return 0.0;
+      double r = 1.0;
+      for (int i = 0; i < b; ++i)
+         r = r * (a - i) / (i + 1);
+      return r;
+   };
+   for (int t = 0; t < nm; ++t)
+   {
+      const int n = d.n[t], m = d.m[t];
+      for (int k = 0; k <= (n - m) / 2; ++k)
+      {
+         const double c = (k % 2 == 0 ? 1.0 : -1.0) * binom(n - k, k) * binom(n - 2 * k, (n - m) / 2 - k);
+         d.zc[d.l[t] >= 0 ? 0 : 1][m][n - 2 * k] += d.coeff[t] * c;
+      }
+   }
    return d;
 }
 
@@ -418,7 +443,8 @@ template <int MP, int YB>
 static void launch_planes(hipStream_t s, int items, const PsfReqDev* reqs, const double* A, const double* B,
                           const _Float16* W, const double* tz, int nz, int Pp, int nYB, float* out)
 {
-   hipLaunchKernelGGL((psf_planes_kernel<MP, YB>), dim3(items), dim3(256), 0, s, reqs, A, B, W, tz, nz, Pp, nYB, out);
+   hipLaunchKernelGGL((psf_planes_kernel<MP, YB>), dim3(items), dim3(kPsfThreads), 0, s, reqs, A, B, W, tz, nz, Pp,
+                      nYB, out);
 }
 
 // Output-row blocks of YB tiles keep the pupil chunk (MP x 40 halves x 4) and the mid rows (YB 16 x (MP + 8) x 4)

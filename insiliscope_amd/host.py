@@ -457,13 +457,21 @@ def PsfGeometry(fields):
     return s.value, z.value, m.value
 
 
+def PsfRequestArray(requests):
+    """The C array of iscg_psf_request for a list of requests (dicts or PsfRequest): build it once and pass it to
+    ComputePsfPlanesBatch when the same requests are computed again."""
+    reqs = [psf_request(r) for r in requests]
+    return (_lib.PsfRequest * len(reqs))(*reqs)
+
+
 def ComputePsfPlanesBatch(requests, normalize: bool = False, out=None, ctx: Context | None = None):
     """ComputeZernikePsfPlanes (Sim/ZernikePsf.h:51) for many requests sharing size, nz and pupil samples, in one
-    launch: (nReq, nz, size, size) float32 (sum-1 planes when normalize, as BuildZernikePsfKernelCache)."""
+    launch: (nReq, nz, size, size) float32 (sum-1 planes when normalize, as BuildZernikePsfKernelCache).
+    requests: a list of requests, or a PsfRequestArray."""
     ctx = ctx or default_context()
-    reqs = [psf_request(r) for r in requests]
-    size, nz, _ = PsfGeometry(reqs[0])
-    arr = (_lib.PsfRequest * len(reqs))(*reqs)
+    arr = requests if isinstance(requests, C.Array) else PsfRequestArray(requests)
+    reqs = arr
+    size, nz, _ = PsfGeometry(arr[0])
     if out is None:
         out = np.zeros((len(reqs), nz, size, size), np.float32)
     p, keep = _ptr(out)
