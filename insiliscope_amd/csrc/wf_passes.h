@@ -26,11 +26,7 @@ __global__ __launch_bounds__(256) void rows_r2c_kernel(const float* __restrict__
 {
    constexpr int T = N / 8, LINES = 256 / T > 0 ? 256 / T : 1, LS = N + 1;
    __shared__ float2 buf[LINES * LS];
-   __shared__ float2 twl[N];  // the twiddles, read by every stage
    const int tid = threadIdx.x;
-   for (int i = tid; i < N; i += blockDim.x)
-      twl[i] = tw[i];
-   __syncthreads();
    const int pairs = (nRows + 1) / 2;
    for (int p0 = blockIdx.x * LINES; p0 < pairs; p0 += gridDim.x * LINES)
    {
@@ -47,7 +43,7 @@ __global__ __launch_bounds__(256) void rows_r2c_kernel(const float* __restrict__
          buf[l * LS + n] = make_float2(a, b);
       }
       __syncthreads();
-      fft_lines<N, false, false>(buf, LS, 1, LINES, tid, twl);
+      fft_lines<N, false, false>(buf, LS, 1, LINES, tid, tw);
       for (int i = tid; i < LINES * (N / 2 + 1); i += blockDim.x)
       {
          const int l = i / (N / 2 + 1), k = i % (N / 2 + 1), pr = p0 + l;
@@ -125,11 +121,7 @@ __global__ __launch_bounds__(256) void rows_c2r_kernel(const float2* __restrict_
 {
    constexpr int T = N / 8, LINES = 256 / T > 0 ? 256 / T : 1, LS = N + 1;
    __shared__ float2 buf[LINES * LS];
-   __shared__ float2 twl[N];  // the twiddles, read by every stage
    const int tid = threadIdx.x;
-   for (int i = tid; i < N; i += blockDim.x)
-      twl[i] = tw[i];
-   __syncthreads();
    const int pairs = (nRows + 1) / 2;
    for (int p0 = blockIdx.x * LINES; p0 < pairs; p0 += gridDim.x * LINES)
    {
@@ -154,7 +146,7 @@ __global__ __launch_bounds__(256) void rows_c2r_kernel(const float2* __restrict_
             buf[l * LS + N - k] = make_float2(A.x + B.y, B.x - A.y);
       }
       __syncthreads();
-      fft_lines<N, true, false>(buf, LS, 1, LINES, tid, twl);
+      fft_lines<N, true, false>(buf, LS, 1, LINES, tid, tw);
       for (int i = tid; i < LINES * 2 * e.cw; i += blockDim.x)
       {
          const int l = i / (2 * e.cw), rem = i % (2 * e.cw), half = rem / e.cw, x = rem % e.cw, pr = p0 + l;
