@@ -284,6 +284,19 @@ int32_t iscg_wf_fits(iscg_context* h, uint64_t geometry, uint32_t NX, uint32_t N
  * kernel is neither carried nor resident fails the call, as the reference's host does. */
 int32_t iscg_wf_images(iscg_context* h, const iscg_wf_job* job, float* out);
 
+/* WidefieldImages::Render (Sim/WidefieldRender.cpp:547-580) for nFrames sets of bleach coefficients, and the camera
+ * noise of the frames. channels: [nChannels][ch][cw] cell images (channel 0 the persistent one when hasPersistent,
+ * then the bleach maps: iscg_wf_images' output is this layout); a: [nFrames][nChannels - hasPersistent] doubles.
+ * Frame k, per camera pixel of the (ch/u) x (cw/u) image: base (base[k], or 0 when base is NULL) + the FP64 sum over
+ * its u x u cells of max(0, P + sum_j (float)a[k][j] B_j) (FP32, j up to the frame's last non-zero coefficient), in
+ * the reference's operation order (bit for bit). outPhotons: [nFrames][ch/u][cw/u] floats, or NULL. With outAdu: the
+ * noise chain of ApplyNoiseChain on those photons (cam, frame ids frames[k], pixel + pixelBase, offset map or NULL),
+ * [nFrames][ch/u][cw/u] uint16. */
+int32_t iscg_wf_render(iscg_context* h, const float* channels, int32_t nChannels, int32_t hasPersistent, uint32_t cw,
+                       uint32_t ch, int32_t upscale, const double* a, uint32_t nFrames, const float* base,
+                       const iscg_camera* cam, const float* offset, uint32_t seed, const uint32_t* frames,
+                       uint32_t pixelBase, float* outPhotons, uint16_t* outAdu);
+
 /* The device memory the resident plane and kernel spectra of all geometries may take (default: half of the
  * device's). When a job needs more, the least recently used other geometries are dropped first. No reference
  * counterpart (the reference's hosts size their pools themselves). */

@@ -98,6 +98,7 @@ SIGNATURES = {
     "iscg_wf_fits": (I32, [P, C.c_uint64, U32, U32, C.c_uint64]),
     "iscg_wf_images": (I32, [P, C.POINTER(WfJob), P]),
     "iscg_wf_set_budget": (I32, [P, C.c_uint64]),
+    "iscg_wf_render": (I32, [P, P, I32, I32, U32, U32, I32, P, U32, P, C.POINTER(Camera), P, U32, P, U32, P, P]),
     "iscg_noise_set_path": (I32, [P, I32]),
     "iscg_noise_bounds": (I32, [P, P]),
     "iscg_noise": (I32, [P, P, U32, U32, C.POINTER(Camera), P, P, P, U32, P, U32, U32, P]),

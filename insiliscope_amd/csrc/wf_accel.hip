@@ -273,6 +273,58 @@ __global__ __launch_bounds__(256) void wf_mac_kernel(const float2* __restrict__ 
    }
 }
 
+// WidefieldImages::Render (Sim/WidefieldRender.cpp:547-580) per (frame, camera pixel): v = persistent + sum_j
+// (float)a_j bleach_j in FP32 for j below the frame's last non-zero coefficient, max(0, v) summed in FP64 over the
+// u x u cells, added (FP32) to the base; then the camera noise when wanted.
+template <bool NOISE, bool EMCCD>
+__global__ __launch_bounds__(256) void wf_render_kernel(const float* __restrict__ chans, int nBleach, int hasP,
+                                                        uint32_t cw, uint32_t ch, int u, const double* __restrict__ a,
+                                                        uint32_t nFrames, const float* __restrict__ base,
+                                                        float* __restrict__ outPh, uint16_t* __restrict__ outAdu,
+                                                        CamParams cp, NoiseQueue q, const float* __restrict__ offset,
+                                                        uint32_t seed, const uint32_t* __restrict__ frames,
+                                                        uint32_t pixelBase)
+{
+   const uint32_t W = cw / u, H = ch / u, npx = W * H;
+   const size_t n = (size_t)cw * ch;
+   const float* P = hasP ? chans : nullptr;
+   const float* B = chans + (hasP ? n : 0);
+   const uint32_t total = npx * nFrames;
+   for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x)
+   {
+      const uint32_t k = idx / npx, p = idx - k * npx;
+      const uint32_t Y = p / W, X = p - Y * W;
+      const double* ak = a + (size_t)k * nBleach;
+      int nb = 0;
+      for (int j = 0; j < nBleach; ++j)
+         if (ak[j] != 0.0)
+            nb = j + 1;
+      float photon = base ? base[idx] : 0.0f;
+      if (P || nb > 0)
+      {
+         double acc = 0.0;
+         for (int sy = 0; sy < u; ++sy)
+         {
+            const size_t row = (size_t)(Y * u + sy) * cw + (size_t)X * u;
+            for (int sx = 0; sx < u; ++sx)
+            {
+               const size_t i = row + sx;
+               float v = P ? P[i] : 0.0f;
+               for (int j = 0; j < nb; ++j)
+                  v = v + (float)ak[j] * B[(size_t)j * n + i];
+               acc += v > 0.0f ? v : 0.0f;
+            }
+         }
+         photon = photon + (float)acc;
+      }
+      if (outPh)
+         outPh[idx] = photon;
+      if (NOISE)
+         noise_store<EMCCD>(cp, q, outAdu, idx, photon, seed, frames[k], p + pixelBase, offset != nullptr,
+                            offset ? offset[p] : 0.0f, false, 0.0f, false, 0.0f);
+   }
+}
+
 }  // namespace iscg
 
 using namespace iscg;
@@ -549,6 +601,65 @@ int32_t iscg_wf_images(iscg_context* h, const iscg_wf_job* job, float* out)
    if (!ok(h, hipGetLastError(), "iscg_wf_images launch"))
       return -1;
    if (!finish_out(h, out, dOut, nC * J.ch * J.cw * sizeof(float)))
+      return -1;
+   return 0;
+}
+
+int32_t iscg_wf_render(iscg_context* h, const float* channels, int32_t nChannels, int32_t hasPersistent, uint32_t cw,
+                       uint32_t ch, int32_t upscale, const double* a, uint32_t nFrames, const float* base,
+                       const iscg_camera* cam, const float* offset, uint32_t seed, const uint32_t* frames,
+                       uint32_t pixelBase, float* outPhotons, uint16_t* outAdu)
+{
+   if (begin(h))
+      return -1;
+   const int hasP = hasPersistent ? 1 : 0, u = upscale > 1 ? upscale : 1;
+   const int nBleach = nChannels - hasP;
+   if (!channels || nChannels < 1 || nBleach < 0 || cw == 0 || ch == 0 || (nBleach > 0 && !a))
+      return fail(h, "iscg_wf_render: null argument or no channel");
+   if (!outPhotons && !outAdu)
+      return fail(h, "iscg_wf_render: no output");
+   if (outAdu && (!cam || !frames))
+      return fail(h, "iscg_wf_render: the noise needs a camera and frame ids");
+   const uint32_t W = cw / (uint32_t)u, H = ch / (uint32_t)u;
+   const size_t npx = (size_t)W * H, nOut = npx * nFrames;
+   if (nFrames == 0 || npx == 0)
+      return 0;
+   if (nOut > 0xFFFFFFFFull)
+      return fail(h, "iscg_wf_render: at most 2^32 - 1 output pixels per call");
+   const float* dCh = (const float*)stage_in(h, 0, channels, (size_t)nChannels * cw * ch * sizeof(float));
+   const double* dA = nBleach ? (const double*)stage_in(h, 1, a, (size_t)nFrames * nBleach * sizeof(double)) : nullptr;
+   const float* dBase = base ? (const float*)stage_in(h, 2, base, nOut * sizeof(float)) : nullptr;
+   const float* dOff = offset ? (const float*)stage_in(h, 3, offset, npx * sizeof(float)) : nullptr;
+   const uint32_t* dFr = frames ? (const uint32_t*)stage_in(h, 4, frames, nFrames * sizeof(uint32_t)) : nullptr;
+   float* dPh = outPhotons ? (float*)stage_out(h, 5, outPhotons, nOut * sizeof(float)) : nullptr;
+   uint16_t* dAdu = outAdu ? (uint16_t*)stage_out(h, 6, outAdu, nOut * sizeof(uint16_t)) : nullptr;
+   if (!dCh || (nBleach && !dA) || (base && !dBase) || (offset && !dOff) || (frames && !dFr) ||
+       (outPhotons && !dPh) || (outAdu && !dAdu))
+      return -1;
+   CamParams cp{};
+   NoiseQueue q{nullptr, nullptr};
+   if (dAdu && (!cam_params_for(h, cam, cp) || !noise_queue(h, nOut, q)))
+      return -1;
+   const unsigned grid = grid_for(nOut, 256, 16);
+   {
+      ProfScope ps(h, "wf_render");
+      if (!dAdu)
+         hipLaunchKernelGGL((wf_render_kernel<false, false>), dim3(grid), dim3(256), 0, h->stream, dCh, nBleach, hasP,
+                            cw, ch, u, dA, nFrames, dBase, dPh, dAdu, cp, q, dOff, seed, dFr, pixelBase);
+      else if (cp.emccd)
+         hipLaunchKernelGGL((wf_render_kernel<true, true>), dim3(grid), dim3(256), 0, h->stream, dCh, nBleach, hasP,
+                            cw, ch, u, dA, nFrames, dBase, dPh, dAdu, cp, q, dOff, seed, dFr, pixelBase);
+      else
+         hipLaunchKernelGGL((wf_render_kernel<true, false>), dim3(grid), dim3(256), 0, h->stream, dCh, nBleach, hasP,
+                            cw, ch, u, dA, nFrames, dBase, dPh, dAdu, cp, q, dOff, seed, dFr, pixelBase);
+      if (!ok(h, hipGetLastError(), "iscg_wf_render launch"))
+         return -1;
+   }
+   if (dAdu && !launch_noise_fallback(h, q, dAdu, (uint32_t)npx, dFr, 0, dOff, nullptr, nullptr, cp, seed, pixelBase))
+      return -1;
+   if (outPhotons && !finish_out(h, outPhotons, dPh, nOut * sizeof(float)))
+      return -1;
+   if (outAdu && !finish_out(h, outAdu, dAdu, nOut * sizeof(uint16_t)))
       return -1;
    return 0;
 }

@@ -644,6 +644,43 @@ class WidefieldGpuHost:
         return True
 
 
+def WidefieldImagesRender(channels, hasPersistent: bool, upscale: int, a, base=None, cam: CameraNoiseParams | None = None,
+                          offset=None, seed: int = 0, frames=None, pixelBase: int = 0, outPhotons=None, outAdu=None,
+                          ctx: Context | None = None):
+    """WidefieldImages::Render (Sim/WidefieldRender.cpp:547-580) of cell images channels (nCh, ch, cw) -- channel 0
+    persistent when hasPersistent -- for each row of bleach coefficients a (nFrames, nCh - hasPersistent), onto base
+    (nFrames, ch/u, cw/u) or zeros; with cam, the camera noise of the frames (frame ids `frames`). Returns
+    (photons or None, adu or None); device tensors in give device tensors out."""
+    ctx = ctx or default_context()
+    chans = _arr(channels, np.float32)
+    nC, chh, cw = (int(x) for x in chans.shape)
+    aa = _arr(a, np.float64)
+    n_frames = int(aa.shape[0]) if aa is not None and len(aa.shape) == 2 else 1
+    u = max(1, int(upscale))
+    shape = (n_frames, chh // u, cw // u)
+    dev = hasattr(chans, "is_cuda") and chans.is_cuda
+    if outPhotons is None and cam is None:
+        if dev:
+            import torch
+            outPhotons = torch.empty(shape, dtype=torch.float32, device=chans.device)
+        else:
+            outPhotons = np.zeros(shape, np.float32)
+    if cam is not None and outAdu is None:
+        if dev:
+            import torch
+            outAdu = torch.empty(shape, dtype=torch.int16, device=chans.device)
+        else:
+            outAdu = np.zeros(shape, np.uint16)
+    fr = _arr(frames if frames is not None else np.arange(n_frames), np.uint32) if cam is not None else None
+    ptrs = [_ptr(x) for x in (chans, aa, _arr(base, np.float32), _arr(offset, np.float32), fr, outPhotons, outAdu)]
+    cc = cam.to_c() if cam is not None else None
+    ctx.follow_torch()
+    ctx.check(ctx.L.iscg_wf_render(ctx.h, ptrs[0][0], nC, int(bool(hasPersistent)), cw, chh, u, ptrs[1][0], n_frames,
+                                   ptrs[2][0], C.byref(cc) if cc else None, ptrs[3][0], seed & 0xFFFFFFFF,
+                                   ptrs[4][0], pixelBase, ptrs[5][0], ptrs[6][0]), "iscg_wf_render")
+    return outPhotons, outAdu
+
+
 # ---- differentiable ROI model -------------------------------------------------------------------------------------
 
 ROI_PARAMS_DTYPE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("photons", "f8"), ("background", "f8")])

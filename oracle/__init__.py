@@ -144,6 +144,7 @@ class Oracle:
         L.orc_fft_shift_tile.argtypes = [PF, I32, D, D, PF]
         L.orc_render_frame_fft.argtypes = [PF, U32, U32, D, PF, D, C.c_void_p, I32, PF, I32, I32, I32, PI16, PI16,
                                            D]
+        L.orc_wf_images_render.argtypes = [PF, I32, I32, U32, U32, I32, PD, I32, PF]
         L.orc_wf_radius.argtypes = [I32, I32, I32]
         L.orc_wf_radius.restype = I32
         L.orc_wf_kernel.argtypes = [PF, I32, I32, I32, I32, PF]
@@ -270,6 +271,17 @@ class Oracle:
                                     _p(lo, C.c_int16), _p(hi, C.c_int16), min_photons)
         return img
 
+    def wf_images_render(self, channels, has_persistent, upscale, a, cam=None):
+        ch_ = np.ascontiguousarray(channels, np.float32)
+        nC, chh, cw = ch_.shape
+        a = np.ascontiguousarray(a, np.float64)
+        if cam is None:
+            cam = np.zeros((chh // max(1, upscale), cw // max(1, upscale)), np.float32)
+        cam = np.ascontiguousarray(cam, np.float32).copy()
+        self.L.orc_wf_images_render(_p(ch_, C.c_float), nC, int(has_persistent), cw, chh, upscale, _p(a, C.c_double),
+                                    len(a), _p(cam, C.c_float))
+        return cam
+
     def wf_radius(self, size, os_, upscale):
         return self.L.orc_wf_radius(size, os_, upscale)
 
@@ -329,7 +341,7 @@ class Reference:
         L = C.CDLL(path)
         self.L = L
         D, I32, U32, I64 = C.c_double, C.c_int32, C.c_uint32, C.c_int64
-        PF, PI16, PI32 = C.POINTER(C.c_float), C.POINTER(C.c_int16), C.POINTER(C.c_int32)
+        PF, PI16, PI32, PD = C.POINTER(C.c_float), C.POINTER(C.c_int16), C.POINTER(C.c_int32), C.POINTER(C.c_double)
         L.ref_psf_planes.argtypes = [C.POINTER(PsfRequest), PF]
         L.ref_psf_planes.restype = I32
         L.ref_psf_kernel_cache.argtypes = [C.POINTER(PsfRequest), PF, PF]
@@ -359,6 +371,8 @@ class Reference:
         L.ref_wf_capture_jobs.argtypes = [PF, I32, I32, I32, D, PF, I32, I32, I32, D, D, C.POINTER(D), I32, D, D,
                                           C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), PF]
         L.ref_wf_capture_jobs.restype = I32
+        L.ref_wf_images_render.argtypes = [PF, I32, I32, U32, U32, I32, PD, I32, PF]
+        L.ref_wf_images_render.restype = I32
 
     def psf_planes(self, req, size, nz):
         out = np.zeros((nz, size, size), np.float32)
@@ -494,6 +508,18 @@ class Reference:
         if r < 0:
             raise RuntimeError(f"WidefieldScene job capture failed ({r})")
         return parse_wf_jobs(blob[:used.value].tobytes(), r), out
+
+    def wf_images_render(self, channels, has_persistent, upscale, a, cam=None):
+        """WidefieldImages::Render of channel images [nCh][ch][cw] with bleach coefficients a, added to cam."""
+        ch_ = np.ascontiguousarray(channels, np.float32)
+        nC, chh, cw = ch_.shape
+        a = np.ascontiguousarray(a, np.float64)
+        if cam is None:
+            cam = np.zeros((chh // max(1, upscale), cw // max(1, upscale)), np.float32)
+        cam = np.ascontiguousarray(cam, np.float32).copy()
+        self.L.ref_wf_images_render(_p(ch_, C.c_float), nC, int(has_persistent), cw, chh, upscale,
+                                    _p(a, C.c_double), len(a), _p(cam, C.c_float))
+        return cam
 
     def psf_library(self, reqs, out=None):
         arr = (PsfRequest * len(reqs))(*reqs)

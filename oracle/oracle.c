@@ -1523,3 +1523,43 @@ double orc_roi_backward(const orc_roi_model* m, const double* zernike, const orc
    roi_tables_free(&t);
    return nll;
 }
+
+/* Sim/WidefieldRender.cpp:547-580 WidefieldImages::Render: per camera pixel the FP64 sum over its u x u cells of
+ * max(0, v), v = persistent + sum_j (float)a_j bleach_j in FP32 (j below the last non-zero coefficient), added to
+ * cam. channels: [nCh][ch][cw], channel 0 the persistent image when hasPersistent. */
+void orc_wf_images_render(const float* channels, int32_t nCh, int32_t hasPersistent, uint32_t cw, uint32_t ch,
+                          int32_t upscale, const double* a, int32_t nA, float* cam)
+{
+   const size_t n = (size_t)cw * ch;
+   const float* P = hasPersistent ? channels : NULL;
+   const float* Bj = channels + (hasPersistent ? n : 0);
+   const int nBleach = nCh - (hasPersistent ? 1 : 0);
+   int nb = 0;
+   for (int j = 0; j < nBleach && j < nA; ++j)
+      if (a[j] != 0.0)
+         nb = j + 1;
+   if (!P && nb == 0)
+      return;
+   const unsigned u = upscale > 1 ? (unsigned)upscale : 1u, W = cw / u, H = ch / u;
+   float af[64];
+   for (int j = 0; j < nb && j < 64; ++j)
+      af[j] = (float)a[j];
+   for (unsigned Y = 0; Y < H; ++Y)
+      for (unsigned X = 0; X < W; ++X)
+      {
+         double acc = 0.0;
+         for (unsigned sy = 0; sy < u; ++sy)
+         {
+            const size_t row = (size_t)(Y * u + sy) * cw + X * u;
+            for (unsigned sx = 0; sx < u; ++sx)
+            {
+               const size_t i = row + sx;
+               float v = P ? P[i] : 0.0f;
+               for (int j = 0; j < nb; ++j)
+                  v += af[j] * Bj[(size_t)j * n + i];
+               acc += v > 0.0f ? v : 0.0f;
+            }
+         }
+         cam[(size_t)Y * W + X] += (float)acc;
+      }
+}

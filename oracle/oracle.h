@@ -120,6 +120,11 @@ void orc_render_frame_fft(float* img, uint32_t width, uint32_t height, double bg
                           const orc_emitter* ems, int32_t nEms, const float* planes, int32_t nzK, int32_t os,
                           int32_t n, const int16_t* spanLo, const int16_t* spanHi, double minPhotons);
 
+/* WidefieldImages::Render (Sim/WidefieldRender.cpp:547-580): channels [nCh][ch][cw] (channel 0 persistent when
+ * hasPersistent), bleach coefficients a[nA], added to cam ((ch/u) x (cw/u)). At most 64 bleach channels. */
+void orc_wf_images_render(const float* channels, int32_t nCh, int32_t hasPersistent, uint32_t cw, uint32_t ch,
+                          int32_t upscale, const double* a, int32_t nA, float* cam);
+
 /* ---- mean field (Sim/WidefieldRender.cpp:263-323,1460-1524; cli/widefield_check.cpp:287-343) ---- */
 /* Grid-pitch kernel plane p of an oversampled plane (KernelWidefieldPsf::Kernel): (2R+1)^2 floats. */
 int32_t orc_wf_radius(int32_t sizeOversampled, int32_t os, int32_t upscale);

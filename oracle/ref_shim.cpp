@@ -16,6 +16,7 @@
 #include "Illumination.h"
 #include "Parallel.h"
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -556,6 +557,34 @@ int32_t ref_wf_capture_jobs(const float* density, int32_t nx, int32_t ny, int32_
       return -2;
    std::memcpy(blob, host.blob.data(), host.blob.size());
    return host.jobs;
+}
+
+} // extern "C"
+
+extern "C" {
+
+// WidefieldImages::Render (Sim/WidefieldRender.cpp:547-580) of channel images (channels: [nCh][ch][cw], channel 0
+// the persistent one when hasPersistent) with bleach coefficients a (nA doubles), added to cam ((ch/u) x (cw/u)).
+int32_t ref_wf_images_render(const float* channels, int32_t nCh, int32_t hasPersistent, uint32_t cw, uint32_t ch,
+                             int32_t upscale, const double* a, int32_t nA, float* cam)
+{
+   sim::WidefieldImages im;
+   im.cw = cw;
+   im.ch = ch;
+   im.upscale = static_cast<unsigned>(upscale);
+   const size_t n = static_cast<size_t>(cw) * ch;
+   int c = 0;
+   if (hasPersistent)
+   {
+      im.persistent.assign(channels, channels + n);
+      c = 1;
+   }
+   for (; c < nCh; ++c)
+      im.bleach.emplace_back(channels + static_cast<size_t>(c) * n, channels + static_cast<size_t>(c + 1) * n);
+   std::vector<float> out(cam, cam + static_cast<size_t>(cw / std::max(1, upscale)) * (ch / std::max(1, upscale)));
+   im.Render(std::vector<double>(a, a + nA), out);
+   std::memcpy(cam, out.data(), out.size() * sizeof(float));
+   return 0;
 }
 
 } // extern "C"

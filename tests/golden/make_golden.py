@@ -188,6 +188,18 @@ def wf(R, P):
     js, img = R.wf_capture_jobs(dens.astype(np.float32), 50.0, planes, 4, 50.0, foci, dx_um=-0.047, dy_um=0.022)
     O.pack_wf_jobs(js, "b_", jobs)
     jobs["b_foci"], jobs["b_images"] = foci, img
+    # WidefieldImages::Render: a persistent and three bleach channels (signed values, so max(0, .) acts), cells
+    # binned 1x1 and 2x2, coefficients with zeros (the last non-zero one bounds the terms), onto a non-zero camera
+    rng2 = np.random.default_rng(21)
+    chans = (rng2.random((4, 40, 48)) * 40.0 - 8.0).astype(np.float32)
+    base = (rng2.random((20, 24)) * 3.0).astype(np.float32)
+    jobs["render_channels"] = chans
+    jobs["render_base"] = base
+    coeffs = np.array([[1.0, 0.5, 0.0], [0.3, 0.0, 0.0], [0.0, 0.0, 0.0], [0.25, 0.75, 1.5]])
+    jobs["render_coeffs"] = coeffs
+    jobs["render_u1"] = np.stack([R.wf_images_render(chans, True, 1, a) for a in coeffs])
+    jobs["render_u2"] = np.stack([R.wf_images_render(chans, True, 2, a, cam=base) for a in coeffs])
+    jobs["render_u2_nop"] = np.stack([R.wf_images_render(chans[1:], False, 2, a) for a in coeffs])
     np.savez_compressed(os.path.join(HERE, "wf_jobs_ref.npz"), **jobs)
 
 

@@ -152,3 +152,44 @@ def test_device_output_and_kernel_change(ctx):
     assert accel.Images(j2, got), accel.err
     for c in range(2):
         assert rel_l2(got[c], 2 * want[c]) < 1e-5
+
+
+def test_widefield_images_render_bit_exact_vs_reference(ctx, golden):
+    """WidefieldImages::Render on the GPU: bleach coefficients (zeros bound the terms), 1x1 and 2x2 binning, with and
+    without the persistent channel, onto a base image: the compiled reference's frames bit for bit."""
+    G = golden("wf_jobs_ref.npz")
+    ch, base, a = G["render_channels"], G["render_base"], G["render_coeffs"]
+    ph, _ = host.WidefieldImagesRender(ch, True, 1, a, ctx=ctx)
+    assert np.array_equal(ph, G["render_u1"])
+    ph, _ = host.WidefieldImagesRender(ch, True, 2, a, base=np.stack([base] * len(a)), ctx=ctx)
+    assert np.array_equal(ph, G["render_u2"])
+    ph, _ = host.WidefieldImagesRender(ch[1:], False, 2, a, ctx=ctx)
+    assert np.array_equal(ph, G["render_u2_nop"])
+
+
+def test_widefield_render_noise_on_accelerator_output(ctx, orc):
+    """The per-focus seam end to end on the device: job -> channel images (device) -> frames with bleach
+    coefficients and sCMOS noise; the ADU equal the noise chain of the rendered photons (oracle), bit for bit."""
+    import torch
+    rng = np.random.default_rng(31)
+    accel = host.WidefieldGpuHost(ctx)
+    fov = (0, 0, 64, 48)
+    j = random_job(rng, 128, 64, 64, 48, 3, 4, 3, 9001, fov, (0.2, -0.4))
+    imgs = torch.empty((3, 48, 64), dtype=torch.float32, device="cuda")
+    assert accel.Images(j, outArray=imgs), accel.err
+    a = np.array([[0.8, 0.1], [0.0, 0.5], [0.0, 0.0]])
+    cam = host.CameraNoiseParams(1.0, 0.0, 2.0, 1.5, False)
+    off = torch.full((48, 64), 100.0, dtype=torch.float32, device="cuda")
+    frames = np.array([7, 8, 9], np.uint32)
+    ph, _ = host.WidefieldImagesRender(imgs, True, 1, torch.from_numpy(a).cuda(), ctx=ctx)
+    _, adu = host.WidefieldImagesRender(imgs, True, 1, torch.from_numpy(a).cuda(), cam=cam, offset=off, seed=3,
+                                        frames=torch.from_numpy(frames.view(np.int32)).cuda(), ctx=ctx)
+    torch.cuda.synchronize()
+    ph = ph.cpu().numpy()
+    adu = adu.cpu().numpy().view(np.uint16)
+    want = orc.wf_images_render(imgs.cpu().numpy(), True, 1, a[0])
+    assert np.array_equal(ph[0], want)
+    oc = O.camera(quantumEfficiency=1.0, gainPhotonsPerAdu=2.0, readNoiseElectrons=1.5, emccd=0)
+    for k in range(3):
+        assert np.array_equal(adu[k], orc.noise(ph[k], oc, 3, int(frames[k]), offset=np.full((48, 64), 100.0,
+                                                                                                    np.float32)))

@@ -423,6 +423,17 @@ def test_wf_job_oracle_matches_reference_scene(golden):
         assert rel_l2(imgs[0], G[pre + "images"][0]) < 1e-6
 
 
+def test_wf_images_render_bit_exact_vs_reference(orc, golden):
+    """WidefieldImages::Render (bleach coefficients with zeros, 1x1 and 2x2 binning, with and without the persistent
+    channel, onto a camera image) restated in C equals the compiled reference bit for bit."""
+    G = golden("wf_jobs_ref.npz")
+    ch, base = G["render_channels"], G["render_base"]
+    for k, a in enumerate(G["render_coeffs"]):
+        assert np.array_equal(orc.wf_images_render(ch, True, 1, a), G["render_u1"][k])
+        assert np.array_equal(orc.wf_images_render(ch, True, 2, a, cam=base), G["render_u2"][k])
+        assert np.array_equal(orc.wf_images_render(ch[1:], False, 2, a), G["render_u2_nop"][k])
+
+
 def _job(NX, NY, nx, ny, planes, kernels_real, deps, frac=(0.0, 0.0), fov=None):
     from types import SimpleNamespace
     fov = fov or (0, 0, nx, ny)
