@@ -93,6 +93,17 @@ int32_t iscg_noise(iscg_context* h, const float* photons, uint32_t width, uint32
                    const float* offset, const float* gain, const float* readNoise, uint32_t seed,
                    const uint32_t* frames, uint32_t nFrames, uint32_t pixelBase, uint16_t* out);
 
+/* The camera's static maps and noise seeds for a random seed (host code; no context): NoiseSetup
+ * (Sim/ScopeMovie.cpp:973-996) with PixelOffsetMap / PixelGainMap / PixelReadNoiseMap::Generate and GaussianRng
+ * (Sim/SMLMNoise.cpp:17-140): one std::mt19937_64(seed), offset then gain then read-noise maps, each over the whole
+ * sensor (sensorHeight rows, or height when sensorHeight is 0), one Box-Muller normal per pixel; the gain floored
+ * at 4 % of nominal, the read noise at 0. With a window (sensorHeight != 0) the maps hold rows rowY0 .. rowY0 +
+ * height - 1 and *pixelBase = rowY0 * width. *noiseSeed = seed ^ 0x9E3779B9. Each map: width x height floats. */
+int32_t iscg_noise_setup(int64_t seed, uint32_t width, uint32_t height, uint32_t sensorHeight, uint32_t rowY0,
+                         double offsetAdu, double offsetStdAdu, double gainPhotonsPerAdu, double gainStdFraction,
+                         double readNoiseElectrons, double readStdFraction, float* offset, float* gain,
+                         float* readNoise, uint32_t* noiseSeed, uint32_t* pixelBase);
+
 /* ---- blink splat (+ noise) -------------------------------------------------------------------------------------- */
 
 /* One kernel stack the splat reads (a PsfKernelCache, Sim/PsfGeneratorBridge.h:243-293): its sum-1 planes; the
@@ -198,6 +209,12 @@ int32_t iscg_psf_geometry(const iscg_psf_request* r, int32_t* size, int32_t* nz,
  * (<= 256). out: nReq x nz x size x size floats (outCap floats available); plane k at defocus (k - (nz-1)/2) zStep. */
 int32_t iscg_psf_planes(iscg_context* h, const iscg_psf_request* reqs, int32_t nReq, int32_t normalize, float* out,
                         size_t outCap);
+
+/* The kernel half-width the CLI asks for (PsfKernelHalfWidthPx, Sim/PsfGeneratorBridge.cpp:1000): the requested
+ * half-width in pixels, raised to 3 Rayleigh radii (capped at 48 px, at least 2). */
+int32_t iscg_psf_kernel_half_width_px(double halfWidthNm, double pixelSizeNm, double wavelengthNm, double na);
+/* CappedPsfOversampling (Sim/PsfGeneratorBridge.cpp:1011): at most ceil(pixel / maxSample) samples per pixel. */
+int32_t iscg_capped_psf_oversampling(int32_t oversampling, double pixelSizeNm, double maxSampleNm);
 
 /* ---- mean field: all foci of a z stack by one 3D FFT convolution ----------------------------------------------- */
 

@@ -94,6 +94,9 @@ SIGNATURES = {
     "iscg_profile_enable": (I32, [P, I32]),
     "iscg_profile_read": (I32, [P, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), I32, I32]),
     "iscg_set_stream": (I32, [P, P]),
+    "iscg_noise_setup": (I32, [C.c_int64, U32, U32, U32, U32, D, D, D, D, D, D, P, P, P, P, P]),
+    "iscg_psf_kernel_half_width_px": (I32, [D, D, D, D]),
+    "iscg_capped_psf_oversampling": (I32, [I32, D, D]),
     "iscg_wf_has_plane": (I32, [P, C.c_uint64, C.c_uint64]),
     "iscg_wf_fits": (I32, [P, C.c_uint64, U32, U32, C.c_uint64]),
     "iscg_wf_images": (I32, [P, C.POINTER(WfJob), P]),

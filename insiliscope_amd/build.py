@@ -1,4 +1,5 @@
-"""Builds libiscg.so (every HIP source under csrc/) for gfx950, in-tree under insiliscope_amd/lib/.
+"""Builds libiscg.so (every HIP source under csrc/ for gfx950, and the host-only .cpp sources with g++), in-tree
+under insiliscope_amd/lib/.
 
 Flags: -O3, no FP contraction (the FP64 noise and splat keep the reference's operation order), no fast-math, and
 packed FP32 instructions disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
@@ -22,9 +23,18 @@ FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
 
 
+GXX = shutil.which("g++") or "g++"
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+              "-I" + os.path.join(os.path.dirname(HERE), "include")]
+
+
 def _compile(src: str) -> str:
-    obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    base, ext = os.path.splitext(os.path.basename(src))
+    obj = os.path.join(OBJ, base + (".host.o" if ext == ".cpp" else ".o"))
+    if ext == ".cpp":  # host code (no device code): the system C++ compiler and library
+        cmd = [GXX, *HOST_FLAGS, "-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}")
@@ -34,7 +44,7 @@ def _compile(src: str) -> str:
 def build(verbose: bool = False) -> str:
     """Compiles every csrc/*.hip and links lib/libiscg.so. Returns the library path."""
     os.makedirs(OBJ, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"

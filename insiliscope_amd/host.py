@@ -158,6 +158,44 @@ class CameraNoiseParams:
                            self.cicElectrons)
 
 
+@dataclass
+class NoiseSetup:
+    """The camera's maps and seeds for a random seed (NoiseSetup, Sim/ScopeMovie.cpp:973-996; the maps of
+    Pixel*Map::Generate, Sim/SMLMNoise.cpp:89-140), made on the host by iscg_noise_setup. sensorHeight > 0: a
+    window of rows rowY0 .. rowY0 + height - 1 of that sensor."""
+
+    offsetMap: np.ndarray
+    gainMap: np.ndarray
+    readNoiseMap: np.ndarray
+    noiseSeed: int
+    pixelBase: int
+
+    @classmethod
+    def Make(cls, seed: int, width: int, height: int, offsetAdu: float, offsetStdAdu: float,
+             gainPhotonsPerAdu: float, pixelGainStdFraction: float, readNoiseElectrons: float,
+             pixelReadNoiseStdFraction: float, sensorHeight: int = 0, rowY0: int = 0) -> "NoiseSetup":
+        L = _lib.lib()
+        maps = [np.zeros((height, width), np.float32) for _ in range(3)]
+        ns, pb = C.c_uint32(), C.c_uint32()
+        rc = L.iscg_noise_setup(int(seed), width, height, sensorHeight, rowY0, offsetAdu, offsetStdAdu,
+                                gainPhotonsPerAdu, pixelGainStdFraction, readNoiseElectrons, pixelReadNoiseStdFraction,
+                                maps[0].ctypes.data, maps[1].ctypes.data, maps[2].ctypes.data, C.byref(ns),
+                                C.byref(pb))
+        if rc != 0:
+            raise IscgError("iscg_noise_setup: bad geometry")
+        return cls(maps[0], maps[1], maps[2], ns.value, pb.value)
+
+
+def PsfKernelHalfWidthPx(halfWidthNm: float, pixelSizeNm: float, wavelengthNm: float, na: float) -> int:
+    """PsfKernelHalfWidthPx (Sim/PsfGeneratorBridge.cpp:1000)."""
+    return int(_lib.lib().iscg_psf_kernel_half_width_px(halfWidthNm, pixelSizeNm, wavelengthNm, na))
+
+
+def CappedPsfOversampling(oversampling: int, pixelSizeNm: float, maxSampleNm: float) -> int:
+    """CappedPsfOversampling (Sim/PsfGeneratorBridge.cpp:1011)."""
+    return int(_lib.lib().iscg_capped_psf_oversampling(oversampling, pixelSizeNm, maxSampleNm))
+
+
 def ApplyNoiseChainFrames(photons, cam: CameraNoiseParams, noiseSeed: int, frames, offsetMap=None, gainMap=None,
                           readNoiseMap=None, pixelBase: int = 0, out=None, ctx: Context | None = None):
     """ApplyNoiseChain (Sim/SMLMNoise.h:136) over a batch: photons (nFrames, height, width) float32 -> ADU uint16.

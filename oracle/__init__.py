@@ -371,6 +371,11 @@ class Reference:
         L.ref_wf_capture_jobs.argtypes = [PF, I32, I32, I32, D, PF, I32, I32, I32, D, D, C.POINTER(D), I32, D, D,
                                           C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), PF]
         L.ref_wf_capture_jobs.restype = I32
+        L.ref_noise_setup.argtypes = [C.c_int64, U32, U32, U32, U32, D, D, D, D, D, D, PF, PF, PF]
+        L.ref_psf_half_width_px.argtypes = [D, D, D, D]
+        L.ref_psf_half_width_px.restype = I32
+        L.ref_capped_psf_oversampling.argtypes = [I32, D, D]
+        L.ref_capped_psf_oversampling.restype = I32
         L.ref_wf_images_render.argtypes = [PF, I32, I32, U32, U32, I32, PD, I32, PF]
         L.ref_wf_images_render.restype = I32
 
@@ -520,6 +525,18 @@ class Reference:
         self.L.ref_wf_images_render(_p(ch_, C.c_float), nC, int(has_persistent), cw, chh, upscale,
                                     _p(a, C.c_double), len(a), _p(cam, C.c_float))
         return cam
+
+    def noise_setup(self, seed, w, h, offset, offset_std, gain, gain_std, read, read_std, sensor_h=0, row_y0=0):
+        maps = [np.zeros((h, w), np.float32) for _ in range(3)]
+        self.L.ref_noise_setup(seed, w, h, sensor_h, row_y0, offset, offset_std, gain, gain_std, read, read_std,
+                               *[_p(m, C.c_float) for m in maps])
+        return maps
+
+    def psf_half_width_px(self, half_nm, pixel_nm, wavelength_nm, na):
+        return self.L.ref_psf_half_width_px(half_nm, pixel_nm, wavelength_nm, na)
+
+    def capped_psf_oversampling(self, os_, pixel_nm, max_sample_nm):
+        return self.L.ref_capped_psf_oversampling(os_, pixel_nm, max_sample_nm)
 
     def psf_library(self, reqs, out=None):
         arr = (PsfRequest * len(reqs))(*reqs)

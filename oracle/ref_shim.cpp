@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <random>
 #include <set>
 #include <string>
 #include <vector>
@@ -585,6 +586,48 @@ int32_t ref_wf_images_render(const float* channels, int32_t nCh, int32_t hasPers
    im.Render(std::vector<double>(a, a + nA), out);
    std::memcpy(cam, out.data(), out.size() * sizeof(float));
    return 0;
+}
+
+} // extern "C"
+
+extern "C" {
+
+// NoiseSetup's maps (Sim/ScopeMovie.cpp:973-996, an anonymous struct there): the three Generate calls on one
+// std::mt19937_64(seed) in its order, whole sensor (sensorH 0) or a window.
+int32_t ref_noise_setup(int64_t seed, uint32_t W, uint32_t H, uint32_t sensorH, uint32_t rowY0, double offsetAdu,
+                        double offsetStd, double gain, double gainStd, double read, double readStd, float* off,
+                        float* gn, float* rn)
+{
+   std::mt19937_64 rng(static_cast<uint64_t>(seed));
+   sim::PixelOffsetMap o;
+   sim::PixelGainMap g;
+   sim::PixelReadNoiseMap r;
+   if (sensorH)
+   {
+      o.Generate(W, sensorH, offsetAdu, offsetStd, rng, rowY0, H);
+      g.Generate(W, sensorH, gain, gainStd, rng, rowY0, H);
+      r.Generate(W, sensorH, read, readStd, rng, rowY0, H);
+   }
+   else
+   {
+      o.Generate(W, H, offsetAdu, offsetStd, rng);
+      g.Generate(W, H, gain, gainStd, rng);
+      r.Generate(W, H, read, readStd, rng);
+   }
+   std::memcpy(off, o.offset.data(), o.offset.size() * sizeof(float));
+   std::memcpy(gn, g.gainPhotonsPerAdu.data(), g.gainPhotonsPerAdu.size() * sizeof(float));
+   std::memcpy(rn, r.readNoiseElectrons.data(), r.readNoiseElectrons.size() * sizeof(float));
+   return 0;
+}
+
+int32_t ref_psf_half_width_px(double halfWidthNm, double pixelSizeNm, double wavelengthNm, double na)
+{
+   return sim::PsfKernelHalfWidthPx(halfWidthNm, pixelSizeNm, wavelengthNm, na);
+}
+
+int32_t ref_capped_psf_oversampling(int32_t os, double pixelSizeNm, double maxSampleNm)
+{
+   return sim::CappedPsfOversampling(os, pixelSizeNm, maxSampleNm);
 }
 
 } // extern "C"

@@ -14,6 +14,7 @@ Outputs (inputs and expected outputs only):
   splat_ref.npz      CollectFrameEmitters, CollectGpuEmitters and RenderPhotonImage frames (Nearest, Linear, Cubic,
                      Fft, halo cut, edge emitters) on seeded events
   wf_ref.npz         KernelWidefieldPsf radius and grid-pitch planes; WidefieldScene::FocusSeries images
+  maps_ref.npz       NoiseSetup's offset / gain / read-noise maps (sensor and window) and the kernel-geometry helpers
   wf_jobs_ref.npz    the WidefieldGpuJobs of GPU-mode scenes (as sent to a WidefieldAccelerator) and the scenes' own
                      CPU images
 """
@@ -203,8 +204,26 @@ def wf(R, P):
     np.savez_compressed(os.path.join(HERE, "wf_jobs_ref.npz"), **jobs)
 
 
+def maps_and_geometry(R):
+    """NoiseSetup's camera maps (whole sensor, and a window of rows) and the kernel-geometry helpers."""
+    out = {}
+    out["maps_full"] = np.stack(R.noise_setup(1234567, 48, 40, 100.0, 2.5, 2.0, 0.05, 1.6, 0.2))
+    out["maps_window"] = np.stack(R.noise_setup(-77, 48, 12, 100.0, 2.5, 1.0 / 300.0, 3.0, 50.0, 1.5,
+                                                sensor_h=40, row_y0=17))
+    hw = [(800.0, 100.0, 670.0, 1.45), (100.0, 100.0, 670.0, 1.45), (5000.0, 65.0, 520.0, 1.4),
+          (300.0, 160.0, 700.0, 0.0), (49.0, 100.0, 600.0, 1.2), (20000.0, 100.0, 670.0, 1.45)]
+    out["half_width_in"] = np.array(hw)
+    out["half_width_out"] = np.array([R.psf_half_width_px(*a) for a in hw], np.int32)
+    cap = [(4, 100.0, 25.0), (4, 100.0, 30.0), (5, 160.0, 40.0), (3, 100.0, 0.0), (8, 100.0, 33.3333333333),
+           (1, 100.0, 10.0), (6, 0.0, 20.0)]
+    out["capped_in"] = np.array(cap)
+    out["capped_out"] = np.array([R.capped_psf_oversampling(int(a), b, c) for a, b, c in cap], np.int32)
+    np.savez_compressed(os.path.join(HERE, "maps_ref.npz"), **out)
+
+
 def main():
     R = O.Reference()
+    maps_and_geometry(R)
     rng_vectors()
     shutil.copyfile(os.path.join(REF, "tests/psf/zernike_ref.bin"), os.path.join(HERE, "zernike_ref.bin"))
     psf(R)

@@ -154,3 +154,26 @@ def test_host_calls_fail_loudly_without_gpu():
         pass
     with pytest.raises(host.IscgError):
         host.Context(0)
+
+
+def test_noise_setup_maps_equal_reference(golden):
+    """NoiseSetup's camera maps (host code in the library: std::mt19937_64, uniform_real_distribution, Box-Muller)
+    equal the compiled reference's bit for bit, for the whole sensor and a window of its rows."""
+    G = golden("maps_ref.npz")
+    s = host.NoiseSetup.Make(1234567, 48, 40, 100.0, 2.5, 2.0, 0.05, 1.6, 0.2)
+    for got, want in zip((s.offsetMap, s.gainMap, s.readNoiseMap), G["maps_full"]):
+        assert np.array_equal(got, want)
+    assert s.noiseSeed == (1234567 ^ 0x9E3779B9) & 0xFFFFFFFF and s.pixelBase == 0
+    w = host.NoiseSetup.Make(-77, 48, 12, 100.0, 2.5, 1.0 / 300.0, 3.0, 50.0, 1.5, sensorHeight=40, rowY0=17)
+    for got, want in zip((w.offsetMap, w.gainMap, w.readNoiseMap), G["maps_window"]):
+        assert np.array_equal(got, want)
+    assert w.pixelBase == 17 * 48
+    assert (w.gainMap >= np.float32(0.04 / 300.0) * 0.999).all() and (w.readNoiseMap >= 0).all()
+
+
+def test_psf_geometry_helpers_equal_reference(golden):
+    G = golden("maps_ref.npz")
+    for args, want in zip(G["half_width_in"], G["half_width_out"]):
+        assert host.PsfKernelHalfWidthPx(*args) == want
+    for (o, p, m), want in zip(G["capped_in"], G["capped_out"]):
+        assert host.CappedPsfOversampling(int(o), p, m) == want
