@@ -112,7 +112,9 @@ def wf_psf_request():
 
 def wf_bytes(nx, ny, nzD, nF, NX, NY, NZ):
     """Algorithmic HBM bytes per launch of each pass: each pass reads its input and writes its output once (half
-    spectra of NX/2 + 1 complex64 columns; zero padding never read)."""
+    spectra of NX/2 + 1 complex64 columns; zero padding never read). wf_slab (the fused y / z / inverse-y pass of
+    the 512 x 512 x 128 grid) reads the x-transformed dye rows and the kernel spectrum and writes the FOV rows; the
+    five-pass names are those of the other grids."""
     h = NX // 2 + 1
     c = 8
     return {
@@ -120,6 +122,7 @@ def wf_bytes(nx, ny, nzD, nF, NX, NY, NZ):
         "wf_cols_fwd": nzD * ny * h * c + nzD * NY * h * c,
         "wf_zconv": nzD * NY * h * c + NZ * NY * h * c + nF * NY * h * c,
         "wf_cols_inv": nF * NY * h * c + nF * ny * h * c,
+        "wf_slab": nzD * ny * h * c + NZ * NY * h * c + nF * ny * h * c,
         "wf_rows_c2r_noise": nF * ny * h * c + nF * ny * nx * 2,
     }
 
@@ -161,7 +164,7 @@ def run_wf(ctx, steps, warmup):
     dom = max(prof, key=lambda k: prof[k][1])
     avg_ms = prof[dom][1] / prof[dom][0]
     achieved = nbytes[dom] / (avg_ms * 1e-3) / 1e9
-    total_bytes = sum(nbytes.values())
+    total_bytes = sum(nbytes[k] for k in prof if k in nbytes)
     kernels = {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 5),
                    "GB_s": round(nbytes.get(k, 0) / (v[1] / v[0] * 1e-3) / 1e9, 1)} for k, v in prof.items()}
     traffic = traffic_for(dom)

@@ -58,6 +58,19 @@ int32_t iscg_profile_enable(iscg_context* h, int32_t on);
 int32_t iscg_profile_read(iscg_context* h, char* names, int64_t* launches, double* totalMs, int32_t cap,
                           int32_t reset);
 
+/* ---- the core's counter hash (core/src/rng.h:18-52) --------------------------------------------------------------- */
+
+/* Pcg4d of n addresses (a[i], b[i], c[i], d[i]): words[4 i .. 4 i + 3] = the four output words. */
+int32_t iscg_pcg4d(iscg_context* h, const uint32_t* a, const uint32_t* b, const uint32_t* c, const uint32_t* d,
+                   uint32_t n, uint32_t* words);
+/* HashUnit (core/src/rng.h:40-44) of n addresses: out[i] = ((Pcg4d(seed, cx[i], cy[i], k[i]).a >> 9) + 0.5) 2^-23. */
+int32_t iscg_hash_units(iscg_context* h, uint32_t seed, const int32_t* cx, const int32_t* cy, const uint32_t* k,
+                        uint32_t n, double* out);
+/* The first n draws of HashStream(seed, cx, cy, base) (core/src/rng.h:46-52): HashUnit(seed, cx, cy, base * 4096 + i),
+ * uint32 wrap. */
+int32_t iscg_hash_stream(iscg_context* h, uint32_t seed, int32_t cx, int32_t cy, uint32_t base, uint32_t n,
+                         double* out);
+
 /* ---- camera noise ---------------------------------------------------------------------------------------------- */
 
 /* sim::CameraNoiseParams (Sim/SMLMNoise.h:35-50). */

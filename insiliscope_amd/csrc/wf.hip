@@ -32,7 +32,15 @@ struct WfState
    int R = 0, nzP = 0;
    bool planned = false;
    DevBuf ghat, x1, x2, twX, twY, twZ, img;
+   // the fused slab pass (NY = 512, NZ = 128, nzD and nF <= 64, ny <= 256): ghat column-major, column strides of x1 and x2
+   bool slab = false;
+   size_t col1 = 0, col2 = 0;
 };
+
+static bool slab_fits(const iscg_wf_desc& d)
+{
+   return d.NY == 512 && d.NZ == 128 && d.nzD <= 64 && d.nF <= 64 && d.ny <= 256 && d.fovY0 + d.ch <= d.NY;
+}
 
 void destroy_wf(iscg_context* h)
 {
@@ -161,6 +169,459 @@ __global__ __launch_bounds__(zconv_cols<N>() * N / 8) void zconv_kernel(const fl
             out[(size_t)z * plane + off + c] = buf[z * CS + c];
       }
       __syncthreads();
+   }
+}
+
+// ---- fused passes B + C + D (NY = 512, NZ = 128) -------------------------------------------------------------
+//
+// One workgroup per column kx of the half spectrum does the whole (y, z) work of that column: its slab never
+// leaves the chip. The column-major passes A and E around it make every global access a contiguous segment: in
+// [kx][z][y < ny], Ghat [kx][ky][kz], out [kx][f][rows]. 1024 lanes; w = tid >> 6 (wave), lane = tid & 63.
+// The y transform is split by the parity h of ky (decimation in frequency): with x' = (x[y] + (-1)^h x[y + 256])
+// e^{-2 pi i h y / 512}, Y[2m + h] = FFT256(x')[m], and the inverse is out[y] = e[y mod 256] + e^{2 pi i y / 512}
+// o[y mod 256] with e, o the 256-point inverses of the even and odd halves. Each half is a 64 x 256 slab held in
+// registers, R[j][i] = S_h[z = (lane & 15) + 16 i][m = m(j, 4 w + (lane >> 4))], 16 values per lane:
+//   1. 2 batches of 32 z lines: two lines per wave (32 lanes each) transformed along y on their own (loads straight
+//      into the first radix-8 stage), then one workgroup-wide transpose through LDS fills R[.][i];
+//   2. 4 chunks of 64 m lines, 4 lines per wave, 16 lanes per line: FFT along z (first stage from registers, the
+//      upper 64 planes zero), times Ghat (loaded before the transform), inverse, the first 64 planes back to R - all
+//      inside the wave, no workgroup barrier;
+//   3. 2 batches of 32 foci: transpose R into LDS, two foci per wave transformed back along y, rows
+//      [row0, row0 + rows) stored from registers (h = 0) or added to (h = 1, the same lane wrote them).
+// Within a wave the lanes run in lockstep and its LDS operations complete in order, so a line that one wave owns
+// needs only a compiler fence (wave_sync) between the read and the write phases of a stage.
+// Columns 0 and NX/2 are real in (y, z) (Hermitian-symmetric rows of the x transform), so one workgroup takes both
+// as u = a + ib: after the (y, z) transform U(k) = A(k) + iB(k) with A(-k) = conj A(k), B likewise, so
+//   A Ghat0 + i B GhatN = U(k) P(k) + conj(U(-k)) Q(k),  P = (Ghat0 + GhatN) / 2,  Q = (Ghat0 - GhatN) / 2,
+// and the inverse gives a' + ib'. ky and -ky have the same parity; the packed workgroup's chunk j holds
+// m = 32 j + p at even positions 2p and the m of -ky at odd ones (128, i.e. ky = 256, in place of the duplicate 0),
+// so k and -k sit on neighbouring lines of one wave. P and Q are Hermitian too, so column 0 of the spectrum holds
+// one value per k, as the others do: P(kz, ky) on the lines 0 < ky < 256, Q(kz, ky) on the line -ky, and
+// the self-paired lines ky = 0 and 256 hold P(kz) for 0 < kz < 64 and Q(kz) at 128 - kz, with the real values at
+// kz = 0 and 64 paired in slots 0 (P) and 64 (Q). This leaves NX/2 workgroups: one per CU at NX = 512.
+
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
+// Butterfly j of a radix-R Stockham stage of length N (NS = product of the earlier radices): inputs at j + r N / R,
+// twiddled by tw[TWS r k N / (NS R)] (k = j mod NS; tw a table of length TWS N), outputs to
+// (j / NS) NS R + k + r NS.
+template <int N, int R, int NS, bool INV, int TWS = 1>
+__device__ __forceinline__ void bfly(float2 (&v)[R], int j, const float2* tw)
+{
+   if (NS > 1)
+   {
+      const int k = j % NS;
+#pragma unroll
+      for (int r = 1; r < R; ++r)
+      {
+         float2 t = tw[TWS * r * k * (N / (NS * R))];
+         if (INV)
+            t.y = -t.y;
+         v[r] = cmul(v[r], t);
+      }
+   }
+   dft<R, INV>(v);
+}
+// Element i of a line sits at slot sp(i) = i + i / 16: one pad slot per 16 elements spreads the strided stores of the
+// first stages (lane stride 8) and the column accesses of the transposes over the LDS banks (at most 2-way).
+__device__ __forceinline__ int sp(int i) { return i + (i >> 4); }
+template <int N, int R>
+__device__ __forceinline__ void bfly_load(const float2* L, int j, float2 (&v)[R])
+{
+#pragma unroll
+   for (int r = 0; r < R; ++r)
+      v[r] = L[sp(j + r * (N / R))];
+}
+template <int N, int R, int NS>
+__device__ __forceinline__ void bfly_store(float2* L, int j, const float2 (&v)[R])
+{
+   const int base = (j / NS) * NS * R + j % NS;
+#pragma unroll
+   for (int r = 0; r < R; ++r)
+      L[sp(base + r * NS)] = v[r];
+}
+
+// 256-point line owned by 32 lanes (q = 0..31; radix 8, 8, 4), twiddles from the 512-point table. In and out,
+// v[r] = element q + 32 r; FIRST_IN_REGS: the first stage takes v as its inputs (else reads L), LAST_IN_REGS: the
+// result stays in v (else in natural order in L).
+template <bool INV, bool FIRST_IN_REGS, bool LAST_IN_REGS>
+__device__ __forceinline__ void line_fft256(float2* L, int q, float2 (&v)[8], const float2* tw512)
+{
+   if (!FIRST_IN_REGS)
+      bfly_load<256, 8>(L, q, v);
+   bfly<256, 8, 1, INV, 2>(v, q, tw512);
+   wave_sync();
+   bfly_store<256, 8, 1>(L, q, v);
+   wave_sync();
+   bfly_load<256, 8>(L, q, v);
+   wave_sync();
+   bfly<256, 8, 8, INV, 2>(v, q, tw512);
+   bfly_store<256, 8, 8>(L, q, v);
+   wave_sync();
+   float2 u[2][4];
+#pragma unroll
+   for (int s = 0; s < 2; ++s)
+   {
+      bfly_load<256, 4>(L, q + 32 * s, u[s]);
+      bfly<256, 4, 64, INV, 2>(u[s], q + 32 * s, tw512);
+   }
+   if (LAST_IN_REGS)
+   {
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+         for (int r = 0; r < 4; ++r)
+            v[s + 2 * r] = u[s][r];
+   }
+   else
+   {
+      wave_sync();
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+         for (int r = 0; r < 4; ++r)
+            L[sp(q + 32 * s + 64 * r)] = u[s][r];
+   }
+}
+
+// 128-point line owned by 16 lanes (q = 0..15; radix 8, 8, 2): v[r] = element q + 16 r in and out (the first stage
+// from v; HALF_OUT: only the first 64 outputs, v[0..3]). L: the line's LDS scratch.
+template <bool INV, bool HALF_OUT>
+__device__ __forceinline__ void line_fft128(float2* L, int q, float2 (&v)[8], const float2* tw)
+{
+   bfly<128, 8, 1, INV>(v, q, tw);
+   wave_sync();
+   bfly_store<128, 8, 1>(L, q, v);
+   wave_sync();
+   bfly_load<128, 8>(L, q, v);
+   wave_sync();
+   bfly<128, 8, 8, INV>(v, q, tw);
+   bfly_store<128, 8, 8>(L, q, v);
+   wave_sync();
+#pragma unroll
+   for (int s = 0; s < 4; ++s)
+   {
+      const int j = q + 16 * s;
+      float2 u[2];
+      bfly_load<128, 2>(L, j, u);
+      if (HALF_OUT)
+      {
+         // element j (< 64) of the radix-2 output only: u0 + tw u1
+         float2 t = tw[j];
+         if (INV)
+            t.y = -t.y;
+         v[s] = cadd(u[0], cmul(u[1], t));
+      }
+      else
+      {
+         bfly<128, 2, 64, INV>(u, j, tw);
+         v[s] = u[0];
+         v[s + 4] = u[1];
+      }
+   }
+}
+
+// The half-slab line m of chunk j at position s (0..63); the packed workgroup pairs ky with -ky (same parity h).
+__device__ __forceinline__ int slab_m(bool packed, int h, int j, int s)
+{
+   if (!packed)
+      return 64 * j + s;
+   const int mp = 32 * j + (s >> 1);
+   if (!(s & 1))
+      return mp;
+   if (h)
+      return 255 - mp;
+   return mp == 0 ? 128 : 256 - mp;
+}
+
+// The work of one workgroup (PACKED: columns 0 and NH together). Two bodies, so that the packed one's extra
+// exchanges do not raise the register allocation of the others.
+template <bool PACKED>
+__device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny, int nzIn, size_t inCol,
+                                          float2* __restrict__ out, int row0, int rows, int nOut, size_t outCol,
+                                          const float2* __restrict__ ghat, int NH, float2* buf, float2* gx,
+                                          const float2* twy, const float2* twz)
+{
+   constexpr int NY = 512, NZ = 128;
+   constexpr int NJ = 4, NI = 4;  // chunks of 64 lines, batches of 16 planes (64 held)
+   constexpr int LSY = 273, LSZ = 137;  // odd strides >= sp(256), sp(128)
+   constexpr bool packed = PACKED;
+   const int tid0 = threadIdx.x;
+   const int c = packed ? 0 : (int)blockIdx.x;
+   const float2* src = in + (size_t)c * inCol;
+   const float2* srcN = in + (size_t)NH * inCol;
+   const float2* G = ghat + (size_t)c * NY * NZ;
+   float2* dst = out + (size_t)c * outCol;
+   float2* dstN = out + (size_t)NH * outCol;
+// the lane ids, opaque to the compiler at each batch and chunk so that it recomputes addresses there instead of
+// keeping every unrolled iteration's addresses live (which would spill the register slab)
+#define SLAB_IDS                   \
+   int tid = tid0;                 \
+   asm volatile("" : "+v"(tid));   \
+   const int lane = tid & 63, w = tid >> 6
+
+#pragma unroll 1
+   for (int h = 0; h < 2; ++h)
+   {
+      float2 R[NJ][NI];
+      // 1. y forward, z lines 32 b + 2 w + (lane >> 5)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+      {
+         SLAB_IDS;
+         if (32 * b >= nzIn)
+         {
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+               R[j][2 * b] = R[j][2 * b + 1] = make_float2(0.0f, 0.0f);
+            continue;
+         }
+         const int zl = 2 * w + (lane >> 5), q = lane & 31, z = 32 * b + zl;
+         float2 v[8];
+#pragma unroll
+         for (int r = 0; r < 8; ++r)
+         {
+            const int y = q + 32 * r;  // < 256: ny <= 256 on this path
+            float2 x = make_float2(0.0f, 0.0f);
+            if (z < nzIn && y < ny)
+            {
+               const size_t o = (size_t)z * ny + y;
+               x = packed ? make_float2(src[o].x, srcN[o].x) : src[o];
+            }
+            v[r] = h ? cmul(x, twy[y]) : x;
+         }
+         line_fft256<false, true, false>(buf + zl * LSY, q, v, twy);
+         __syncthreads();
+#pragma unroll
+         for (int j = 0; j < NJ; ++j)
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+               R[j][2 * b + t] = buf[((lane & 15) + 16 * t) * LSY + sp(slab_m(packed, h, j, 4 * w + (lane >> 4)))];
+         __syncthreads();
+      }
+
+      // 2. z convolution, lines 4 w .. 4 w + 3 of each chunk per wave
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+      {
+         SLAB_IDS;
+         const int pos = 4 * w + (lane >> 4), q = lane & 15;
+         const int ky = 2 * slab_m(packed, h, j, pos) + h;
+         float2* Z = buf + pos * LSZ;
+         // the spectrum, in flight during the forward transform
+         float2 g[8];
+#pragma unroll
+         for (int m = 0; m < 8; ++m)
+            g[m] = G[(size_t)ky * NZ + q + 16 * m];
+         float2 v[8];
+#pragma unroll
+         for (int r = 0; r < 8; ++r)
+            v[r] = r < NI ? R[j][r < NI ? r : 0] : make_float2(0.0f, 0.0f);
+         line_fft128<false, false>(Z, q, v, twz);
+         // v[m] = U at kz = q + 16 m
+         if (packed)
+         {
+            // the partner line (-ky) is the neighbouring position (itself for ky = 0 and 256)
+            const bool self = ky == 0 || ky == NY / 2, plus = ky < NY / 2;
+            const int pp = self ? pos : (pos ^ 1);
+            const float2* Zp = buf + pp * LSZ;
+            float2* Gown = gx + pos * LSZ;
+            const float2* Gp = gx + pp * LSZ;
+            wave_sync();
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+            {
+               Z[sp(q + 16 * m)] = v[m];
+               Gown[sp(q + 16 * m)] = g[m];
+            }
+            wave_sync();
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+            {
+               const int kz = q + 16 * m, nkz = (NZ - kz) & (NZ - 1);
+               const float2 up = Zp[sp(nkz)];
+               float2 P, Q;
+               if (self)
+               {
+                  if (q == 0 && (m == 0 || m == 4))
+                  {
+                     const float2 p = Gown[0], qq = Gown[sp(NZ / 2)];
+                     P = make_float2(m == 0 ? p.x : p.y, 0.0f);
+                     Q = make_float2(m == 0 ? qq.x : qq.y, 0.0f);
+                  }
+                  else if (kz < NZ / 2)
+                  {
+                     P = g[m];
+                     Q = Gown[sp(nkz)];
+                  }
+                  else
+                  {
+                     const float2 p = Gown[sp(nkz)];
+                     P = make_float2(p.x, -p.y);
+                     Q = make_float2(g[m].x, -g[m].y);
+                  }
+               }
+               else if (plus)
+               {
+                  P = g[m];
+                  Q = Gp[sp(kz)];
+               }
+               else
+               {
+                  const float2 p = Gp[sp(nkz)], qq = Gown[sp(nkz)];
+                  P = make_float2(p.x, -p.y);
+                  Q = make_float2(qq.x, -qq.y);
+               }
+               v[m] = cadd(cmul(v[m], P), cmul(make_float2(up.x, -up.y), Q));
+               // one k at a time: the scheduler would otherwise issue every iteration's LDS reads at once
+               __builtin_amdgcn_sched_barrier(0);
+            }
+         }
+         else
+         {
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+               v[m] = cmul(v[m], g[m]);
+         }
+         wave_sync();
+         line_fft128<true, true>(Z, q, v, twz);
+#pragma unroll
+         for (int i = 0; i < NI; ++i)
+            R[j][i] = v[i];
+         wave_sync();
+      }
+      __syncthreads();
+
+      // 3. y inverse, foci 32 b + 2 w + (lane >> 5); rows [row0, row0 + rows)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+      {
+         SLAB_IDS;
+         if (32 * b >= nOut)
+            break;
+#pragma unroll
+         for (int j = 0; j < NJ; ++j)
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+               buf[((lane & 15) + 16 * t) * LSY + sp(slab_m(packed, h, j, 4 * w + (lane >> 4)))] = R[j][2 * b + t];
+         __syncthreads();
+         const int fl = 2 * w + (lane >> 5), q = lane & 31, f = 32 * b + fl;
+         float2* d = dst + (size_t)f * rows;
+         float2* dN = dstN + (size_t)f * rows;
+         // h = 1 adds to the rows this lane stored for h = 0: their loads overlap the transform (the first 256)
+         constexpr int KP = 8;
+         float2 prev[KP];
+#pragma unroll
+         for (int k = 0; k < KP; ++k)
+         {
+            const int yo = q + 32 * k;
+            prev[k] = make_float2(0.0f, 0.0f);
+            if (h && f < nOut && yo < rows)
+               prev[k] = packed ? make_float2(d[yo].x, dN[yo].x) : d[yo];
+         }
+         float2 v[8];
+         float2* Lf = buf + fl * LSY;
+         line_fft256<true, false, false>(Lf, q, v, twy);
+         wave_sync();
+         if (f < nOut)
+         {
+            // output row y = row0 + yo takes element y mod 256 of this half (twisted by e^{2 pi i y / 512} for h = 1)
+            auto put = [&](int yo, float2 p) {
+               const int y = row0 + yo;
+               float2 val = Lf[sp(y & 255)];
+               if (h)
+               {
+                  const float2 tw = twy[y];
+                  val = cadd(cmul(val, make_float2(tw.x, -tw.y)), p);
+               }
+               if (packed)
+               {
+                  d[yo] = make_float2(val.x, 0.0f);
+                  dN[yo] = make_float2(val.y, 0.0f);
+               }
+               else
+                  d[yo] = val;
+            };
+#pragma unroll
+            for (int k = 0; k < KP; ++k)
+               if (q + 32 * k < rows)
+                  put(q + 32 * k, prev[k]);
+#pragma unroll 1
+            for (int yo = q + 32 * KP; yo < rows; yo += 32)
+               put(yo, h ? (packed ? make_float2(d[yo].x, dN[yo].x) : d[yo]) : make_float2(0.0f, 0.0f));
+         }
+         __syncthreads();
+      }
+   }
+#undef SLAB_IDS
+}
+
+template <int NY, int NZ>
+__global__ __launch_bounds__(1024) void slab_kernel(const float2* __restrict__ in, int ny, int nzIn, size_t inCol,
+                                                    float2* __restrict__ out, int row0, int rows, int nOut,
+                                                    size_t outCol, const float2* __restrict__ ghat, int NH,
+                                                    const float2* __restrict__ twY, const float2* __restrict__ twZ)
+{
+   static_assert(NY == 512 && NZ == 128, "the slab pass is written for a 512 x 128 (y, z) slab");
+   constexpr int LSY = 273, LSZ = 137;
+   constexpr int BUF = 32 * LSY > 64 * LSZ ? 32 * LSY : 64 * LSZ;
+   __shared__ float2 buf[BUF];
+   __shared__ float2 gx[64 * LSZ];  // the packed workgroup's spectrum lines, exchanged between partners
+   __shared__ float2 twy[NY];
+   __shared__ float2 twz[NZ];
+   for (int i = threadIdx.x; i < NY; i += 1024)
+      twy[i] = twY[i];
+   for (int i = threadIdx.x; i < NZ; i += 1024)
+      twz[i] = twZ[i];
+   __syncthreads();
+   if (blockIdx.x == 0)
+      slab_body<true>(in, ny, nzIn, inCol, out, row0, rows, nOut, outCol, ghat, NH, buf, gx, twy, twz);
+   else
+      slab_body<false>(in, ny, nzIn, inCol, out, row0, rows, nOut, outCol, ghat, NH, buf, gx, twy, twz);
+}
+
+// The kernel spectrum for the slab pass: out[c][ky][kz] = ghat[kz][ky][c] for columns 1 .. NH - 1; column 0 holds
+// the packed workgroup's P = (Ghat0 + GhatNH) / 2 and Q = (Ghat0 - GhatNH) / 2, one value per k (P(kz, ky) on the
+// lines 0 < ky < NY/2, Q(kz, ky) on the line NY - ky, both folded into the self-paired lines ky = 0 and NY/2).
+__global__ void ghat_columns_kernel(const float2* __restrict__ ghat, int NZ, int NY, int S, int NH,
+                                    float2* __restrict__ out)
+{
+   const size_t plane = (size_t)NZ * NY, total = (size_t)NH * plane;
+   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x)
+   {
+      const int c = (int)(idx / plane);
+      const int rem = (int)(idx - (size_t)c * plane), ky = rem / NZ, kz = rem % NZ;
+      if (c != 0)
+      {
+         out[idx] = ghat[((size_t)kz * NY + ky) * S + c];
+         continue;
+      }
+      // P(kz, ky) and Q(kz, ky) of column 0
+      auto P = [&](int z, int y) {
+         const float2* s = ghat + ((size_t)z * NY + y) * S;
+         return make_float2(0.5f * (s[0].x + s[NH].x), 0.5f * (s[0].y + s[NH].y));
+      };
+      auto Q = [&](int z, int y) {
+         const float2* s = ghat + ((size_t)z * NY + y) * S;
+         return make_float2(0.5f * (s[0].x - s[NH].x), 0.5f * (s[0].y - s[NH].y));
+      };
+      float2 v;
+      if (ky == 0 || ky == NY / 2)
+      {
+         if (kz == 0)
+            v = make_float2(P(0, ky).x, P(NZ / 2, ky).x);
+         else if (kz == NZ / 2)
+            v = make_float2(Q(0, ky).x, Q(NZ / 2, ky).x);
+         else if (kz < NZ / 2)
+            v = P(kz, ky);
+         else
+            v = Q(NZ - kz, ky);
+      }
+      else if (ky < NY / 2)
+         v = P(kz, ky);
+      else
+         v = Q(kz, NY - ky);
+      out[idx] = v;
    }
 }
 
@@ -313,6 +774,24 @@ int32_t iscg_wf_conv3d_plan(iscg_context* h, const iscg_wf_desc* desc, const flo
                           s.ghat.as<float2>(), (int)d.NZ, (const float2*)nullptr, (int)d.NY, (int)S,
                           (int)(d.NX / 2 + 1), scale, s.twZ.as<const float2>()))
       return fail(h, "iscg_wf_conv3d_plan: unsupported FFT size");
+   s.slab = slab_fits(d);
+   if (s.slab)
+   {
+      // column-major copy for the slab pass (then the row-major one is dropped)
+      const uint32_t NH = d.NX / 2;
+      DevBuf gc;
+      const size_t n = (size_t)NH * d.NZ * d.NY;
+      if (!ok(h, gc.reserve(n * sizeof(float2)), "spectrum allocation"))
+         return -1;
+      hipLaunchKernelGGL(ghat_columns_kernel, dim3(grid_for(n, 256, 8)), dim3(256), 0, h->stream,
+                         s.ghat.as<const float2>(), (int)d.NZ, (int)d.NY, (int)S, (int)NH, gc.as<float2>());
+      if (!ok(h, hipGetLastError(), "plan transpose") || !ok(h, hipStreamSynchronize(h->stream), "plan synchronize"))
+         return -1;
+      std::swap(s.ghat.p, gc.p);
+      std::swap(s.ghat.bytes, gc.bytes);
+      s.col1 = ((size_t)d.nzD * d.ny + 31) / 32 * 32;
+      s.col2 = ((size_t)d.nF * d.ch + 31) / 32 * 32;
+   }
    if (!ok(h, hipGetLastError(), "plan transform") || !ok(h, hipStreamSynchronize(h->stream), "plan synchronize"))
       return -1;
    s.planned = true;
@@ -341,16 +820,6 @@ int32_t iscg_wf_conv3d(iscg_context* h, const float* density, const iscg_camera*
       return -1;
    float2* x1 = s.x1.as<float2>();
    float2* x2 = s.x2.as<float2>();
-   // A + B: [nzD][ny][nx] -> x2 [nzD][NY][S]
-   if (!forward_xy(h, grid_of(s), dIn, (int)d.nzD, (int)d.ny, (int)d.nx, x1, x2, (size_t)d.NY * S))
-      return fail(h, "iscg_wf_conv3d: unsupported FFT size");
-   // C: x2 -> x1 [nF][NY][S]
-   {
-   ProfScope ps(h, "wf_zconv");
-   dispatch_n<ZConv>((int)d.NZ, h->stream, false, (const float2*)x2, (int)d.nzD, x1, (int)d.nF,
-                     s.ghat.as<const float2>(), (int)d.NY, S, (int)(d.NX / 2 + 1), 1.0f, s.twZ.as<const float2>());
-   }
-   // D + E: x1 [nF][NY][S] -> x2 [nF][ch][S] (rows fovY0 ..) -> images / ADU
    C2rEpilogue e;
    e.raw = 0;
    e.img = dImg;
@@ -371,7 +840,42 @@ int32_t iscg_wf_conv3d(iscg_context* h, const float* density, const iscg_camera*
       return fail(h, "iscg_wf_conv3d: ADU output holds at most 2^32 - 1 pixels");
    if (dAdu && !noise_queue(h, nOut, e.q))
       return -1;
+   if (s.slab)
+   {
+      // A: [nzD][ny][nx] -> x1 [kx][nzD * ny]; B + C + D: -> x2 [kx][nF * ch]; E -> images / ADU
+      {
+         ProfScope ps(h, "wf_rows_r2c");
+         if (!dispatch_n<RowsR2Ccm, 1024>((int)d.NX, h->stream, dIn, (size_t)d.nx, (int)(d.nzD * d.ny), (int)d.nx,
+                                          x1, s.col1, s.twX.as<const float2>()))
+            return fail(h, "iscg_wf_conv3d: unsupported FFT size");
+      }
+      {
+         ProfScope ps(h, "wf_slab");
+         hipLaunchKernelGGL((slab_kernel<512, 128>), dim3(d.NX / 2), dim3(1024), 0, h->stream,
+                            (const float2*)x1, (int)d.ny, (int)d.nzD, s.col1, x2, (int)d.fovY0, (int)d.ch, (int)d.nF,
+                            s.col2, s.ghat.as<const float2>(), (int)(d.NX / 2), s.twY.as<const float2>(),
+                            s.twZ.as<const float2>());
+      }
+      {
+         ProfScope ps(h, "wf_rows_c2r_noise");
+         dispatch_n<RowsC2Rcm, 1024>((int)d.NX, h->stream, (const float2*)x2, s.col2, (int)(d.nF * d.ch), e,
+                                     s.twX.as<const float2>());
+      }
+   }
+   else
+   {
+   // A + B: [nzD][ny][nx] -> x2 [nzD][NY][S]
+   if (!forward_xy(h, grid_of(s), dIn, (int)d.nzD, (int)d.ny, (int)d.nx, x1, x2, (size_t)d.NY * S))
+      return fail(h, "iscg_wf_conv3d: unsupported FFT size");
+   // C: x2 -> x1 [nF][NY][S]
+   {
+   ProfScope ps(h, "wf_zconv");
+   dispatch_n<ZConv>((int)d.NZ, h->stream, false, (const float2*)x2, (int)d.nzD, x1, (int)d.nF,
+                     s.ghat.as<const float2>(), (int)d.NY, S, (int)(d.NX / 2 + 1), 1.0f, s.twZ.as<const float2>());
+   }
+   // D + E: x1 [nF][NY][S] -> x2 [nF][ch][S] (rows fovY0 ..) -> images / ADU
    inverse_xy(h, grid_of(s), x1, (int)d.nF, x2, (int)d.fovY0, e, "wf_rows_c2r_noise");
+   }
    if (!ok(h, hipGetLastError(), "iscg_wf_conv3d launch"))
       return -1;
    if (dAdu && !launch_noise_fallback(h, e.q, dAdu, d.ch * d.cw, nullptr, frameBase, dOff, nullptr, nullptr, e.cp,

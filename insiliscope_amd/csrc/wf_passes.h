@@ -112,6 +112,36 @@ struct C2rEpilogue
    int ch, cw, x0;
 };
 
+// The epilogue of the C2R passes: line l of the block holds rows 2 (p0 + l) (real part) and 2 (p0 + l) + 1
+// (imaginary part) after the inverse transform; columns [x0, x0 + cw) kept. The frame is max(0, image) + background
+// (WidefieldImages::Render, Sim/WidefieldRender.cpp:547-580), then the noise chain (frame frameBase + row / ch,
+// pixel (row % ch) cw + x + pixelBase).
+template <bool NOISE, bool EMCCD>
+__device__ __forceinline__ void c2r_epilogue(const C2rEpilogue& e, const float2* buf, int LS, int LINES, int p0,
+                                             int pairs, int nRows, int tid)
+{
+   for (int i = tid; i < LINES * 2 * e.cw; i += blockDim.x)
+   {
+      const int l = i / (2 * e.cw), rem = i % (2 * e.cw), half = rem / e.cw, x = rem % e.cw, pr = p0 + l;
+      const int row = 2 * pr + half;
+      if (pr >= pairs || row >= nRows)
+         continue;
+      const float2 z = buf[l * LS + e.x0 + x];
+      const float v = half ? z.y : z.x;
+      const float ph = e.raw ? v : (float)((double)(v > 0.0f ? v : 0.0f) + e.background);
+      const int f = row / e.ch, y = row % e.ch;
+      const size_t o = (size_t)row * e.cw + x;
+      if (e.img)
+         e.img[o] = ph;
+      if (NOISE)
+      {
+         const uint32_t pix = (uint32_t)(y * e.cw + x);
+         noise_store<EMCCD>(e.cp, e.q, e.adu, (uint32_t)o, ph, e.seed, e.frameBase + (uint32_t)f, pix + e.pixelBase,
+                            e.offset != nullptr, e.offset ? e.offset[pix] : 0.0f, false, 0.0f, false, 0.0f);
+      }
+   }
+}
+
 // Pass E: rows C2R. Half spectra A (row r) and B (row r + 1) become Z = A + iB over the full length (Hermitian
 // extension; DC and Nyquist real), the inverse FFT gives a + ib. Columns [x0, x0 + cw) kept; max(0, .) +
 // background, then the noise chain (frame frameBase + r / ch, pixel (r % ch) cw + x + pixelBase).
@@ -147,27 +177,97 @@ __global__ __launch_bounds__(256) void rows_c2r_kernel(const float2* __restrict_
       }
       __syncthreads();
       fft_lines<N, true, false>(buf, LS, 1, LINES, tid, tw);
-      for (int i = tid; i < LINES * 2 * e.cw; i += blockDim.x)
+      c2r_epilogue<NOISE, EMCCD>(e, buf, LS, LINES, p0, pairs, nRows, tid);
+      __syncthreads();
+   }
+}
+
+// Passes A and E for the fused slab pass (wf.hip), whose half spectra are column-major: column k's rows are
+// contiguous, element (row, k) at k * colStride + row (colStride even). kCmThreads threads, kCmThreads / (N / 8) lines
+// per block (N = 512: 8 complex lines = 16 rows, 128-byte column segments); rows r and r + 1 move as one 16-byte pair.
+constexpr int kCmThreads = 256;
+template <int N>
+__global__ __launch_bounds__(kCmThreads) void rows_r2c_cm_kernel(const float* __restrict__ in, size_t inRowStride,
+                                                                 int nRows, int nIn, float2* __restrict__ out,
+                                                                 size_t colStride, const float2* __restrict__ tw)
+{
+   constexpr int T = N / 8, LINES = kCmThreads / T > 0 ? kCmThreads / T : 1, LS = N + 1;
+   __shared__ float2 buf[LINES * LS];
+   const int tid = threadIdx.x;
+   const int pairs = (nRows + 1) / 2;
+   for (int p0 = blockIdx.x * LINES; p0 < pairs; p0 += gridDim.x * LINES)
+   {
+      for (int i = tid; i < LINES * N; i += blockDim.x)
       {
-         const int l = i / (2 * e.cw), rem = i % (2 * e.cw), half = rem / e.cw, x = rem % e.cw, pr = p0 + l;
-         const int row = 2 * pr + half;
-         if (pr >= pairs || row >= nRows)
-            continue;
-         const float2 z = buf[l * LS + e.x0 + x];
-         const float v = half ? z.y : z.x;
-         // the frame: max(0, image) + background (WidefieldImages::Render, Sim/WidefieldRender.cpp:547-580)
-         const float ph = e.raw ? v : (float)((double)(v > 0.0f ? v : 0.0f) + e.background);
-         const int f = row / e.ch, y = row % e.ch;
-         const size_t o = (size_t)row * e.cw + x;
-         if (e.img)
-            e.img[o] = ph;
-         if (NOISE)
+         const int l = i / N, n = i % N, pr = p0 + l;
+         const int ra = 2 * pr, rb = 2 * pr + 1;
+         float a = 0.0f, b = 0.0f;
+         if (pr < pairs && n < nIn)
          {
-            const uint32_t pix = (uint32_t)(y * e.cw + x);
-            noise_store<EMCCD>(e.cp, e.q, e.adu, (uint32_t)o, ph, e.seed, e.frameBase + (uint32_t)f, pix + e.pixelBase,
-                               e.offset != nullptr, e.offset ? e.offset[pix] : 0.0f, false, 0.0f, false, 0.0f);
+            a = in[(size_t)ra * inRowStride + n];
+            b = rb < nRows ? in[(size_t)rb * inRowStride + n] : 0.0f;
          }
+         buf[l * LS + n] = make_float2(a, b);
       }
+      __syncthreads();
+      fft_lines<N, false, false>(buf, LS, 1, LINES, tid, tw);
+      for (int i = tid; i < LINES * (N / 2 + 1); i += blockDim.x)
+      {
+         const int k = i / LINES, l = i % LINES, pr = p0 + l;
+         if (pr >= pairs)
+            continue;
+         const float2 z = buf[l * LS + k], zc = buf[l * LS + ((N - k) & (N - 1))];
+         const float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+         const float2 B = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+         float2* o = out + (size_t)k * colStride + 2 * pr;
+         if (2 * pr + 1 < nRows)
+            *reinterpret_cast<float4*>(o) = make_float4(A.x, A.y, B.x, B.y);
+         else
+            o[0] = A;
+      }
+      __syncthreads();
+   }
+}
+
+template <int N, bool NOISE, bool EMCCD>
+__global__ __launch_bounds__(kCmThreads) void rows_c2r_cm_kernel(const float2* __restrict__ in, size_t colStride,
+                                                                 int nRows, C2rEpilogue e,
+                                                                 const float2* __restrict__ tw)
+{
+   constexpr int T = N / 8, LINES = kCmThreads / T > 0 ? kCmThreads / T : 1, LS = N + 1;
+   __shared__ float2 buf[LINES * LS];
+   const int tid = threadIdx.x;
+   const int pairs = (nRows + 1) / 2;
+   for (int p0 = blockIdx.x * LINES; p0 < pairs; p0 += gridDim.x * LINES)
+   {
+      for (int i = tid; i < LINES * (N / 2 + 1); i += blockDim.x)
+      {
+         const int k = i / LINES, l = i % LINES, pr = p0 + l;
+         float2 A = make_float2(0.0f, 0.0f), B = make_float2(0.0f, 0.0f);
+         if (pr < pairs)
+         {
+            const float2* s = in + (size_t)k * colStride + 2 * pr;
+            if (2 * pr + 1 < nRows)
+            {
+               const float4 ab = *reinterpret_cast<const float4*>(s);
+               A = make_float2(ab.x, ab.y);
+               B = make_float2(ab.z, ab.w);
+            }
+            else
+               A = s[0];
+         }
+         if (k == 0 || k == N / 2)
+         {
+            A.y = 0.0f;
+            B.y = 0.0f;
+         }
+         buf[l * LS + k] = make_float2(A.x - B.y, A.y + B.x);
+         if (k != 0 && k != N / 2)
+            buf[l * LS + N - k] = make_float2(A.x + B.y, B.x - A.y);
+      }
+      __syncthreads();
+      fft_lines<N, true, false>(buf, LS, 1, LINES, tid, tw);
+      c2r_epilogue<NOISE, EMCCD>(e, buf, LS, LINES, p0, pairs, nRows, tid);
       __syncthreads();
    }
 }
@@ -243,6 +343,37 @@ struct RowsC2R
       else
          hipLaunchKernelGGL((rows_c2r_kernel<N, true, false>), dim3(wgs(items)), dim3(256), 0, s, in, S, nRows, e,
                             tw);
+   }
+};
+
+template <int N>
+struct RowsR2Ccm
+{
+   static void run(hipStream_t s, const float* in, size_t inRowStride, int nRows, int nIn, float2* out,
+                   size_t colStride, const float2* tw)
+   {
+      constexpr int LINES = kCmThreads / (N / 8) > 0 ? kCmThreads / (N / 8) : 1;
+      const size_t items = ((size_t)(nRows + 1) / 2 + LINES - 1) / LINES;
+      hipLaunchKernelGGL(rows_r2c_cm_kernel<N>, dim3(wgs(items, 256 * 8)), dim3(kCmThreads), 0, s, in, inRowStride,
+                         nRows, nIn, out, colStride, tw);
+   }
+};
+
+template <int N>
+struct RowsC2Rcm
+{
+   static void run(hipStream_t s, const float2* in, size_t colStride, int nRows, const C2rEpilogue& e,
+                   const float2* tw)
+   {
+      constexpr int LINES = kCmThreads / (N / 8) > 0 ? kCmThreads / (N / 8) : 1;
+      const size_t items = ((size_t)(nRows + 1) / 2 + LINES - 1) / LINES;
+      const dim3 g(wgs(items, 256 * 8)), b(kCmThreads);
+      if (!e.adu)
+         hipLaunchKernelGGL((rows_c2r_cm_kernel<N, false, false>), g, b, 0, s, in, colStride, nRows, e, tw);
+      else if (e.cp.emccd)
+         hipLaunchKernelGGL((rows_c2r_cm_kernel<N, true, true>), g, b, 0, s, in, colStride, nRows, e, tw);
+      else
+         hipLaunchKernelGGL((rows_c2r_cm_kernel<N, true, false>), g, b, 0, s, in, colStride, nRows, e, tw);
    }
 };
 

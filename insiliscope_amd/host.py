@@ -158,6 +158,40 @@ class CameraNoiseParams:
                            self.cicElectrons)
 
 
+def Pcg4d(a, b, c, d, ctx: Context | None = None):
+    """isc::Pcg4d (core/src/rng.h:18-38) of arrays of addresses on the GPU: (n, 4) uint32 words."""
+    ctx = ctx or default_context()
+    arrs = [np.ascontiguousarray(np.asarray(x, np.int64) & 0xFFFFFFFF, np.uint32) for x in (a, b, c, d)]
+    n = int(arrs[0].size)
+    out = np.zeros((n, 4), np.uint32)
+    ctx.follow_torch()
+    ctx.check(ctx.L.iscg_pcg4d(ctx.h, *[x.ctypes.data for x in arrs], n, out.ctypes.data), "iscg_pcg4d")
+    return out
+
+
+def HashUnits(seed: int, cx, cy, k, ctx: Context | None = None):
+    """isc::HashUnit (core/src/rng.h:40-44) of arrays of (cx, cy, k) on the GPU: float64 (0, 1) draws."""
+    ctx = ctx or default_context()
+    x = np.ascontiguousarray(np.asarray(cx, np.int64).astype(np.int32))
+    y = np.ascontiguousarray(np.asarray(cy, np.int64).astype(np.int32))
+    kk = np.ascontiguousarray(np.asarray(k, np.int64) & 0xFFFFFFFF, np.uint32)
+    out = np.zeros(x.size)
+    ctx.follow_torch()
+    ctx.check(ctx.L.iscg_hash_units(ctx.h, seed & 0xFFFFFFFF, x.ctypes.data, y.ctypes.data, kk.ctypes.data, x.size,
+                                    out.ctypes.data), "iscg_hash_units")
+    return out
+
+
+def HashStream(seed: int, cx: int, cy: int, base: int, n: int, ctx: Context | None = None):
+    """The first n draws of isc::HashStream(seed, cx, cy, base) (core/src/rng.h:46-52) on the GPU."""
+    ctx = ctx or default_context()
+    out = np.zeros(n)
+    ctx.follow_torch()
+    ctx.check(ctx.L.iscg_hash_stream(ctx.h, seed & 0xFFFFFFFF, int(np.int32(cx)), int(np.int32(cy)),
+                                     base & 0xFFFFFFFF, n, out.ctypes.data), "iscg_hash_stream")
+    return out
+
+
 @dataclass
 class NoiseSetup:
     """The camera's maps and seeds for a random seed (NoiseSetup, Sim/ScopeMovie.cpp:973-996; the maps of

@@ -24,16 +24,19 @@ def small_case(golden, seed=3, nx=48, ny=40, nzD=6):
     return planes, dens.astype(np.float32)
 
 
+# fft (0, 0, 0): the smallest grid (64 x 64 x 16, the five-pass path); (64, 512, 128): the fused slab path
+# (one workgroup per kx column does the y and z work; columns 0 and NX/2 packed into one workgroup)
+@pytest.mark.parametrize("fft", [(0, 0, 0), (64, 512, 128)])
 @pytest.mark.parametrize("delta_um,fov", [(0.0, (0, 0, 0, 0)), (0.013, (5, 3, 30, 25)), (-0.031, (0, 7, 48, 20))])
-def test_wf_focus_series_vs_direct(ctx, orc, golden, delta_um, fov):
+def test_wf_focus_series_vs_direct(ctx, orc, golden, delta_um, fov, fft):
     planes, dens = small_case(golden)
     nzD, ny, nx = dens.shape
     dz, k0, nF = 0.05, -3, 7
     focus0 = (k0 + 0.5) * dz + delta_um
-    print("delta", delta_um, "fov", fov, flush=True)
+    print("delta", delta_um, "fov", fov, "fft", fft, flush=True)
     cache = host.PsfKernelCache(planes, oversampling=4, zStepNm=50.0)
     wf = host.WidefieldStack(ctx)
-    assert wf.Plan(cache, nx, ny, nzD, dz, k0, focus0, nF, fov=fov), wf.err
+    assert wf.Plan(cache, nx, ny, nzD, dz, k0, focus0, nF, fov=fov, fft=fft), wf.err
     nF_, ch, cw = wf.image_shape
     img = np.zeros((nF, ch, cw), np.float32)
     assert wf.FocusSeries(dens, outImages=img), wf.err
@@ -48,13 +51,14 @@ def test_wf_focus_series_vs_direct(ctx, orc, golden, delta_um, fov):
         assert e <= TOL, f"focus {f}: rel L2 {e:.3g}"
 
 
-def test_wf_noise_epilogue_bit_exact(ctx, orc, golden):
+@pytest.mark.parametrize("fft", [(0, 0, 0), (64, 512, 128)])
+def test_wf_noise_epilogue_bit_exact(ctx, orc, golden, fft):
     planes, dens = small_case(golden, seed=4)
     nzD, ny, nx = dens.shape
     dens *= 40.0
     cache = host.PsfKernelCache(planes, oversampling=4, zStepNm=50.0)
     wf = host.WidefieldStack(ctx)
-    assert wf.Plan(cache, nx, ny, nzD, 0.05, 0, 0.025, 5), wf.err
+    assert wf.Plan(cache, nx, ny, nzD, 0.05, 0, 0.025, 5, fft=fft), wf.err
     shp = wf.image_shape
     off = np.full(shp[1:], 100.0, np.float32)
     cam = host.CameraNoiseParams(quantumEfficiency=1.0, gainPhotonsPerAdu=2.0, readNoiseElectrons=1.5)
@@ -89,3 +93,40 @@ def test_wf_delta_density_reproduces_kernel(ctx, orc, golden):
                 want[y, x] = K[dy + R, dx + R]
     assert rel_l2(img[0], want) < TOL
     assert abs(img[0].astype(np.float64).sum() - want.sum()) < 1e-5
+
+
+def test_wf_slab_full_size_vs_five_pass(ctx, orc, golden):
+    """The headline geometry (256 x 256 x 64 dye cells, 64 foci, 512 x 512 x 128 grid: the slab path) against the
+    five-pass path on a 512 x 512 x 256 grid (the same linear convolution, no wrap), with a FOV crop; and one dye
+    at full size reproducing its kernel plane."""
+    P = golden("psf_ref.npz")
+    planes = P["small_planes"]
+    rng = np.random.default_rng(11)
+    nx = ny = 256
+    nzD = nF = 64
+    dens = np.where(rng.random((nzD, ny, nx)) < 0.03, 1.0 + np.floor(rng.random((nzD, ny, nx)) * 3), 0.0)
+    dens = dens.astype(np.float32)
+    cache = host.PsfKernelCache(planes, oversampling=4, zStepNm=50.0)
+    fov = (3, 17, 250, 200)
+    imgs = []
+    for fft in ((512, 512, 128), (512, 512, 256)):
+        print("fft", fft, flush=True)
+        wf = host.WidefieldStack(ctx)
+        assert wf.Plan(cache, nx, ny, nzD, 0.05, 0, 0.025, nF, fov=fov, fft=fft), wf.err
+        img = np.zeros(wf.image_shape, np.float32)
+        assert wf.FocusSeries(dens, outImages=img), wf.err
+        imgs.append(img)
+    for f in range(nF):
+        e = rel_l2(imgs[0][f], imgs[1][f])
+        assert e <= TOL, f"focus {f}: rel L2 {e:.3g}"
+    one = np.zeros((nzD, ny, nx), np.float32)
+    one[40, 100, 201] = 1.0
+    wf = host.WidefieldStack(ctx)
+    assert wf.Plan(cache, nx, ny, nzD, 0.05, 0, 0.05 * 40 + 0.025, nF, fft=(512, 512, 128)), wf.err
+    img = np.zeros(wf.image_shape, np.float32)
+    assert wf.FocusSeries(one, outImages=img), wf.err
+    R = orc.wf_radius(65, 4, 1)
+    K = orc.wf_kernel(planes[4], 4, 1, R)
+    want = np.zeros((ny, nx))
+    want[100 - R:100 + R + 1, 201 - R:201 + R + 1] = K
+    assert rel_l2(img[0], want) < TOL
