@@ -202,6 +202,20 @@ __global__ __launch_bounds__(zconv_cols<N>() * N / 8) void zconv_kernel(const fl
 
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 
+// Complex products of the slab pass with fused multiply-adds (a transform within FP32 rounding either way; the
+// library's other arithmetic is not contracted)
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b)
+{
+#pragma clang fp contract(fast)
+   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a b + c
+__device__ __forceinline__ float2 cmaf(float2 a, float2 b, float2 c)
+{
+#pragma clang fp contract(fast)
+   return make_float2(a.x * b.x - a.y * b.y + c.x, a.x * b.y + a.y * b.x + c.y);
+}
+
 // Butterfly j of a radix-R Stockham stage of length N (NS = product of the earlier radices): inputs at j + r N / R,
 // twiddled by tw[TWS r k N / (NS R)] (k = j mod NS; tw a table of length TWS N), outputs to
 // (j / NS) NS R + k + r NS.
@@ -217,7 +231,7 @@ __device__ __forceinline__ void bfly(float2 (&v)[R], int j, const float2* tw)
          float2 t = tw[TWS * r * k * (N / (NS * R))];
          if (INV)
             t.y = -t.y;
-         v[r] = cmul(v[r], t);
+         v[r] = cmulf(v[r], t);
       }
    }
    dft<R, INV>(v);
@@ -310,7 +324,7 @@ __device__ __forceinline__ void line_fft128(float2* L, int q, float2 (&v)[8], co
          float2 t = tw[j];
          if (INV)
             t.y = -t.y;
-         v[s] = cadd(u[0], cmul(u[1], t));
+         v[s] = cmaf(u[1], t, u[0]);
       }
       else
       {
@@ -388,7 +402,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
                const size_t o = (size_t)z * ny + y;
                x = packed ? make_float2(src[o].x, srcN[o].x) : src[o];
             }
-            v[r] = h ? cmul(x, twy[y]) : x;
+            v[r] = h ? cmulf(x, twy[y]) : x;
          }
          line_fft256<false, true, false>(buf + zl * LSY, q, v, twy);
          __syncthreads();
@@ -472,7 +486,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
                   P = make_float2(p.x, -p.y);
                   Q = make_float2(qq.x, -qq.y);
                }
-               v[m] = cadd(cmul(v[m], P), cmul(make_float2(up.x, -up.y), Q));
+               v[m] = cmaf(make_float2(up.x, -up.y), Q, cmulf(v[m], P));
                // one k at a time: the scheduler would otherwise issue every iteration's LDS reads at once
                __builtin_amdgcn_sched_barrier(0);
             }
@@ -481,7 +495,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
          {
 #pragma unroll
             for (int m = 0; m < 8; ++m)
-               v[m] = cmul(v[m], g[m]);
+               v[m] = cmulf(v[m], g[m]);
          }
          wave_sync();
          line_fft128<true, true>(Z, q, v, twz);
@@ -532,7 +546,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
                if (h)
                {
                   const float2 tw = twy[y];
-                  val = cadd(cmul(val, make_float2(tw.x, -tw.y)), p);
+                  val = cmaf(val, make_float2(tw.x, -tw.y), p);
                }
                if (packed)
                {
