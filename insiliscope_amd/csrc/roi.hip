@@ -11,10 +11,11 @@
 //   -Im(P'_mn conj(B_mn)); x, y, z and the Zernike coefficients are linear in the phase, so their gradients are
 //   weighted sums of that map. The pupil is recomputed in the backward pass, never stored.
 //
-// One workgroup (256 lanes) per ROI, grid-stride over ROIs. The four 64 x 64 x 64 complex products run in FP32
-// from LDS with 4 x 4 register tiles (sizes padded to 64 with zero W rows); the pupil phase is FP64, wrapped before
-// the FP32 sincos. Per-ROI partial sums (Zernike gradient, NLL) are reduced by a second kernel in a fixed order, so
-// results do not depend on scheduling.
+// One workgroup (512 lanes) per CU, ROIs taken in a fixed order. The four 64 x 64 x 64 complex products run on FP32
+// MFMA from LDS (sizes padded to 64 with zero W rows); the pupil phase is FP64, wrapped before the FP32 sincos. The
+// shared Zernike gradient is the dot product of dNLL/dphase with each basis function: each workgroup sums its ROIs'
+// phase-gradient maps (FP64), and two small kernels add the maps in workgroup order and take the 28 dot products,
+// so results do not depend on scheduling.
 #include "iscg_internal.h"
 #include "fft_lds.h"
 
@@ -26,10 +27,12 @@ namespace iscg {
 constexpr int kN = 64;     // padded matrix edge
 constexpr int kLS = kN + 1; // LDS row (float2)
 constexpr double kPiR = 3.14159265358979323846;
+constexpr int kPartials = 28 + 1;  // Zernike gradient, NLL
+constexpr int kRoiGrid = 256;      // workgroups of the ROI kernels: one per CU, ROIs assigned in a fixed order
 
 struct RoiState
 {
-   DevBuf A, B, Zb, kx, W, WT, partial;
+   DevBuf A, B, Zb, kx, W, partial;
    iscg_roi_model model{};
    double zern[28] = {0};
    bool tablesValid = false;
@@ -50,16 +53,16 @@ struct RoiConst
 };
 
 // Pupil tables, FP64: A (base phase, NaN outside), B (phase per metre of ti - ti0), Zernike basis values (FP32) per
-// OSA slot; kx; W and W^T (FP32, zero beyond Ps / M).
+// OSA slot; kx; W (FP32, zero beyond Ps / M).
 __global__ void roi_tables_kernel(int M, int Ps, int os, double k0, double NA, double ni, double ns, double bMax,
                                   double kMax, double dk, double kMin, double depth, double pix, const double* zern,
-                                  double* A, double* B, float* Zb, double* kx, float2* W, float2* WT)
+                                  double* A, double* B, float* Zb, double* kx, float2* W)
 {
    const int total = kN * kN;
    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x)
    {
       const int iy = idx / kN, ix = idx % kN;
-      // W[s][m] and W^T
+      // W[s][m]
       {
          const int s = iy, m = ix;
          float2 w = make_float2(0.0f, 0.0f);
@@ -73,7 +76,6 @@ __global__ void roi_tables_kernel(int M, int Ps, int os, double k0, double NA, d
             w = make_float2(cs, sn);
          }
          W[s * kN + m] = w;
-         WT[m * kN + s] = w;
       }
       if (iy == 0 && ix < kN)
          kx[ix] = kMin + ix * dk;
@@ -134,50 +136,66 @@ __global__ void roi_tables_kernel(int M, int Ps, int os, double k0, double NA, d
    }
 }
 
-// C (4 x 4 complex tile per lane, rows 4 ti.., cols 4 tj..) = sum_k A[i][k] B[k][j] over k < 64.
-// AL/BL: operand in LDS (row stride kLS) or global (row stride kN); CA/CB: conjugate the operand.
-template <bool AL, bool CA, bool BL, bool CB>
-__device__ __forceinline__ void cgemm(const float2* __restrict__ Am, const float2* __restrict__ Bm, float2 (&c)[4][4],
-                                      int ti, int tj)
+// ---- complex 64 x 64 x 64 products on FP32 MFMA -------------------------------------------------------------------
+// 512 lanes = 8 waves; wave w computes rows 16 (w >> 1) .. + 15 and columns 32 (w & 1) .. + 31 of C as two 16 x 16
+// tiles with v_mfma_f32_16x16x4_f32 (A operand: lane l holds A[l % 16][k0 + l / 16]; B: B[k0 + l / 16][l % 16];
+// D register r: row 4 (l / 16) + r, column l % 16). A complex product is four real ones:
+// C_re += a_re b_re + (-a_im) b_im, C_im += a_re b_im + a_im b_re.
+constexpr int kRoiThreads = 512;
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4v mfma4(float a, float b, float4v c)
+{
+   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+struct CTile
+{
+   float4v re[2], im[2];  // tiles t = 0, 1: element (row 16 rb + 4 (lane >> 4) + r, col 32 ch + 16 t + (lane & 15))
+};
+
+// C = A B over k < 64, operands in LDS (row stride kLS). TA: A(i, k) = Al[k][i] (else Al[i][k]); TB: B(k, j) =
+// Bl[j][k] (else Bl[k][j]); CA / CB: conjugate the operand.
+template <bool TA, bool CA, bool TB, bool CB>
+__device__ __forceinline__ void cgemm_mfma(const float2* Al, const float2* Bl, CTile& c, int rb, int ch, int lane)
 {
 #pragma unroll
-   for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-         c[i][j] = make_float2(0.0f, 0.0f);
-   constexpr int sa = AL ? kLS : kN, sb = BL ? kLS : kN;
+   for (int t = 0; t < 2; ++t)
+      c.re[t] = c.im[t] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+   const int i = 16 * rb + (lane & 15), kk = lane >> 4;
+   const int j0 = 32 * ch + (lane & 15), j1 = j0 + 16;
 #pragma unroll 4
-   for (int k = 0; k < kN; ++k)
+   for (int k0 = 0; k0 < kN; k0 += 4)
    {
-      float2 a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
+      const int k = k0 + kk;
+      float2 a = TA ? Al[k * kLS + i] : Al[i * kLS + k];
+      float2 b0 = TB ? Bl[j0 * kLS + k] : Bl[k * kLS + j0];
+      float2 b1 = TB ? Bl[j1 * kLS + k] : Bl[k * kLS + j1];
+      if (CA)
+         a.y = -a.y;
+      if (CB)
       {
-         a[i] = Am[(4 * ti + i) * sa + k];
-         if (CA)
-            a[i].y = -a[i].y;
-         b[i] = Bm[k * sb + 4 * tj + i];
-         if (CB)
-            b[i].y = -b[i].y;
+         b0.y = -b0.y;
+         b1.y = -b1.y;
       }
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-         for (int j = 0; j < 4; ++j)
-         {
-            c[i][j].x += a[i].x * b[j].x - a[i].y * b[j].y;
-            c[i][j].y += a[i].x * b[j].y + a[i].y * b[j].x;
-         }
+      c.re[0] = mfma4(a.x, b0.x, c.re[0]);
+      c.re[1] = mfma4(a.x, b1.x, c.re[1]);
+      c.im[0] = mfma4(a.x, b0.y, c.im[0]);
+      c.im[1] = mfma4(a.x, b1.y, c.im[1]);
+      c.re[0] = mfma4(-a.y, b0.y, c.re[0]);
+      c.re[1] = mfma4(-a.y, b1.y, c.re[1]);
+      c.im[0] = mfma4(a.y, b0.x, c.im[0]);
+      c.im[1] = mfma4(a.y, b1.x, c.im[1]);
    }
 }
 
-__device__ __forceinline__ void store_tile(float2* buf, const float2 (&c)[4][4], int ti, int tj)
+__device__ __forceinline__ void store_ctile(float2* buf, const CTile& c, int rb, int ch, int lane)
 {
 #pragma unroll
-   for (int i = 0; i < 4; ++i)
+   for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-         buf[(4 * ti + i) * kLS + 4 * tj + j] = c[i][j];
+      for (int r = 0; r < 4; ++r)
+         buf[(16 * rb + 4 * (lane >> 4) + r) * kLS + 32 * ch + 16 * t + (lane & 15)] = make_float2(c.re[t][r], c.im[t][r]);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -188,41 +206,43 @@ __device__ __forceinline__ double wave_sum(double v)
    return v;
 }
 
-// Sums v over the workgroup (every lane gets the total). red: 4 doubles of LDS.
-__device__ __forceinline__ double block_sum(double v, double* red)
-{
-   v = wave_sum(v);
-   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-   __syncthreads();
-   if (lane == 0)
-      red[wave] = v;
-   __syncthreads();
-   return red[0] + red[1] + red[2] + red[3];
-}
+constexpr int kMapStride = kN * kN + 1;  // per workgroup: the phase-gradient map, then the NLL
 
-constexpr int kPartials = 28 + 1;  // Zernike gradient, NLL
-
+// One workgroup per CU, ROIs blockIdx.x, + gridDim.x, ... in order. Backward: per ROI the x, y, z, N, b gradients;
+// per workgroup the sum over its ROIs of the phase-gradient map dNLL/dphase (FP64, [n][m]) and of the NLL, reduced
+// afterwards in workgroup order (the shared Zernike gradient is the map's dot product with each basis function).
 template <bool BACKWARD>
-__global__ __launch_bounds__(256) void roi_kernel(RoiConst rc, const iscg_roi_params* __restrict__ params, int nRoi,
-                                                  const double* __restrict__ A, const double* __restrict__ B,
-                                                  const float* __restrict__ Zb, const double* __restrict__ kx,
-                                                  const float2* __restrict__ W, const float2* __restrict__ WT,
-                                                  const float* __restrict__ data, float* __restrict__ mu,
-                                                  double* __restrict__ gradParams, double* __restrict__ partial)
+__global__ __launch_bounds__(kRoiThreads) void roi_kernel(RoiConst rc, const iscg_roi_params* __restrict__ params,
+                                                          int nRoi, const double* __restrict__ A,
+                                                          const double* __restrict__ B, const double* __restrict__ kx,
+                                                          const float2* __restrict__ W, const float* __restrict__ data,
+                                                          float* __restrict__ mu, double* __restrict__ gradParams,
+                                                          double* __restrict__ wgMaps)
 {
-   __shared__ float2 buf0[kN * kLS];   // P'
-   __shared__ float2 buf1[kN * kLS];   // T, then G, then U
-   __shared__ float inten[kN * kN];    // |E|^2 per sample
-   __shared__ float wbuf[kN * kN];     // 1 - d / mu per pixel
-   __shared__ double red[4];
-   const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
-   const int M = rc.M, R = rc.R, os = rc.os, Ps = rc.Ps;
-   for (int r = blockIdx.x; r < nRoi; r += gridDim.x)
+   __shared__ float2 Pl[kN * kLS];  // P'
+   __shared__ float2 Xl[kN * kLS];  // T, then |E|^2 (as float [64][64]), then G, then U
+   __shared__ float2 Wl[kN * kLS];  // W
+   __shared__ float wbuf[kN * kN];  // 1 - d / mu per pixel
+   __shared__ double red[8][6];
+   float* inten = reinterpret_cast<float*>(Xl);
+   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rb = w >> 1, ch = w & 1;
+   const int R = rc.R, os = rc.os, Ps = rc.Ps;
+   for (int idx = tid; idx < kN * kN; idx += kRoiThreads)
+      Wl[(idx / kN) * kLS + idx % kN] = W[idx];
+   double gmap[2][4];  // this lane's elements of the phase-gradient map, summed over the workgroup's ROIs
+   double nllWg = 0.0;
+#pragma unroll
+   for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+         gmap[t][r] = 0.0;
+   __syncthreads();
+   for (int roi = blockIdx.x; roi < nRoi; roi += gridDim.x)
    {
-      const iscg_roi_params q = params[r];
+      const iscg_roi_params q = params[roi];
       const double tz = -rc.focalShift + q.z * 1e-9;
       // 1. the shifted pupil (FP64 phase)
-      for (int idx = tid; idx < kN * kN; idx += 256)
+      for (int idx = tid; idx < kN * kN; idx += kRoiThreads)
       {
          const int n = idx / kN, m = idx % kN;
          float2 v = make_float2(0.0f, 0.0f);
@@ -235,25 +255,27 @@ __global__ __launch_bounds__(256) void roi_kernel(RoiConst rc, const iscg_roi_pa
             sincosf((float)rr, &sn, &cs);
             v = make_float2(cs, sn);
          }
-         buf0[n * kLS + m] = v;
+         Pl[n * kLS + m] = v;
       }
       __syncthreads();
-      float2 c[4][4];
-      // 2. T[n][s] = sum_m P'[n][m] W^T[m][s]
-      cgemm<true, false, false, false>(buf0, WT, c, ti, tj);
-      store_tile(buf1, c, ti, tj);
+      CTile c;
+      // 2. T[n][s] = sum_m P'[n][m] W[s][m]
+      cgemm_mfma<false, false, true, false>(Pl, Wl, c, rb, ch, lane);
+      store_ctile(Xl, c, rb, ch, lane);
       __syncthreads();
       // 3. E[t][s] = sum_n W[t][n] T[n][s]  (kept in registers)
-      cgemm<false, false, true, false>(W, buf1, c, ti, tj);
+      cgemm_mfma<false, false, false, false>(Wl, Xl, c, rb, ch, lane);
+      __syncthreads();
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int t = 0; t < 2; ++t)
 #pragma unroll
-         for (int j = 0; j < 4; ++j)
-            inten[(4 * ti + i) * kN + 4 * tj + j] = c[i][j].x * c[i][j].x + c[i][j].y * c[i][j].y;
+         for (int r = 0; r < 4; ++r)
+            inten[(16 * rb + 4 * (lane >> 4) + r) * kN + 32 * ch + 16 * t + (lane & 15)] =
+               c.re[t][r] * c.re[t][r] + c.im[t][r] * c.im[t][r];
       __syncthreads();
       // 4. pixels: mu, and for the adjoint w = 1 - d / mu, NLL, dN, db
       double nll = 0.0, gN = 0.0, gb = 0.0;
-      for (int p = tid; p < R * R; p += 256)
+      for (int p = tid; p < R * R; p += kRoiThreads)
       {
          const int py = p / R, px = p % R;
          double acc = 0.0;
@@ -263,105 +285,123 @@ __global__ __launch_bounds__(256) void roi_kernel(RoiConst rc, const iscg_roi_pa
          const double f = rc.norm * acc;
          const double m = q.photons * f + q.background;
          if (!BACKWARD)
-            mu[(size_t)r * R * R + p] = (float)m;
+            mu[(size_t)roi * R * R + p] = (float)m;
          else
          {
-            const double d = data[(size_t)r * R * R + p];
+            const double d = data[(size_t)roi * R * R + p];
             nll += m - d * log(m);
-            const double w = 1.0 - d / m;
-            wbuf[p] = (float)w;
-            gN += w * f;
-            gb += w;
+            const double wv = 1.0 - d / m;
+            wbuf[p] = (float)wv;
+            gN += wv * f;
+            gb += wv;
          }
       }
-      if (!BACKWARD)
-      {
-         __syncthreads();
-         continue;
-      }
       __syncthreads();
+      if (!BACKWARD)
+         continue;
       // 5. G_s = 2 N norm w_p(s) E_s
       const float gscale = (float)(2.0 * q.photons * rc.norm);
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int t = 0; t < 2; ++t)
 #pragma unroll
-         for (int j = 0; j < 4; ++j)
+         for (int r = 0; r < 4; ++r)
          {
-            const int t = 4 * ti + i, s = 4 * tj + j;
-            const float g = (t < Ps && s < Ps) ? gscale * wbuf[(t / os) * R + s / os] : 0.0f;
-            buf1[t * kLS + s] = make_float2(g * c[i][j].x, g * c[i][j].y);
+            const int ty = 16 * rb + 4 * (lane >> 4) + r, s = 32 * ch + 16 * t + (lane & 15);
+            const float g = (ty < Ps && s < Ps) ? gscale * wbuf[(ty / os) * R + s / os] : 0.0f;
+            Xl[ty * kLS + s] = make_float2(g * c.re[t][r], g * c.im[t][r]);
          }
       __syncthreads();
-      // 6. U[n][s] = sum_t conj(W[t][n]) G[t][s] = sum_t conj(W^T[n][t]) G[t][s]
-      cgemm<false, true, true, false>(WT, buf1, c, ti, tj);
+      // 6. U[n][s] = sum_t conj(W[t][n]) G[t][s]
+      cgemm_mfma<true, true, false, false>(Wl, Xl, c, rb, ch, lane);
       __syncthreads();
-      store_tile(buf1, c, ti, tj);
+      store_ctile(Xl, c, rb, ch, lane);
       __syncthreads();
       // 7. B[n][m] = sum_s U[n][s] conj(W[s][m])
-      cgemm<true, false, false, true>(buf1, W, c, ti, tj);
-      // 8. dNLL/dphase = -Im(P' conj(B)); reductions
+      cgemm_mfma<false, false, false, true>(Xl, Wl, c, rb, ch, lane);
+      // 8. dNLL/dphase = -Im(P' conj(B)) (zero outside the aperture, where P' is zero); reductions
       double gx = 0.0, gy = 0.0, gz = 0.0;
-      double gzern[28];
 #pragma unroll
-      for (int j = 0; j < 28; ++j)
-         gzern[j] = 0.0;
+      for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-         for (int j = 0; j < 4; ++j)
+         for (int r = 0; r < 4; ++r)
          {
-            const int n = 4 * ti + i, m = 4 * tj + j;
-            const int idx = n * kN + m;
-            const float2 P = buf0[n * kLS + m];
-            const double g = -((double)P.y * c[i][j].x - (double)P.x * c[i][j].y);
-            if (n < M && m < M && A[idx] == A[idx])
-            {
-               gx += g * (-kx[m]) * rc.pix;
-               gy += g * (-kx[n]) * rc.pix;
-               gz += g * B[idx] * 1e-9;
-#pragma unroll
-               for (int z = 0; z < 28; ++z)
-                  gzern[z] += g * (2.0 * kPiR) * (double)Zb[(size_t)z * kN * kN + idx];
-            }
+            const int n = 16 * rb + 4 * (lane >> 4) + r, m = 32 * ch + 16 * t + (lane & 15);
+            const float2 P = Pl[n * kLS + m];
+            const double g = -((double)P.y * c.re[t][r] - (double)P.x * c.im[t][r]);
+            gx += g * (-kx[m]) * rc.pix;
+            gy += g * (-kx[n]) * rc.pix;
+            gz += g * B[n * kN + m] * 1e-9;
+            gmap[t][r] += g;
          }
-      gx = block_sum(gx, red);
-      gy = block_sum(gy, red);
-      gz = block_sum(gz, red);
-      gN = block_sum(gN, red);
-      gb = block_sum(gb, red);
-      nll = block_sum(nll, red);
-      double* part = partial + (size_t)r * kPartials;
-#pragma unroll 1
-      for (int z = 0; z < 28; ++z)
+      const double vals[6] = {gx, gy, gz, gN, gb, nll};
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
       {
-         const double s = block_sum(gzern[z], red);
-         if (tid == 0)
-            part[z] = s;
+         const double s = wave_sum(vals[k]);
+         if (lane == 0)
+            red[w][k] = s;
       }
-      if (tid == 0)
+      __syncthreads();
+      if (tid < 6)
       {
-         part[28] = nll;
-         double* gp = gradParams + (size_t)r * 5;
-         gp[0] = gx;
-         gp[1] = gy;
-         gp[2] = gz;
-         gp[3] = gN;
-         gp[4] = gb;
+         double s = 0.0;
+         for (int k = 0; k < 8; ++k)
+            s += red[k][tid];
+         if (tid < 5)
+            gradParams[(size_t)roi * 5 + tid] = s;
+         else
+            nllWg += s;
       }
       __syncthreads();
    }
+   if (BACKWARD)
+   {
+      double* map = wgMaps + (size_t)blockIdx.x * kMapStride;
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+         for (int r = 0; r < 4; ++r)
+            map[(16 * rb + 4 * (lane >> 4) + r) * kN + 32 * ch + 16 * t + (lane & 15)] = gmap[t][r];
+      if (tid == 5)
+         map[kN * kN] = nllWg;
+   }
 }
 
-// Sum of the per-ROI partials in ROI order (one lane per quantity).
-__global__ void roi_reduce_kernel(const double* __restrict__ partial, int nRoi, double* __restrict__ out)
+// The workgroups' maps and NLLs summed in workgroup order (one lane per map element, and the NLL).
+__global__ void roi_map_reduce_kernel(const double* __restrict__ wgMaps, int nWg, double* __restrict__ total)
 {
-   const int q = threadIdx.x;
-   if (q >= kPartials)
+   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+   if (idx >= kMapStride)
       return;
    double s = 0.0;
-   for (int r = 0; r < nRoi; ++r)
-      s += partial[(size_t)r * kPartials + q];
-   out[q] = s;
+   for (int g = 0; g < nWg; ++g)
+      s += wgMaps[(size_t)g * kMapStride + idx];
+   total[idx] = s;
+}
+
+// dNLL/dc_z = 2 pi sum_idx map[idx] Zb[z][idx] (one workgroup per OSA slot, fixed-order tree), and the NLL.
+__global__ __launch_bounds__(256) void roi_zernike_kernel(const double* __restrict__ total, const float* __restrict__ Zb,
+                                                          double* __restrict__ out)
+{
+   __shared__ double part[256];
+   const int z = blockIdx.x, tid = threadIdx.x;
+   double s = 0.0;
+   for (int idx = tid; idx < kN * kN; idx += 256)
+      s += total[idx] * (double)Zb[(size_t)z * kN * kN + idx];
+   part[tid] = s;
+   __syncthreads();
+   for (int o = 128; o > 0; o >>= 1)
+   {
+      if (tid < o)
+         part[tid] += part[tid + o];
+      __syncthreads();
+   }
+   if (tid == 0)
+   {
+      out[z] = 2.0 * kPiR * part[0];
+      if (z == 0)
+         out[28] = total[kN * kN];
+   }
 }
 
 static int32_t prepare(iscg_context* h, const iscg_roi_model* m, const double* zernike, RoiConst& rc)
@@ -391,8 +431,7 @@ static int32_t prepare(iscg_context* h, const iscg_roi_model* m, const double* z
       if (!ok(h, s.A.reserve(kN * kN * sizeof(double)), "alloc") ||
           !ok(h, s.B.reserve(kN * kN * sizeof(double)), "alloc") ||
           !ok(h, s.Zb.reserve((size_t)28 * kN * kN * sizeof(float)), "alloc") ||
-          !ok(h, s.kx.reserve(kN * sizeof(double)), "alloc") || !ok(h, s.W.reserve(kN * kN * sizeof(float2)), "alloc") ||
-          !ok(h, s.WT.reserve(kN * kN * sizeof(float2)), "alloc"))
+          !ok(h, s.kx.reserve(kN * sizeof(double)), "alloc") || !ok(h, s.W.reserve(kN * kN * sizeof(float2)), "alloc"))
          return -1;
       DevBuf dz;
       if (!ok(h, dz.reserve(28 * sizeof(double)), "alloc") ||
@@ -402,7 +441,7 @@ static int32_t prepare(iscg_context* h, const iscg_roi_model* m, const double* z
          ProfScope ps(h, "roi_tables");
          hipLaunchKernelGGL(roi_tables_kernel, dim3(16), dim3(256), 0, h->stream, M, R * os, os, k0, NA, ni, ns, bMax,
                             kMax, dk, kMin, depth, pix, dz.as<const double>(), s.A.as<double>(), s.B.as<double>(),
-                            s.Zb.as<float>(), s.kx.as<double>(), s.W.as<float2>(), s.WT.as<float2>());
+                            s.Zb.as<float>(), s.kx.as<double>(), s.W.as<float2>());
       }
       if (!ok(h, hipGetLastError(), "roi tables") || !ok(h, hipStreamSynchronize(h->stream), "synchronize"))
          return -1;
@@ -461,10 +500,10 @@ int32_t iscg_roi_forward(iscg_context* h, const iscg_roi_model* model, const dou
       return -1;
    {
       ProfScope ps(h, "roi_forward");
-      hipLaunchKernelGGL((roi_kernel<false>), dim3((unsigned)std::min(nRoi, 256 * 8)), dim3(256), 0, h->stream, rc, dP,
-                         nRoi, s.A.as<const double>(), s.B.as<const double>(), s.Zb.as<const float>(),
-                         s.kx.as<const double>(), s.W.as<const float2>(), s.WT.as<const float2>(), (const float*)nullptr,
-                         dMu, (double*)nullptr, (double*)nullptr);
+      hipLaunchKernelGGL((roi_kernel<false>), dim3((unsigned)std::min(nRoi, kRoiGrid)), dim3(kRoiThreads), 0,
+                         h->stream, rc, dP, nRoi, s.A.as<const double>(), s.B.as<const double>(),
+                         s.kx.as<const double>(), s.W.as<const float2>(), (const float*)nullptr, dMu, (double*)nullptr,
+                         (double*)nullptr);
    }
    if (!ok(h, hipGetLastError(), "roi forward"))
       return -1;
@@ -493,16 +532,21 @@ int32_t iscg_roi_backward(iscg_context* h, const iscg_roi_model* model, const do
    const iscg_roi_params* dP = (const iscg_roi_params*)stage_in(h, 0, params, nRoi * sizeof(iscg_roi_params));
    const float* dD = (const float*)stage_in(h, 1, data, npx * sizeof(float));
    double* dG = (double*)stage_out(h, 2, gradParams, (size_t)nRoi * 5 * sizeof(double));
-   if (!dP || !dD || !dG || !ok(h, s.partial.reserve(((size_t)nRoi + 1) * kPartials * sizeof(double)), "alloc"))
+   const int nWg = std::min(nRoi, kRoiGrid);
+   const size_t mapDoubles = (size_t)(nWg + 1) * kMapStride;
+   if (!dP || !dD || !dG || !ok(h, s.partial.reserve((mapDoubles + kPartials) * sizeof(double)), "alloc"))
       return -1;
-   double* sums = s.partial.as<double>() + (size_t)nRoi * kPartials;
+   double* total = s.partial.as<double>() + (size_t)nWg * kMapStride;
+   double* sums = s.partial.as<double>() + mapDoubles;
    {
       ProfScope ps(h, "roi_backward");
-      hipLaunchKernelGGL((roi_kernel<true>), dim3((unsigned)std::min(nRoi, 256 * 8)), dim3(256), 0, h->stream, rc, dP,
-                         nRoi, s.A.as<const double>(), s.B.as<const double>(), s.Zb.as<const float>(),
-                         s.kx.as<const double>(), s.W.as<const float2>(), s.WT.as<const float2>(), dD, (float*)nullptr,
-                         dG, s.partial.as<double>());
-      hipLaunchKernelGGL(roi_reduce_kernel, dim3(1), dim3(64), 0, h->stream, s.partial.as<const double>(), nRoi, sums);
+      hipLaunchKernelGGL((roi_kernel<true>), dim3((unsigned)nWg), dim3(kRoiThreads), 0, h->stream, rc, dP, nRoi,
+                         s.A.as<const double>(), s.B.as<const double>(), s.kx.as<const double>(),
+                         s.W.as<const float2>(), dD, (float*)nullptr, dG, s.partial.as<double>());
+      hipLaunchKernelGGL(roi_map_reduce_kernel, dim3((kMapStride + 255) / 256), dim3(256), 0, h->stream,
+                         s.partial.as<const double>(), nWg, total);
+      hipLaunchKernelGGL(roi_zernike_kernel, dim3(28), dim3(256), 0, h->stream, (const double*)total,
+                         s.Zb.as<const float>(), sums);
    }
    if (!ok(h, hipGetLastError(), "roi backward"))
       return -1;
