@@ -447,6 +447,39 @@ def cpu_c5(z, prm, data):
             "sample": f"all {len(p)} ROIs ({el:.2f} s, FP64, one thread; the reference has no adjoint)"}
 
 
+def measure_peaks():
+    """The rates this box attains, beside the datasheet peaks the rooflines use: a 1 GiB device-to-device copy
+    (read + write bytes / time) and square GEMMs through torch (hipBLASLt / rocBLAS) in FP32, FP16 and BF16."""
+    import torch
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps * 1e-3
+
+    out = {"metric": "attainable peaks", "datasheet": {"hbm_GB_s": HBM_PEAK_GBS, "fp32_TFLOP_s": FP32_PEAK_TFS,
+                                                       "fp16_dense_TFLOP_s": 2500.0}}
+    n = 1 << 28  # 1 GiB of float32
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    t = timed(lambda: dst.copy_(src), 20)
+    out["copy_GB_s"] = round(2 * n * 4 / t / 1e9, 1)
+    del src, dst
+    for name, dt, m in (("fp32", torch.float32, 8192), ("fp16", torch.float16, 16384), ("bf16", torch.bfloat16, 16384)):
+        a = torch.randn(m, m, device="cuda").to(dt)
+        b = torch.randn(m, m, device="cuda").to(dt)
+        t = timed(lambda: torch.matmul(a, b), 10)
+        out[f"gemm_{name}_TFLOP_s"] = round(2 * m ** 3 / t / 1e12, 1)
+        del a, b
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -457,6 +490,8 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a kernel of known traffic for PMC byte counters)")
+    ap.add_argument("--peaks", action="store_true",
+                    help="only measure the box's attainable rates (device copy, FP32/FP16/BF16 GEMM) and print them")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the image-formation path has no multi-GPU mode in this build "
@@ -468,6 +503,9 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU", file=sys.stderr)
         sys.exit(1)
+    if args.peaks:
+        print(json.dumps(measure_peaks()))
+        return
     ctx = host.Context(0)
     if args.calibrate_copy:
         n = args.calibrate_copy << 18  # float32 elements
