@@ -320,6 +320,174 @@ __global__ __launch_bounds__(kPsfThreads) void psf_planes_kernel(const PsfReqDev
    }
 }
 
+// Small planes (padded size Pp <= 80), E = W (Pup W^T) with E in registers. Per 32-row chunk n0 of the pupil: the
+// split pupil rows Pc (32 x MP) in LDS; mid[n][x] = sum_m Pc[n][m] W[x][m] (32 x Pp), stored split and transposed
+// (midT[x][n]); E[y][x] += sum_n W[y][n0 + n] mid[n][x].
+// WLDS (MP <= 128): kZGroups workgroups per request, each walking a contiguous quarter of its planes with the split W
+// (4 x 80 x (MP + 8) halves) staged in LDS once: the products read no global memory. The workgroups of a request
+// are dispatched to one XCD (workgroup b runs on XCD b mod 8), so the request's FP64 pupil tables, read again for
+// every plane, stay in that XCD's L2 (8 requests x 256 KB at a time at M 128). Otherwise one workgroup per
+// (request, plane), W from global memory, LDS 4 x 32 x (MP + 8) + 4 x 80 x 40 halves (two workgroups per CU).
+constexpr int kSmallPp = 80, kZGroups = 4, kXcds = 8;
+template <int MP, bool WLDS>
+__global__ __launch_bounds__(kPsfThreads, WLDS ? 2 : 4) void psf_small_kernel(const PsfReqDev* __restrict__ reqs,
+                                                                             const double* __restrict__ A,
+                                                                             const double* __restrict__ B,
+                                                                             const _Float16* __restrict__ W,
+                                                                             const double* __restrict__ tz, int nz,
+                                                                             int Pp, int nReq, float* __restrict__ out)
+{
+   constexpr int PS = MP + 8;                                    // pupil / W row (halves)
+   constexpr int CTX = kSmallPp / 16;                            // max tiles per side
+   constexpr int PE = (CTX * CTX + kPsfWaves - 1) / kPsfWaves;   // E tiles per wave (max)
+   constexpr int PM = (2 * CTX + kPsfWaves - 1) / kPsfWaves;     // mid tiles per wave (max)
+   __shared__ __attribute__((aligned(16))) _Float16 pc[4][32][PS];
+   __shared__ __attribute__((aligned(16))) _Float16 midT[4][kSmallPp][kKP];
+   __shared__ __attribute__((aligned(16))) _Float16 wl[WLDS ? 4 : 1][WLDS ? kSmallPp : 1][WLDS ? PS : 8];
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int b = (int)blockIdx.x, slot = b / kXcds;
+   const int r = WLDS ? (slot / kZGroups) * kXcds + b % kXcds : b / nz;
+   if (r >= nReq)
+      return;  // the last XCD row's spare workgroups
+   const PsfReqDev& q = reqs[r];
+   const int P = q.size;
+   const int CT = Pp / 16, TE = CT * CT, TM = 2 * CT;
+   const size_t wPer = (size_t)Pp * MP;
+   const _Float16* Wr = W + (size_t)r * 4 * wPer;
+   const double* Ar = A + (size_t)r * MP * MP;
+   const double* Br = B + (size_t)r * MP * MP;
+   const int lr = lane & 15, lk = 8 * (lane >> 4);
+   // operand (part, row, column) of W: row x < Pp, 8 halves from column c
+   auto wld = [&](int part, int row, int col) -> half8 {
+      if constexpr (WLDS)
+         return ld8(&wl[part][row][col]);
+      else
+         return ld8(Wr + part * wPer + (size_t)row * MP + col);
+   };
+   if constexpr (WLDS)
+   {
+      for (int i = tid; i < 4 * Pp * (MP / 8); i += kPsfThreads)
+      {
+         const int part = i / (Pp * (MP / 8)), rem = i % (Pp * (MP / 8)), row = rem / (MP / 8), c8 = rem % (MP / 8);
+         *(half8*)&wl[part][row][8 * c8] = ld8(Wr + part * wPer + (size_t)row * MP + 8 * c8);
+      }
+   }
+   const int g = slot % kZGroups;
+   const int z0 = WLDS ? g * nz / kZGroups : b % nz, z1 = WLDS ? (g + 1) * nz / kZGroups : z0 + 1;
+   for (int zi = z0; zi < z1; ++zi)
+   {
+      const double t = tz[(size_t)r * nz + zi];
+      float4v eR[PE], eI[PE];
+#pragma unroll
+      for (int s = 0; s < PE; ++s)
+         eR[s] = eI[s] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int n0 = 0; n0 < MP; n0 += 32)
+      {
+         // the pupil rows [n0, n0 + 32): FP64 phase, wrapped, FP32 sincos, split; 8 consecutive samples per lane.
+         // (The barrier below also orders the previous chunk's reads of pc and midT before these writes.)
+         for (int i = tid; i < 32 * (MP / 8); i += kPsfThreads)
+         {
+            const int k = i / (MP / 8), m0 = (i % (MP / 8)) * 8;
+            half8 chi, clo, shi, slo;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+            {
+               const size_t o = (size_t)(n0 + k) * MP + m0 + j;
+               const double a = Ar[o];
+               float s = 0.0f, c = 0.0f;
+               if (a == a)  // inside the aperture
+                  wrapped_sincos(a + t * Br[o], &s, &c);
+               const _Float16 ch = (_Float16)c, sh = (_Float16)s;
+               chi[j] = ch;
+               clo[j] = (_Float16)(c - (float)ch);
+               shi[j] = sh;
+               slo[j] = (_Float16)(s - (float)sh);
+            }
+            *(half8*)&pc[0][k][m0] = chi;
+            *(half8*)&pc[1][k][m0] = clo;
+            *(half8*)&pc[2][k][m0] = shi;
+            *(half8*)&pc[3][k][m0] = slo;
+         }
+         __syncthreads();
+         // mid tiles: rows 16 (tm / CT) .., columns 16 (tm % CT) ..
+#pragma unroll
+         for (int s = 0; s < PM; ++s)
+         {
+            const int tm = wave + kPsfWaves * s;
+            if (tm >= TM)
+               continue;
+            const int rb = tm / CT, cb = tm % CT;
+            float4v mR = {0.0f, 0.0f, 0.0f, 0.0f}, mI = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+            for (int k0 = 0; k0 < MP; k0 += 32)
+            {
+               const int row = 16 * rb + lr, wrow = 16 * cb + lr;
+               const half8 arh = ld8(&pc[0][row][k0 + lk]), arl = ld8(&pc[1][row][k0 + lk]);
+               const half8 aih = ld8(&pc[2][row][k0 + lk]), ail = ld8(&pc[3][row][k0 + lk]);
+               const half8 brh = wld(0, wrow, k0 + lk), brl = wld(1, wrow, k0 + lk);
+               const half8 bih = wld(2, wrow, k0 + lk), bil = wld(3, wrow, k0 + lk);
+               mR = mfma3(arh, arl, brh, brl, mR);
+               mR = mfma3(-aih, -ail, bih, bil, mR);
+               mI = mfma3(arh, arl, bih, bil, mI);
+               mI = mfma3(aih, ail, brh, brl, mI);
+            }
+            // D rows n = 16 rb + 4 (lane >> 4) + i (four consecutive), column x = 16 cb + lr -> midT[.][x][n..n+3]
+            const int x = 16 * cb + lr, n = 16 * rb + 4 * (lane >> 4);
+            typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+            half4 rh, rl, ih, il;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+            {
+               const _Float16 a = (_Float16)mR[i], b = (_Float16)mI[i];
+               rh[i] = a;
+               rl[i] = (_Float16)(mR[i] - (float)a);
+               ih[i] = b;
+               il[i] = (_Float16)(mI[i] - (float)b);
+            }
+            *(half4*)&midT[0][x][n] = rh;
+            *(half4*)&midT[1][x][n] = rl;
+            *(half4*)&midT[2][x][n] = ih;
+            *(half4*)&midT[3][x][n] = il;
+         }
+         __syncthreads();
+         // E tiles += W[y][n0 + n] mid[n][x] (K = 32)
+#pragma unroll
+         for (int s = 0; s < PE; ++s)
+         {
+            const int te = wave + kPsfWaves * s;
+            if (te >= TE)
+               continue;
+            const int ry = te / CT, cx = te % CT, wrow = 16 * ry + lr;
+            const half8 arh = wld(0, wrow, n0 + lk), arl = wld(1, wrow, n0 + lk);
+            const half8 aih = wld(2, wrow, n0 + lk), ail = wld(3, wrow, n0 + lk);
+            const int x = 16 * cx + lr;
+            const half8 brh = ld8(&midT[0][x][lk]), brl = ld8(&midT[1][x][lk]);
+            const half8 bih = ld8(&midT[2][x][lk]), bil = ld8(&midT[3][x][lk]);
+            eR[s] = mfma3(arh, arl, brh, brl, eR[s]);
+            eR[s] = mfma3(-aih, -ail, bih, bil, eR[s]);
+            eI[s] = mfma3(arh, arl, bih, bil, eI[s]);
+            eI[s] = mfma3(aih, ail, brh, brl, eI[s]);
+         }
+      }
+      float* plane = out + ((size_t)r * nz + zi) * P * P;
+#pragma unroll
+      for (int s = 0; s < PE; ++s)
+      {
+         const int te = wave + kPsfWaves * s;
+         if (te >= TE)
+            continue;
+         const int ry = te / CT, cx = te % CT, x = 16 * cx + lr;
+#pragma unroll
+         for (int i = 0; i < 4; ++i)
+         {
+            const int y = 16 * ry + 4 * (lane >> 4) + i;
+            if (y < P && x < P)
+               plane[(size_t)y * P + x] = eR[s][i] * eR[s][i] + eI[s][i] * eI[s][i];
+         }
+      }
+   }
+}
+
 // Sum-1 normalisation per plane (Sim/ZernikePsf.cpp:401-410): FP64 sum, v = float(v / sum).
 __global__ __launch_bounds__(256) void psf_normalize_kernel(float* __restrict__ planes, int n2)
 {
@@ -454,6 +622,19 @@ static bool psf_planes_launch(iscg_context* h, int nReq, int nz, int size, int M
    PsfState& s = *h->psf;
    auto go = [&](auto mp, auto yb) {
       constexpr int MP = decltype(mp)::value, YB = decltype(yb)::value;
+      if (Pp <= kSmallPp)
+      {
+         if constexpr (MP <= 128)
+            hipLaunchKernelGGL((psf_small_kernel<MP, true>), dim3((nReq + kXcds - 1) / kXcds * kXcds * kZGroups),
+                               dim3(kPsfThreads), 0, h->stream, s.reqs.as<const PsfReqDev>(),
+                               s.tabA.as<const double>(), s.tabB.as<const double>(), s.W.as<const _Float16>(),
+                               s.tz.as<const double>(), nz, Pp, nReq, out);
+         else
+            hipLaunchKernelGGL((psf_small_kernel<MP, false>), dim3(nReq * nz), dim3(kPsfThreads), 0, h->stream,
+                               s.reqs.as<const PsfReqDev>(), s.tabA.as<const double>(), s.tabB.as<const double>(),
+                               s.W.as<const _Float16>(), s.tz.as<const double>(), nz, Pp, nReq, out);
+         return;
+      }
       const int nYB = (Pp + YB * 16 - 1) / (YB * 16);
       launch_planes<MP, YB>(h->stream, nReq * nz * nYB, s.reqs.as<const PsfReqDev>(), s.tabA.as<const double>(),
                             s.tabB.as<const double>(), s.W.as<const _Float16>(), s.tz.as<const double>(), nz, Pp, nYB,
