@@ -187,6 +187,23 @@ int32_t iscg_splat_render_records(iscg_context* h, const iscg_splat_emitter* ems
                                   const float* bgScales, const float* extras, const iscg_camera* cam, uint32_t seed,
                                   uint16_t* outAdu, float* outPhotons);
 
+/* ---- the binned blink regime (sim::BinnedBlinkRenderer, Sim/BinnedBlinks.h:37-52) ---------------------------------- */
+
+/* Replaces BinnedBlinkRenderer::Setup (Sim/BinnedBlinks.h:43; .cpp:94-163). kernel: the blink kernel's planes (NULL:
+ * the in-focus Gaussian of sigmaPx); haloRadius2: the largest radius2 of its halo cut (PsfKernelCache::halo), -1 for
+ * none; upscale: requested cells per camera pixel (made a divisor of the oversampling). The grid (W u + 2R) x
+ * (H u + 2R) must fit a 1024 x 1024 FFT. Per-plane kernel spectra are made here and kept: a context holds one binned
+ * renderer (a second one takes a second context). */
+int32_t iscg_binned_setup(iscg_context* h, const iscg_kernel_view* kernel, int32_t haloRadius2, double sigmaPx,
+                          uint32_t width, uint32_t height, int32_t upscale);
+/* BinnedBlinkRenderer::Upscale / RadiusCells (Sim/BinnedBlinks.h:45-46) of the last setup. */
+int32_t iscg_binned_geometry(iscg_context* h, int32_t* upscale, int32_t* radiusCells);
+/* Replaces BinnedBlinkRenderer::Render (Sim/BinnedBlinks.h:52; .cpp:165-246) for nFrames frames at once: frame f's
+ * emitters are em[frameStart[f] .. frameStart[f+1]) (xPx, yPx, photons, zIndex of sim::FrameEmitter; kernel
+ * ignored), in host memory; img (nFrames x height x width, host or device) is added to. */
+int32_t iscg_binned_render(iscg_context* h, const iscg_frame_emitter* em, const int32_t* frameStart, int32_t nFrames,
+                           float* img);
+
 /* ---- PSF planes ------------------------------------------------------------------------------------------------ */
 
 /* The numeric fields of sim::PsfGeneratorRequest (Sim/PsfGeneratorBridge.h:91-170) for the GibsonLanniZernike model;

@@ -94,6 +94,9 @@ SIGNATURES = {
     "iscg_profile_enable": (I32, [P, I32]),
     "iscg_profile_read": (I32, [P, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), I32, I32]),
     "iscg_set_stream": (I32, [P, P]),
+    "iscg_binned_setup": (I32, [P, P, I32, D, U32, U32, I32]),
+    "iscg_binned_geometry": (I32, [P, P, P]),
+    "iscg_binned_render": (I32, [P, P, P, I32, P]),
     "iscg_pcg4d": (I32, [P, P, P, P, P, U32, P]),
     "iscg_hash_units": (I32, [P, U32, P, P, P, U32, P]),
     "iscg_hash_stream": (I32, [P, U32, I32, I32, U32, U32, P]),

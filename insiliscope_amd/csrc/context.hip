@@ -10,6 +10,7 @@ void destroy_psf(iscg_context* h);
 void destroy_wf(iscg_context* h);
 void destroy_roi(iscg_context* h);
 void destroy_wf_accel(iscg_context* h);
+void destroy_binned(iscg_context* h);
 
 int32_t fail(iscg_context* h, const std::string& msg)
 {
@@ -193,6 +194,7 @@ void iscg_destroy(iscg_context* h)
    destroy_wf(h);
    destroy_wf_accel(h);
    destroy_roi(h);
+   destroy_binned(h);
    for (auto& b : h->stage)
       b.release();
    for (auto& p : h->pending)

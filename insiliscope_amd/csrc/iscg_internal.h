@@ -60,6 +60,7 @@ struct PsfState;
 struct WfState;
 struct WfAccelState;
 struct RoiState;
+struct BinnedState;
 
 }  // namespace iscg
 
@@ -88,6 +89,7 @@ struct iscg_context
    iscg::WfState* wf = nullptr;
    iscg::WfAccelState* wfAccel = nullptr;
    iscg::RoiState* roi = nullptr;
+   iscg::BinnedState* binned = nullptr;
    int noiseExact = 0;  // iscg_noise_set_path: 1 evaluates every pixel with the FP64 chain
    iscg::DevBuf noiseItems;            // deferred pixels (NoiseQueue::items)
    uint32_t* noiseCount = nullptr;     // NoiseQueue::count, 2 words

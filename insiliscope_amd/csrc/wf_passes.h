@@ -60,15 +60,18 @@ __global__ __launch_bounds__(256) void rows_r2c_kernel(const float* __restrict__
    }
 }
 
-// Passes B and D: FFT along the rows of [planes][rows][S] for blocks of 16 columns; rows >= rowsIn are zero,
-// rows [row0, row0 + rowsOut) are written.
-template <int N, bool INV>
-__global__ __launch_bounds__(kCols * N / 8) void cols_kernel(const float2* __restrict__ in, int rowsIn,
-                                                             size_t inPlaneStride, float2* __restrict__ out,
-                                                             int row0, int rowsOut, size_t outPlaneStride,
-                                                             int nPlanes, int S, int nColsValid,
-                                                             const float2* __restrict__ tw)
+// Passes B and D: FFT along the rows of [planes][rows][S] for blocks of KC columns (16; 8 at N = 1024, so that a
+// block stays within 1024 lanes); rows >= rowsIn are zero, rows [row0, row0 + rowsOut) are written.
+template <int N>
+constexpr int cols_block() { return N > 512 ? 8 : kCols; }
+template <int N, bool INV, int KC = cols_block<N>()>
+__global__ __launch_bounds__(KC * N / 8) void cols_kernel(const float2* __restrict__ in, int rowsIn,
+                                                          size_t inPlaneStride, float2* __restrict__ out,
+                                                          int row0, int rowsOut, size_t outPlaneStride,
+                                                          int nPlanes, int S, int nColsValid,
+                                                          const float2* __restrict__ tw)
 {
+   constexpr int kCols = KC;
    constexpr int CS = kCols + 1;  // padded row of the block in LDS
    __shared__ float2 buf[N * CS];
    __shared__ float2 twl[N];  // the twiddles, read by every stage
@@ -318,13 +321,14 @@ struct ColsF
    static void run(hipStream_t s, bool inv, const float2* in, int rowsIn, size_t inPS, float2* out, int row0,
                    int rowsOut, size_t outPS, int nPlanes, int S, int nCols, const float2* tw)
    {
-      const size_t items = (size_t)nPlanes * ((nCols + kCols - 1) / kCols);
+      constexpr int KC = cols_block<N>();
+      const size_t items = (size_t)nPlanes * ((nCols + KC - 1) / KC);
       if (inv)
-         hipLaunchKernelGGL((cols_kernel<N, true>), dim3(wgs(items)), dim3(kCols * N / 8), 0, s, in, rowsIn, inPS,
-                            out, row0, rowsOut, outPS, nPlanes, S, nCols, tw);
+         hipLaunchKernelGGL((cols_kernel<N, true>), dim3(wgs(items)), dim3(KC * N / 8), 0, s, in, rowsIn, inPS, out,
+                            row0, rowsOut, outPS, nPlanes, S, nCols, tw);
       else
-         hipLaunchKernelGGL((cols_kernel<N, false>), dim3(wgs(items)), dim3(kCols * N / 8), 0, s, in, rowsIn, inPS,
-                            out, row0, rowsOut, outPS, nPlanes, S, nCols, tw);
+         hipLaunchKernelGGL((cols_kernel<N, false>), dim3(wgs(items)), dim3(KC * N / 8), 0, s, in, rowsIn, inPS, out,
+                            row0, rowsOut, outPS, nPlanes, S, nCols, tw);
    }
 };
 
@@ -419,7 +423,7 @@ static bool forward_xy(iscg_context* h, const XyGrid& g, const float* in, int nz
          return false;
    }
    ProfScope ps(h, "wf_cols_fwd");
-   return dispatch_n<ColsF>((int)g.NY, h->stream, false, (const float2*)x1, nyIn, (size_t)nyIn * S, x2, 0,
+   return dispatch_n<ColsF, 1024>((int)g.NY, h->stream, false, (const float2*)x1, nyIn, (size_t)nyIn * S, x2, 0,
                             (int)g.NY, outPlaneStride, nzIn, S, (int)(g.NX / 2 + 1), g.twY);
 }
 
@@ -431,7 +435,7 @@ static bool inverse_xy(iscg_context* h, const XyGrid& g, const float2* in, int n
    const int S = (int)g.S;
    {
       ProfScope ps(h, "wf_cols_inv");
-      if (!dispatch_n<ColsF>((int)g.NY, h->stream, true, in, (int)g.NY, (size_t)g.NY * S, x2, fovY0, e.ch,
+      if (!dispatch_n<ColsF, 1024>((int)g.NY, h->stream, true, in, (int)g.NY, (size_t)g.NY * S, x2, fovY0, e.ch,
                              (size_t)e.ch * S, nPlanes, S, (int)(g.NX / 2 + 1), g.twY))
          return false;
    }

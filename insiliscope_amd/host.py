@@ -568,6 +568,83 @@ def BuildZernikePsfKernelCache(fields, ctx: Context | None = None) -> PsfKernelC
 
 # ---- mean field --------------------------------------------------------------------------------------------------
 
+class BinnedBlinkRenderer:
+    """sim::BinnedBlinkRenderer (Sim/BinnedBlinks.h:37-62): the binned blink regime on the GPU. Setup makes the
+    per-plane kernel spectra once; Render / RenderFrames add the emitters' photon images (FFT-convolved per PSF
+    plane on a grid of u cells per pixel, max(0, .), binned) to the frames."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.err = ""
+        self.size = None
+
+    def _call(self, rc: int) -> bool:
+        self.err = "" if rc == 0 else self.ctx.error()
+        return rc == 0
+
+    def Setup(self, kernel: PsfKernelCache | None, sigmaPx: float, width: int, height: int, upscale: int) -> bool:
+        """BinnedBlinkRenderer::Setup (Sim/BinnedBlinks.cpp:94): kernel None (or without planes) = the in-focus
+        Gaussian of sigmaPx."""
+        view, keep = None, None
+        r2 = -1
+        if kernel is not None and kernel.planes is not None:
+            pl, keep = _ptr(_arr(kernel.planes, np.float32))
+            view = _lib.KernelView(pl, None, None, kernel.zStepNm, kernel.serial, kernel.nz, kernel.sizeOversampled,
+                                   kernel.oversampling, kernel.interpMode)
+            if kernel.halo is not None and len(kernel.halo.radius2):
+                r2 = int(np.max(kernel.halo.radius2))
+        self.ctx.follow_torch()
+        ok = self._call(self.ctx.L.iscg_binned_setup(self.ctx.h, C.byref(view) if view is not None else None, r2,
+                                                      float(sigmaPx), width, height, upscale))
+        del keep
+        self.size = (width, height) if ok else None
+        return ok
+
+    def Valid(self) -> bool:
+        return self.size is not None
+
+    def _geometry(self):
+        u, R = C.c_int32(0), C.c_int32(0)
+        if self.ctx.L.iscg_binned_geometry(self.ctx.h, C.byref(u), C.byref(R)) != 0:
+            return 1, 1
+        return u.value, R.value
+
+    def Upscale(self) -> int:
+        return self._geometry()[0]
+
+    def RadiusCells(self) -> int:
+        return self._geometry()[1]
+
+    def RenderFrames(self, frames, imgs) -> bool:
+        """frames: per frame a FRAME_EMITTER_DTYPE array (event order); imgs: (nF, H, W) float32 numpy or device
+        tensor, added to."""
+        em = np.concatenate([np.asarray(f, FRAME_EMITTER_DTYPE) for f in frames]) if len(frames) else \
+            np.zeros(0, FRAME_EMITTER_DTYPE)
+        start = np.zeros(len(frames) + 1, np.int32)
+        start[1:] = np.cumsum([len(f) for f in frames])
+        em = np.ascontiguousarray(em)
+        out, keep = _ptr(imgs)
+        self.ctx.follow_torch()
+        rc = self.ctx.L.iscg_binned_render(self.ctx.h, em.ctypes.data if em.size else None, start.ctypes.data,
+                                           len(frames), out)
+        del keep
+        return self._call(rc)
+
+    def Render(self, emitters, img, alone=None) -> bool:
+        """BinnedBlinkRenderer::Render (Sim/BinnedBlinks.cpp:165): one frame; img (H x W, float32 numpy, added to
+        in place) and alone (optional, the same values added)."""
+        if not isinstance(img, np.ndarray) or img.dtype != np.float32 or not img.flags["C_CONTIGUOUS"]:
+            raise TypeError("img: a C-contiguous float32 numpy array")
+        if alone is None:
+            return self.RenderFrames([emitters], img.reshape(1, *img.shape))
+        add = np.zeros_like(img)  # 0 + (float) sum: the very values the reference adds to img and to alone
+        if not self.RenderFrames([emitters], add.reshape(1, *add.shape)):
+            return False
+        img += add
+        alone += add
+        return True
+
+
 class WidefieldStack:
     """A focus series of a dye density as one 3D FFT convolution on the GPU: the per-focus work of
     sim::WidefieldScene (FocusSeries, Sim/WidefieldRender.h:588) for foci spaced by the dye-plane thickness, with

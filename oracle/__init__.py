@@ -145,6 +145,10 @@ class Oracle:
         L.orc_render_frame_fft.argtypes = [PF, U32, U32, D, PF, D, C.c_void_p, I32, PF, I32, I32, I32, PI16, PI16,
                                            D]
         L.orc_wf_images_render.argtypes = [PF, I32, I32, U32, U32, I32, PD, I32, PF]
+        PI32 = C.POINTER(C.c_int32)
+        L.orc_binned_geometry.argtypes = [I32, I32, I32, I32, D, I32, PI32, PI32]
+        L.orc_binned_gauss.argtypes = [D, I32, I32, PF]
+        L.orc_binned_render.argtypes = [PF, I32, I32, I32, I32, I32, PD, I32, PF]
         L.orc_wf_radius.argtypes = [I32, I32, I32]
         L.orc_wf_radius.restype = I32
         L.orc_wf_kernel.argtypes = [PF, I32, I32, I32, I32, PF]
@@ -285,6 +289,33 @@ class Oracle:
     def wf_radius(self, size, os_, upscale):
         return self.L.orc_wf_radius(size, os_, upscale)
 
+    def binned_geometry(self, planes, os_, upscale, sigma_px=0.0, halo_r2=-1):
+        """(u, R) of BinnedBlinkRenderer::Setup (Sim/BinnedBlinks.cpp:94-129); planes None: the Gaussian."""
+        u, R = C.c_int32(0), C.c_int32(0)
+        size = 0 if planes is None else np.asarray(planes).shape[-1]
+        self.L.orc_binned_geometry(int(planes is not None), size, os_, upscale, sigma_px, halo_r2, C.byref(u),
+                                   C.byref(R))
+        return u.value, R.value
+
+    def binned_kernels(self, planes, os_, upscale, R, sigma_px=0.0):
+        """The per-plane grid kernels nz x (2R+1)^2: KernelWidefieldPsf::Kernel, or the Gaussian (:146-160)."""
+        if planes is None:
+            u, _ = self.binned_geometry(None, 1, upscale, sigma_px)
+            out = np.zeros((1, 2 * R + 1, 2 * R + 1), np.float32)
+            self.L.orc_binned_gauss(sigma_px, u, R, _p(out, C.c_float))
+            return out
+        return np.stack([self.wf_kernel(p, os_, upscale, R) for p in planes])
+
+    def binned_render(self, kernels, u, R, W, H, emitters, img=None):
+        """BinnedBlinkRenderer::Render (Sim/BinnedBlinks.cpp:165-246) of one frame's emitters (n x 4: xPx, yPx,
+        photons, zIndex) added to img (H x W), as a direct FP64 convolution of the per-plane deposits."""
+        k = np.ascontiguousarray(kernels, np.float32)
+        em = np.ascontiguousarray(emitters, np.float64).reshape(-1, 4)
+        img = np.zeros((H, W), np.float32) if img is None else np.ascontiguousarray(img, np.float32).copy()
+        self.L.orc_binned_render(_p(k, C.c_float), k.shape[0], u, R, W, H, _p(em, C.c_double), em.shape[0],
+                                 _p(img, C.c_float))
+        return img
+
     def wf_kernel(self, plane, os_, upscale, R):
         plane = np.ascontiguousarray(plane, np.float32)
         out = np.zeros((2 * R + 1, 2 * R + 1), np.float32)
@@ -378,6 +409,8 @@ class Reference:
         L.ref_capped_psf_oversampling.restype = I32
         L.ref_wf_images_render.argtypes = [PF, I32, I32, U32, U32, I32, PD, I32, PF]
         L.ref_wf_images_render.restype = I32
+        L.ref_binned_render.argtypes = [C.c_void_p, I32, I32, I32, D, D, I32, I32, I32, PD, PI32, I32, PF, PI32]
+        L.ref_binned_render.restype = I32
 
     def psf_planes(self, req, size, nz):
         out = np.zeros((nz, size, size), np.float32)
@@ -525,6 +558,28 @@ class Reference:
         self.L.ref_wf_images_render(_p(ch_, C.c_float), nC, int(has_persistent), cw, chh, upscale,
                                     _p(a, C.c_double), len(a), _p(cam, C.c_float))
         return cam
+
+    def binned_render(self, planes, os_, upscale, W, H, frames, sigma_px=0.0, halo_threshold=0.0, img=None):
+        """BinnedBlinkRenderer Setup + Render per frame (frames: a list of n x 4 emitter arrays xPx, yPx, photons,
+        zIndex); planes None: the Gaussian of sigma_px. Returns (images nF x H x W added to img, (u, R))."""
+        em = np.ascontiguousarray(np.concatenate([np.asarray(f, np.float64).reshape(-1, 4) for f in frames])
+                                  if frames else np.zeros((0, 4)), np.float64)
+        start = np.zeros(len(frames) + 1, np.int32)
+        start[1:] = np.cumsum([np.asarray(f).reshape(-1, 4).shape[0] for f in frames])
+        out = (np.zeros((len(frames), H, W), np.float32) if img is None
+               else np.ascontiguousarray(img, np.float32).copy())
+        uR = np.zeros(2, np.int32)
+        pl = None
+        nz = size = 0
+        if planes is not None:
+            pl = np.ascontiguousarray(planes, np.float32)
+            nz, size = pl.shape[0], pl.shape[1]
+        rc = self.L.ref_binned_render(None if pl is None else pl.ctypes.data, nz, size, os_, halo_threshold,
+                                      sigma_px, W, H, upscale, _p(em, C.c_double), _p(start, C.c_int32), len(frames),
+                                      _p(out, C.c_float), _p(uR, C.c_int32))
+        if rc != 0:
+            raise RuntimeError("BinnedBlinkRenderer::Setup failed")
+        return out, (int(uR[0]), int(uR[1]))
 
     def noise_setup(self, seed, w, h, offset, offset_std, gain, gain_std, read, read_std, sensor_h=0, row_y0=0):
         maps = [np.zeros((h, w), np.float32) for _ in range(3)]

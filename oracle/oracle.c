@@ -1563,3 +1563,125 @@ void orc_wf_images_render(const float* channels, int32_t nCh, int32_t hasPersist
          cam[(size_t)Y * W + X] += (float)acc;
       }
 }
+
+/* ---- the binned blink regime (Sim/BinnedBlinks.cpp) ------------------------------------------------------------- */
+
+/* BinnedBlinkRenderer::Setup (:94-163): cells per pixel u and kernel radius R. planes null: the in-focus Gaussian
+ * (u = clamp(upscale, 1, 8), R = max(1, ceil(3 sigma u))); else KernelWidefieldPsf's u and Radius, cut to
+ * ceil((sqrt(haloR2) + 1) u) when haloR2 >= 0 (the halo's largest radius2). */
+void orc_binned_geometry(int32_t hasKernel, int32_t size, int32_t os, int32_t upscale, double sigmaPx, int32_t haloR2,
+                         int32_t* u, int32_t* R)
+{
+   if (hasKernel)
+   {
+      *u = valid_upscale(imax(1, os), upscale);
+      int r = orc_wf_radius(size, os, upscale);
+      if (haloR2 >= 0)
+      {
+         const int cut = (int)ceil((sqrt((double)haloR2) + 1.0) * *u);
+         r = imin(r, cut);
+      }
+      *R = imax(1, r);
+   }
+   else
+   {
+      *u = imax(1, imin(upscale, 8));
+      *R = imax(1, (int)ceil(3.0 * sigmaPx * *u));
+   }
+}
+
+/* The Gaussian kernel of :146-160: (2R+1)^2 cells exp(-(x^2 + y^2) / (2 (sigma u)^2)), normalised in FP64, then
+ * float. */
+void orc_binned_gauss(double sigmaPx, int32_t u, int32_t R, float* out)
+{
+   const int D = 2 * R + 1;
+   const double sc = sigmaPx * u, twoS2 = 2.0 * sc * sc;
+   double* k = (double*)malloc(sizeof(double) * (size_t)D * D);
+   double sum = 0.0;
+   for (int y = -R; y <= R; ++y)
+      for (int x = -R; x <= R; ++x)
+      {
+         const double v = exp(-((double)x * x + (double)y * y) / twoS2);
+         k[(size_t)(y + R) * D + (x + R)] = v;
+         sum += v;
+      }
+   for (int i = 0; i < D * D; ++i)
+      out[i] = (float)(k[i] / sum);
+   free(k);
+}
+
+/* BinnedBlinkRenderer::Render (:170-246) by the convolution theorem's other side: the per-plane deposits (cell
+ * floor((x + 0.5) u) + R, photons summed in float in event order, emitters with photons <= 0 or outside the
+ * (W u + 2R) x (H u + 2R) grid skipped, plane clamped to [0, nz - 1]) convolved directly with the plane kernels in
+ * FP64 over the cropped FOV cells [R, R + W u) x [R, R + H u); then max(0, (float) cell) summed in FP64 over u x u
+ * cells and added (float) to img. kernels: nz x (2R+1)^2 (orc_wf_kernel or orc_binned_gauss). em: n x 4 doubles
+ * (xPx, yPx, photons, zIndex). */
+void orc_binned_render(const float* kernels, int32_t nz, int32_t u, int32_t R, int32_t W, int32_t H, const double* em,
+                       int32_t n, float* img)
+{
+   const int gw = W * u + 2 * R, gh = H * u + 2 * R, D = 2 * R + 1;
+   const int cw = W * u, ch = H * u;
+   float* dens = (float*)calloc((size_t)gw * gh, sizeof(float));
+   double* cells = (double*)calloc((size_t)cw * ch, sizeof(double));
+   int any = 0;
+   for (int p = 0; p < nz; ++p)
+   {
+      int touched = 0;
+      for (int i = 0; i < n; ++i)
+      {
+         const double* e = em + 4 * (size_t)i;
+         if (!(e[2] > 0.0))
+            continue;
+         int pi = (int)e[3];
+         pi = imax(0, imin(nz - 1, pi));
+         if (pi != p)
+            continue;
+         const int cx = (int)floor((e[0] + 0.5) * u) + R;
+         const int cy = (int)floor((e[1] + 0.5) * u) + R;
+         if (cx < 0 || cy < 0 || cx >= gw || cy >= gh)
+            continue;
+         dens[(size_t)cy * gw + cx] += (float)e[2];
+         touched = 1;
+      }
+      if (!touched)
+         continue;
+      any = 1;
+      const float* K = kernels + (size_t)p * D * D;
+      for (int cy = 0; cy < gh; ++cy)
+         for (int cx = 0; cx < gw; ++cx)
+         {
+            const float d = dens[(size_t)cy * gw + cx];
+            if (d == 0.0f)
+               continue;
+            for (int dy = -R; dy <= R; ++dy)
+            {
+               const int y = cy + dy - R;
+               if (y < 0 || y >= ch)
+                  continue;
+               for (int dx = -R; dx <= R; ++dx)
+               {
+                  const int x = cx + dx - R;
+                  if (x < 0 || x >= cw)
+                     continue;
+                  cells[(size_t)y * cw + x] += (double)d * K[(size_t)(dy + R) * D + (dx + R)];
+               }
+            }
+            dens[(size_t)cy * gw + cx] = 0.0f;
+         }
+   }
+   if (any)
+      for (int y = 0; y < H; ++y)
+         for (int x = 0; x < W; ++x)
+         {
+            double sum = 0.0;
+            for (int dy = 0; dy < u; ++dy)
+               for (int dx = 0; dx < u; ++dx)
+               {
+                  const float c = (float)cells[(size_t)(y * u + dy) * cw + (size_t)x * u + dx];
+                  sum += c > 0.0f ? c : 0.0f;
+               }
+            img[(size_t)y * W + x] += (float)sum;
+         }
+   free(dens);
+   free(cells);
+}

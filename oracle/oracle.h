@@ -164,6 +164,13 @@ void orc_roi_forward(const orc_roi_model* m, const double* zernike, const orc_ro
 double orc_roi_backward(const orc_roi_model* m, const double* zernike, const orc_roi_params* p, int32_t nRoi,
                         const double* data, double* gradParams, double* gradZernike);
 
+/* the binned blink regime (Sim/BinnedBlinks.cpp) */
+void orc_binned_geometry(int32_t hasKernel, int32_t size, int32_t os, int32_t upscale, double sigmaPx, int32_t haloR2,
+                         int32_t* u, int32_t* R);
+void orc_binned_gauss(double sigmaPx, int32_t u, int32_t R, float* out);
+void orc_binned_render(const float* kernels, int32_t nz, int32_t u, int32_t R, int32_t W, int32_t H, const double* em,
+                       int32_t n, float* img);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,7 @@
 #include "SMLMSimulation.h"
 #include "SMLMZernike.h"
 #include "WidefieldRender.h"
+#include "BinnedBlinks.h"
 #include "ZernikePsf.h"
 #include "Illumination.h"
 #include "Parallel.h"
@@ -400,6 +401,46 @@ int32_t ref_wf_focus_series(const float* density, int32_t nx, int32_t ny, int32_
       std::memcpy(out + static_cast<size_t>(f) * nx * ny, im.persistent.data(), im.persistent.size() * sizeof(float));
    }
    return 1;
+}
+
+// BinnedBlinkRenderer (Sim/BinnedBlinks.cpp:94-246): Setup with the caller's planes (null: the Gaussian of sigmaPx),
+// cut by WithHaloCut at haloThreshold when > 0, then Render of each frame's emitters (em: n x 4 doubles xPx, yPx,
+// photons, zIndex; frame f holds [start[f], start[f+1])) onto img (nFrames x H x W, added to). uR: u and R out.
+int32_t ref_binned_render(const float* planes, int32_t nz, int32_t size, int32_t os, double haloThreshold,
+                          double sigmaPx, int32_t W, int32_t H, int32_t upscale, const double* em, const int32_t* start,
+                          int32_t nFrames, float* img, int32_t* uR)
+{
+   sim::PsfKernelCache c;
+   if (planes)
+   {
+      c = MakeCache(planes, nz, size, os, 100.0, 0);
+      if (haloThreshold > 0.0)
+         c = sim::WithHaloCut(c, haloThreshold);
+   }
+   sim::BinnedBlinkRenderer r;
+   if (!r.Setup(planes ? &c : nullptr, sigmaPx, static_cast<unsigned>(W), static_cast<unsigned>(H), upscale))
+      return -1;
+   uR[0] = r.Upscale();
+   uR[1] = r.RadiusCells();
+   const size_t npx = static_cast<size_t>(W) * H;
+   for (int f = 0; f < nFrames; ++f)
+   {
+      std::vector<sim::FrameEmitter> ems;
+      for (int i = start[f]; i < start[f + 1]; ++i)
+      {
+         sim::FrameEmitter e{};
+         e.xPx = em[4 * i];
+         e.yPx = em[4 * i + 1];
+         e.photons = em[4 * i + 2];
+         e.overlap = 1.0;
+         e.zIndex = static_cast<int>(em[4 * i + 3]);
+         ems.push_back(e);
+      }
+      std::vector<float> frame(img + f * npx, img + (f + 1) * npx);
+      r.Render(ems, frame);
+      std::memcpy(img + f * npx, frame.data(), npx * sizeof(float));
+   }
+   return 0;
 }
 
 } // extern "C"

@@ -17,6 +17,8 @@ Outputs (inputs and expected outputs only):
   maps_ref.npz       NoiseSetup's offset / gain / read-noise maps (sensor and window) and the kernel-geometry helpers
   wf_jobs_ref.npz    the WidefieldGpuJobs of GPU-mode scenes (as sent to a WidefieldAccelerator) and the scenes' own
                      CPU images
+  binned_ref.npz     BinnedBlinkRenderer Setup + Render frames (kernel planes at upscale 1, 2, 4, a halo cut, the
+                     Gaussian), with edge-case emitters, added to non-zero images
 """
 from __future__ import annotations
 
@@ -221,6 +223,48 @@ def maps_and_geometry(R):
     np.savez_compressed(os.path.join(HERE, "maps_ref.npz"), **out)
 
 
+def binned_cases(seed, W, H, nz, n_frames=3, n=40):
+    """Per frame n emitters (xPx, yPx, photons, zIndex) with the edge cases Render skips or merges: positions beyond
+    the grid, photons <= 0, several emitters in one cell, z indices outside [0, nz); one empty frame."""
+    rng = np.random.default_rng(seed)
+    frames = []
+    for f in range(n_frames):
+        em = np.stack([rng.uniform(-12, W + 12, n), rng.uniform(-12, H + 12, n), rng.uniform(-100, 3000, n),
+                       rng.integers(-2, nz + 2, n).astype(np.float64)], 1)
+        em[:4, :2] = em[4, :2]           # five emitters in one cell
+        em[5, 2] = 0.0
+        frames.append(em)
+    frames.insert(1, np.zeros((0, 4)))  # an empty frame
+    return frames
+
+
+def binned(R, P):
+    """BinnedBlinkRenderer Setup + Render (Sim/BinnedBlinks.cpp) of seeded frames: kernel planes at upscale 1, 2, 4,
+    with a halo cut, and the in-focus Gaussian; each added to a non-zero image."""
+    planes = P["small_planes"]  # 9 planes, 65^2, os 4
+    out = {}
+    W, H = 40, 32
+    cases = [("k1", planes, 1, 0.0, 0.0), ("k2", planes, 2, 0.0, 0.0), ("k4", planes, 4, 0.0, 0.0),
+             ("halo", planes, 2, 0.0, 2e-3), ("gauss", None, 3, 1.3, 0.0)]
+    for i, (name, pl, up, sigma, thr) in enumerate(cases):
+        nz = 1 if pl is None else pl.shape[0]
+        frames = binned_cases(100 + i, W, H, nz)
+        img0 = np.random.default_rng(200 + i).uniform(0, 5, (len(frames), H, W)).astype(np.float32)
+        img, (u, Rc) = R.binned_render(pl, 4, up, W, H, frames, sigma_px=sigma, halo_threshold=thr, img=img0)
+        out[name + "_em"] = np.concatenate(frames)
+        out[name + "_start"] = np.concatenate([[0], np.cumsum([len(f) for f in frames])]).astype(np.int32)
+        out[name + "_img0"] = img0
+        out[name + "_img"] = img
+        out[name + "_uR"] = np.array([u, Rc, up], np.int32)
+        out[name + "_sigma"] = np.float64(sigma)
+        r2 = -1
+        if thr > 0.0:
+            r2 = int(R.halo_spans(pl, 4, 0, thr)[2].max())
+        out[name + "_haloR2"] = np.int32(r2)
+    out["W"], out["H"] = np.int32(W), np.int32(H)
+    np.savez_compressed(os.path.join(HERE, "binned_ref.npz"), **out)
+
+
 def main():
     R = O.Reference()
     maps_and_geometry(R)
@@ -231,6 +275,7 @@ def main():
     noise(R)
     splat(R, P)
     wf(R, P)
+    binned(R, P)
     print("fixtures written to", HERE)
 
 

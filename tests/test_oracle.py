@@ -554,3 +554,36 @@ def test_roi_forward_normalisation_and_shift(orc):
     mu = orc.roi_forward(m, np.zeros(28), p)
     assert 0.85 < mu[0].sum() < 1.0
     assert np.allclose(mu[1][:, 1:], mu[0][:, :-1], rtol=1e-9, atol=1e-15)
+
+
+# ---- the binned blink regime (Sim/BinnedBlinks.cpp) ----------------------------------------------------------------
+
+BINNED_CASES = ("k1", "k2", "k4", "halo", "gauss")
+
+
+def binned_frames(G, name):
+    em, st = G[name + "_em"], G[name + "_start"]
+    return [em[st[f]:st[f + 1]] for f in range(len(st) - 1)]
+
+
+@pytest.mark.parametrize("name", BINNED_CASES)
+def test_binned_oracle_vs_reference(orc, golden, name):
+    """The FP64 direct convolution of the per-plane deposits equals the reference's FFT renderer (FP32, FastSize
+    grid) on its fixtures: geometry exactly, images to FP32 transform rounding."""
+    G = golden("binned_ref.npz")
+    P = golden("psf_ref.npz")["small_planes"]
+    W, H = int(G["W"]), int(G["H"])
+    u_ref, R_ref, up = (int(v) for v in G[name + "_uR"])
+    planes = None if name == "gauss" else P
+    sigma = float(G[name + "_sigma"])
+    u, R = orc.binned_geometry(planes, 4, up, sigma, int(G[name + "_haloR2"]))
+    assert (u, R) == (u_ref, R_ref)
+    K = orc.binned_kernels(planes, 4, up, R, sigma)
+    for f, fr in enumerate(binned_frames(G, name)):
+        got = orc.binned_render(K, u, R, W, H, fr, img=G[name + "_img0"][f]) - G[name + "_img0"][f]
+        want = G[name + "_img"][f] - G[name + "_img0"][f]
+        if not len(fr):
+            assert np.array_equal(G[name + "_img"][f], G[name + "_img0"][f])
+            continue
+        e = rel_l2(got, want)
+        assert e < 2e-6, (name, f, e)
