@@ -250,6 +250,48 @@ def run_smlm(ctx, steps, warmup, n_frames=2000):
             "_events": (ev, st, np.asarray(planes.cpu()))}
 
 
+def run_binned(ctx, ev, st, planes, n_frames=64):
+    """The binned blink regime (BinnedBlinkRenderer) on C4's frames: the same emitters as the splat, deposited per
+    z plane on the 512 + 2R grid (u = 1) and FFT-convolved with the 81 plane kernels (1024 x 1024 transforms)."""
+    import torch
+    from insiliscope_amd import host
+    cache = host.PsfKernelCache(planes, oversampling=4, zStepNm=15.0, interpMode=2)
+    frames = [host.CollectFrameEmitters(ev[st[f]:st[f + 1]], f, 100.0, 1.0, psfCache=cache) for f in range(n_frames)]
+    r = host.BinnedBlinkRenderer(ctx)
+    if not r.Setup(cache, 0.0, 512, 512, 1):
+        raise RuntimeError(r.err)
+    imgs = torch.zeros((n_frames, 512, 512), dtype=torch.float32, device="cuda")
+
+    def step():
+        if not r.RenderFrames(frames, imgs):
+            raise RuntimeError(r.err)
+
+    step()
+    with Timer(ctx) as t:
+        step()
+        step()
+    ms = t.ms / 2
+    planes_per_frame = float(np.mean([len(np.unique(np.clip(f["zIndex"], 0, 80))) for f in frames]))
+    return {"value": n_frames * 1000.0 / ms, "unit": "frames/s", "ms_per_step": ms,
+            "config": {"workload": f"binned blinks: C4's first {n_frames} frames (512x512, u 1, R {r.RadiusCells()}, "
+                                   f"81 kernel planes, {planes_per_frame:.1f} occupied per frame), 1024x1024 FFT grid"},
+            "_frames": [np.stack([f["xPx"], f["yPx"], f["photons"], f["zIndex"].astype(np.float64)], 1)
+                        for f in frames[:4]]}
+
+
+def cpu_binned(frames, planes):
+    import oracle as O
+    ref = O.load_reference()
+    if ref is None:
+        return None
+    ref.binned_render(planes, 4, 1, 512, 512, frames[:1])
+    t0 = time.time()
+    ref.binned_render(planes, 4, 1, 512, 512, frames)
+    el = time.time() - t0
+    return {"value": round(len(frames) / el, 2), "unit": "frames/s", "cores": CORES, "kind": "reference",
+            "sample": f"BinnedBlinkRenderer::Render of {len(frames)} frames ({el:.2f} s, FFT lines on all cores)"}
+
+
 def cpu_smlm(ev, st, planes, frames=96):
     import oracle as O
     from insiliscope_amd import configs
@@ -535,6 +577,7 @@ def main():
         r["roofline"] = None
     cpu = None
     if args.workload == "wf" and not args.no_secondary:
+        ev = None
         try:
             s = run_smlm(ctx, 2, 1)
             ev, st, planes = s.pop("_events")
@@ -546,6 +589,18 @@ def main():
                     s["speedup_vs_cpu"] = round(s["value"] / c["value"], 1)
         except Exception as e:  # a secondary line must not hide the primary one
             secondary["smlm_c4"] = {"error": str(e)}
+        if ev is not None:
+            try:
+                b = run_binned(ctx, ev, st, planes)
+                frames = b.pop("_frames")
+                secondary["binned_c4"] = b
+                if not args.no_cpu:
+                    c = cpu_binned(frames, planes)
+                    if c:
+                        b["cpu_baseline"] = c
+                        b["speedup_vs_cpu"] = round(b["value"] / c["value"], 1)
+            except Exception as e:
+                secondary["binned_c4"] = {"error": str(e)}
         try:
             c = run_c1(ctx, 5, 1)
             ev, st = c.pop("_events")
