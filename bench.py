@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "frames/s (SMLM 512² & WF 256²×64z) + HBM GB/s; fwd+adjoint at ≤1e-5 rel-L2"
 HBM_PEAK_GBS = 8000.0
-# host threads for the CPU baseline: the GPU box's CPU share per GPU (16), or fewer where the host has fewer
+# host threads for the CPU baseline: the host's CPU share per GPU (16 on an MI355X node), or fewer where it has fewer
 CORES = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "16")), os.cpu_count() or 1))
 MFMA_FP16_PEAK_TFS = 2500.0
 FP32_PEAK_TFS = 157.3
