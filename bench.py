@@ -489,6 +489,86 @@ def cpu_c5(z, prm, data):
             "sample": f"all {len(p)} ROIs ({el:.2f} s, FP64, one thread; the reference has no adjoint)"}
 
 
+# ---- dye schedules (LabelSchedule over a batch of dye addresses) ----------------------------------------------------
+
+DYE_KINETICS = dict(activationRatePerSec=0.2, onSec=0.05, offSec=1.0, bleachProb=0.05, photonCV=0.4,
+                    initialOnSec=3.0)
+DYE_T_MAX = 300.0
+
+
+def dye_addresses(n, seed=17):
+    rng = np.random.default_rng(seed)
+    d = np.zeros((n, 3), np.uint32)
+    d[:, 0] = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    d[:, 1] = rng.integers(0, 64, n).astype(np.uint32)
+    d[:, 2] = rng.integers(0, 1 << 24, n).astype(np.uint32)
+    return d
+
+
+def run_dyes(ctx, steps, warmup, n=1 << 21):
+    """isc::LabelSchedule (dSTORM: initial ON, then activation / ON / brightness / bleach / OFF draws, about 20
+    blinks per dye over 300 s) for n dye addresses resident in HBM, blinks written to HBM: one step = count, scan
+    and fill launches of one iscg_label_blinks call."""
+    import torch
+    from insiliscope_amd import host
+    d = dye_addresses(n)
+    dd = torch.from_numpy(d.view(np.int32).reshape(-1)).cuda()
+    kin = host.Kinetics(**DYE_KINETICS)
+    offsets = torch.zeros(n + 1, dtype=torch.int64, device="cuda")
+    total = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ctx.follow_torch()
+    L = ctx.L
+    ctx.check(L.iscg_label_blinks(ctx.h, dd.data_ptr(), n, 0, host.C.byref(kin), DYE_T_MAX, offsets.data_ptr(), None,
+                                  0, total.data_ptr()), "iscg_label_blinks")
+    torch.cuda.synchronize()
+    cap = int(total.item())
+    out = torch.empty(cap * 3, dtype=torch.float64, device="cuda")
+
+    def step(_=0):
+        ctx.check(L.iscg_label_blinks(ctx.h, dd.data_ptr(), n, 0, host.C.byref(kin), DYE_T_MAX, offsets.data_ptr(),
+                                      out.data_ptr(), cap, total.data_ptr()), "iscg_label_blinks")
+
+    for _ in range(warmup):
+        step()
+    with Timer(ctx) as t:
+        for _ in range(steps):
+            step()
+    ms = t.ms / steps
+    prof = profiled(ctx, step, 3)
+    kernels = {k: round(v[1] / max(v[0], 1) * 1000, 2) for k, v in prof.items() if k.startswith("dye_")}
+    fill_us = kernels.get("dye_fill")
+    # the fill launch's algorithmic HBM bytes: addresses 12 B + counts 4 B + offsets 8 B per dye, 24 B per blink
+    fill_bytes = 24 * n + 24 * cap
+    roof = None
+    if fill_us:
+        ach = fill_bytes / (fill_us * 1e-6) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(ach / 8000.0, 4), "traffic": None, "kernel": "dye_fill",
+                "note": "FP64 VALU-bound (a log per draw; log, cos, exp per brightness), not HBM-bound"}
+    return {"value": n * 1000.0 / ms, "unit": "dyes/s", "ms_per_step": ms, "blinks_per_s": cap * 1000.0 / ms,
+            "kernels_us": kernels, "roofline": roof,
+            "config": {"workload": f"LabelSchedule dSTORM blinks of {n} dye addresses, tMax {DYE_T_MAX} s, "
+                                   f"{cap / n:.2f} blinks per dye ({cap} blinks, 24 B each)",
+                       "kinetics": DYE_KINETICS}}
+
+
+def cpu_dyes(n=200000):
+    import ctypes as C
+    import oracle as O
+    ref = O.load_reference()
+    if ref is None:
+        return None
+    d = np.ascontiguousarray(dye_addresses(n, 23))
+    kin = O.kinetics(**DYE_KINETICS)
+    counts = np.zeros(n, np.int32)
+    t0 = time.time()
+    total = ref.L.ref_label_blinks(d.ctypes.data, n, 0, C.byref(kin), DYE_T_MAX, counts.ctypes.data_as(
+        C.POINTER(C.c_int32)), None, 0)
+    el = time.time() - t0
+    return {"value": round(n / el, 1), "unit": "dyes/s", "cores": 1, "kind": "reference",
+            "sample": f"isc::LabelSchedule of {n} dyes ({total} blinks) in one thread, {el:.2f} s"}
+
+
 def measure_peaks():
     """The rates this box attains, beside the datasheet peaks the rooflines use: a 1 GiB device-to-device copy
     (read + write bytes / time) and square GEMMs through torch (hipBLASLt / rocBLAS) in FP32, FP16 and BF16."""
@@ -527,7 +607,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="wf", choices=["wf", "smlm", "psf"])
+    ap.add_argument("--workload", default="wf", choices=["wf", "smlm", "psf", "dyes"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -571,6 +651,9 @@ def main():
         r.pop("_events")
         config, unit, dtype = r["config"], "frames/s", "f64"
         r["roofline"] = None
+    elif args.workload == "dyes":
+        r = run_dyes(ctx, args.steps, args.warmup)
+        config, unit, dtype = r["config"], "dyes/s", "f64"
     else:
         r = run_psf(ctx, args.steps, args.warmup)
         config, unit, dtype = r["config"], "PSFs/s", "f16x3/f32"
@@ -632,6 +715,16 @@ def main():
                     p["speedup_vs_cpu"] = round(p["value"] / c["value"], 1)
         except Exception as e:
             secondary["psf_library_c3"] = {"error": str(e)}
+        try:
+            dy = run_dyes(ctx, 5, 1)
+            secondary["dye_schedules"] = dy
+            if not args.no_cpu:
+                c = cpu_dyes()
+                if c:
+                    dy["cpu_baseline"] = c
+                    dy["speedup_vs_cpu"] = round(dy["value"] / c["value"], 1)
+        except Exception as e:
+            secondary["dye_schedules"] = {"error": str(e)}
     if not args.no_cpu and args.workload == "wf":
         try:
             cpu = cpu_wf(np.ascontiguousarray(r["_psf"].cpu().numpy()))

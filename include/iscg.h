@@ -381,6 +381,75 @@ int32_t iscg_roi_backward(iscg_context* h, const iscg_roi_model* model, const do
                           const iscg_roi_params* params, int32_t nRoi, const float* data, double* gradParams,
                           double* gradZernike, double* nll);
 
+/* ---- dye schedules --------------------------------------------------------------------------------------------- */
+
+/* One dye's address: the hash H1 of its structure block (DyeH1, core/src/dyes.cpp:175) and its site (k, n). */
+typedef struct
+{
+   uint32_t h1;
+   int32_t k;
+   int32_t n;
+} iscg_dye_address;
+
+/* isc::Kinetics (core/src/dyes.h:129-136). */
+typedef struct
+{
+   double activationRatePerSec, onSec, offSec, bleachProb, photonCV, initialOnSec;
+} iscg_kinetics;
+
+/* isc::Blink (core/src/dyes.h:197) and isc::BinBlink (:200). */
+typedef struct
+{
+   double tOn, tOff, brightness;
+} iscg_blink;
+typedef struct
+{
+   double tOn, tOff, brightness;
+   uint32_t bin, j;
+} iscg_bin_blink;
+
+/* isc::ContWindow (core/src/dyes.h:190): state 1 PRE, 2 INITIAL_ON, 3 ALWAYS_ON. */
+typedef struct
+{
+   double tOn, tOff;
+   uint8_t state;
+   double aux;
+} iscg_cont_window;
+
+/* The schedule calls below share one output form. dyes: nDyes addresses (host or device). offsets (device or host,
+ * nDyes + 1): dye i's records are out[offsets[i] .. offsets[i+1]), offsets[nDyes] = *total, the number of records.
+ * out (device or host, capacity records; may be null with capacity 0) receives every dye whose records end at or
+ * before capacity; when *total > capacity the call succeeds with the later dyes left out, and a second call with
+ * capacity >= *total (the values are a pure function of the addresses) gives all of them. */
+
+/* Replaces isc::LabelSchedule's blinks (core/src/dyes.cpp:440-471, dyes.h:235) for nDyes dyes of one label (mode:
+ * 0 dSTORM, 1 PALM, 2 DNA-PAINT, 3 WideField = isc::LabelMode; one set of kinetics, no history): each dye's blinks
+ * in time order, stopping at the first starting at or after tMax, bit-identical to the reference's. */
+int32_t iscg_label_blinks(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, int32_t mode,
+                          const iscg_kinetics* kin, double tMax, int64_t* offsets, iscg_blink* out, int64_t capacity,
+                          int64_t* total);
+
+/* Replaces isc::LabelSchedule's continuous windows (core/src/dyes.cpp:447-477): at most one per dye for one set of
+ * kinetics; has[i] = 1 and out[i] set when dye i has one, else has[i] = 0 (out[i] untouched). */
+int32_t iscg_label_cont(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, int32_t mode, int32_t preState,
+                        const iscg_kinetics* kin, iscg_cont_window* out, uint8_t* has);
+
+/* Replaces isc::PersistentBlinks (core/src/dyes.cpp:636-646, dyes.h:250): each persistent (DNA-PAINT) dye's blinks
+ * overlapping the window (t0, t1), by time bin then index, one set of kinetics. */
+int32_t iscg_persistent_blinks(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, const iscg_kinetics* kin,
+                               double t0, double t1, int64_t* offsets, iscg_blink* out, int64_t capacity,
+                               int64_t* total);
+
+/* Replaces isc::PersistentBlinksInBins (core/src/dyes.cpp:648-655, dyes.h:256): every blink of bins [binLo, binHi]
+ * with its bin and index, unfiltered. */
+int32_t iscg_persistent_bins(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, const iscg_kinetics* kin,
+                             int64_t binLo, int64_t binHi, int64_t* offsets, iscg_bin_blink* out, int64_t capacity,
+                             int64_t* total);
+
+/* The device's jsm::sin (fn 0), cos (1), exp (2), log (3) (core/src/jsmath.h:15-22; fdlibm as V8 runs it) of n
+ * values, the functions the schedules use; sin/cos for |x| <= 2^19 pi/2 (NaN beyond). */
+int32_t iscg_jsm(iscg_context* h, int32_t fn, const double* x, int64_t n, double* out);
+
 #ifdef __cplusplus
 }
 #endif

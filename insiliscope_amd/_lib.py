@@ -78,6 +78,12 @@ class PsfRequest(C.Structure):
                 ("pupilSamples", C.c_int32), ("interpMode", C.c_int32), ("reserved", C.c_int32)]
 
 
+class Kinetics(C.Structure):
+    """iscg_kinetics = isc::Kinetics (core/src/dyes.h:129-136)."""
+    _fields_ = [(n, C.c_double) for n in ("activationRatePerSec", "onSec", "offSec", "bleachProb", "photonCV",
+                                          "initialOnSec")]
+
+
 _lib = None
 
 I32, U32, U64, D, P = C.c_int32, C.c_uint32, C.c_uint64, C.c_double, C.c_void_p
@@ -121,6 +127,11 @@ SIGNATURES = {
     "iscg_roi_backward": (I32, [P, C.POINTER(RoiModel), C.POINTER(D), P, I32, P, P, C.POINTER(D), C.POINTER(D)]),
     "iscg_wf_conv3d_plan": (I32, [P, C.POINTER(WfDesc), P, I32, I32, I32, D, I32]),
     "iscg_wf_conv3d": (I32, [P, P, C.POINTER(Camera), P, D, U32, U32, U32, P, P]),
+    "iscg_label_blinks": (I32, [P, P, I32, I32, C.POINTER(Kinetics), D, P, P, C.c_int64, P]),
+    "iscg_label_cont": (I32, [P, P, I32, I32, I32, C.POINTER(Kinetics), P, P]),
+    "iscg_persistent_blinks": (I32, [P, P, I32, C.POINTER(Kinetics), D, D, P, P, C.c_int64, P]),
+    "iscg_persistent_bins": (I32, [P, P, I32, C.POINTER(Kinetics), C.c_int64, C.c_int64, P, P, C.c_int64, P]),
+    "iscg_jsm": (I32, [P, I32, P, C.c_int64, P]),
 }
 
 

@@ -90,6 +90,8 @@ struct iscg_context
    iscg::WfAccelState* wfAccel = nullptr;
    iscg::RoiState* roi = nullptr;
    iscg::BinnedState* binned = nullptr;
+   iscg::DevBuf dyeCounts;             // per-dye record counts of a schedule call
+   iscg::DevBuf dyeBlocks;             // per-workgroup sums, then their exclusive prefixes
    int noiseExact = 0;  // iscg_noise_set_path: 1 evaluates every pixel with the FP64 chain
    iscg::DevBuf noiseItems;            // deferred pixels (NoiseQueue::items)
    uint32_t* noiseCount = nullptr;     // NoiseQueue::count, 2 words

@@ -925,3 +925,131 @@ __all__ = ["Context", "CameraNoiseParams", "ApplyNoiseChain", "ApplyNoiseChainFr
            "BuildZernikePsfKernelCache", "PsfGeometry", "psf_request", "ROI_PARAMS_DTYPE", "roi_model", "RoiForward",
            "RoiBackward", "PhaseRetrieval", "CollectFrameEmitters", "CollectGpuEmitters", "FRAME_EMITTER_DTYPE",
            "SPLAT_EMITTER_DTYPE", "BLINK_EVENT_DTYPE", "INTERP", "IscgError"]
+
+
+# ---- dye schedules (core/src/dyes.h:129-261) ---------------------------------------------------------------------
+
+# iscg_dye_address (h1 of the structure block, site k, n), iscg_blink = isc::Blink, iscg_bin_blink = isc::BinBlink,
+# iscg_cont_window = isc::ContWindow (core/src/dyes.h:190-200)
+DYE_ADDRESS_DTYPE = np.dtype([("h1", "u4"), ("k", "i4"), ("n", "i4")])
+BLINK_DTYPE = np.dtype([("tOn", "f8"), ("tOff", "f8"), ("brightness", "f8")])
+BIN_BLINK_DTYPE = np.dtype([("tOn", "f8"), ("tOff", "f8"), ("brightness", "f8"), ("bin", "u4"), ("j", "u4")])
+CONT_WINDOW_DTYPE = np.dtype({"names": ["tOn", "tOff", "state", "aux"], "formats": ["f8", "f8", "u1", "f8"],
+                              "offsets": [0, 8, 16, 24], "itemsize": 32})
+assert DYE_ADDRESS_DTYPE.itemsize == 12 and BLINK_DTYPE.itemsize == 24 and BIN_BLINK_DTYPE.itemsize == 32
+
+# isc::LabelMode (core/src/dyes.h:155); isc::EventState of a continuous window (:159)
+LABEL_MODES = {"dSTORM": 0, "PALM": 1, "DnaPaint": 2, "WideField": 3}
+STATE_PRE, STATE_INITIAL_ON, STATE_ALWAYS_ON = 1, 2, 3
+
+
+def Kinetics(activationRatePerSec: float = 0.01, onSec: float = 0.05, offSec: float = 1.0, bleachProb: float = 1.0,
+             photonCV: float = 0.5, initialOnSec: float = 0.0) -> _lib.Kinetics:
+    """isc::Kinetics with the reference's defaults (core/src/dyes.h:129-136)."""
+    return _lib.Kinetics(activationRatePerSec, onSec, offSec, bleachProb, photonCV, initialOnSec)
+
+
+@dataclass
+class Label:
+    """The fields of isc::Label (core/src/dyes.h:166-179) a schedule reads: mode, kinetics, PALM pre state."""
+
+    mode: str = "DnaPaint"
+    kin: _lib.Kinetics = field(default_factory=Kinetics)
+    preState: bool = False
+
+
+def DyeAddresses(h1, k, n):
+    """An iscg_dye_address array from per-dye h1 (DyeH1 of the block, core/src/dyes.cpp:175), k and n."""
+    h1 = np.asarray(h1, np.int64)
+    out = np.zeros(h1.size, DYE_ADDRESS_DTYPE)
+    out["h1"] = (h1 & 0xFFFFFFFF).astype(np.uint32)
+    out["k"] = np.asarray(k, np.int64).astype(np.int32)
+    out["n"] = np.asarray(n, np.int64).astype(np.int32)
+    return out
+
+
+def _dyes(dyes):
+    if hasattr(dyes, "data_ptr"):
+        if dyes.numel() * dyes.element_size() % 12:
+            raise IscgError("dye addresses: 12 bytes per dye")
+        return dyes, dyes.numel() * dyes.element_size() // 12
+    d = np.ascontiguousarray(dyes)
+    if d.dtype != DYE_ADDRESS_DTYPE:
+        d = np.ascontiguousarray(np.asarray(dyes, np.int64).reshape(-1, 3).astype(np.uint32)).view(DYE_ADDRESS_DTYPE)
+        d = d.reshape(-1)
+    return d, d.size
+
+
+def _records(ctx, fn, args, n, dtype, capacity):
+    """The two-step capacity call shared by the schedule entry points: (offsets n + 1, records)."""
+    offsets = np.zeros(n + 1, np.int64)
+    total = np.zeros(1, np.int64)
+    cap = int(capacity) if capacity is not None else max(64, 4 * n)
+    for _ in range(2):
+        out = np.zeros(cap, dtype)
+        ctx.check(fn(ctx.h, *args, offsets.ctypes.data, out.ctypes.data if cap else None, cap, total.ctypes.data),
+                  fn.__name__)
+        if total[0] <= cap:
+            return offsets, out[: total[0]]
+        cap = int(total[0])
+    raise IscgError("dye schedule: record count changed between calls")
+
+
+def LabelSchedule(dyes, label: Label, tMax: float = float("inf"), ctx: Context | None = None, capacity=None):
+    """isc::LabelSchedule's blinks (core/src/dyes.cpp:440-471) for a batch of dyes of one label on the GPU: (offsets,
+    blinks) with dye i's blinks at blinks[offsets[i]:offsets[i+1]] (BLINK_DTYPE), in time order, bit-identical to
+    the reference's. One set of kinetics (no KineticsHistory)."""
+    ctx = ctx or default_context()
+    d, n = _dyes(dyes)
+    p, keep = _ptr(d)
+    ctx.follow_torch()
+    return _records(ctx, ctx.L.iscg_label_blinks, (p, n, LABEL_MODES[label.mode], C.byref(label.kin), float(tMax)), n,
+                    BLINK_DTYPE, capacity)
+
+
+def LabelContWindows(dyes, label: Label, ctx: Context | None = None):
+    """isc::LabelSchedule's continuous windows (core/src/dyes.cpp:447-477): (has, windows) per dye, CONT_WINDOW_DTYPE
+    (at most one per dye for one set of kinetics)."""
+    ctx = ctx or default_context()
+    d, n = _dyes(dyes)
+    p, keep = _ptr(d)
+    has = np.zeros(n, np.uint8)
+    out = np.zeros(n, CONT_WINDOW_DTYPE)
+    ctx.follow_torch()
+    ctx.check(ctx.L.iscg_label_cont(ctx.h, p, n, LABEL_MODES[label.mode], int(label.preState), C.byref(label.kin),
+                                    out.ctypes.data, has.ctypes.data), "iscg_label_cont")
+    return has.astype(bool), out
+
+
+def PersistentBlinks(dyes, kin: _lib.Kinetics, t0: float, t1: float, ctx: Context | None = None, capacity=None):
+    """isc::PersistentBlinks (core/src/dyes.cpp:636-646) of a batch of persistent (DNA-PAINT) dyes: the blinks
+    overlapping (t0, t1), as (offsets, blinks)."""
+    ctx = ctx or default_context()
+    d, n = _dyes(dyes)
+    p, keep = _ptr(d)
+    ctx.follow_torch()
+    return _records(ctx, ctx.L.iscg_persistent_blinks, (p, n, C.byref(kin), float(t0), float(t1)), n, BLINK_DTYPE,
+                    capacity)
+
+
+def PersistentBlinksInBins(dyes, kin: _lib.Kinetics, binLo: int, binHi: int, ctx: Context | None = None,
+                           capacity=None):
+    """isc::PersistentBlinksInBins (core/src/dyes.cpp:648-655): every blink of bins [binLo, binHi] with its bin and
+    index (BIN_BLINK_DTYPE), as (offsets, blinks)."""
+    ctx = ctx or default_context()
+    d, n = _dyes(dyes)
+    p, keep = _ptr(d)
+    ctx.follow_torch()
+    return _records(ctx, ctx.L.iscg_persistent_bins, (p, n, C.byref(kin), int(binLo), int(binHi)), n,
+                    BIN_BLINK_DTYPE, capacity)
+
+
+def JsMath(fn: str, x, ctx: Context | None = None):
+    """The device's jsm::sin / cos / exp / log (core/src/jsmath.h) of each element of x (float64)."""
+    ctx = ctx or default_context()
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty_like(x)
+    ctx.follow_torch()
+    ctx.check(ctx.L.iscg_jsm(ctx.h, ("sin", "cos", "exp", "log").index(fn), x.ctypes.data, x.size,
+                             out.ctypes.data), "iscg_jsm")
+    return out

@@ -76,6 +76,23 @@ GPU_EMITTER_DTYPE = np.dtype([("x0", "i4"), ("y0", "i4"), ("bx", "i4"), ("by", "
                               ("wy", "f4", 4)])
 
 
+class Kinetics(C.Structure):
+    """core/src/dyes.h:129-136."""
+    _fields_ = [(n, C.c_double) for n in ("activationRatePerSec", "onSec", "offSec", "bleachProb", "photonCV",
+                                          "initialOnSec")]
+
+
+def kinetics(**kw) -> Kinetics:
+    """Kinetics with the reference's defaults (core/src/dyes.h:130-135), overridden by kw."""
+    k = Kinetics(0.01, 0.05, 1.0, 1.0, 0.5, 0.0)
+    for a, v in kw.items():
+        setattr(k, a, float(v))
+    return k
+
+
+JSM_FUNCS = ("sin", "cos", "exp", "log")
+
+
 def _p(a, ct):
     return None if a is None else a.ctypes.data_as(C.POINTER(ct))
 
@@ -156,6 +173,16 @@ class Oracle:
         L.orc_roi_forward.argtypes = [C.POINTER(RoiModel), PD, C.c_void_p, I32, PD]
         L.orc_roi_backward.argtypes = [C.POINTER(RoiModel), PD, C.c_void_p, I32, PD, PD, PD]
         L.orc_roi_backward.restype = D
+        for f in JSM_FUNCS:
+            getattr(L, "orc_jsm_" + f).argtypes = [D]
+            getattr(L, "orc_jsm_" + f).restype = D
+        PK = C.POINTER(Kinetics)
+        L.orc_label_blinks.argtypes = [U32, I32, I32, I32, PK, D, PD, I32]
+        L.orc_label_blinks.restype = I32
+        L.orc_label_cont.argtypes = [U32, I32, I32, I32, I32, PK, PD]
+        L.orc_label_cont.restype = I32
+        L.orc_persistent_blinks.argtypes = [U32, I32, I32, PK, I32, D, D, I64, I64, PD, I32]
+        L.orc_persistent_blinks.restype = I32
 
     # -- RNG --
     def pcg4d(self, a, b, c, d):
@@ -355,6 +382,65 @@ class Oracle:
         return nll, gp, gz
 
 
+    # -- JS-compatible math and dye schedules --
+    def jsm(self, fn: str, x):
+        """jsm::sin / cos / exp / log (fdlibm as V8 runs it) of each element of x."""
+        g = getattr(self.L, "orc_jsm_" + fn)
+        x = np.asarray(x, np.float64)
+        return np.array([g(float(v)) for v in x.ravel()], np.float64).reshape(x.shape)
+
+    def label_blinks(self, dyes, mode, kin, t_max):
+        """LabelSchedule's blinks per dye (dyes: n x 3 uint32 h1, k, n). Returns (counts, blinks total x 3)."""
+        dyes = np.asarray(dyes, np.uint32).reshape(-1, 3)
+        buf = np.zeros((1024, 3))
+        counts = np.zeros(len(dyes), np.int32)
+        out = []
+        for i, (h1, k, n) in enumerate(dyes):
+            c = self.L.orc_label_blinks(int(h1), int(np.int32(k)), int(np.int32(n)), mode, C.byref(kin), t_max,
+                                        _p(buf, C.c_double), buf.shape[0])
+            if c > buf.shape[0]:
+                buf = np.zeros((c, 3))
+                c = self.L.orc_label_blinks(int(h1), int(np.int32(k)), int(np.int32(n)), mode, C.byref(kin), t_max,
+                                            _p(buf, C.c_double), buf.shape[0])
+            counts[i] = c
+            out.append(buf[:c].copy())
+        return counts, (np.concatenate(out) if out else np.zeros((0, 3)))
+
+    def label_cont(self, dyes, mode, pre_state, kin):
+        """LabelSchedule's continuous window per dye: (has n, windows n x 4 tOn, tOff, state, aux)."""
+        dyes = np.asarray(dyes, np.uint32).reshape(-1, 3)
+        has = np.zeros(len(dyes), np.int32)
+        w = np.zeros((len(dyes), 4))
+        one = np.zeros(4)
+        for i, (h1, k, n) in enumerate(dyes):
+            has[i] = self.L.orc_label_cont(int(h1), int(np.int32(k)), int(np.int32(n)), mode, int(pre_state),
+                                           C.byref(kin), _p(one, C.c_double))
+            if has[i]:
+                w[i] = one
+        return has, w
+
+    def persistent_blinks(self, dyes, kin, t0=0.0, t1=0.0, bins=None):
+        """PersistentBlinks over the window (t0, t1), or PersistentBlinksInBins over bins = (lo, hi): (counts,
+        blinks total x 5 tOn, tOff, brightness, bin, j; bin and j 0 for the window form)."""
+        dyes = np.asarray(dyes, np.uint32).reshape(-1, 3)
+        buf = np.zeros((4096, 5))
+        counts = np.zeros(len(dyes), np.int32)
+        out = []
+        lo, hi = bins if bins is not None else (0, 0)
+        for i, (h1, k, n) in enumerate(dyes):
+            args = (int(h1), int(np.int32(k)), int(np.int32(n)), C.byref(kin), int(bins is not None), t0, t1, lo, hi)
+            c = self.L.orc_persistent_blinks(*args, _p(buf, C.c_double), buf.shape[0])
+            if c > buf.shape[0]:
+                buf = np.zeros((c, 5))
+                c = self.L.orc_persistent_blinks(*args, _p(buf, C.c_double), buf.shape[0])
+            counts[i] = c
+            b = buf[:c].copy()
+            if bins is None:
+                b[:, 3:] = 0
+            out.append(b)
+        return counts, (np.concatenate(out) if out else np.zeros((0, 5)))
+
+
 def roi_model(**kw) -> RoiModel:
     m = RoiModel(wavelengthNm=670.0, na=1.45, immersionIndex=1.518, sampleIndex=1.33, sampleDepthNm=0.0,
                  pixelSizeNm=100.0, pupilSamples=64, roiSize=15, oversampling=4, reserved=0)
@@ -411,6 +497,13 @@ class Reference:
         L.ref_wf_images_render.restype = I32
         L.ref_binned_render.argtypes = [C.c_void_p, I32, I32, I32, D, D, I32, I32, I32, PD, PI32, I32, PF, PI32]
         L.ref_binned_render.restype = I32
+        L.ref_jsm.argtypes = [I32, PD, I64, PD]
+        PK = C.POINTER(Kinetics)
+        L.ref_label_blinks.argtypes = [C.c_void_p, I32, I32, PK, D, PI32, PD, I64]
+        L.ref_label_blinks.restype = I64
+        L.ref_label_cont.argtypes = [C.c_void_p, I32, I32, I32, PK, PI32, PD]
+        L.ref_persistent_blinks.argtypes = [C.c_void_p, I32, PK, I32, D, D, I64, I64, PI32, PD, I64]
+        L.ref_persistent_blinks.restype = I64
 
     def psf_planes(self, req, size, nz):
         out = np.zeros((nz, size, size), np.float32)
@@ -580,6 +673,43 @@ class Reference:
         if rc != 0:
             raise RuntimeError("BinnedBlinkRenderer::Setup failed")
         return out, (int(uR[0]), int(uR[1]))
+
+    def jsm(self, fn: str, x):
+        """The compiled reference's jsm:: function (fdlibm as V8 runs it) of each element of x."""
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.empty_like(x)
+        self.L.ref_jsm(JSM_FUNCS.index(fn), _p(x, C.c_double), x.size, _p(out, C.c_double))
+        return out
+
+    def label_blinks(self, dyes, mode, kin, t_max):
+        """LabelSchedule (core/src/dyes.cpp:440) blinks per dye: (counts, blinks total x 3)."""
+        d = np.ascontiguousarray(dyes, np.uint32).reshape(-1, 3)
+        counts = np.zeros(len(d), np.int32)
+        total = self.L.ref_label_blinks(d.ctypes.data, len(d), mode, C.byref(kin), t_max, _p(counts, C.c_int32),
+                                        None, 0)
+        out = np.zeros((max(total, 1), 3))
+        self.L.ref_label_blinks(d.ctypes.data, len(d), mode, C.byref(kin), t_max, _p(counts, C.c_int32),
+                                _p(out, C.c_double), total)
+        return counts, out[:total]
+
+    def label_cont(self, dyes, mode, pre_state, kin):
+        d = np.ascontiguousarray(dyes, np.uint32).reshape(-1, 3)
+        has = np.zeros(len(d), np.int32)
+        w = np.zeros((len(d), 4))
+        self.L.ref_label_cont(d.ctypes.data, len(d), mode, int(pre_state), C.byref(kin), _p(has, C.c_int32),
+                              _p(w, C.c_double))
+        return has, w
+
+    def persistent_blinks(self, dyes, kin, t0=0.0, t1=0.0, bins=None):
+        """PersistentBlinks (window) or PersistentBlinksInBins (bins = (lo, hi)): (counts, blinks total x 5)."""
+        d = np.ascontiguousarray(dyes, np.uint32).reshape(-1, 3)
+        counts = np.zeros(len(d), np.int32)
+        lo, hi = bins if bins is not None else (0, 0)
+        args = (d.ctypes.data, len(d), C.byref(kin), int(bins is not None), t0, t1, lo, hi, _p(counts, C.c_int32))
+        total = self.L.ref_persistent_blinks(*args, None, 0)
+        out = np.zeros((max(total, 1), 5))
+        self.L.ref_persistent_blinks(*args, _p(out, C.c_double), total)
+        return counts, out[:total]
 
     def noise_setup(self, seed, w, h, offset, offset_std, gain, gain_std, read, read_std, sensor_h=0, row_y0=0):
         maps = [np.zeros((h, w), np.float32) for _ in range(3)]

@@ -18,6 +18,12 @@
  *                                     block sums total os^2.
  *   mean-field kernel + direct conv .. compiled-reference KernelWidefieldPsf planes (bit-exact); delta density
  *                                     reproduces the kernel; linearity; brute force on tiny grids.
+ *   jsm log / exp / sin / cos ........ spec/golden/ref.txt `m` lines (sin/cos for |x| <= 2^19 pi/2), compiled
+ *                                     reference at the reduction boundaries (bit-exact); libm to one ulp
+ *                                     (tests/test_oracle_dyes.py).
+ *   dye schedules .................... compiled-reference LabelSchedule / PersistentBlinks(InBins) fixtures and
+ *                                     random kinetics (bit-exact); the activation, ON, brightness, bleach and
+ *                                     Poisson laws (tests/test_oracle_dyes.py).
  */
 #include "oracle.h"
 
@@ -1684,4 +1690,574 @@ void orc_binned_render(const float* kernels, int32_t nz, int32_t u, int32_t R, i
          }
    free(dens);
    free(cells);
+}
+
+/* ======================================================================= */
+/* JS-compatible FP64 math: the published fdlibm 5.3 algorithms (Sun      */
+/* Microsystems, 1993) in the form V8 runs them for Math.* and the         */
+/* reference compiles as jsm:: (core/src/jsmath_fdlibm.cpp:137-1564).      */
+/* Only the pieces the dye schedules call: log, exp, and sin/cos with the  */
+/* argument reductions up to 2^19 pi/2.                                    */
+/* ======================================================================= */
+
+static int32_t hi_of(double x)
+{
+   uint64_t u;
+   memcpy(&u, &x, 8);
+   return (int32_t)(uint32_t)(u >> 32);
+}
+
+static uint32_t lo_of(double x)
+{
+   uint64_t u;
+   memcpy(&u, &x, 8);
+   return (uint32_t)u;
+}
+
+static double of_words(uint32_t hi, uint32_t lo)
+{
+   const uint64_t u = (uint64_t)hi << 32 | lo;
+   double x;
+   memcpy(&x, &u, 8);
+   return x;
+}
+
+/* e_log.c: log(x) = k ln2 + log(1 + f), log(1 + f) = 2s + s R(s^2) with s = f / (2 + f) and R Remez-fitted on
+ * [0, 0.1716] (coefficients Lg1..Lg7); k and f from the exponent and mantissa with 1 + f in [sqrt(2)/2, sqrt(2)).
+ * V8 form (jsmath_fdlibm.cpp:1278-1357): log(+-0) = -inf, log(<0) = NaN. */
+double orc_jsm_log(double x)
+{
+   const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+   const double two54 = 1.80143985094819840000e+16;
+   const double Lg1 = 6.666666666666735130e-01, Lg2 = 3.999999999940941908e-01, Lg3 = 2.857142874366239149e-01,
+                Lg4 = 2.222219843214978396e-01, Lg5 = 1.818357216161805012e-01, Lg6 = 1.531383769920937332e-01,
+                Lg7 = 1.479819860511658591e-01;
+   int32_t hx = hi_of(x);
+   const uint32_t lx = lo_of(x);
+   int32_t k = 0;
+   if (hx < 0x00100000)
+   {
+      if (((hx & 0x7fffffff) | (int32_t)lx) == 0)
+         return -INFINITY;
+      if (hx < 0)
+         return NAN;
+      k -= 54;
+      x *= two54;
+      hx = hi_of(x);
+   }
+   if (hx >= 0x7ff00000)
+      return x + x;
+   k += (hx >> 20) - 1023;
+   hx &= 0x000fffff;
+   const int32_t norm = (hx + 0x95f64) & 0x100000;
+   x = of_words((uint32_t)(hx | (norm ^ 0x3ff00000)), lo_of(x));
+   k += norm >> 20;
+   const double f = x - 1.0;
+   if ((0x000fffff & (2 + hx)) < 3)
+   {
+      if (f == 0.0)
+      {
+         if (k == 0)
+            return 0.0;
+         const double dk = (double)k;
+         return dk * ln2_hi + dk * ln2_lo;
+      }
+      const double R = f * f * (0.5 - 0.33333333333333333 * f);
+      if (k == 0)
+         return f - R;
+      const double dk = (double)k;
+      return dk * ln2_hi - ((R - dk * ln2_lo) - f);
+   }
+   const double s = f / (2.0 + f);
+   const double dk = (double)k;
+   const double z = s * s;
+   int32_t i = hx - 0x6147a;
+   const double w = z * z;
+   const int32_t j = 0x6b851 - hx;
+   const double t1 = w * (Lg2 + w * (Lg4 + w * Lg6));
+   const double t2 = z * (Lg1 + w * (Lg3 + w * (Lg5 + w * Lg7)));
+   i |= j;
+   const double R = t2 + t1;
+   if (i > 0)
+   {
+      const double hfsq = 0.5 * f * f;
+      if (k == 0)
+         return f - (hfsq - s * (hfsq + R));
+      return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+   }
+   if (k == 0)
+      return f - s * (f - R);
+   return dk * ln2_hi - ((s * (f - R) - dk * ln2_lo) - f);
+}
+
+/* e_exp.c: x = k ln2 + r with |r| <= ln2/2 (ln2 split hi + lo), exp(r) = 1 + r + r c / (2 - c) with c = r - r^2
+ * P(r^2) (P1..P5 Remez-fitted), scaled by 2^k. V8 form (jsmath_fdlibm.cpp:1135-1276): exp(1) returns the double
+ * nearest e, the scale is a multiply by 2^k built from its exponent bits, k = 1024 and k < -1021 split the scale. */
+double orc_jsm_exp(double x)
+{
+   const double halF[2] = {0.5, -0.5};
+   const double o_threshold = 7.09782712893383973096e+02, u_threshold = -7.45133219101941108420e+02;
+   const double ln2HI[2] = {6.93147180369123816490e-01, -6.93147180369123816490e-01};
+   const double ln2LO[2] = {1.90821492927058770002e-10, -1.90821492927058770002e-10};
+   const double invln2 = 1.44269504088896338700e+00;
+   const double P1 = 1.66666666666666019037e-01, P2 = -2.77777777770155933842e-03, P3 = 6.61375632143793436117e-05,
+                P4 = -1.65339022054652515390e-06, P5 = 4.13813679705723846039e-08;
+   const double E = 2.718281828459045;
+   volatile double huge = 1.0e+300, twom1000 = 9.33263618503218878990e-302, two1023 = 8.988465674311579539e307;
+   double hi = 0.0, lo = 0.0;
+   int32_t k = 0;
+   uint32_t hx = (uint32_t)hi_of(x);
+   const int32_t xsb = (int32_t)((hx >> 31) & 1);
+   hx &= 0x7fffffff;
+   if (hx >= 0x40862e42)
+   {
+      if (hx >= 0x7ff00000)
+      {
+         if (((hx & 0xfffff) | lo_of(x)) != 0)
+            return x + x;
+         return xsb == 0 ? x : 0.0;
+      }
+      if (x > o_threshold)
+         return huge * huge;
+      if (x < u_threshold)
+         return twom1000 * twom1000;
+   }
+   if (hx > 0x3fd62e42)
+   {
+      if (hx < 0x3ff0a2b2)
+      {
+         if (x == 1.0)
+            return E;
+         hi = x - ln2HI[xsb];
+         lo = ln2LO[xsb];
+         k = 1 - xsb - xsb;
+      }
+      else
+      {
+         k = (int32_t)(invln2 * x + halF[xsb]);
+         const double t = k;
+         hi = x - t * ln2HI[0];
+         lo = t * ln2LO[0];
+      }
+      x = hi - lo;
+   }
+   else if (hx < 0x3e300000)
+   {
+      if (huge + x > 1.0)
+         return 1.0 + x;
+   }
+   else
+      k = 0;
+   const double t = x * x;
+   double twopk;
+   if (k >= -1021)
+      twopk = of_words(0x3ff00000u + ((uint32_t)k << 20), 0);
+   else
+      twopk = of_words(0x3ff00000u + ((uint32_t)(k + 1000) << 20), 0);
+   const double c = x - t * (P1 + t * (P2 + t * (P3 + t * (P4 + t * P5))));
+   if (k == 0)
+      return 1.0 - ((x * c) / (c - 2.0) - x);
+   const double y = 1.0 - ((lo - (x * c) / (2.0 - c)) - hi);
+   if (k >= -1021)
+   {
+      if (k == 1024)
+         return y * 2.0 * two1023;
+      return y * twopk;
+   }
+   return y * twopk * twom1000;
+}
+
+/* High words of n pi/2, n = 1..32 (e_rem_pio2.c's npio2_hw): a reduction of such an x cancels, so it takes the
+ * second step. */
+static const int32_t kNpio2Hw[32] = {
+   0x3ff921fb, 0x400921fb, 0x4012d97c, 0x401921fb, 0x401f6a7a, 0x4022d97c, 0x4025fdbb, 0x402921fb,
+   0x402c463a, 0x402f6a7a, 0x4031475c, 0x4032d97c, 0x40346b9c, 0x4035fdbb, 0x40378fdb, 0x403921fb,
+   0x403ab41b, 0x403c463a, 0x403dd85a, 0x403f6a7a, 0x40407e4c, 0x4041475c, 0x4042106c, 0x4042d97c,
+   0x4043a28c, 0x40446b9c, 0x404534ac, 0x4045fdbb, 0x4046c6cb, 0x40478fdb, 0x404858eb, 0x404921fb};
+
+/* e_rem_pio2.c for |x| <= 2^19 pi/2: x = n pi/2 + (y0 + y1), pi/2 in 33-bit pieces (pio2_1, pio2_2, pio2_3 and the
+ * tails) applied one, two or three at a time as cancellation demands. Larger |x|: NaN (the Payne-Hanek branch is
+ * not restated). */
+static int32_t rem_pio2(double x, double* y)
+{
+   const double invpio2 = 6.36619772367581382433e-01;
+   const double pio2_1 = 1.57079632673412561417e+00, pio2_1t = 6.07710050650619224932e-11;
+   const double pio2_2 = 6.07710050630396597660e-11, pio2_2t = 2.02226624879595063154e-21;
+   const double pio2_3 = 2.02226624871116645580e-21, pio2_3t = 8.47842766036889956997e-32;
+   const int32_t hx = hi_of(x);
+   const int32_t ix = hx & 0x7fffffff;
+   if (ix <= 0x3fe921fb)
+   {
+      y[0] = x;
+      y[1] = 0.0;
+      return 0;
+   }
+   if (ix < 0x4002d97c)
+   {
+      const double s = hx > 0 ? 1.0 : -1.0;
+      double z = x - s * pio2_1;
+      if (ix != 0x3ff921fb)
+      {
+         y[0] = z - s * pio2_1t;
+         y[1] = (z - y[0]) - s * pio2_1t;
+      }
+      else
+      {
+         z -= s * pio2_2;
+         y[0] = z - s * pio2_2t;
+         y[1] = (z - y[0]) - s * pio2_2t;
+      }
+      return hx > 0 ? 1 : -1;
+   }
+   if (ix > 0x413921fb)
+   {
+      y[0] = y[1] = NAN;
+      return 0;
+   }
+   const double t0 = fabs(x);
+   const int32_t n = (int32_t)(t0 * invpio2 + 0.5);
+   const double fn = (double)n;
+   double r = t0 - fn * pio2_1;
+   double w = fn * pio2_1t;
+   y[0] = r - w;
+   if (!(n < 32 && ix != kNpio2Hw[n - 1]))
+   {
+      const int32_t j = ix >> 20;
+      int32_t i = j - ((hi_of(y[0]) >> 20) & 0x7ff);
+      if (i > 16)
+      {
+         double t = r;
+         w = fn * pio2_2;
+         r = t - w;
+         w = fn * pio2_2t - ((t - r) - w);
+         y[0] = r - w;
+         i = j - ((hi_of(y[0]) >> 20) & 0x7ff);
+         if (i > 49)
+         {
+            t = r;
+            w = fn * pio2_3;
+            r = t - w;
+            w = fn * pio2_3t - ((t - r) - w);
+            y[0] = r - w;
+         }
+      }
+   }
+   y[1] = (r - y[0]) - w;
+   if (hx < 0)
+   {
+      y[0] = -y[0];
+      y[1] = -y[1];
+      return -n;
+   }
+   return n;
+}
+
+/* k_cos.c: cos(x + y) on [-pi/4, pi/4], 1 - x^2/2 + x^4 C(x^2) (C1..C6), with the x^2/2 term split off against
+ * qx (x/4 from the high word, or 0.28125 above 0.78125) for |x| >= 0.3. */
+static double kcos(double x, double y)
+{
+   const double C1 = 4.16666666666666019037e-02, C2 = -1.38888888888741095749e-03, C3 = 2.48015872894767294178e-05,
+                C4 = -2.75573143513906633035e-07, C5 = 2.08757232129817482790e-09,
+                C6 = -1.13596475577881948265e-11;
+   const int32_t ix = hi_of(x) & 0x7fffffff;
+   if (ix < 0x3e400000 && (int)x == 0)
+      return 1.0;
+   const double z = x * x;
+   const double r = z * (C1 + z * (C2 + z * (C3 + z * (C4 + z * (C5 + z * C6)))));
+   if (ix < 0x3fd33333)
+      return 1.0 - (0.5 * z - (z * r - x * y));
+   const double qx = ix > 0x3fe90000 ? 0.28125 : of_words((uint32_t)(ix - 0x00200000), 0);
+   const double hz = 0.5 * z - qx;
+   const double a = 1.0 - qx;
+   return a - (hz - (z * r - x * y));
+}
+
+/* k_sin.c: sin(x + y) on [-pi/4, pi/4], x + x^3 (S1 + x^2 S(x^2)) (S1..S6); iy = 0 when y is 0. */
+static double ksin(double x, double y, int iy)
+{
+   const double S1 = -1.66666666666666324348e-01, S2 = 8.33333333332248946124e-03, S3 = -1.98412698298579493134e-04,
+                S4 = 2.75573137070700676789e-06, S5 = -2.50507602534068634195e-08, S6 = 1.58969099521155010221e-10;
+   const int32_t ix = hi_of(x) & 0x7fffffff;
+   if (ix < 0x3e400000 && (int)x == 0)
+      return x;
+   const double z = x * x;
+   const double v = z * x;
+   const double r = S2 + z * (S3 + z * (S4 + z * (S5 + z * S6)));
+   if (iy == 0)
+      return x + v * (S1 + z * r);
+   return x - ((z * (0.5 * y - v * r) - y) - v * S1);
+}
+
+/* s_cos.c / s_sin.c: reduce by n pi/2, then the kernel of quadrant n mod 4. */
+double orc_jsm_cos(double x)
+{
+   const int32_t ix = hi_of(x) & 0x7fffffff;
+   if (ix <= 0x3fe921fb)
+      return kcos(x, 0.0);
+   if (ix >= 0x7ff00000)
+      return x - x;
+   double y[2];
+   const int32_t n = rem_pio2(x, y);
+   switch (n & 3)
+   {
+   case 0: return kcos(y[0], y[1]);
+   case 1: return -ksin(y[0], y[1], 1);
+   case 2: return -kcos(y[0], y[1]);
+   default: return ksin(y[0], y[1], 1);
+   }
+}
+
+double orc_jsm_sin(double x)
+{
+   const int32_t ix = hi_of(x) & 0x7fffffff;
+   if (ix <= 0x3fe921fb)
+      return ksin(x, 0.0, 0);
+   if (ix >= 0x7ff00000)
+      return x - x;
+   double y[2];
+   const int32_t n = rem_pio2(x, y);
+   switch (n & 3)
+   {
+   case 0: return ksin(y[0], y[1], 1);
+   case 1: return kcos(y[0], y[1]);
+   case 2: return -ksin(y[0], y[1], 1);
+   default: return -kcos(y[0], y[1]);
+   }
+}
+
+/* ======================================================================= */
+/* Dye schedules: core/src/dyes.cpp:254-321 (LogNormal, Poisson,           */
+/* DyeSchedule), :440-478 (LabelSchedule), :537-657 (PersistentGen and its */
+/* two callers), for one set of kinetics (no history). The channel        */
+/* numbers are core/src/dyes.h:45-57.                                      */
+/* ======================================================================= */
+
+enum
+{
+   CH_ACT = 4,
+   CH_PERSIST = 5,
+   CH_INIT_ON = 7,
+   CH_AUX = 8,
+   CH_SCHED0 = 16,
+   CH_SCHED_STRIDE = 8,
+   MAX_BLINKS = 1000
+};
+
+static double unit_of(uint32_t a) { return ((double)(a >> 9) + 0.5) * 1.1920928955078125e-7; }
+
+static double dye_u(uint32_t h1, int32_t k, int32_t n, uint32_t ch)
+{
+   uint32_t v[4] = {h1, (uint32_t)k, (uint32_t)n, ch};
+   orc_pcg4d(v);
+   return unit_of(v[0]);
+}
+
+static const double kJsPi = 3.141592653589793;
+
+/* LogNormalFor / LogNormalMean1 (dyes.cpp:257-268): mean 1, CV cv, Box-Muller normal from (u1, u2). */
+static void lognormal_for(double cv, double* sigma, double* mu)
+{
+   const double s2 = orc_jsm_log(1 + cv * cv);
+   *sigma = sqrt(s2);
+   *mu = -s2 / 2;
+}
+
+static double lognormal_mean1(double sigma, double mu, double u1, double u2)
+{
+   const double z = sqrt(-2 * orc_jsm_log(u1)) * orc_jsm_cos(2 * kJsPi * u2);
+   return orc_jsm_exp(mu + sigma * z);
+}
+
+/* PoissonInverse / PoissonFromUniform (dyes.cpp:270-290). */
+static long poisson_inverse(double m, double expM, double u)
+{
+   double p = expM, F = p;
+   long c = 0;
+   while (u > F && c < 1000)
+   {
+      c++;
+      p *= m / c;
+      F += p;
+   }
+   return c;
+}
+
+static long poisson_from_uniform(double m, double u, double u2)
+{
+   if (!(m > 0))
+      return 0;
+   if (m > 30)
+   {
+      const double z = sqrt(-2 * orc_jsm_log(u)) * orc_jsm_cos(2 * kJsPi * u2);
+      const long c = (long)floor(m + sqrt(m) * z + 0.5);
+      return c > 0 ? c : 0;
+   }
+   return poisson_inverse(m, orc_jsm_exp(-m), u);
+}
+
+static void put_blink(double* out, int32_t cap, int32_t i, double tOn, double tOff, double b)
+{
+   if (i < cap)
+   {
+      out[3 * i] = tOn;
+      out[3 * i + 1] = tOff;
+      out[3 * i + 2] = b;
+   }
+}
+
+/* DyeScheduleFrom (dyes.cpp:300-321): blinks from time t, appended at out[count..]; returns the new count. */
+static int32_t schedule_from(uint32_t h1, int32_t k, int32_t n, const orc_kinetics* kin, double t, double tMax,
+                             double* out, int32_t cap, int32_t count)
+{
+   const double pBleach = fmin(1.0, fmax(0.01, kin->bleachProb));
+   const double cv = fmax(0.0, kin->photonCV);
+   double sigma, mu;
+   lognormal_for(cv, &sigma, &mu);
+   for (int j = 0; j < MAX_BLINKS; j++)
+   {
+      if (t >= tMax)
+         break;
+      const uint32_t base = CH_SCHED0 + (uint32_t)j * CH_SCHED_STRIDE;
+      const double on = -orc_jsm_log(dye_u(h1, k, n, base + 0)) * kin->onSec;
+      double b = 1;
+      if (cv > 0)
+      {
+         const double u1 = dye_u(h1, k, n, base + 1);
+         const double u2 = dye_u(h1, k, n, base + 2);
+         b = lognormal_mean1(sigma, mu, u1, u2);
+      }
+      put_blink(out, cap, count++, t, t + on, b);
+      if (dye_u(h1, k, n, base + 3) < pBleach)
+         break;
+      t += on - orc_jsm_log(dye_u(h1, k, n, base + 4)) * kin->offSec;
+   }
+   return count;
+}
+
+/* LabelSchedule's blinks (dyes.cpp:440-471): dSTORM shifts DyeSchedule by the initial ON draw, PALM runs it from
+ * 0, DNA-PAINT and WideField have none here. */
+int32_t orc_label_blinks(uint32_t h1, int32_t k, int32_t n, int32_t mode, const orc_kinetics* kin, double tMax,
+                         double* out, int32_t cap)
+{
+   if (mode == 2 || mode == 3)
+      return 0;
+   double shift = 0.0;
+   if (mode == 0 && kin->initialOnSec > 0)
+      shift = -orc_jsm_log(dye_u(h1, k, n, CH_INIT_ON)) * kin->initialOnSec;
+   /* DyeSchedule (dyes.cpp:293-298) with tMax - shift */
+   if (!(kin->activationRatePerSec > 0))
+      return 0;
+   const double t = -orc_jsm_log(dye_u(h1, k, n, CH_ACT)) / kin->activationRatePerSec;
+   const int32_t count = schedule_from(h1, k, n, kin, t, tMax - shift, out, cap, 0);
+   if (shift != 0)
+      for (int32_t i = 0; i < count && i < cap; ++i)
+      {
+         out[3 * i] += shift;
+         out[3 * i + 1] += shift;
+      }
+   return count;
+}
+
+/* LabelSchedule's continuous windows (dyes.cpp:447-477): WideField ALWAYS_ON (3) with the AUX draw, dSTORM
+ * INITIAL_ON (2) for a nonzero initial ON, PALM with a pre state PRE (1) until the first activation. */
+int32_t orc_label_cont(uint32_t h1, int32_t k, int32_t n, int32_t mode, int32_t preState, const orc_kinetics* kin,
+                       double w[4])
+{
+   if (mode == 3)
+   {
+      w[0] = 0.0;
+      w[1] = INFINITY;
+      w[2] = 3;
+      w[3] = -orc_jsm_log(dye_u(h1, k, n, CH_AUX));
+      return 1;
+   }
+   if (mode == 0 && kin->initialOnSec > 0)
+   {
+      const double shift = -orc_jsm_log(dye_u(h1, k, n, CH_INIT_ON)) * kin->initialOnSec;
+      if (shift != 0)
+      {
+         w[0] = 0.0;
+         w[1] = shift;
+         w[2] = 2;
+         w[3] = 0.0;
+         return 1;
+      }
+      return 0;
+   }
+   if (mode == 1 && preState)
+   {
+      w[0] = 0.0;
+      w[1] = kin->activationRatePerSec > 0
+                ? -orc_jsm_log(dye_u(h1, k, n, CH_ACT)) / kin->activationRatePerSec
+                : INFINITY;
+      w[2] = 1;
+      w[3] = -orc_jsm_log(dye_u(h1, k, n, CH_AUX));
+      return 1;
+   }
+   return 0;
+}
+
+/* PersistentGen (dyes.cpp:566-626) for one set of kinetics, with PersistentBlinks' window filter and bin range
+ * (dyes.cpp:636-646) or PersistentBlinksInBins' unfiltered bins (dyes.cpp:648-655). */
+int32_t orc_persistent_blinks(uint32_t h1, int32_t k, int32_t n, const orc_kinetics* kin, int32_t bins, double t0,
+                              double t1, int64_t binLo, int64_t binHi, double* out, int32_t cap)
+{
+   const double maxOn = 20.0 * kin->onSec;
+   long b0 = (long)binLo, b1 = (long)binHi;
+   if (!bins)
+   {
+      if (!(t1 > t0))
+         return 0;
+      const long f = (long)floor((t0 - maxOn) / 1.0);
+      b0 = f > 0 ? f : 0;
+      b1 = (long)floor(t1 / 1.0);
+   }
+   if (!(kin->activationRatePerSec > 0))
+      return 0;
+   uint32_t v[4] = {h1, (uint32_t)k, (uint32_t)n, CH_PERSIST};
+   orc_pcg4d(v);
+   const uint32_t key = v[0];
+   /* BinKineticsOf (dyes.cpp:544-558) */
+   const double rate = kin->activationRatePerSec, onSec = kin->onSec;
+   const double cv = fmax(0.0, kin->photonCV);
+   double sigma, mu;
+   lognormal_for(cv, &sigma, &mu);
+   const double m = rate * 1.0;
+   const int small = m <= 30;
+   const double expM = small ? orc_jsm_exp(-m) : 0.0;
+   int32_t count = 0;
+   for (long b = b0 > 0 ? b0 : 0; b <= b1; b++)
+   {
+      const uint32_t bin = (uint32_t)b;
+      const long c = small ? poisson_inverse(m, expM, dye_u(key, (int32_t)bin, 0, 0))
+                           : poisson_from_uniform(m, dye_u(key, (int32_t)bin, 0, 0), dye_u(key, (int32_t)bin, 0, 1));
+      for (long j = 0; j < c; j++)
+      {
+         const uint32_t jj = (uint32_t)j;
+         const double u = dye_u(key, (int32_t)bin, (int32_t)jj, 2);
+         const double tOn = (bin + u) * 1.0;
+         double on = -orc_jsm_log(dye_u(key, (int32_t)bin, (int32_t)jj, 3)) * onSec;
+         on = fmin(maxOn, on);
+         if (!bins && !(tOn < t1 && tOn + on > t0))
+            continue;
+         double br = 1;
+         if (cv > 0)
+         {
+            const double u1 = dye_u(key, (int32_t)bin, (int32_t)jj, 4);
+            const double u2 = dye_u(key, (int32_t)bin, (int32_t)jj, 5);
+            br = lognormal_mean1(sigma, mu, u1, u2);
+         }
+         if (count < cap)
+         {
+            double* o = out + 5 * (size_t)count;
+            o[0] = tOn;
+            o[1] = tOn + on;
+            o[2] = br;
+            o[3] = bin;
+            o[4] = jj;
+         }
+         count++;
+      }
+   }
+   return count;
 }

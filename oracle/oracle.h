@@ -171,6 +171,31 @@ void orc_binned_gauss(double sigmaPx, int32_t u, int32_t R, float* out);
 void orc_binned_render(const float* kernels, int32_t nz, int32_t u, int32_t R, int32_t W, int32_t H, const double* em,
                        int32_t n, float* img);
 
+/* ---- JS-compatible FP64 math (core/src/jsmath_fdlibm.cpp: fdlibm 5.3 as V8 runs it) ----
+ * sin/cos restate the reductions for |x| <= 2^19 pi/2 only (beyond: NaN). */
+double orc_jsm_sin(double x);
+double orc_jsm_cos(double x);
+double orc_jsm_exp(double x);
+double orc_jsm_log(double x);
+
+/* ---- dye schedules (core/src/dyes.h:129-261, core/src/dyes.cpp:254-657), one set of kinetics ---- */
+typedef struct
+{
+   double activationRatePerSec, onSec, offSec, bleachProb, photonCV, initialOnSec;
+} orc_kinetics;
+/* LabelSchedule's blinks of dye (h1, k, n) under mode (0 dSTORM, 1 PALM, 2 DNA-PAINT, 3 WideField): out holds
+ * cap x 3 doubles (tOn, tOff, brightness); returns the count (written up to cap). */
+int32_t orc_label_blinks(uint32_t h1, int32_t k, int32_t n, int32_t mode, const orc_kinetics* kin, double tMax,
+                         double* out, int32_t cap);
+/* LabelSchedule's continuous window (at most one for a single set of kinetics): 1 and w = tOn, tOff, state, aux,
+ * or 0. */
+int32_t orc_label_cont(uint32_t h1, int32_t k, int32_t n, int32_t mode, int32_t preState, const orc_kinetics* kin,
+                       double w[4]);
+/* PersistentGen: window (t0, t1) when bins is 0 (PersistentBlinks), else every blink of bins [binLo, binHi]
+ * (PersistentBlinksInBins). out: cap x 5 doubles (tOn, tOff, brightness, bin, j); returns the count. */
+int32_t orc_persistent_blinks(uint32_t h1, int32_t k, int32_t n, const orc_kinetics* kin, int32_t bins, double t0,
+                              double t1, int64_t binLo, int64_t binHi, double* out, int32_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,8 @@
 #include "ZernikePsf.h"
 #include "Illumination.h"
 #include "Parallel.h"
+#include "dyes.h"
+#include "jsmath.h"
 
 #include <algorithm>
 #include <cstdint>
@@ -441,6 +443,111 @@ int32_t ref_binned_render(const float* planes, int32_t nz, int32_t size, int32_t
       std::memcpy(img + f * npx, frame.data(), npx * sizeof(float));
    }
    return 0;
+}
+
+// jsm::sin / cos / exp / log (core/src/jsmath_fdlibm.cpp) of n inputs: fn 0 sin, 1 cos, 2 exp, 3 log.
+void ref_jsm(int32_t fn, const double* x, int64_t n, double* out)
+{
+   for (int64_t i = 0; i < n; ++i)
+      out[i] = fn == 0 ? isc::jsm::sin(x[i]) : fn == 1 ? isc::jsm::cos(x[i]) : fn == 2 ? isc::jsm::exp(x[i])
+                                                                                  : isc::jsm::log(x[i]);
+}
+
+static isc::Label MakeLabel(int32_t mode, int32_t preState, const orc_kinetics* kin)
+{
+   isc::Label l;
+   l.mode = static_cast<isc::LabelMode>(mode);
+   l.preState = preState != 0;
+   l.kin.activationRatePerSec = kin->activationRatePerSec;
+   l.kin.onSec = kin->onSec;
+   l.kin.offSec = kin->offSec;
+   l.kin.bleachProb = kin->bleachProb;
+   l.kin.photonCV = kin->photonCV;
+   l.kin.initialOnSec = kin->initialOnSec;
+   return l;
+}
+
+// LabelSchedule (core/src/dyes.cpp:440) for nDyes addresses (h1, k, n as 3 x uint32), blinks only: counts[i] per
+// dye, out = every blink (tOn, tOff, brightness) in dye order, up to cap. Returns the total.
+int64_t ref_label_blinks(const uint32_t* dyes, int32_t nDyes, int32_t mode, const orc_kinetics* kin, double tMax,
+                         int32_t* counts, double* out, int64_t cap)
+{
+   const isc::Label l = MakeLabel(mode, 0, kin);
+   std::vector<isc::Blink> b;
+   int64_t total = 0;
+   for (int32_t i = 0; i < nDyes; ++i)
+   {
+      b.clear();
+      isc::LabelSchedule(dyes[3 * i], (int32_t)dyes[3 * i + 1], (int32_t)dyes[3 * i + 2], l, &b, nullptr, tMax);
+      counts[i] = (int32_t)b.size();
+      for (const isc::Blink& x : b)
+      {
+         if (total < cap)
+         {
+            out[3 * total] = x.tOn;
+            out[3 * total + 1] = x.tOff;
+            out[3 * total + 2] = x.brightness;
+         }
+         ++total;
+      }
+   }
+   return total;
+}
+
+// LabelSchedule's continuous windows for nDyes addresses: has[i] windows (0 or 1), w = 4 doubles per dye.
+void ref_label_cont(const uint32_t* dyes, int32_t nDyes, int32_t mode, int32_t preState, const orc_kinetics* kin,
+                    int32_t* has, double* w)
+{
+   const isc::Label l = MakeLabel(mode, preState, kin);
+   std::vector<isc::ContWindow> c;
+   for (int32_t i = 0; i < nDyes; ++i)
+   {
+      c.clear();
+      isc::LabelSchedule(dyes[3 * i], (int32_t)dyes[3 * i + 1], (int32_t)dyes[3 * i + 2], l, nullptr, &c);
+      has[i] = (int32_t)c.size();
+      if (!c.empty())
+      {
+         w[4 * i] = c[0].tOn;
+         w[4 * i + 1] = c[0].tOff;
+         w[4 * i + 2] = c[0].state;
+         w[4 * i + 3] = c[0].aux;
+      }
+   }
+}
+
+// PersistentBlinks (bins 0: window t0, t1) or PersistentBlinksInBins (bins [binLo, binHi]) (core/src/dyes.cpp:636,
+// :648) for nDyes addresses: counts per dye, out = 5 doubles per blink (tOn, tOff, brightness, bin, j; bin and j 0
+// for the window form) up to cap. Returns the total.
+int64_t ref_persistent_blinks(const uint32_t* dyes, int32_t nDyes, const orc_kinetics* kin, int32_t bins, double t0,
+                              double t1, int64_t binLo, int64_t binHi, int32_t* counts, double* out, int64_t cap)
+{
+   const isc::Label l = MakeLabel(2, 0, kin);
+   int64_t total = 0;
+   for (int32_t i = 0; i < nDyes; ++i)
+   {
+      const uint32_t h1 = dyes[3 * i];
+      const int32_t k = (int32_t)dyes[3 * i + 1], n = (int32_t)dyes[3 * i + 2];
+      std::vector<double> rec;
+      if (bins)
+      {
+         std::vector<isc::BinBlink> b;
+         isc::PersistentBlinksInBins(h1, k, n, l.kin, (long)binLo, (long)binHi, b);
+         for (const isc::BinBlink& x : b)
+            rec.insert(rec.end(), { x.tOn, x.tOff, x.brightness, (double)x.bin, (double)x.j });
+      }
+      else
+      {
+         std::vector<isc::Blink> b;
+         isc::PersistentBlinks(h1, k, n, l.kin, t0, t1, b);
+         for (const isc::Blink& x : b)
+            rec.insert(rec.end(), { x.tOn, x.tOff, x.brightness, 0.0, 0.0 });
+      }
+      counts[i] = (int32_t)(rec.size() / 5);
+      for (size_t r = 0; r < rec.size(); r += 5, ++total)
+         if (total < cap)
+            std::memcpy(out + 5 * total, rec.data() + r, 5 * sizeof(double));
+   }
+   return total;
 }
 
 } // extern "C"

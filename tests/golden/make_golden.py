@@ -19,6 +19,9 @@ Outputs (inputs and expected outputs only):
                      CPU images
   binned_ref.npz     BinnedBlinkRenderer Setup + Render frames (kernel planes at upscale 1, 2, 4, a halo cut, the
                      Gaussian), with edge-case emitters, added to non-zero images
+  dyes_ref.npz       LabelSchedule blinks and continuous windows (dSTORM, PALM, DNA-PAINT, WideField; several
+                     kinetics), PersistentBlinks and PersistentBlinksInBins of seeded dye addresses; jsm:: values of
+                     seeded inputs around the reduction boundaries
 """
 from __future__ import annotations
 
@@ -40,7 +43,7 @@ REF = "/root/reference"
 
 
 def rng_vectors():
-    rng, stream = [], []
+    rng, stream, math = [], [], []
     with open(os.path.join(REF, "spec/golden/ref.txt")) as f:
         for line in f:
             t = line.split()
@@ -52,8 +55,14 @@ def rng_vectors():
             elif t[0] == "stream":
                 stream.append({"seed": int(t[1]), "cx": int(t[2]), "cy": int(t[3]), "base": int(t[4]),
                                "draws": t[5:]})
+            elif t[0] == "m":
+                # MathCheck (tests/parity/parity_main.cpp:41-58): the bits of sin(x), cos(x), atan2, hypot, atan(x),
+                # exp((u1 - 0.5) 1500), log(u2 s), asin, cbrt, hypot3 of line i's hashed inputs
+                math.append({"i": int(t[1]), "bits": t[2:12]})
     with open(os.path.join(HERE, "rng_vectors.json"), "w") as f:
-        json.dump({"source": "spec/golden/ref.txt", "rng": rng, "stream": stream}, f, indent=1)
+        json.dump({"source": "spec/golden/ref.txt", "rng": rng, "stream": stream,
+                   "math_columns": ["sin", "cos", "atan2", "hypot", "atan", "exp", "log", "asin", "cbrt", "hypot3"],
+                   "math": math}, f, indent=1)
 
 
 def req(**kw):
@@ -265,6 +274,79 @@ def binned(R, P):
     np.savez_compressed(os.path.join(HERE, "binned_ref.npz"), **out)
 
 
+# (name, mode, preState, kinetics, tMax) of the LabelSchedule cases
+DYE_LABEL_CASES = [
+    ("dstorm_default", 0, 0, dict(), 1000.0),
+    ("dstorm_multi", 0, 0, dict(activationRatePerSec=0.2, bleachProb=0.05, initialOnSec=3.0, photonCV=0.4), 300.0),
+    ("dstorm_nocv", 0, 0, dict(activationRatePerSec=1.0, bleachProb=0.02, photonCV=0.0), 40.0),
+    ("palm", 1, 1, dict(activationRatePerSec=0.05, bleachProb=0.3, photonCV=0.0), 500.0),
+    ("palm_cv", 1, 0, dict(activationRatePerSec=0.5, bleachProb=0.001, offSec=0.2, photonCV=1.5), 8.0),
+    ("palm_long", 1, 0, dict(activationRatePerSec=50.0, bleachProb=0.0, onSec=0.001, offSec=0.001), 1e9),
+    ("palm_off", 1, 1, dict(activationRatePerSec=0.0), 100.0),
+    ("paint", 2, 0, dict(), 100.0),
+    ("widefield", 3, 0, dict(), 100.0),
+]
+# (name, kinetics, t0, t1, bins)
+DYE_PERSIST_CASES = [
+    ("win", dict(activationRatePerSec=0.3, onSec=0.2), 10.0, 20.0, None),
+    ("win_nocv", dict(activationRatePerSec=2.0, photonCV=0.0), 0.0, 5.5, None),
+    ("win_dense", dict(activationRatePerSec=45.0, onSec=0.01), 3.0, 4.0, None),
+    ("win_empty", dict(activationRatePerSec=1.0), 5.0, 5.0, None),
+    ("bins", dict(activationRatePerSec=0.7), 0.0, 0.0, (3, 9)),
+    ("bins_dense", dict(activationRatePerSec=40.0), 0.0, 0.0, (0, 0)),
+]
+
+
+def dye_addresses(seed, n):
+    rng = np.random.default_rng(seed)
+    d = np.zeros((n, 3), np.uint32)
+    d[:, 0] = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    d[:, 1] = rng.integers(0, 64, n).astype(np.uint32)
+    d[:, 2] = rng.integers(-50, 2 ** 31 - 1, n).astype(np.int64).astype(np.uint32)
+    d[:4] = [[0, 0, 0], [0xFFFFFFFF, 63, 0x7FFFFFFF], [1249, 12, 0xFFFFFFFF], [7, 0, 1]]
+    return d
+
+
+def jsm_inputs():
+    """Seeded inputs of the jsm:: checks: each reduction branch and its boundaries, the exp/log special cases."""
+    rng = np.random.default_rng(77)
+    pio2 = np.pi / 2
+    near = np.array([n * pio2 + d for n in range(1, 70) for d in (0.0, 1e-12, -1e-12, 3e-9)])
+    trig = np.concatenate([rng.uniform(-800000, 800000, 1500), rng.uniform(-10, 10, 1500),
+                           rng.uniform(-1e-8, 1e-8, 500), near, -near, [0.0, 0.3, 0.78125, 0.7853981633974483,
+                                                                       2.356194490192345, 823549.6, -823549.6]])
+    ex = np.concatenate([rng.uniform(-745, 710, 1500), rng.uniform(-2, 2, 1500), rng.uniform(-1e-9, 1e-9, 200),
+                         [1.0, 0.0, 709.78, -745.1, 1e-300, 0.34657359027997264, 1.0397207708399179, -1.0,
+                          709.782712893384, -708.39, -744.4, -740.0, 800.0, -800.0]])
+    lg = np.concatenate([rng.uniform(0, 1, 1500), np.exp(rng.uniform(-700, 700, 1500)), 1 + rng.uniform(-1e-6, 1e-6,
+                         500), [1.0, 2.0, 0.5, 1e-310, 5e-324, np.e, 1.0000009536743164, 0.7071067811865476,
+                                1.4142135623730951, 1.1920928955078125e-7 * 0.5]])
+    return {"sin": trig, "cos": trig, "exp": ex, "log": lg}
+
+
+def dyes(R):
+    out = {}
+    d = dye_addresses(11, 800)
+    out["dyes"] = d
+    for name, mode, pre, kw, t_max in DYE_LABEL_CASES:
+        kin = O.kinetics(**kw)
+        dd = d[:100] if name == "palm_long" else d
+        c, b = R.label_blinks(dd, mode, kin, t_max)
+        out["label_" + name + "_counts"] = c
+        out["label_" + name + "_blinks"] = b
+        has, w = R.label_cont(dd, mode, pre, kin)
+        out["cont_" + name + "_has"] = has
+        out["cont_" + name + "_w"] = w * (has[:, None] != 0)
+    for name, kw, t0, t1, bins in DYE_PERSIST_CASES:
+        c, b = R.persistent_blinks(d[:300], O.kinetics(**kw), t0, t1, bins)
+        out["persist_" + name + "_counts"] = c
+        out["persist_" + name + "_blinks"] = b
+    for fn, x in jsm_inputs().items():
+        out["jsm_" + fn + "_x"] = x
+        out["jsm_" + fn + "_y"] = R.jsm(fn, x)
+    np.savez_compressed(os.path.join(HERE, "dyes_ref.npz"), **out)
+
+
 def main():
     R = O.Reference()
     maps_and_geometry(R)
@@ -276,6 +358,7 @@ def main():
     splat(R, P)
     wf(R, P)
     binned(R, P)
+    dyes(R)
     print("fixtures written to", HERE)
 
 
