@@ -435,7 +435,8 @@ int32_t iscg_label_cont(iscg_context* h, const iscg_dye_address* dyes, int32_t n
                         const iscg_kinetics* kin, iscg_cont_window* out, uint8_t* has);
 
 /* Replaces isc::PersistentBlinks (core/src/dyes.cpp:636-646, dyes.h:250): each persistent (DNA-PAINT) dye's blinks
- * overlapping the window (t0, t1), by time bin then index, one set of kinetics. */
+ * overlapping the window (t0, t1), by time bin then index, one set of kinetics. Both persistent calls take at most
+ * 2^20 time bins per dye and an activation rate of at most 1e5 per bin (else -1). */
 int32_t iscg_persistent_blinks(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, const iscg_kinetics* kin,
                                double t0, double t1, int64_t* offsets, iscg_blink* out, int64_t capacity,
                                int64_t* total);

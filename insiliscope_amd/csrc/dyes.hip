@@ -38,6 +38,30 @@ __device__ __forceinline__ double draw(uint32_t a, uint32_t b, uint32_t c, uint3
    return CounterRng::unit_of(a);
 }
 
+// A dye's draws share (h1, k, n): their first LCG step is taken once per dye, each draw then takes its channel's.
+struct Address
+{
+   uint32_t a, b, c;
+   __device__ __forceinline__ static Address of(uint32_t h1, uint32_t k, uint32_t n)
+   {
+      return Address{h1 * 1664525u + 1013904223u, k * 1664525u + 1013904223u, n * 1664525u + 1013904223u};
+   }
+   // Unit(Pcg4d(h1, k, n, ch).a): pcg4d after its first step
+   __device__ __forceinline__ double draw(uint32_t ch) const
+   {
+      uint32_t x = a, y = b, z = c, w = ch * 1664525u + 1013904223u;
+      x += y * w;
+      y += z * x;
+      z += x * y;
+      w += y * z;
+      x ^= x >> 16;
+      y ^= y >> 16;
+      w ^= w >> 16;
+      x += y * w;  // the rest of the second round does not reach the first word
+      return CounterRng::unit_of(x);
+   }
+};
+
 // LogNormalFor / LogNormalMean1 (core/src/dyes.cpp:257-268): a mean-1 log-normal with CV cv from two uniforms.
 struct LogNormal
 {
@@ -77,31 +101,10 @@ __device__ __forceinline__ long long poisson_normal(double m, double u, double u
    return c > 0 ? c : 0;
 }
 
-// The writers a schedule emits through: none (count pass) or records from a base pointer.
-struct NoSink
-{
-   static constexpr bool kWrite = false;
-   __device__ __forceinline__ void operator()(int, double, double, double, uint32_t, uint32_t) const {}
-};
-struct BlinkSink
-{
-   static constexpr bool kWrite = true;
-   iscg_blink* out;
-   __device__ __forceinline__ void operator()(int i, double tOn, double tOff, double b, uint32_t, uint32_t) const
-   {
-      out[i] = iscg_blink{tOn, tOff, b};
-   }
-};
-struct BinBlinkSink
-{
-   static constexpr bool kWrite = true;
-   iscg_bin_blink* out;
-   __device__ __forceinline__ void operator()(int i, double tOn, double tOff, double b, uint32_t bin,
-                                              uint32_t j) const
-   {
-      out[i] = iscg_bin_blink{tOn, tOff, b, bin, j};
-   }
-};
+// A schedule is a small state machine: init() draws what precedes the first blink, each step() makes one blink (or,
+// for the persistent form, one candidate blink or one bin change) and reports the end. The kernels advance one step
+// per loop iteration on every lane and hand a finished lane the next dye of its wave's chunk, so lanes whose
+// schedule ended early do not idle while the longest one of the wave runs on.
 
 // LabelSchedule's blinks (core/src/dyes.cpp:440-471 with DyeSchedule/DyeScheduleFrom, :293-321).
 struct LabelGen
@@ -110,104 +113,170 @@ struct LabelGen
    iscg_kinetics kin;
    double tMax;
    using Record = iscg_blink;
-   using Sink = BlinkSink;
 
-   template <class S>
-   __device__ int run(uint32_t h1, uint32_t k, uint32_t n, const S& sink) const
+   struct Consts
    {
-      if (mode == 2 || mode == 3)
-         return 0;
-      double shift = 0.0;
-      if (mode == 0 && kin.initialOnSec > 0)
-         shift = -jsm::log(draw(h1, k, n, CH_INIT_ON)) * kin.initialOnSec;
-      if (!(kin.activationRatePerSec > 0))
-         return 0;
-      const double tStop = tMax - shift;
-      double t = -jsm::log(draw(h1, k, n, CH_ACT)) / kin.activationRatePerSec;
-      const double pBleach = fmin(1.0, fmax(0.01, kin.bleachProb));
+      double pBleach, cv;
+      LogNormal ln;
+   };
+   struct State
+   {
+      Address at;
+      int j, count;
+      bool done;
+      double t, shift, tStop;
+   };
+
+   template <bool WRITE>
+   __device__ Consts consts() const
+   {
       const double cv = fmax(0.0, kin.photonCV);
-      const LogNormal ln(S::kWrite && cv > 0 ? cv : 0.0);
-      int count = 0;
-      for (int j = 0; j < kMaxBlinks; j++)
+      return Consts{fmin(1.0, fmax(0.01, kin.bleachProb)), cv, LogNormal(WRITE && cv > 0 ? cv : 0.0)};
+   }
+
+   __device__ State init(uint32_t h1, uint32_t k, uint32_t n) const
+   {
+      State s{Address::of(h1, k, n), 0, 0, true, 0.0, 0.0, 0.0};
+      if (mode == 2 || mode == 3)
+         return s;
+      if (mode == 0 && kin.initialOnSec > 0)
+         s.shift = -jsm::log(s.at.draw(CH_INIT_ON)) * kin.initialOnSec;
+      if (!(kin.activationRatePerSec > 0))
+         return s;
+      s.tStop = tMax - s.shift;
+      s.t = -jsm::log(s.at.draw(CH_ACT)) / kin.activationRatePerSec;
+      s.done = false;
+      return s;
+   }
+
+   // One blink j (core/src/dyes.cpp:307-320); true when the schedule has ended.
+   template <bool WRITE>
+   __device__ bool step(State& s, const Consts& c, Record* out) const
+   {
+      if (s.j >= kMaxBlinks || s.t >= s.tStop)
+         return true;
+      const uint32_t base = CH_SCHED0 + static_cast<uint32_t>(s.j) * CH_SCHED_STRIDE;
+      const double on = -jsm::log(s.at.draw(base + CH_ON)) * kin.onSec;
+      if constexpr (WRITE)
       {
-         if (t >= tStop)
-            break;
-         const uint32_t base = CH_SCHED0 + static_cast<uint32_t>(j) * CH_SCHED_STRIDE;
-         const double on = -jsm::log(draw(h1, k, n, base + CH_ON)) * kin.onSec;
-         if constexpr (S::kWrite)
-         {
-            double b = 1;
-            if (cv > 0)
-               b = ln(draw(h1, k, n, base + CH_BRIGHT1), draw(h1, k, n, base + CH_BRIGHT2));
-            if (shift != 0)
-               sink(count, t + shift, (t + on) + shift, b, 0, 0);
-            else
-               sink(count, t, t + on, b, 0, 0);
-         }
-         ++count;
-         if (draw(h1, k, n, base + CH_BLEACH) < pBleach)
-            break;
-         t += on - jsm::log(draw(h1, k, n, base + CH_OFF)) * kin.offSec;
+         double b = 1;
+         if (c.cv > 0)
+            b = c.ln(s.at.draw(base + CH_BRIGHT1), s.at.draw(base + CH_BRIGHT2));
+         // LabelSchedule adds the dSTORM shift to both ends after the schedule is made
+         out[s.count] = s.shift != 0 ? iscg_blink{s.t + s.shift, (s.t + on) + s.shift, b}
+                                     : iscg_blink{s.t, s.t + on, b};
       }
-      return count;
+      ++s.count;
+      if (s.at.draw(base + CH_BLEACH) < c.pBleach)
+         return true;
+      s.t += on - jsm::log(s.at.draw(base + CH_OFF)) * kin.offSec;
+      ++s.j;
+      return false;
    }
 };
 
 // PersistentGen (core/src/dyes.cpp:566-626) for one set of kinetics: bins [b0, b1], each bin's Poisson count of
 // blinks at (bin + u) s; the window form keeps those overlapping (t0, t1) (PersistentBlinks, :636-646).
-template <bool WINDOW, class RecordT, class SinkT>
+template <bool WINDOW, class RecordT>
 struct PersistGen
 {
    iscg_kinetics kin;
    double t0, t1;
    long long b0, b1;
    using Record = RecordT;
-   using Sink = SinkT;
 
-   template <class S>
-   __device__ int run(uint32_t h1, uint32_t k, uint32_t n, const S& sink) const
+   // BinKineticsOf (core/src/dyes.cpp:544-558)
+   struct Consts
    {
-      if (!(kin.activationRatePerSec > 0))
-         return 0;
-      const uint32_t key = [&] {
-         uint32_t a = h1, b = k, c = n, d = CH_PERSIST;
-         pcg4d(a, b, c, d);
-         return a;
-      }();
-      // BinKineticsOf (core/src/dyes.cpp:544-558)
-      const double onSec = kin.onSec;
+      double cv, maxOn, m, expM;
+      bool small;
+      LogNormal ln;
+   };
+   struct State
+   {
+      uint32_t key;
+      int count;
+      bool done;
+      long long b, c, j;
+   };
+
+   template <bool WRITE>
+   __device__ Consts consts() const
+   {
       const double cv = fmax(0.0, kin.photonCV);
-      const LogNormal ln(S::kWrite && cv > 0 ? cv : 0.0);
-      const double maxOn = kOnCap * kin.onSec;
       const double m = kin.activationRatePerSec * kBinSec;
       const bool small = m <= 30;
-      const double expM = small ? jsm::exp(-m) : 0.0;
-      int count = 0;
-      for (long long b = b0 > 0 ? b0 : 0; b <= b1; b++)
+      return Consts{cv, kOnCap * kin.onSec, m, small ? jsm::exp(-m) : 0.0, small,
+                    LogNormal(WRITE && cv > 0 ? cv : 0.0)};
+   }
+
+   __device__ long long bin_count(uint32_t key, uint32_t bin, const Consts& q) const
+   {
+      const double uc = draw(key, bin, 0, 0);
+      return q.small ? poisson_inverse(q.m, q.expM, uc) : poisson_normal(q.m, uc, draw(key, bin, 0, 1));
+   }
+
+   __device__ State init(uint32_t h1, uint32_t k, uint32_t n, const Consts& q) const
+   {
+      State s{0, 0, true, 0, 0, 0};
+      if (!(kin.activationRatePerSec > 0))
+         return s;
+      uint32_t a = h1, b = k, c = n, d = CH_PERSIST;
+      pcg4d(a, b, c, d);
+      s.key = a;
+      s.b = b0 > 0 ? b0 : 0;
+      if (s.b > b1)
+         return s;
+      s.c = bin_count(s.key, static_cast<uint32_t>(s.b), q);
+      s.done = false;
+      return s;
+   }
+
+   // Blink j of bin b, or the next bin; true when past b1.
+   template <bool WRITE>
+   __device__ bool step(State& s, const Consts& q, Record* out) const
+   {
+      if (s.j >= s.c)
       {
-         const uint32_t bin = static_cast<uint32_t>(b);
-         const double uc = draw(key, bin, 0, 0);
-         const long long c = small ? poisson_inverse(m, expM, uc) : poisson_normal(m, uc, draw(key, bin, 0, 1));
-         for (long long j = 0; j < c; j++)
-         {
-            const uint32_t jj = static_cast<uint32_t>(j);
-            const double tOn = (bin + draw(key, bin, jj, 2)) * kBinSec;
-            const double on = fmin(maxOn, -jsm::log(draw(key, bin, jj, 3)) * onSec);
-            if (WINDOW && !(tOn < t1 && tOn + on > t0))
-               continue;
-            if constexpr (S::kWrite)
-            {
-               double br = 1;
-               if (cv > 0)
-                  br = ln(draw(key, bin, jj, 4), draw(key, bin, jj, 5));
-               sink(count, tOn, tOn + on, br, bin, jj);
-            }
-            ++count;
-         }
+         if (++s.b > b1)
+            return true;
+         s.c = bin_count(s.key, static_cast<uint32_t>(s.b), q);
+         s.j = 0;
+         return false;
       }
-      return count;
+      const uint32_t bin = static_cast<uint32_t>(s.b), jj = static_cast<uint32_t>(s.j);
+      ++s.j;
+      const double tOn = (bin + draw(s.key, bin, jj, 2)) * kBinSec;
+      const double on = fmin(q.maxOn, -jsm::log(draw(s.key, bin, jj, 3)) * kin.onSec);
+      if (WINDOW && !(tOn < t1 && tOn + on > t0))
+         return false;
+      if constexpr (WRITE)
+      {
+         double br = 1;
+         if (q.cv > 0)
+            br = q.ln(draw(s.key, bin, jj, 4), draw(s.key, bin, jj, 5));
+         if constexpr (WINDOW)
+            out[s.count] = iscg_blink{tOn, tOn + on, br};
+         else
+            out[s.count] = iscg_bin_blink{tOn, tOn + on, br, bin, jj};
+      }
+      ++s.count;
+      return false;
    }
 };
+
+__device__ __forceinline__ LabelGen::State gen_init(const LabelGen& g, const iscg_dye_address& d,
+                                                    const LabelGen::Consts&)
+{
+   return g.init(d.h1, static_cast<uint32_t>(d.k), static_cast<uint32_t>(d.n));
+}
+template <bool W, class R>
+__device__ __forceinline__ typename PersistGen<W, R>::State gen_init(const PersistGen<W, R>& g,
+                                                                     const iscg_dye_address& d,
+                                                                     const typename PersistGen<W, R>::Consts& q)
+{
+   return g.init(d.h1, static_cast<uint32_t>(d.k), static_cast<uint32_t>(d.n), q);
+}
 
 // Wave-wide inclusive prefix sum of v (64 lanes).
 __device__ __forceinline__ long long wave_inclusive(long long v)
@@ -246,21 +315,87 @@ __device__ __forceinline__ long long block_exclusive(long long v, long long& tot
    return before + inc - v;
 }
 
+// Dyes per wave and per workgroup: each wave runs the schedules of kChunk consecutive dyes. Finished lanes take new
+// dyes once at least kRefill of them are idle (or the wave has no work), so the init branch runs for several lanes at
+// a time instead of for one or two in most iterations.
+#ifndef ISCG_DYE_CHUNK
+#define ISCG_DYE_CHUNK 512
+#endif
+#ifndef ISCG_DYE_REFILL
+#define ISCG_DYE_REFILL 8
+#endif
+constexpr int kWaves = kDyeThreads / 64;
+constexpr int kChunk = ISCG_DYE_CHUNK;
+constexpr int kRefill = ISCG_DYE_REFILL;
+constexpr int kWgDyes = kWaves * kChunk;
+
+// The schedules of dyes [begin, end) on one wave. Count pass (WRITE false): counts[i] per dye, returns the lane's
+// sum. Fill pass: dye i's records at out + base[i - wgBegin] when they fit the capacity (counts from the count pass).
+template <bool WRITE, class Gen>
+__device__ long long wave_schedules(const Gen& g, const iscg_dye_address* __restrict__ dyes, int begin, int end,
+                                    int32_t* __restrict__ counts, const long long* base, int wgBegin,
+                                    typename Gen::Record* __restrict__ out, int64_t capacity)
+{
+   const typename Gen::Consts q = g.template consts<WRITE>();
+   const int lane = threadIdx.x & 63;
+   const unsigned long long below = (1ull << lane) - 1;
+   typename Gen::State st{};
+   typename Gen::Record* dst = nullptr;
+   int cur = -1, next = begin;
+   long long sum = 0;
+   for (;;)
+   {
+      const unsigned long long idle = __ballot(cur < 0);
+      const bool refill = __popcll(idle) >= kRefill || idle == ~0ull;
+      const bool need = refill && cur < 0;
+      const unsigned long long m = refill ? idle : 0ull;
+      if (need)
+      {
+         const int i = next + __popcll(m & below);
+         if (i < end)
+         {
+            bool work = true;
+            if constexpr (WRITE)
+            {
+               const long long off = base[i - wgBegin];
+               const int c = counts[i];
+               work = out != nullptr && c > 0 && off + c <= capacity;
+               dst = out + off;
+            }
+            if (work)
+            {
+               st = gen_init(g, dyes[i], q);
+               cur = i;
+            }
+         }
+      }
+      next += __popcll(m);
+      if (__ballot(cur >= 0) == 0 && next >= end)
+         break;
+      if (cur >= 0 && (st.done || g.template step<WRITE>(st, q, dst)))
+      {
+         if constexpr (!WRITE)
+         {
+            counts[cur] = st.count;
+            sum += st.count;
+         }
+         cur = -1;
+      }
+   }
+   return sum;
+}
+
 template <class Gen>
 __global__ __launch_bounds__(kDyeThreads) void dye_count_kernel(Gen g, const iscg_dye_address* __restrict__ dyes,
                                                                  int32_t nDyes, int32_t* __restrict__ counts,
                                                                  long long* __restrict__ blockSums)
 {
-   const int i = blockIdx.x * kDyeThreads + threadIdx.x;
-   int c = 0;
-   if (i < nDyes)
-   {
-      const iscg_dye_address d = dyes[i];
-      c = g.run(d.h1, static_cast<uint32_t>(d.k), static_cast<uint32_t>(d.n), NoSink{});
-      counts[i] = c;
-   }
+   const int wgBegin = blockIdx.x * kWgDyes;
+   const int begin = wgBegin + (threadIdx.x >> 6) * kChunk;
+   const int end = min(begin + kChunk, nDyes);
+   const long long mine = wave_schedules<false>(g, dyes, begin, end, counts, nullptr, wgBegin, nullptr, 0);
    long long total;
-   (void)block_exclusive<kDyeThreads / 64>(c, total);
+   (void)block_exclusive<kWaves>(mine, total);
    if (threadIdx.x == 0)
       blockSums[blockIdx.x] = total;
 }
@@ -290,24 +425,39 @@ __global__ __launch_bounds__(kScanThreads) void dye_scan_kernel(long long* __res
 
 template <class Gen>
 __global__ __launch_bounds__(kDyeThreads) void dye_fill_kernel(Gen g, const iscg_dye_address* __restrict__ dyes,
-                                                                int32_t nDyes, const int32_t* __restrict__ counts,
+                                                                int32_t nDyes, int32_t* __restrict__ counts,
                                                                 const long long* __restrict__ blockBase,
                                                                 int64_t* __restrict__ offsets,
                                                                 typename Gen::Record* __restrict__ out,
                                                                 int64_t capacity)
 {
-   const int i = blockIdx.x * kDyeThreads + threadIdx.x;
-   const int c = i < nDyes ? counts[i] : 0;
-   long long total;
-   const long long off = blockBase[blockIdx.x] + block_exclusive<kDyeThreads / 64>(c, total);
-   if (i >= nDyes)
-      return;
-   offsets[i] = off;
-   if (out && c > 0 && off + c <= capacity)
+   // the workgroup's offsets: each lane sums kPer consecutive counts, one workgroup scan, then per dye into LDS
+   constexpr int kPer = kWgDyes / kDyeThreads;
+   __shared__ long long base[kWgDyes];
+   const int wgBegin = blockIdx.x * kWgDyes;
+   const int first = wgBegin + threadIdx.x * kPer;
+   int c[kPer];
+   long long mine = 0;
+#pragma unroll
+   for (int r = 0; r < kPer; ++r)
    {
-      const iscg_dye_address d = dyes[i];
-      g.run(d.h1, static_cast<uint32_t>(d.k), static_cast<uint32_t>(d.n), typename Gen::Sink{out + off});
+      c[r] = first + r < nDyes ? counts[first + r] : 0;
+      mine += c[r];
    }
+   long long total;
+   long long off = blockBase[blockIdx.x] + block_exclusive<kWaves>(mine, total);
+#pragma unroll
+   for (int r = 0; r < kPer; ++r)
+   {
+      base[threadIdx.x * kPer + r] = off;
+      if (first + r < nDyes)
+         offsets[first + r] = off;
+      off += c[r];
+   }
+   __syncthreads();
+   const int begin = wgBegin + (threadIdx.x >> 6) * kChunk;
+   const int end = min(begin + kChunk, nDyes);
+   (void)wave_schedules<true>(g, dyes, begin, end, counts, base, wgBegin, out, capacity);
 }
 
 // LabelSchedule's continuous windows (core/src/dyes.cpp:447-477), one set of kinetics: at most one per dye.
@@ -377,7 +527,7 @@ int32_t run_schedule(iscg_context* h, const char* what, const Gen& g, const iscg
       capacity > 0 ? (typename Gen::Record*)stage_in(h, 3, out, (size_t)capacity * sizeof(*out)) : nullptr;
    if ((nDyes && !dd) || !doff || !dtot || (capacity > 0 && !dout))
       return -1;
-   const int nBlocks = nDyes ? (nDyes + kDyeThreads - 1) / kDyeThreads : 0;
+   const int nBlocks = nDyes ? (nDyes + kWgDyes - 1) / kWgDyes : 0;
    if (!ok(h, h->dyeCounts.reserve((size_t)(nDyes ? nDyes : 1) * sizeof(int32_t)), "dye counts") ||
        !ok(h, h->dyeBlocks.reserve((size_t)(nBlocks ? nBlocks : 1) * sizeof(long long)), "dye blocks"))
       return -1;
@@ -410,6 +560,27 @@ int32_t run_schedule(iscg_context* h, const char* what, const Gen& g, const iscg
 }
 
 bool bad_kinetics(const iscg_kinetics* kin) { return kin == nullptr; }
+
+// Every lane's loop over bins and candidate blinks must end within a launch: at most kMaxBins bins per dye and a
+// per-bin mean count of at most kMaxPerBin (the reference's callers ask for a few bins at a few blinks each).
+constexpr long long kMaxBins = 1 << 20;
+constexpr double kMaxPerBin = 1e5;
+
+bool persist_bounded(iscg_context* h, const char* what, const iscg_kinetics* kin, long long b0, long long b1)
+{
+   const long long lo = b0 > 0 ? b0 : 0;
+   if (b1 >= lo && b1 - lo >= kMaxBins)
+   {
+      fail(h, std::string(what) + ": more than 2^20 time bins per dye");
+      return false;
+   }
+   if (kin->activationRatePerSec * kBinSec > kMaxPerBin)
+   {
+      fail(h, std::string(what) + ": activation rate above 1e5 blinks per bin");
+      return false;
+   }
+   return true;
+}
 
 }  // namespace
 
@@ -444,7 +615,9 @@ int32_t iscg_persistent_blinks(iscg_context* h, const iscg_dye_address* dyes, in
       b0 = f > 0 ? f : 0;
       b1 = (long long)std::floor(t1 / kBinSec);
    }
-   const PersistGen<true, iscg_blink, BlinkSink> g{*kin, t0, t1, b0, b1};
+   if (!persist_bounded(h, "iscg_persistent_blinks", kin, b0, b1))
+      return -1;
+   const PersistGen<true, iscg_blink> g{*kin, t0, t1, b0, b1};
    return run_schedule(h, "iscg_persistent_blinks", g, dyes, nDyes, offsets, out, capacity, total);
 }
 
@@ -456,7 +629,9 @@ int32_t iscg_persistent_bins(iscg_context* h, const iscg_dye_address* dyes, int3
       return -1;
    if (bad_kinetics(kin) || (nDyes > 0 && !dyes))
       return fail(h, "iscg_persistent_bins: bad argument");
-   const PersistGen<false, iscg_bin_blink, BinBlinkSink> g{*kin, 0.0, 0.0, (long long)binLo, (long long)binHi};
+   if (!persist_bounded(h, "iscg_persistent_bins", kin, binLo, binHi))
+      return -1;
+   const PersistGen<false, iscg_bin_blink> g{*kin, 0.0, 0.0, (long long)binLo, (long long)binHi};
    return run_schedule(h, "iscg_persistent_bins", g, dyes, nDyes, offsets, out, capacity, total);
 }
 

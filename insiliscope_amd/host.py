@@ -958,6 +958,29 @@ class Label:
     preState: bool = False
 
 
+# core/src/dyes.h:33-41
+DYE_SALT, NPC_DYE_SALT, LAMIN_DYE_SALT = 0x9E3779B9, 0x4E504344, 0x4C414D44
+
+
+def DyeH1(seed: int, cx, cy, mtIndex, ctx: Context | None = None):
+    """isc::DyeH1 (core/src/dyes.cpp:175-178) of arrays of (cx, cy, mtIndex): Pcg4d(seed ^ DYE_SALT, cx, cy,
+    mtIndex).a, on the GPU."""
+    cx = np.atleast_1d(np.asarray(cx, np.int64))
+    return Pcg4d(np.full(cx.size, (seed ^ DYE_SALT) & 0xFFFFFFFF), cx, cy, mtIndex, ctx)[:, 0]
+
+
+def NpcH1(seed: int, cx, cy, ctx: Context | None = None):
+    """isc::NpcH1 (core/src/dyes.cpp:180-183): Pcg4d(seed ^ NPC_DYE_SALT, cx, cy, 0).a."""
+    cx = np.atleast_1d(np.asarray(cx, np.int64))
+    return Pcg4d(np.full(cx.size, (seed ^ NPC_DYE_SALT) & 0xFFFFFFFF), cx, cy, np.zeros(cx.size), ctx)[:, 0]
+
+
+def LaminH1(seed: int, cx, cy, ctx: Context | None = None):
+    """isc::LaminH1 (core/src/dyes.cpp:219-222): Pcg4d(seed ^ LAMIN_DYE_SALT, cx, cy, 0).a."""
+    cx = np.atleast_1d(np.asarray(cx, np.int64))
+    return Pcg4d(np.full(cx.size, (seed ^ LAMIN_DYE_SALT) & 0xFFFFFFFF), cx, cy, np.zeros(cx.size), ctx)[:, 0]
+
+
 def DyeAddresses(h1, k, n):
     """An iscg_dye_address array from per-dye h1 (DyeH1 of the block, core/src/dyes.cpp:175), k and n."""
     h1 = np.asarray(h1, np.int64)

@@ -498,6 +498,8 @@ class Reference:
         L.ref_binned_render.argtypes = [C.c_void_p, I32, I32, I32, D, D, I32, I32, I32, PD, PI32, I32, PF, PI32]
         L.ref_binned_render.restype = I32
         L.ref_jsm.argtypes = [I32, PD, I64, PD]
+        L.ref_dye_h1.argtypes = [I32, U32, I32, I32, I32]
+        L.ref_dye_h1.restype = U32
         PK = C.POINTER(Kinetics)
         L.ref_label_blinks.argtypes = [C.c_void_p, I32, I32, PK, D, PI32, PD, I64]
         L.ref_label_blinks.restype = I64
@@ -673,6 +675,10 @@ class Reference:
         if rc != 0:
             raise RuntimeError("BinnedBlinkRenderer::Setup failed")
         return out, (int(uR[0]), int(uR[1]))
+
+    def dye_h1(self, which, seed, cx, cy, mt=0):
+        """DyeH1 (which 0), NpcH1 (1), LaminH1 (2) of one block."""
+        return int(self.L.ref_dye_h1(which, seed & 0xFFFFFFFF, int(np.int32(cx)), int(np.int32(cy)), int(mt)))
 
     def jsm(self, fn: str, x):
         """The compiled reference's jsm:: function (fdlibm as V8 runs it) of each element of x."""

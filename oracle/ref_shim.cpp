@@ -445,6 +445,13 @@ int32_t ref_binned_render(const float* planes, int32_t nz, int32_t size, int32_t
    return 0;
 }
 
+// DyeH1 / NpcH1 / LaminH1 (core/src/dyes.cpp:175-222): which 0, 1, 2.
+uint32_t ref_dye_h1(int32_t which, uint32_t seed, int32_t cx, int32_t cy, int32_t mtIndex)
+{
+   return which == 0 ? isc::DyeH1(seed, cx, cy, mtIndex) : which == 1 ? isc::NpcH1(seed, cx, cy)
+                                                                     : isc::LaminH1(seed, cx, cy);
+}
+
 // jsm::sin / cos / exp / log (core/src/jsmath_fdlibm.cpp) of n inputs: fn 0 sin, 1 cos, 2 exp, 3 log.
 void ref_jsm(int32_t fn, const double* x, int64_t n, double* out)
 {

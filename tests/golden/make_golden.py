@@ -341,6 +341,13 @@ def dyes(R):
         c, b = R.persistent_blinks(d[:300], O.kinetics(**kw), t0, t1, bins)
         out["persist_" + name + "_counts"] = c
         out["persist_" + name + "_blinks"] = b
+    rng = np.random.default_rng(31)
+    h1 = []
+    for _ in range(64):
+        seed, cx, cy, mt = (int(rng.integers(0, 2 ** 32)), int(rng.integers(-2 ** 31, 2 ** 31)),
+                            int(rng.integers(-2 ** 31, 2 ** 31)), int(rng.integers(0, 4000)))
+        h1.append([seed, cx, cy, mt] + [R.dye_h1(w, seed, cx, cy, mt) for w in range(3)])
+    out["h1"] = np.array(h1, np.int64)
     for fn, x in jsm_inputs().items():
         out["jsm_" + fn + "_x"] = x
         out["jsm_" + fn + "_y"] = R.jsm(fn, x)

@@ -125,6 +125,11 @@ def test_capacity_and_empty(ctx, orc):
     assert np.all(out["tOn"][n_in:] == -7.0)
     off0, rec0 = host.LabelSchedule(np.zeros((0, 3), np.uint32), lab, 200.0, ctx=ctx)
     assert list(off0) == [0] and len(rec0) == 0
+    # loops that would not end within a launch are refused on the host
+    with pytest.raises(host.IscgError, match="time bins"):
+        host.PersistentBlinksInBins(d[:4], host.Kinetics(activationRatePerSec=1.0), 0, 1 << 21, ctx=ctx)
+    with pytest.raises(host.IscgError, match="activation rate"):
+        host.PersistentBlinks(d[:4], host.Kinetics(activationRatePerSec=1e6), 0.0, 1.0, ctx=ctx)
 
 
 def test_device_tensors_and_large_batch(ctx, orc):
@@ -160,3 +165,12 @@ def test_device_tensors_and_large_batch(ctx, orc):
     assert np.array_equal(off[sample + 1] - off[sample], c)
     got = np.concatenate([rec[off[i]:off[i + 1]] for i in sample])
     assert np.array_equal(got.view(np.uint64), b.view(np.uint64))
+
+
+def test_block_hashes(ctx, golden):
+    G = golden("dyes_ref.npz")["h1"]
+    for seed in np.unique(G[:, 0])[:8]:
+        rows = G[G[:, 0] == seed]
+        assert np.array_equal(host.DyeH1(int(seed), rows[:, 1], rows[:, 2], rows[:, 3], ctx=ctx), rows[:, 4])
+        assert np.array_equal(host.NpcH1(int(seed), rows[:, 1], rows[:, 2], ctx=ctx), rows[:, 5])
+        assert np.array_equal(host.LaminH1(int(seed), rows[:, 1], rows[:, 2], ctx=ctx), rows[:, 6])

@@ -215,3 +215,14 @@ def test_persistent_blinks_distributions(orc):
     # above m = 30 the count is the rounded normal approximation: mean m, variance m
     c, _ = orc.persistent_blinks(d, O.kinetics(activationRatePerSec=60.0), bins=(5, 5))
     assert abs(c.mean() - 60) < 5 * math.sqrt(60 / 2000) and abs(c.var() / 60 - 1) < 0.15
+
+
+def test_block_hashes_vs_reference_fixture(orc, golden):
+    """DyeH1 / NpcH1 / LaminH1 (core/src/dyes.cpp:175-222) of random blocks: Pcg4d(seed ^ salt, cx, cy, mtIndex or 0).a
+    with the salts of core/src/dyes.h:33-41, as the compiled reference computes them."""
+    G = golden("dyes_ref.npz")
+    for seed, cx, cy, mt, dye, npc, lamin in G["h1"]:
+        s = int(seed)
+        assert orc.pcg4d(s ^ 0x9E3779B9, int(cx), int(cy), int(mt))[0] == dye
+        assert orc.pcg4d(s ^ 0x4E504344, int(cx), int(cy), 0)[0] == npc
+        assert orc.pcg4d(s ^ 0x4C414D44, int(cx), int(cy), 0)[0] == lamin
