@@ -51,6 +51,20 @@ int32_t iscg_set_stream(iscg_context* h, void* stream);
  * would (a caller's CPU fallback, Sim/ScopeMovie.cpp:7634-7645, is exercised without faulting a GPU). */
 int32_t iscg_inject_failure(iscg_context* h, int32_t count, const char* message);
 
+/* Launch-parameter autotuning. No reference counterpart: the reference's GPU paths use fixed tilings. Some launch
+ * parameters have several values that give bit-identical results (the persistent splat grid's workgroups per CU, the
+ * workgroups that share one PSF request's planes). With mode 1 (the default) the first call with a new shape key
+ * (sizes, batch size rounded up to a power of two) times every candidate on the context's stream, one warm-up and 3
+ * timed launches each, and keeps the fastest for that key; mode 0 uses the fixed defaults. No tuning happens while
+ * the stream is being captured into a graph (defaults, nothing kept), so tune with an eager call first. */
+int32_t iscg_autotune_set(iscg_context* h, int32_t mode);
+/* The kept choices as text, one "key=value" line each, into buf (cap bytes, NUL-terminated, may be NULL); returns
+ * the length of the whole text. iscg_autotune_load adds the choices of such a text (a cache kept across processes,
+ * as the reference keeps PSF kernels on disk) and returns how many; a kept value that is not a candidate of this
+ * build is tuned again on use. */
+int32_t iscg_autotune_dump(iscg_context* h, char* buf, int32_t cap);
+int32_t iscg_autotune_load(iscg_context* h, const char* text);
+
 /* Per-kernel timing on the context's stream (the ISC_PROFILE analogue, Sim/Timing.h:1-60): with profiling on, each
  * kernel launch is bracketed by HIP events. iscg_profile_read waits for the stream and reports, per kernel name,
  * launches and total milliseconds (names: cap x 64 chars); returns the number of kernels. reset != 0 clears. */

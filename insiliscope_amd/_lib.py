@@ -100,6 +100,9 @@ SIGNATURES = {
     "iscg_profile_enable": (I32, [P, I32]),
     "iscg_profile_read": (I32, [P, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), I32, I32]),
     "iscg_set_stream": (I32, [P, P]),
+    "iscg_autotune_set": (I32, [P, I32]),
+    "iscg_autotune_dump": (I32, [P, C.c_char_p, I32]),
+    "iscg_autotune_load": (I32, [P, C.c_char_p]),
     "iscg_binned_setup": (I32, [P, P, I32, D, U32, U32, I32]),
     "iscg_binned_geometry": (I32, [P, P, P]),
     "iscg_binned_render": (I32, [P, P, P, I32, P]),

@@ -2,6 +2,7 @@
 #include "iscg_internal.h"
 
 #include <cstdio>
+#include <cstdlib>
 
 namespace iscg {
 
@@ -40,6 +41,72 @@ int32_t begin(iscg_context* h)
    if (!ok(h, hipSetDevice(h->device), "hipSetDevice"))
       return -1;
    return 0;
+}
+
+int tuned_choice(iscg_context* h, const std::string& key, const std::vector<int>& cands, int dflt,
+                 const std::function<bool(int)>& launch)
+{
+   if (h->tuneMode == 0 || cands.size() < 2)
+      return dflt;
+   const auto it = h->tuned.find(key);
+   if (it != h->tuned.end())
+      for (int c : cands)
+         if (c == it->second)
+            return c;
+   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+   if (hipStreamIsCapturing(h->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+   {
+      (void)hipGetLastError();
+      return dflt;
+   }
+   hipEvent_t e0 = nullptr, e1 = nullptr;
+   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+   {
+      (void)hipGetLastError();
+      if (e0)
+         (void)hipEventDestroy(e0);
+      return dflt;
+   }
+   int best = dflt;
+   float bestMs = 0.0f;
+   bool any = false, good = true;
+   for (int c : cands)
+   {
+      if (!launch(c))
+      {
+         good = false;
+         break;
+      }
+      if (hipEventRecord(e0, h->stream) != hipSuccess)
+      {
+         good = false;
+         break;
+      }
+      for (int r = 0; r < kTuneReps && good; ++r)
+         good = launch(c);
+      float ms = 0.0f;
+      if (!good || hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      {
+         good = false;
+         break;
+      }
+      if (!any || ms < bestMs)
+      {
+         best = c;
+         bestMs = ms;
+         any = true;
+      }
+   }
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   if (!good)
+   {
+      (void)hipGetLastError();
+      return dflt;
+   }
+   h->tuned[key] = best;
+   return best;
 }
 
 bool is_device_ptr(const void* p)
@@ -285,6 +352,63 @@ int32_t iscg_profile_read(iscg_context* h, char* names, int64_t* launches, doubl
    if (reset)
       h->profTotals.clear();
    return i;
+}
+
+int32_t iscg_autotune_set(iscg_context* h, int32_t mode)
+{
+   if (!h)
+      return -1;
+   if (mode != 0 && mode != 1)
+      return fail(h, "iscg_autotune_set: mode is 0 (fixed defaults) or 1 (tune on first use)");
+   h->tuneMode = mode;
+   return 0;
+}
+
+int32_t iscg_autotune_dump(iscg_context* h, char* buf, int32_t cap)
+{
+   if (!h)
+      return -1;
+   std::string text;
+   for (const auto& kv : h->tuned)
+      text += kv.first + "=" + std::to_string(kv.second) + "\n";
+   if (text.size() > 0x7FFFFFFFu)
+      return fail(h, "iscg_autotune_dump: too many choices");
+   if (buf && cap > 0)
+   {
+      const size_t n = text.size() < (size_t)cap - 1 ? text.size() : (size_t)cap - 1;
+      std::memcpy(buf, text.data(), n);
+      buf[n] = 0;
+   }
+   return (int32_t)text.size();
+}
+
+int32_t iscg_autotune_load(iscg_context* h, const char* text)
+{
+   if (!h)
+      return -1;
+   if (!text)
+      return fail(h, "iscg_autotune_load: null text");
+   std::map<std::string, int> add;
+   const char* p = text;
+   while (*p)
+   {
+      const char* e = std::strchr(p, '\n');
+      const std::string line = e ? std::string(p, (size_t)(e - p)) : std::string(p);
+      p = e ? e + 1 : p + line.size();
+      if (line.empty())
+         continue;
+      const size_t eq = line.rfind('=');
+      if (eq == std::string::npos || eq == 0 || eq + 1 == line.size())
+         return fail(h, "iscg_autotune_load: a line is not key=value: " + line);
+      char* end = nullptr;
+      const long v = std::strtol(line.c_str() + eq + 1, &end, 10);
+      if (*end != 0 || v < 0 || v > 1 << 20)
+         return fail(h, "iscg_autotune_load: bad value in: " + line);
+      add[line.substr(0, eq)] = (int)v;
+   }
+   for (const auto& kv : add)
+      h->tuned[kv.first] = kv.second;
+   return (int32_t)add.size();
 }
 
 int32_t iscg_inject_failure(iscg_context* h, int32_t count, const char* message)

@@ -8,6 +8,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -95,6 +96,8 @@ struct iscg_context
    int noiseExact = 0;  // iscg_noise_set_path: 1 evaluates every pixel with the FP64 chain
    iscg::DevBuf noiseItems;            // deferred pixels (NoiseQueue::items)
    uint32_t* noiseCount = nullptr;     // NoiseQueue::count, 2 words
+   int tuneMode = 1;                   // iscg_autotune_set: 1 tunes launch parameters on first use of a shape key
+   std::map<std::string, int> tuned;   // shape key -> chosen launch parameter
 };
 
 namespace iscg {
@@ -305,6 +308,25 @@ __device__ __forceinline__ uint16_t noise_pixel(const CamParams& cp, float photo
    double adu = offset + (ne + readNoise * g) / gain;
    adu = adu < 0.0 ? 0.0 : (adu > 65535.0 ? 65535.0 : adu);
    return (uint16_t)floor(adu + 0.5);
+}
+
+// The autotuner. A tunable launch parameter has several values that all give the same results (the size of a
+// persistent grid, how many workgroups share a request's planes). On the first call with a new shape key, with
+// tuning on, each candidate is launched once (warm-up), then timed over kTuneReps launches between two HIP events on
+// the context's stream, and the fastest is kept for the key; later calls reuse it. launch(c) enqueues the kernel(s)
+// with parameter c and returns false on a launch error (the default is then returned). While the stream is being
+// captured into a graph no timing is possible: the default is used and nothing is kept. A kept value that is not
+// among the candidates (a loaded cache of another build) is tuned again.
+constexpr int kTuneReps = 3;
+int tuned_choice(iscg_context* h, const std::string& key, const std::vector<int>& cands, int dflt,
+                 const std::function<bool(int)>& launch);
+// Rounds up to a power of two (shape keys bucket batch sizes so that nearby sizes share a choice).
+inline long long pow2_bucket(long long n)
+{
+   long long b = 1;
+   while (b < n)
+      b <<= 1;
+   return b;
 }
 
 // Times the launches made in its scope when profiling is on: events recorded on the context's stream before and

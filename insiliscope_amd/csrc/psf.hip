@@ -323,19 +323,21 @@ __global__ __launch_bounds__(kPsfThreads) void psf_planes_kernel(const PsfReqDev
 // Small planes (padded size Pp <= 80), E = W (Pup W^T) with E in registers. Per 32-row chunk n0 of the pupil: the
 // split pupil rows Pc (32 x MP) in LDS; mid[n][x] = sum_m Pc[n][m] W[x][m] (32 x Pp), stored split and transposed
 // (midT[x][n]); E[y][x] += sum_n W[y][n0 + n] mid[n][x].
-// WLDS (MP <= 128): kZGroups workgroups per request, each walking a contiguous quarter of its planes with the split W
+// WLDS (MP <= 128): zg workgroups per request (zg tuned among 1-32, 4 by default), each walking a contiguous zg-th of its planes
+// with the split W
 // (4 x 80 x (MP + 8) halves) staged in LDS once: the products read no global memory. The workgroups of a request
 // are dispatched to one XCD (workgroup b runs on XCD b mod 8), so the request's FP64 pupil tables, read again for
 // every plane, stay in that XCD's L2 (8 requests x 256 KB at a time at M 128). Otherwise one workgroup per
 // (request, plane), W from global memory, LDS 4 x 32 x (MP + 8) + 4 x 80 x 40 halves (two workgroups per CU).
-constexpr int kSmallPp = 80, kZGroups = 4, kXcds = 8;
+constexpr int kSmallPp = 80, kZGroups = 4, kXcds = 8;  // kZGroups: the default zg
 template <int MP, bool WLDS>
 __global__ __launch_bounds__(kPsfThreads, WLDS ? 2 : 4) void psf_small_kernel(const PsfReqDev* __restrict__ reqs,
                                                                              const double* __restrict__ A,
                                                                              const double* __restrict__ B,
                                                                              const _Float16* __restrict__ W,
                                                                              const double* __restrict__ tz, int nz,
-                                                                             int Pp, int nReq, float* __restrict__ out)
+                                                                             int Pp, int nReq, int zg,
+                                                                             float* __restrict__ out)
 {
    constexpr int PS = MP + 8;                                    // pupil / W row (halves)
    constexpr int CTX = kSmallPp / 16;                            // max tiles per side
@@ -346,7 +348,7 @@ __global__ __launch_bounds__(kPsfThreads, WLDS ? 2 : 4) void psf_small_kernel(co
    __shared__ __attribute__((aligned(16))) _Float16 wl[WLDS ? 4 : 1][WLDS ? kSmallPp : 1][WLDS ? PS : 8];
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int b = (int)blockIdx.x, slot = b / kXcds;
-   const int r = WLDS ? (slot / kZGroups) * kXcds + b % kXcds : b / nz;
+   const int r = WLDS ? (slot / zg) * kXcds + b % kXcds : b / nz;
    if (r >= nReq)
       return;  // the last XCD row's spare workgroups
    const PsfReqDev& q = reqs[r];
@@ -372,8 +374,8 @@ __global__ __launch_bounds__(kPsfThreads, WLDS ? 2 : 4) void psf_small_kernel(co
          *(half8*)&wl[part][row][8 * c8] = ld8(Wr + part * wPer + (size_t)row * MP + 8 * c8);
       }
    }
-   const int g = slot % kZGroups;
-   const int z0 = WLDS ? g * nz / kZGroups : b % nz, z1 = WLDS ? (g + 1) * nz / kZGroups : z0 + 1;
+   const int g = slot % zg;
+   const int z0 = WLDS ? g * nz / zg : b % nz, z1 = WLDS ? (g + 1) * nz / zg : z0 + 1;
    for (int zi = z0; zi < z1; ++zi)
    {
       const double t = tz[(size_t)r * nz + zi];
@@ -617,7 +619,8 @@ static void launch_planes(hipStream_t s, int items, const PsfReqDev* reqs, const
 
 // Output-row blocks of YB tiles keep the pupil chunk (MP x 40 halves x 4) and the mid rows (YB 16 x (MP + 8) x 4)
 // within ~150 KB of LDS.
-static bool psf_planes_launch(iscg_context* h, int nReq, int nz, int size, int Mp, int Pp, float* out)
+// zg: workgroups per request of the small-plane kernel (planes of padded size <= 80 and M <= 128).
+static bool psf_planes_launch(iscg_context* h, int nReq, int nz, int size, int Mp, int Pp, int zg, float* out)
 {
    PsfState& s = *h->psf;
    auto go = [&](auto mp, auto yb) {
@@ -625,14 +628,14 @@ static bool psf_planes_launch(iscg_context* h, int nReq, int nz, int size, int M
       if (Pp <= kSmallPp)
       {
          if constexpr (MP <= 128)
-            hipLaunchKernelGGL((psf_small_kernel<MP, true>), dim3((nReq + kXcds - 1) / kXcds * kXcds * kZGroups),
+            hipLaunchKernelGGL((psf_small_kernel<MP, true>), dim3((nReq + kXcds - 1) / kXcds * kXcds * zg),
                                dim3(kPsfThreads), 0, h->stream, s.reqs.as<const PsfReqDev>(),
                                s.tabA.as<const double>(), s.tabB.as<const double>(), s.W.as<const _Float16>(),
-                               s.tz.as<const double>(), nz, Pp, nReq, out);
+                               s.tz.as<const double>(), nz, Pp, nReq, zg, out);
          else
             hipLaunchKernelGGL((psf_small_kernel<MP, false>), dim3(nReq * nz), dim3(kPsfThreads), 0, h->stream,
                                s.reqs.as<const PsfReqDev>(), s.tabA.as<const double>(), s.tabB.as<const double>(),
-                               s.W.as<const _Float16>(), s.tz.as<const double>(), nz, Pp, nReq, out);
+                               s.W.as<const _Float16>(), s.tz.as<const double>(), nz, Pp, nReq, 1, out);
          return;
       }
       const int nYB = (Pp + YB * 16 - 1) / (YB * 16);
@@ -752,9 +755,24 @@ int32_t iscg_psf_planes(iscg_context* h, const iscg_psf_request* reqs, int32_t n
                       s.W.as<_Float16>());
    if (!ok(h, hipGetLastError(), "PSF table kernels"))
       return -1;
+   // the small-plane kernel's workgroups per request: every candidate computes each plane in one workgroup the
+   // same way, so the planes are bit-identical whichever is chosen (and the planes are written, not accumulated:
+   // the timing launches before normalisation leave the same output)
+   int zg = kZGroups;
+   if (Pp <= kSmallPp && Mp <= 128)
+   {
+      std::vector<int> cands;
+      for (int c : {1, 2, 4, 8, 16, 32})
+         if (c <= nz)
+            cands.push_back(c);
+      const std::string key = "psf_small M" + std::to_string(Mp) + " P" + std::to_string(Pp) + " nz" +
+                              std::to_string(nz) + " req" + std::to_string(pow2_bucket(nReq));
+      zg = tuned_choice(h, key, cands, nz >= kZGroups ? kZGroups : 1,
+                        [&](int c) { return psf_planes_launch(h, nReq, nz, size, Mp, Pp, c, dOut); });
+   }
    {
       ProfScope ps(h, "psf_planes_mfma");
-      if (!psf_planes_launch(h, nReq, nz, size, Mp, Pp, dOut))
+      if (!psf_planes_launch(h, nReq, nz, size, Mp, Pp, zg, dOut))
          return -1;
    }
    if (normalize)

@@ -782,21 +782,43 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
       if (!noise_queue(h, (size_t)nFrames * npx, a.q))
          return -1;
    }
+   // The persistent grid: workgroups per CU (256 CUs), tuned. Each (frame, tile) item is computed by one workgroup
+   // whatever the grid, so the frames are bit-identical for every choice; the timing launches write the same
+   // outputs again (the deferred-noise queue is drained and reset by each fallback launch). Not tuned when the
+   // caller's extra images are the photon output (in-place accumulation).
    const size_t items = (size_t)a.nTiles * nFrames;
-   const unsigned grid = (unsigned)(items < 256 * 16 ? items : 256 * 16);
-   {
-      ProfScope ps(h, "splat_tiles_noise");
+   auto tiles = [&](int perCu) {
+      const size_t cap = (size_t)256 * perCu;
+      const unsigned grid = (unsigned)(items < cap ? items : cap);
       if (!dAdu)
          hipLaunchKernelGGL((splat_tiles_kernel<false, false>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
       else if (cp.emccd)
          hipLaunchKernelGGL((splat_tiles_kernel<true, true>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
       else
          hipLaunchKernelGGL((splat_tiles_kernel<true, false>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
-      if (!ok(h, hipGetLastError(), "splat launch"))
+      return ok(h, hipGetLastError(), "splat launch");
+   };
+   auto fallback = [&]() {
+      return !dAdu || launch_noise_fallback(h, a.q, dAdu, (uint32_t)npx, dFrames, 0, a.offset, a.gain, a.readN, cp,
+                                            seed, s.pixelBase);
+   };
+   int perCu = 16;
+   const bool inPlace = dExtras && dPh && dExtras < (const float*)dPh + nFrames * npx && (const float*)dPh < dExtras +
+                        nFrames * npx;
+   if (!inPlace && items > 256 * 4)
+   {
+      const std::string key = "splat_tiles " + std::to_string(s.w) + "x" + std::to_string(s.h) + " frames" +
+                              std::to_string(pow2_bucket(nFrames)) + " em/frame" +
+                              std::to_string(pow2_bucket(nEmitters / (nFrames ? nFrames : 1))) + " noise" +
+                              std::to_string(dAdu ? 1 + cp.emccd : 0) + " fft" + std::to_string(a.fftSums ? 1 : 0);
+      perCu = tuned_choice(h, key, {4, 8, 16, 32}, 16, [&](int c) { return tiles(c) && fallback(); });
+   }
+   {
+      ProfScope ps(h, "splat_tiles_noise");
+      if (!tiles(perCu))
          return -1;
    }
-   if (dAdu && !launch_noise_fallback(h, a.q, dAdu, (uint32_t)npx, dFrames, 0, a.offset, a.gain, a.readN, cp, seed,
-                                      s.pixelBase))
+   if (!fallback())
       return -1;
    if (outAdu && !finish_out(h, outAdu, dAdu, nFrames * npx * sizeof(uint16_t)))
       return -1;

@@ -122,6 +122,32 @@ class Context:
         """exact=True: every noise pixel through the FP64 chain; False: the verified FP32 filter (same ADU)."""
         self.check(self.L.iscg_noise_set_path(self.h, 1 if exact else 0))
 
+    def set_autotune(self, on: bool):
+        """on=True (the default): the first call with a new shape key times the candidates of each tunable launch
+        parameter and keeps the fastest (the results are identical for every candidate); False: fixed defaults."""
+        self.check(self.L.iscg_autotune_set(self.h, 1 if on else 0), "iscg_autotune_set")
+
+    def autotune_choices(self) -> dict:
+        """The kept choices, {shape key: value}."""
+        n = self.L.iscg_autotune_dump(self.h, None, 0)
+        if n < 0:
+            raise IscgError(self.error())
+        buf = C.create_string_buffer(n + 1)
+        self.L.iscg_autotune_dump(self.h, buf, n + 1)
+        out = {}
+        for line in buf.value.decode().splitlines():
+            k, v = line.rsplit("=", 1)
+            out[k] = int(v)
+        return out
+
+    def load_autotune(self, choices: dict | str) -> int:
+        """Adds kept choices (a dict from autotune_choices, or its key=value text); returns how many."""
+        text = choices if isinstance(choices, str) else "".join(f"{k}={v}\n" for k, v in choices.items())
+        n = self.L.iscg_autotune_load(self.h, text.encode())
+        if n < 0:
+            raise IscgError(self.error())
+        return n
+
     def noise_bounds(self) -> dict:
         out = (C.c_double * 3)()
         self.check(self.L.iscg_noise_bounds(self.h, out))
