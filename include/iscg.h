@@ -254,6 +254,20 @@ int32_t iscg_psf_geometry(const iscg_psf_request* r, int32_t* size, int32_t* nz,
 int32_t iscg_psf_planes(iscg_context* h, const iscg_psf_request* reqs, int32_t nReq, int32_t normalize, float* out,
                         size_t outCap);
 
+/* A PSF library held on the device, for batches computed again and again (ComputeZernikePsfPlanes over a fixed set
+ * of requests; the north star's PSF library of wavelengths x aberration sets). create validates the requests (one
+ * size, plane count and pupil sampling, as iscg_psf_planes) and uploads them; run enqueues only kernels (pupil
+ * tables, DFT matrices, planes, normalisation) writing out as iscg_psf_planes does, so a run on a device output can
+ * be captured into a HIP graph and replayed (one graph launch per library); update replaces the requests' values
+ * (same count and geometry, same device buffers: a captured run computes the new requests on its next replay).
+ * No reference counterpart as an object: each run equals iscg_psf_planes of the same requests. */
+typedef struct iscg_psf_library iscg_psf_library;
+int32_t iscg_psf_library_create(iscg_context* h, const iscg_psf_request* reqs, int32_t nReq, iscg_psf_library** lib);
+int32_t iscg_psf_library_update(iscg_context* h, iscg_psf_library* lib, const iscg_psf_request* reqs, int32_t nReq);
+int32_t iscg_psf_library_geometry(const iscg_psf_library* lib, int32_t* nReq, int32_t* size, int32_t* nz);
+int32_t iscg_psf_library_run(iscg_context* h, iscg_psf_library* lib, int32_t normalize, float* out, size_t outCap);
+int32_t iscg_psf_library_destroy(iscg_context* h, iscg_psf_library* lib);
+
 /* The kernel half-width the CLI asks for (PsfKernelHalfWidthPx, Sim/PsfGeneratorBridge.cpp:1000): the requested
  * half-width in pixels, raised to 3 Rayleigh radii (capped at 48 px, at least 2). */
 int32_t iscg_psf_kernel_half_width_px(double halfWidthNm, double pixelSizeNm, double wavelengthNm, double na);

@@ -100,6 +100,11 @@ SIGNATURES = {
     "iscg_profile_enable": (I32, [P, I32]),
     "iscg_profile_read": (I32, [P, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), I32, I32]),
     "iscg_set_stream": (I32, [P, P]),
+    "iscg_psf_library_create": (I32, [P, P, I32, C.POINTER(P)]),
+    "iscg_psf_library_update": (I32, [P, P, P, I32]),
+    "iscg_psf_library_geometry": (I32, [P, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
+    "iscg_psf_library_run": (I32, [P, P, I32, P, C.c_size_t]),
+    "iscg_psf_library_destroy": (I32, [P, P]),
     "iscg_autotune_set": (I32, [P, I32]),
     "iscg_autotune_dump": (I32, [P, C.c_char_p, I32]),
     "iscg_autotune_load": (I32, [P, C.c_char_p]),

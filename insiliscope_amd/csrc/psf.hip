@@ -40,9 +40,19 @@ struct PsfReqDev
    int M, size, nz, nModes, doubleHelix, maskModes, pad[2];
 };
 
+// One batch of requests on the device: the request records and per-plane defocus (uploaded from the host), the FP64
+// pupil tables and the split DFT matrices (computed by kernels from the records).
+struct PsfBatch
+{
+   DevBuf reqs, tabA, tabB, W, tz;
+   std::vector<PsfReqDev> hostReqs;   // kept while an upload from them may be in flight
+   std::vector<double> hostTz;
+   int nReq = 0, size = 0, nz = 0, M = 0, Mp = 0, Pp = 0;
+};
+
 struct PsfState
 {
-   DevBuf reqs, tabA, tabB, W, tz, sums;
+   PsfBatch batch;  // iscg_psf_planes's requests
 };
 
 void destroy_psf(iscg_context* h)
@@ -620,9 +630,9 @@ static void launch_planes(hipStream_t s, int items, const PsfReqDev* reqs, const
 // Output-row blocks of YB tiles keep the pupil chunk (MP x 40 halves x 4) and the mid rows (YB 16 x (MP + 8) x 4)
 // within ~150 KB of LDS.
 // zg: workgroups per request of the small-plane kernel (planes of padded size <= 80 and M <= 128).
-static bool psf_planes_launch(iscg_context* h, int nReq, int nz, int size, int Mp, int Pp, int zg, float* out)
+static bool psf_planes_launch(iscg_context* h, const PsfBatch& s, int zg, float* out)
 {
-   PsfState& s = *h->psf;
+   const int nReq = s.nReq, nz = s.nz, Mp = s.Mp, Pp = s.Pp;
    auto go = [&](auto mp, auto yb) {
       constexpr int MP = decltype(mp)::value, YB = decltype(yb)::value;
       if (Pp <= kSmallPp)
@@ -656,8 +666,122 @@ static bool psf_planes_launch(iscg_context* h, int nReq, int nz, int size, int M
    case 256: go(std::integral_constant<int, 256>{}, std::integral_constant<int, 2>{}); break;
    default: return fail(h, "iscg_psf_planes: unsupported pupil size") == 0;
    }
-   (void)size;
    return ok(h, hipGetLastError(), "PSF planes launch");
+}
+
+// Host side of a batch: validates the requests (one size, plane count and pupil sampling), fills the device records
+// and per-plane ti - ti0, reserves the device buffers and uploads records and defocus on the context's stream. A
+// batch that already holds nReq requests of the same geometry keeps its buffers (same addresses).
+static int32_t psf_batch_set(iscg_context* h, PsfBatch& b, const iscg_psf_request* reqs, int32_t nReq,
+                             const char* what)
+{
+   int size = 0, nz = 0, M = 0;
+   std::vector<PsfReqDev> dev((size_t)nReq);
+   for (int i = 0; i < nReq; ++i)
+   {
+      const iscg_psf_request* r = reqs + i;
+      if (!(r->na > 0.0) || !(r->wavelengthNm > 0.0) || !(r->immersionIndex > 0.0) || !(r->sampleIndex > 0.0) ||
+          !(r->pixelSizeNm > 0.0) || !(r->zStepNm > 0.0))
+         return fail(h, "GibsonLanniZernike PSF: NA, wavelength, refractive indices, pixel size and z step must be > 0.");
+      int sz, z, m;
+      geometry(r, &sz, &z, &m);
+      if (i == 0)
+      {
+         size = sz;
+         nz = z;
+         M = m;
+      }
+      else if (sz != size || z != nz || m != M)
+         return fail(h, std::string(what) + ": the requests of one call must share the kernel size, plane count and "
+                                            "pupil samples");
+      dev[(size_t)i] = request_dev(r, sz, z, m);
+   }
+   const int Mp = (M + 31) / 32 * 32;
+   if (Mp > 256)
+      return fail(h, std::string(what) + ": pupil samples above 256 are not supported on this path");
+   const int Pp = (size + 15) / 16 * 16;
+   // per (request, plane): ti - ti0, as the reference forms it (Sim/ZernikePsf.cpp:238-244)
+   std::vector<double> tz((size_t)nReq * nz);
+   for (int i = 0; i < nReq; ++i)
+      for (int z = 0; z < nz; ++z)
+      {
+         const PsfReqDev& d = dev[(size_t)i];
+         const double ti = (d.ti0 - d.focalShift) + d.zStepM * (z - (nz - 1.0) / 2.0);
+         tz[(size_t)i * nz + z] = ti - d.ti0;
+      }
+   // the previous upload from the host copies completes before they are replaced
+   if (!ok(h, hipStreamSynchronize(h->stream), "PSF request upload"))
+      return -1;
+   const size_t wHalves = (size_t)nReq * 4 * Pp * Mp;
+   if (!ok(h, b.reqs.reserve(dev.size() * sizeof(PsfReqDev)), "allocation") ||
+       !ok(h, b.tabA.reserve((size_t)nReq * Mp * Mp * sizeof(double)), "allocation") ||
+       !ok(h, b.tabB.reserve((size_t)nReq * Mp * Mp * sizeof(double)), "allocation") ||
+       !ok(h, b.W.reserve(wHalves * sizeof(_Float16)), "allocation") ||
+       !ok(h, b.tz.reserve(tz.size() * sizeof(double)), "allocation"))
+      return -1;
+   b.hostReqs = std::move(dev);
+   b.hostTz = std::move(tz);
+   if (!ok(h, hipMemcpyAsync(b.reqs.p, b.hostReqs.data(), b.hostReqs.size() * sizeof(PsfReqDev),
+                             hipMemcpyHostToDevice, h->stream), "copy") ||
+       !ok(h, hipMemcpyAsync(b.tz.p, b.hostTz.data(), b.hostTz.size() * sizeof(double), hipMemcpyHostToDevice,
+                             h->stream), "copy"))
+      return -1;
+   b.nReq = nReq;
+   b.size = size;
+   b.nz = nz;
+   b.M = M;
+   b.Mp = Mp;
+   b.Pp = Pp;
+   return 0;
+}
+
+// The kernels of a batch, in stream order: pupil tables, split DFT matrices, planes, optional sum-1 normalisation
+// into out (nReq x nz x size x size floats, device memory). No host copies, allocations or synchronisation unless
+// the small-plane kernel's launch parameter is tuned for a new shape (never while the stream is captured).
+static int32_t psf_batch_run(iscg_context* h, const PsfBatch& b, int32_t normalize, float* dOut)
+{
+   const int nReq = b.nReq, nz = b.nz, Mp = b.Mp, Pp = b.Pp;
+   const PsfReqDev* dReq = b.reqs.as<const PsfReqDev>();
+   const size_t nTab = (size_t)nReq * Mp * Mp;
+   {
+      ProfScope ps(h, "psf_tables");
+      hipLaunchKernelGGL(psf_tables_kernel, dim3(grid_for(nTab, 256, 16)), dim3(256), 0, h->stream, dReq, nReq, Mp,
+                         b.tabA.as<double>(), b.tabB.as<double>());
+   }
+   const size_t nW = (size_t)nReq * Pp * Mp;
+   hipLaunchKernelGGL(psf_w_kernel, dim3(grid_for(nW, 256, 16)), dim3(256), 0, h->stream, dReq, nReq, Pp, Mp,
+                      b.W.as<_Float16>());
+   if (!ok(h, hipGetLastError(), "PSF table kernels"))
+      return -1;
+   // the small-plane kernel's workgroups per request: every candidate computes each plane in one workgroup the
+   // same way, so the planes are bit-identical whichever is chosen (and the planes are written, not accumulated:
+   // the timing launches before normalisation leave the same output)
+   int zg = kZGroups;
+   if (Pp <= kSmallPp && Mp <= 128)
+   {
+      std::vector<int> cands;
+      for (int c : {1, 2, 4, 8, 16, 32})
+         if (c <= nz)
+            cands.push_back(c);
+      const std::string key = "psf_small M" + std::to_string(Mp) + " P" + std::to_string(Pp) + " nz" +
+                              std::to_string(nz) + " req" + std::to_string(pow2_bucket(nReq));
+      zg = tuned_choice(h, key, cands, nz >= kZGroups ? kZGroups : 1,
+                        [&](int c) { return psf_planes_launch(h, b, c, dOut); });
+   }
+   {
+      ProfScope ps(h, "psf_planes_mfma");
+      if (!psf_planes_launch(h, b, zg, dOut))
+         return -1;
+   }
+   if (normalize)
+   {
+      ProfScope ps(h, "psf_normalize");
+      hipLaunchKernelGGL(psf_normalize_kernel, dim3((unsigned)(nReq * nz)), dim3(256), 0, h->stream, dOut,
+                         b.size * b.size);
+      if (!ok(h, hipGetLastError(), "PSF normalisation"))
+         return -1;
+   }
+   return 0;
 }
 
 }  // namespace iscg
@@ -688,103 +812,115 @@ int32_t iscg_psf_planes(iscg_context* h, const iscg_psf_request* reqs, int32_t n
       return -1;
    if (!reqs || nReq <= 0 || !out)
       return fail(h, "iscg_psf_planes: null argument or no request");
-   int size = 0, nz = 0, M = 0;
-   std::vector<PsfReqDev> dev((size_t)nReq);
-   for (int i = 0; i < nReq; ++i)
-   {
-      const iscg_psf_request* r = reqs + i;
-      if (!(r->na > 0.0) || !(r->wavelengthNm > 0.0) || !(r->immersionIndex > 0.0) || !(r->sampleIndex > 0.0) ||
-          !(r->pixelSizeNm > 0.0) || !(r->zStepNm > 0.0))
-         return fail(h, "GibsonLanniZernike PSF: NA, wavelength, refractive indices, pixel size and z step must be > 0.");
-      int s, z, m;
-      geometry(r, &s, &z, &m);
-      if (i == 0)
-      {
-         size = s;
-         nz = z;
-         M = m;
-      }
-      else if (s != size || z != nz || m != M)
-         return fail(h, "iscg_psf_planes: the requests of one call must share the kernel size, plane count and "
-                        "pupil samples");
-      dev[(size_t)i] = request_dev(r, s, z, m);
-   }
-   const int Mp = (M + 31) / 32 * 32;
-   if (Mp > 256)
-      return fail(h, "iscg_psf_planes: pupil samples above 256 are not supported on this path");
-   const int Pp = (size + 15) / 16 * 16;
-   const size_t planeFloats = (size_t)size * size;
-   const size_t need = (size_t)nReq * nz * planeFloats;
-   if (outCap < need)
-      return fail(h, "iscg_psf_planes: output too small (" + std::to_string(need) + " floats needed)");
    if (!h->psf)
       h->psf = new PsfState();
-   PsfState& s = *h->psf;
-   // per (request, plane): ti - ti0, as the reference forms it (Sim/ZernikePsf.cpp:238-244)
-   std::vector<double> tz((size_t)nReq * nz);
-   for (int i = 0; i < nReq; ++i)
-      for (int z = 0; z < nz; ++z)
-      {
-         const PsfReqDev& d = dev[(size_t)i];
-         const double ti = (d.ti0 - d.focalShift) + d.zStepM * (z - (nz - 1.0) / 2.0);
-         tz[(size_t)i * nz + z] = ti - d.ti0;
-      }
-   const size_t wHalves = (size_t)nReq * 4 * Pp * Mp;
-   if (!ok(h, s.reqs.reserve(dev.size() * sizeof(PsfReqDev)), "allocation") ||
-       !ok(h, s.tabA.reserve((size_t)nReq * Mp * Mp * sizeof(double)), "allocation") ||
-       !ok(h, s.tabB.reserve((size_t)nReq * Mp * Mp * sizeof(double)), "allocation") ||
-       !ok(h, s.W.reserve(wHalves * sizeof(_Float16)), "allocation") ||
-       !ok(h, s.tz.reserve(tz.size() * sizeof(double)), "allocation"))
+   PsfBatch& b = h->psf->batch;
+   if (psf_batch_set(h, b, reqs, nReq, "iscg_psf_planes"))
       return -1;
-   if (!ok(h, hipMemcpyAsync(s.reqs.p, dev.data(), dev.size() * sizeof(PsfReqDev), hipMemcpyHostToDevice, h->stream),
-           "copy") ||
-       !ok(h, hipMemcpyAsync(s.tz.p, tz.data(), tz.size() * sizeof(double), hipMemcpyHostToDevice, h->stream), "copy"))
-      return -1;
+   const size_t need = (size_t)nReq * b.nz * b.size * b.size;
+   if (outCap < need)
+      return fail(h, "iscg_psf_planes: output too small (" + std::to_string(need) + " floats needed)");
    float* dOut = (float*)stage_out(h, 0, out, need * sizeof(float));
    if (!dOut)
       return -1;
-   const PsfReqDev* dReq = s.reqs.as<const PsfReqDev>();
-   const size_t nTab = (size_t)nReq * Mp * Mp;
-   {
-   ProfScope ps(h, "psf_tables");
-   hipLaunchKernelGGL(psf_tables_kernel, dim3(grid_for(nTab, 256, 16)), dim3(256), 0, h->stream, dReq, nReq, Mp,
-                      s.tabA.as<double>(), s.tabB.as<double>());
-   }
-   const size_t nW = (size_t)nReq * Pp * Mp;
-   hipLaunchKernelGGL(psf_w_kernel, dim3(grid_for(nW, 256, 16)), dim3(256), 0, h->stream, dReq, nReq, Pp, Mp,
-                      s.W.as<_Float16>());
-   if (!ok(h, hipGetLastError(), "PSF table kernels"))
+   if (psf_batch_run(h, b, normalize, dOut))
       return -1;
-   // the small-plane kernel's workgroups per request: every candidate computes each plane in one workgroup the
-   // same way, so the planes are bit-identical whichever is chosen (and the planes are written, not accumulated:
-   // the timing launches before normalisation leave the same output)
-   int zg = kZGroups;
-   if (Pp <= kSmallPp && Mp <= 128)
-   {
-      std::vector<int> cands;
-      for (int c : {1, 2, 4, 8, 16, 32})
-         if (c <= nz)
-            cands.push_back(c);
-      const std::string key = "psf_small M" + std::to_string(Mp) + " P" + std::to_string(Pp) + " nz" +
-                              std::to_string(nz) + " req" + std::to_string(pow2_bucket(nReq));
-      zg = tuned_choice(h, key, cands, nz >= kZGroups ? kZGroups : 1,
-                        [&](int c) { return psf_planes_launch(h, nReq, nz, size, Mp, Pp, c, dOut); });
-   }
-   {
-      ProfScope ps(h, "psf_planes_mfma");
-      if (!psf_planes_launch(h, nReq, nz, size, Mp, Pp, zg, dOut))
-         return -1;
-   }
-   if (normalize)
-   {
-      ProfScope ps(h, "psf_normalize");
-      hipLaunchKernelGGL(psf_normalize_kernel, dim3((unsigned)(nReq * nz)), dim3(256), 0, h->stream, dOut,
-                         (int)planeFloats);
-      if (!ok(h, hipGetLastError(), "PSF normalisation"))
-         return -1;
-   }
    if (!finish_out(h, out, dOut, need * sizeof(float)))
       return -1;
+   return 0;
+}
+
+struct iscg_psf_library
+{
+   iscg::PsfBatch b;
+   int device = 0;
+};
+
+int32_t iscg_psf_library_create(iscg_context* h, const iscg_psf_request* reqs, int32_t nReq, iscg_psf_library** lib)
+{
+   if (begin(h))
+      return -1;
+   if (!reqs || nReq <= 0 || !lib)
+      return fail(h, "iscg_psf_library_create: null argument or no request");
+   *lib = nullptr;
+   iscg_psf_library* L = new iscg_psf_library();
+   L->device = h->device;
+   if (psf_batch_set(h, L->b, reqs, nReq, "iscg_psf_library_create") ||
+       !ok(h, hipStreamSynchronize(h->stream), "iscg_psf_library_create"))
+   {
+      delete L;
+      return -1;
+   }
+   *lib = L;
+   return 0;
+}
+
+int32_t iscg_psf_library_update(iscg_context* h, iscg_psf_library* lib, const iscg_psf_request* reqs, int32_t nReq)
+{
+   if (begin(h))
+      return -1;
+   if (!lib || !reqs)
+      return fail(h, "iscg_psf_library_update: null argument");
+   if (lib->device != h->device)
+      return fail(h, "iscg_psf_library_update: the library belongs to another device");
+   if (nReq != lib->b.nReq)
+      return fail(h, "iscg_psf_library_update: the request count must stay " + std::to_string(lib->b.nReq));
+   for (int i = 0; i < nReq; ++i)
+   {
+      int sz, z, m;
+      geometry(reqs + i, &sz, &z, &m);
+      if (sz != lib->b.size || z != lib->b.nz || m != lib->b.M)
+         return fail(h, "iscg_psf_library_update: the kernel size, plane count and pupil samples must stay the same");
+   }
+   const void* before = lib->b.reqs.p;
+   if (psf_batch_set(h, lib->b, reqs, nReq, "iscg_psf_library_update"))
+      return -1;
+   if (lib->b.reqs.p != before)
+      return fail(h, "iscg_psf_library_update: device buffers moved");
+   return ok(h, hipStreamSynchronize(h->stream), "iscg_psf_library_update") ? 0 : -1;
+}
+
+int32_t iscg_psf_library_geometry(const iscg_psf_library* lib, int32_t* nReq, int32_t* size, int32_t* nz)
+{
+   if (!lib)
+      return -1;
+   if (nReq)
+      *nReq = lib->b.nReq;
+   if (size)
+      *size = lib->b.size;
+   if (nz)
+      *nz = lib->b.nz;
+   return 0;
+}
+
+int32_t iscg_psf_library_run(iscg_context* h, iscg_psf_library* lib, int32_t normalize, float* out, size_t outCap)
+{
+   if (begin(h))
+      return -1;
+   if (!lib || !out)
+      return fail(h, "iscg_psf_library_run: null argument");
+   if (lib->device != h->device)
+      return fail(h, "iscg_psf_library_run: the library belongs to another device");
+   const size_t need = (size_t)lib->b.nReq * lib->b.nz * lib->b.size * lib->b.size;
+   if (outCap < need)
+      return fail(h, "iscg_psf_library_run: output too small (" + std::to_string(need) + " floats needed)");
+   float* dOut = (float*)stage_out(h, 0, out, need * sizeof(float));
+   if (!dOut)
+      return -1;
+   if (psf_batch_run(h, lib->b, normalize, dOut))
+      return -1;
+   if (!finish_out(h, out, dOut, need * sizeof(float)))
+      return -1;
+   return 0;
+}
+
+int32_t iscg_psf_library_destroy(iscg_context* h, iscg_psf_library* lib)
+{
+   if (!lib)
+      return 0;
+   if (h && hipSetDevice(lib->device) == hipSuccess)
+      (void)hipStreamSynchronize(h->stream);
+   delete lib;
    return 0;
 }
 

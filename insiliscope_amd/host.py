@@ -580,6 +580,51 @@ def ComputePsfPlanesBatch(requests, normalize: bool = False, out=None, ctx: Cont
     return out
 
 
+class PsfLibrary:
+    """A fixed set of PSF requests held on the device (iscg_psf_library_*): Run() computes their planes with kernels
+    only, so it can be captured into a HIP graph (torch.cuda.CUDAGraph) and replayed; Update() changes the requests
+    (same count and geometry) for the next run or replay. Each Run equals ComputePsfPlanesBatch of the requests."""
+
+    def __init__(self, requests, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self._arr = requests if isinstance(requests, C.Array) else PsfRequestArray(requests)
+        h = C.c_void_p()
+        self.ctx.follow_torch()
+        self.ctx.check(self.ctx.L.iscg_psf_library_create(self.ctx.h, self._arr, len(self._arr), C.byref(h)),
+                       "iscg_psf_library_create")
+        self.h = h
+        n, size, nz = C.c_int32(), C.c_int32(), C.c_int32()
+        self.ctx.L.iscg_psf_library_geometry(h, C.byref(n), C.byref(size), C.byref(nz))
+        self.shape = (n.value, nz.value, size.value, size.value)
+
+    def Update(self, requests):
+        self._arr = requests if isinstance(requests, C.Array) else PsfRequestArray(requests)
+        self.ctx.follow_torch()
+        self.ctx.check(self.ctx.L.iscg_psf_library_update(self.ctx.h, self.h, self._arr, len(self._arr)),
+                       "iscg_psf_library_update")
+
+    def Run(self, out=None, normalize: bool = False):
+        if out is None:
+            out = np.zeros(self.shape, np.float32)
+        p, keep = _ptr(out)
+        self.ctx.follow_torch()
+        self.ctx.check(self.ctx.L.iscg_psf_library_run(self.ctx.h, self.h, int(bool(normalize)), p,
+                                                       int(np.prod(self.shape))), "iscg_psf_library_run")
+        del keep
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.iscg_psf_library_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def ComputeZernikePsfPlanes(fields, ctx: Context | None = None):
     """sim::ComputeZernikePsfPlanes for one request: (nz, size, size) raw |E|^2 planes."""
     return ComputePsfPlanesBatch([fields], ctx=ctx)[0]

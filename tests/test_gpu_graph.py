@@ -1,5 +1,6 @@
 """Batches captured in HIP graphs (torch.cuda.CUDAGraph over the library's launches on the capturing stream): a
-mean-field stack on the slab path with sCMOS noise, and a splat + EMCCD batch of frames with uneven blink counts.
+mean-field stack on the slab path with sCMOS noise, a splat + EMCCD batch of frames with uneven blink counts, and
+the 512-PSF library.
 Each graph is replayed several times, with the program's own eager calls and synchronisations between replays,
 and every replay's output is compared bit for bit with the eager result. The calls are warmed up on the capture
 stream first, so that nothing is allocated and the library does not switch streams while capturing."""
@@ -102,3 +103,46 @@ def test_graph_splat_batch_with_noise(ctx, orc):
             call(eager)
         torch.cuda.synchronize()
         assert np.array_equal(eager.cpu().numpy(), want)
+
+
+def test_graph_psf_library_c3(ctx):
+    """The whole C3 library (8 wavelengths x 64 Zernike sets, 41 planes of 65x65, M 128) as one captured graph:
+    replays equal the eager run and the one-shot batch call bit for bit; after Update with other aberrations, a
+    replay computes the new library."""
+    import torch
+    reqs = configs.c3_requests(seed=1)
+    reqs2 = configs.c3_requests(seed=2)
+    pick = [0, 100, 511]
+    want = host.ComputePsfPlanesBatch([reqs[i] for i in pick], normalize=True, ctx=ctx)
+    want2 = host.ComputePsfPlanesBatch([reqs2[i] for i in pick], normalize=True, ctx=ctx)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        lib = host.PsfLibrary(reqs, ctx)
+        assert lib.shape == (512, 41, 65, 65)
+        out = torch.zeros(lib.shape, dtype=torch.float32, device="cuda")
+        eager = torch.zeros_like(out)
+        lib.Run(eager, normalize=True)
+        lib.Run(out, normalize=True)
+    torch.cuda.synchronize()
+    for j, i in enumerate(pick):
+        assert np.array_equal(eager[i].cpu().numpy(), want[j]), i
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        lib.Run(out, normalize=True)
+    for k in range(3):
+        out.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        print("psf replay", k, flush=True)
+        assert torch.equal(out, eager), f"replay {k}"
+        with torch.cuda.stream(s):
+            lib.Run(eager, normalize=True)  # eager work between replays
+        torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        lib.Update(reqs2)
+    out.zero_()
+    g.replay()
+    torch.cuda.synchronize()
+    for j, i in enumerate(pick):
+        assert np.array_equal(out[i].cpu().numpy(), want2[j]), i
+    lib.close()
