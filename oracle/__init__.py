@@ -493,6 +493,13 @@ class Reference:
         L.ref_psf_half_width_px.restype = I32
         L.ref_capped_psf_oversampling.argtypes = [I32, D, D]
         L.ref_capped_psf_oversampling.restype = I32
+        L.ref_parse_zernike.argtypes = [C.c_char_p, PD]
+        L.ref_parse_zernike.restype = I32
+        L.ref_format_zernike.argtypes = [PD, C.c_char, C.c_char_p, I32]
+        L.ref_format_zernike.restype = I32
+        L.ref_compute_psf_kernel_cache.argtypes = [C.POINTER(PsfRequest), C.c_char_p, C.c_char_p, I32, C.c_char_p,
+                                                   I32, PF, C.POINTER(C.c_uint64)]
+        L.ref_compute_psf_kernel_cache.restype = I32
         L.ref_wf_images_render.argtypes = [PF, I32, I32, U32, U32, I32, PD, I32, PF]
         L.ref_wf_images_render.restype = I32
         L.ref_binned_render.argtypes = [C.c_void_p, I32, I32, I32, D, D, I32, I32, I32, PD, PI32, I32, PF, PI32]
@@ -728,6 +735,28 @@ class Reference:
 
     def capped_psf_oversampling(self, os_, pixel_nm, max_sample_nm):
         return self.L.ref_capped_psf_oversampling(os_, pixel_nm, max_sample_nm)
+
+    def parse_zernike(self, text: str):
+        """ParseZernikeCoefficients: (28 values, ok)."""
+        out = np.zeros(28)
+        ok = self.L.ref_parse_zernike(text.encode("latin-1"), _p(out, C.c_double))
+        return out, bool(ok)
+
+    def format_zernike(self, values, separator=","):
+        v = np.ascontiguousarray(values, np.float64)
+        buf = C.create_string_buffer(4096)
+        self.L.ref_format_zernike(_p(v, C.c_double), separator.encode(), buf, 4096)
+        return buf.value.decode()
+
+    def compute_psf_kernel_cache(self, req, zernike_text, disk_dir, size, nz, memo_entries=0):
+        """ComputePsfKernelCache with the disk store in disk_dir: (ok, log lines, sum-1 planes, serial)."""
+        log = C.create_string_buffer(1 << 14)
+        planes = np.zeros((nz, size, size), np.float32)
+        serial = C.c_uint64()
+        rc = self.L.ref_compute_psf_kernel_cache(C.byref(req), None if zernike_text is None else
+                                                 zernike_text.encode(), disk_dir.encode(), memo_entries, log,
+                                                 1 << 14, _p(planes, C.c_float), C.byref(serial))
+        return rc == 0, log.value.decode().splitlines(), planes, serial.value
 
     def psf_library(self, reqs, out=None):
         arr = (PsfRequest * len(reqs))(*reqs)

@@ -786,3 +786,70 @@ int32_t ref_capped_psf_oversampling(int32_t os, double pixelSizeNm, double maxSa
 }
 
 } // extern "C"
+
+extern "C" {
+
+// ParseZernikeCoefficients (Sim/SMLMZernike.cpp:28): 1 and the 28 values, or 0 (out all zero).
+int32_t ref_parse_zernike(const char* text, double* out)
+{
+   bool ok = false;
+   const sim::ZernikeCoefficients z = sim::ParseZernikeCoefficients(text ? text : "", ok);
+   std::memcpy(out, z.data(), z.size() * sizeof(double));
+   return ok ? 1 : 0;
+}
+
+// FormatZernikeCoefficients (Sim/SMLMZernike.cpp:16): the text (NUL-terminated in buf); returns its length.
+int32_t ref_format_zernike(const double* in, char separator, char* buf, int32_t cap)
+{
+   sim::ZernikeCoefficients z{};
+   std::memcpy(z.data(), in, z.size() * sizeof(double));
+   const std::string s = sim::FormatZernikeCoefficients(z, separator);
+   if (buf && cap > 0)
+   {
+      const size_t n = std::min(s.size(), static_cast<size_t>(cap) - 1);
+      std::memcpy(buf, s.data(), n);
+      buf[n] = 0;
+   }
+   return static_cast<int32_t>(s.size());
+}
+
+// ComputePsfKernelCache (Sim/PsfGeneratorBridge.cpp:895) of a GibsonLanniZernike request whose coefficient string
+// is zernikeText (NULL: the request's values at %.17g), with the disk store in diskDir ("" none) and memoEntries
+// (0: unchanged); the log messages, one per line, into log; the sum-1 planes into planesOut (may be NULL) and the
+// cache's serial into serial. Returns 0, or -1 with the error in log.
+int32_t ref_compute_psf_kernel_cache(const orc_psf_request* r, const char* zernikeText, const char* diskDir,
+                                     int32_t memoEntries, char* log, int32_t logCap, float* planesOut,
+                                     uint64_t* serial)
+{
+   sim::PsfGeneratorRequest q = ToRequest(r);
+   if (zernikeText)
+      q.zernikeCoefficients = zernikeText;
+   sim::SetPsfKernelDiskCacheDir(diskDir ? diskDir : "");
+   if (memoEntries > 0)
+      sim::SetPsfKernelMemoEntries(static_cast<size_t>(memoEntries));
+   std::string text;
+   sim::PsfKernelCache cache;
+   std::string err;
+   const bool ok = sim::ComputePsfKernelCache(q, cache, err, [&](const std::string& m) { text += m + "\n"; });
+   if (!ok)
+      text += err + "\n";
+   if (log && logCap > 0)
+   {
+      const size_t n = std::min(text.size(), static_cast<size_t>(logCap) - 1);
+      std::memcpy(log, text.data(), n);
+      log[n] = 0;
+   }
+   if (!ok)
+      return -1;
+   if (planesOut)
+   {
+      const size_t plane = static_cast<size_t>(cache.sizeOversampled) * cache.sizeOversampled;
+      for (int z = 0; z < cache.nz; ++z)
+         std::memcpy(planesOut + z * plane, cache.Planes()[static_cast<size_t>(z)].data(), plane * sizeof(float));
+   }
+   if (serial)
+      *serial = cache.Serial();
+   return 0;
+}
+
+} // extern "C"

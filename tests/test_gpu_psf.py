@@ -82,3 +82,34 @@ def test_psf_mixed_batch_rejected(ctx):
         host.ComputePsfPlanesBatch([a, b], ctx=ctx)
     with pytest.raises(host.IscgError, match="must be > 0"):
         host.ComputePsfPlanesBatch([dict(a, na=0.0)], ctx=ctx)
+
+
+def test_compute_psf_kernel_cache_on_gpu(ctx, tmp_path):
+    """ComputePsfKernelCache with the planes computed on the GPU: sum-1 planes within 1e-5 relative L2 of the
+    compiled reference's BuildZernikePsfKernelCache, stored to disk (the file reads back bit for bit), and a repeat
+    (another interpMode) is a memo hit with the same planes and serial."""
+    import time
+    from insiliscope_amd import psf_cache as P
+    text = "0,0,0,0.05,-0.03,0.02,0,0,0,0,0.01,0,0,0,%.17g" % (1e-3 * (time.time() % 1.0))
+    fields = dict(wavelengthNm=670.0, na=1.45, immersionIndex=1.518, sampleIndex=1.33, workingDistanceUm=170.0,
+                  sampleDepthNm=500.0, pixelSizeNm=100.0, zStepNm=50.0, oversampling=2, kernelHalfWidthPx=6, nz=9,
+                  pupilSamples=64, interpMode=2)
+    req = P.PsfGeneratorRequest(model=P.GIBSON_LANNI_ZERNIKE, zernikeCoefficients=text, **fields)
+    P.SetPsfKernelDiskCacheDir(str(tmp_path))
+    ok, c, err = P.ComputePsfKernelCache(req, ctx=ctx)
+    assert ok, err
+    size = 2 * 6 * 2 + 1
+    assert c.planes.shape == (9, size, size) and c.interpMode == 2 and c.oversampling == 2
+    z, zok = P.ParseZernikeCoefficients(text)
+    assert zok
+    want, _ = O.Reference().psf_kernel_cache(O.psf_request(zernike=z, **fields), size, 9, 2)
+    for k in range(9):
+        assert rel_l2(c.planes[k], want[k]) <= TOL, k
+    back = P.LoadKernelFromDisk(req)
+    assert back is not None and np.array_equal(back.planes, c.planes)
+    log = []
+    ok, c2, _ = P.ComputePsfKernelCache(P.PsfGeneratorRequest(**{**req.__dict__, "interpMode": 1}), log.append,
+                                        ctx=ctx)
+    assert ok and c2.serial == c.serial and c2.planes is c.planes and c2.interpMode == 1
+    assert log == ["PSF: kernel unchanged, reusing the one already computed."]
+    P.SetPsfKernelDiskCacheDir("")
