@@ -172,6 +172,16 @@ typedef struct
    float wy[4];
 } iscg_splat_emitter;
 
+/* Replaces WithHaloCut / HaloPlaneSpans (Sim/PsfGeneratorBridge.cpp:1137-1243): the halo cut of a kernel stack
+ * (nz sum-1 planes of size x size, host or device) at threshold (a pixel's share of an emitter's photons; Cubic and
+ * Fft reads keep a 1.25^2 margin). Per plane and dy in [-camRad, camRad], camRad = ((size - 1) / 2) / os: the
+ * first and last kept dx into lo, hi (nz x (2 camRad + 1) int16, host or device; lo > hi: none) and the largest
+ * kept dx^2 + dy^2 into radius2 (nz int32; -1: none); keptFraction (may be NULL) the kept share of the squares.
+ * Returns camRad, 0 when the reference makes no cut (threshold <= 0 or camRad < 1: outputs untouched), -1 on
+ * error. Synchronizes the context's stream. */
+int32_t iscg_halo_spans(iscg_context* h, const float* planes, int32_t nz, int32_t size, int32_t oversampling,
+                        int32_t interpMode, double threshold, int16_t* lo, int16_t* hi, int32_t* radius2,
+                        double* keptFraction);
 /* Replaces SplatGpuHost::SetKernels / GpuSimulator::SetKernels (Sim/SplatGpu.h:50, Sim/GpuSimD3D11.h:75):
  * up to 4 kernels, resident until replaced. */
 int32_t iscg_splat_set_kernels(iscg_context* h, const iscg_kernel_view* kernels, int32_t n);

@@ -105,6 +105,7 @@ SIGNATURES = {
     "iscg_psf_library_geometry": (I32, [P, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
     "iscg_psf_library_run": (I32, [P, P, I32, P, C.c_size_t]),
     "iscg_psf_library_destroy": (I32, [P, P]),
+    "iscg_halo_spans": (I32, [P, P, I32, I32, I32, I32, D, P, P, P, C.POINTER(D)]),
     "iscg_autotune_set": (I32, [P, I32]),
     "iscg_autotune_dump": (I32, [P, C.c_char_p, I32]),
     "iscg_autotune_load": (I32, [P, C.c_char_p]),

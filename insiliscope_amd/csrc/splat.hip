@@ -56,6 +56,7 @@ struct SplatState
    // the Fft placement: per-emitter shifted block sums, per-block scratch, radix-2 tables of length fftN
    DevBuf fftSums, fftScratch, fftTables;
    int fftN = 0;
+   DevBuf haloSums, haloOut;  // iscg_halo_spans: block sums, then [lo | hi | radius2 | kept] per plane
 };
 
 void destroy_splat(iscg_context* h)
@@ -831,6 +832,68 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
 
 using namespace iscg;
 
+namespace iscg {
+
+// HaloPlaneSpans (Sim/PsfGeneratorBridge.cpp:1137-1186) for one plane per workgroup, from its block sums B (bw x bw,
+// row-major). Camera pixel (dx, dy) of a splat reads block sums in rows [halfOv - 2 + dy os, halfOv + os + 1 + dy os]
+// and the same columns for dx (clipped to the table); the pixel is kept when the largest of them (the maximum starts
+// at 0 and takes a value only when it is larger, as std::max(m, v)) is >= thr. Per dy: the first and last kept dx
+// (lo > hi: none); per plane: the largest dy^2 + max(lo^2, hi^2) kept (-1: none) and the kept pixel count.
+// Maxima and the comparison are exact, so the spans are those of the reference bit for bit.
+__global__ __launch_bounds__(256) void halo_spans_kernel(const float* __restrict__ sums, int bw, int os, int halfOv,
+                                                         int camRad, double thr, int16_t* __restrict__ lo,
+                                                         int16_t* __restrict__ hi, int32_t* __restrict__ radius2,
+                                                         long long* __restrict__ kept)
+{
+   __shared__ int r2s;
+   __shared__ unsigned long long keptS;
+   const int z = (int)blockIdx.x, n = 2 * camRad + 1;
+   const float* B = sums + (size_t)z * bw * bw;
+   if (threadIdx.x == 0)
+   {
+      r2s = -1;
+      keptS = 0;
+   }
+   __syncthreads();
+   for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x)
+   {
+      const int dy = i - camRad;
+      const int r0 = max(0, halfOv - 2 + dy * os), r1 = min(bw - 1, halfOv + os + 1 + dy * os);
+      int l = camRad + 1, h = -camRad - 1;
+      for (int dx = -camRad; dx <= camRad; ++dx)
+      {
+         const int c0 = max(0, halfOv - 2 + dx * os), c1 = min(bw - 1, halfOv + os + 1 + dx * os);
+         float m = 0.0f;
+         for (int r = r0; r <= r1; ++r)
+            for (int c = c0; c <= c1; ++c)
+            {
+               const float v = B[(size_t)r * bw + c];
+               m = m < v ? v : m;
+            }
+         if ((double)m >= thr)
+         {
+            l = min(l, dx);
+            h = max(h, dx);
+         }
+      }
+      lo[(size_t)z * n + i] = (int16_t)l;
+      hi[(size_t)z * n + i] = (int16_t)h;
+      if (l <= h)
+      {
+         atomicAdd(&keptS, (unsigned long long)(h - l + 1));
+         atomicMax(&r2s, dy * dy + max(l * l, h * h));
+      }
+   }
+   __syncthreads();
+   if (threadIdx.x == 0)
+   {
+      radius2[z] = r2s;
+      kept[z] = (long long)keptS;
+   }
+}
+
+}  // namespace iscg
+
 extern "C" {
 
 int32_t iscg_splat_set_kernels(iscg_context* h, const iscg_kernel_view* kernels, int32_t n)
@@ -897,6 +960,60 @@ int32_t iscg_splat_set_kernels(iscg_context* h, const iscg_kernel_view* kernels,
    }
    s.nk = n;
    return 0;
+}
+
+int32_t iscg_halo_spans(iscg_context* h, const float* planes, int32_t nz, int32_t size, int32_t oversampling,
+                        int32_t interpMode, double threshold, int16_t* lo, int16_t* hi, int32_t* radius2,
+                        double* keptFraction)
+{
+   if (begin(h))
+      return -1;
+   if (!planes || nz < 1 || size < 1 || (size % 2) != 1 || interpMode < 0 || interpMode > 3)
+      return fail(h, "iscg_halo_spans: bad kernel (planes, odd size, nz >= 1, interp 0-3)");
+   const int os = oversampling > 1 ? oversampling : 1, halfOv = (size - 1) / 2, camRad = halfOv / os;
+   if (!(threshold > 0.0) || camRad < 1 || camRad > 16000)
+      return 0;  // no cut: the whole square (WithHaloCut returns the kernel unchanged)
+   if (!lo || !hi || !radius2)
+      return fail(h, "iscg_halo_spans: null output");
+   const int n = 2 * camRad + 1, bw = size + os - 1;
+   // Catmull-Rom weights sum to at most 1.25 in absolute value per axis: Cubic and Fft reads keep a 1.25^2 margin
+   const double overshoot = interpMode == 2 || interpMode == 3 ? 1.5625 : 1.0;
+   SplatState& s = state(h);
+   const size_t planeBytes = (size_t)nz * size * size * sizeof(float);
+   const size_t spanBytes = (size_t)nz * n * sizeof(int16_t);
+   const size_t outBytes = 2 * spanBytes + (size_t)nz * (sizeof(int32_t) + sizeof(long long)) + 16;
+   const float* dPlanes = (const float*)stage_in(h, 0, planes, planeBytes);
+   if (!dPlanes || !ok(h, s.haloSums.reserve((size_t)nz * bw * bw * sizeof(float)), "halo allocation") ||
+       !ok(h, s.haloOut.reserve(outBytes), "halo allocation"))
+      return -1;
+   char* o = (char*)s.haloOut.p;
+   int16_t* dLo = (int16_t*)o;
+   int16_t* dHi = dLo + (size_t)nz * n;
+   int32_t* dR2 = (int32_t*)(o + ((2 * spanBytes + 7) & ~(size_t)7));
+   long long* dKept = (long long*)((char*)dR2 + (((size_t)nz * sizeof(int32_t) + 7) & ~(size_t)7));
+   hipLaunchKernelGGL(block_sums_kernel, dim3(grid_for((size_t)nz * bw * bw, 256, 4)), dim3(256), 0, h->stream,
+                      dPlanes, s.haloSums.as<float>(), size, os, nz);
+   hipLaunchKernelGGL(halo_spans_kernel, dim3((unsigned)nz), dim3(256), 0, h->stream,
+                      (const float*)s.haloSums.as<float>(), bw, os, halfOv, camRad, threshold / overshoot, dLo, dHi,
+                      dR2, dKept);
+   if (!ok(h, hipGetLastError(), "halo spans launch"))
+      return -1;
+   std::vector<long long> kept((size_t)nz);
+   if (!ok(h, hipMemcpyAsync(lo, dLo, spanBytes, hipMemcpyDefault, h->stream), "halo copy") ||
+       !ok(h, hipMemcpyAsync(hi, dHi, spanBytes, hipMemcpyDefault, h->stream), "halo copy") ||
+       !ok(h, hipMemcpyAsync(radius2, dR2, (size_t)nz * sizeof(int32_t), hipMemcpyDefault, h->stream), "halo copy") ||
+       !ok(h, hipMemcpyAsync(kept.data(), dKept, (size_t)nz * sizeof(long long), hipMemcpyDefault, h->stream),
+           "halo copy") ||
+       !ok(h, hipStreamSynchronize(h->stream), "halo spans"))
+      return -1;
+   if (keptFraction)
+   {
+      double sum = 0.0;
+      for (long long k : kept)
+         sum += (double)k;
+      *keptFraction = sum / ((double)n * n * nz);
+   }
+   return camRad;
 }
 
 int32_t iscg_splat_set_static(iscg_context* h, uint32_t width, uint32_t height, uint32_t pixelBase,

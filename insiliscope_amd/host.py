@@ -10,8 +10,10 @@ Array arguments may be numpy arrays (host memory, staged by the library) or torc
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import itertools
 import sys
+import threading
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -311,6 +313,8 @@ class PsfHaloSpans:
     radius2: np.ndarray     # int32 (nz,)
     camRad: int
     threshold: float = 0.0
+    keptFraction: float = 0.0
+    serial: int = field(default_factory=lambda: next(_serials))
 
 
 @dataclass
@@ -344,6 +348,11 @@ class PsfKernelCache:
     def camRad(self) -> int:
         return self.halfWidthOversampled // self.oversampling
 
+    @property
+    def SplatSerial(self) -> int:
+        """What a splat reads (Sim/PsfGeneratorBridge.h:273): the halo cut's serial, else the planes'."""
+        return self.halo.serial if self.halo is not None else self.serial
+
     def NearestZIndex(self, zUm):
         """PsfKernelCache::NearestZIndex (Sim/PsfGeneratorBridge.cpp:40-60), vectorised: (index, clamped)."""
         z = np.asarray(zUm, np.float64)
@@ -354,6 +363,49 @@ class PsfKernelCache:
         idx = np.where(idx_f >= 0, np.floor(idx_f + 0.5), np.ceil(idx_f - 0.5)).astype(np.int64)
         clamped = (idx < 0) | (idx >= self.nz)
         return np.clip(idx, 0, self.nz - 1).astype(np.int32), clamped
+
+
+_halo_memo: list = []   # ((serial, threshold + 10 interpMode), PsfHaloSpans), newest first, at most 8
+_halo_lock = threading.Lock()
+
+
+def WithHaloCut(cache: PsfKernelCache, threshold: float, ctx: Context | None = None) -> PsfKernelCache:
+    """sim::WithHaloCut (Sim/PsfGeneratorBridge.cpp:1188-1243): a copy of cache whose splats leave out every camera
+    pixel whose share of an emitter's photons is below threshold (the spans computed on the GPU, iscg_halo_spans);
+    threshold <= 0 or a kernel under one camera pixel: the whole square. Spans are kept per (kernel serial,
+    threshold, interpolation), the last 8."""
+    out = dataclasses.replace(cache, halo=None)
+    os_ = max(1, cache.oversampling)
+    camRad = cache.halfWidthOversampled // os_
+    if not threshold > 0.0 or camRad < 1 or camRad > 16000:
+        return out
+    key = (cache.serial, threshold + 10.0 * int(cache.interpMode))
+    with _halo_lock:
+        for k, spans in _halo_memo:
+            if k == key:
+                out.halo = spans
+                return out
+    ctx = ctx or default_context()
+    nz, n = cache.nz, 2 * camRad + 1
+    lo = np.zeros((nz, n), np.int16)
+    hi = np.zeros((nz, n), np.int16)
+    r2 = np.zeros(nz, np.int32)
+    kept = C.c_double()
+    planes, keep = _ptr(_arr(cache.planes, np.float32))
+    ctx.follow_torch()
+    rc = ctx.L.iscg_halo_spans(ctx.h, planes, nz, cache.sizeOversampled, os_, int(cache.interpMode), float(threshold),
+                               lo.ctypes.data, hi.ctypes.data, r2.ctypes.data, C.byref(kept))
+    del keep
+    if rc < 0:
+        raise IscgError("iscg_halo_spans: " + ctx.error())
+    spans = PsfHaloSpans(lo, hi, r2, camRad=camRad, threshold=float(threshold))
+    spans.keptFraction = kept.value
+    spans.serial = next(_serials)
+    with _halo_lock:
+        _halo_memo.insert(0, (key, spans))
+        del _halo_memo[8:]
+    out.halo = spans
+    return out
 
 
 def CollectFrameEmitters(events, frameIndex: int, pixelSizeNm: float, photonsPerBlink: float,
@@ -455,7 +507,7 @@ class SplatGpuHost:
                 lo, a1 = _ptr(np.ascontiguousarray(k.halo.lo, np.int16))
                 hi, a2 = _ptr(np.ascontiguousarray(k.halo.hi, np.int16))
                 keep += [a1, a2]
-            views[i] = _lib.KernelView(pl, lo, hi, k.zStepNm, k.serial, k.nz, k.sizeOversampled, k.oversampling,
+            views[i] = _lib.KernelView(pl, lo, hi, k.zStepNm, k.SplatSerial, k.nz, k.sizeOversampled, k.oversampling,
                                        k.interpMode)
         self.kernels = list(kernels)
         self.ctx.follow_torch()

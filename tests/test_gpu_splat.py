@@ -205,3 +205,28 @@ def test_splat_fft_placement_many_emitters_vs_oracle(ctx, orc, golden):
     assert not sp.RenderBatch(rec, np.array([0, len(rec)], np.uint32), np.zeros(1, np.uint32), None, 0,
                               outPhotons=ph[:1], records=True)
     assert "Fft" in sp.err
+
+
+@pytest.mark.parametrize("interp", [0, 1, 2, 3])
+def test_halo_spans_bit_exact_vs_reference(ctx, orc, interp):
+    """WithHaloCut's spans from the GPU (block sums, then per-pixel window maxima) equal the compiled reference's
+    lo, hi and radius2 at C4's kernel geometry (hw 8, os 4) and at os 1 and 3, over thresholds from a full to an
+    empty cut; a new cut gets a new splat serial, the same cut again the same spans."""
+    ref = O.Reference()
+    req = configs.c4_psf(seed=1)
+    req.update(nz=9, pupilSamples=64)
+    for os_, hw in ((4, 8), (1, 12), (3, 5)):
+        req.update(oversampling=os_, kernelHalfWidthPx=hw)
+        planes = orc.psf_planes(O.psf_request(**req), normalize=True)
+        cache = host.PsfKernelCache(planes, oversampling=os_, zStepNm=15.0, interpMode=interp)
+        serials = set()
+        for thr in (1e-7, 3e-4, 1e-3, 1e-2, 0.3):
+            c = host.WithHaloCut(cache, thr, ctx=ctx)
+            lo, hi, r2 = ref.halo_spans(planes, os_, interp, thr)
+            assert np.array_equal(c.halo.lo, lo) and np.array_equal(c.halo.hi, hi), (os_, thr)
+            assert np.array_equal(c.halo.radius2, r2), (os_, thr)
+            assert c.serial == cache.serial and c.SplatSerial == c.halo.serial
+            serials.add(c.halo.serial)
+            assert host.WithHaloCut(cache, thr, ctx=ctx).halo is c.halo
+        assert len(serials) == 5
+    assert host.WithHaloCut(cache, 0.0, ctx=ctx).halo is None
