@@ -408,6 +408,34 @@ def WithHaloCut(cache: PsfKernelCache, threshold: float, ctx: Context | None = N
     return out
 
 
+def _to_long(x):
+    """static_cast<long>(std::floor(x)) as x86-64 evaluates it: values outside the long range, and NaN, give
+    LONG_MIN (the conversion's "integer indefinite")."""
+    f = np.floor(np.asarray(x, np.float64))
+    ok = np.isfinite(f) & (f >= -2.0 ** 63) & (f < 2.0 ** 63)
+    return np.where(ok, np.where(ok, f, 0.0).astype(np.int64), np.iinfo(np.int64).min)
+
+
+def BucketEventsByFrame(events, nFrames: int):
+    """sim::BucketEventsByFrame (Sim/SMLMSimulation.cpp:203-217): for each frame f < nFrames the indices of the
+    events that overlap it (tEnd > 0, tStart < nFrames, f in [floor(tStart), floor(tEnd)] clipped to the movie), in
+    event order. Returns (start, idx): frame f's list is idx[start[f]:start[f + 1]] (start int64, idx uint32)."""
+    ev = np.asarray(events, dtype=BLINK_EVENT_DTYPE)
+    n = max(0, int(nFrames))
+    ts, te = ev["tStart"], ev["tEnd"]
+    keep = (te > 0.0) & (ts < float(n))
+    f0 = np.maximum(0, _to_long(ts))
+    f1 = np.minimum(n - 1, _to_long(te))
+    cnt = np.where(keep & (f1 >= f0), f1 - np.minimum(f0, f1) + 1, 0).astype(np.int64)
+    which = np.repeat(np.arange(len(ev), dtype=np.int64), cnt)
+    first = np.repeat(np.cumsum(cnt) - cnt, cnt)
+    frame = np.repeat(f0, cnt) + (np.arange(len(which), dtype=np.int64) - first)
+    order = np.argsort(frame, kind="stable")
+    start = np.zeros(n + 1, np.int64)
+    start[1:] = np.cumsum(np.bincount(frame, minlength=n)[:n]) if n else []
+    return start, which[order].astype(np.uint32)
+
+
 def CollectFrameEmitters(events, frameIndex: int, pixelSizeNm: float, photonsPerBlink: float,
                          driftOffsetXPx: float = 0.0, driftOffsetYPx: float = 0.0,
                          psfCache: PsfKernelCache | None = None, globalZOffsetUm: float = 0.0, kernel: int = 0):

@@ -493,6 +493,8 @@ class Reference:
         L.ref_psf_half_width_px.restype = I32
         L.ref_capped_psf_oversampling.argtypes = [I32, D, D]
         L.ref_capped_psf_oversampling.restype = I32
+        L.ref_bucket_events.argtypes = [C.c_void_p, I32, I64, C.POINTER(C.c_int64), C.POINTER(C.c_uint32), I64]
+        L.ref_bucket_events.restype = I64
         L.ref_parse_zernike.argtypes = [C.c_char_p, PD]
         L.ref_parse_zernike.restype = I32
         L.ref_format_zernike.argtypes = [PD, C.c_char, C.c_char_p, I32]
@@ -735,6 +737,19 @@ class Reference:
 
     def capped_psf_oversampling(self, os_, pixel_nm, max_sample_nm):
         return self.L.ref_capped_psf_oversampling(os_, pixel_nm, max_sample_nm)
+
+    def bucket_events(self, events, n_frames):
+        """BucketEventsByFrame: (counts per frame, the concatenated index lists)."""
+        events = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+        counts = np.zeros(max(0, n_frames), np.int64)
+        cap = 4 * len(events) + 16
+        while True:
+            idx = np.zeros(cap, np.uint32)
+            n = self.L.ref_bucket_events(events.ctypes.data, len(events), n_frames, _p(counts, C.c_int64),
+                                         _p(idx, C.c_uint32), cap)
+            if n >= 0:
+                return counts, idx[:n]
+            cap *= 4
 
     def parse_zernike(self, text: str):
         """ParseZernikeCoefficients: (28 values, ok)."""

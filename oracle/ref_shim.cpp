@@ -853,3 +853,25 @@ int32_t ref_compute_psf_kernel_cache(const orc_psf_request* r, const char* zerni
 }
 
 } // extern "C"
+
+extern "C" {
+
+// BucketEventsByFrame (Sim/SMLMSimulation.cpp:203): per frame the indices of the events it overlaps, in event order;
+// counts[nFrames], then the lists concatenated into idx (cap entries). Returns the total, or -1 if over cap.
+int64_t ref_bucket_events(const ref_event* ev, int32_t nEv, int64_t nFrames, int64_t* counts, uint32_t* idx,
+                          int64_t cap)
+{
+   const std::vector<std::vector<uint32_t>> b = sim::BucketEventsByFrame(ToEvents(ev, nEv), static_cast<long>(nFrames));
+   int64_t total = 0;
+   for (size_t f = 0; f < b.size(); ++f)
+   {
+      counts[f] = static_cast<int64_t>(b[f].size());
+      if (total + counts[f] > cap)
+         return -1;
+      std::memcpy(idx + total, b[f].data(), b[f].size() * sizeof(uint32_t));
+      total += counts[f];
+   }
+   return total;
+}
+
+} // extern "C"

@@ -177,3 +177,25 @@ def test_psf_geometry_helpers_equal_reference(golden):
         assert host.PsfKernelHalfWidthPx(*args) == want
     for (o, p, m), want in zip(G["capped_in"], G["capped_out"]):
         assert host.CappedPsfOversampling(int(o), p, m) == want
+
+
+def test_bucket_events_by_frame_vs_reference():
+    """host.BucketEventsByFrame against the compiled reference's BucketEventsByFrame: events spanning frames,
+    before the movie, after it, ending at 0, fractional, NaN and infinite times; identical lists in event order."""
+    import oracle as O
+    from insiliscope_amd import host
+    rng = np.random.default_rng(11)
+    n = 4000
+    ev = np.zeros(n, host.BLINK_EVENT_DTYPE)
+    ts = rng.uniform(-5.0, 60.0, n)
+    ev["tStart"] = ts
+    ev["tEnd"] = ts + rng.exponential(1.5, n)
+    special = [(-1e300, 2.5), (3.0, np.inf), (np.nan, 4.0), (2.0, np.nan), (-np.inf, 0.0), (49.999, 50.0),
+               (50.0, 51.0), (0.0, 0.0), (-0.5, 1e-300), (7.0, 3.0), (-2.0, 1e30)]
+    for i, (a, b) in enumerate(special):
+        ev["tStart"][i], ev["tEnd"][i] = a, b
+    R = O.Reference()
+    for nF in (0, 1, 50):
+        start, idx = host.BucketEventsByFrame(ev, nF)
+        counts, want = R.bucket_events(ev, nF)
+        assert np.array_equal(np.diff(start), counts) and np.array_equal(idx, want), nF
