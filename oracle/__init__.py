@@ -170,6 +170,7 @@ class Oracle:
         L.orc_wf_radius.restype = I32
         L.orc_wf_kernel.argtypes = [PF, I32, I32, I32, I32, PF]
         L.orc_wf_direct.argtypes = [PF, I32, I32, I32, D, I64, PF, I32, I32, D, PD, I32, PD]
+        L.orc_wf_plane_weight.argtypes = [D, D, I32, D, C.POINTER(I32), PD]
         L.orc_roi_forward.argtypes = [C.POINTER(RoiModel), PD, C.c_void_p, I32, PD]
         L.orc_roi_backward.argtypes = [C.POINTER(RoiModel), PD, C.c_void_p, I32, PD, PD, PD]
         L.orc_roi_backward.restype = D
@@ -348,6 +349,12 @@ class Oracle:
         out = np.zeros((2 * R + 1, 2 * R + 1), np.float32)
         self.L.orc_wf_kernel(_p(plane, C.c_float), plane.shape[0], os_, upscale, R, _p(out, C.c_float))
         return out
+
+    def wf_plane_weight(self, centre_um, focus_um, nzP, z_step_nm):
+        """PlanFocus's blend of one dye plane at one focus: (p0, fr): plane p0 weighs 1 - fr, plane p0 + 1 fr."""
+        p0, fr = C.c_int32(), C.c_double()
+        self.L.orc_wf_plane_weight(centre_um, focus_um, nzP, z_step_nm, C.byref(p0), C.byref(fr))
+        return p0.value, fr.value
 
     def wf_direct(self, density, dz_um, k0, kernels, z_step_nm, foci_um):
         density = np.ascontiguousarray(density, np.float32)

@@ -1169,12 +1169,32 @@ void orc_wf_kernel(const float* P, int32_t n, int32_t os, int32_t upscale, int32
 
 /* cli/widefield_check.cpp:287-329 DirectImage (before the crop, max and bin), with PlanFocus's plane coordinate
  * (Sim/WidefieldRender.cpp:1476-1490) and KernelWidefieldPsf::PlaneCoord (:285). */
+/* PlanFocus's plane blend for one dye plane and focus (Sim/WidefieldRender.cpp:1460-1524): t = the PSF plane
+ * coordinate of (centre - focus), clamped to [0, nzP - 1]; plane p0 = floor(t) with weight 1 - fr, plane p0 + 1 with
+ * fr (fr = 0 at the last plane). */
+void orc_wf_plane_weight(double centreUm, double focusUm, int32_t nzP, double zStepNm, int32_t* p0Out,
+                         double* frOut)
+{
+   const int maxPlane = imax(0, nzP - 1);
+   double t = (nzP <= 1) ? 0.0 : (centreUm - focusUm) * 1000.0 / zStepNm + (nzP - 1) / 2.0;
+   t = t < 0.0 ? 0.0 : (t > maxPlane ? maxPlane : t);
+   int p0 = (int)floor(t);
+   double fr = t - p0;
+   if (p0 >= maxPlane)
+   {
+      p0 = maxPlane;
+      fr = 0.0;
+   }
+   *p0Out = p0;
+   *frOut = fr;
+}
+
 void orc_wf_direct(const float* density, int32_t nx, int32_t ny, int32_t nzD, double dzUm, int64_t k0,
                    const float* kernels, int32_t nzP, int32_t R, double zStepNm, const double* focusUm, int32_t nF,
                    double* out)
 {
    const size_t n2 = (size_t)nx * ny;
-   const int D = 2 * R + 1, maxPlane = imax(0, nzP - 1);
+   const int D = 2 * R + 1;
    memset(out, 0, sizeof(double) * n2 * (size_t)nF);
    for (int32_t f = 0; f < nF; ++f)
    {
@@ -1182,15 +1202,9 @@ void orc_wf_direct(const float* density, int32_t nx, int32_t ny, int32_t nzD, do
       for (int32_t i = 0; i < nzD; ++i)
       {
          const double centre = ((double)(k0 + i) + 0.5) * dzUm;
-         double t = (nzP <= 1) ? 0.0 : (centre - focusUm[f]) * 1000.0 / zStepNm + (nzP - 1) / 2.0;
-         t = t < 0.0 ? 0.0 : (t > maxPlane ? maxPlane : t);
-         int p0 = (int)floor(t);
-         double fr = t - p0;
-         if (p0 >= maxPlane)
-         {
-            p0 = maxPlane;
-            fr = 0.0;
-         }
+         int32_t p0;
+         double fr;
+         orc_wf_plane_weight(centre, focusUm[f], nzP, zStepNm, &p0, &fr);
          for (int s = 0; s < 2; ++s)
          {
             const double w = s ? fr : 1 - fr;

@@ -133,6 +133,8 @@ void orc_wf_kernel(const float* plane, int32_t sizeOversampled, int32_t os, int3
  * (k0 + i + 0.5) dz um) with PSF planes (nzP planes of (2R+1)^2), linear z blend with clamp at the end planes as
  * PlanFocus: t = ((k0 + i + 0.5) dz - focus_f) * 1000 / zStepNm + (nzP - 1) / 2. out: nF x ny x nx, FP64, before
  * max(0, .) and binning. */
+void orc_wf_plane_weight(double centreUm, double focusUm, int32_t nzP, double zStepNm, int32_t* p0Out,
+                         double* frOut);
 void orc_wf_direct(const float* density, int32_t nx, int32_t ny, int32_t nzD, double dzUm, int64_t k0,
                    const float* kernels, int32_t nzP, int32_t R, double zStepNm, const double* focusUm, int32_t nF,
                    double* out);

@@ -38,3 +38,13 @@ def ctx():
     c = Context(0)
     yield c
     c.close()
+
+
+def record_accuracy(case: str, **values):
+    """Appends one measured accuracy record (max abs and relative L2 errors against the FP64 oracle, ...) as a JSON
+    line to the file named by ISCG_ACCURACY_OUT, when set; the GPU tests call it where they measure an error."""
+    path = os.environ.get("ISCG_ACCURACY_OUT")
+    if path:
+        import json
+        with open(path, "a") as f:
+            f.write(json.dumps({"case": case, **{k: float(v) for k, v in values.items()}}) + "\n")

@@ -6,6 +6,7 @@ image <= 1e-5 per frame; frames nothing reaches are left bit for bit."""
 import numpy as np
 import pytest
 
+from conftest import record_accuracy
 from insiliscope_amd import host
 
 pytestmark = pytest.mark.gpu
@@ -53,12 +54,15 @@ def test_binned_vs_reference(ctx, golden, name):
     frames = [emitters(em[st[f]:st[f + 1]]) for f in range(len(st) - 1)]
     img = G[name + "_img0"].copy()
     assert r.RenderFrames(frames, img), r.err
+    errs = [0.0]
     for f in range(len(frames)):
         if not len(frames[f]):
             assert np.array_equal(img[f], G[name + "_img0"][f])
             continue
         e = rel_l2(img[f] - G[name + "_img0"][f], G[name + "_img"][f] - G[name + "_img0"][f])
+        errs.append(e)
         assert e <= TOL, (name, f, e)
+    record_accuracy("binned_" + name + "_vs_compiled_reference", rel_l2_max=max(errs))
 
 
 def test_binned_single_frame_render_alone_and_device_images(ctx, golden, orc):

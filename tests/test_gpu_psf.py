@@ -8,6 +8,7 @@ import pytest
 
 import oracle as O
 from insiliscope_amd import configs, host
+from conftest import record_accuracy
 from test_oracle import ZREF_CASES, GOLDEN
 
 pytestmark = pytest.mark.gpu
@@ -31,11 +32,16 @@ def test_psf_vs_zernike_ref_bin(ctx):
                          sampleDepthNm=depth, pixelSizeNm=100.0, oversampling=4, kernelHalfWidthPx=8, nz=3,
                          zStepNm=100.0, maskModes=5, maskWaist=1.0, pupilSamples=64, zernike=zern, maskType=mask))
     got = host.ComputePsfPlanesBatch(reqs, ctx=ctx)
+    errs, mx = [], []
     for c in range(cases):
         for z in range(nz):
             e = rel_l2(got[c, z], ref[c, z])
             print(ZREF_CASES[c][0], z, e, flush=True)
+            errs.append(e)
+            mx.append(np.abs(got[c, z] - ref[c, z]).max() / np.abs(ref[c, z]).max())
             assert e <= TOL and np.argmax(got[c, z]) == np.argmax(ref[c, z])
+    record_accuracy("psf_zernike_ref_bin_4_cases_vs_reference_fixture", rel_l2_max=max(errs),
+                    max_abs_rel_to_peak=max(mx))
 
 
 def test_psf_c1_all_planes_vs_oracle(ctx, orc, golden):
@@ -46,6 +52,9 @@ def test_psf_c1_all_planes_vs_oracle(ctx, orc, golden):
     assert got.shape == want.shape == (41, 33, 33)
     errs = [rel_l2(got[z], want[z]) for z in range(41)]
     print("C1 max rel L2", max(errs), flush=True)
+    record_accuracy("psf_c1_41_planes_33x33_vs_fp64_oracle", rel_l2_max=max(errs),
+                    max_abs_rel_to_peak=max(np.abs(got[z] - want[z]).max() / np.abs(want[z]).max()
+                                            for z in range(41)))
     assert max(errs) <= TOL
     for j, z in enumerate(G["c1_planes_idx"]):
         assert rel_l2(got[z], G["c1_planes"][j]) <= TOL
@@ -56,10 +65,13 @@ def test_psf_c3_batch_vs_reference(ctx, golden):
     reqs = configs.c3_requests(seed=1)
     pick = [int(i) for i in G["c3_pick"]]
     got = host.ComputePsfPlanesBatch([reqs[i] for i in pick], ctx=ctx)
+    errs = []
     for j in range(len(pick)):
         for k, z in enumerate((0, 20, 40)):
             e = rel_l2(got[j, z], G["c3_planes"][j, k])
+            errs.append(e)
             assert e <= TOL, (j, z, e)
+    record_accuracy("psf_c3_65x65_vs_compiled_reference_fp64", rel_l2_max=max(errs))
 
 
 def test_psf_normalized_and_wide_kernel(ctx, orc):

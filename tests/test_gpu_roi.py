@@ -6,6 +6,7 @@ import pytest
 
 import oracle as O
 from insiliscope_amd import configs, host
+from conftest import record_accuracy
 from test_oracle import roi_case
 
 pytestmark = pytest.mark.gpu
@@ -22,6 +23,8 @@ def test_roi_forward_vs_oracle(ctx, orc):
     want = orc.roi_forward(omodel(m), z, p)
     err = np.sqrt(((mu - want) ** 2).sum(axis=(1, 2)) / (want ** 2).sum(axis=(1, 2)))
     print("forward max rel L2", err.max(), flush=True)
+    record_accuracy("roi_forward_64_rois_vs_fp64_oracle", rel_l2_max=err.max(),
+                    max_abs_rel_to_peak=(np.abs(mu - want).max(axis=(1, 2)) / np.abs(want).max(axis=(1, 2))).max())
     assert err.max() <= 1e-5
 
 
@@ -42,6 +45,9 @@ def test_roi_backward_vs_oracle(ctx, orc):
     sel = [configs.NOLL_TO_OSA[j] for j in range(4, 25)]
     err = np.abs(gz[sel] - gz0[sel]).max() / np.abs(gz0[sel]).max()
     print("zernike grad rel", err, flush=True)
+    record_accuracy("roi_gradients_64_rois_vs_fp64_oracle", zernike_max_rel=err, nll_rel=abs(nll - nll0) / abs(nll0),
+                    param_max_rel=max(np.abs(gp[:, k] - gp0[:, k]).max() / np.abs(gp0[:, k]).max()
+                                      for k in range(5)))
     assert err <= 1e-4
 
 

@@ -1,10 +1,13 @@
 """The mean-field focus series on the GPU (iscg_wf_conv3d: one 3D FFT convolution) against the oracle's direct
 per-focus convolution (cli/widefield_check.cpp:287-343 DirectImage, PlanFocus's blend and clamp): FP32 path vs FP64,
 relative L2 <= 1e-5 per focus; the fused camera noise bit-exact against the oracle's chain on the same frames."""
+import os
+
 import numpy as np
 import pytest
 
 import oracle as O
+from conftest import record_accuracy
 from insiliscope_amd import host
 
 pytestmark = pytest.mark.gpu
@@ -46,9 +49,11 @@ def test_wf_focus_series_vs_direct(ctx, orc, golden, delta_um, fov, fft):
     want = orc.wf_direct(dens, dz, k0, K, 50.0, foci)
     x0, y0 = fov[0], fov[1]
     want = np.maximum(want[:, y0:y0 + ch, x0:x0 + cw], 0.0)
+    errs = [rel_l2(img[f], want[f]) for f in range(nF)]
+    record_accuracy(f"wf_small_fft{fft[1]}_delta{delta_um}_vs_fp64_direct", rel_l2_max=max(errs),
+                    max_abs_rel_to_peak=np.abs(img - want).max() / np.abs(want).max())
     for f in range(nF):
-        e = rel_l2(img[f], want[f])
-        assert e <= TOL, f"focus {f}: rel L2 {e:.3g}"
+        assert errs[f] <= TOL, f"focus {f}: rel L2 {errs[f]:.3g}"
 
 
 @pytest.mark.parametrize("fft", [(0, 0, 0), (64, 512, 128)])
@@ -130,3 +135,77 @@ def test_wf_slab_full_size_vs_five_pass(ctx, orc, golden):
     want = np.zeros((ny, nx))
     want[100 - R:100 + R + 1, 201 - R:201 + R + 1] = K
     assert rel_l2(img[0], want) < TOL
+
+
+def fp64_fft_stack(orc, dens, K, dz, k0, foci, z_step_nm, grid):
+    """The focus stack in FP64 by another route than the oracle's direct sum: out[f] = sum_k D[k] (*) H[f - k] as
+    one zero-padded 3D FFT convolution in float64 (scipy.fft), H[m] the PlanFocus blend (the oracle's weights) of
+    the resampled kernels K. Requires foci spaced by dz (the blend then depends on f - k only, up to the rounding of
+the plane coordinate; asserted)."""
+    import scipy.fft as sf
+    nzD, ny, nx = dens.shape
+    nzP, D, _ = K.shape
+    R = (D - 1) // 2
+    NZ, NY, NX = grid
+    lag = {}
+    for f, fo in enumerate(foci):
+        for k in range(nzD):
+            w = orc.wf_plane_weight((k0 + k + 0.5) * dz, fo, nzP, z_step_nm)
+            first = lag.setdefault(f - k, w)
+            assert abs((first[0] + first[1]) - (w[0] + w[1])) <= 1e-9  # one blend per lag, up to rounding
+    H = np.zeros((NZ, NY, NX))
+    iy = np.arange(-R, R + 1) % NY
+    ix = np.arange(-R, R + 1) % NX
+    for m, (p0, fr) in lag.items():
+        h = (1.0 - fr) * K[p0].astype(np.float64)
+        if fr != 0.0:
+            h = h + fr * K[p0 + 1].astype(np.float64)
+        H[m % NZ][np.ix_(iy, ix)] = h
+    Dp = np.zeros((NZ, NY, NX))
+    Dp[:nzD, :ny, :nx] = dens
+    w = min(16, os.cpu_count() or 1)
+    out = sf.irfftn(sf.rfftn(Dp, workers=w) * sf.rfftn(H, workers=w), s=(NZ, NY, NX), workers=w)
+    return np.maximum(out[:len(foci), :ny, :nx], 0.0)
+
+
+def test_wf_c2_full_size_vs_fp64(ctx, orc, golden):
+    """C2 at its real shape and inputs: the seeded 256 x 256 x 64 density, the 127-plane Gibson-Lanni PSF of 257 x 257
+    at 100 nm (computed on the GPU, as the bench does), 64 foci at the dye-plane centres, the 512 x 512 x 128 slab
+    path, FP32 throughout; against an FP64 3D FFT convolution of the same density with the oracle's resampled
+    kernels and PlanFocus weights. That FP64 route is checked first against the oracle's direct sum (small case).
+    Per focus: relative L2 <= 1e-5; the max abs and relative-L2 errors are recorded."""
+    planes, dens = small_case(golden)
+    R = orc.wf_radius(65, 4, 1)
+    K = np.stack([orc.wf_kernel(p, 4, 1, R) for p in planes])
+    foci = (-3 + 0.5) * 0.05 + 0.013 + 0.05 * np.arange(7)
+    direct = np.maximum(orc.wf_direct(dens, 0.05, -3, K, 50.0, foci), 0.0)
+    fft64 = fp64_fft_stack(orc, dens, K, 0.05, -3, foci, 50.0, (32, 64, 64))
+    assert np.abs(fft64 - direct).max() <= 1e-9 * np.abs(direct).max()
+
+    from insiliscope_amd import configs
+    psf_req = configs.psf_request_fields(wavelengthNm=670.0, na=1.45, immersionIndex=1.518, sampleIndex=1.33,
+                                         workingDistanceUm=170.0, pixelSizeNm=100.0, zStepNm=50.0, oversampling=1,
+                                         kernelHalfWidthPx=128, nz=127, pupilSamples=256,
+                                         zernike=configs.zernike_noll(1, range(5, 16), amplitude=0.05))
+    psf = host.ComputePsfPlanesBatch([psf_req], normalize=True, ctx=ctx)[0]
+    cache = host.PsfKernelCache(psf, oversampling=1, zStepNm=50.0)
+    nx = ny = 256
+    nzD = nF = 64
+    wf = host.WidefieldStack(ctx)
+    assert wf.Plan(cache, nx, ny, nzD, 0.05, 0, 0.025, nF, fft=(512, 512, 128)), wf.err
+    dens = configs.c2_density(seed=1)
+    img = np.zeros(wf.image_shape, np.float32)
+    assert wf.FocusSeries(dens, outImages=img), wf.err
+    print("gpu stack done", flush=True)
+    R = orc.wf_radius(257, 1, 1)
+    K = np.stack([orc.wf_kernel(p, 1, 1, R) for p in psf])
+    foci = 0.025 + 0.05 * np.arange(nF)
+    want = fp64_fft_stack(orc, dens, K, 0.05, 0, foci, 50.0, (128, 512, 512))
+    errs = [rel_l2(img[f], want[f]) for f in range(nF)]
+    mx = [float(np.abs(img[f] - want[f]).max()) for f in range(nF)]
+    print("C2 per-focus rel L2: max %.3g median %.3g; max abs %.3g (image max %.4g)" % (
+        max(errs), float(np.median(errs)), max(mx), float(want.max())), flush=True)
+    record_accuracy("wf_c2_full_256x256x64_vs_fp64", rel_l2_max=max(errs), rel_l2_median=float(np.median(errs)),
+                    max_abs=max(mx), image_max=float(want.max()))
+    for f in range(nF):
+        assert errs[f] <= TOL, f"focus {f}: rel L2 {errs[f]:.3g}"
