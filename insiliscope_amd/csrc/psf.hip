@@ -500,6 +500,162 @@ __global__ __launch_bounds__(kPsfThreads, WLDS ? 2 : 4) void psf_small_kernel(co
    }
 }
 
+// Small planes, row-block form (planes of size P <= 72, MP in {32, 64, 128}): the same products as psf_small_kernel
+// (bit for bit: the same split operands, accumulated in the same order) with the work laid out per plane as
+//   1. each wave owns one 16-row block rb of the pupil (MP / 16 blocks; with fewer blocks than waves, G = 8 / (MP /
+//      16) waves share a block and split its column tiles): it evaluates those rows' FP64 phase and FP32 sincos
+//      straight into its MFMA A fragments (registers, no LDS), then mid[n][x] = sum_m Pup[n][m] W[x][m] for its
+//      column tiles x over all MP (K) -- the A fragments are reused across the tiles, each W fragment read once;
+//   2. mid, split and transposed, in LDS (midT[x][n], rows x < P only); E[y][x] = sum_n W[y][n] mid[n][x] for the
+//      wave's E tiles over all MP (K).
+// Two barriers per plane. LDS holds W and midT for the P valid rows only (rows >= P read as zeros): 2 x 4 x 72 x 136
+// halves at MP 128 (157 KB), one workgroup per CU. Workgroups per request (zg) and their XCD placement as in
+// psf_small_kernel.
+constexpr int kRbRows = 72;
+template <int MP>
+__global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfReqDev* __restrict__ reqs,
+                                                                     const double* __restrict__ A,
+                                                                     const double* __restrict__ B,
+                                                                     const _Float16* __restrict__ W,
+                                                                     const double* __restrict__ tz, int nz, int Pp,
+                                                                     int nReq, int zg, float* __restrict__ out)
+{
+   constexpr int PS = MP + 8;                 // LDS row (halves)
+   constexpr int NRB = MP / 16;               // pupil row blocks
+   constexpr int G = kPsfWaves / NRB;         // waves per row block
+   constexpr int KS = MP / 32;                // k-steps of 32
+   constexpr int CTX = kSmallPp / 16;         // max column tiles
+   constexpr int PM = (CTX + G - 1) / G;      // mid tiles per wave (max)
+   constexpr int PE = (CTX * CTX + kPsfWaves - 1) / kPsfWaves;  // E tiles per wave (max)
+   static_assert(NRB * G == kPsfWaves, "row blocks must divide the waves");
+   __shared__ __attribute__((aligned(16))) _Float16 wl[4][kRbRows][PS];
+   __shared__ __attribute__((aligned(16))) _Float16 midT[4][kRbRows][PS];
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int b = (int)blockIdx.x, slot = b / kXcds;
+   const int r = (slot / zg) * kXcds + b % kXcds;
+   if (r >= nReq)
+      return;  // the last XCD row's spare workgroups
+   const PsfReqDev& q = reqs[r];
+   const int P = q.size;
+   const int CT = Pp / 16, TE = CT * CT;
+   const size_t wPer = (size_t)Pp * MP;
+   const _Float16* Wr = W + (size_t)r * 4 * wPer;
+   const double* Ar = A + (size_t)r * MP * MP;
+   const double* Br = B + (size_t)r * MP * MP;
+   const int lr = lane & 15, lk = 8 * (lane >> 4);
+   const half8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+   for (int i = tid; i < 4 * P * (MP / 8); i += kPsfThreads)
+   {
+      const int part = i / (P * (MP / 8)), rem = i % (P * (MP / 8)), row = rem / (MP / 8), c8 = rem % (MP / 8);
+      *(half8*)&wl[part][row][8 * c8] = ld8(Wr + part * wPer + (size_t)row * MP + 8 * c8);
+   }
+   const int rb = wave % NRB, g0 = wave / NRB;
+   const int g = slot % zg;
+   const int z0 = g * nz / zg, z1 = (g + 1) * nz / zg;
+   for (int zi = z0; zi < z1; ++zi)
+   {
+      const double t = tz[(size_t)r * nz + zi];
+      // 1. this wave's pupil rows n = 16 rb + lr, columns m = 32 s + lk + j, as split A fragments
+      half8 prh[KS], prl[KS], pih[KS], pil[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+      {
+#pragma unroll
+         for (int j = 0; j < 8; ++j)
+         {
+            const size_t o = (size_t)(16 * rb + lr) * MP + 32 * ks + lk + j;
+            const double a = Ar[o];
+            float sn = 0.0f, c = 0.0f;
+            if (a == a)  // inside the aperture
+               wrapped_sincos(a + t * Br[o], &sn, &c);
+            const _Float16 ch = (_Float16)c, sh = (_Float16)sn;
+            prh[ks][j] = ch;
+            prl[ks][j] = (_Float16)(c - (float)ch);
+            pih[ks][j] = sh;
+            pil[ks][j] = (_Float16)(sn - (float)sh);
+         }
+         // one k-step's table loads in flight at a time (keeps the FP64 operands of all 32 samples out of registers)
+         __builtin_amdgcn_sched_barrier(0);
+      }
+      // (the previous plane's E tiles have read midT: the barrier at the end of the plane loop)
+#pragma unroll 1
+      for (int s = 0; s < PM; ++s)
+      {
+         const int ct = g0 + G * s;
+         if (ct >= CT)
+            continue;
+         const int x = 16 * ct + lr;
+         float4v mR = {0.0f, 0.0f, 0.0f, 0.0f}, mI = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+         for (int ks = 0; ks < KS; ++ks)
+         {
+            const bool in = x < P;
+            const half8 brh = in ? ld8(&wl[0][x][32 * ks + lk]) : zero8, brl = in ? ld8(&wl[1][x][32 * ks + lk]) : zero8;
+            const half8 bih = in ? ld8(&wl[2][x][32 * ks + lk]) : zero8, bil = in ? ld8(&wl[3][x][32 * ks + lk]) : zero8;
+            mR = mfma3(prh[ks], prl[ks], brh, brl, mR);
+            mR = mfma3(-pih[ks], -pil[ks], bih, bil, mR);
+            mI = mfma3(prh[ks], prl[ks], bih, bil, mI);
+            mI = mfma3(pih[ks], pil[ks], brh, brl, mI);
+         }
+         // D rows n = 16 rb + 4 (lane >> 4) + i, column x -> midT[.][x][n .. n + 3]
+         if (x < P)
+         {
+            const int n = 16 * rb + 4 * (lane >> 4);
+            typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+            half4 rh, rl, ih, il;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+            {
+               const _Float16 a = (_Float16)mR[i], bb = (_Float16)mI[i];
+               rh[i] = a;
+               rl[i] = (_Float16)(mR[i] - (float)a);
+               ih[i] = bb;
+               il[i] = (_Float16)(mI[i] - (float)bb);
+            }
+            *(half4*)&midT[0][x][n] = rh;
+            *(half4*)&midT[1][x][n] = rl;
+            *(half4*)&midT[2][x][n] = ih;
+            *(half4*)&midT[3][x][n] = il;
+         }
+      }
+      __syncthreads();
+      // 2. E tiles: E[y][x] = sum_n W[y][n] mid[n][x]
+      float* plane = out + ((size_t)r * nz + zi) * P * P;
+#pragma unroll 1
+      for (int s = 0; s < PE; ++s)
+      {
+         const int te = wave + kPsfWaves * s;
+         if (te >= TE)
+            continue;
+         const int ry = te / CT, cx = te % CT, y = 16 * ry + lr, x = 16 * cx + lr;
+         float4v eR = {0.0f, 0.0f, 0.0f, 0.0f}, eI = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+         for (int ks = 0; ks < KS; ++ks)
+         {
+            const bool iy = y < P, ix = x < P;
+            const int k = 32 * ks + lk;
+            const half8 arh = iy ? ld8(&wl[0][y][k]) : zero8, arl = iy ? ld8(&wl[1][y][k]) : zero8;
+            const half8 aih = iy ? ld8(&wl[2][y][k]) : zero8, ail = iy ? ld8(&wl[3][y][k]) : zero8;
+            const half8 brh = ix ? ld8(&midT[0][x][k]) : zero8, brl = ix ? ld8(&midT[1][x][k]) : zero8;
+            const half8 bih = ix ? ld8(&midT[2][x][k]) : zero8, bil = ix ? ld8(&midT[3][x][k]) : zero8;
+            eR = mfma3(arh, arl, brh, brl, eR);
+            eR = mfma3(-aih, -ail, bih, bil, eR);
+            eI = mfma3(arh, arl, bih, bil, eI);
+            eI = mfma3(aih, ail, brh, brl, eI);
+         }
+         const int xo = 16 * cx + lr;
+#pragma unroll
+         for (int i = 0; i < 4; ++i)
+         {
+            const int yo = 16 * ry + 4 * (lane >> 4) + i;
+            if (yo < P && xo < P)
+               plane[(size_t)yo * P + xo] = eR[i] * eR[i] + eI[i] * eI[i];
+         }
+      }
+      __syncthreads();
+   }
+}
+
 // Sum-1 normalisation per plane (Sim/ZernikePsf.cpp:401-410): FP64 sum, v = float(v / sum).
 __global__ __launch_bounds__(256) void psf_normalize_kernel(float* __restrict__ planes, int n2)
 {
@@ -629,12 +785,31 @@ static void launch_planes(hipStream_t s, int items, const PsfReqDev* reqs, const
 
 // Output-row blocks of YB tiles keep the pupil chunk (MP x 40 halves x 4) and the mid rows (YB 16 x (MP + 8) x 4)
 // within ~150 KB of LDS.
-// zg: workgroups per request of the small-plane kernel (planes of padded size <= 80 and M <= 128).
-static bool psf_planes_launch(iscg_context* h, const PsfBatch& s, int zg, float* out)
+// The small-plane kernels' launch choice: workgroups per request zg (1-32) plus kRbVariant for the row-block form.
+constexpr int kRbVariant = 64;
+static bool rb_eligible(const PsfBatch& s)
+{
+   return s.Pp <= kSmallPp && s.size <= kRbRows && (s.Mp == 32 || s.Mp == 64 || s.Mp == 128);
+}
+
+// choice: for planes of padded size <= 80 and M <= 128, the workgroups per request, + kRbVariant for
+// psf_small_rb_kernel (rb_eligible shapes only).
+static bool psf_planes_launch(iscg_context* h, const PsfBatch& s, int choice, float* out)
 {
    const int nReq = s.nReq, nz = s.nz, Mp = s.Mp, Pp = s.Pp;
+   const int zg = choice % kRbVariant;
+   const bool rb = choice >= kRbVariant && rb_eligible(s);
    auto go = [&](auto mp, auto yb) {
       constexpr int MP = decltype(mp)::value, YB = decltype(yb)::value;
+      if constexpr (MP == 32 || MP == 64 || MP == 128)
+         if (rb)
+         {
+            hipLaunchKernelGGL((psf_small_rb_kernel<MP>), dim3((nReq + kXcds - 1) / kXcds * kXcds * zg),
+                               dim3(kPsfThreads), 0, h->stream, s.reqs.as<const PsfReqDev>(),
+                               s.tabA.as<const double>(), s.tabB.as<const double>(), s.W.as<const _Float16>(),
+                               s.tz.as<const double>(), nz, Pp, nReq, zg, out);
+            return;
+         }
       if (Pp <= kSmallPp)
       {
          if constexpr (MP <= 128)
@@ -756,17 +931,22 @@ static int32_t psf_batch_run(iscg_context* h, const PsfBatch& b, int32_t normali
    // the small-plane kernel's workgroups per request: every candidate computes each plane in one workgroup the
    // same way, so the planes are bit-identical whichever is chosen (and the planes are written, not accumulated:
    // the timing launches before normalisation leave the same output)
+   // (the row-block form, where the shape allows it, is a candidate too: the same products in the same order)
    int zg = kZGroups;
    if (Pp <= kSmallPp && Mp <= 128)
    {
       std::vector<int> cands;
-      for (int c : {1, 2, 4, 8, 16, 32})
-         if (c <= nz)
-            cands.push_back(c);
-      const std::string key = "psf_small M" + std::to_string(Mp) + " P" + std::to_string(Pp) + " nz" +
-                              std::to_string(nz) + " req" + std::to_string(pow2_bucket(nReq));
-      zg = tuned_choice(h, key, cands, nz >= kZGroups ? kZGroups : 1,
-                        [&](int c) { return psf_planes_launch(h, b, c, dOut); });
+      for (int v = 0; v < (rb_eligible(b) ? 2 : 1); ++v)
+         for (int c : {1, 2, 4, 8, 16, 32})
+            if (c <= nz)
+               cands.push_back(c + kRbVariant * v);
+      const std::string key = "psf_small M" + std::to_string(Mp) + " P" + std::to_string(Pp) + " size" +
+                              std::to_string(b.size) + " nz" + std::to_string(nz) + " req" +
+                              std::to_string(pow2_bucket(nReq));
+      // default (tuning off, or under graph capture before a tuning call): the row-block form with one workgroup
+      // per request where it applies (the fastest at 512 requests of 65 x 65 x 41 planes, M 128)
+      const int dflt = rb_eligible(b) ? kRbVariant + 1 : (nz >= kZGroups ? kZGroups : 1);
+      zg = tuned_choice(h, key, cands, dflt, [&](int c) { return psf_planes_launch(h, b, c, dOut); });
    }
    {
       ProfScope ps(h, "psf_planes_mfma");
