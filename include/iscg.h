@@ -65,6 +65,29 @@ int32_t iscg_autotune_set(iscg_context* h, int32_t mode);
 int32_t iscg_autotune_dump(iscg_context* h, char* buf, int32_t cap);
 int32_t iscg_autotune_load(iscg_context* h, const char* text);
 
+/* The movie file (replaces TiffWriter, cli/tiff_writer.h:24-199; host code, no context): little-endian baseline
+ * TIFF, one uncompressed grey page per frame, the reference's layout byte for byte. open: BigTIFF when bigTiff != 0
+ * or expectedBytes + 64 MiB >= 2^32 - 1 (iscg_tiff_bytes gives the expected size); the writer exists even when
+ * open fails (read iscg_tiff_error, then destroy). page_u16 / page_f32: a 16-bit unsigned or 32-bit float page of
+ * w x h samples with an optional description (NULL or ""); page_rows_u16: a page whose first `rows` rows are given,
+ * the rest written later by rows_u16 (rows y0 .. y0 + rows - 1 of a 0-based page). frames_u16: nFrames pages from
+ * host or device memory, the description on the first. A classic file that would pass 4 GB is refused. close
+ * flushes and closes (0, or -1 with the error kept); destroy frees. Every call returns 0 or -1. */
+typedef struct iscg_tiff iscg_tiff;
+uint64_t iscg_tiff_bytes(uint64_t w, uint64_t h, uint64_t pages, uint64_t bytesPerSample, uint64_t descBytes);
+int32_t iscg_tiff_open(const char* path, uint64_t expectedBytes, int32_t bigTiff, iscg_tiff** out);
+int32_t iscg_tiff_page_u16(iscg_tiff* t, const uint16_t* px, uint32_t w, uint32_t h, const char* desc);
+int32_t iscg_tiff_page_f32(iscg_tiff* t, const float* px, uint32_t w, uint32_t h, const char* desc);
+int32_t iscg_tiff_page_rows_u16(iscg_tiff* t, const uint16_t* px, uint32_t rows, uint32_t w, uint32_t h,
+                                const char* desc);
+int32_t iscg_tiff_rows_u16(iscg_tiff* t, uint64_t page, uint32_t y0, uint32_t rows, const uint16_t* px);
+int32_t iscg_tiff_frames_u16(iscg_tiff* t, const uint16_t* frames, uint32_t nFrames, uint32_t w, uint32_t h,
+                             const char* firstDesc);
+int32_t iscg_tiff_info(const iscg_tiff* t, uint64_t* pages, uint64_t* bytes, int32_t* big);
+const char* iscg_tiff_error(const iscg_tiff* t);
+int32_t iscg_tiff_close(iscg_tiff* t);
+void iscg_tiff_destroy(iscg_tiff* t);
+
 /* Per-kernel timing on the context's stream (the ISC_PROFILE analogue, Sim/Timing.h:1-60): with profiling on, each
  * kernel launch is bracketed by HIP events. iscg_profile_read waits for the stream and reports, per kernel name,
  * launches and total milliseconds (names: cap x 64 chars); returns the number of kernels. reset != 0 clears. */

@@ -106,6 +106,17 @@ SIGNATURES = {
     "iscg_psf_library_run": (I32, [P, P, I32, P, C.c_size_t]),
     "iscg_psf_library_destroy": (I32, [P, P]),
     "iscg_halo_spans": (I32, [P, P, I32, I32, I32, I32, D, P, P, P, C.POINTER(D)]),
+    "iscg_tiff_bytes": (C.c_uint64, [C.c_uint64] * 5),
+    "iscg_tiff_open": (I32, [C.c_char_p, C.c_uint64, I32, C.POINTER(P)]),
+    "iscg_tiff_page_u16": (I32, [P, P, U32, U32, C.c_char_p]),
+    "iscg_tiff_page_f32": (I32, [P, P, U32, U32, C.c_char_p]),
+    "iscg_tiff_page_rows_u16": (I32, [P, P, U32, U32, U32, C.c_char_p]),
+    "iscg_tiff_rows_u16": (I32, [P, C.c_uint64, U32, U32, P]),
+    "iscg_tiff_frames_u16": (I32, [P, P, U32, U32, U32, C.c_char_p]),
+    "iscg_tiff_info": (I32, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(I32)]),
+    "iscg_tiff_error": (C.c_char_p, [P]),
+    "iscg_tiff_close": (I32, [P]),
+    "iscg_tiff_destroy": (None, [P]),
     "iscg_autotune_set": (I32, [P, I32]),
     "iscg_autotune_dump": (I32, [P, C.c_char_p, I32]),
     "iscg_autotune_load": (I32, [P, C.c_char_p]),

@@ -365,6 +365,82 @@ class PsfKernelCache:
         return np.clip(idx, 0, self.nz - 1).astype(np.int32), clamped
 
 
+class TiffWriter:
+    """TiffWriter (cli/tiff_writer.h:24-199) over iscg_tiff_*: a little-endian baseline TIFF, one uncompressed grey
+    page per frame (uint16 ADU or float32), classic or BigTIFF, the reference's bytes. Host code (no GPU needed);
+    Frames() also takes a device tensor of frames."""
+
+    @staticmethod
+    def Bytes(w: int, h: int, pages: int, bytesPerSample: int, descBytes: int) -> int:
+        return int(_lib.lib().iscg_tiff_bytes(w, h, pages, bytesPerSample, descBytes))
+
+    def __init__(self, path: str, expectedBytes: int = 0, bigTiff: bool = False):
+        self.L = _lib.lib()
+        self.t = C.c_void_p()
+        self.L.iscg_tiff_open(str(path).encode(), int(expectedBytes), int(bool(bigTiff)), C.byref(self.t))
+
+    def _rc(self, rc: int) -> bool:
+        return rc == 0
+
+    def ok(self) -> bool:
+        return bool(self.t) and not self.error()
+
+    def error(self) -> str:
+        return self.L.iscg_tiff_error(self.t).decode()
+
+    def _info(self):
+        p, b, g = C.c_uint64(), C.c_uint64(), C.c_int32()
+        self.L.iscg_tiff_info(self.t, C.byref(p), C.byref(b), C.byref(g))
+        return p.value, b.value, bool(g.value)
+
+    def big(self) -> bool:
+        return self._info()[2]
+
+    def pages(self) -> int:
+        return self._info()[0]
+
+    def bytes(self) -> int:
+        return self._info()[1]
+
+    def Page(self, px, w: int, h: int, desc: str = "") -> bool:
+        a = np.ascontiguousarray(px)
+        if a.size != w * h:
+            raise ValueError("a page holds w * h samples")
+        if a.dtype == np.uint16:
+            return self._rc(self.L.iscg_tiff_page_u16(self.t, a.ctypes.data, w, h, desc.encode()))
+        a = np.ascontiguousarray(a, np.float32)
+        return self._rc(self.L.iscg_tiff_page_f32(self.t, a.ctypes.data, w, h, desc.encode()))
+
+    def PageRows(self, px, rows: int, w: int, h: int, desc: str = "") -> bool:
+        a = np.ascontiguousarray(px, np.uint16)
+        if a.size < min(rows, h) * w:
+            raise ValueError("fewer samples than rows * w")
+        return self._rc(self.L.iscg_tiff_page_rows_u16(self.t, a.ctypes.data, rows, w, h, desc.encode()))
+
+    def Rows(self, page: int, y0: int, rows: int, px) -> bool:
+        a = np.ascontiguousarray(px, np.uint16)
+        return self._rc(self.L.iscg_tiff_rows_u16(self.t, page, y0, rows, a.ctypes.data))
+
+    def Frames(self, frames, firstDesc: str = "") -> bool:
+        """Every frame of a (n, h, w) uint16 array or device tensor as a page, the description on the first."""
+        n, h, w = frames.shape
+        p, keep = _ptr(frames if hasattr(frames, "data_ptr") else np.ascontiguousarray(frames, np.uint16))
+        rc = self.L.iscg_tiff_frames_u16(self.t, p, n, w, h, firstDesc.encode())
+        del keep
+        return self._rc(rc)
+
+    def Close(self) -> bool:
+        return self._rc(self.L.iscg_tiff_close(self.t))
+
+    def __del__(self):
+        try:
+            if self.t:
+                self.L.iscg_tiff_destroy(self.t)
+                self.t = C.c_void_p()
+        except Exception:
+            pass
+
+
 _halo_memo: list = []   # ((serial, threshold + 10 interpMode), PsfHaloSpans), newest first, at most 8
 _halo_lock = threading.Lock()
 

@@ -502,6 +502,19 @@ class Reference:
         L.ref_capped_psf_oversampling.restype = I32
         L.ref_bucket_events.argtypes = [C.c_void_p, I32, I64, C.POINTER(C.c_int64), C.POINTER(C.c_uint32), I64]
         L.ref_bucket_events.restype = I64
+        L.ref_tiff_open.argtypes = [C.c_char_p, C.c_uint64, I32]
+        L.ref_tiff_open.restype = C.c_void_p
+        for nm in ("ref_tiff_page_u16", "ref_tiff_page_f32"):
+            getattr(L, nm).argtypes = [C.c_void_p, C.c_void_p, U32, U32, C.c_char_p]
+            getattr(L, nm).restype = I32
+        L.ref_tiff_page_rows_u16.argtypes = [C.c_void_p, C.c_void_p, U32, U32, U32, C.c_char_p]
+        L.ref_tiff_page_rows_u16.restype = I32
+        L.ref_tiff_rows_u16.argtypes = [C.c_void_p, C.c_uint64, U32, U32, C.c_void_p]
+        L.ref_tiff_rows_u16.restype = I32
+        L.ref_tiff_close.argtypes = [C.c_void_p]
+        L.ref_tiff_close.restype = I32
+        L.ref_tiff_bytes.argtypes = [C.c_uint64] * 5
+        L.ref_tiff_bytes.restype = C.c_uint64
         L.ref_parse_zernike.argtypes = [C.c_char_p, PD]
         L.ref_parse_zernike.restype = I32
         L.ref_format_zernike.argtypes = [PD, C.c_char, C.c_char_p, I32]

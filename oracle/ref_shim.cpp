@@ -875,3 +875,51 @@ int64_t ref_bucket_events(const ref_event* ev, int32_t nEv, int64_t nFrames, int
 }
 
 } // extern "C"
+
+#include "tiff_writer.h"
+
+extern "C" {
+
+// TiffWriter (cli/tiff_writer.h), one call per method.
+void* ref_tiff_open(const char* path, uint64_t expectedBytes, int32_t big)
+{
+   return new TiffWriter(path, expectedBytes, big != 0);
+}
+
+int32_t ref_tiff_page_u16(void* t, const uint16_t* px, uint32_t w, uint32_t h, const char* desc)
+{
+   return static_cast<TiffWriter*>(t)->Page(std::vector<uint16_t>(px, px + static_cast<size_t>(w) * h), w, h,
+                                            desc ? desc : "") ? 0 : -1;
+}
+
+int32_t ref_tiff_page_f32(void* t, const float* px, uint32_t w, uint32_t h, const char* desc)
+{
+   return static_cast<TiffWriter*>(t)->Page(std::vector<float>(px, px + static_cast<size_t>(w) * h), w, h,
+                                            desc ? desc : "") ? 0 : -1;
+}
+
+int32_t ref_tiff_page_rows_u16(void* t, const uint16_t* px, uint32_t rows, uint32_t w, uint32_t h, const char* desc)
+{
+   return static_cast<TiffWriter*>(t)->PageRows(px, rows, w, h, desc ? desc : "") ? 0 : -1;
+}
+
+int32_t ref_tiff_rows_u16(void* t, uint64_t page, uint32_t y0, uint32_t rows, const uint16_t* px)
+{
+   return static_cast<TiffWriter*>(t)->Rows(page, y0, rows, px) ? 0 : -1;
+}
+
+// Closes and frees; 0 when the file is whole.
+int32_t ref_tiff_close(void* t)
+{
+   TiffWriter* w = static_cast<TiffWriter*>(t);
+   const bool ok = w->Close();
+   delete w;
+   return ok ? 0 : -1;
+}
+
+uint64_t ref_tiff_bytes(uint64_t w, uint64_t h, uint64_t pages, uint64_t bps, uint64_t desc)
+{
+   return TiffWriter::Bytes(w, h, pages, bps, desc);
+}
+
+} // extern "C"

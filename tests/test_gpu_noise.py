@@ -108,3 +108,20 @@ def test_noise_fp32_path_equals_fp64_chain_full_size(ctx, emccd, read):
 def test_noise_set_path_rejects_bad_value(ctx):
     with pytest.raises(host.IscgError, match="exact must be 0 or 1"):
         ctx.check(ctx.L.iscg_noise_set_path(ctx.h, 2))
+
+
+def test_tiff_frames_from_device_memory(ctx, tmp_path):
+    """A stack of ADU frames rendered into device memory goes to the movie file through iscg_tiff_frames_u16 (the
+    copy from the device inside the library): the same bytes as the pages written from host memory."""
+    import torch
+    from insiliscope_amd import configs
+    ph = np.random.default_rng(4).gamma(2.0, 30.0, (4, 33, 47)).astype(np.float32)
+    cam = configs.scmos_c1()
+    adu = torch.zeros((4, 33, 47), dtype=torch.int16, device="cuda")
+    host.ApplyNoiseChainFrames(torch.from_numpy(ph).cuda(), cam, 7, np.arange(4, dtype=np.uint32), out=adu, ctx=ctx)
+    torch.cuda.synchronize()
+    a = host.TiffWriter(str(tmp_path / "dev.tif"))
+    assert a.Frames(adu, "gpu frames") and a.Close(), a.error()
+    b = host.TiffWriter(str(tmp_path / "host.tif"))
+    assert b.Frames(adu.cpu().numpy().view(np.uint16), "gpu frames") and b.Close(), b.error()
+    assert open(tmp_path / "dev.tif", "rb").read() == open(tmp_path / "host.tif", "rb").read()

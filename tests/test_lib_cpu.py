@@ -199,3 +199,70 @@ def test_bucket_events_by_frame_vs_reference():
         start, idx = host.BucketEventsByFrame(ev, nF)
         counts, want = R.bucket_events(ev, nF)
         assert np.array_equal(np.diff(start), counts) and np.array_equal(idx, want), nF
+
+
+def _tiff_script(ops, page_u16, page_f32, page_rows, rows):
+    for op in ops:
+        kind = op[0]
+        if kind == "u16":
+            assert page_u16(*op[1:]) == 0
+        elif kind == "f32":
+            assert page_f32(*op[1:]) == 0
+        elif kind == "prows":
+            assert page_rows(*op[1:]) == 0
+        else:
+            assert rows(*op[1:]) == 0
+
+
+def test_tiff_writer_bytes_identical_to_reference(tmp_path):
+    """The movie file (iscg_tiff_*) against the reference's TiffWriter compiled from cli/tiff_writer.h: the same
+    calls (16-bit and float pages, descriptions stored in place and out of it, odd sizes that need the pad byte, a
+    page written in strips: first rows, then the others), classic, forced BigTIFF and BigTIFF by expected size;
+    identical files. The writer refuses rows outside a page."""
+    import ctypes as C
+    import oracle as O
+    from insiliscope_amd import _lib, host
+    R = O.Reference()
+    L = _lib.lib()
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 65536, (23, 37), dtype=np.uint16)
+    b = rng.integers(0, 65536, (8, 9), dtype=np.uint16)
+    f = rng.normal(size=(5, 7)).astype(np.float32)
+    s = rng.integers(0, 65536, (17, 11), dtype=np.uint16)
+    def ops(u16, f32, prows, rows):
+        return [("u16", u16(a), 37, 23, b""), ("u16", u16(b), 9, 8, b"abc"), ("u16", u16(b), 9, 8, b"abcdefg"),
+                ("f32", f32(f), 7, 5, b"a float page with a longer description {\"k\": 1}"),
+                ("prows", u16(s), 4, 11, 17, b"strips"), ("u16", u16(a), 37, 23, b"x"),
+                ("rows", 4, 4, 9, u16(s[4:13])), ("rows", 4, 13, 4, u16(s[13:]))]
+    for mode, (expected, big) in {"classic": (0, 0), "forced": (0, 1), "expected": (1 << 32, 0)}.items():
+        keep = []
+        def ptr(x):
+            x = np.ascontiguousarray(x)
+            keep.append(x)
+            return x.ctypes.data
+        pr, po = str(tmp_path / f"ref_{mode}.tif"), str(tmp_path / f"ours_{mode}.tif")
+        rt = R.L.ref_tiff_open(pr.encode(), expected, big)
+        _tiff_script(ops(ptr, ptr, None, None), lambda p, w, h, d: R.L.ref_tiff_page_u16(rt, p, w, h, d),
+                     lambda p, w, h, d: R.L.ref_tiff_page_f32(rt, p, w, h, d),
+                     lambda p, r, w, h, d: R.L.ref_tiff_page_rows_u16(rt, p, r, w, h, d),
+                     lambda pg, y0, n, p: R.L.ref_tiff_rows_u16(rt, pg, y0, n, p))
+        assert R.L.ref_tiff_close(rt) == 0
+        t = C.c_void_p()
+        assert L.iscg_tiff_open(po.encode(), expected, big, C.byref(t)) == 0
+        _tiff_script(ops(ptr, ptr, None, None), lambda p, w, h, d: L.iscg_tiff_page_u16(t, p, w, h, d),
+                     lambda p, w, h, d: L.iscg_tiff_page_f32(t, p, w, h, d),
+                     lambda p, r, w, h, d: L.iscg_tiff_page_rows_u16(t, p, r, w, h, d),
+                     lambda pg, y0, n, p: L.iscg_tiff_rows_u16(t, pg, y0, n, p))
+        assert L.iscg_tiff_rows_u16(t, 9, 0, 1, ptr(s)) != 0 and b"not in it" in L.iscg_tiff_error(t)
+        L.iscg_tiff_destroy(t)
+        ours, ref = open(po, "rb").read(), open(pr, "rb").read()
+        assert ours == ref, mode
+        assert ref[2] == (43 if big or expected else 42)
+    assert host.TiffWriter.Bytes(512, 512, 2000, 2, 100) == R.L.ref_tiff_bytes(512, 512, 2000, 2, 100)
+    # the host mirror: a stack of frames (the description on the first page) equals page-by-page writes
+    frames = rng.integers(0, 4096, (3, 6, 10), dtype=np.uint16)
+    w1 = host.TiffWriter(str(tmp_path / "stack.tif"))
+    assert w1.Frames(frames, "movie") and w1.pages() == 3 and w1.Close()
+    w2 = host.TiffWriter(str(tmp_path / "pages.tif"))
+    assert all(w2.Page(fr, 10, 6, "movie" if i == 0 else "") for i, fr in enumerate(frames)) and w2.Close()
+    assert open(tmp_path / "stack.tif", "rb").read() == open(tmp_path / "pages.tif", "rb").read()
