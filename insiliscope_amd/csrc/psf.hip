@@ -97,7 +97,10 @@ __global__ void psf_tables_kernel(const PsfReqDev* __restrict__ reqs, int nReq, 
    const size_t per = (size_t)Mp * Mp, total = per * nReq;
    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x)
    {
-      const int r = (int)(idx / per);
+      // per = Mp^2 is a multiple of 64 and a wave's 64 lanes take 64 consecutive idx from a multiple of 64 (all of
+      // them, as total is a multiple of 64 too): the request is the same across the wave, so its record (the
+      // folded Zernike polynomials: ~100 doubles) is read with scalar loads once per wave, not per lane
+      const int r = __builtin_amdgcn_readfirstlane((int)(idx / per));
       const int rem = (int)(idx - (size_t)r * per);
       const int iy = rem / Mp, ix = rem % Mp;
       const PsfReqDev& q = reqs[r];
@@ -173,7 +176,8 @@ __global__ void psf_w_kernel(const PsfReqDev* __restrict__ reqs, int nReq, int P
    const size_t per = (size_t)Pp * Mp, total = per * nReq;
    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x)
    {
-      const int r = (int)(idx / per);
+      // per = Pp Mp is a multiple of 512: one request per wave (scalar loads of its record)
+      const int r = __builtin_amdgcn_readfirstlane((int)(idx / per));
       const int rem = (int)(idx - (size_t)r * per);
       const int p = rem / Mp, m = rem % Mp;
       const PsfReqDev& q = reqs[r];
