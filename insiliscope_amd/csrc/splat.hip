@@ -30,7 +30,7 @@ bool launch_noise(iscg_context* h, const float* photons, uint16_t* out, uint32_t
                   const CamParams& cp, uint32_t seed, uint32_t pixelBase);
 
 constexpr int kMaxKernels = 4;
-constexpr int kTile = 32;
+constexpr int kTile = 32;  // the default tile edge (16 is the other candidate of the tuned launch)
 constexpr int kBlock = 256;
 
 struct KernelDev
@@ -467,10 +467,10 @@ struct TileArgs
    uint32_t w, h, tilesX, nTiles, nFrames, nEmitters, seed, pixelBase;
 };
 
-template <bool NOISE, bool EMCCD>
+template <int TILE, bool NOISE, bool EMCCD>
 __global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelTable kt, CamParams cp)
 {
-   __shared__ float img[kTile * kTile];
+   __shared__ float img[TILE * TILE];
    __shared__ int list[kBlock];
    __shared__ int waveHits[kBlock / 64];
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -480,14 +480,14 @@ __global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelT
    {
       const uint32_t f = (uint32_t)(item / a.nTiles);
       const uint32_t t = (uint32_t)(item - (size_t)f * a.nTiles);
-      const int tx0 = (int)(t % a.tilesX) * kTile, ty0 = (int)(t / a.tilesX) * kTile;
+      const int tx0 = (int)(t % a.tilesX) * TILE, ty0 = (int)(t / a.tilesX) * TILE;
       const float* extra = a.extras ? a.extras + (size_t)f * npx : nullptr;
       const double bgScale = a.bgScales ? (double)a.bgScales[f] : 1.0;
       const bool flat = a.bgMap == nullptr && bgScale == 1.0;
       // the tile's starting photons: the caller's extra image, else the background (RenderPhotonImage :133-149)
-      for (int p = tid; p < kTile * kTile; p += kBlock)
+      for (int p = tid; p < TILE * TILE; p += kBlock)
       {
-         const int x = tx0 + (p & (kTile - 1)), y = ty0 + p / kTile;
+         const int x = tx0 + (p & (TILE - 1)), y = ty0 + p / TILE;
          float v = 0.0f;
          if (x < (int)a.w && y < (int)a.h)
          {
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelT
          if (e < e1)
          {
             const int2 q = a.pos[e];
-            hit = q.x + R >= tx0 && q.x - R <= tx0 + kTile - 1 && q.y + R >= ty0 && q.y - R <= ty0 + kTile - 1;
+            hit = q.x + R >= tx0 && q.x - R <= tx0 + TILE - 1 && q.y + R >= ty0 && q.y - R <= ty0 + TILE - 1;
          }
          const unsigned long long mask = __ballot(hit);
          if (lane == 0)
@@ -531,7 +531,9 @@ __global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelT
          // splat the hits in order: one pixel of the clipped patch per lane
          for (int k = 0; k < nHits; ++k)
          {
-            const SetupRec& r = a.recs[list[k]];
+            // the hit is the same across the workgroup: its record is read with scalar loads
+            const int ek = __builtin_amdgcn_readfirstlane(list[k]);
+            const SetupRec& r = a.recs[ek];
             const int kk = r.kernel;
             const int camRad = kt.camRad[kk], os = kt.os[kk], bw = kt.bw[kk], NT = r.nTaps, off = os - 1;
             // patch rows/cols inside the image (Rows<NT> :87-89) and the tile
@@ -540,14 +542,21 @@ __global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelT
             int dyLo = -camRad > -r.y0 ? -camRad : -r.y0;
             int dyHi = camRad < (int)a.h - 1 - r.y0 ? camRad : (int)a.h - 1 - r.y0;
             dxLo = dxLo > tx0 - r.x0 ? dxLo : tx0 - r.x0;
-            dxHi = dxHi < tx0 + kTile - 1 - r.x0 ? dxHi : tx0 + kTile - 1 - r.x0;
+            dxHi = dxHi < tx0 + TILE - 1 - r.x0 ? dxHi : tx0 + TILE - 1 - r.x0;
             dyLo = dyLo > ty0 - r.y0 ? dyLo : ty0 - r.y0;
-            dyHi = dyHi < ty0 + kTile - 1 - r.y0 ? dyHi : ty0 + kTile - 1 - r.y0;
+            dyHi = dyHi < ty0 + TILE - 1 - r.y0 ? dyHi : ty0 + TILE - 1 - r.y0;
             const int rw = dxHi - dxLo + 1, rh = dyHi - dyLo + 1;
             if (rw > 0 && rh > 0)
             {
-               const float* B = r.pad[0] == 1 ? a.fftSums + (size_t)list[k] * a.fftStride
+               const float* B = r.pad[0] == 1 ? a.fftSums + (size_t)ek * a.fftStride
                                               : kt.sums[kk] + (size_t)r.plane * bw * bw;
+               double wx[4], wy[4];
+#pragma unroll
+               for (int i = 0; i < 4; ++i)
+               {
+                  wx[i] = r.wx[i];
+                  wy[i] = r.wy[i];
+               }
                const int nSpan = 2 * camRad + 1;
                const int16_t* lo = kt.lo[kk] ? kt.lo[kk] + (size_t)r.plane * nSpan : nullptr;
                const int16_t* hi = kt.hi[kk] ? kt.hi[kk] + (size_t)r.plane * nSpan : nullptr;
@@ -561,16 +570,22 @@ __global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelT
                   const int r0 = r.by + dy * os + off, cc = r.bx + dx * os + off;
                   const int jLo = 0 > -r0 ? 0 : -r0, jHi = NT < bw - r0 ? NT : bw - r0;
                   const int iLo = 0 > -cc ? 0 : -cc, iHi = NT < bw - cc ? NT : bw - cc;
+                  // taps j in [jLo, jHi), i in [iLo, iHi) (at most 4 each), summed in that order
                   double sum = 0.0;
-                  for (int j = jLo; j < jHi; ++j)
+#pragma unroll
+                  for (int j = 0; j < 4; ++j)
                   {
+                     if (j < jLo || j >= jHi)
+                        continue;
                      const float* brow = B + (size_t)(r0 + j) * bw + cc;
                      double row = 0.0;
-                     for (int i = iLo; i < iHi; ++i)
-                        row += r.wx[i] * brow[i];
-                     sum += r.wy[j] * row;
+#pragma unroll
+                     for (int i = 0; i < 4; ++i)
+                        if (i >= iLo && i < iHi)
+                           row += wx[i] * brow[i];
+                     sum += wy[j] * row;
                   }
-                  const int li = (r.y0 + dy - ty0) * kTile + (r.x0 + dx - tx0);
+                  const int li = (r.y0 + dy - ty0) * TILE + (r.x0 + dx - tx0);
                   img[li] += (float)(r.photons * sum);
                }
             }
@@ -578,9 +593,9 @@ __global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelT
          }
       }
       // out: photons and/or the noise chain, per pixel
-      for (int p = tid; p < kTile * kTile; p += kBlock)
+      for (int p = tid; p < TILE * TILE; p += kBlock)
       {
-         const int x = tx0 + (p & (kTile - 1)), y = ty0 + p / kTile;
+         const int x = tx0 + (p & (TILE - 1)), y = ty0 + p / TILE;
          if (x >= (int)a.w || y >= (int)a.h)
             continue;
          const size_t i = (size_t)y * a.w + x;
@@ -766,8 +781,6 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
    a.flatBg = s.flatBg;
    a.w = s.w;
    a.h = s.h;
-   a.tilesX = (s.w + kTile - 1) / kTile;
-   a.nTiles = a.tilesX * ((s.h + kTile - 1) / kTile);
    a.nFrames = nFrames;
    a.nEmitters = nEmitters;
    a.seed = seed;
@@ -783,40 +796,60 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
       if (!noise_queue(h, (size_t)nFrames * npx, a.q))
          return -1;
    }
-   // The persistent grid: workgroups per CU (256 CUs), tuned. Each (frame, tile) item is computed by one workgroup
-   // whatever the grid, so the frames are bit-identical for every choice; the timing launches write the same
-   // outputs again (the deferred-noise queue is drained and reset by each fallback launch). Not tuned when the
-   // caller's extra images are the photon output (in-place accumulation).
-   const size_t items = (size_t)a.nTiles * nFrames;
-   auto tiles = [&](int perCu) {
-      const size_t cap = (size_t)256 * perCu;
-      const unsigned grid = (unsigned)(items < cap ? items : cap);
-      if (!dAdu)
-         hipLaunchKernelGGL((splat_tiles_kernel<false, false>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
-      else if (cp.emccd)
-         hipLaunchKernelGGL((splat_tiles_kernel<true, true>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
+   // The tile edge (32 or 16) and the persistent grid (workgroups per CU, 256 CUs), tuned: choice = workgroups per
+   // CU + kTile16 for 16 x 16 tiles. Each (frame, tile) item is computed by one workgroup, and each pixel receives
+   // the same emitters in the same order whatever the tiling, so the frames are bit-identical for every choice;
+   // the timing launches write the same outputs again (the deferred-noise queue is drained and reset by each
+   // fallback launch). Not tuned when the caller's extra images are the photon output (in-place accumulation).
+   constexpr int kTile16 = 64;
+   auto tiles = [&](int choice) {
+      const int perCu = choice % kTile16, edge = choice >= kTile16 ? 16 : kTile;
+      TileArgs t = a;
+      t.tilesX = (s.w + edge - 1) / edge;
+      t.nTiles = t.tilesX * ((s.h + edge - 1) / edge);
+      const size_t items = (size_t)t.nTiles * nFrames, cap = (size_t)256 * perCu;
+      const unsigned grid = (unsigned)(items < cap ? (items ? items : 1) : cap);
+      auto go = [&](auto tile) {
+         constexpr int T = decltype(tile)::value;
+         if (!dAdu)
+            hipLaunchKernelGGL((splat_tiles_kernel<T, false, false>), dim3(grid), dim3(kBlock), 0, h->stream, t, kt,
+                               cp);
+         else if (cp.emccd)
+            hipLaunchKernelGGL((splat_tiles_kernel<T, true, true>), dim3(grid), dim3(kBlock), 0, h->stream, t, kt,
+                               cp);
+         else
+            hipLaunchKernelGGL((splat_tiles_kernel<T, true, false>), dim3(grid), dim3(kBlock), 0, h->stream, t, kt,
+                               cp);
+      };
+      if (edge == 16)
+         go(std::integral_constant<int, 16>{});
       else
-         hipLaunchKernelGGL((splat_tiles_kernel<true, false>), dim3(grid), dim3(kBlock), 0, h->stream, a, kt, cp);
+         go(std::integral_constant<int, kTile>{});
       return ok(h, hipGetLastError(), "splat launch");
    };
    auto fallback = [&]() {
       return !dAdu || launch_noise_fallback(h, a.q, dAdu, (uint32_t)npx, dFrames, 0, a.offset, a.gain, a.readN, cp,
                                             seed, s.pixelBase);
    };
-   int perCu = 16;
+   int choice = 16;
    const bool inPlace = dExtras && dPh && dExtras < (const float*)dPh + nFrames * npx && (const float*)dPh < dExtras +
                         nFrames * npx;
-   if (!inPlace && items > 256 * 4)
+   if (!inPlace && nFrames > 0)
    {
       const std::string key = "splat_tiles " + std::to_string(s.w) + "x" + std::to_string(s.h) + " frames" +
                               std::to_string(pow2_bucket(nFrames)) + " em/frame" +
                               std::to_string(pow2_bucket(nEmitters / (nFrames ? nFrames : 1))) + " noise" +
-                              std::to_string(dAdu ? 1 + cp.emccd : 0) + " fft" + std::to_string(a.fftSums ? 1 : 0);
-      perCu = tuned_choice(h, key, {4, 8, 16, 32}, 16, [&](int c) { return tiles(c) && fallback(); });
+                              std::to_string(dAdu ? 1 + cp.emccd : 0) + " fft" + std::to_string(a.fftSums ? 1 : 0) +
+                              " camRad" + std::to_string(kt.maxCamRad);
+      std::vector<int> cands;
+      for (int t16 = 0; t16 < 2; ++t16)
+         for (int c : {4, 8, 16, 32})
+            cands.push_back(c + kTile16 * t16);
+      choice = tuned_choice(h, key, cands, 16, [&](int c) { return tiles(c) && fallback(); });
    }
    {
       ProfScope ps(h, "splat_tiles_noise");
-      if (!tiles(perCu))
+      if (!tiles(choice))
          return -1;
    }
    if (!fallback())

@@ -40,8 +40,8 @@ def test_psf_small_kernel_every_candidate_identical():
 
 
 def test_splat_grid_every_candidate_identical(orc):
-    """16 frames of 512x512 with Poisson(800) blinks, Catmull-Rom, EMCCD (4096 tile items): the persistent grid's
-    workgroups per CU (4, 8, 16, 32) give the same photons and ADU; frame 0 equals the oracle's."""
+    """16 frames of 512x512 with Poisson(800) blinks, Catmull-Rom, EMCCD: the persistent grid's workgroups per CU
+    (4, 8, 16, 32) and the tile edge (32, or 16: + 64) give the same photons and ADU; frame 0 equals the oracle's."""
     req = configs.c4_psf(seed=1)
     req.update(nz=11, pupilSamples=64)
     planes = orc.psf_planes(O.psf_request(**req), normalize=True)
@@ -75,12 +75,13 @@ def test_splat_grid_every_candidate_identical(orc):
     adu, ph = render(c)
     ch = c.autotune_choices()
     key = _only_key(ch, "splat_tiles")
-    assert ch[key] in (4, 8, 16, 32), ch
+    cands = [per_cu + 64 * t16 for t16 in (0, 1) for per_cu in (4, 8, 16, 32)]
+    assert ch[key] in cands, ch
     assert np.array_equal(adu, want_adu) and np.array_equal(ph, want_ph)
-    for per_cu in (4, 8, 16, 32):
-        c.load_autotune({key: per_cu})
+    for cand in cands:
+        c.load_autotune({key: cand})
         adu, ph = render(c)
-        assert np.array_equal(adu, want_adu) and np.array_equal(ph, want_ph), f"{per_cu} workgroups per CU"
+        assert np.array_equal(adu, want_adu) and np.array_equal(ph, want_ph), f"choice {cand}"
     c.close()
 
 
