@@ -1,6 +1,6 @@
 """Throughput of the image-formation path on one MI355X, against the reference CPU path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload wf|smlm|psf|dyes|adjoint] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload wf|smlm|psf|dyes|adjoint|c1] [--no-cpu]
                     [--no-secondary]
 
 Default workload (`wf`): the widefield mean field of a 256 x 256 x 64 dye density (100 x 100 x 50 nm voxels, seeded)
@@ -641,7 +641,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="wf", choices=["wf", "smlm", "psf", "dyes", "adjoint"])
+    ap.add_argument("--workload", default="wf", choices=["wf", "smlm", "psf", "dyes", "adjoint", "c1"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -688,6 +688,11 @@ def main():
     elif args.workload == "dyes":
         r = run_dyes(ctx, args.steps, args.warmup)
         config, unit, dtype = r["config"], "dyes/s", "f64"
+    elif args.workload == "c1":
+        r = run_c1(ctx, args.steps, args.warmup)
+        r.pop("_events")
+        config, unit, dtype = r["config"], "frames/s", "f64"
+        r["roofline"] = None
     elif args.workload == "adjoint":
         r = run_c5(ctx, args.steps, args.warmup)
         r.pop("_case")
