@@ -216,6 +216,27 @@ int32_t iscg_splat_set_static(iscg_context* h, uint32_t width, uint32_t height, 
                               const float* offset, const float* gain, const float* readNoise, const float* background,
                               double flatBackground, uint64_t key);
 
+/* A blink event as the splat's caller holds it (sim::BlinkEvent, Sim/SMLMSimulation.h:82-110: the fields the
+ * image uses): position in um, height z in nm, [tStart, tEnd) in frames, brightness and excitation factors. 56 bytes. */
+typedef struct
+{
+   double xUm, yUm, zNm, tStart, tEnd, brightness, excite;
+} iscg_blink_event;
+
+/* Replaces CollectFrameEmitters (Sim/SMLMSimulation.cpp:55-101) for a batch of frames on the device. Frame k (index
+ * frameIndex[k]) takes the events bucketIdx[bucketStart[k] .. bucketStart[k+1]) (event order; BucketEventsByFrame)
+ * that overlap it; driftPx: 2 nFrames (x, y) pixel offsets or NULL; nz > 0: the kernel's plane count and zStepNm for
+ * NearestZIndex (0: zIndex 0); illumination: width x height factors at the undrifted site, or NULL; kernel: the
+ * records' kernel index. Writes the records into out (outCap records; frame k's are [outStart[k], outStart[k+1]),
+ * nFrames + 1 offsets), the same bits as the reference's, and zCounts (may be NULL) = {clamped, total}. Arrays may
+ * be host or device memory; the call synchronizes the context's stream. */
+int32_t iscg_collect_frame_emitters(iscg_context* h, const iscg_blink_event* events, uint32_t nEvents,
+                                    const uint32_t* bucketStart, const uint32_t* bucketIdx, const int64_t* frameIndex,
+                                    uint32_t nFrames, uint32_t width, uint32_t height, double pixelSizeNm,
+                                    double photonsPerBlink, const double* driftPx, int32_t nz, double zStepNm,
+                                    double globalZOffsetUm, int32_t kernel, const float* illumination,
+                                    iscg_frame_emitter* out, uint64_t outCap, uint64_t* outStart, int64_t* zCounts);
+
 /* Replaces SplatGpuHost::RenderFrames / GpuSimulator::RenderFrames (Sim/SplatGpu.h:57, Sim/GpuSimD3D11.h:99) and,
  * bit for bit, RenderPhotonImage (Sim/SMLMSimulation.cpp:103) + ApplyNoiseChain per frame.
  * ems: nEmitters records, frame k's are [emStart[k], emStart[k+1]) in event order (nFrames + 1 offsets).

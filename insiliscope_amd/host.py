@@ -31,6 +31,9 @@ SPLAT_EMITTER_DTYPE = np.dtype([("x0", "i4"), ("y0", "i4"), ("bx", "i4"), ("by",
                                 ("nTaps", "i4"), ("photons", "f4"), ("cutRadius2", "i4"), ("wx", "f4", 4),
                                 ("wy", "f4", 4)])
 # sim::BlinkEvent's hot-path fields (Sim/SMLMSimulation.h:82-105)
+# iscg_blink_event: sim::BlinkEvent's image fields, 56 bytes (the device collect's input)
+DEVICE_EVENT_DTYPE = np.dtype([("xUm", "f8"), ("yUm", "f8"), ("zNm", "f8"), ("tStart", "f8"), ("tEnd", "f8"),
+                               ("brightness", "f8"), ("excite", "f8")])
 BLINK_EVENT_DTYPE = np.dtype([("xUm", "f8"), ("yUm", "f8"), ("zNm", "f8"), ("tStart", "f8"), ("tEnd", "f8"),
                               ("brightness", "f8")])
 assert FRAME_EMITTER_DTYPE.itemsize == 32 and SPLAT_EMITTER_DTYPE.itemsize == 64
@@ -534,6 +537,44 @@ def CollectFrameEmitters(events, frameIndex: int, pixelSizeNm: float, photonsPer
         out["zIndex"], _ = psfCache.NearestZIndex(ev["zNm"] / 1000.0 - globalZOffsetUm)
     out["kernel"] = kernel
     return out
+
+
+def CollectFrameEmittersBatch(events, bucketStart, bucketIdx, frameIndex, width: int, height: int,
+                              pixelSizeNm: float, photonsPerBlink: float, drift=None, psfCache: PsfKernelCache | None = None,
+                              globalZOffsetUm: float = 0.0, kernel: int = 0, illumination=None,
+                              ctx: Context | None = None):
+    """sim::CollectFrameEmitters (Sim/SMLMSimulation.cpp:55-101) for a batch of frames on the GPU
+    (iscg_collect_frame_emitters). events: DEVICE_EVENT_DTYPE records (or BLINK_EVENT_DTYPE, excite 1); frame k
+    takes events bucketIdx[bucketStart[k]:bucketStart[k+1]] (BucketEventsByFrame). drift: (nFrames, 2) pixel offsets.
+    Returns (records FRAME_EMITTER_DTYPE, start offsets nFrames + 1, (z clamped, z total))."""
+    ctx = ctx or default_context()
+    ev = np.asarray(events)
+    if ev.dtype != DEVICE_EVENT_DTYPE:
+        e7 = np.zeros(len(ev), DEVICE_EVENT_DTYPE)
+        for k in ("xUm", "yUm", "zNm", "tStart", "tEnd", "brightness"):
+            e7[k] = ev[k]
+        e7["excite"] = ev["excite"] if "excite" in (ev.dtype.names or ()) else 1.0
+        ev = e7
+    ev = np.ascontiguousarray(ev)
+    st = np.ascontiguousarray(bucketStart, np.uint32)
+    idx = np.ascontiguousarray(bucketIdx, np.uint32)
+    fi = np.ascontiguousarray(frameIndex, np.int64)
+    n = len(fi)
+    dr = None if drift is None else np.ascontiguousarray(drift, np.float64).reshape(n, 2)
+    il = None if illumination is None else np.ascontiguousarray(illumination, np.float32)
+    cap = int(st[-1]) if len(st) else 0
+    out = np.zeros(max(cap, 1), FRAME_EMITTER_DTYPE)
+    start = np.zeros(n + 1, np.uint64)
+    z = np.zeros(2, np.int64)
+    nz = psfCache.nz if psfCache is not None else 0
+    zs = psfCache.zStepNm if psfCache is not None else 0.0
+    ctx.follow_torch()
+    ctx.check(ctx.L.iscg_collect_frame_emitters(
+        ctx.h, ev.ctypes.data if len(ev) else None, len(ev), st.ctypes.data, idx.ctypes.data if len(idx) else None,
+        fi.ctypes.data, n, width, height, pixelSizeNm, photonsPerBlink, None if dr is None else dr.ctypes.data, nz,
+        zs, globalZOffsetUm, kernel, None if il is None else il.ctypes.data, out.ctypes.data, len(out),
+        start.ctypes.data, z.ctypes.data), "iscg_collect_frame_emitters")
+    return out[: int(start[-1])], start, (int(z[0]), int(z[1]))
 
 
 def _catmull_rom(f):
@@ -1151,7 +1192,7 @@ __all__ = ["Context", "CameraNoiseParams", "ApplyNoiseChain", "ApplyNoiseChainFr
            "PsfHaloSpans", "SplatGpuHost", "WidefieldStack", "ComputePsfPlanesBatch", "ComputeZernikePsfPlanes",
            "BuildZernikePsfKernelCache", "PsfGeometry", "psf_request", "ROI_PARAMS_DTYPE", "roi_model", "RoiForward",
            "RoiBackward", "PhaseRetrieval", "CollectFrameEmitters", "CollectGpuEmitters", "FRAME_EMITTER_DTYPE",
-           "SPLAT_EMITTER_DTYPE", "BLINK_EVENT_DTYPE", "INTERP", "IscgError"]
+           "SPLAT_EMITTER_DTYPE", "BLINK_EVENT_DTYPE", "DEVICE_EVENT_DTYPE", "INTERP", "IscgError"]
 
 
 # ---- dye schedules (core/src/dyes.h:129-261) ---------------------------------------------------------------------

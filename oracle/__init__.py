@@ -500,6 +500,9 @@ class Reference:
         L.ref_psf_half_width_px.restype = I32
         L.ref_capped_psf_oversampling.argtypes = [I32, D, D]
         L.ref_capped_psf_oversampling.restype = I32
+        L.ref_collect_emitters_full.argtypes = [C.c_void_p, I32, I64, U32, U32, D, D, D, D, I32, D, D, PF, C.c_void_p,
+                                                I32, C.POINTER(C.c_int64)]
+        L.ref_collect_emitters_full.restype = I32
         L.ref_bucket_events.argtypes = [C.c_void_p, I32, I64, C.POINTER(C.c_int64), C.POINTER(C.c_uint32), I64]
         L.ref_bucket_events.restype = I64
         L.ref_tiff_open.argtypes = [C.c_char_p, C.c_uint64, I32]
@@ -757,6 +760,20 @@ class Reference:
 
     def capped_psf_oversampling(self, os_, pixel_nm, max_sample_nm):
         return self.L.ref_capped_psf_oversampling(os_, pixel_nm, max_sample_nm)
+
+    def collect_emitters_full(self, ev7, frame, w, h, pixel_nm, photons_per_blink, drift=(0.0, 0.0), nz=0,
+                              z_step_nm=0.0, global_z_um=0.0, illum=None):
+        """CollectFrameEmitters with excite, drift and an illumination field: (records (x, y, photons, zIndex,
+        kernel), (z clamped, z total))."""
+        ev7 = np.ascontiguousarray(ev7, np.float64).reshape(-1, 7)
+        out = np.zeros(max(1, len(ev7)), EMITTER_DTYPE)
+        z = np.zeros(2, np.int64)
+        il = None if illum is None else np.ascontiguousarray(illum, np.float32)
+        n = self.L.ref_collect_emitters_full(ev7.ctypes.data, len(ev7), frame, w, h, pixel_nm, photons_per_blink,
+                                             drift[0], drift[1], nz, z_step_nm, global_z_um,
+                                             None if il is None else _p(il, C.c_float), out.ctypes.data, len(out),
+                                             z.ctypes.data_as(C.POINTER(C.c_int64)))
+        return out[:n], (int(z[0]), int(z[1]))
 
     def bucket_events(self, events, n_frames):
         """BucketEventsByFrame: (counts per frame, the concatenated index lists)."""

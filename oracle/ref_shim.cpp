@@ -923,3 +923,52 @@ uint64_t ref_tiff_bytes(uint64_t w, uint64_t h, uint64_t pages, uint64_t bps, ui
 }
 
 } // extern "C"
+
+extern "C" {
+
+// CollectFrameEmitters (Sim/SMLMSimulation.cpp:55) with every input: events as 7 doubles (x, y, z, tStart, tEnd,
+// brightness, excite), drift, an illumination field (NULL: none), a kernel z geometry (nz 0: no kernel); records
+// (x, y, photons, zIndex) and the clamped / total z counts. Returns the count.
+int32_t ref_collect_emitters_full(const double* ev7, int32_t nEv, int64_t frame, uint32_t w, uint32_t h,
+                                  double pixelSizeNm, double photonsPerBlink, double driftX, double driftY,
+                                  int32_t nzK, double zStepNm, double globalZUm, const float* illum, orc_emitter* out,
+                                  int32_t cap, int64_t* zCounts)
+{
+   std::vector<sim::BlinkEvent> evs(static_cast<size_t>(nEv));
+   for (int32_t i = 0; i < nEv; ++i)
+   {
+      sim::BlinkEvent& e = evs[static_cast<size_t>(i)];
+      const double* p = ev7 + 7 * static_cast<size_t>(i);
+      e.xUm = p[0];
+      e.yUm = p[1];
+      e.zNm = p[2];
+      e.tStart = p[3];
+      e.tEnd = p[4];
+      e.brightness = p[5];
+      e.excite = p[6];
+   }
+   sim::PsfKernelCache c;
+   c.nz = nzK;
+   c.zStepNm = zStepNm;
+   std::vector<float> field;
+   if (illum)
+      field.assign(illum, illum + static_cast<size_t>(w) * h);
+   std::vector<sim::FrameEmitter> ems;
+   long clamped = 0, total = 0;
+   sim::CollectFrameEmitters(evs, frame, w, h, pixelSizeNm, photonsPerBlink, driftX, driftY, nzK > 0 ? &c : nullptr,
+                             globalZUm, illum ? &field : nullptr, ems, &clamped, &total);
+   const int32_t n = static_cast<int32_t>(ems.size());
+   for (int32_t i = 0; i < n && i < cap; ++i)
+   {
+      out[i].xPx = ems[static_cast<size_t>(i)].xPx;
+      out[i].yPx = ems[static_cast<size_t>(i)].yPx;
+      out[i].photons = ems[static_cast<size_t>(i)].photons;
+      out[i].zIndex = ems[static_cast<size_t>(i)].zIndex;
+      out[i].kernel = 0;
+   }
+   zCounts[0] = clamped;
+   zCounts[1] = total;
+   return n;
+}
+
+} // extern "C"

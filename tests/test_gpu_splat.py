@@ -230,3 +230,47 @@ def test_halo_spans_bit_exact_vs_reference(ctx, orc, interp):
             assert host.WithHaloCut(cache, thr, ctx=ctx).halo is c.halo
         assert len(serials) == 5
     assert host.WithHaloCut(cache, 0.0, ctx=ctx).halo is None
+
+
+def test_collect_frame_emitters_on_gpu_vs_reference():
+    """CollectFrameEmitters for a batch of frames on the GPU (iscg_collect_frame_emitters) against the compiled
+    reference's, frame by frame: events spanning frames, ending exactly on a frame edge, NaN and infinite times,
+    brightness and excitation factors, per-frame drift, an illumination field (read at sites off the sensor too) and
+    z beyond the stack (clamped); the records bit for bit, in event order, and the clamped / total z counts."""
+    R = O.Reference()
+    rng = np.random.default_rng(21)
+    n, nF, w, h = 3000, 12, 90, 70
+    ev = np.zeros(n, host.DEVICE_EVENT_DTYPE)
+    ev["xUm"] = rng.uniform(-0.5, 9.5, n)
+    ev["yUm"] = rng.uniform(-0.5, 7.5, n)
+    ev["zNm"] = rng.uniform(-900.0, 900.0, n)
+    ts = rng.uniform(-1.0, nF + 1.0, n)
+    ev["tStart"] = ts
+    ev["tEnd"] = ts + rng.exponential(0.8, n)
+    ev["brightness"] = np.where(rng.random(n) < 0.3, 1.0, rng.lognormal(0.0, 0.4, n))
+    ev["excite"] = np.where(rng.random(n) < 0.5, 1.0, rng.uniform(0.2, 1.5, n))
+    ev["tStart"][:4] = [3.0, np.nan, 2.0, -np.inf]
+    ev["tEnd"][:4] = [3.0, 5.0, np.inf, 1.5]
+    illum = rng.uniform(0.5, 1.5, (h, w)).astype(np.float32)
+    drift = rng.normal(0.0, 0.7, (nF, 2))
+    frames = np.arange(nF, dtype=np.int64)
+    bst, bidx = host.BucketEventsByFrame(ev[["xUm", "yUm", "zNm", "tStart", "tEnd", "brightness"]].astype(
+        host.BLINK_EVENT_DTYPE), nF)
+    cache = host.PsfKernelCache(np.zeros((41, 5, 5), np.float32), zStepNm=40.0)
+    for il in (None, illum):
+        recs, start, zc = host.CollectFrameEmittersBatch(ev, bst, bidx, frames, w, h, 110.0, 1500.0, drift=drift,
+                                                         psfCache=cache, globalZOffsetUm=0.1, kernel=2,
+                                                         illumination=il)
+        zt = [0, 0]
+        for f in range(nF):
+            sub = ev[bidx[bst[f]:bst[f + 1]]]
+            e7 = np.stack([sub[k] for k in host.DEVICE_EVENT_DTYPE.names], axis=1)
+            want, (c, t) = R.collect_emitters_full(e7, f, w, h, 110.0, 1500.0, tuple(drift[f]), 41, 40.0, 0.1, il)
+            zt[0] += c
+            zt[1] += t
+            got = recs[start[f]:start[f + 1]]
+            assert len(got) == len(want), f
+            for k in ("xPx", "yPx", "photons", "zIndex"):
+                assert np.array_equal(got[k], want[k]), (f, k)
+            assert (got["kernel"] == 2).all()
+        assert zc == tuple(zt) and zt[0] > 0
