@@ -266,6 +266,11 @@ def run_smlm(ctx, steps, warmup, n_frames=2000):
             "config": {"workload": "SMLM DNA-PAINT 512x512 x 2000 frames, Poisson(800) blinks/frame, Catmull-Rom "
                                    "splat from 65x65x81 (os 4, 15 nm), EMCCD gain 300 read 50 e-",
                        "blinks": int(len(ev))},
+            "roofline": {"bound": "hbm", "achieved": round(b / (t_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(b / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                         "kernel": "splat_tiles_noise", "algorithmic_bytes": b,
+                         "note": "bound by the noise chain's VALU work (pcg4d counter draws, the Knuth Poisson loop and "
+                                 "the EMCCD Gamma per pixel), not by HBM"},
             "splat_noise_kernel": {"avg_ms": round(t_ms, 4), "GB_s": round(b / (t_ms * 1e-3) / 1e9, 1),
                                    "bound": "fp64 valu (counter RNG + Knuth/Gamma noise)"},
             "_events": (ev, st, np.asarray(planes.cpu()))}
@@ -378,6 +383,11 @@ def run_psf(ctx, steps, warmup):
             "planes_per_s": planes * 1000.0 / ms, "ms_per_step_one_shot_call": round(ms_call, 4),
             "config": {"workload": "PSF library: 8 wavelengths x 64 Zernike sets (Noll 4-21, sigma 0.04), 41 planes "
                                    "of 65x65, M 128; a step = one replay of the library's captured HIP graph"},
+            "roofline": {"bound": "mfma", "achieved": round(3 * tf, 1), "peak": MFMA_FP16_PEAK_TFS,
+                         "unit": "TFLOP/s", "frac": round(3 * tf / MFMA_FP16_PEAK_TFS, 4), "traffic": None,
+                         "kernel": "psf_planes_mfma",
+                         "note": "FP16 flops of the split products (3 per algorithmic complex-FP32 product, P 65, "
+                                 "M 128); dense FP16 peak"},
             "mfma_kernel": {"avg_ms": round(kms, 4), "TFLOP_s_complex_fp32_equiv": round(tf, 1),
                             "TFLOP_s_executed_fp16": round(3 * tf, 1),
                             "frac_of_fp16_dense_peak": round(3 * tf / MFMA_FP16_PEAK_TFS, 4)},
@@ -508,6 +518,10 @@ def run_c5(ctx, steps, warmup):
     return {"value": len(p) * 1000.0 / ms, "unit": "ROI gradients/s", "ms_per_step": ms,
             "config": {"workload": "C5 adjoint: 4096 ROIs 15x15, Gibson-Lanni + 21 Zernike (Noll 4-24), pupil M 64, "
                                    "os 4, Poisson NLL gradient wrt x, y, z, N, b and the shared coefficients"},
+            "roofline": {"bound": "mfma", "achieved": round(flops / (kms * 1e-3) / 1e12, 2), "peak": FP32_PEAK_TFS,
+                         "unit": "TFLOP/s", "frac": round(flops / (kms * 1e-3) / 1e12 / FP32_PEAK_TFS, 4),
+                         "traffic": None, "kernel": "roi_backward",
+                         "note": "algorithmic FP32 flops of the forward and adjoint matrix DFTs; FP32 MFMA peak"},
             "roi_kernel": {"avg_ms": round(kms, 4), "TFLOP_s_fp32": round(flops / (kms * 1e-3) / 1e12, 2),
                            "frac_of_fp32_mfma_peak": round(flops / (kms * 1e-3) / 1e12 / FP32_PEAK_TFS, 4),
                            "flops": "algorithmic (P = 60 samples per axis); executed on 64-padded FP32 MFMA"},
@@ -690,7 +704,6 @@ def main():
         r = run_smlm(ctx, args.steps, args.warmup)
         r.pop("_events")
         config, unit, dtype = r["config"], "frames/s", "f64"
-        r["roofline"] = None
     elif args.workload == "dyes":
         r = run_dyes(ctx, args.steps, args.warmup)
         config, unit, dtype = r["config"], "dyes/s", "f64"
@@ -703,11 +716,9 @@ def main():
         r = run_c5(ctx, args.steps, args.warmup)
         r.pop("_case")
         config, unit, dtype = r["config"], "ROI gradients/s", "f32"
-        r["roofline"] = None
     else:
         r = run_psf(ctx, args.steps, args.warmup)
         config, unit, dtype = r["config"], "PSFs/s", "f16x3/f32"
-        r["roofline"] = None
     cpu = None
     if args.workload == "wf" and not args.no_secondary:
         ev = None
