@@ -223,6 +223,13 @@ typedef struct
    double xUm, yUm, zNm, tStart, tEnd, brightness, excite;
 } iscg_blink_event;
 
+/* Replaces BucketEventsByFrame (Sim/SMLMSimulation.cpp:203-217) on the device: for each frame f < nFrames the
+ * indices of the events overlapping it (tEnd > 0, tStart < nFrames, f in [floor(tStart), floor(tEnd)] clipped to
+ * the movie), in event order: frame f's list is bucketIdx[bucketStart[f] .. bucketStart[f+1]) (nFrames + 1 offsets).
+ * total receives the number of entries; more than idxCap fails (with total set, so the caller can size the output).
+ * Arrays may be host or device memory; the call synchronizes the context's stream. */
+int32_t iscg_bucket_events(iscg_context* h, const iscg_blink_event* events, uint32_t nEvents, uint32_t nFrames,
+                           uint32_t* bucketStart, uint32_t* bucketIdx, uint64_t idxCap, uint64_t* total);
 /* Replaces CollectFrameEmitters (Sim/SMLMSimulation.cpp:55-101) for a batch of frames on the device. Frame k (index
  * frameIndex[k]) takes the events bucketIdx[bucketStart[k] .. bucketStart[k+1]) (event order; BucketEventsByFrame)
  * that overlap it; driftPx: 2 nFrames (x, y) pixel offsets or NULL; nz > 0: the kernel's plane count and zStepNm for

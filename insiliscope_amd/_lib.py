@@ -119,6 +119,7 @@ SIGNATURES = {
     "iscg_tiff_destroy": (None, [P]),
     "iscg_collect_frame_emitters": (I32, [P, P, U32, P, P, P, U32, U32, U32, D, D, P, I32, D, D, I32, P, P, U64, P,
                                           P]),
+    "iscg_bucket_events": (I32, [P, P, U32, U32, P, P, U64, C.POINTER(U64)]),
     "iscg_autotune_set": (I32, [P, I32]),
     "iscg_autotune_dump": (I32, [P, C.c_char_p, I32]),
     "iscg_autotune_load": (I32, [P, C.c_char_p]),

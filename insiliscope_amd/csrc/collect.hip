@@ -15,6 +15,8 @@
 // prefix over the workgroup's waves keep them in bucket order).
 #include "iscg_internal.h"
 
+#include <hipcub/hipcub.hpp>
+
 namespace iscg {
 namespace {
 
@@ -215,6 +217,71 @@ __global__ __launch_bounds__(kCollectThreads) void collect_fill_kernel(CollectAr
    }
 }
 
+
+// ---- BucketEventsByFrame (Sim/SMLMSimulation.cpp:203-217) -------------------------------------------------------
+// Event e belongs to frames [f0, f1] with f0 = max(0, floor(tStart)), f1 = min(nFrames - 1, floor(tEnd)) when
+// tEnd > 0 and tStart < nFrames (floor then the conversion to long as the reference's x86-64 build evaluates it:
+// out of range or NaN gives LONG_MIN). The (frame, event) pairs are written in event order and sorted by frame with a
+// stable radix sort, so each frame's list keeps event order; the frame offsets come from the sorted keys.
+__device__ __forceinline__ long long floor_to_long(double t)
+{
+   const double f = floor(t);
+   return (f >= -9.2233720368547758e18 && f < 9.2233720368547758e18) ? (long long)f
+                                                                     : (long long)0x8000000000000000ull;
+}
+
+__device__ __forceinline__ void frame_span(const iscg_blink_event& e, uint32_t nFrames, long long& f0, long long& f1)
+{
+   f0 = 1;
+   f1 = 0;  // empty
+   if (!(e.tEnd > 0.0) || !(e.tStart < (double)nFrames))
+      return;
+   const long long a = floor_to_long(e.tStart), b = floor_to_long(e.tEnd), last = (long long)nFrames - 1;
+   f0 = a > 0 ? a : 0;
+   f1 = b < last ? b : last;
+}
+
+__global__ void bucket_count_kernel(const iscg_blink_event* __restrict__ ev, uint32_t n, uint32_t nFrames,
+                                    uint32_t* __restrict__ cnt)
+{
+   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+   {
+      long long f0, f1;
+      frame_span(ev[e], nFrames, f0, f1);
+      cnt[e] = f1 >= f0 ? (uint32_t)(f1 - f0 + 1) : 0u;
+   }
+}
+
+__global__ void bucket_pairs_kernel(const iscg_blink_event* __restrict__ ev, uint32_t n, uint32_t nFrames,
+                                    const unsigned long long* __restrict__ off, uint32_t* __restrict__ keys,
+                                    uint32_t* __restrict__ vals)
+{
+   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+   {
+      long long f0, f1;
+      frame_span(ev[e], nFrames, f0, f1);
+      unsigned long long o = off[e];
+      for (long long f = f0; f <= f1; ++f, ++o)
+      {
+         keys[o] = (uint32_t)f;
+         vals[o] = e;
+      }
+   }
+}
+
+// start[k] = the first sorted position with key >= k (k = 0 .. nFrames); keys sorted ascending, P of them
+__global__ void bucket_starts_kernel(const uint32_t* __restrict__ keys, unsigned long long P, uint32_t nFrames,
+                                     uint32_t* __restrict__ start)
+{
+   for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i <= P;
+        i += (unsigned long long)gridDim.x * blockDim.x)
+   {
+      const long long lo = i == 0 ? -1 : (long long)keys[i - 1];
+      const long long hi = i == P ? (long long)nFrames : (long long)keys[i];
+      for (long long k = lo + 1; k <= hi; ++k)
+         start[k] = (uint32_t)i;
+   }
+}
 }  // namespace
 }  // namespace iscg
 
@@ -308,6 +375,87 @@ int32_t iscg_collect_frame_emitters(iscg_context* h, const iscg_blink_event* eve
       zCounts[0] = z[0];
       zCounts[1] = z[1];
    }
+   return 0;
+}
+
+
+int32_t iscg_bucket_events(iscg_context* h, const iscg_blink_event* events, uint32_t nEvents, uint32_t nFrames,
+                           uint32_t* bucketStart, uint32_t* bucketIdx, uint64_t idxCap, uint64_t* total)
+{
+   if (begin(h))
+      return -1;
+   if (!bucketStart || !total || (nEvents && !events))
+      return fail(h, "iscg_bucket_events: null argument");
+   *total = 0;
+   const iscg_blink_event* dEv =
+      nEvents ? (const iscg_blink_event*)stage_in(h, 0, events, (size_t)nEvents * sizeof(iscg_blink_event)) : nullptr;
+   if (nEvents && !dEv)
+      return -1;
+   DevBuf* t = h->bucketTmp;
+   unsigned long long P = 0;
+   if (nEvents)
+   {
+      if (!ok(h, t[0].reserve((size_t)nEvents * 4), "bucket allocation") ||
+          !ok(h, t[1].reserve((size_t)nEvents * 8), "bucket allocation"))
+         return -1;
+      hipLaunchKernelGGL(bucket_count_kernel, dim3(grid_for(nEvents, 256, 8)), dim3(256), 0, h->stream, dEv,
+                         nEvents, nFrames, t[0].as<uint32_t>());
+      size_t tmpBytes = 0;
+      if (!ok(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, t[0].as<uint32_t>(),
+                                                  t[1].as<unsigned long long>(), nEvents, h->stream), "bucket scan") ||
+          !ok(h, t[5].reserve(tmpBytes), "bucket allocation") ||
+          !ok(h, hipcub::DeviceScan::ExclusiveSum(t[5].p, tmpBytes, t[0].as<uint32_t>(),
+                                                  t[1].as<unsigned long long>(), nEvents, h->stream), "bucket scan"))
+         return -1;
+      uint32_t lastCnt = 0;
+      unsigned long long lastOff = 0;
+      if (!ok(h, hipMemcpyAsync(&lastCnt, t[0].as<uint32_t>() + nEvents - 1, 4, hipMemcpyDeviceToHost, h->stream),
+              "bucket total") ||
+          !ok(h, hipMemcpyAsync(&lastOff, t[1].as<unsigned long long>() + nEvents - 1, 8, hipMemcpyDeviceToHost,
+                                h->stream), "bucket total") ||
+          !ok(h, hipStreamSynchronize(h->stream), "bucket total"))
+         return -1;
+      P = lastOff + lastCnt;
+   }
+   *total = P;
+   if (P > 0x7FFFFFFFull)
+      return fail(h, "iscg_bucket_events: more than 2^31 - 1 (frame, event) pairs");
+   if (P > idxCap || (P && !bucketIdx))
+      return fail(h, "iscg_bucket_events: " + std::to_string(P) + " entries, the index output holds " +
+                        std::to_string(idxCap));
+   uint32_t* dStart = (uint32_t*)stage_out(h, 6, bucketStart, ((size_t)nFrames + 1) * 4);
+   uint32_t* dIdx = P ? (uint32_t*)stage_out(h, 7, bucketIdx, (size_t)P * 4) : nullptr;
+   if (!dStart || (P && !dIdx))
+      return -1;
+   if (P)
+   {
+      if (!ok(h, t[2].reserve((size_t)P * 4), "bucket allocation") ||
+          !ok(h, t[3].reserve((size_t)P * 4), "bucket allocation") ||
+          !ok(h, t[4].reserve((size_t)P * 4), "bucket allocation"))
+         return -1;
+      hipLaunchKernelGGL(bucket_pairs_kernel, dim3(grid_for(nEvents, 256, 8)), dim3(256), 0, h->stream, dEv, nEvents,
+                         nFrames, (const unsigned long long*)t[1].as<unsigned long long>(), t[2].as<uint32_t>(),
+                         t[3].as<uint32_t>());
+      int bits = 1;
+      while (bits < 32 && (1ull << bits) < (unsigned long long)nFrames)
+         ++bits;
+      size_t tmpBytes = 0;
+      if (!ok(h, hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, t[2].as<uint32_t>(), t[4].as<uint32_t>(),
+                                                     t[3].as<uint32_t>(), dIdx, (int)P, 0, bits, h->stream),
+              "bucket sort") ||
+          !ok(h, t[5].reserve(tmpBytes), "bucket allocation") ||
+          !ok(h, hipcub::DeviceRadixSort::SortPairs(t[5].p, tmpBytes, t[2].as<uint32_t>(), t[4].as<uint32_t>(),
+                                                     t[3].as<uint32_t>(), dIdx, (int)P, 0, bits, h->stream),
+              "bucket sort"))
+         return -1;
+   }
+   hipLaunchKernelGGL(bucket_starts_kernel, dim3(grid_for((size_t)P + 1, 256, 8)), dim3(256), 0, h->stream,
+                      (const uint32_t*)(P ? t[4].as<uint32_t>() : nullptr), (unsigned long long)P, nFrames, dStart);
+   if (!ok(h, hipGetLastError(), "bucket launch"))
+      return -1;
+   if (!finish_out(h, bucketStart, dStart, ((size_t)nFrames + 1) * 4) ||
+       (P && !finish_out(h, bucketIdx, dIdx, (size_t)P * 4)))
+      return -1;
    return 0;
 }
 

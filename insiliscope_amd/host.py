@@ -539,6 +539,38 @@ def CollectFrameEmitters(events, frameIndex: int, pixelSizeNm: float, photonsPer
     return out
 
 
+def _device_events(events):
+    ev = np.asarray(events)
+    if ev.dtype == DEVICE_EVENT_DTYPE:
+        return np.ascontiguousarray(ev)
+    e7 = np.zeros(len(ev), DEVICE_EVENT_DTYPE)
+    for k in ("xUm", "yUm", "zNm", "tStart", "tEnd", "brightness"):
+        e7[k] = ev[k]
+    e7["excite"] = ev["excite"] if "excite" in (ev.dtype.names or ()) else 1.0
+    return e7
+
+
+def BucketEventsByFrameGpu(events, nFrames: int, ctx: Context | None = None):
+    """sim::BucketEventsByFrame on the GPU (iscg_bucket_events): (start int64 [nFrames + 1], idx uint32), as
+    BucketEventsByFrame returns them."""
+    ctx = ctx or default_context()
+    ev = _device_events(events)
+    n = max(0, int(nFrames))
+    start = np.zeros(n + 1, np.uint32)
+    total = C.c_uint64(0)
+    cap = 4 * len(ev) + 16
+    idx = np.zeros(cap, np.uint32)
+    ctx.follow_torch()
+    rc = ctx.L.iscg_bucket_events(ctx.h, ev.ctypes.data if len(ev) else None, len(ev), n, start.ctypes.data,
+                                  idx.ctypes.data, cap, C.byref(total))
+    if rc != 0 and total.value > cap:
+        idx = np.zeros(total.value, np.uint32)
+        rc = ctx.L.iscg_bucket_events(ctx.h, ev.ctypes.data, len(ev), n, start.ctypes.data, idx.ctypes.data,
+                                      total.value, C.byref(total))
+    ctx.check(rc, "iscg_bucket_events")
+    return start.astype(np.int64), idx[: total.value]
+
+
 def CollectFrameEmittersBatch(events, bucketStart, bucketIdx, frameIndex, width: int, height: int,
                               pixelSizeNm: float, photonsPerBlink: float, drift=None, psfCache: PsfKernelCache | None = None,
                               globalZOffsetUm: float = 0.0, kernel: int = 0, illumination=None,
@@ -548,14 +580,7 @@ def CollectFrameEmittersBatch(events, bucketStart, bucketIdx, frameIndex, width:
     takes events bucketIdx[bucketStart[k]:bucketStart[k+1]] (BucketEventsByFrame). drift: (nFrames, 2) pixel offsets.
     Returns (records FRAME_EMITTER_DTYPE, start offsets nFrames + 1, (z clamped, z total))."""
     ctx = ctx or default_context()
-    ev = np.asarray(events)
-    if ev.dtype != DEVICE_EVENT_DTYPE:
-        e7 = np.zeros(len(ev), DEVICE_EVENT_DTYPE)
-        for k in ("xUm", "yUm", "zNm", "tStart", "tEnd", "brightness"):
-            e7[k] = ev[k]
-        e7["excite"] = ev["excite"] if "excite" in (ev.dtype.names or ()) else 1.0
-        ev = e7
-    ev = np.ascontiguousarray(ev)
+    ev = _device_events(events)
     st = np.ascontiguousarray(bucketStart, np.uint32)
     idx = np.ascontiguousarray(bucketIdx, np.uint32)
     fi = np.ascontiguousarray(frameIndex, np.int64)

@@ -274,3 +274,40 @@ def test_collect_frame_emitters_on_gpu_vs_reference():
                 assert np.array_equal(got[k], want[k]), (f, k)
             assert (got["kernel"] == 2).all()
         assert zc == tuple(zt) and zt[0] > 0
+
+
+def test_bucket_events_on_gpu_vs_reference():
+    """BucketEventsByFrame on the GPU (iscg_bucket_events: stable radix sort of the (frame, event) pairs) against the
+    compiled reference's lists: events spanning frames, before and after the movie, NaN and infinite times; then
+    events -> buckets -> frame records entirely on the device, equal to the reference's CollectFrameEmitters."""
+    R = O.Reference()
+    rng = np.random.default_rng(13)
+    n = 20000
+    ev = np.zeros(n, host.BLINK_EVENT_DTYPE)
+    ts = rng.uniform(-5.0, 300.0, n)
+    ev["tStart"] = ts
+    ev["tEnd"] = ts + rng.exponential(1.5, n)
+    ev["xUm"] = rng.uniform(0.0, 5.0, n)
+    ev["yUm"] = rng.uniform(0.0, 5.0, n)
+    ev["brightness"] = 1.0
+    special = [(-1e300, 2.5), (3.0, np.inf), (np.nan, 4.0), (2.0, np.nan), (-np.inf, 0.0), (49.999, 50.0),
+               (50.0, 51.0), (0.0, 0.0), (-0.5, 1e-300), (7.0, 3.0), (-2.0, 1e30)]
+    for i, (a, b) in enumerate(special):
+        ev["tStart"][i], ev["tEnd"][i] = a, b
+    for nF in (0, 1, 50, 297):
+        start, idx = host.BucketEventsByFrameGpu(ev, nF)
+        counts, want = R.bucket_events(ev, nF)
+        assert np.array_equal(np.diff(start), counts) and np.array_equal(idx, want), nF
+    nF = 297
+    start, idx = host.BucketEventsByFrameGpu(ev, nF)
+    recs, rst, _ = host.CollectFrameEmittersBatch(ev, start, idx, np.arange(nF, dtype=np.int64), 50, 50, 100.0,
+                                                  800.0)
+    for f in (0, 1, 150, 296):
+        e7 = np.zeros((bst := int(start[f + 1] - start[f]), 7))
+        sub = ev[idx[start[f]:start[f + 1]]]
+        for j, k in enumerate(("xUm", "yUm", "zNm", "tStart", "tEnd", "brightness")):
+            e7[:, j] = sub[k]
+        e7[:, 6] = 1.0
+        want, _ = R.collect_emitters_full(e7, f, 50, 50, 100.0, 800.0)
+        got = recs[rst[f]:rst[f + 1]]
+        assert len(got) == len(want) and all(np.array_equal(got[k], want[k]) for k in ("xPx", "yPx", "photons")), f
