@@ -411,8 +411,11 @@ def cpu_psf(n=64):
 
 def run_c1(ctx, steps, warmup):
     """One step = the C1 PSF on the GPU (41 planes, 129x129 at os 4, M 128), its splat kernel (block sums) set up on
-    the device, and 50 frames of 128x128 rendered with Catmull-Rom splats, bg 20 and sCMOS noise: the movie's
-    GPU work end to end (the blink events are collected on the host beforehand)."""
+    the device, the movie's blink events (resident in HBM, bucketed by frame once beforehand) collected into frame
+    emitters on the device, and 50 frames of 128x128 rendered with Catmull-Rom splats, bg 20 and sCMOS noise: the
+    movie end to end, as the reference's CPU timing covers it (RenderPhotonImage collects each frame's emitters)."""
+    import ctypes as C
+
     import torch
     from insiliscope_amd import configs, host
     req = configs.c1_psf(seed=1, oversampling=4)
@@ -421,12 +424,29 @@ def run_c1(ctx, steps, warmup):
     host.ComputePsfPlanesBatch([req], normalize=True, out=planes, ctx=ctx)
     cache = host.PsfKernelCache(planes[0], oversampling=4, zStepNm=50.0, interpMode=2)
     ev, st = configs.c1_events(seed=1, n_frames=50)
-    ems = np.zeros(len(ev), host.FRAME_EMITTER_DTYPE)
-    for f in range(50):
-        ems[st[f]:st[f + 1]] = host.CollectFrameEmitters(ev[st[f]:st[f + 1]], f, 100.0, 1.0, psfCache=cache)
-    d_ems = torch.from_numpy(ems.view(np.uint8).reshape(len(ems), host.FRAME_EMITTER_DTYPE.itemsize)).cuda()
-    d_start = torch.from_numpy(st.astype(np.int32)).cuda()
+    # the blink events resident in HBM, bucketed by frame once; each step collects the frames' emitters on the
+    # device (CollectFrameEmitters, as RenderPhotonImage does per frame on the CPU)
+    ev7 = host._device_events(ev)
+    d_ev = torch.from_numpy(ev7.view(np.uint8).reshape(len(ev7), ev7.dtype.itemsize)).cuda()
+    d_bst = torch.empty(51, dtype=torch.int32, device="cuda")
+    d_bidx = torch.empty(4 * len(ev) + 16, dtype=torch.int32, device="cuda")
+    d_fi = torch.arange(50, dtype=torch.int64, device="cuda")
+    d_ems = torch.empty((4 * len(ev) + 16, host.FRAME_EMITTER_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    d_start = torch.empty(51, dtype=torch.int32, device="cuda")
     d_frames = torch.arange(50, dtype=torch.int32, device="cuda")
+    total = C.c_uint64(0)
+
+    # the frames' event lists, once per movie (the reference buckets a movie's events once, ScopeMovie.cpp:6183)
+    ctx.follow_torch()
+    ctx.check(ctx.L.iscg_bucket_events(ctx.h, d_ev.data_ptr(), len(ev), 50, d_bst.data_ptr(), d_bidx.data_ptr(),
+                                       d_bidx.numel(), C.byref(total)), "iscg_bucket_events")
+
+    def collect():
+        ctx.follow_torch()
+        ctx.check(ctx.L.iscg_collect_frame_emitters(
+            ctx.h, d_ev.data_ptr(), len(ev), d_bst.data_ptr(), d_bidx.data_ptr(), d_fi.data_ptr(), 50, 128, 128, 100.0,
+            1.0, None, nz, 50.0, 0.0, 0, None, d_ems.data_ptr(), d_ems.shape[0], d_start.data_ptr(), None),
+            "iscg_collect_frame_emitters")
     w = h = 128
     off = torch.full((h, w), 100.0, dtype=torch.float32, device="cuda")
     adu = torch.empty((50, h, w), dtype=torch.int16, device="cuda")
@@ -440,7 +460,9 @@ def run_c1(ctx, steps, warmup):
         host.ComputePsfPlanesBatch([req], normalize=True, out=planes, ctx=ctx)
         serial[0] += 1
         cache.serial = serial[0]  # a new stack: the block sums are made again
-        if not (sp.SetKernels([cache]) and sp.RenderBatch(d_ems, d_start, d_frames, cam, 1, outAdu=adu)):
+        collect()
+        n = int(d_start[-1].item())
+        if not (sp.SetKernels([cache]) and sp.RenderBatch(d_ems[:n], d_start, d_frames, cam, 1, outAdu=adu)):
             raise RuntimeError(sp.err)
 
     for _ in range(warmup):
@@ -452,7 +474,8 @@ def run_c1(ctx, steps, warmup):
     prof = profiled(ctx, lambda k: step(), min(steps, 3))
     return {"value": 50 * 1000.0 / ms, "unit": "frames/s", "ms_per_step": ms,
             "config": {"workload": "SMLM 128x128 x 50 frames end to end: Gibson-Lanni PSF (41 planes 129x129, os 4, "
-                                   "M 128, Noll 5-15 +-0.05) + Catmull-Rom splats of ~200 blinks/frame + sCMOS",
+                                   "M 128, Noll 5-15 +-0.05) + frame emitters collected from the events on the GPU + "
+                                   "Catmull-Rom splats of ~200 blinks/frame + sCMOS",
                        "blinks": int(len(ev))},
             "kernels": {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4)} for k, v in prof.items()},
             "_events": (ev, st)}

@@ -235,14 +235,17 @@ int32_t iscg_bucket_events(iscg_context* h, const iscg_blink_event* events, uint
  * that overlap it; driftPx: 2 nFrames (x, y) pixel offsets or NULL; nz > 0: the kernel's plane count and zStepNm for
  * NearestZIndex (0: zIndex 0); illumination: width x height factors at the undrifted site, or NULL; kernel: the
  * records' kernel index. Writes the records into out (outCap records; frame k's are [outStart[k], outStart[k+1]),
- * nFrames + 1 offsets), the same bits as the reference's, and zCounts (may be NULL) = {clamped, total}. Arrays may
- * be host or device memory; the call synchronizes the context's stream. */
+ * nFrames + 1 offsets, uint32 as iscg_splat_render_frames takes them), the same bits as the reference's, and zCounts
+ * (may be NULL) = {clamped, total}. Arrays may be host or device memory. The call reads the bucket offsets on the
+ * host; when out and outStart are device memory, out holds every bucket entry (outCap >= bucketStart[nFrames]) and
+ * zCounts is NULL, it then returns without waiting for the device (outStart[nFrames] holds the count), else it
+ * synchronizes the context's stream. */
 int32_t iscg_collect_frame_emitters(iscg_context* h, const iscg_blink_event* events, uint32_t nEvents,
                                     const uint32_t* bucketStart, const uint32_t* bucketIdx, const int64_t* frameIndex,
                                     uint32_t nFrames, uint32_t width, uint32_t height, double pixelSizeNm,
                                     double photonsPerBlink, const double* driftPx, int32_t nz, double zStepNm,
                                     double globalZOffsetUm, int32_t kernel, const float* illumination,
-                                    iscg_frame_emitter* out, uint64_t outCap, uint64_t* outStart, int64_t* zCounts);
+                                    iscg_frame_emitter* out, uint64_t outCap, uint32_t* outStart, int64_t* zCounts);
 
 /* Replaces SplatGpuHost::RenderFrames / GpuSimulator::RenderFrames (Sim/SplatGpu.h:57, Sim/GpuSimD3D11.h:99) and,
  * bit for bit, RenderPhotonImage (Sim/SMLMSimulation.cpp:103) + ApplyNoiseChain per frame.

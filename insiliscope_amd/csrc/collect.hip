@@ -143,6 +143,13 @@ __global__ __launch_bounds__(1024) void collect_scan_kernel(const uint32_t* __re
       start[n] = carry;
 }
 
+__global__ void collect_start32_kernel(const unsigned long long* __restrict__ start, uint32_t n,
+                                       uint32_t* __restrict__ out)
+{
+   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x)
+      out[i] = (uint32_t)start[i];
+}
+
 __global__ __launch_bounds__(kCollectThreads) void collect_fill_kernel(CollectArgs a,
                                                                        const unsigned long long* __restrict__ start,
                                                                        iscg_frame_emitter* __restrict__ out,
@@ -294,7 +301,7 @@ int32_t iscg_collect_frame_emitters(iscg_context* h, const iscg_blink_event* eve
                                     uint32_t nFrames, uint32_t width, uint32_t height, double pixelSizeNm,
                                     double photonsPerBlink, const double* driftPx, int32_t nz, double zStepNm,
                                     double globalZOffsetUm, int32_t kernel, const float* illumination,
-                                    iscg_frame_emitter* out, uint64_t outCap, uint64_t* outStart, int64_t* zCounts)
+                                    iscg_frame_emitter* out, uint64_t outCap, uint32_t* outStart, int64_t* zCounts)
 {
    if (begin(h))
       return -1;
@@ -334,7 +341,8 @@ int32_t iscg_collect_frame_emitters(iscg_context* h, const iscg_blink_event* eve
    // counts, then offsets: in a scratch of the context (counts [nFrames] u32, start [nFrames + 1] u64, z counts 2)
    DevBuf& scr = h->stage[6];
    const size_t cntBytes = ((size_t)nFrames * 4 + 7) / 8 * 8, stBytes = ((size_t)nFrames + 1) * 8;
-   if (!ok(h, scr.reserve(cntBytes + stBytes + 16), "collect scratch"))
+   const size_t st32Bytes = ((size_t)nFrames + 1) * 4;
+   if (!ok(h, scr.reserve(cntBytes + stBytes + 16 + st32Bytes), "collect scratch"))
       return -1;
    uint32_t* dCnt = (uint32_t*)scr.p;
    unsigned long long* dStart = (unsigned long long*)((char*)scr.p + cntBytes);
@@ -347,24 +355,41 @@ int32_t iscg_collect_frame_emitters(iscg_context* h, const iscg_blink_event* eve
    hipLaunchKernelGGL(collect_scan_kernel, dim3(1), dim3(1024), 0, h->stream, (const uint32_t*)dCnt, nFrames, dStart);
    if (!ok(h, hipGetLastError(), "collect launch"))
       return -1;
-   unsigned long long total = 0;
-   if (!ok(h, hipMemcpyAsync(&total, dStart + nFrames, sizeof(total), hipMemcpyDeviceToHost, h->stream), "collect") ||
-       !ok(h, hipStreamSynchronize(h->stream), "collect"))
-      return -1;
-   if (total > outCap)
-      return fail(h, "iscg_collect_frame_emitters: " + std::to_string(total) + " emitters, output holds " +
-                        std::to_string(outCap));
+   // The kept emitters number at most the bucket entries (nIdx). With device outputs that hold nIdx records and no
+   // z counts asked for, nothing needs the exact total on the host: the fill is enqueued without waiting. Otherwise
+   // the total is read and checked first, and the call waits for its copies.
+   const bool async = is_device_ptr(out) && is_device_ptr(outStart) && !zCounts && outCap >= nIdx &&
+                      nIdx <= 0xFFFFFFFFull;
+   unsigned long long total = nIdx;
+   if (!async)
+   {
+      if (!ok(h, hipMemcpyAsync(&total, dStart + nFrames, sizeof(total), hipMemcpyDeviceToHost, h->stream),
+              "collect") ||
+          !ok(h, hipStreamSynchronize(h->stream), "collect"))
+         return -1;
+      if (total > outCap)
+         return fail(h, "iscg_collect_frame_emitters: " + std::to_string(total) + " emitters, output holds " +
+                           std::to_string(outCap));
+      if (total > 0xFFFFFFFFull)
+         return fail(h, "iscg_collect_frame_emitters: a batch holds at most 2^32 - 1 emitters");
+   }
    iscg_frame_emitter* dOut = total ? (iscg_frame_emitter*)stage_out(h, 7, out, total * sizeof(iscg_frame_emitter)) : nullptr;
    if (total && !dOut)
       return -1;
    hipLaunchKernelGGL(collect_fill_kernel, dim3(nFrames), dim3(kCollectThreads), 0, h->stream, a,
                       (const unsigned long long*)dStart, dOut, dZ);
-   if (!ok(h, hipGetLastError(), "collect launch"))
+   // the offsets as uint32 (the splat's emStart), in the scratch after the z counts, then to the caller's array
+   uint32_t* dOst = (uint32_t*)((char*)dZ + 16);
+   hipLaunchKernelGGL(collect_start32_kernel, dim3(grid_for((size_t)nFrames + 1, 256, 4)), dim3(256), 0, h->stream,
+                      (const unsigned long long*)dStart, nFrames, dOst);
+   if (!ok(h, hipGetLastError(), "collect launch") ||
+       !ok(h, hipMemcpyAsync(outStart, dOst, st32Bytes, is_device_ptr(outStart) ? hipMemcpyDeviceToDevice :
+                             hipMemcpyDeviceToHost, h->stream), "collect offsets"))
       return -1;
+   if (async)
+      return 0;
    long long z[2] = {0, 0};
-   if (!ok(h, hipMemcpyAsync(outStart, dStart, stBytes, is_device_ptr(outStart) ? hipMemcpyDeviceToDevice :
-                             hipMemcpyDeviceToHost, h->stream), "collect offsets") ||
-       !ok(h, hipMemcpyAsync(z, dZ, 16, hipMemcpyDeviceToHost, h->stream), "collect counts"))
+   if (!ok(h, hipMemcpyAsync(z, dZ, 16, hipMemcpyDeviceToHost, h->stream), "collect counts"))
       return -1;
    if (total && !finish_out(h, out, dOut, total * sizeof(iscg_frame_emitter)))
       return -1;

@@ -589,7 +589,7 @@ def CollectFrameEmittersBatch(events, bucketStart, bucketIdx, frameIndex, width:
     il = None if illumination is None else np.ascontiguousarray(illumination, np.float32)
     cap = int(st[-1]) if len(st) else 0
     out = np.zeros(max(cap, 1), FRAME_EMITTER_DTYPE)
-    start = np.zeros(n + 1, np.uint64)
+    start = np.zeros(n + 1, np.uint32)
     z = np.zeros(2, np.int64)
     nz = psfCache.nz if psfCache is not None else 0
     zs = psfCache.zStepNm if psfCache is not None else 0.0

@@ -311,3 +311,38 @@ def test_bucket_events_on_gpu_vs_reference():
         want, _ = R.collect_emitters_full(e7, f, 50, 50, 100.0, 800.0)
         got = recs[rst[f]:rst[f + 1]]
         assert len(got) == len(want) and all(np.array_equal(got[k], want[k]) for k in ("xPx", "yPx", "photons")), f
+
+
+def test_collect_frame_emitters_device_outputs_without_waiting():
+    """The collect with device outputs and no z counts returns without waiting for the device; the records and
+    offsets it leaves there equal those of the waiting (host-output) call."""
+    import ctypes as C
+    import torch
+    from insiliscope_amd import host as H
+    ctx = H.default_context()
+    rng = np.random.default_rng(8)
+    n, nF = 5000, 40
+    ev = np.zeros(n, H.DEVICE_EVENT_DTYPE)
+    ts = rng.uniform(0.0, nF, n)
+    ev["tStart"], ev["tEnd"] = ts, ts + rng.exponential(0.7, n)
+    ev["xUm"], ev["yUm"], ev["zNm"] = rng.uniform(0, 6, n), rng.uniform(0, 6, n), rng.uniform(-500, 500, n)
+    ev["brightness"], ev["excite"] = 1.0, 1.0
+    start, idx = H.BucketEventsByFrameGpu(ev, nF, ctx=ctx)
+    cache = H.PsfKernelCache(np.zeros((21, 5, 5), np.float32), zStepNm=50.0)
+    want, wst, _ = H.CollectFrameEmittersBatch(ev, start, idx, np.arange(nF, dtype=np.int64), 64, 64, 100.0, 900.0,
+                                               psfCache=cache, ctx=ctx)
+    d_ev = torch.from_numpy(ev.view(np.uint8).reshape(n, -1)).cuda()
+    d_st = torch.from_numpy(start.astype(np.uint32).view(np.int32)).cuda()
+    d_idx = torch.from_numpy(idx.view(np.int32)).cuda()
+    d_fi = torch.arange(nF, dtype=torch.int64, device="cuda")
+    d_out = torch.zeros((len(idx), H.FRAME_EMITTER_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    d_ost = torch.zeros(nF + 1, dtype=torch.int32, device="cuda")
+    ctx.follow_torch()
+    ctx.check(ctx.L.iscg_collect_frame_emitters(ctx.h, d_ev.data_ptr(), n, d_st.data_ptr(), d_idx.data_ptr(),
+                                                d_fi.data_ptr(), nF, 64, 64, 100.0, 900.0, None, cache.nz, 50.0, 0.0,
+                                                0, None, d_out.data_ptr(), d_out.shape[0], d_ost.data_ptr(), None))
+    torch.cuda.synchronize()
+    ost = d_ost.cpu().numpy().view(np.uint32)
+    assert np.array_equal(ost, wst)
+    got = d_out.cpu().numpy()[: ost[-1]].copy().view(H.FRAME_EMITTER_DTYPE).reshape(-1)
+    assert got.tobytes() == want.tobytes()
