@@ -269,8 +269,8 @@ def run_smlm(ctx, steps, warmup, n_frames=2000):
             "roofline": {"bound": "hbm", "achieved": round(b / (t_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(b / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                          "kernel": "splat_tiles_noise", "algorithmic_bytes": b,
-                         "note": "bound by the noise chain's VALU work (pcg4d counter draws, the Knuth Poisson loop and "
-                                 "the EMCCD Gamma per pixel), not by HBM"},
+                         "note": "bound by the noise chain's VALU work (pcg4d counter draws, the Knuth Poisson "
+                                 "loop and the EMCCD Gamma per pixel), not by HBM"},
             "splat_noise_kernel": {"avg_ms": round(t_ms, 4), "GB_s": round(b / (t_ms * 1e-3) / 1e9, 1),
                                    "bound": "fp64 valu (counter RNG + Knuth/Gamma noise)"},
             "_events": (ev, st, np.asarray(planes.cpu()))}

@@ -330,12 +330,14 @@ int32_t iscg_collect_frame_emitters(iscg_context* h, const iscg_blink_event* eve
    const size_t nIdx = st[nFrames];
    if (nIdx && !bucketIdx)
       return fail(h, "iscg_collect_frame_emitters: null bucket lists");
-   const iscg_blink_event* dEv = nEvents ? (const iscg_blink_event*)stage_in(h, 0, events, (size_t)nEvents * sizeof(iscg_blink_event)) : nullptr;
+   const size_t evBytes = (size_t)nEvents * sizeof(iscg_blink_event);
+   const iscg_blink_event* dEv = nEvents ? (const iscg_blink_event*)stage_in(h, 0, events, evBytes) : nullptr;
    const uint32_t* dSt = (const uint32_t*)stage_in(h, 1, bucketStart, (nFrames + 1) * sizeof(uint32_t));
    const uint32_t* dIdx = nIdx ? (const uint32_t*)stage_in(h, 2, bucketIdx, nIdx * sizeof(uint32_t)) : nullptr;
    const int64_t* dFi = (const int64_t*)stage_in(h, 3, frameIndex, nFrames * sizeof(int64_t));
    const double* dDr = driftPx ? (const double*)stage_in(h, 4, driftPx, 2 * nFrames * sizeof(double)) : nullptr;
-   const float* dIl = illumination ? (const float*)stage_in(h, 5, illumination, (size_t)width * height * sizeof(float)) : nullptr;
+   const float* dIl =
+      illumination ? (const float*)stage_in(h, 5, illumination, (size_t)width * height * sizeof(float)) : nullptr;
    if ((nEvents && !dEv) || !dSt || (nIdx && !dIdx) || !dFi || (driftPx && !dDr) || (illumination && !dIl))
       return -1;
    // counts, then offsets: in a scratch of the context (counts [nFrames] u32, start [nFrames + 1] u64, z counts 2)
@@ -373,7 +375,8 @@ int32_t iscg_collect_frame_emitters(iscg_context* h, const iscg_blink_event* eve
       if (total > 0xFFFFFFFFull)
          return fail(h, "iscg_collect_frame_emitters: a batch holds at most 2^32 - 1 emitters");
    }
-   iscg_frame_emitter* dOut = total ? (iscg_frame_emitter*)stage_out(h, 7, out, total * sizeof(iscg_frame_emitter)) : nullptr;
+   iscg_frame_emitter* dOut =
+      total ? (iscg_frame_emitter*)stage_out(h, 7, out, total * sizeof(iscg_frame_emitter)) : nullptr;
    if (total && !dOut)
       return -1;
    hipLaunchKernelGGL(collect_fill_kernel, dim3(nFrames), dim3(kCollectThreads), 0, h->stream, a,

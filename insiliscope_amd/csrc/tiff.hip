@@ -87,8 +87,8 @@ int32_t write_page(iscg_tiff* t, const void* px, uint32_t bps, uint16_t sampleFo
    if (!t->big && t->end + pageBytes + descLen + 2 + 2 + 11 * 12 + 4 > kClassicEnd)
    {
       t->err = "cannot write " + t->path + ": page " + std::to_string(t->pages.size() + 1) +
-               " would pass the 4 GB of a classic TIFF (the file was expected to be smaller; insiliscope_cli --bigtiff 1 "
-               "writes a BigTIFF)";
+               " would pass the 4 GB of a classic TIFF (the file was expected to be smaller; insiliscope_cli "
+               "--bigtiff 1 writes a BigTIFF)";
       return -1;
    }
    errno = 0;
@@ -260,7 +260,8 @@ int32_t iscg_tiff_frames_u16(iscg_tiff* t, const uint16_t* frames, uint32_t nFra
          const hipError_t e = hipMemcpy(host.data(), src, n * sizeof(uint16_t), hipMemcpyDeviceToHost);
          if (e != hipSuccess)
          {
-            t->err = std::string("cannot read frame ") + std::to_string(f) + " from the device: " + hipGetErrorString(e);
+            t->err =
+               std::string("cannot read frame ") + std::to_string(f) + " from the device: " + hipGetErrorString(e);
             return -1;
          }
          src = host.data();

@@ -572,9 +572,9 @@ def BucketEventsByFrameGpu(events, nFrames: int, ctx: Context | None = None):
 
 
 def CollectFrameEmittersBatch(events, bucketStart, bucketIdx, frameIndex, width: int, height: int,
-                              pixelSizeNm: float, photonsPerBlink: float, drift=None, psfCache: PsfKernelCache | None = None,
-                              globalZOffsetUm: float = 0.0, kernel: int = 0, illumination=None,
-                              ctx: Context | None = None):
+                              pixelSizeNm: float, photonsPerBlink: float, drift=None,
+                              psfCache: PsfKernelCache | None = None, globalZOffsetUm: float = 0.0, kernel: int = 0,
+                              illumination=None, ctx: Context | None = None):
     """sim::CollectFrameEmitters (Sim/SMLMSimulation.cpp:55-101) for a batch of frames on the GPU
     (iscg_collect_frame_emitters). events: DEVICE_EVENT_DTYPE records (or BLINK_EVENT_DTYPE, excite 1); frame k
     takes events bucketIdx[bucketStart[k]:bucketStart[k+1]] (BucketEventsByFrame). drift: (nFrames, 2) pixel offsets.

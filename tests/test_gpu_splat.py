@@ -303,7 +303,7 @@ def test_bucket_events_on_gpu_vs_reference():
     recs, rst, _ = host.CollectFrameEmittersBatch(ev, start, idx, np.arange(nF, dtype=np.int64), 50, 50, 100.0,
                                                   800.0)
     for f in (0, 1, 150, 296):
-        e7 = np.zeros((bst := int(start[f + 1] - start[f]), 7))
+        e7 = np.zeros((int(start[f + 1] - start[f]), 7))
         sub = ev[idx[start[f]:start[f + 1]]]
         for j, k in enumerate(("xUm", "yUm", "zNm", "tStart", "tEnd", "brightness")):
             e7[:, j] = sub[k]
