@@ -44,7 +44,7 @@ struct PsfReqDev
 // pupil tables and the split DFT matrices (computed by kernels from the records).
 struct PsfBatch
 {
-   DevBuf reqs, tabA, tabB, W, tz;
+   DevBuf reqs, tabA, tabB, tabP, W, tz;  // tabP: exp(i A) as FP32 pairs, for the row-block kernel
    std::vector<PsfReqDev> hostReqs;   // kept while an upload from them may be in flight
    std::vector<double> hostTz;
    int nReq = 0, size = 0, nz = 0, M = 0, Mp = 0, Pp = 0;
@@ -90,9 +90,12 @@ __device__ double mask_phase_d(int maskModes, double maskWaist, double rhoNorm, 
    return (re == 0.0 && im == 0.0) ? 0.0 : atan2(im, re);
 }
 
+__device__ __forceinline__ void wrapped_sincos(double ang, float* s, float* c);
+
 // A (base phase) and B (phase per metre of ti - ti0) on the Mp x Mp grid; NaN outside the aperture or the M grid.
+// With P0 given, also exp(i A) as FP32 (cos, sin): (0, 0) where A is NaN.
 __global__ void psf_tables_kernel(const PsfReqDev* __restrict__ reqs, int nReq, int Mp, double* __restrict__ A,
-                                  double* __restrict__ B)
+                                  double* __restrict__ B, float2* __restrict__ P0)
 {
    const size_t per = (size_t)Mp * Mp, total = per * nReq;
    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x)
@@ -153,6 +156,13 @@ __global__ void psf_tables_kernel(const PsfReqDev* __restrict__ reqs, int nReq, 
       }
       A[idx] = a;
       B[idx] = b;
+      if (P0)
+      {
+         float sn = 0.0f, c = 0.0f;
+         if (a == a)
+            wrapped_sincos(a, &sn, &c);
+         P0[idx] = make_float2(c, sn);
+      }
    }
 }
 
@@ -504,21 +514,28 @@ __global__ __launch_bounds__(kPsfThreads, WLDS ? 2 : 4) void psf_small_kernel(co
    }
 }
 
-// Small planes, row-block form (planes of size P <= 72, MP in {32, 64, 128}): the same products as psf_small_kernel
-// (bit for bit: the same split operands, accumulated in the same order) with the work laid out per plane as
+// Small planes, row-block form (planes of size P <= 66, MP in {32, 64, 128}): the products of psf_small_kernel
+// with the pupil factored and the work laid out per plane as
+//   0. the defocus factor exp(i t B) depends on the pupil radius only: B is a function of kx^2 + ky^2, the same bits
+//      for (i, j), (j, i) and either sign of the offsets i, j from the grid centre c0. Per plane the workgroup
+//      evaluates it once per pair hi = max(|i|, |j|), lo = min(|i|, |j|) <= c0 (FP64 t B, wrapped, FP32 sincos;
+//      (c0 + 1)(c0 + 2) / 2 entries, a 16th of the M^2 samples at M 128) into LDS;
 //   1. each wave owns one 16-row block rb of the pupil (MP / 16 blocks; with fewer blocks than waves, G = 8 / (MP /
-//      16) waves share a block and split its column tiles): it evaluates those rows' FP64 phase and FP32 sincos
-//      straight into its MFMA A fragments (registers, no LDS), then mid[n][x] = sum_m Pup[n][m] W[x][m] for its
-//      column tiles x over all MP (K) -- the A fragments are reused across the tiles, each W fragment read once;
+//      16) waves share a block and split its column tiles): pupil = exp(i A) (an FP32 table made once per request,
+//      zero outside the aperture) times its radius's defocus factor, split straight into its MFMA A fragments
+//      (registers, no LDS); then mid[n][x] = sum_m Pup[n][m] W[x][m] for its column tiles x over all MP (K) -- the A
+//      fragments are reused across the tiles, each W fragment read once;
 //   2. mid, split and transposed, in LDS (midT[x][n], rows x < P only); E[y][x] = sum_n W[y][n] mid[n][x] for the
 //      wave's E tiles over all MP (K).
-// Two barriers per plane. LDS holds W and midT for the P valid rows only (rows >= P read as zeros): 2 x 4 x 72 x 136
-// halves at MP 128 (157 KB), one workgroup per CU. Workgroups per request (zg) and their XCD placement as in
-// psf_small_kernel.
-constexpr int kRbRows = 72;
+// Two barriers per plane: after step 0 (the factors are in LDS and every wave has left the previous plane's step 2,
+// so midT may be rewritten) and after step 1. LDS holds W and midT for the P valid rows only (rows >= P read as
+// zeros): 2 x 4 x 66 x 136 halves + 2145 factors at MP 128 (157 KB), one workgroup per CU. Workgroups per request
+// (zg) and their XCD placement as in psf_small_kernel. The pupil samples differ from the FP64-phase kernels' by
+// FP32 rounding (one complex product more), so this form is not a tuning candidate beside them.
+constexpr int kRbRows = 66;
 template <int MP>
 __global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfReqDev* __restrict__ reqs,
-                                                                     const double* __restrict__ A,
+                                                                     const float2* __restrict__ P0,
                                                                      const double* __restrict__ B,
                                                                      const _Float16* __restrict__ W,
                                                                      const double* __restrict__ tz, int nz, int Pp,
@@ -531,9 +548,12 @@ __global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfR
    constexpr int CTX = kSmallPp / 16;         // max column tiles
    constexpr int PM = (CTX + G - 1) / G;      // mid tiles per wave (max)
    constexpr int PE = (CTX * CTX + kPsfWaves - 1) / kPsfWaves;  // E tiles per wave (max)
+   constexpr int HM = MP / 2;                                 // c0 = M / 2 <= MP / 2
+   constexpr int ND = (HM + 1) * (HM + 2) / 2;                // radial pairs (hi, lo), lo <= hi <= HM
    static_assert(NRB * G == kPsfWaves, "row blocks must divide the waves");
    __shared__ __attribute__((aligned(16))) _Float16 wl[4][kRbRows][PS];
    __shared__ __attribute__((aligned(16))) _Float16 midT[4][kRbRows][PS];
+   __shared__ float2 defocus[ND];
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int b = (int)blockIdx.x, slot = b / kXcds;
    const int r = (slot / zg) * kXcds + b % kXcds;
@@ -544,8 +564,9 @@ __global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfR
    const int CT = Pp / 16, TE = CT * CT;
    const size_t wPer = (size_t)Pp * MP;
    const _Float16* Wr = W + (size_t)r * 4 * wPer;
-   const double* Ar = A + (size_t)r * MP * MP;
+   const float2* P0r = P0 + (size_t)r * MP * MP;
    const double* Br = B + (size_t)r * MP * MP;
+   const int c0 = q.M / 2, nd = (c0 + 1) * (c0 + 2) / 2;  // M <= MP: c0 <= HM, nd <= ND
    const int lr = lane & 15, lk = 8 * (lane >> 4);
    const half8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
    for (int i = tid; i < 4 * P * (MP / 8); i += kPsfThreads)
@@ -556,32 +577,51 @@ __global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfR
    const int rb = wave % NRB, g0 = wave / NRB;
    const int g = slot % zg;
    const int z0 = g * nz / zg, z1 = (g + 1) * nz / zg;
+   // this lane's pupil row n = 16 rb + lr: its offset from the centre (rows past the M grid are zero in P0 and take
+   // any in-range factor)
+   const int ai = min(abs(16 * rb + lr - c0), c0);
    for (int zi = z0; zi < z1; ++zi)
    {
+      // 0. the plane's defocus factors; entry e = hi (hi + 1) / 2 + lo, read at B[c0 - hi][c0 - lo]
       const double t = tz[(size_t)r * nz + zi];
+      for (int e = tid; e < nd; e += kPsfThreads)
+      {
+         int hi = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+         while (hi * (hi + 1) / 2 > e)
+            --hi;
+         while ((hi + 1) * (hi + 2) / 2 <= e)
+            ++hi;
+         const int lo = e - hi * (hi + 1) / 2;
+         float sn, c;
+         wrapped_sincos(t * Br[(size_t)(c0 - hi) * MP + (c0 - lo)], &sn, &c);
+         defocus[e] = make_float2(c, sn);
+      }
+      __syncthreads();
       // 1. this wave's pupil rows n = 16 rb + lr, columns m = 32 s + lk + j, as split A fragments
       half8 prh[KS], prl[KS], pih[KS], pil[KS];
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks)
       {
+         const float4* src = reinterpret_cast<const float4*>(P0r + (size_t)(16 * rb + lr) * MP + 32 * ks + lk);
+         float4 v[4];
+#pragma unroll
+         for (int u = 0; u < 4; ++u)
+            v[u] = src[u];
 #pragma unroll
          for (int j = 0; j < 8; ++j)
          {
-            const size_t o = (size_t)(16 * rb + lr) * MP + 32 * ks + lk + j;
-            const double a = Ar[o];
-            float sn = 0.0f, c = 0.0f;
-            if (a == a)  // inside the aperture
-               wrapped_sincos(a + t * Br[o], &sn, &c);
+            const float pc = (j & 1) ? v[j >> 1].z : v[j >> 1].x, ps = (j & 1) ? v[j >> 1].w : v[j >> 1].y;
+            const int aj = min(abs(32 * ks + lk + j - c0), c0);
+            const int hi = max(ai, aj), lo = min(ai, aj);
+            const float2 d = defocus[hi * (hi + 1) / 2 + lo];
+            const float c = pc * d.x - ps * d.y, sn = pc * d.y + ps * d.x;
             const _Float16 ch = (_Float16)c, sh = (_Float16)sn;
             prh[ks][j] = ch;
             prl[ks][j] = (_Float16)(c - (float)ch);
             pih[ks][j] = sh;
             pil[ks][j] = (_Float16)(sn - (float)sh);
          }
-         // one k-step's table loads in flight at a time (keeps the FP64 operands of all 32 samples out of registers)
-         __builtin_amdgcn_sched_barrier(0);
       }
-      // (the previous plane's E tiles have read midT: the barrier at the end of the plane loop)
 #pragma unroll 1
       for (int s = 0; s < PM; ++s)
       {
@@ -656,7 +696,7 @@ __global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfR
                plane[(size_t)yo * P + xo] = eR[i] * eR[i] + eI[i] * eI[i];
          }
       }
-      __syncthreads();
+      // (the next plane's first barrier orders these midT reads before its step 1 rewrites midT)
    }
 }
 
@@ -810,7 +850,7 @@ static bool psf_planes_launch(iscg_context* h, const PsfBatch& s, int choice, fl
          {
             hipLaunchKernelGGL((psf_small_rb_kernel<MP>), dim3((nReq + kXcds - 1) / kXcds * kXcds * zg),
                                dim3(kPsfThreads), 0, h->stream, s.reqs.as<const PsfReqDev>(),
-                               s.tabA.as<const double>(), s.tabB.as<const double>(), s.W.as<const _Float16>(),
+                               s.tabP.as<const float2>(), s.tabB.as<const double>(), s.W.as<const _Float16>(),
                                s.tz.as<const double>(), nz, Pp, nReq, zg, out);
             return;
          }
@@ -895,6 +935,7 @@ static int32_t psf_batch_set(iscg_context* h, PsfBatch& b, const iscg_psf_reques
    if (!ok(h, b.reqs.reserve(dev.size() * sizeof(PsfReqDev)), "allocation") ||
        !ok(h, b.tabA.reserve((size_t)nReq * Mp * Mp * sizeof(double)), "allocation") ||
        !ok(h, b.tabB.reserve((size_t)nReq * Mp * Mp * sizeof(double)), "allocation") ||
+       !ok(h, b.tabP.reserve((size_t)nReq * Mp * Mp * sizeof(float2)), "allocation") ||
        !ok(h, b.W.reserve(wHalves * sizeof(_Float16)), "allocation") ||
        !ok(h, b.tz.reserve(tz.size() * sizeof(double)), "allocation"))
       return -1;
@@ -925,7 +966,7 @@ static int32_t psf_batch_run(iscg_context* h, const PsfBatch& b, int32_t normali
    {
       ProfScope ps(h, "psf_tables");
       hipLaunchKernelGGL(psf_tables_kernel, dim3(grid_for(nTab, 256, 16)), dim3(256), 0, h->stream, dReq, nReq, Mp,
-                         b.tabA.as<double>(), b.tabB.as<double>());
+                         b.tabA.as<double>(), b.tabB.as<double>(), rb_eligible(b) ? b.tabP.as<float2>() : nullptr);
    }
    const size_t nW = (size_t)nReq * Pp * Mp;
    hipLaunchKernelGGL(psf_w_kernel, dim3(grid_for(nW, 256, 16)), dim3(256), 0, h->stream, dReq, nReq, Pp, Mp,
@@ -934,16 +975,16 @@ static int32_t psf_batch_run(iscg_context* h, const PsfBatch& b, int32_t normali
       return -1;
    // the small-plane kernel's workgroups per request: every candidate computes each plane in one workgroup the
    // same way, so the planes are bit-identical whichever is chosen (and the planes are written, not accumulated:
-   // the timing launches before normalisation leave the same output)
-   // (the row-block form, where the shape allows it, is a candidate too: the same products in the same order)
+   // the timing launches before normalisation leave the same output). Where the shape allows the row-block form,
+   // the candidates are its workgroups per request only (its pupil samples round differently from the other form's)
    int zg = kZGroups;
    if (Pp <= kSmallPp && Mp <= 128)
    {
       std::vector<int> cands;
-      for (int v = 0; v < (rb_eligible(b) ? 2 : 1); ++v)
-         for (int c : {1, 2, 4, 8, 16, 32})
-            if (c <= nz)
-               cands.push_back(c + kRbVariant * v);
+      const int v = rb_eligible(b) ? 1 : 0;
+      for (int c : {1, 2, 4, 8, 16, 32})
+         if (c <= nz)
+            cands.push_back(c + kRbVariant * v);
       const std::string key = "psf_small M" + std::to_string(Mp) + " P" + std::to_string(Pp) + " size" +
                               std::to_string(b.size) + " nz" + std::to_string(nz) + " req" +
                               std::to_string(pow2_bucket(nReq));
