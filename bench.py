@@ -414,56 +414,22 @@ def run_c1(ctx, steps, warmup):
     the device, the movie's blink events (resident in HBM, bucketed by frame once beforehand) collected into frame
     emitters on the device, and 50 frames of 128x128 rendered with Catmull-Rom splats, bg 20 and sCMOS noise: the
     movie end to end, as the reference's CPU timing covers it (RenderPhotonImage collects each frame's emitters)."""
-    import ctypes as C
-
     import torch
     from insiliscope_amd import configs, host
     req = configs.c1_psf(seed=1, oversampling=4)
-    nz, size = 41, 129
-    planes = torch.empty((1, nz, size, size), dtype=torch.float32, device="cuda")
-    host.ComputePsfPlanesBatch([req], normalize=True, out=planes, ctx=ctx)
-    cache = host.PsfKernelCache(planes[0], oversampling=4, zStepNm=50.0, interpMode=2)
     ev, st = configs.c1_events(seed=1, n_frames=50)
-    # the blink events resident in HBM, bucketed by frame once; each step collects the frames' emitters on the
-    # device (CollectFrameEmitters, as RenderPhotonImage does per frame on the CPU)
-    ev7 = host._device_events(ev)
-    d_ev = torch.from_numpy(ev7.view(np.uint8).reshape(len(ev7), ev7.dtype.itemsize)).cuda()
-    d_bst = torch.empty(51, dtype=torch.int32, device="cuda")
-    d_bidx = torch.empty(4 * len(ev) + 16, dtype=torch.int32, device="cuda")
-    d_fi = torch.arange(50, dtype=torch.int64, device="cuda")
-    d_ems = torch.empty((4 * len(ev) + 16, host.FRAME_EMITTER_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
-    d_start = torch.empty(51, dtype=torch.int32, device="cuda")
-    d_frames = torch.arange(50, dtype=torch.int32, device="cuda")
-    total = C.c_uint64(0)
-
-    # the frames' event lists, once per movie (the reference buckets a movie's events once, ScopeMovie.cpp:6183)
-    ctx.follow_torch()
-    ctx.check(ctx.L.iscg_bucket_events(ctx.h, d_ev.data_ptr(), len(ev), 50, d_bst.data_ptr(), d_bidx.data_ptr(),
-                                       d_bidx.numel(), C.byref(total)), "iscg_bucket_events")
-
-    def collect():
-        ctx.follow_torch()
-        ctx.check(ctx.L.iscg_collect_frame_emitters(
-            ctx.h, d_ev.data_ptr(), len(ev), d_bst.data_ptr(), d_bidx.data_ptr(), d_fi.data_ptr(), 50, 128, 128, 100.0,
-            1.0, None, nz, 50.0, 0.0, 0, None, d_ems.data_ptr(), d_ems.shape[0], d_start.data_ptr(), None),
-            "iscg_collect_frame_emitters")
     w = h = 128
     off = torch.full((h, w), 100.0, dtype=torch.float32, device="cuda")
     adu = torch.empty((50, h, w), dtype=torch.int16, device="cuda")
     cam = configs.scmos_c1()
-    sp = host.SplatGpuHost(ctx)
-    if not sp.SetStatic(w, h, offset=off, flatBackground=20.0):
-        raise RuntimeError(sp.err)
-    serial = [0]
+    # the blink events resident in HBM, bucketed by frame once (the reference buckets a movie's events once,
+    # ScopeMovie.cpp:6183); each step computes the PSF, collects every frame's emitters on the device (as
+    # RenderPhotonImage does per frame on the CPU), splats and noises the 50 frames
+    movie = host.SmlmMovieGpu(ev, 50, w, h, 100.0, 1.0, 20.0, offset=off, ctx=ctx)
 
     def step():
-        host.ComputePsfPlanesBatch([req], normalize=True, out=planes, ctx=ctx)
-        serial[0] += 1
-        cache.serial = serial[0]  # a new stack: the block sums are made again
-        collect()
-        n = int(d_start[-1].item())
-        if not (sp.SetKernels([cache]) and sp.RenderBatch(d_ems[:n], d_start, d_frames, cam, 1, outAdu=adu)):
-            raise RuntimeError(sp.err)
+        if not movie.Render(req, 4, 50.0, 2, cam, 1, outAdu=adu):
+            raise RuntimeError(movie.splat.err)
 
     for _ in range(warmup):
         step()

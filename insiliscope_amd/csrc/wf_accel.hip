@@ -329,6 +329,9 @@ __global__ __launch_bounds__(256) void wf_render_kernel(const float* __restrict_
 
 using namespace iscg;
 
+// The FFT grids the per-focus passes take (iscg_wf_fits and iscg_wf_images share this rule).
+static bool accel_grid_ok(uint32_t NX, uint32_t NY) { return pow2(NX, 1024) && pow2(NY, 512); }
+
 extern "C" {
 
 int32_t iscg_wf_set_budget(iscg_context* h, uint64_t bytes)
@@ -351,6 +354,10 @@ int32_t iscg_wf_fits(iscg_context* h, uint64_t geometry, uint32_t NX, uint32_t N
 {
    if (begin(h))
       return -1;
+   // a grid the passes do not take does not fit: the caller renders that scene on the CPU instead of failing in
+   // Images (WidefieldAccelerator::Fits, Sim/WidefieldRender.h:342-361)
+   if (!accel_grid_ok(NX, NY))
+      return 0;
    WfAccelState& a = accel(h);
    const size_t need = (size_t)planes * NY * spec_stride(NX) * sizeof(float2);
    if (others_bytes(a, geometry) + need > budget_of(h, a))
@@ -375,7 +382,7 @@ int32_t iscg_wf_images(iscg_context* h, const iscg_wf_job* job, float* out)
    if (!job)
       return fail(h, "iscg_wf_images: null job");
    const iscg_wf_job& J = *job;
-   if (!pow2(J.NX, 1024) || !pow2(J.NY, 512))
+   if (!accel_grid_ok(J.NX, J.NY))
       return fail(h, "iscg_wf_images: FFT grid must be powers of two, NX in [16, 1024], NY in [16, 512]");
    if (J.nx == 0 || J.ny == 0 || J.nx > J.NX || J.ny > J.NY || J.cw == 0 || J.ch == 0 || J.fovX0 + J.cw > J.NX ||
        J.fovY0 + J.ch > J.NY)

@@ -56,6 +56,11 @@ def _ptr(a):
     raise TypeError(f"unsupported array type {type(a)!r}")
 
 
+def _numel(a) -> int:
+    """Element count of a numpy array or torch tensor (np.size does not count a tensor's elements)."""
+    return int(a.numel()) if hasattr(a, "numel") else int(np.size(a))
+
+
 def _arr(a, dtype):
     if a is None:
         return None
@@ -686,7 +691,8 @@ class SplatGpuHost:
     def SetStatic(self, width: int, height: int, pixelBase: int = 0, offset=None, gain=None, readNoise=None,
                   background=None, flatBackground: float = 0.0, key: int = 0) -> bool:
         n = width * height
-        fit = lambda m: None if m is None or np.size(m) != n else _arr(m, np.float32)
+        # maps whose size is not width*height are ignored, as the reference's size checks do (SMLMNoise.cpp:151-158)
+        fit = lambda m: None if m is None or _numel(m) != n else _arr(m, np.float32)
         ps = [_ptr(fit(m)) for m in (offset, gain, readNoise, background)]
         self.width, self.height = width, height
         self.ctx.follow_torch()
@@ -743,6 +749,80 @@ class SplatGpuHost:
     def MaxBatchFrames(self) -> int:
         """Frames per call within a 1 GiB output budget (the device holds the whole batch in HBM)."""
         return max(1, (1 << 30) // max(1, 6 * self.width * self.height))
+
+
+class SmlmMovieGpu:
+    """An SMLM movie's frames on the device from its blink events, as the reference's SMLM loop makes them frame by
+    frame on the CPU (Sim/ScopeMovie.cpp:6183 buckets a movie's events once; :7248-7259 collects and renders each
+    frame, :7642-7650 applies the camera noise): the events are uploaded and bucketed by frame once
+    (BucketEventsByFrame, Sim/SMLMSimulation.cpp:203-217); each Render computes the PSF stack on the GPU
+    (ComputeZernikePsfPlanes + the sum-1 normalisation), its block sums, every frame's emitters
+    (CollectFrameEmitters, Sim/SMLMSimulation.cpp:55-101), and splats and noises all frames in one batch
+    (RenderPhotonImage + ApplyNoiseChain). Every buffer is a torch tensor on the context's device."""
+
+    def __init__(self, events, nFrames: int, width: int, height: int, pixelSizeNm: float, photonsPerBlink: float,
+                 flatBackground: float, offset=None, ctx: Context | None = None):
+        import torch
+        self.ctx = ctx or default_context()
+        ev7 = _device_events(events)
+        self.nEvents, self.nFrames, self.width, self.height = len(ev7), int(nFrames), int(width), int(height)
+        self.pixelSizeNm, self.photonsPerBlink = float(pixelSizeNm), float(photonsPerBlink)
+        dev = "cuda"
+        self.d_ev = torch.from_numpy(ev7.view(np.uint8).reshape(len(ev7), ev7.dtype.itemsize)).to(dev)
+        self.d_bst = torch.empty(self.nFrames + 1, dtype=torch.int32, device=dev)
+        cap = 4 * len(ev7) + 16
+        self.d_bidx = torch.empty(cap, dtype=torch.int32, device=dev)
+        self.d_fi = torch.arange(self.nFrames, dtype=torch.int64, device=dev)
+        self.d_ems = torch.empty((cap, FRAME_EMITTER_DTYPE.itemsize), dtype=torch.uint8, device=dev)
+        self.d_start = torch.empty(self.nFrames + 1, dtype=torch.int32, device=dev)
+        self.d_frames = torch.arange(self.nFrames, dtype=torch.int32, device=dev)
+        total = C.c_uint64(0)
+        self.ctx.follow_torch()
+        self.ctx.check(self.ctx.L.iscg_bucket_events(self.ctx.h, self.d_ev.data_ptr(), self.nEvents, self.nFrames,
+                                                     self.d_bst.data_ptr(), self.d_bidx.data_ptr(), cap,
+                                                     C.byref(total)), "iscg_bucket_events")
+        self.splat = SplatGpuHost(self.ctx)
+        if offset is not None and not hasattr(offset, "data_ptr"):
+            offset = torch.from_numpy(np.ascontiguousarray(offset, np.float32)).to(dev)
+        self.offset = offset
+        if not self.splat.SetStatic(self.width, self.height, offset=offset, flatBackground=flatBackground):
+            raise IscgError(self.splat.err)
+        self.planes = None
+        self.cache = None
+        self._req = None
+        self._serial = 0
+
+    def Render(self, psfRequest, oversampling: int, zStepNm: float, interpMode: int, cam: CameraNoiseParams | None,
+               seed: int, outAdu=None, outPhotons=None) -> bool:
+        """The whole movie: PSF -> block sums -> emitters -> splat + noise into outAdu / outPhotons ((nFrames, h, w)
+        uint16 / float32 tensors). False with splat.err set when a step fails."""
+        import torch
+        if self._req is None or self._req is not psfRequest:
+            self._req = psfRequest
+            self._reqArr = PsfRequestArray([psfRequest])
+            size, nz, _ = PsfGeometry(self._reqArr[0])
+            self.planes = torch.empty((1, nz, size, size), dtype=torch.float32, device="cuda")
+        ComputePsfPlanesBatch(self._reqArr, normalize=True, out=self.planes, ctx=self.ctx)
+        self._serial += 1
+        self.cache = PsfKernelCache(self.planes[0], oversampling=oversampling, zStepNm=zStepNm,
+                                    interpMode=interpMode)
+        self.cache.serial = self._serial  # a new stack: its block sums are made again
+        ctx = self.ctx
+        ctx.follow_torch()
+        ctx.check(ctx.L.iscg_collect_frame_emitters(
+            ctx.h, self.d_ev.data_ptr(), self.nEvents, self.d_bst.data_ptr(), self.d_bidx.data_ptr(),
+            self.d_fi.data_ptr(), self.nFrames, self.width, self.height, self.pixelSizeNm, self.photonsPerBlink, None,
+            self.cache.nz, float(zStepNm), 0.0, 0, None, self.d_ems.data_ptr(), self.d_ems.shape[0],
+            self.d_start.data_ptr(), None), "iscg_collect_frame_emitters")
+        n = int(self.d_start[-1].item())
+        return self.splat.SetKernels([self.cache]) and self.splat.RenderBatch(
+            self.d_ems[:n], self.d_start, self.d_frames, cam, seed, outAdu=outAdu, outPhotons=outPhotons)
+
+    def Emitters(self):
+        """The last Render's frame emitters (FRAME_EMITTER_DTYPE) and per-frame offsets, copied to the host."""
+        st = self.d_start.cpu().numpy().astype(np.uint32)
+        ems = self.d_ems[: int(st[-1])].cpu().numpy().reshape(-1).view(FRAME_EMITTER_DTYPE)
+        return ems, st
 
 
 # ---- PSF planes --------------------------------------------------------------------------------------------------

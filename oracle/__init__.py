@@ -468,6 +468,8 @@ class Reference:
         PF, PI16, PI32, PD = C.POINTER(C.c_float), C.POINTER(C.c_int16), C.POINTER(C.c_int32), C.POINTER(C.c_double)
         L.ref_psf_planes.argtypes = [C.POINTER(PsfRequest), PF]
         L.ref_psf_planes.restype = I32
+        L.ref_zernike_pupil_samples.argtypes = [C.POINTER(PsfRequest)]
+        L.ref_zernike_pupil_samples.restype = I32
         L.ref_psf_kernel_cache.argtypes = [C.POINTER(PsfRequest), PF, PF]
         L.ref_psf_kernel_cache.restype = I32
         L.ref_block_sums.argtypes = [PF, I32, I32, PF]
@@ -544,6 +546,9 @@ class Reference:
         if self.L.ref_psf_planes(C.byref(req), _p(out, C.c_float)) != 0:
             raise ValueError("bad PSF request")
         return out
+
+    def pupil_samples(self, req):
+        return int(self.L.ref_zernike_pupil_samples(C.byref(req)))
 
     def psf_kernel_cache(self, req, size, nz, os_):
         bw = size + os_ - 1

@@ -121,6 +121,12 @@ int32_t ref_psf_planes(const orc_psf_request* r, float* out)
    return 0;
 }
 
+// ZernikePupilSamples (Sim/ZernikePsf.cpp:134-148): the pupil samples per axis M of a request.
+int32_t ref_zernike_pupil_samples(const orc_psf_request* r)
+{
+   return sim::ZernikePupilSamples(ToRequest(r));
+}
+
 // BuildZernikePsfKernelCache (Sim/ZernikePsf.cpp:382): sum-1 planes and every plane's row-major block sums.
 int32_t ref_psf_kernel_cache(const orc_psf_request* r, float* planesOut, float* blockSumsOut)
 {

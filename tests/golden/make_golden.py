@@ -22,6 +22,13 @@ Outputs (inputs and expected outputs only):
   dyes_ref.npz       LabelSchedule blinks and continuous windows (dSTORM, PALM, DNA-PAINT, WideField; several
                      kinetics), PersistentBlinks and PersistentBlinksInBins of seeded dye addresses; jsm:: values of
                      seeded inputs around the reduction boundaries
+  roi_ref.npz        ComputeZernikePsfPlanes at the ROI model's optics (C5's Zernike set, M 64), on grids whose samples
+                     coincide with the ROI model's sub-samples (os 4 hw 8 and os 1 hw 7), with and without a sample
+                     depth
+
+  pupil_samples_ref.json  ZernikePupilSamples of 400 seeded requests (automatic and explicit pupil samples)
+
+`python tests/golden/make_golden.py roi` writes only the named fixtures.
 """
 from __future__ import annotations
 
@@ -354,8 +361,55 @@ def dyes(R):
     np.savez_compressed(os.path.join(HERE, "dyes_ref.npz"), **out)
 
 
+ROI_PIN_CASES = {  # name: (oversampling, kernelHalfWidthPx, sampleDepthNm, sampleIndex)
+    "os4": (4, 8, 0.0, 1.33),
+    "os4_depth": (4, 8, 3000.0, 1.33),
+    "os1": (1, 7, 0.0, 1.33),
+    "os1_matched": (1, 7, 1500.0, 1.518),
+}
+ROI_PIN_NZ, ROI_PIN_ZSTEP = 5, 250.0
+
+
+def roi(R):
+    """The reference's PSF planes at the ROI model's optics (670 nm, NA 1.45, ni 1.518, 100 nm pixels, M 64) and
+    C5's Zernike set: 5 planes 250 nm apart, on a (2 hw os + 1)^2 grid. The ROI model's sub-samples of a 15 x 15 ROI
+    are grid samples d, d + 1, ... (d = (2 hw os + 1 - 15 os) // 2) for an emitter at x = y = (hw os - d + 0.5) / os
+    pixels: 7.625 at os 4, hw 8 and 7.5 at os 1, hw 7."""
+    z, _ = configs.c5_rois(seed=1, n=1)
+    out = {"zernike": z, "nz": np.int32(ROI_PIN_NZ), "zstep": np.float64(ROI_PIN_ZSTEP)}
+    for name, (os_, hw, depth, ns) in ROI_PIN_CASES.items():
+        r = req(wavelengthNm=670.0, na=1.45, immersionIndex=1.518, sampleIndex=ns, sampleDepthNm=depth,
+                pixelSizeNm=100.0, zStepNm=ROI_PIN_ZSTEP, oversampling=os_, kernelHalfWidthPx=hw, nz=ROI_PIN_NZ,
+                pupilSamples=64, zernike=z)
+        out[name + "_case"] = np.array([os_, hw, depth, ns])
+        out[name + "_planes"] = R.psf_planes(r, 2 * hw * os_ + 1, ROI_PIN_NZ)
+    np.savez_compressed(os.path.join(HERE, "roi_ref.npz"), **out)
+
+
+def pupil_samples(R):
+    """ZernikePupilSamples (Sim/ZernikePsf.cpp:134-148) of seeded requests: mostly automatic (pupilSamples 0) over
+    half-widths 1-48 px, oversampling 1-8, NA 0.3-1.7, 350-900 nm and 20-250 nm pixels; some explicit values below
+    8, inside [8, 1024] and above 1024."""
+    rng = np.random.default_rng(41)
+    cases = []
+    for i in range(400):
+        f = dict(kernelHalfWidthPx=int(rng.integers(1, 49)), oversampling=int(rng.integers(1, 9)),
+                 na=float(rng.uniform(0.3, 1.7)), wavelengthNm=float(rng.uniform(350, 900)),
+                 pixelSizeNm=float(rng.uniform(20, 250)),
+                 pupilSamples=int(rng.choice([0] * 6 + [-3, 5, 8, 96, 130, 777, 1024, 1500])))
+        cases.append({**f, "M": R.pupil_samples(req(**f))})
+    with open(os.path.join(HERE, "pupil_samples_ref.json"), "w") as fh:
+        json.dump({"source": "ZernikePupilSamples, compiled reference", "cases": cases}, fh, indent=0)
+
+
 def main():
     R = O.Reference()
+    only = set(sys.argv[1:])
+    if only:
+        for name in only:
+            globals()[name](R)
+        print("fixtures", sorted(only), "written to", HERE)
+        return
     maps_and_geometry(R)
     rng_vectors()
     shutil.copyfile(os.path.join(REF, "tests/psf/zernike_ref.bin"), os.path.join(HERE, "zernike_ref.bin"))
@@ -366,6 +420,8 @@ def main():
     wf(R, P)
     binned(R, P)
     dyes(R)
+    roi(R)
+    pupil_samples(R)
     print("fixtures written to", HERE)
 
 

@@ -38,6 +38,46 @@ def test_psf_small_kernel_every_candidate_identical():
     c.close()
 
 
+@pytest.mark.parametrize("shape", ["auto_m96_33", "m128_69"])
+def test_psf_chunked_kernel_vs_oracle_every_candidate_identical(orc, shape):
+    """The chunked small-plane kernel (psf_small_kernel: pupil sizes or plane sizes the row-block form does not
+    take): C1's optics with the automatic pupil rule (pupilSamples 0 -> M 96) at 33 x 33, and M 128 at 69 x 69
+    (padded 80): every plane within 1e-5 relative L2 of the FP64 oracle, and every workgroups-per-request
+    candidate (1 to 32) bit-identical."""
+    base = configs.c1_psf(seed=1)
+    reqs = []
+    for i in range(4):
+        r = dict(base, zernike=configs.zernike_noll(10 + i, range(5, 16), amplitude=0.05))
+        if shape == "auto_m96_33":
+            r.update(pupilSamples=0, nz=21)
+        else:
+            r.update(kernelHalfWidthPx=34, nz=9)
+        reqs.append(r)
+    size, nz, M = host.PsfGeometry(reqs[0])
+    assert (size, M) == ((33, 96) if shape == "auto_m96_33" else (69, 128)), (size, M)
+    fixed = host.Context(0)
+    fixed.set_autotune(False)
+    want = host.ComputePsfPlanesBatch(reqs, ctx=fixed)
+    fixed.close()
+    errs = []
+    for i, r in enumerate(reqs):
+        o = orc.psf_planes(O.psf_request(**r))
+        for k in range(nz):
+            errs.append(np.sqrt(((want[i, k] - o[k]) ** 2).sum() / (o[k] ** 2).sum()))
+    print(shape, "max rel L2 vs oracle", max(errs), flush=True)
+    assert max(errs) <= 1e-5
+    c = host.Context(0)
+    got = host.ComputePsfPlanesBatch(reqs, ctx=c)
+    key = _only_key(c.autotune_choices(), "psf_small")
+    cands = [zg for zg in (1, 2, 4, 8, 16, 32) if zg <= nz]
+    assert c.autotune_choices()[key] in cands
+    assert np.array_equal(got, want)
+    for c_ in cands:
+        c.load_autotune({key: c_})
+        assert np.array_equal(host.ComputePsfPlanesBatch(reqs, ctx=c), want), f"choice {c_}"
+    c.close()
+
+
 def test_splat_grid_every_candidate_identical(orc):
     """16 frames of 512x512 with Poisson(800) blinks, Catmull-Rom, EMCCD: the persistent grid's workgroups per CU
     (4, 8, 16, 32) and the tile edge (32, or 16: + 64) give the same photons and ADU; frame 0 equals the oracle's."""

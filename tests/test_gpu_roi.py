@@ -7,7 +7,7 @@ import pytest
 import oracle as O
 from insiliscope_amd import configs, host
 from conftest import record_accuracy
-from test_oracle import roi_case
+from test_oracle import ROI_PIN_CASES, rel_l2, roi_case, roi_pin_errors, roi_pin_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -25,6 +25,45 @@ def test_roi_forward_vs_oracle(ctx, orc):
     print("forward max rel L2", err.max(), flush=True)
     record_accuracy("roi_forward_64_rois_vs_fp64_oracle", rel_l2_max=err.max(),
                     max_abs_rel_to_peak=(np.abs(mu - want).max(axis=(1, 2)) / np.abs(want).max(axis=(1, 2))).max())
+    assert err.max() <= 1e-5
+
+
+def hmodel(m):
+    return host.roi_model(**{k: getattr(m, k) for k, _ in m._fields_})
+
+
+@pytest.mark.parametrize("name", ROI_PIN_CASES)
+def test_roi_forward_vs_reference_psf(ctx, golden, name):
+    """iscg_roi_forward is the reference's PSF (ComputeZernikePsfPlanes, Sim/ZernikePsf.cpp:150-334; fixture
+    roi_ref.npz): ROIs whose sub-samples are the reference grid's samples, C5's Zernike set, 15 x 15 pixels at os 4
+    (C5's geometry) and os 1, 5 planes over +-500 nm: sum-1 images within 1e-5 relative L2 of the reference's
+    block sums, and their ratio constant to 1e-5 (relative L2 of the ratio map about its mean)."""
+    G = golden("roi_ref.npz")
+    m, p, crop = roi_pin_inputs(G, name)
+    mu = host.RoiForward(hmodel(m), G["zernike"], p, ctx=ctx).astype(np.float64)
+    e = roi_pin_errors(mu, G[name + "_planes"], crop, m.oversampling)
+    print(name, "sum-1 rel L2 max", e[:, 0].max(), "ratio spread max", e[:, 1].max(), flush=True)
+    record_accuracy("roi_forward_%s_vs_reference_psf" % name, rel_l2_max=e[:, 0].max(), ratio_spread=e[:, 1].max())
+    assert e[:, 0].max() <= 1e-5 and e[:, 1].max() <= 1e-5, e
+
+
+def test_roi_forward_c5_batch_vs_oracle(ctx, orc):
+    """The C5 batch as benchmarked (configs.c5_rois(seed=1): 4096 ROIs of 15 x 15 at os 4, M 64, 21 Zernike terms)
+    in one call: every ROI within 1e-5 relative L2 of the FP64 oracle (which the test above pins to the
+    reference's PSF)."""
+    from concurrent.futures import ThreadPoolExecutor
+    m = host.roi_model()
+    z, prm = configs.c5_rois(seed=1, n=4096)
+    p = np.zeros(len(prm), host.ROI_PARAMS_DTYPE)
+    for k, name in enumerate(p.dtype.names):
+        p[name] = prm[:, k]
+    mu = host.RoiForward(m, z, p, ctx=ctx).astype(np.float64)
+    chunks = np.array_split(np.arange(len(p)), 16)
+    with ThreadPoolExecutor(8) as ex:  # the oracle's ctypes calls release the GIL
+        want = np.concatenate(list(ex.map(lambda i: orc.roi_forward(omodel(m), z, p[i]), chunks)))
+    err = np.sqrt(((mu - want) ** 2).sum(axis=(1, 2)) / (want ** 2).sum(axis=(1, 2)))
+    print("C5 batch forward max rel L2", err.max(), "median", np.median(err), flush=True)
+    record_accuracy("roi_forward_c5_4096_rois_vs_fp64_oracle", rel_l2_max=err.max(), rel_l2_median=np.median(err))
     assert err.max() <= 1e-5
 
 

@@ -121,6 +121,10 @@ def test_fits_and_budget(ctx):
     accel = host.WidefieldGpuHost(ctx)
     assert accel.Fits(91, 512, 512, 100)
     assert not accel.Fits(92, 2048, 2048, 10 ** 7)
+    # grids the passes do not take do not fit (the scene then renders on the CPU instead of failing in Images)
+    for geom, NX, NY in ((94, 2048, 512), (95, 1024, 1024), (96, 96, 64), (97, 8, 64)):
+        assert not accel.Fits(geom, NX, NY, 1), (NX, NY)
+    assert accel.Fits(98, 1024, 512, 1)
     accel.SetBudget(3 * 64 * 48 * 8)  # room for three 64-point plane spectra
     try:
         rng = np.random.default_rng(2)

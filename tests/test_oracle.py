@@ -250,6 +250,24 @@ def test_chirp_z_equals_direct_dft(orc):
         assert np.max(np.abs(E[z] - direct)) < 1e-9 * np.max(np.abs(direct))
 
 
+def test_pupil_samples_vs_reference():
+    """ZernikePupilSamples (Sim/ZernikePsf.cpp:134-148): the oracle's rule and the product's (iscg_psf_geometry)
+    give the reference's M for 400 seeded requests (automatic: the smallest multiple of 4 in [64, 512] covering the
+    window diagonal, raised to fill the chirp-Z length; explicit: clamped to [8, 1024])."""
+    from insiliscope_amd import host
+    orc = O.Oracle()
+    with open(os.path.join(GOLDEN, "pupil_samples_ref.json")) as f:
+        cases = json.load(f)["cases"]
+    auto = 0
+    for c in cases:
+        f = {k: v for k, v in c.items() if k != "M"}
+        r = O.psf_request(**f)
+        assert orc.psf_geometry(r)[2] == c["M"], c
+        assert host.PsfGeometry(f)[2] == c["M"], c
+        auto += f["pupilSamples"] <= 0
+    assert auto > 100 and len({c["M"] for c in cases}) > 20
+
+
 # ---- splat: compiled-reference RenderPhotonImage frames ---------------------------------------------------------
 
 def splat_inputs(G, P):
@@ -541,6 +559,70 @@ def test_roi_adjoint_vs_central_differences(orc):
         b[j] -= h
         fd = (NLL(p, a) - NLL(p, b)) / (2 * h)
         assert abs(gz[j] - fd) <= 1e-4 * abs(fd) + 1e-9, (j, gz[j], fd)
+
+
+ROI_PIN_CASES = ("os4", "os4_depth", "os1", "os1_matched")
+
+
+def roi_pin_inputs(G, name, z_sign=1.0):
+    """The ROI model and parameters whose sub-samples are the samples of the reference's PSF grid in roi_ref.npz
+    (ComputeZernikePsfPlanes, Sim/ZernikePsf.cpp:150-334): one ROI per plane, at the plane's z, photons 1,
+    background 0. Returns (model, params, crop) with crop the grid rows/columns the ROI's sub-samples hit."""
+    os_, hw, depth, ns = G[name + "_case"]
+    os_, hw = int(os_), int(hw)
+    R = 15
+    d = (2 * hw * os_ + 1 - R * os_) // 2
+    nz, zstep = int(G["nz"]), float(G["zstep"])
+    m = O.roi_model(oversampling=os_, sampleDepthNm=float(depth), sampleIndex=float(ns), pupilSamples=64,
+                    roiSize=R)
+    p = np.zeros(nz, O.ROI_PARAMS_DTYPE)
+    p["x"] = p["y"] = (hw * os_ - d + 0.5) / os_
+    p["z"] = z_sign * zstep * (np.arange(nz) - (nz - 1) / 2.0)
+    p["photons"] = 1.0
+    return m, p, slice(d, d + R * os_)
+
+
+def roi_pin_errors(mu, planes, crop, os_):
+    """Per plane: relative L2 of the sum-1 images (the ROI's pixels against the reference samples summed over each
+    pixel's os x os block) and the relative spread of their ratio (the model is norm x |E|^2: a constant)."""
+    out = []
+    for k in range(len(planes)):
+        b = planes[k][crop, crop].astype(np.float64)
+        b = b.reshape(15, os_, 15, os_).sum(axis=(1, 3))
+        a = mu[k] / mu[k].sum()
+        ratio = mu[k] / b
+        out.append((rel_l2(a, b / b.sum()), ratio.std() / ratio.mean()))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("name", ROI_PIN_CASES)
+def test_roi_forward_vs_reference_psf(orc, name):
+    """The ROI forward model is the reference's Gibson-Lanni + Zernike PSF: with the emitter placed so the ROI's
+    sub-samples are the reference grid's samples, each ROI pixel equals the sum of the reference's |E|^2 over its
+    os x os block times one constant (<= 1e-6 relative L2 of the sum-1 images; the reference's planes are float32),
+    for C5's 21 Zernike terms, 5 planes over +-500 nm, with a sample depth and with matched indices."""
+    G = np.load(os.path.join(GOLDEN, "roi_ref.npz"))
+    m, p, crop = roi_pin_inputs(G, name)
+    mu = orc.roi_forward(m, G["zernike"], p)
+    e = roi_pin_errors(mu, G[name + "_planes"], crop, m.oversampling)
+    assert e[:, 0].max() <= 1e-6 and e[:, 1].max() <= 1e-6, e
+
+
+def test_roi_forward_pin_catches_planted_mistakes(orc):
+    """The pin fails for the mistakes it is meant to catch: a flipped defocus sign, the Noll indices used as OSA
+    slots, and a 2 % lateral scale error (a wrong pupil-to-image frequency scale)."""
+    G = np.load(os.path.join(GOLDEN, "roi_ref.npz"))
+    planes, z = G["os4_planes"], G["zernike"]
+    m, p, crop = roi_pin_inputs(G, "os4", z_sign=-1.0)
+    e = roi_pin_errors(orc.roi_forward(m, z, p), planes, crop, 4)
+    assert np.delete(e[:, 0], 2).min() > 1e-2  # every plane but the focal one
+    m, p, crop = roi_pin_inputs(G, "os4")
+    noll = np.zeros(28)
+    for j in range(4, 25):
+        noll[j] = z[configs.NOLL_TO_OSA[j]]
+    assert roi_pin_errors(orc.roi_forward(m, noll, p), planes, crop, 4)[:, 0].min() > 1e-2
+    m.pixelSizeNm = 102.0
+    assert roi_pin_errors(orc.roi_forward(m, z, p), planes, crop, 4)[:, 0].min() > 1e-2
 
 
 def test_roi_forward_normalisation_and_shift(orc):
