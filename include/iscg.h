@@ -143,6 +143,17 @@ int32_t iscg_noise(iscg_context* h, const float* photons, uint32_t width, uint32
                    const float* offset, const float* gain, const float* readNoise, uint32_t seed,
                    const uint32_t* frames, uint32_t nFrames, uint32_t pixelBase, uint16_t* out);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): iscg_noise on device arrays only, allocating nothing. Its
+ * deferred-pixel queue lives in a fixed device array of 256 B + 8 B x ISCG_NOISE_NOALLOC_MAX_PIXELS (32 MiB per
+ * device, held until the process ends); a call takes at most ISCG_NOISE_NOALLOC_MAX_PIXELS pixels (width x height x
+ * nFrames) and fails with an error above that or when any array is host memory. Same ADU as iscg_noise. Calls on one
+ * GPU share the array: calls that overlap on one GPU (other streams, other contexts, other threads) race. */
+#define ISCG_NOISE_NOALLOC_MAX_PIXELS (1u << 22)
+int32_t iscg_noise_noalloc(iscg_context* h, const float* photons, uint32_t width, uint32_t height,
+                           const iscg_camera* cam, const float* offset, const float* gain, const float* readNoise,
+                           uint32_t seed, const uint32_t* frames, uint32_t nFrames, uint32_t pixelBase, uint16_t* out);
+
 /* The camera's static maps and noise seeds for a random seed (host code; no context): NoiseSetup
  * (Sim/ScopeMovie.cpp:973-996) with PixelOffsetMap / PixelGainMap / PixelReadNoiseMap::Generate and GaussianRng
  * (Sim/SMLMNoise.cpp:17-140): one std::mt19937_64(seed), offset then gain then read-noise maps, each over the whole

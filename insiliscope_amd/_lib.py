@@ -140,6 +140,7 @@ SIGNATURES = {
     "iscg_noise_set_path": (I32, [P, I32]),
     "iscg_noise_bounds": (I32, [P, P]),
     "iscg_noise": (I32, [P, P, U32, U32, C.POINTER(Camera), P, P, P, U32, P, U32, U32, P]),
+    "iscg_noise_noalloc": (I32, [P, P, U32, U32, C.POINTER(Camera), P, P, P, U32, P, U32, U32, P]),
     "iscg_splat_set_kernels": (I32, [P, C.POINTER(KernelView), I32]),
     "iscg_splat_set_static": (I32, [P, U32, U32, U32, P, P, P, P, D, U64]),
     "iscg_splat_render_frames": (I32, [P, P, U32, P, U32, P, P, P, C.POINTER(Camera), U32, D, P, P]),

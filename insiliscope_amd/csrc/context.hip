@@ -244,6 +244,10 @@ int32_t iscg_create(int32_t device, iscg_context** out)
       delete h;
       return -1;
    }
+   // the device address of the allocation-free noise entry point's scratch array, looked up once per device with
+   // the device current; a failed look-up only disables that entry point (it reports it)
+   std::string scratchErr;
+   (void)noalloc_setup(device, scratchErr);
    g_createError.clear();
    *out = h;
    return 0;

@@ -486,7 +486,13 @@ bool launch_noise_fallback(iscg_context* h, const NoiseQueue& q, uint16_t* out, 
 
 // Runs the device verification once per device (cached for the process); true when the fast path may be used.
 // bounds (optional, 2 doubles): the verified |g_fast - g| bound and relative threshold bound.
-bool noise_verify_device(iscg_context* h, double* bounds);
+// work: 3 device words for the maxima, or null (then allocated and freed here).
+bool noise_verify_device(iscg_context* h, double* bounds, unsigned long long* work);
+// True once the verification has run on this device in this process.
+bool noise_verified(int device);
+// Looks up, once per device and with that device current, the device address of the allocation-free noise entry
+// point's scratch array (iscg_noise_noalloc). False with err set when the look-up fails.
+bool noalloc_setup(int device, std::string& err);
 // The CamParams of a launch on h: cam_params plus the path (fast unless h asks for the exact chain or the device
 // failed verification). False (error set) when the verification could not run.
 bool cam_params_for(iscg_context* h, const iscg_camera* c, CamParams& out);

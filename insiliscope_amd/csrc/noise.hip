@@ -192,36 +192,50 @@ __global__ __launch_bounds__(256) void verify_exp_kernel(unsigned long long* out
       atomic_max_nonneg(out + 2, m);
 }
 
+__global__ void zero_words_kernel(uint32_t* p, uint32_t n)
+{
+   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+      p[i] = 0u;
+}
+
 namespace {
+constexpr int kMaxDevices = 64;
 std::mutex g_verifyMutex;
 struct Verified
 {
    bool done = false, fastOk = false;
    double gaussBound = 0.0, expBound = 0.0;
 };
-std::map<int, Verified> g_verified;
+Verified g_verified[kMaxDevices];
 }  // namespace
 
-bool noise_verify_device(iscg_context* h, double* bounds)
+bool noise_verified(int device)
 {
    std::lock_guard<std::mutex> lock(g_verifyMutex);
+   return device >= 0 && device < kMaxDevices && g_verified[device].done;
+}
+
+bool noise_verify_device(iscg_context* h, double* bounds, unsigned long long* work)
+{
+   std::lock_guard<std::mutex> lock(g_verifyMutex);
+   if (h->device < 0 || h->device >= kMaxDevices)
+      return false;  // the FP64 chain
    Verified& v = g_verified[h->device];
    if (!v.done)
    {
-      unsigned long long* d = nullptr;
+      unsigned long long* d = work;
       unsigned long long r[3] = {0, 0, 0};
-      if (!ok(h, hipMalloc((void**)&d, sizeof(r)), "noise verification alloc"))
+      if (!d && !ok(h, hipMalloc((void**)&d, sizeof(r)), "noise verification alloc"))
          return false;
-      bool good = ok(h, hipMemsetAsync(d, 0, sizeof(r), h->stream), "noise verification memset");
-      if (good)
-      {
-         hipLaunchKernelGGL(verify_gauss_kernel, dim3(1024), dim3(256), 0, h->stream, d);
-         hipLaunchKernelGGL(verify_exp_kernel, dim3(8192), dim3(256), 0, h->stream, d);
-         good = ok(h, hipGetLastError(), "noise verification launch") &&
-                ok(h, hipMemcpyAsync(r, d, sizeof(r), hipMemcpyDeviceToHost, h->stream), "noise verification copy") &&
-                ok(h, hipStreamSynchronize(h->stream), "noise verification");
-      }
-      (void)hipFree(d);
+      hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, h->stream, (uint32_t*)d, 6u);
+      hipLaunchKernelGGL(verify_gauss_kernel, dim3(1024), dim3(256), 0, h->stream, d);
+      hipLaunchKernelGGL(verify_exp_kernel, dim3(8192), dim3(256), 0, h->stream, d);
+      const bool good =
+         ok(h, hipGetLastError(), "noise verification launch") &&
+         ok(h, hipMemcpyAsync(r, d, sizeof(r), hipMemcpyDeviceToHost, h->stream), "noise verification copy") &&
+         ok(h, hipStreamSynchronize(h->stream), "noise verification");
+      if (!work)
+         (void)hipFree(d);
       if (!good)
          return false;
       double m[3];
@@ -247,11 +261,53 @@ bool cam_params_for(iscg_context* h, const iscg_camera* c, CamParams& out)
    if (h->noiseExact)
       return true;
    const std::string before = h->err;
-   const bool fast = noise_verify_device(h, nullptr);
+   const bool fast = noise_verify_device(h, nullptr, nullptr);
    if (!fast && h->err != before)
       return false;  // the verification itself failed
    out.fast = fast ? 1 : 0;
    return true;
+}
+
+// ---- the allocation-free entry point's scratch (iscg_noise_noalloc) ----------------------------------------------
+//
+// One fixed array per device, loaded with the code object and held until the process ends: the deferred-pixel
+// queue's two counters (words 0 and 1), the verification's three maxima (bytes 64..87), then the queue's items, one
+// per pixel of a call (ISCG_NOISE_NOALLOC_MAX_PIXELS at most). In host code `scratch` names a host-side stand-in, not
+// device memory: the device address is looked up once per device (noalloc_setup) and only that address is used.
+constexpr size_t kNoallocHeader = 256;
+static __device__ __align__(256) unsigned char scratch[kNoallocHeader + (size_t)ISCG_NOISE_NOALLOC_MAX_PIXELS * 8];
+
+namespace {
+std::mutex g_noallocMutex;
+void* g_noallocPtr[kMaxDevices] = {};
+}  // namespace
+
+bool noalloc_setup(int device, std::string& err)
+{
+   std::lock_guard<std::mutex> lock(g_noallocMutex);
+   if (device < 0 || device >= kMaxDevices)
+   {
+      err = "device index beyond the no-alloc table";
+      return false;
+   }
+   if (g_noallocPtr[device])
+      return true;
+   void* p = nullptr;
+   const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // the current device's copy
+   if (e != hipSuccess || !p)
+   {
+      (void)hipGetLastError();
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return false;
+   }
+   g_noallocPtr[device] = p;
+   return true;
+}
+
+static unsigned char* noalloc_ptr(int device)
+{
+   std::lock_guard<std::mutex> lock(g_noallocMutex);
+   return device >= 0 && device < kMaxDevices ? (unsigned char*)g_noallocPtr[device] : nullptr;
 }
 
 }  // namespace iscg
@@ -276,7 +332,7 @@ extern "C" int32_t iscg_noise_bounds(iscg_context* h, double* out)
       return fail(h, "iscg_noise_bounds: out is NULL");
    double b[2] = {0.0, 0.0};
    const std::string before = h->err;
-   const bool fast = noise_verify_device(h, b);
+   const bool fast = noise_verify_device(h, b, nullptr);
    if (h->err != before)
       return -1;
    out[0] = fast && !h->noiseExact ? 1.0 : 0.0;
@@ -313,6 +369,70 @@ extern "C" int32_t iscg_noise(iscg_context* h, const float* photons, uint32_t wi
        !launch_noise(h, dPh, dOut, (uint32_t)npx, nFrames, dFr, dOff, dGain, dRead, cp, seed, pixelBase))
       return -1;
    if (!finish_out(h, out, dOut, nAll * sizeof(uint16_t)))
+      return -1;
+   return 0;
+}
+
+extern "C" int32_t iscg_noise_noalloc(iscg_context* h, const float* photons, uint32_t width, uint32_t height,
+                                      const iscg_camera* cam, const float* offset, const float* gain,
+                                      const float* readNoise, uint32_t seed, const uint32_t* frames, uint32_t nFrames,
+                                      uint32_t pixelBase, uint16_t* out)
+{
+   if (begin(h))
+      return -1;
+   if (!cam || !frames || !out || (!photons && nFrames))
+      return fail(h, "iscg_noise_noalloc: null argument");
+   const size_t npx = (size_t)width * height;
+   if (npx == 0 || nFrames == 0)
+      return 0;
+   const size_t nAll = npx * nFrames;
+   if (nAll > (size_t)ISCG_NOISE_NOALLOC_MAX_PIXELS)
+      return fail(h, "iscg_noise_noalloc: " + std::to_string(nAll) + " pixels in one call; at most " +
+                        std::to_string((size_t)ISCG_NOISE_NOALLOC_MAX_PIXELS) + " (iscg_noise takes any size)");
+   // no staging without allocation: every array must already be device memory
+   if (!is_device_ptr(photons) || !is_device_ptr(frames) || !is_device_ptr(out) ||
+       (offset && !is_device_ptr(offset)) || (gain && !is_device_ptr(gain)) ||
+       (readNoise && !is_device_ptr(readNoise)))
+      return fail(h, "iscg_noise_noalloc: every array must be device memory");
+   unsigned char* base = noalloc_ptr(h->device);
+   if (!base)
+      return fail(h, "iscg_noise_noalloc: no scratch on this device (its look-up failed at iscg_create)");
+   NoiseQueue q;
+   q.count = (uint32_t*)base;
+   q.items = (uint2*)(base + kNoallocHeader);
+   CamParams cp = cam_params(cam);
+   if (!h->noiseExact)
+   {
+      // the FP32 path once this device's bounds are verified; the verification runs here (on the scratch, no
+      // allocation) unless the stream is being captured, where the FP64 chain gives the same ADU
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      const bool capturing = hipStreamIsCapturing(h->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+      (void)hipGetLastError();
+      if (!capturing || noise_verified(h->device))
+      {
+         const std::string before = h->err;
+         const bool fast = noise_verify_device(h, nullptr, (unsigned long long*)(base + 64));
+         if (!fast && h->err != before)
+            return -1;
+         cp.fast = fast ? 1 : 0;
+      }
+   }
+   // the counters start at zero (the fallback's last block also zeroes them; a call that failed between the two
+   // launches would not have)
+   hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, h->stream, q.count, 2u);
+   const unsigned block = 256, grid = grid_for(nAll, block, 16);
+   {
+      ProfScope ps(h, "noise");
+      if (cp.emccd)
+         hipLaunchKernelGGL(noise_kernel<true>, dim3(grid), dim3(block), 0, h->stream, photons, out, (uint32_t)npx,
+                            nFrames, frames, offset, gain, readNoise, cp, seed, pixelBase, q);
+      else
+         hipLaunchKernelGGL(noise_kernel<false>, dim3(grid), dim3(block), 0, h->stream, photons, out, (uint32_t)npx,
+                            nFrames, frames, offset, gain, readNoise, cp, seed, pixelBase, q);
+      if (!ok(h, hipGetLastError(), "noise kernel launch"))
+         return -1;
+   }
+   if (!launch_noise_fallback(h, q, out, (uint32_t)npx, frames, 0, offset, gain, readNoise, cp, seed, pixelBase))
       return -1;
    return 0;
 }
