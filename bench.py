@@ -205,15 +205,33 @@ def cpu_wf(planes):
         return None
     dens = configs.c2_density(seed=1)
     foci = (np.arange(64) + 0.5) * 0.05
+    cam = configs.scmos_c1()
+    oc = O.camera(quantumEfficiency=1.0, gainPhotonsPerAdu=cam.gainPhotonsPerAdu,
+                  readNoiseElectrons=cam.readNoiseElectrons, emccd=0)
+    off = np.full((256, 256), 100.0, np.float32)
+
+    def stack():
+        # the same work as the GPU step: the 64 focus images, then ApplyNoiseChain (sCMOS, offset map) on each,
+        # its rows in parallel as the reference runs it (ScopeMovie.cpp:7642-7650)
+        imgs = ref.wf_focus_series(dens, 50.0, planes, 1, 50.0, foci, kernel_cap_um=12.8)
+        for f in range(len(foci)):
+            ref.noise(imgs[f], oc, 1, f, offset=off, parallel=True)
+
     ref.wf_focus_series(dens, 50.0, planes, 1, 50.0, foci[:2], kernel_cap_um=12.8)
-    reps, el = timed_repeat(lambda: ref.wf_focus_series(dens, 50.0, planes, 1, 50.0, foci, kernel_cap_um=12.8),
-                            CPU_MIN_S)
+    reps, el = timed_repeat(stack, CPU_MIN_S)
     return {"value": round(reps * len(foci) / el, 2), "unit": "frames/s", "cores": CORES, "kind": "reference",
-            "sample": f"WidefieldScene::UpdateFromGrid + FocusSeries of the 256x256x64 density at all 64 foci, "
-                      f"{reps} stacks in {el:.2f} s (kernel radius 128 cells; camera noise not included)"}
+            "sample": f"WidefieldScene::UpdateFromGrid + FocusSeries of the 256x256x64 density at all 64 foci, then "
+                      f"ApplyNoiseChain (sCMOS, offset map, rows in parallel) on each image, {reps} stacks in "
+                      f"{el:.2f} s (kernel radius 128 cells)"}
 
 
 # ---- SMLM DNA-PAINT (C4) ---------------------------------------------------------------------------------------
+
+# VALU wave instructions of one splat_tiles_kernel<32> launch over C4's 2000 frames (SQ_INSTS_VALU per dispatch,
+# rocprofv3 --pmc on bench.py --workload smlm), and the VALU issue peak in G wave-instructions/s
+SPLAT_C4_VALU_INSTS = 14_856_556_313
+VALU_ISSUE_PEAK_G = 256 * 2.4
+
 
 def run_smlm(ctx, steps, warmup, n_frames=2000):
     import torch
@@ -225,31 +243,22 @@ def run_smlm(ctx, steps, warmup, n_frames=2000):
                                         ctx=ctx)[0]
     cache = host.PsfKernelCache(planes, oversampling=4, zStepNm=15.0, interpMode=2)
     ev, st = configs.c4_events(seed=1, n_frames=n_frames)
-    # the frames' emitters collected on the GPU (CollectFrameEmitters for the whole batch); each event is one frame's
-    tc = time.time()
-    ems, est, _ = host.CollectFrameEmittersBatch(ev, st, np.arange(len(ev), dtype=np.uint32),
-                                                 np.arange(n_frames, dtype=np.int64), 512, 512, 100.0, 1.0,
-                                                 psfCache=cache, ctx=ctx)
-    collect_s = time.time() - tc
-    if not np.array_equal(est.astype(np.int64), st.astype(np.int64)):
-        raise RuntimeError("collect: every C4 event lies within its frame")
-    d_ems = torch.from_numpy(ems.view(np.uint8).reshape(len(ems), host.FRAME_EMITTER_DTYPE.itemsize)).cuda()
-    d_start = torch.from_numpy(st.astype(np.int32)).cuda()
-    d_frames = torch.arange(n_frames, dtype=torch.int32, device="cuda")
     w = h = 512
-    off = np.full((h, w), 100.0, np.float32)
-    sp = host.SplatGpuHost(ctx)
-    if not (sp.SetKernels([cache]) and sp.SetStatic(w, h, offset=off, flatBackground=50.0)):
-        raise RuntimeError(sp.err)
+    off = torch.full((h, w), 100.0, dtype=torch.float32, device="cuda")
+    # the blink events resident in HBM and bucketed by frame once; each step collects every frame's emitters on the
+    # device (as the reference's RenderPhotonImage does per frame), then splats and noises the 2000 frames
+    tc = time.time()
+    movie = host.SmlmMovieGpu(ev, n_frames, w, h, 100.0, 1.0, 50.0, offset=off, ctx=ctx)
+    bucket_s = time.time() - tc
     adu = torch.empty((n_frames, h, w), dtype=torch.int16, device="cuda")
     cam = configs.emccd_c4()
     ctx.synchronize()
-    log(f"smlm setup {time.time() - t0:.1f} s ({len(ev)} blinks over {n_frames} frames; GPU collect of the events "
-        f"{collect_s * 1e3:.1f} ms with host staging)")
+    log(f"smlm setup {time.time() - t0:.1f} s ({len(ev)} blinks over {n_frames} frames; events uploaded and "
+        f"bucketed on the GPU in {bucket_s * 1e3:.1f} ms)")
 
     def step():
-        if not sp.RenderBatch(d_ems, d_start, d_frames, cam, 1, outAdu=adu):
-            raise RuntimeError(sp.err)
+        if not movie.RenderFrames(cache, cam, 1, outAdu=adu):
+            raise RuntimeError(movie.splat.err)
 
     for _ in range(warmup):
         step()
@@ -273,6 +282,16 @@ def run_smlm(ctx, steps, warmup, n_frames=2000):
                                  "loop and the EMCCD Gamma per pixel), not by HBM"},
             "splat_noise_kernel": {"avg_ms": round(t_ms, 4), "GB_s": round(b / (t_ms * 1e-3) / 1e9, 1),
                                    "bound": "fp64 valu (counter RNG + Knuth/Gamma noise)"},
+            # the kernel's real roof: VALU issue. Work = the VALU wave instructions one launch over these 2000
+            # frames issues (SQ_INSTS_VALU, rocprofv3 --pmc; the inputs are seeded, so the count is fixed), over the
+            # launch time measured here; peak = one wave instruction per CU per cycle (4 SIMDs, 4 cycles per wave64
+            # instruction) x 256 CUs x 2.4 GHz
+            "valu_roofline": {"bound": "valu", "instructions": SPLAT_C4_VALU_INSTS,
+                              "achieved": round(SPLAT_C4_VALU_INSTS / (t_ms * 1e-3) / 1e9, 1),
+                              "peak": VALU_ISSUE_PEAK_G, "unit": "G wave-instructions/s",
+                              "frac": round(SPLAT_C4_VALU_INSTS / (t_ms * 1e-3) / 1e9 / VALU_ISSUE_PEAK_G, 4),
+                              "lane_instructions_per_pixel": round(SPLAT_C4_VALU_INSTS * 64 / (n_frames * w * h),
+                                                                   1)},
             "_events": (ev, st, np.asarray(planes.cpu()))}
 
 
@@ -500,20 +519,28 @@ def run_c5(ctx, steps, warmup):
             step()
     ms = t.ms / steps
     prof = profiled(ctx, lambda k: step(), min(steps, 3))
-    P = 15 * 4
-    flops = len(p) * 3 * 8.0 * P * 64 * (64 + P)  # forward 8 P M (M + P), adjoint about twice that
+    # the work roi_kernel<true> executes per ROI: the forward field recomputed (2 complex matrix products) and the
+    # adjoint (2 more), each a 64^3 complex product (M 64; the 60 sub-samples per axis padded to 64) = 4 real
+    # 64^3 products on FP32 MFMA: 4 x 8 x 64^3 flops. rocprofv3 counts the same 3.44e10 FP32 MFMA flops per
+    # 4096-ROI dispatch (SQ_INSTS_VALU_MFMA_MOPS_F32 x 512).
+    P, M = 15 * 4, 64
+    executed = len(p) * 4 * 8.0 * M ** 3
+    algorithmic = len(p) * 2 * 8.0 * P * M * (M + P)  # what the adjoint needs: 2 x 8 P M (M + P), P unpadded
     k = prof.get("roi_backward", (1, ms))
-    kms = k[1] / k[0]
+    kms = k[1] / k[0]  # roi_kernel<true> alone (the reductions are timed as roi_reduce)
+    tf = executed / (kms * 1e-3) / 1e12
     return {"value": len(p) * 1000.0 / ms, "unit": "ROI gradients/s", "ms_per_step": ms,
             "config": {"workload": "C5 adjoint: 4096 ROIs 15x15, Gibson-Lanni + 21 Zernike (Noll 4-24), pupil M 64, "
                                    "os 4, Poisson NLL gradient wrt x, y, z, N, b and the shared coefficients"},
-            "roofline": {"bound": "mfma", "achieved": round(flops / (kms * 1e-3) / 1e12, 2), "peak": FP32_PEAK_TFS,
-                         "unit": "TFLOP/s", "frac": round(flops / (kms * 1e-3) / 1e12 / FP32_PEAK_TFS, 4),
+            "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": FP32_PEAK_TFS,
+                         "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFS, 4),
                          "traffic": None, "kernel": "roi_backward",
-                         "note": "algorithmic FP32 flops of the forward and adjoint matrix DFTs; FP32 MFMA peak"},
-            "roi_kernel": {"avg_ms": round(kms, 4), "TFLOP_s_fp32": round(flops / (kms * 1e-3) / 1e12, 2),
-                           "frac_of_fp32_mfma_peak": round(flops / (kms * 1e-3) / 1e12 / FP32_PEAK_TFS, 4),
-                           "flops": "algorithmic (P = 60 samples per axis); executed on 64-padded FP32 MFMA"},
+                         "note": "executed FP32 MFMA flops of roi_kernel<true> (4 complex 64^3 products per ROI: "
+                                 "the forward recomputed and the adjoint) over its own launch time; FP32 MFMA peak"},
+            "roi_kernel": {"avg_ms": round(kms, 4), "TFLOP_s_fp32_executed": round(tf, 2),
+                           "TFLOP_s_fp32_algorithmic": round(algorithmic / (kms * 1e-3) / 1e12, 2),
+                           "flops_executed_per_roi": 4 * 8 * M ** 3,
+                           "flops_algorithmic_per_roi": 2 * 8 * P * M * (M + P)},
             "kernels": {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4)} for k, v in prof.items()},
             "_case": (z, prm, data.cpu().numpy())}
 

@@ -6,6 +6,17 @@
 //
 // Coverage: sin/cos restate the argument reductions for |x| <= 2^19 pi/2 (823,549.6); larger finite arguments
 // return NaN (the Payne-Hanek branch is not restated; the schedules only take cos(2 pi u), u < 1).
+//
+// The algorithms and coefficients restated here are fdlibm's, which carries this notice:
+// ====================================================
+// Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//
+// Developed at SunSoft, a Sun Microsystems, Inc. business.
+// Permission to use, copy, modify, and distribute this
+// software is freely granted, provided that this notice
+// is preserved.
+// ====================================================
+// V8's form of them (src/base/ieee754.cc): Copyright 2016 the V8 project authors. All rights reserved.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -543,6 +543,9 @@ int32_t iscg_roi_backward(iscg_context* h, const iscg_roi_model* model, const do
       hipLaunchKernelGGL((roi_kernel<true>), dim3((unsigned)nWg), dim3(kRoiThreads), 0, h->stream, rc, dP, nRoi,
                          s.A.as<const double>(), s.B.as<const double>(), s.kx.as<const double>(),
                          s.W.as<const float2>(), dD, (float*)nullptr, dG, s.partial.as<double>());
+   }
+   {
+      ProfScope ps(h, "roi_reduce");
       hipLaunchKernelGGL(roi_map_reduce_kernel, dim3((kMapStride + 255) / 256), dim3(256), 0, h->stream,
                          s.partial.as<const double>(), nWg, total);
       hipLaunchKernelGGL(roi_zernike_kernel, dim3(28), dim3(256), 0, h->stream, (const double*)total,

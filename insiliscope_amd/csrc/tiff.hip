@@ -39,8 +39,8 @@ constexpr uint64_t kClassicEnd = 0xFFFFFFFFull;
 void fail_io(iscg_tiff* t, const char* what)
 {
    const int e = errno;
-   t->err = "cannot write " + t->path + " (" + what + ", page " + std::to_string(t->pages.size() + 1) + ", " +
-            std::to_string(t->end / 1000000) + " MB written): " + (e ? std::strerror(e) : "write error");
+   t->err = "TIFF " + t->path + ": " + what + " failed at page index " + std::to_string(t->pages.size()) +
+            " after " + std::to_string(t->end) + " bytes (" + (e ? std::strerror(e) : "I/O error") + ")";
 }
 
 bool put(iscg_tiff* t, const void* p, size_t n)
@@ -49,7 +49,7 @@ bool put(iscg_tiff* t, const void* p, size_t n)
       return false;
    if (n && std::fwrite(p, 1, n, t->f) != n)
    {
-      fail_io(t, "writing");
+      fail_io(t, "write");
       return false;
    }
    t->end += n;
@@ -60,7 +60,7 @@ bool seek(iscg_tiff* t, uint64_t at)
 {
    if (fseeko(t->f, (off_t)at, SEEK_SET) != 0)
    {
-      fail_io(t, "seeking");
+      fail_io(t, "seek");
       return false;
    }
    return true;
@@ -86,9 +86,9 @@ int32_t write_page(iscg_tiff* t, const void* px, uint32_t bps, uint16_t sampleFo
    const uint32_t descLen = dlen ? (uint32_t)dlen + 1 : 0;
    if (!t->big && t->end + pageBytes + descLen + 2 + 2 + 11 * 12 + 4 > kClassicEnd)
    {
-      t->err = "cannot write " + t->path + ": page " + std::to_string(t->pages.size() + 1) +
-               " would pass the 4 GB of a classic TIFF (the file was expected to be smaller; insiliscope_cli "
-               "--bigtiff 1 writes a BigTIFF)";
+      t->err = "TIFF " + t->path + ": page index " + std::to_string(t->pages.size()) +
+               " ends beyond the 32-bit offsets of a classic TIFF; open the writer with bigTiff = 1 (or a larger "
+               "expectedBytes) for files of this size";
       return -1;
    }
    errno = 0;
@@ -162,12 +162,12 @@ int32_t write_page(iscg_tiff* t, const void* px, uint32_t bps, uint16_t sampleFo
    t->end = keep;
    if (std::fseek(t->f, 0, SEEK_END) != 0)
    {
-      fail_io(t, "seeking");
+      fail_io(t, "seek");
       return -1;
    }
    if (std::fflush(t->f) != 0 || std::ferror(t->f))
    {
-      fail_io(t, "writing");
+      fail_io(t, "write");
       return -1;
    }
    t->link = linkPos;
@@ -194,7 +194,7 @@ int32_t iscg_tiff_open(const char* path, uint64_t expectedBytes, int32_t bigTiff
    t->f = path ? std::fopen(path, "wb") : nullptr;
    if (!t->f)
    {
-      t->err = "cannot open " + t->path + " for writing: " + std::strerror(errno);
+      t->err = "TIFF " + t->path + ": open for writing failed (" + std::strerror(errno) + ")";
       return -1;
    }
    const uint8_t classic[8] = {'I', 'I', 42, 0, 0, 0, 0, 0};
@@ -225,8 +225,8 @@ int32_t iscg_tiff_rows_u16(iscg_tiff* t, uint64_t page, uint32_t y0, uint32_t ro
       return -1;
    if (page >= t->pages.size() || (uint64_t)y0 + rows > t->pages[page].h)
    {
-      t->err = "cannot write " + t->path + ": rows " + std::to_string(y0) + ".." + std::to_string(y0 + rows) +
-               " of page " + std::to_string(page + 1) + " are not in it";
+      t->err = "TIFF " + t->path + ": rows [" + std::to_string(y0) + ", " + std::to_string(y0 + rows) +
+               ") lie outside page index " + std::to_string(page) + " (or the page does not exist)";
       return -1;
    }
    const uint64_t keep = t->end, row = (uint64_t)t->pages[page].w * 2;
@@ -238,7 +238,7 @@ int32_t iscg_tiff_rows_u16(iscg_tiff* t, uint64_t page, uint32_t y0, uint32_t ro
       return -1;
    if (std::fflush(t->f) != 0 || std::ferror(t->f))
    {
-      fail_io(t, "writing");
+      fail_io(t, "write");
       return -1;
    }
    return 0;
@@ -261,7 +261,8 @@ int32_t iscg_tiff_frames_u16(iscg_tiff* t, const uint16_t* frames, uint32_t nFra
          if (e != hipSuccess)
          {
             t->err =
-               std::string("cannot read frame ") + std::to_string(f) + " from the device: " + hipGetErrorString(e);
+               std::string("TIFF: device-to-host copy of frame ") + std::to_string(f) + " failed (" +
+               hipGetErrorString(e) + ")";
             return -1;
          }
          src = host.data();
@@ -297,7 +298,7 @@ int32_t iscg_tiff_close(iscg_tiff* t)
       const bool closed = std::fclose(t->f) == 0;
       t->f = nullptr;
       if (!closed && t->err.empty())
-         fail_io(t, "closing it");
+         fail_io(t, "close");
    }
    return t->err.empty() ? 0 : -1;
 }

@@ -804,18 +804,24 @@ class SmlmMovieGpu:
             self.planes = torch.empty((1, nz, size, size), dtype=torch.float32, device="cuda")
         ComputePsfPlanesBatch(self._reqArr, normalize=True, out=self.planes, ctx=self.ctx)
         self._serial += 1
-        self.cache = PsfKernelCache(self.planes[0], oversampling=oversampling, zStepNm=zStepNm,
-                                    interpMode=interpMode)
-        self.cache.serial = self._serial  # a new stack: its block sums are made again
+        cache = PsfKernelCache(self.planes[0], oversampling=oversampling, zStepNm=zStepNm, interpMode=interpMode)
+        cache.serial = self._serial  # a new stack: its block sums are made again
+        return self.RenderFrames(cache, cam, seed, outAdu, outPhotons)
+
+    def RenderFrames(self, cache: PsfKernelCache, cam: CameraNoiseParams | None, seed: int, outAdu=None,
+                     outPhotons=None) -> bool:
+        """The movie on a given kernel stack: every frame's emitters collected from the events, then the splat and
+        the noise of all frames (no PSF computation)."""
+        self.cache = cache
         ctx = self.ctx
         ctx.follow_torch()
         ctx.check(ctx.L.iscg_collect_frame_emitters(
             ctx.h, self.d_ev.data_ptr(), self.nEvents, self.d_bst.data_ptr(), self.d_bidx.data_ptr(),
             self.d_fi.data_ptr(), self.nFrames, self.width, self.height, self.pixelSizeNm, self.photonsPerBlink, None,
-            self.cache.nz, float(zStepNm), 0.0, 0, None, self.d_ems.data_ptr(), self.d_ems.shape[0],
+            cache.nz, float(cache.zStepNm), 0.0, 0, None, self.d_ems.data_ptr(), self.d_ems.shape[0],
             self.d_start.data_ptr(), None), "iscg_collect_frame_emitters")
         n = int(self.d_start[-1].item())
-        return self.splat.SetKernels([self.cache]) and self.splat.RenderBatch(
+        return self.splat.SetKernels([cache]) and self.splat.RenderBatch(
             self.d_ems[:n], self.d_start, self.d_frames, cam, seed, outAdu=outAdu, outPhotons=outPhotons)
 
     def Emitters(self):

@@ -1713,6 +1713,18 @@ void orc_binned_render(const float* kernels, int32_t nz, int32_t u, int32_t R, i
 /* Only the pieces the dye schedules call: log, exp, and sin/cos with the  */
 /* argument reductions up to 2^19 pi/2.                                    */
 /* ======================================================================= */
+/*
+ * The algorithms and coefficients restated here are fdlibm's, which carries this notice:
+ * ====================================================
+ * Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+ *
+ * Developed at SunSoft, a Sun Microsystems, Inc. business.
+ * Permission to use, copy, modify, and distribute this
+ * software is freely granted, provided that this notice
+ * is preserved.
+ * ====================================================
+ * V8's form of them (src/base/ieee754.cc): Copyright 2016 the V8 project authors. All rights reserved.
+ */
 
 static int32_t hi_of(double x)
 {

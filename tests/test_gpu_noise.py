@@ -125,3 +125,45 @@ def test_tiff_frames_from_device_memory(ctx, tmp_path):
     b = host.TiffWriter(str(tmp_path / "host.tif"))
     assert b.Frames(adu.cpu().numpy().view(np.uint16), "gpu frames") and b.Close(), b.error()
     assert open(tmp_path / "dev.tif", "rb").read() == open(tmp_path / "host.tif", "rb").read()
+
+
+def test_noise_setup_maps_feed_gpu_noise(ctx, orc, golden):
+    """NoiseSetup's pixel maps (iscg_noise_setup: Pixel*Map::Generate, Sim/SMLMNoise.cpp:89-140, equal to the
+    compiled reference's maps in maps_ref.npz) as the GPU noise's inputs (Sim/ScopeMovie.cpp:973-996 feeds them to
+    ApplyNoiseChain): a whole sensor (sCMOS, offset + gain + read-noise maps) and a row window (EMCCD, pixelBase of
+    the window), through iscg_noise and through the splat's fused noise, ADU bit-exact against the oracle's chain
+    on the same maps."""
+    G = golden("maps_ref.npz")
+    rng = np.random.default_rng(23)
+    cases = [
+        (host.NoiseSetup.Make(1234567, 48, 40, 100.0, 2.5, 2.0, 0.05, 1.6, 0.2), G["maps_full"],
+         host.CameraNoiseParams(1.0, 0.0, 2.0, 1.6, False, 300.0, 0.002, 16)),
+        (host.NoiseSetup.Make(-77, 48, 12, 100.0, 2.5, 1.0 / 300.0, 3.0, 50.0, 1.5, sensorHeight=40, rowY0=17),
+         G["maps_window"], host.CameraNoiseParams(1.0, 0.0, 1.0 / 300.0, 50.0, True, 300.0, 0.002, 16)),
+    ]
+    for s, want_maps, cam in cases:
+        for got, want in zip((s.offsetMap, s.gainMap, s.readNoiseMap), want_maps):
+            assert np.array_equal(got, want)
+        h, w = s.offsetMap.shape
+        print("maps", (h, w), "pixelBase", s.pixelBase, "emccd", cam.emccd, flush=True)
+        ocam = O.camera(quantumEfficiency=cam.quantumEfficiency, gainPhotonsPerAdu=cam.gainPhotonsPerAdu,
+                        readNoiseElectrons=cam.readNoiseElectrons, emccd=int(cam.emccd), emGain=cam.emGain,
+                        cicElectrons=cam.cicElectrons)
+        ph = (rng.gamma(0.8, 60.0, (3, h, w))).astype(np.float32)
+        frames = np.array([0, 5, 9], np.uint32)
+        adu = host.ApplyNoiseChainFrames(ph, cam, s.noiseSeed, frames, s.offsetMap, s.gainMap, s.readNoiseMap,
+                                         pixelBase=s.pixelBase, ctx=ctx)
+        for k, f in enumerate(frames):
+            want = orc.noise(ph[k], ocam, s.noiseSeed, int(f), offset=s.offsetMap, gain=s.gainMap,
+                             read=s.readNoiseMap, pixel_base=s.pixelBase)
+            assert np.array_equal(adu[k], want), f"iscg_noise frame {f}: {(adu[k] != want).sum()} ADU differ"
+        # the splat's fused noise with the same maps: an empty frame's flat background, noised
+        sp = host.SplatGpuHost(ctx)
+        assert sp.SetStatic(w, h, pixelBase=s.pixelBase, offset=s.offsetMap, gain=s.gainMap,
+                            readNoise=s.readNoiseMap, flatBackground=33.0), sp.err
+        out = np.zeros((1, h, w), np.uint16)
+        assert sp.RenderBatch(np.zeros(0, host.FRAME_EMITTER_DTYPE), np.zeros(2, np.uint32),
+                              np.array([4], np.uint32), cam, s.noiseSeed, outAdu=out), sp.err
+        want = orc.noise(np.full((h, w), 33.0, np.float32), ocam, s.noiseSeed, 4, offset=s.offsetMap,
+                         gain=s.gainMap, read=s.readNoiseMap, pixel_base=s.pixelBase)
+        assert np.array_equal(out[0], want), f"fused noise: {(out[0] != want).sum()} ADU differ"
