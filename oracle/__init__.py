@@ -20,10 +20,11 @@ REFERENCE_ROOT = "/root/reference"
 
 
 def build(ref: bool = True) -> None:
-    """Compile liboracle.so, and libiscref.so when the reference checkout is present."""
+    """Compile liboracle.so, and libiscref.so and libiscbridge.so (which links libiscg.so: build it first) when the
+    reference checkout is present."""
     subprocess.run(["make", "-C", HERE, "oracle"], check=True, stdout=subprocess.DEVNULL)
     if ref and os.path.isdir(REFERENCE_ROOT):
-        subprocess.run(["make", "-C", HERE, "-j8", "ref", f"REF={REFERENCE_ROOT}"], check=True,
+        subprocess.run(["make", "-C", HERE, "-j8", "ref", "bridge", f"REF={REFERENCE_ROOT}"], check=True,
                        stdout=subprocess.DEVNULL)
 
 
@@ -821,6 +822,79 @@ class Reference:
 
 
 # ---- WidefieldGpuJob (Sim/WidefieldRender.h:306-340): parsing captured jobs and the CPU statement of the hosts --
+
+BRIDGE_PATH = os.path.join(HERE, "_ref", "libiscbridge.so")
+
+
+class Bridge:
+    """The reference's engine with this build's link-time drop-ins (bridge_check.cpp over libiscg.so)."""
+
+    def __init__(self, path: str = BRIDGE_PATH):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = C.CDLL(path)
+        self.L = L
+        D, I32, U32, I64 = C.c_double, C.c_int32, C.c_uint32, C.c_int64
+        PD, PF, PU16 = C.POINTER(D), C.POINTER(C.c_float), C.POINTER(C.c_uint16)
+        L.bridge_error.restype = C.c_char_p
+        L.bridge_splat_frames.argtypes = [I32, PD, I32, I32, U32, U32, D, D, D, PF, I32, I32, I32, D, I32, PD, D, U32,
+                                          PU16, PU16]
+        L.bridge_splat_frames.restype = I32
+        L.bridge_movie.argtypes = [C.POINTER(C.c_char_p), PD, I32, I32, PU16, C.c_uint64, C.POINTER(I64),
+                                   C.c_char_p, I32]
+        L.bridge_movie.restype = I64
+        L.bridge_wf_scene.argtypes = [PF, I32, I32, I32, D, PF, I32, I32, I32, D, D, PD, I32, D, D, PF, PF,
+                                      C.POINTER(I32)]
+        L.bridge_wf_scene.restype = I32
+
+    def error(self) -> str:
+        return self.L.bridge_error().decode()
+
+    def splat_frames(self, via, events, n_frames, w, h, pixel_nm, per_blink, bg, planes, os_, z_step_nm, interp,
+                     cam8, offset, seed):
+        ev = np.ascontiguousarray(np.stack([events[k] for k in EVENT_DTYPE.names], axis=1), np.float64)
+        planes = np.ascontiguousarray(planes, np.float32)
+        cam = np.ascontiguousarray(cam8, np.float64)
+        gpu = np.zeros((n_frames, h, w), np.uint16)
+        cpu = np.zeros((n_frames, h, w), np.uint16)
+        rc = self.L.bridge_splat_frames(via, _p(ev, C.c_double), len(ev), n_frames, w, h, pixel_nm, per_blink, bg,
+                                        _p(planes, C.c_float), planes.shape[0], planes.shape[1], os_, z_step_nm,
+                                        interp, _p(cam, C.c_double), offset, seed, _p(gpu, C.c_uint16),
+                                        _p(cpu, C.c_uint16))
+        if rc != 0:
+            raise RuntimeError(self.error())
+        return gpu, cpu
+
+    def movie(self, options: dict, gpu: bool, max_pixels: int):
+        names = (C.c_char_p * len(options))(*[k.encode() for k in options])
+        vals = np.ascontiguousarray(list(options.values()), np.float64)
+        out = np.zeros(max_pixels, np.uint16)
+        info = (C.c_int64 * 4)()
+        text = C.create_string_buffer(256)
+        n = self.L.bridge_movie(names, _p(vals, C.c_double), len(options), int(gpu), _p(out, C.c_uint16), max_pixels,
+                                info, text, 256)
+        if n < 0:
+            raise RuntimeError(self.error())
+        w, h = int(info[0]), int(info[1])
+        return out[: n * w * h].reshape(n, h, w), {"frames": int(info[2]), "blinks": int(info[3]),
+                                                   "host": text.value.decode()}
+
+    def wf_scene(self, density, dz_nm, planes, os_, z_step_nm, foci_um, kernel_cap_um=12.8, dx_um=0.0, dy_um=0.0):
+        density = np.ascontiguousarray(density, np.float32)
+        planes = np.ascontiguousarray(planes, np.float32)
+        foci = np.ascontiguousarray(foci_um, np.float64)
+        nz, ny, nx = density.shape
+        gpu = np.zeros((len(foci), ny, nx), np.float32)
+        cpu = np.zeros_like(gpu)
+        used = C.c_int32(0)
+        rc = self.L.bridge_wf_scene(_p(density, C.c_float), nx, ny, nz, dz_nm, _p(planes, C.c_float),
+                                    planes.shape[0], planes.shape[1], os_, z_step_nm, kernel_cap_um,
+                                    _p(foci, C.c_double), len(foci), dx_um, dy_um, _p(gpu, C.c_float),
+                                    _p(cpu, C.c_float), C.byref(used))
+        if rc != 0:
+            raise RuntimeError(self.error())
+        return gpu, cpu, bool(used.value)
+
 
 def parse_wf_jobs(blob: bytes, n: int) -> list:
     """Jobs in ref_wf_capture_jobs's byte layout -> namespaces with WidefieldGpuJob's fields (kernels [(p, spec)],

@@ -266,3 +266,24 @@ def test_tiff_writer_bytes_identical_to_reference(tmp_path):
     w2 = host.TiffWriter(str(tmp_path / "pages.tif"))
     assert all(w2.Page(fr, 10, 6, "movie" if i == 0 else "") for i, fr in enumerate(frames)) and w2.Close()
     assert open(tmp_path / "stack.tif", "rb").read() == open(tmp_path / "pages.tif", "rb").read()
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libiscbridge.so")),
+                    reason="libiscbridge.so not built (needs the reference checkout at build time)")
+def test_engine_links_the_dropins():
+    """The reference's engine built with this build's drop-ins: sim::SplatGpuHost's and sim::GpuSimulator's methods
+    are defined by insiliscope_amd/integration (SplatGpuIscg.cpp, GpuSimIscg.cpp) and call libiscg's C ABI, which is
+    the only thing the library leaves undefined; a short CPU movie renders through the engine."""
+    import oracle as O
+    path = O.BRIDGE_PATH
+    defined = subprocess.run(["nm", "-DC", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    for sym in ("sim::SplatGpuHost::RenderFrames", "sim::SplatGpuHost::SetKernels", "sim::MakeSplatGpuHost",
+                "sim::GpuSimulator::Create", "sim::GpuSimulator::RenderFrames", "sim::RenderScopeMovie"):
+        assert sym in defined, sym
+    undefined = subprocess.run(["nm", "-D", "--undefined-only", path], capture_output=True, text=True,
+                               check=True).stdout
+    iscg = set(re.findall(r"\bU (iscg_\w+)", undefined))
+    assert {"iscg_splat_render_records", "iscg_splat_set_kernels", "iscg_wf_images"} <= iscg
+    adu, info = O.Bridge().movie({"size": 48, "frames": 2, "world-seed": 1, "seed": 1, "disk-cache": 0}, False,
+                                 48 * 48 * 4)
+    assert adu.shape == (2, 48, 48) and info["host"] == "cpu" and adu.min() > 0
