@@ -401,7 +401,7 @@ typedef struct
  * values may be host or device memory. A plane's cells are distinct (the scene's sparse planes are). */
 typedef struct
 {
-   uint32_t NX, NY;               /* FFT grid: powers of two, NX in [16, 1024], NY in [16, 512] */
+   uint32_t NX, NY;               /* FFT grid: powers of two in [16, 2048] (WidefieldGpuJob::kMaxGpuFft) */
    uint32_t nx, ny;               /* dye grid (<= NX, NY); cell = y * nx + x */
    uint32_t fovX0, fovY0, cw, ch; /* the FOV cells in the grid (fovX0 + cw <= NX, fovY0 + ch <= NY) */
    double fracX, fracY;           /* sub-cell shift */

@@ -344,6 +344,155 @@ __global__ __launch_bounds__(kPsfThreads) void psf_planes_kernel(const PsfReqDev
    }
 }
 
+// Pupils above 256 samples per axis (the reference takes up to 1024, Sim/ZernikePsf.cpp:134-148): the pupil's
+// columns in chunks of kBigMC, so LDS holds one chunk of the pupil (kBigMC x 32 per step, split) and of mid
+// (YB 16 x kBigMC); E = sum over chunks of mid_chunk W_chunk^T stays in registers across the chunks (PER2 tiles of
+// 16 x 16 per wave). One workgroup per (request, plane, block of YB 16 output rows); the pupil chunk is evaluated
+// again for every row block, as the smaller kernel does. Same split products, so the same accuracy.
+constexpr int kBigMC = 128;
+
+template <int YB, int PER2>
+__global__ __launch_bounds__(kPsfThreads) void psf_planes_big_kernel(const PsfReqDev* __restrict__ reqs,
+                                                                     const double* __restrict__ A,
+                                                                     const double* __restrict__ B,
+                                                                     const _Float16* __restrict__ W,
+                                                                     const double* __restrict__ tz, int nz, int Mp,
+                                                                     int Pp, int nYBlocks, float* __restrict__ out)
+{
+   constexpr int MC = kBigMC, CT1 = MC / 16, T1 = YB * CT1;
+   constexpr int PER1 = (T1 + kPsfWaves - 1) / kPsfWaves;
+   constexpr int MS = MC + 8;
+   __shared__ __attribute__((aligned(16))) _Float16 pup[4][MC][kKP];      // pupil chunk, [part][m][k]
+   __shared__ __attribute__((aligned(16))) _Float16 mid[4][YB * 16][MS];  // mid chunk, [part][y][m]
+
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int item = blockIdx.x;
+   const int yb = item % nYBlocks;
+   const int zi = (item / nYBlocks) % nz;
+   const int r = item / (nYBlocks * nz);
+   const PsfReqDev& q = reqs[r];
+   const int P = q.size;
+   const int y0 = yb * YB * 16;
+   const size_t wPer = (size_t)Pp * Mp;
+   const _Float16* Wr = W + (size_t)r * 4 * wPer;
+   const double* Ar = A + (size_t)r * Mp * Mp;
+   const double* Br = B + (size_t)r * Mp * Mp;
+   const double t = tz[(size_t)r * nz + zi];
+   const int lr = lane & 15, lk = 8 * (lane >> 4);
+   const int CT2 = Pp / 16, T2 = YB * CT2;  // T2 <= 8 PER2 (the host's choice of PER2)
+
+   float4v eR[PER2], eI[PER2];
+#pragma unroll
+   for (int s = 0; s < PER2; ++s)
+      eR[s] = eI[s] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+   for (int mc0 = 0; mc0 < Mp; mc0 += MC)
+   {
+      const int mcn = Mp - mc0 < MC ? Mp - mc0 : MC;  // columns of this chunk (a multiple of 32)
+      // ---- GEMM 1 on the chunk: mid[y][m] = sum_n W[y0 + y][n] Pup[n][mc0 + m] ----
+      float4v accR[PER1], accI[PER1];
+#pragma unroll
+      for (int s = 0; s < PER1; ++s)
+         accR[s] = accI[s] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int n0 = 0; n0 < Mp; n0 += 32)
+      {
+         for (int i = tid; i < 32 * MC; i += kPsfThreads)
+         {
+            const int k = i / MC, m = i % MC;
+            float s = 0.0f, c = 0.0f;
+            if (m < mcn)
+            {
+               const size_t o = (size_t)(n0 + k) * Mp + mc0 + m;
+               const double a = Ar[o];
+               if (a == a)  // inside the aperture
+                  wrapped_sincos(a + t * Br[o], &s, &c);
+            }
+            split_store(c, &pup[0][m][k], &pup[1][m][k]);
+            split_store(s, &pup[2][m][k], &pup[3][m][k]);
+         }
+         __syncthreads();
+#pragma unroll
+         for (int s = 0; s < PER1; ++s)
+         {
+            const int ti = wave + kPsfWaves * s;
+            const int rb = ti / CT1, cb = ti % CT1;
+            if (ti < T1 && y0 + rb * 16 < Pp && cb * 16 < mcn)
+            {
+               const size_t wo = (size_t)(y0 + rb * 16 + lr) * Mp + n0 + lk;
+               const half8 arh = ld8(Wr + wo), arl = ld8(Wr + wPer + wo);
+               const half8 aih = ld8(Wr + 2 * wPer + wo), ail = ld8(Wr + 3 * wPer + wo);
+               const int mc = cb * 16 + lr;
+               const half8 brh = ld8(&pup[0][mc][lk]), brl = ld8(&pup[1][mc][lk]);
+               const half8 bih = ld8(&pup[2][mc][lk]), bil = ld8(&pup[3][mc][lk]);
+               accR[s] = mfma3(arh, arl, brh, brl, accR[s]);
+               accR[s] = mfma3(-aih, -ail, bih, bil, accR[s]);
+               accI[s] = mfma3(arh, arl, bih, bil, accI[s]);
+               accI[s] = mfma3(aih, ail, brh, brl, accI[s]);
+            }
+         }
+         __syncthreads();
+      }
+#pragma unroll
+      for (int s = 0; s < PER1; ++s)
+      {
+         const int ti = wave + kPsfWaves * s;
+         if (ti < T1)
+         {
+            const int rb = ti / CT1, cb = ti % CT1;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+            {
+               const int y = rb * 16 + 4 * (lane >> 4) + i, m = cb * 16 + lr;
+               split_store(accR[s][i], &mid[0][y][m], &mid[1][y][m]);
+               split_store(accI[s][i], &mid[2][y][m], &mid[3][y][m]);
+            }
+         }
+      }
+      __syncthreads();
+      // ---- GEMM 2 on the chunk: E[y][x] += sum_m mid[y][m] W[x][mc0 + m] ----
+#pragma unroll
+      for (int s = 0; s < PER2; ++s)
+      {
+         const int ti = wave + kPsfWaves * s;
+         if (ti < T2)
+         {
+            const int rb = ti / CT2, cb = ti % CT2;
+            const int yr = rb * 16 + lr;
+            const size_t wx = (size_t)(cb * 16 + lr) * Mp + mc0 + lk;
+            for (int m0 = 0; m0 < mcn; m0 += 32)
+            {
+               const half8 arh = ld8(&mid[0][yr][m0 + lk]), arl = ld8(&mid[1][yr][m0 + lk]);
+               const half8 aih = ld8(&mid[2][yr][m0 + lk]), ail = ld8(&mid[3][yr][m0 + lk]);
+               const half8 brh = ld8(Wr + wx + m0), brl = ld8(Wr + wPer + wx + m0);
+               const half8 bih = ld8(Wr + 2 * wPer + wx + m0), bil = ld8(Wr + 3 * wPer + wx + m0);
+               eR[s] = mfma3(arh, arl, brh, brl, eR[s]);
+               eR[s] = mfma3(-aih, -ail, bih, bil, eR[s]);
+               eI[s] = mfma3(arh, arl, bih, bil, eI[s]);
+               eI[s] = mfma3(aih, ail, brh, brl, eI[s]);
+            }
+         }
+      }
+      __syncthreads();  // the next chunk's pupil and mid overwrite the LDS the products read
+   }
+   float* plane = out + ((size_t)r * nz + zi) * P * P;
+#pragma unroll
+   for (int s = 0; s < PER2; ++s)
+   {
+      const int ti = wave + kPsfWaves * s;
+      if (ti < T2)
+      {
+         const int rb = ti / CT2, cb = ti % CT2;
+         const int x = cb * 16 + lr;
+#pragma unroll
+         for (int i = 0; i < 4; ++i)
+         {
+            const int y = y0 + rb * 16 + 4 * (lane >> 4) + i;
+            if (y < P && x < P)
+               plane[(size_t)y * P + x] = eR[s][i] * eR[s][i] + eI[s][i] * eI[s][i];
+         }
+      }
+   }
+}
+
 // Small planes (padded size Pp <= 80), E = W (Pup W^T) with E in registers. Per 32-row chunk n0 of the pupil: the
 // split pupil rows Pc (32 x MP) in LDS; mid[n][x] = sum_m Pc[n][m] W[x][m] (32 x Pp), stored split and transposed
 // (midT[x][n]); E[y][x] += sum_n W[y][n0 + n] mid[n][x].
@@ -872,6 +1021,39 @@ static bool psf_planes_launch(iscg_context* h, const PsfBatch& s, int choice, fl
                             s.tabB.as<const double>(), s.W.as<const _Float16>(), s.tz.as<const double>(), nz, Pp, nYB,
                             out);
    };
+   if (Mp > 256)
+   {
+      // the chunked kernel: YB row blocks of 16 and PER2 E tiles per wave (PER2 <= 8: Pp <= 512 at YB 2, <= 1024 at 1)
+      const int yb = Pp <= 512 ? 2 : 1;
+      const int per2 = (yb * (Pp / 16) + kPsfWaves - 1) / kPsfWaves;
+      const int nYB = (Pp + yb * 16 - 1) / (yb * 16);
+      auto big = [&](auto ybc, auto perc) {
+         constexpr int YB = decltype(ybc)::value, PER2 = decltype(perc)::value;
+         hipLaunchKernelGGL((psf_planes_big_kernel<YB, PER2>), dim3(nReq * nz * nYB), dim3(kPsfThreads), 0,
+                            h->stream, s.reqs.as<const PsfReqDev>(), s.tabA.as<const double>(),
+                            s.tabB.as<const double>(), s.W.as<const _Float16>(), s.tz.as<const double>(), nz, Mp, Pp,
+                            nYB, out);
+      };
+      auto byPer = [&](auto ybc) {
+         switch (per2)
+         {
+         case 1: big(ybc, std::integral_constant<int, 1>{}); break;
+         case 2: big(ybc, std::integral_constant<int, 2>{}); break;
+         case 3: big(ybc, std::integral_constant<int, 3>{}); break;
+         case 4: big(ybc, std::integral_constant<int, 4>{}); break;
+         case 5: big(ybc, std::integral_constant<int, 5>{}); break;
+         case 6: big(ybc, std::integral_constant<int, 6>{}); break;
+         case 7: big(ybc, std::integral_constant<int, 7>{}); break;
+         case 8: big(ybc, std::integral_constant<int, 8>{}); break;
+         default: return false;
+         }
+         return true;
+      };
+      if (Mp > 1024 || Pp > 1024 || !(yb == 2 ? byPer(std::integral_constant<int, 2>{})
+                                               : byPer(std::integral_constant<int, 1>{})))
+         return fail(h, "iscg_psf_planes: pupil or plane size beyond 1024") == 0;
+      return ok(h, hipGetLastError(), "PSF planes launch");
+   }
    using I32 = std::integral_constant<int, 32>;
    switch (Mp)
    {
@@ -916,8 +1098,11 @@ static int32_t psf_batch_set(iscg_context* h, PsfBatch& b, const iscg_psf_reques
       dev[(size_t)i] = request_dev(r, sz, z, m);
    }
    const int Mp = (M + 31) / 32 * 32;
-   if (Mp > 256)
-      return fail(h, std::string(what) + ": pupil samples above 256 are not supported on this path");
+   if (Mp > 1024)
+      return fail(h, std::string(what) + ": more than 1024 pupil samples");
+   const int PpCheck = (size + 15) / 16 * 16;
+   if (Mp > 256 && PpCheck > 1024)
+      return fail(h, std::string(what) + ": planes wider than 1024 samples with more than 256 pupil samples");
    const int Pp = (size + 15) / 16 * 16;
    // per (request, plane): ti - ti0, as the reference forms it (Sim/ZernikePsf.cpp:238-244)
    std::vector<double> tz((size_t)nReq * nz);

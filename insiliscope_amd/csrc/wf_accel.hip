@@ -329,8 +329,9 @@ __global__ __launch_bounds__(256) void wf_render_kernel(const float* __restrict_
 
 using namespace iscg;
 
-// The FFT grids the per-focus passes take (iscg_wf_fits and iscg_wf_images share this rule).
-static bool accel_grid_ok(uint32_t NX, uint32_t NY) { return pow2(NX, 1024) && pow2(NY, 512); }
+// The FFT grids the per-focus passes take, as the reference's hosts do (WidefieldGpuJob::kMaxGpuFft = 2048,
+// Sim/WidefieldRender.h:336); iscg_wf_fits and iscg_wf_images share this rule.
+static bool accel_grid_ok(uint32_t NX, uint32_t NY) { return pow2(NX, 2048) && pow2(NY, 2048); }
 
 extern "C" {
 
@@ -383,7 +384,7 @@ int32_t iscg_wf_images(iscg_context* h, const iscg_wf_job* job, float* out)
       return fail(h, "iscg_wf_images: null job");
    const iscg_wf_job& J = *job;
    if (!accel_grid_ok(J.NX, J.NY))
-      return fail(h, "iscg_wf_images: FFT grid must be powers of two, NX in [16, 1024], NY in [16, 512]");
+      return fail(h, "iscg_wf_images: FFT grid must be powers of two in [16, 2048]");
    if (J.nx == 0 || J.ny == 0 || J.nx > J.NX || J.ny > J.NY || J.cw == 0 || J.ch == 0 || J.fovX0 + J.cw > J.NX ||
        J.fovY0 + J.ch > J.NY)
       return fail(h, "iscg_wf_images: dye grid or FOV outside the FFT grid");

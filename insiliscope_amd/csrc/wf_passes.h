@@ -60,10 +60,11 @@ __global__ __launch_bounds__(256) void rows_r2c_kernel(const float* __restrict__
    }
 }
 
-// Passes B and D: FFT along the rows of [planes][rows][S] for blocks of KC columns (16; 8 at N = 1024, so that a
-// block stays within 1024 lanes); rows >= rowsIn are zero, rows [row0, row0 + rowsOut) are written.
+// Passes B and D: FFT along the rows of [planes][rows][S] for blocks of KC columns (16; 8 at N = 1024 and 4 at 2048,
+// so that a block stays within 1024 lanes and 100 KB of LDS); rows >= rowsIn are zero, rows [row0, row0 + rowsOut)
+// are written.
 template <int N>
-constexpr int cols_block() { return N > 512 ? 8 : kCols; }
+constexpr int cols_block() { return N > 1024 ? 4 : (N > 512 ? 8 : kCols); }
 template <int N, bool INV, int KC = cols_block<N>()>
 __global__ __launch_bounds__(KC * N / 8) void cols_kernel(const float2* __restrict__ in, int rowsIn,
                                                           size_t inPlaneStride, float2* __restrict__ out,
@@ -277,7 +278,7 @@ __global__ __launch_bounds__(kCmThreads) void rows_c2r_cm_kernel(const float2* _
 
 // ---- dispatch over the supported lengths ----------------------------------------------------------------------
 
-// Lengths 16..MAXN (row passes: 1024; column passes use 16 N / 8 lanes, so 512).
+// Lengths 16..MAXN (at most 2048: a column block of cols_block<N>() lines fits 1024 lanes).
 template <template <int> class F, int MAXN = 512, typename... A>
 static bool dispatch_n(int n, A... a)
 {
@@ -293,6 +294,13 @@ static bool dispatch_n(int n, A... a)
       if constexpr (MAXN >= 1024)
       {
          F<1024>::run(a...);
+         return true;
+      }
+      return false;
+   case 2048:
+      if constexpr (MAXN >= 2048)
+      {
+         F<2048>::run(a...);
          return true;
       }
       return false;
@@ -419,11 +427,11 @@ static bool forward_xy(iscg_context* h, const XyGrid& g, const float* in, int nz
    const int S = (int)g.S;
    {
       ProfScope ps(h, "wf_rows_r2c");
-      if (!dispatch_n<RowsR2C, 1024>((int)g.NX, h->stream, in, (size_t)nxIn, nzIn * nyIn, nxIn, x1, S, g.twX))
+      if (!dispatch_n<RowsR2C, 2048>((int)g.NX, h->stream, in, (size_t)nxIn, nzIn * nyIn, nxIn, x1, S, g.twX))
          return false;
    }
    ProfScope ps(h, "wf_cols_fwd");
-   return dispatch_n<ColsF, 1024>((int)g.NY, h->stream, false, (const float2*)x1, nyIn, (size_t)nyIn * S, x2, 0,
+   return dispatch_n<ColsF, 2048>((int)g.NY, h->stream, false, (const float2*)x1, nyIn, (size_t)nyIn * S, x2, 0,
                             (int)g.NY, outPlaneStride, nzIn, S, (int)(g.NX / 2 + 1), g.twY);
 }
 
@@ -435,12 +443,12 @@ static bool inverse_xy(iscg_context* h, const XyGrid& g, const float2* in, int n
    const int S = (int)g.S;
    {
       ProfScope ps(h, "wf_cols_inv");
-      if (!dispatch_n<ColsF, 1024>((int)g.NY, h->stream, true, in, (int)g.NY, (size_t)g.NY * S, x2, fovY0, e.ch,
+      if (!dispatch_n<ColsF, 2048>((int)g.NY, h->stream, true, in, (int)g.NY, (size_t)g.NY * S, x2, fovY0, e.ch,
                              (size_t)e.ch * S, nPlanes, S, (int)(g.NX / 2 + 1), g.twY))
          return false;
    }
    ProfScope ps(h, c2rName);
-   return dispatch_n<RowsC2R, 1024>((int)g.NX, h->stream, (const float2*)x2, S, nPlanes * e.ch, e, g.twX);
+   return dispatch_n<RowsC2R, 2048>((int)g.NX, h->stream, (const float2*)x2, S, nPlanes * e.ch, e, g.twX);
 }
 
 }  // namespace iscg

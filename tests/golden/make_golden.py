@@ -27,6 +27,8 @@ Outputs (inputs and expected outputs only):
                      depth
 
   pupil_samples_ref.json  ZernikePupilSamples of 400 seeded requests (automatic and explicit pupil samples)
+  psf_big_ref.npz    ComputeZernikePsfPlanes with pupils above 256 samples: explicit M 1024 and 288, and an
+                     automatic M above 256 (wide window, coarse pixels)
 
 `python tests/golden/make_golden.py roi` writes only the named fixtures.
 """
@@ -402,6 +404,26 @@ def pupil_samples(R):
         json.dump({"source": "ZernikePupilSamples, compiled reference", "cases": cases}, fh, indent=0)
 
 
+PSF_BIG_CASES = {  # name: request fields over C1's optics and Zernike set
+    "m1024": dict(pupilSamples=1024, kernelHalfWidthPx=8, oversampling=4, nz=3, zStepNm=200.0),
+    "m288": dict(pupilSamples=288, kernelHalfWidthPx=16, oversampling=4, nz=3, zStepNm=250.0),
+    "auto": dict(pupilSamples=0, kernelHalfWidthPx=48, oversampling=1, nz=3, zStepNm=300.0, pixelSizeNm=320.0,
+                 wavelengthNm=500.0),
+}
+
+
+def psf_big(R):
+    """ComputeZernikePsfPlanes (Sim/ZernikePsf.cpp:150-334) at pupils above 256 samples (raw planes)."""
+    out = {}
+    for name, f in PSF_BIG_CASES.items():
+        fields = dict(configs.c1_psf(seed=1), **f)
+        r = req(**fields)
+        size, nz, M = O.Oracle().psf_geometry(r)
+        out[name + "_M"] = np.int32(R.pupil_samples(r))
+        out[name + "_planes"] = R.psf_planes(r, size, nz)
+    np.savez_compressed(os.path.join(HERE, "psf_big_ref.npz"), **out)
+
+
 def main():
     R = O.Reference()
     only = set(sys.argv[1:])
@@ -422,6 +444,7 @@ def main():
     dyes(R)
     roi(R)
     pupil_samples(R)
+    psf_big(R)
     print("fixtures written to", HERE)
 
 

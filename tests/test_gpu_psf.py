@@ -125,3 +125,20 @@ def test_compute_psf_kernel_cache_on_gpu(ctx, tmp_path):
     assert ok and c2.serial == c.serial and c2.planes is c.planes and c2.interpMode == 1
     assert log == ["PSF: kernel unchanged, reusing the one already computed."]
     P.SetPsfKernelDiskCacheDir("")
+
+
+@pytest.mark.parametrize("name", ["m1024", "m288", "auto"])
+def test_psf_large_pupils_vs_reference(ctx, golden, name):
+    """Pupils above 256 samples on the chunked kernel (psf_planes_big_kernel): explicit M 1024 (65 x 65) and 288
+    (129 x 129), and the automatic rule's M 416 (97 x 97): each plane within 1e-5 relative L2 of the compiled
+    reference's (psf_big_ref.npz); two requests in one call give the same planes as one."""
+    from test_oracle import PSF_BIG
+    G = golden("psf_big_ref.npz")
+    fields = dict(configs.c1_psf(seed=1), **PSF_BIG[name])
+    assert host.PsfGeometry(fields)[2] == int(G[name + "_M"])
+    got = host.ComputePsfPlanesBatch([fields, fields], ctx=ctx)
+    want = G[name + "_planes"]
+    errs = [rel_l2(got[0, k], want[k]) for k in range(len(want))]
+    print(name, "M", int(G[name + "_M"]), "max rel L2", max(errs), flush=True)
+    record_accuracy("psf_large_pupil_%s_vs_compiled_reference" % name, rel_l2_max=max(errs))
+    assert max(errs) <= 1e-5 and np.array_equal(got[0], got[1])

@@ -72,7 +72,9 @@ def random_job(rng, NX, NY, nx, ny, nPlanes, nKernels, nChannels, geometry, fov,
 
 @pytest.mark.parametrize("NX,NY,nx,ny,fov", [(512, 512, 256, 256, (0, 0, 256, 256)),
                                              (1024, 256, 700, 200, (37, 11, 600, 180)),
-                                             (64, 128, 40, 90, (3, 5, 30, 70))])
+                                             (64, 128, 40, 90, (3, 5, 30, 70)),
+                                             (1024, 2048, 900, 1800, (50, 100, 800, 1600)),
+                                             (2048, 512, 1500, 400, (10, 20, 1400, 360))])
 def test_synthetic_jobs_vs_oracle(ctx, NX, NY, nx, ny, fov):
     """Two jobs per geometry: the first carries the planes, the second only refers to them (resident), with
     another sub-cell shift and other blends; three channels."""
@@ -121,10 +123,12 @@ def test_fits_and_budget(ctx):
     accel = host.WidefieldGpuHost(ctx)
     assert accel.Fits(91, 512, 512, 100)
     assert not accel.Fits(92, 2048, 2048, 10 ** 7)
-    # grids the passes do not take do not fit (the scene then renders on the CPU instead of failing in Images)
-    for geom, NX, NY in ((94, 2048, 512), (95, 1024, 1024), (96, 96, 64), (97, 8, 64)):
+    # grids the passes do not take do not fit (the scene then renders on the CPU instead of failing in Images):
+    # powers of two up to WidefieldGpuJob::kMaxGpuFft = 2048 fit
+    for geom, NX, NY in ((96, 96, 64), (97, 8, 64), (99, 4096, 64), (100, 64, 4096)):
         assert not accel.Fits(geom, NX, NY, 1), (NX, NY)
-    assert accel.Fits(98, 1024, 512, 1)
+    for geom, NX, NY in ((94, 2048, 512), (95, 1024, 1024), (98, 2048, 2048)):
+        assert accel.Fits(geom, NX, NY, 1), (NX, NY)
     accel.SetBudget(3 * 64 * 48 * 8)  # room for three 64-point plane spectra
     try:
         rng = np.random.default_rng(2)

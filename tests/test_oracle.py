@@ -250,6 +250,27 @@ def test_chirp_z_equals_direct_dft(orc):
         assert np.max(np.abs(E[z] - direct)) < 1e-9 * np.max(np.abs(direct))
 
 
+PSF_BIG = {  # the request fields of psf_big_ref.npz over C1's optics (tests/golden/make_golden.py)
+    "m1024": dict(pupilSamples=1024, kernelHalfWidthPx=8, oversampling=4, nz=3, zStepNm=200.0),
+    "m288": dict(pupilSamples=288, kernelHalfWidthPx=16, oversampling=4, nz=3, zStepNm=250.0),
+    "auto": dict(pupilSamples=0, kernelHalfWidthPx=48, oversampling=1, nz=3, zStepNm=300.0, pixelSizeNm=320.0,
+                 wavelengthNm=500.0),
+}
+
+
+@pytest.mark.parametrize("name", list(PSF_BIG))
+def test_psf_large_pupils_vs_reference(orc, name):
+    """Pupils above 256 samples (explicit 1024 and 288, and the automatic rule's 416 on a wide window of coarse
+    pixels): the oracle's planes within 1e-6 relative L2 of the compiled reference's (psf_big_ref.npz), same M."""
+    G = np.load(os.path.join(GOLDEN, "psf_big_ref.npz"))
+    r = O.psf_request(**dict(configs.c1_psf(seed=1), **PSF_BIG[name]))
+    assert orc.psf_geometry(r)[2] == int(G[name + "_M"])
+    got = orc.psf_planes(r)
+    want = G[name + "_planes"]
+    for k in range(len(want)):
+        assert rel_l2(got[k], want[k]) < 1e-6, (name, k)
+
+
 def test_pupil_samples_vs_reference():
     """ZernikePupilSamples (Sim/ZernikePsf.cpp:134-148): the oracle's rule and the product's (iscg_psf_geometry)
     give the reference's M for 400 seeded requests (automatic: the smallest multiple of 4 in [64, 512] covering the
