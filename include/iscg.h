@@ -434,6 +434,38 @@ int32_t iscg_wf_fits(iscg_context* h, uint64_t geometry, uint32_t NX, uint32_t N
  * kernel is neither carried nor resident fails the call, as the reference's host does. */
 int32_t iscg_wf_images(iscg_context* h, const iscg_wf_job* job, float* out);
 
+/* ---- drifting samples on the mean field: RenderShiftedImages (Sim/WidefieldRender.h:292-300) --------------- */
+
+/* The kept spectra a drift's frames are made from (WidefieldImages with WidefieldScene::SetKeepSpectra,
+ * Sim/WidefieldRender.h:281-290): nImages focus images (WidefieldDriftFrames' focus grid), each nChannels channels
+ * (the persistent one first when hasPersistent, then the bleach basis), each a RealFft2d half spectrum of NY rows of
+ * NX/2 + 1 complex floats; the FOV's cells in the grid, the bin factor and the images' sub-cell pose. */
+typedef struct
+{
+   uint32_t NX, NY;               /* powers of two in [16, 2048] */
+   uint32_t fovX0, fovY0, cw, ch; /* FOV cells (cw, ch multiples of upscale) */
+   int32_t upscale;
+   int32_t nImages, nChannels, hasPersistent;
+   double fracX, fracY;           /* WidefieldImages::fracX / fracY */
+} iscg_wf_shift_desc;
+
+/* One frame: images i0 and i1 (-1: none) interpolated as (1 - w) i0 + w i1 (w = 0: i0 alone), the sample moved by
+ * (dxCells, dyCells) grid cells (WidefieldDriftFrames::Render, Sim/WidefieldRender.cpp:2964-2977). */
+typedef struct
+{
+   int32_t i0, i1;
+   double w, dxCells, dyCells;
+} iscg_wf_shift_frame;
+
+/* Replaces RenderShiftedImages (Sim/WidefieldRender.cpp:2905-2935) for nFrames frames: frame f's spectrum is the
+ * float-coefficient sum of its images' channels (persistent x the image's weight, bleach j x a[f][j] x the weight),
+ * phase-ramped by (fracX - dxCells, fracY - dyCells) (ApplyShiftRamp, Sim/DriftShift.cpp:24-48), inverse-transformed
+ * on the FOV, max(0, .) and summed over upscale x upscale cells (ShiftedCamera, :50-73), added to cam
+ * (nFrames x (ch / upscale) x (cw / upscale) floats). spectra: nImages x nChannels x NY x (NX/2 + 1) complex floats;
+ * a: nFrames x (nChannels - hasPersistent) doubles (NULL: no bleach terms). Host or device arrays. */
+int32_t iscg_wf_shifted_frames(iscg_context* h, const iscg_wf_shift_desc* desc, const float* spectra,
+                               const iscg_wf_shift_frame* frames, const double* a, uint32_t nFrames, float* cam);
+
 /* WidefieldImages::Render (Sim/WidefieldRender.cpp:547-580) for nFrames sets of bleach coefficients, and the camera
  * noise of the frames. channels: [nChannels][ch][cw] cell images (channel 0 the persistent one when hasPersistent,
  * then the bleach maps: iscg_wf_images' output is this layout); a: [nFrames][nChannels - hasPersistent] doubles.

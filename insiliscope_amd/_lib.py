@@ -58,6 +58,21 @@ class WfJob(C.Structure):
                 ("channelStart", C.c_void_p), ("deps", C.c_void_p)]
 
 
+class WfShiftDesc(C.Structure):
+    """iscg_wf_shift_desc: the kept spectra of a drift's focus images (Sim/WidefieldRender.h:281-290)."""
+
+    _fields_ = [("NX", C.c_uint32), ("NY", C.c_uint32), ("fovX0", C.c_uint32), ("fovY0", C.c_uint32),
+                ("cw", C.c_uint32), ("ch", C.c_uint32), ("upscale", C.c_int32), ("nImages", C.c_int32),
+                ("nChannels", C.c_int32), ("hasPersistent", C.c_int32), ("fracX", C.c_double), ("fracY", C.c_double)]
+
+
+class WfShiftFrame(C.Structure):
+    """iscg_wf_shift_frame: one drift frame (WidefieldDriftFrames::Render, Sim/WidefieldRender.cpp:2964-2977)."""
+
+    _fields_ = [("i0", C.c_int32), ("i1", C.c_int32), ("w", C.c_double), ("dxCells", C.c_double),
+                ("dyCells", C.c_double)]
+
+
 class RoiModel(C.Structure):
     """iscg_roi_model."""
 
@@ -136,6 +151,7 @@ SIGNATURES = {
     "iscg_wf_fits": (I32, [P, C.c_uint64, U32, U32, C.c_uint64]),
     "iscg_wf_images": (I32, [P, C.POINTER(WfJob), P]),
     "iscg_wf_set_budget": (I32, [P, C.c_uint64]),
+    "iscg_wf_shifted_frames": (I32, [P, C.POINTER(WfShiftDesc), P, P, P, U32, P]),
     "iscg_wf_render": (I32, [P, P, I32, I32, U32, U32, I32, P, U32, P, C.POINTER(Camera), P, U32, P, U32, P, P]),
     "iscg_noise_set_path": (I32, [P, I32]),
     "iscg_noise_bounds": (I32, [P, P]),

@@ -12,6 +12,7 @@ void destroy_wf(iscg_context* h);
 void destroy_roi(iscg_context* h);
 void destroy_wf_accel(iscg_context* h);
 void destroy_binned(iscg_context* h);
+void destroy_wf_drift(iscg_context* h);
 
 int32_t fail(iscg_context* h, const std::string& msg)
 {
@@ -266,6 +267,7 @@ void iscg_destroy(iscg_context* h)
    destroy_wf_accel(h);
    destroy_roi(h);
    destroy_binned(h);
+   destroy_wf_drift(h);
    for (auto& b : h->stage)
       b.release();
    for (auto& p : h->pending)

@@ -60,6 +60,7 @@ struct SplatState;
 struct PsfState;
 struct WfState;
 struct WfAccelState;
+struct WfDriftState;
 struct RoiState;
 struct BinnedState;
 
@@ -89,6 +90,7 @@ struct iscg_context
    iscg::PsfState* psf = nullptr;
    iscg::WfState* wf = nullptr;
    iscg::WfAccelState* wfAccel = nullptr;
+   iscg::WfDriftState* wfDrift = nullptr;
    iscg::RoiState* roi = nullptr;
    iscg::BinnedState* binned = nullptr;
    iscg::DevBuf dyeCounts;             // per-dye record counts of a schedule call

@@ -1172,6 +1172,116 @@ class WidefieldGpuHost:
         return True
 
 
+# ---- drifting samples on the mean field (Sim/Drift.h:118-157, Sim/WidefieldRender.h:292-300,880-911) ------------
+
+kDriftFocusStepNm = 10.0
+
+
+@dataclass
+class DriftFocusGrid:
+    """sim::DriftFocusGrid (Sim/Drift.h:136-144, Sim/Drift.cpp:145-166): foci every kDriftFocusStepNm over the
+    trajectory's z range; a frame's image interpolates its two neighbours."""
+
+    k0: int = 0
+    n: int = 1
+
+    @staticmethod
+    def For(zLoNm: float, zHiNm: float) -> "DriftFocusGrid":
+        lo = int(np.floor(zLoNm / kDriftFocusStepNm))
+        hi = int(np.ceil(zHiNm / kDriftFocusStepNm))
+        return DriftFocusGrid(lo, hi - lo + 1)
+
+    def DzNm(self, k: int) -> float:
+        return (self.k0 + k) * kDriftFocusStepNm
+
+    def Weights(self, dzNm: float):
+        """(k, w): image(dz) = (1 - w) image(k) + w image(k + 1)."""
+        if self.n <= 1:
+            return 0, 0.0
+        t = dzNm / kDriftFocusStepNm - float(self.k0)
+        k = min(self.n - 2, max(0, int(np.floor(t))))
+        return k, min(1.0, max(0.0, t - k))
+
+
+def DriftRange(driftNm):
+    """sim::DriftRange (Sim/Drift.cpp:124-137): the trajectory's bounds, (xLo, xHi, yLo, yHi, zLo, zHi) nm, each
+    including 0."""
+    d = np.asarray(driftNm, np.float64).reshape(-1, 3)
+    lo = np.minimum(d.min(axis=0), 0.0) if len(d) else np.zeros(3)
+    hi = np.maximum(d.max(axis=0), 0.0) if len(d) else np.zeros(3)
+    return lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]
+
+
+def WidefieldShiftedFrames(desc: dict, spectra, frames, a=None, cam=None, ctx: Context | None = None):
+    """RenderShiftedImages (Sim/WidefieldRender.cpp:2905-2935) for a batch of frames (iscg_wf_shifted_frames).
+    desc: NX, NY, fovX0, fovY0, cw, ch, upscale, fracX, fracY, hasPersistent; spectra (nImages, nChannels, NY,
+    NX/2 + 1) complex64 (numpy) or the same as a float32 device tensor; frames: (i0, i1, w, dxCells, dyCells) per
+    frame; a: (nFrames, nBleach) coefficients or None; cam (nFrames, ch/up, cw/up) float32, added to (zeros when
+    None). Returns cam."""
+    ctx = ctx or default_context()
+    if isinstance(spectra, np.ndarray):
+        sp = np.ascontiguousarray(spectra, np.complex64)
+        nImages, nCh = sp.shape[0], sp.shape[1]
+        sp = sp.view(np.float32)
+    else:
+        sp = spectra
+        nImages, nCh = int(desc["nImages"]), int(desc["nChannels"])
+    up = max(1, int(desc.get("upscale", 1)))
+    fr = (_lib.WfShiftFrame * max(1, len(frames)))()
+    for k, f in enumerate(frames):
+        fr[k] = _lib.WfShiftFrame(int(f[0]), int(f[1]), float(f[2]), float(f[3]), float(f[4]))
+    if cam is None:
+        cam = np.zeros((len(frames), desc["ch"] // up, desc["cw"] // up), np.float32)
+    d = _lib.WfShiftDesc(desc["NX"], desc["NY"], desc["fovX0"], desc["fovY0"], desc["cw"], desc["ch"], up, nImages,
+                         nCh, int(desc.get("hasPersistent", 1)), float(desc.get("fracX", 0.0)),
+                         float(desc.get("fracY", 0.0)))
+    aa = None if a is None else np.ascontiguousarray(a, np.float64)
+    ps, k1 = _ptr(sp)
+    pa, k2 = _ptr(aa)
+    pc, k3 = _ptr(cam)
+    ctx.follow_torch()
+    ctx.check(ctx.L.iscg_wf_shifted_frames(ctx.h, C.byref(d), ps, fr, pa, len(frames), pc), "iscg_wf_shifted_frames")
+    return cam
+
+
+class WidefieldDriftFrames:
+    """sim::WidefieldDriftFrames (Sim/WidefieldRender.h:880-911, .cpp:2937-2977) on the GPU: the focus images on the
+    drift's focus grid around each base focus (their kept spectra, SetSeries), each frame interpolated in z and
+    shifted in xy by iscg_wf_shifted_frames, many frames per call."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.grid = DriftFocusGrid()
+        self.base = []
+        self.desc = None
+        self.spectra = None
+
+    def Begin(self, bounds, baseFocusWorldUm):
+        """bounds: DriftRange(trajectory); baseFocusWorldUm: the undrifted focal planes."""
+        self.grid = DriftFocusGrid.For(bounds[4], bounds[5])
+        self.base = list(baseFocusWorldUm)
+        self.spectra = None
+
+    def Foci(self):
+        """The foci of the series, base-major (WidefieldDriftFrames::Refresh: the sample sits dz higher, so the
+        focal plane is dz lower in it)."""
+        return [f - self.grid.DzNm(k) / 1000.0 for f in self.base for k in range(self.grid.n)]
+
+    def SetSeries(self, desc: dict, spectra, pitchNm: float):
+        """The series' kept spectra (len(Foci()) images, in Foci() order) and the grid pitch."""
+        self.desc, self.spectra, self.pitchNm = dict(desc), spectra, float(pitchNm)
+
+    def Render(self, base: int, driftNm, a=None, cam=None):
+        """The frames of base focus `base` at each drift (x, y, z nm), added to cam."""
+        frames = []
+        for dx, dy, dz in np.asarray(driftNm, np.float64).reshape(-1, 3):
+            k, w = self.grid.Weights(dz)
+            i = base * self.grid.n + k
+            two = w != 0.0 and k + 1 < self.grid.n
+            frames.append((i, i + 1 if two else -1, w if two else 0.0, dx / self.pitchNm, dy / self.pitchNm))
+        return WidefieldShiftedFrames(self.desc, self.spectra, frames, a, cam, ctx=self.ctx)
+
+
 def WidefieldImagesRender(channels, hasPersistent: bool, upscale: int, a, base=None, cam: CameraNoiseParams | None = None,
                           offset=None, seed: int = 0, frames=None, pixelBase: int = 0, outPhotons=None, outAdu=None,
                           ctx: Context | None = None):

@@ -498,6 +498,9 @@ class Reference:
         L.ref_wf_capture_jobs.argtypes = [PF, I32, I32, I32, D, PF, I32, I32, I32, D, D, C.POINTER(D), I32, D, D,
                                           C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), PF]
         L.ref_wf_capture_jobs.restype = I32
+        L.ref_wf_drift_frames.argtypes = [PF, I32, I32, I32, D, PF, I32, I32, I32, D, D, D, PD, I32, PF, PF,
+                                          C.c_size_t, PI32, PD, PD]
+        L.ref_wf_drift_frames.restype = I32
         L.ref_noise_setup.argtypes = [C.c_int64, U32, U32, U32, U32, D, D, D, D, D, D, PF, PF, PF]
         L.ref_psf_half_width_px.argtypes = [D, D, D, D]
         L.ref_psf_half_width_px.restype = I32
@@ -659,6 +662,33 @@ class Reference:
         if r < 0:
             raise RuntimeError("WidefieldScene failed")
         return out
+
+    def wf_drift_frames(self, density, dz_nm, planes, os_, z_step_nm, base_focus_um, drift_nm, kernel_cap_um=12.8):
+        """WidefieldDriftFrames on a persistent population: (frames (nF, ny, nx), spectra (nImages, NY, NX/2+1)
+        complex64, geometry dict)."""
+        density = np.ascontiguousarray(density, np.float32)
+        planes = np.ascontiguousarray(planes, np.float32)
+        drift = np.ascontiguousarray(drift_nm, np.float64).reshape(-1, 3)
+        nz, ny, nx = density.shape
+        cam = np.zeros((len(drift), ny, nx), np.float32)
+        cap = 1 << 26
+        spec = np.zeros(cap, np.float32)
+        geo = np.zeros(9, np.int32)
+        frac = np.zeros(2)
+        pitch = C.c_double()
+        r = self.L.ref_wf_drift_frames(_p(density, C.c_float), nx, ny, nz, dz_nm, _p(planes, C.c_float),
+                                       planes.shape[0], planes.shape[1], os_, z_step_nm, kernel_cap_um, base_focus_um,
+                                       _p(drift, C.c_double), len(drift), _p(cam, C.c_float), _p(spec, C.c_float),
+                                       cap, _p(geo, C.c_int32), _p(frac, C.c_double), C.byref(pitch))
+        if r != 0:
+            raise RuntimeError(f"WidefieldDriftFrames failed ({r})")
+        NX, NY, n = int(geo[0]), int(geo[1]), int(geo[7])
+        W = NX // 2 + 1
+        spectra = spec[: 2 * n * NY * W].view(np.complex64).reshape(n, NY, W).copy()
+        g = dict(NX=NX, NY=NY, fovX0=int(geo[2]), fovY0=int(geo[3]), cw=int(geo[4]), ch=int(geo[5]),
+                 upscale=int(geo[6]), nImages=n, k0=int(geo[8]), fracX=float(frac[0]), fracY=float(frac[1]),
+                 pitchNm=pitch.value)
+        return cam, spectra, g
 
     def wf_capture_jobs(self, density, dz_nm, planes, os_, z_step_nm, foci_um, dx_um=0.0, dy_um=0.0,
                         kernel_cap_um=12.8):

@@ -12,6 +12,7 @@
 #include "SMLMSimulation.h"
 #include "SMLMZernike.h"
 #include "WidefieldRender.h"
+#include "Drift.h"
 #include "BinnedBlinks.h"
 #include "ZernikePsf.h"
 #include "Illumination.h"
@@ -409,6 +410,111 @@ int32_t ref_wf_focus_series(const float* density, int32_t nx, int32_t ny, int32_
       std::memcpy(out + static_cast<size_t>(f) * nx * ny, im.persistent.data(), im.persistent.size() * sizeof(float));
    }
    return 1;
+}
+
+// WidefieldDriftFrames (Sim/WidefieldRender.cpp:2937-2977) over a spectra-keeping scene of a persistent population
+// (density as ref_wf_focus_series): Begin(DriftRange(drift), {baseFocusUm}), Refresh, then Render of each frame's
+// drift (x, y, z nm: drift3[3 f ..]) into outCam (nFrames x ny x nx, from zero). Also what the frames are made from:
+// the focus series on the drift's focus grid (foci base - DzNm(k) / 1000), each image's persistent spectrum
+// (nImages x NY x (NX/2 + 1) complex floats, up to specCap floats) and geometry geo = {NX, NY, fovX0, fovY0, cw, ch,
+// upscale, nImages, k0}, frac = {fracX, fracY}, pitchNm. Returns 0, -1 on failure, -2 when specCap is too small.
+int32_t ref_wf_drift_frames(const float* density, int32_t nx, int32_t ny, int32_t nzD, double dzNm,
+                            const float* planes, int32_t nzP, int32_t size, int32_t os, double zStepNm,
+                            double kernelCapUm, double baseFocusUm, const double* drift3, int32_t nFrames,
+                            float* outCam, float* specOut, size_t specCap, int32_t* geo, double* frac, double* pitchNm)
+{
+   sim::PsfKernelCache c = MakeCache(planes, nzP, size, os, zStepNm, 0);
+   sim::KernelWidefieldPsf psf(c, 1);
+   sim::WidefieldSceneSpec spec;
+   spec.pixelUm = 0.1;
+   spec.originXUm = 0.05;
+   spec.originYUm = 0.05;
+   spec.width = static_cast<unsigned>(nx);
+   spec.height = static_cast<unsigned>(ny);
+   spec.focusWorldUm = baseFocusUm;
+   spec.slabHalfUm = 0.0;
+   spec.grid.upscale = 1;
+   spec.grid.zPlaneNm = dzNm;
+   spec.marginUm = 0.0;
+   spec.kernelCapUm = kernelCapUm;
+   spec.eta = 1.0;
+   spec.unitDose = 1.0;
+   sim::FlatIllumination ill(nx * 0.1, ny * 0.1);
+   unsigned fx = 0, fy = 0;
+   double frx = 0, fry = 0;
+   const sim::WidefieldGridSpec g = sim::WidefieldScene::GridSpecFor(ill, spec, &fx, &fy, &frx, &fry);
+   sim::WidefieldDyeGrid G;
+   G.spec = g;
+   G.k0 = 0;
+   G.nz = static_cast<unsigned>(nzD);
+   const size_t gp = static_cast<size_t>(g.nx) * g.ny;
+   G.persistent.assign(gp * nzD, 0.0f);
+   double total = 0.0;
+   for (int z = 0; z < nzD; ++z)
+      for (int y = 0; y < ny; ++y)
+         for (int x = 0; x < nx; ++x)
+         {
+            const float v = density[(static_cast<size_t>(z) * ny + y) * nx + x];
+            G.persistent[z * gp + static_cast<size_t>(fy + y) * g.nx + fx + x] = v;
+            total += v;
+         }
+   G.nPersistent = static_cast<long>(total);
+   sim::WidefieldScene sc;
+   sc.SetKeepSpectra(true);
+   std::string err;
+   if (!sc.UpdateFromGrid(G, ill, spec, psf, err))
+      return -1;
+   std::vector<sim::DriftNm> traj(static_cast<size_t>(nFrames));
+   for (int f = 0; f < nFrames; ++f)
+   {
+      traj[static_cast<size_t>(f)].x = drift3[3 * f];
+      traj[static_cast<size_t>(f)].y = drift3[3 * f + 1];
+      traj[static_cast<size_t>(f)].z = drift3[3 * f + 2];
+   }
+   const sim::DriftBounds b = sim::DriftRange(traj);
+   sim::WidefieldDriftFrames df;
+   df.Begin(b, {baseFocusUm});
+   if (!df.Refresh(sc, err))
+      return -1;
+   const size_t npx = static_cast<size_t>(nx) * ny;
+   for (int f = 0; f < nFrames; ++f)
+   {
+      std::vector<float> cam(npx, 0.0f);
+      df.Render(sc, 0, traj[static_cast<size_t>(f)], {}, cam);
+      if (cam.size() != npx)
+         return -1;
+      std::memcpy(outCam + f * npx, cam.data(), npx * sizeof(float));
+   }
+   const sim::DriftFocusGrid grid = sim::DriftFocusGrid::For(b);
+   std::vector<double> foci;
+   for (int k = 0; k < grid.n; ++k)
+      foci.push_back(baseFocusUm - grid.DzNm(k) / 1000.0);
+   std::vector<sim::WidefieldImages> series;
+   if (!sc.FocusSeries(foci, series, err) || series.empty() || !series[0].HasSpectra())
+      return -1;
+   const sim::WidefieldImages& i0 = series[0];
+   const size_t n = i0.fft->SpecSize();
+   geo[0] = static_cast<int32_t>(i0.fft->Nx());
+   geo[1] = static_cast<int32_t>(i0.fft->Ny());
+   geo[2] = static_cast<int32_t>(i0.fovX0);
+   geo[3] = static_cast<int32_t>(i0.fovY0);
+   geo[4] = static_cast<int32_t>(i0.cw);
+   geo[5] = static_cast<int32_t>(i0.ch);
+   geo[6] = static_cast<int32_t>(i0.upscale);
+   geo[7] = grid.n;
+   geo[8] = static_cast<int32_t>(grid.k0);
+   frac[0] = i0.fracX;
+   frac[1] = i0.fracY;
+   *pitchNm = sc.Grid().pitchUm * 1000.0;
+   if (2 * n * series.size() > specCap)
+      return -2;
+   for (size_t m = 0; m < series.size(); ++m)
+   {
+      if (series[m].specPersistent.size() != n)
+         return -1;
+      std::memcpy(specOut + 2 * n * m, series[m].specPersistent.data(), n * sizeof(sim::cfloat));
+   }
+   return 0;
 }
 
 // BinnedBlinkRenderer (Sim/BinnedBlinks.cpp:94-246): Setup with the caller's planes (null: the Gaussian of sigmaPx),

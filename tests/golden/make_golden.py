@@ -30,6 +30,9 @@ Outputs (inputs and expected outputs only):
   psf_big_ref.npz    ComputeZernikePsfPlanes with pupils above 256 samples: explicit M 1024 and 288, and an
                      automatic M above 256 (wide window, coarse pixels)
 
+  wf_drift_ref.npz   WidefieldDriftFrames (RenderShiftedImages) frames of a spectra-keeping scene under seeded drift
+                     trajectories, with the focus series' kept spectra and geometry
+
 `python tests/golden/make_golden.py roi` writes only the named fixtures.
 """
 from __future__ import annotations
@@ -424,6 +427,27 @@ def psf_big(R):
     np.savez_compressed(os.path.join(HERE, "psf_big_ref.npz"), **out)
 
 
+def wf_drift(R):
+    """WidefieldDriftFrames (Sim/WidefieldRender.cpp:2937-2977) on wf_ref.npz's scene: two base foci, 12 frames each
+    of a seeded drift (xy within +-260 nm, z within +-34 nm, so frames fall between the 10 nm focus images)."""
+    W = np.load(os.path.join(HERE, "wf_ref.npz"))
+    P = np.load(os.path.join(HERE, "psf_ref.npz"))["small_planes"]
+    rng = np.random.default_rng(29)
+    out = {}
+    for name, base in (("a", 0.125), ("b", 0.21)):
+        drift = np.stack([rng.uniform(-260, 260, 12), rng.uniform(-260, 260, 12), rng.uniform(-34, 34, 12)], axis=1)
+        drift[0] = 0.0
+        cam, spec, g = R.wf_drift_frames(W["scene_density"], 50.0, P, 4, 50.0, base, drift)
+        out[name + "_base"] = np.float64(base)
+        out[name + "_drift"] = drift
+        out[name + "_frames"] = cam
+        out[name + "_spectra"] = spec
+        out[name + "_geometry"] = np.array([g[k] for k in ("NX", "NY", "fovX0", "fovY0", "cw", "ch", "upscale",
+                                                           "nImages", "k0")], np.int32)
+        out[name + "_frac_pitch"] = np.array([g["fracX"], g["fracY"], g["pitchNm"]])
+    np.savez_compressed(os.path.join(HERE, "wf_drift_ref.npz"), **out)
+
+
 def main():
     R = O.Reference()
     only = set(sys.argv[1:])
@@ -445,6 +469,7 @@ def main():
     roi(R)
     pupil_samples(R)
     psf_big(R)
+    wf_drift(R)
     print("fixtures written to", HERE)
 
 

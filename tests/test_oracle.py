@@ -445,6 +445,70 @@ def test_wf_direct_is_linear(orc):
     assert np.allclose(ia + ib, iab, rtol=1e-12, atol=1e-12)
 
 
+# ---- drifting samples: WidefieldDriftFrames / RenderShiftedImages restated ---------------------------------------
+
+def drift_case(G, name):
+    """The host-side frame specs of WidefieldDriftFrames::Render (Sim/WidefieldRender.cpp:2964-2977) for a
+    wf_drift_ref.npz case: (geometry dict, [(i0, i1, w, dxCells, dyCells)])."""
+    from insiliscope_amd import host
+    NX, NY, fx, fy, cw, ch, up, nImages, k0 = (int(v) for v in G[name + "_geometry"])
+    fracX, fracY, pitch = (float(v) for v in G[name + "_frac_pitch"])
+    drift = G[name + "_drift"]
+    grid = host.DriftFocusGrid.For(*host.DriftRange(drift)[4:])
+    assert (grid.k0, grid.n) == (k0, nImages)
+    frames = []
+    for dx, dy, dz in drift:
+        k, w = grid.Weights(dz)
+        two = w != 0.0 and k + 1 < grid.n
+        frames.append((k, k + 1 if two else -1, w if two else 0.0, dx / pitch, dy / pitch))
+    desc = dict(NX=NX, NY=NY, fovX0=fx, fovY0=fy, cw=cw, ch=ch, upscale=up, fracX=fracX, fracY=fracY,
+                hasPersistent=1)
+    return desc, frames
+
+
+def shifted_frames_restated(desc, spectra, frames):
+    """RenderShiftedImages + ShiftedCamera (Sim/WidefieldRender.cpp:2905-2935, Sim/DriftShift.cpp:24-73) for the
+    persistent channel, in FP64: the weighted spectra, the phase ramp (cosine at the Nyquist bins), the inverse
+    real FFT cropped to the FOV, max(0, .) summed over up x up cells."""
+    NX, NY, up = desc["NX"], desc["NY"], desc["upscale"]
+    W = NX // 2 + 1
+    kx = np.arange(W)
+    kys = np.where(np.arange(NY) <= NY // 2, np.arange(NY), np.arange(NY) - NY)
+    out = []
+    for i0, i1, w, dx, dy in frames:
+        S = (np.float32(1.0 - w) if i1 >= 0 else np.float32(1.0)) * spectra[i0].astype(np.complex128)
+        if i1 >= 0 and w != 0.0:
+            S = S + np.float32(w) * spectra[i1].astype(np.complex128)
+        fx, fy = desc["fracX"] - dx, desc["fracY"] - dy
+        px = np.exp(2j * np.pi * kx * fx / NX)
+        px[NX // 2] = np.cos(2 * np.pi * (NX // 2) * fx / NX)
+        py = np.exp(2j * np.pi * kys * fy / NY)
+        py[NY // 2] = np.cos(2 * np.pi * (NY // 2) * fy / NY)
+        img = np.fft.irfft2(S * py[:, None] * px[None, :], s=(NY, NX))
+        img = img[desc["fovY0"]:desc["fovY0"] + desc["ch"], desc["fovX0"]:desc["fovX0"] + desc["cw"]]
+        img = np.maximum(img, 0.0)
+        out.append(img.reshape(desc["ch"] // up, up, desc["cw"] // up, up).sum(axis=(1, 3)))
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("name", ["a", "b"])
+def test_drift_frames_restatement_vs_reference(name):
+    """The restatement reproduces the compiled reference's WidefieldDriftFrames frames (wf_drift_ref.npz: 12 frames
+    of xy drift within +-260 nm and z within +-34 nm, between the 10 nm focus images) within 1e-6 relative L2
+    (the reference's FFT is FP32); the focus grid and each frame's images and weights follow DriftFocusGrid."""
+    G = np.load(os.path.join(GOLDEN, "wf_drift_ref.npz"))
+    desc, frames = drift_case(G, name)
+    assert any(f[1] >= 0 and 0.0 < f[2] < 1.0 for f in frames)
+    got = shifted_frames_restated(desc, G[name + "_spectra"], frames)
+    want = G[name + "_frames"]
+    for f in range(len(want)):
+        assert rel_l2(got[f], want[f]) < 1e-6, (name, f, rel_l2(got[f], want[f]))
+    # a planted sign error in the drift (the ramp's direction) fails
+    bad = [(a, b, w, -dx, -dy) for a, b, w, dx, dy in frames]
+    worst = max(rel_l2(g, w_) for g, w_ in zip(shifted_frames_restated(desc, G[name + "_spectra"], bad), want))
+    assert worst > 1e-2
+
+
 # ---- the per-focus job seam (WidefieldAccelerator): JobReferenceHost restated ------------------------------------
 
 def test_wf_job_oracle_matches_reference_scene(golden):
