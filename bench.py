@@ -446,9 +446,15 @@ def run_c1(ctx, steps, warmup):
     # RenderPhotonImage does per frame on the CPU), splats and noises the 50 frames
     movie = host.SmlmMovieGpu(ev, 50, w, h, 100.0, 1.0, 20.0, offset=off, ctx=ctx)
 
-    def step():
+    def render():
         if not movie.Render(req, 4, 50.0, 2, cam, 1, outAdu=adu):
             raise RuntimeError(movie.splat.err)
+
+    # the whole movie (PSF, block sums, collect, splat + noise) captured once as a HIP graph; a step is one replay
+    movie.Capture(req, 4, 50.0, 2, cam, 1, outAdu=adu)
+
+    def step():
+        movie.Replay()
 
     for _ in range(warmup):
         step()
@@ -456,11 +462,12 @@ def run_c1(ctx, steps, warmup):
         for _ in range(steps):
             step()
     ms = t.ms / steps
-    prof = profiled(ctx, lambda k: step(), min(steps, 3))
+    prof = profiled(ctx, lambda k: render(), min(steps, 3))  # per-kernel times from eager renders
     return {"value": 50 * 1000.0 / ms, "unit": "frames/s", "ms_per_step": ms,
             "config": {"workload": "SMLM 128x128 x 50 frames end to end: Gibson-Lanni PSF (41 planes 129x129, os 4, "
                                    "M 128, Noll 5-15 +-0.05) + frame emitters collected from the events on the GPU + "
-                                   "Catmull-Rom splats of ~200 blinks/frame + sCMOS",
+                                   "Catmull-Rom splats of ~200 blinks/frame + sCMOS; a step = one replay of the "
+                                   "movie's captured HIP graph",
                        "blinks": int(len(ev))},
             "kernels": {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4)} for k, v in prof.items()},
             "_events": (ev, st)}
@@ -726,7 +733,7 @@ def main():
     elif args.workload == "c1":
         r = run_c1(ctx, args.steps, args.warmup)
         r.pop("_events")
-        config, unit, dtype = r["config"], "frames/s", "f64"
+        config, unit, dtype = dict(r["config"], kernels=r["kernels"]), "frames/s", "f64"
         r["roofline"] = None
     elif args.workload == "adjoint":
         r = run_c5(ctx, args.steps, args.warmup)

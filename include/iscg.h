@@ -250,7 +250,9 @@ int32_t iscg_bucket_events(iscg_context* h, const iscg_blink_event* events, uint
  * (may be NULL) = {clamped, total}. Arrays may be host or device memory. The call reads the bucket offsets on the
  * host; when out and outStart are device memory, out holds every bucket entry (outCap >= bucketStart[nFrames]) and
  * zCounts is NULL, it then returns without waiting for the device (outStart[nFrames] holds the count), else it
- * synchronizes the context's stream. */
+ * synchronizes the context's stream. On a stream being captured into a HIP graph, device bucket offsets must be the
+ * array the last eager call on the context validated (same address, same nFrames, unchanged since): its total is
+ * reused and every bucket range is clamped to it; such a call can then be captured. */
 int32_t iscg_collect_frame_emitters(iscg_context* h, const iscg_blink_event* events, uint32_t nEvents,
                                     const uint32_t* bucketStart, const uint32_t* bucketIdx, const int64_t* frameIndex,
                                     uint32_t nFrames, uint32_t width, uint32_t height, double pixelSizeNm,

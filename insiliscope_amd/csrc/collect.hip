@@ -15,6 +15,8 @@
 // prefix over the workgroup's waves keep them in bucket order).
 #include "iscg_internal.h"
 
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 
 namespace iscg {
@@ -33,6 +35,7 @@ struct CollectArgs
    uint32_t nEvents, width, height;
    double pixelSizeNm, photonsPerBlink, zStepNm, globalZOffsetUm;
    int32_t nz, kernel;
+   uint32_t nIdx;         // the bucket lists' length as validated on the host: every range is clamped to it
 };
 
 // overlap of event e with frame F (<= 0: not in it)
@@ -86,7 +89,7 @@ __global__ __launch_bounds__(kCollectThreads) void collect_count_kernel(CollectA
 {
    __shared__ uint32_t part[kCollectThreads / 64];
    const int f = (int)blockIdx.x, tid = threadIdx.x;
-   const uint32_t b0 = a.bStart[f], b1 = a.bStart[f + 1];
+   const uint32_t b1 = min(a.bStart[f + 1], a.nIdx), b0 = min(a.bStart[f], b1);
    const int64_t F = a.frameIndex[f];
    uint32_t n = 0;
    for (uint32_t i = b0 + tid; i < b1; i += kCollectThreads)
@@ -110,12 +113,17 @@ __global__ __launch_bounds__(kCollectThreads) void collect_count_kernel(CollectA
 }
 
 // Exclusive scan of n counts into start[0..n] (start[n] = the total), one workgroup of 1024 lanes, 64-bit sums.
+// zCounts (2 words) are zeroed here for the fill kernel after it: a kernel, not a memset, so the call can be captured
+// in a HIP graph and replayed.
 __global__ __launch_bounds__(1024) void collect_scan_kernel(const uint32_t* __restrict__ counts, uint32_t n,
-                                                            unsigned long long* __restrict__ start)
+                                                            unsigned long long* __restrict__ start,
+                                                            unsigned long long* __restrict__ zCounts)
 {
    __shared__ unsigned long long sums[1024];
    __shared__ unsigned long long carry;
    const int tid = threadIdx.x;
+   if (tid < 2)
+      zCounts[tid] = 0;
    if (tid == 0)
       carry = 0;
    __syncthreads();
@@ -157,7 +165,7 @@ __global__ __launch_bounds__(kCollectThreads) void collect_fill_kernel(CollectAr
 {
    __shared__ uint32_t waveKept[kCollectThreads / 64];
    const int f = (int)blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-   const uint32_t b0 = a.bStart[f], b1 = a.bStart[f + 1];
+   const uint32_t b1 = min(a.bStart[f + 1], a.nIdx), b0 = min(a.bStart[f], b1);
    const int64_t F = a.frameIndex[f];
    const double dx = a.drift ? a.drift[2 * f] : 0.0, dy = a.drift ? a.drift[2 * f + 1] : 0.0;
    unsigned long long base = start[f];
@@ -313,21 +321,42 @@ int32_t iscg_collect_frame_emitters(iscg_context* h, const iscg_blink_event* eve
       return fail(h, "iscg_collect_frame_emitters: pixel size must be > 0");
    if (illumination && (width == 0 || height == 0))
       return fail(h, "iscg_collect_frame_emitters: an illumination field needs the frame size");
-   // the bucket offsets decide every access: read them (and check them) on the host first
-   std::vector<uint32_t> st(nFrames + 1);
-   if (is_device_ptr(bucketStart))
+   // the bucket offsets decide every access: read them (and check them) on the host first. A stream being
+   // captured into a HIP graph cannot wait for that: there the offsets must be the device array the last eager call
+   // on this context validated (same address and frame count), whose total is reused; the kernels clamp every range
+   // to that total and skip event indices beyond nEvents, so no read leaves the validated lists.
+   size_t nIdx = 0;
+   const bool devSt = is_device_ptr(bucketStart);
+   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+   const bool capturing = hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+   (void)hipGetLastError();
+   if (devSt && capturing)
    {
-      if (!ok(h, hipMemcpyAsync(st.data(), bucketStart, st.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                h->stream), "bucket offsets") ||
-          !ok(h, hipStreamSynchronize(h->stream), "bucket offsets"))
-         return -1;
+      if (h->collectStPtr != bucketStart || h->collectStFrames != nFrames)
+         return fail(h, "iscg_collect_frame_emitters: under capture, the bucket offsets must be the device array an "
+                        "eager call on this context validated");
+      nIdx = h->collectStTotal;
    }
    else
-      std::memcpy(st.data(), bucketStart, st.size() * sizeof(uint32_t));
-   for (uint32_t f = 0; f < nFrames; ++f)
-      if (st[f + 1] < st[f])
-         return fail(h, "iscg_collect_frame_emitters: bucket offsets must not decrease");
-   const size_t nIdx = st[nFrames];
+   {
+      std::vector<uint32_t> st(nFrames + 1);
+      if (devSt)
+      {
+         if (!ok(h, hipMemcpyAsync(st.data(), bucketStart, st.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   h->stream), "bucket offsets") ||
+             !ok(h, hipStreamSynchronize(h->stream), "bucket offsets"))
+            return -1;
+      }
+      else
+         std::memcpy(st.data(), bucketStart, st.size() * sizeof(uint32_t));
+      for (uint32_t f = 0; f < nFrames; ++f)
+         if (st[f + 1] < st[f])
+            return fail(h, "iscg_collect_frame_emitters: bucket offsets must not decrease");
+      nIdx = st[nFrames];
+      h->collectStPtr = devSt ? bucketStart : nullptr;
+      h->collectStFrames = nFrames;
+      h->collectStTotal = nIdx;
+   }
    if (nIdx && !bucketIdx)
       return fail(h, "iscg_collect_frame_emitters: null bucket lists");
    const size_t evBytes = (size_t)nEvents * sizeof(iscg_blink_event);
@@ -350,11 +379,10 @@ int32_t iscg_collect_frame_emitters(iscg_context* h, const iscg_blink_event* eve
    unsigned long long* dStart = (unsigned long long*)((char*)scr.p + cntBytes);
    unsigned long long* dZ = (unsigned long long*)((char*)scr.p + cntBytes + stBytes);
    CollectArgs a{dEv, dSt, dIdx, dFi, dDr, dIl, nEvents, width, height, pixelSizeNm, photonsPerBlink, zStepNm,
-                 globalZOffsetUm, nz, kernel};
-   if (!ok(h, hipMemsetAsync(dZ, 0, 16, h->stream), "collect counters"))
-      return -1;
+                 globalZOffsetUm, nz, kernel, (uint32_t)std::min<size_t>(nIdx, 0xFFFFFFFFu)};
    hipLaunchKernelGGL(collect_count_kernel, dim3(nFrames), dim3(kCollectThreads), 0, h->stream, a, dCnt);
-   hipLaunchKernelGGL(collect_scan_kernel, dim3(1), dim3(1024), 0, h->stream, (const uint32_t*)dCnt, nFrames, dStart);
+   hipLaunchKernelGGL(collect_scan_kernel, dim3(1), dim3(1024), 0, h->stream, (const uint32_t*)dCnt, nFrames, dStart,
+                      dZ);
    if (!ok(h, hipGetLastError(), "collect launch"))
       return -1;
    // The kept emitters number at most the bucket entries (nIdx). With device outputs that hold nIdx records and no

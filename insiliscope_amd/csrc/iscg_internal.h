@@ -98,6 +98,10 @@ struct iscg_context
    int noiseExact = 0;  // iscg_noise_set_path: 1 evaluates every pixel with the FP64 chain
    iscg::DevBuf noiseItems;            // deferred pixels (NoiseQueue::items)
    uint32_t* noiseCount = nullptr;     // NoiseQueue::count, 2 words
+   // the last bucket offsets iscg_collect_frame_emitters validated from device memory (reused under capture)
+   const void* collectStPtr = nullptr;
+   uint32_t collectStFrames = 0;
+   size_t collectStTotal = 0;
    iscg::DevBuf bucketTmp[6];          // iscg_bucket_events: counts, offsets, keys and values (twice), sort scratch
    int tuneMode = 1;                   // iscg_autotune_set: 1 tunes launch parameters on first use of a shape key
    std::map<std::string, int> tuned;   // shape key -> chosen launch parameter

@@ -850,25 +850,30 @@ __global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfR
 }
 
 // Sum-1 normalisation per plane (Sim/ZernikePsf.cpp:401-410): FP64 sum, v = float(v / sum).
-__global__ __launch_bounds__(256) void psf_normalize_kernel(float* __restrict__ planes, int n2)
+// One 1024-lane workgroup per plane: the FP64 sum (per lane, then a wave shuffle tree and the 16 wave sums in
+// order), then v = (float)(v x (1 / sum)) in FP64 (within an FP64 ulp of the division, so the same float but at a
+// rounding tie).
+__global__ __launch_bounds__(1024) void psf_normalize_kernel(float* __restrict__ planes, int n2)
 {
-   __shared__ double part[256];
+   __shared__ double part[16];
    float* p = planes + (size_t)blockIdx.x * n2;
    double s = 0.0;
-   for (int i = threadIdx.x; i < n2; i += 256)
+   for (int i = threadIdx.x; i < n2; i += 1024)
       s += p[i];
-   part[threadIdx.x] = s;
+   for (int o = 32; o > 0; o >>= 1)
+      s += __shfl_xor(s, o);
+   if ((threadIdx.x & 63) == 0)
+      part[threadIdx.x >> 6] = s;
    __syncthreads();
-   for (int w = 128; w > 0; w >>= 1)
-   {
-      if ((int)threadIdx.x < w)
-         part[threadIdx.x] += part[threadIdx.x + w];
-      __syncthreads();
-   }
-   const double sum = part[0];
+   double sum = 0.0;
+   for (int w = 0; w < 16; ++w)
+      sum += part[w];
    if (sum > 0.0)
-      for (int i = threadIdx.x; i < n2; i += 256)
-         p[i] = (float)(p[i] / sum);
+   {
+      const double inv = 1.0 / sum;
+      for (int i = threadIdx.x; i < n2; i += 1024)
+         p[i] = (float)((double)p[i] * inv);
+   }
 }
 
 // ---- host ---------------------------------------------------------------------------------------------------------
@@ -1016,10 +1021,24 @@ static bool psf_planes_launch(iscg_context* h, const PsfBatch& s, int choice, fl
                                s.W.as<const _Float16>(), s.tz.as<const double>(), nz, Pp, nReq, 1, out);
          return;
       }
-      const int nYB = (Pp + YB * 16 - 1) / (YB * 16);
-      launch_planes<MP, YB>(h->stream, nReq * nz * nYB, s.reqs.as<const PsfReqDev>(), s.tabA.as<const double>(),
-                            s.tabB.as<const double>(), s.W.as<const _Float16>(), s.tz.as<const double>(), nz, Pp, nYB,
-                            out);
+      // row blocks of YB tiles, fewer when the planes would leave CUs idle (each block evaluates the pupil again,
+      // so the largest block count that still fills the 256 CUs)
+      auto items = [&](int yb) { return nReq * nz * ((Pp + yb * 16 - 1) / (yb * 16)); };
+      if (YB > 2 && items(YB) < 256 && items(2) >= 256)
+         launch_planes<MP, 2>(h->stream, items(2), s.reqs.as<const PsfReqDev>(), s.tabA.as<const double>(),
+                              s.tabB.as<const double>(), s.W.as<const _Float16>(), s.tz.as<const double>(), nz, Pp,
+                              (Pp + 31) / 32, out);
+      else if (YB > 1 && items(YB) < 256)
+         launch_planes<MP, 1>(h->stream, items(1), s.reqs.as<const PsfReqDev>(), s.tabA.as<const double>(),
+                              s.tabB.as<const double>(), s.W.as<const _Float16>(), s.tz.as<const double>(), nz, Pp,
+                              (Pp + 15) / 16, out);
+      else
+      {
+         const int nYB = (Pp + YB * 16 - 1) / (YB * 16);
+         launch_planes<MP, YB>(h->stream, nReq * nz * nYB, s.reqs.as<const PsfReqDev>(), s.tabA.as<const double>(),
+                               s.tabB.as<const double>(), s.W.as<const _Float16>(), s.tz.as<const double>(), nz, Pp,
+                               nYB, out);
+      }
    };
    if (Mp > 256)
    {
@@ -1186,7 +1205,7 @@ static int32_t psf_batch_run(iscg_context* h, const PsfBatch& b, int32_t normali
    if (normalize)
    {
       ProfScope ps(h, "psf_normalize");
-      hipLaunchKernelGGL(psf_normalize_kernel, dim3((unsigned)(nReq * nz)), dim3(256), 0, h->stream, dOut,
+      hipLaunchKernelGGL(psf_normalize_kernel, dim3((unsigned)(nReq * nz)), dim3(1024), 0, h->stream, dOut,
                          b.size * b.size);
       if (!ok(h, hipGetLastError(), "PSF normalisation"))
          return -1;

@@ -446,6 +446,45 @@ __global__ __launch_bounds__(kFftLines * kFftLane) void fft_shift_sums_kernel(co
    }
 }
 
+// The taps of one pixel (SplatRows' inner sums, Sim/SplatKernel.inl:80-167): sum_j wy[j] (sum_i wx[i] B[r0 + j][cc + i])
+// in FP64 over the taps inside the block sums, in j then i order. A 4 x 4 window wholly inside (the common case)
+// takes the unpredicated form: the same operations in the same order, so the same bits.
+__device__ __forceinline__ double tap_sum(const float* B, int bw, int r0, int cc, int NT, const double* wx,
+                                          const double* wy)
+{
+   double sum = 0.0;
+   if (NT == 4 && r0 >= 0 && cc >= 0 && r0 + 4 <= bw && cc + 4 <= bw)
+   {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+      {
+         const float* brow = B + (size_t)(r0 + j) * bw + cc;
+         double row = 0.0;
+#pragma unroll
+         for (int i = 0; i < 4; ++i)
+            row += wx[i] * brow[i];
+         sum += wy[j] * row;
+      }
+      return sum;
+   }
+   const int jLo = 0 > -r0 ? 0 : -r0, jHi = NT < bw - r0 ? NT : bw - r0;
+   const int iLo = 0 > -cc ? 0 : -cc, iHi = NT < bw - cc ? NT : bw - cc;
+#pragma unroll
+   for (int j = 0; j < 4; ++j)
+   {
+      if (j < jLo || j >= jHi)
+         continue;
+      const float* brow = B + (size_t)(r0 + j) * bw + cc;
+      double row = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+         if (i >= iLo && i < iHi)
+            row += wx[i] * brow[i];
+      sum += wy[j] * row;
+   }
+   return sum;
+}
+
 struct TileArgs
 {
    const SetupRec* recs;
@@ -609,6 +648,143 @@ __global__ __launch_bounds__(kBlock) void splat_tiles_kernel(TileArgs a, KernelT
                                a.readN ? a.readN[i] : 0.0f);
       }
       __syncthreads();
+   }
+}
+
+// The per-pixel form of the tile pass. Each lane owns PPL = TILE^2 / kBlock pixels of the tile (pixel tid + kBlock m)
+// and walks the tile's ordered hits itself, adding into registers; the hits' setup records are staged in LDS once per
+// chunk of kBlock emitters. The walk has no barrier, and a hit's record and tap loads do not wait on the previous
+// hit's adds, so the loads of consecutive hits overlap. A pixel receives the same values in the same event order as
+// in splat_tiles_kernel (the set of hits that reach it, the taps and their FP64 order are the same): the same bits.
+template <int TILE, bool NOISE, bool EMCCD>
+__global__ __launch_bounds__(kBlock) void splat_pix_kernel(TileArgs a, KernelTable kt, CamParams cp)
+{
+   static_assert(TILE * TILE % kBlock == 0, "whole pixels per lane");
+   constexpr int PPL = TILE * TILE / kBlock;
+   constexpr int kStage = 64;  // hits staged per pass: a small LDS footprint keeps more workgroups resident
+   __shared__ SetupRec rec[kStage];
+   __shared__ int list[kBlock];
+   __shared__ int waveHits[kBlock / 64];
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const size_t npx = (size_t)a.w * a.h;
+   const size_t items = (size_t)a.nTiles * a.nFrames;
+   for (size_t item = blockIdx.x; item < items; item += gridDim.x)
+   {
+      const uint32_t f = (uint32_t)(item / a.nTiles);
+      const uint32_t t = (uint32_t)(item - (size_t)f * a.nTiles);
+      const int tx0 = (int)(t % a.tilesX) * TILE, ty0 = (int)(t / a.tilesX) * TILE;
+      const float* extra = a.extras ? a.extras + (size_t)f * npx : nullptr;
+      const double bgScale = a.bgScales ? (double)a.bgScales[f] : 1.0;
+      const bool flat = a.bgMap == nullptr && bgScale == 1.0;
+      float acc[PPL];
+      int px[PPL], py[PPL];
+      bool in[PPL];
+#pragma unroll
+      for (int m = 0; m < PPL; ++m)
+      {
+         const int p = tid + kBlock * m;
+         px[m] = tx0 + (p & (TILE - 1));
+         py[m] = ty0 + p / TILE;
+         in[m] = px[m] < (int)a.w && py[m] < (int)a.h;
+         float v = 0.0f;
+         if (in[m])
+         {
+            const size_t i = (size_t)py[m] * a.w + px[m];
+            if (extra)
+               v = extra[i];
+            else if (flat)
+               v = (float)a.flatBg;
+            else
+               v = (float)((a.bgMap ? (double)a.bgMap[i] : a.flatBg) * bgScale);
+         }
+         acc[m] = v;
+      }
+      const uint32_t e1 = min(a.emStart[f + 1], a.nEmitters), e0 = min(a.emStart[f], e1);
+      const int R = kt.maxCamRad;
+      for (uint32_t c0 = e0; c0 < e1; c0 += kBlock)
+      {
+         const uint32_t e = c0 + tid;
+         bool hit = false;
+         if (e < e1)
+         {
+            const int2 q = a.pos[e];
+            hit = q.x + R >= tx0 && q.x - R <= tx0 + TILE - 1 && q.y + R >= ty0 && q.y - R <= ty0 + TILE - 1;
+         }
+         const unsigned long long mask = __ballot(hit);
+         if (lane == 0)
+            waveHits[wave] = __popcll(mask);
+         __syncthreads();
+         int before = 0, nHits = 0;
+         for (int w = 0; w < kBlock / 64; ++w)
+         {
+            before += w < wave ? waveHits[w] : 0;
+            nHits += waveHits[w];
+         }
+         if (hit)
+            list[before + __popcll(mask & ((1ull << lane) - 1ull))] = (int)e;
+         __syncthreads();
+         for (int b0 = 0; b0 < nHits; b0 += kStage)
+         {
+         const int nb = nHits - b0 < kStage ? nHits - b0 : kStage;
+         if (tid < nb)
+            rec[tid] = a.recs[list[b0 + tid]];
+         __syncthreads();
+         for (int k = 0; k < nb; ++k)
+         {
+            const SetupRec& r = rec[k];  // every lane reads the same record: an LDS broadcast
+            const int kk = __builtin_amdgcn_readfirstlane(r.kernel);
+            const int x0 = __builtin_amdgcn_readfirstlane(r.x0), y0 = __builtin_amdgcn_readfirstlane(r.y0);
+            const int camRad = kt.camRad[kk], os = kt.os[kk], bw = kt.bw[kk], off = os - 1;
+            const int NT = __builtin_amdgcn_readfirstlane(r.nTaps), cutR2 = __builtin_amdgcn_readfirstlane(r.cutR2);
+            const int plane = __builtin_amdgcn_readfirstlane(r.plane), bx = __builtin_amdgcn_readfirstlane(r.bx),
+                      by = __builtin_amdgcn_readfirstlane(r.by);
+            const float* B = __builtin_amdgcn_readfirstlane(r.pad[0]) == 1
+                                ? a.fftSums + (size_t)__builtin_amdgcn_readfirstlane(list[b0 + k]) * a.fftStride
+                                : kt.sums[kk] + (size_t)plane * bw * bw;
+            const int nSpan = 2 * camRad + 1;
+            const int16_t* lo = kt.lo[kk] ? kt.lo[kk] + (size_t)plane * nSpan : nullptr;
+            const int16_t* hi = kt.hi[kk] ? kt.hi[kk] + (size_t)plane * nSpan : nullptr;
+            const double photons = r.photons;
+            double wx[4], wy[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+            {
+               wx[i] = r.wx[i];
+               wy[i] = r.wy[i];
+            }
+#pragma unroll
+            for (int m = 0; m < PPL; ++m)
+            {
+               const int dx = px[m] - x0, dy = py[m] - y0;
+               if (!in[m] || dx < -camRad || dx > camRad || dy < -camRad || dy > camRad)
+                  continue;
+               if (lo && (dx < lo[dy + camRad] || dx > hi[dy + camRad]))
+                  continue;
+               if (dx * dx + dy * dy > cutR2)
+                  continue;
+               const int r0 = by + dy * os + off, cc = bx + dx * os + off;
+               const double sum = tap_sum(B, bw, r0, cc, NT, wx, wy);
+               acc[m] += (float)(photons * sum);
+            }
+         }
+         __syncthreads();  // the next pass overwrites rec, the next chunk's scan list
+         }
+      }
+#pragma unroll
+      for (int m = 0; m < PPL; ++m)
+      {
+         if (!in[m])
+            continue;
+         const size_t i = (size_t)py[m] * a.w + px[m];
+         const float v = acc[m];
+         if (a.outPhotons)
+            a.outPhotons[(size_t)f * npx + i] = v;
+         if (NOISE)
+            noise_store<EMCCD>(cp, a.q, a.outAdu, (uint32_t)((size_t)f * npx + i), v, a.seed, a.frames[f],
+                               (uint32_t)i + a.pixelBase, a.offset != nullptr, a.offset ? a.offset[i] : 0.0f,
+                               a.gain != nullptr, a.gain ? a.gain[i] : 0.0f, a.readN != nullptr,
+                               a.readN ? a.readN[i] : 0.0f);
+      }
    }
 }
 
@@ -801,9 +977,11 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
    // the same emitters in the same order whatever the tiling, so the frames are bit-identical for every choice;
    // the timing launches write the same outputs again (the deferred-noise queue is drained and reset by each
    // fallback launch). Not tuned when the caller's extra images are the photon output (in-place accumulation).
-   constexpr int kTile16 = 64;
+   // + kPixForm: the per-pixel form of the same pass (splat_pix_kernel), bit-identical as well.
+   constexpr int kTile16 = 64, kPixForm = 128;
    auto tiles = [&](int choice) {
-      const int perCu = choice % kTile16, edge = choice >= kTile16 ? 16 : kTile;
+      const int perCu = choice % kTile16, edge = (choice / kTile16) % 2 ? 16 : kTile;
+      const bool pix = choice >= kPixForm;
       TileArgs t = a;
       t.tilesX = (s.w + edge - 1) / edge;
       t.nTiles = t.tilesX * ((s.h + edge - 1) / edge);
@@ -811,6 +989,19 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
       const unsigned grid = (unsigned)(items < cap ? (items ? items : 1) : cap);
       auto go = [&](auto tile) {
          constexpr int T = decltype(tile)::value;
+         if (pix)
+         {
+            if (!dAdu)
+               hipLaunchKernelGGL((splat_pix_kernel<T, false, false>), dim3(grid), dim3(kBlock), 0, h->stream, t,
+                                  kt, cp);
+            else if (cp.emccd)
+               hipLaunchKernelGGL((splat_pix_kernel<T, true, true>), dim3(grid), dim3(kBlock), 0, h->stream, t, kt,
+                                  cp);
+            else
+               hipLaunchKernelGGL((splat_pix_kernel<T, true, false>), dim3(grid), dim3(kBlock), 0, h->stream, t,
+                                  kt, cp);
+            return;
+         }
          if (!dAdu)
             hipLaunchKernelGGL((splat_tiles_kernel<T, false, false>), dim3(grid), dim3(kBlock), 0, h->stream, t, kt,
                                cp);
@@ -842,9 +1033,10 @@ static int32_t render(iscg_context* h, const void* ems, bool records, uint32_t n
                               std::to_string(dAdu ? 1 + cp.emccd : 0) + " fft" + std::to_string(a.fftSums ? 1 : 0) +
                               " camRad" + std::to_string(kt.maxCamRad);
       std::vector<int> cands;
-      for (int t16 = 0; t16 < 2; ++t16)
-         for (int c : {4, 8, 16, 32})
-            cands.push_back(c + kTile16 * t16);
+      for (int form = 0; form < 2; ++form)
+         for (int t16 = 0; t16 < 2; ++t16)
+            for (int c : {4, 8, 16, 32})
+               cands.push_back(c + kTile16 * t16 + kPixForm * form);
       choice = tuned_choice(h, key, cands, 16, [&](int c) { return tiles(c) && fallback(); });
    }
    {
@@ -987,8 +1179,9 @@ int32_t iscg_splat_set_kernels(iscg_context* h, const iscg_kernel_view* kernels,
                  "span copy"))
             return -1;
       }
-      // the staged planes must outlive the block-sum kernel before the staging buffer is reused
-      if (!ok(h, hipStreamSynchronize(h->stream), "stream synchronize"))
+      // staged (host) planes must outlive the block-sum kernel before the staging buffer is reused; device planes
+      // are read in place, so nothing waits (the call can then be captured in a HIP graph)
+      if (dPlanes != v.planes && !ok(h, hipStreamSynchronize(h->stream), "stream synchronize"))
          return -1;
    }
    s.nk = n;

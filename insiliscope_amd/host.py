@@ -756,9 +756,14 @@ class SmlmMovieGpu:
     frame on the CPU (Sim/ScopeMovie.cpp:6183 buckets a movie's events once; :7248-7259 collects and renders each
     frame, :7642-7650 applies the camera noise): the events are uploaded and bucketed by frame once
     (BucketEventsByFrame, Sim/SMLMSimulation.cpp:203-217); each Render computes the PSF stack on the GPU
-    (ComputeZernikePsfPlanes + the sum-1 normalisation), its block sums, every frame's emitters
-    (CollectFrameEmitters, Sim/SMLMSimulation.cpp:55-101), and splats and noises all frames in one batch
-    (RenderPhotonImage + ApplyNoiseChain). Every buffer is a torch tensor on the context's device."""
+    (ComputeZernikePsfPlanes + the sum-1 normalisation, from a device-held PsfLibrary), its block sums, every frame's
+    emitters (CollectFrameEmitters, Sim/SMLMSimulation.cpp:55-101), and splats and noises all frames in one batch
+    (RenderPhotonImage + ApplyNoiseChain). Every buffer is a torch tensor on the context's device, and a render
+    waits for nothing on the host, so Capture() records it as one HIP graph that Replay() runs again.
+
+    The splat is handed the bucket total (the (frame, event) pairs, known after bucketing) as its emitter count:
+    the collect keeps at most that many, each frame's range comes from the device offsets, and the records beyond
+    the kept ones are never splatted (they start zeroed, so the setup pass reads valid indices)."""
 
     def __init__(self, events, nFrames: int, width: int, height: int, pixelSizeNm: float, photonsPerBlink: float,
                  flatBackground: float, offset=None, ctx: Context | None = None):
@@ -773,14 +778,20 @@ class SmlmMovieGpu:
         cap = 4 * len(ev7) + 16
         self.d_bidx = torch.empty(cap, dtype=torch.int32, device=dev)
         self.d_fi = torch.arange(self.nFrames, dtype=torch.int64, device=dev)
-        self.d_ems = torch.empty((cap, FRAME_EMITTER_DTYPE.itemsize), dtype=torch.uint8, device=dev)
         self.d_start = torch.empty(self.nFrames + 1, dtype=torch.int32, device=dev)
         self.d_frames = torch.arange(self.nFrames, dtype=torch.int32, device=dev)
         total = C.c_uint64(0)
         self.ctx.follow_torch()
-        self.ctx.check(self.ctx.L.iscg_bucket_events(self.ctx.h, self.d_ev.data_ptr(), self.nEvents, self.nFrames,
-                                                     self.d_bst.data_ptr(), self.d_bidx.data_ptr(), cap,
-                                                     C.byref(total)), "iscg_bucket_events")
+        rc = self.ctx.L.iscg_bucket_events(self.ctx.h, self.d_ev.data_ptr(), self.nEvents, self.nFrames,
+                                           self.d_bst.data_ptr(), self.d_bidx.data_ptr(), cap, C.byref(total))
+        if rc != 0 and total.value > cap:
+            self.d_bidx = torch.empty(int(total.value), dtype=torch.int32, device=dev)
+            rc = self.ctx.L.iscg_bucket_events(self.ctx.h, self.d_ev.data_ptr(), self.nEvents, self.nFrames,
+                                               self.d_bst.data_ptr(), self.d_bidx.data_ptr(), int(total.value),
+                                               C.byref(total))
+        self.ctx.check(rc, "iscg_bucket_events")
+        self.nPairs = int(total.value)
+        self.d_ems = torch.zeros((max(1, self.nPairs), FRAME_EMITTER_DTYPE.itemsize), dtype=torch.uint8, device=dev)
         self.splat = SplatGpuHost(self.ctx)
         if offset is not None and not hasattr(offset, "data_ptr"):
             offset = torch.from_numpy(np.ascontiguousarray(offset, np.float32)).to(dev)
@@ -789,20 +800,21 @@ class SmlmMovieGpu:
             raise IscgError(self.splat.err)
         self.planes = None
         self.cache = None
+        self.library = None
         self._req = None
         self._serial = 0
+        self.graph = None
 
     def Render(self, psfRequest, oversampling: int, zStepNm: float, interpMode: int, cam: CameraNoiseParams | None,
                seed: int, outAdu=None, outPhotons=None) -> bool:
         """The whole movie: PSF -> block sums -> emitters -> splat + noise into outAdu / outPhotons ((nFrames, h, w)
         uint16 / float32 tensors). False with splat.err set when a step fails."""
         import torch
-        if self._req is None or self._req is not psfRequest:
+        if self.library is None or self._req is not psfRequest:
             self._req = psfRequest
-            self._reqArr = PsfRequestArray([psfRequest])
-            size, nz, _ = PsfGeometry(self._reqArr[0])
-            self.planes = torch.empty((1, nz, size, size), dtype=torch.float32, device="cuda")
-        ComputePsfPlanesBatch(self._reqArr, normalize=True, out=self.planes, ctx=self.ctx)
+            self.library = PsfLibrary([psfRequest], ctx=self.ctx)
+            self.planes = torch.empty(self.library.shape, dtype=torch.float32, device="cuda")
+        self.library.Run(self.planes, normalize=True)
         self._serial += 1
         cache = PsfKernelCache(self.planes[0], oversampling=oversampling, zStepNm=zStepNm, interpMode=interpMode)
         cache.serial = self._serial  # a new stack: its block sums are made again
@@ -820,9 +832,30 @@ class SmlmMovieGpu:
             self.d_fi.data_ptr(), self.nFrames, self.width, self.height, self.pixelSizeNm, self.photonsPerBlink, None,
             cache.nz, float(cache.zStepNm), 0.0, 0, None, self.d_ems.data_ptr(), self.d_ems.shape[0],
             self.d_start.data_ptr(), None), "iscg_collect_frame_emitters")
-        n = int(self.d_start[-1].item())
         return self.splat.SetKernels([cache]) and self.splat.RenderBatch(
-            self.d_ems[:n], self.d_start, self.d_frames, cam, seed, outAdu=outAdu, outPhotons=outPhotons)
+            self.d_ems[: self.nPairs], self.d_start, self.d_frames, cam, seed, outAdu=outAdu, outPhotons=outPhotons)
+
+    def Capture(self, psfRequest, oversampling: int, zStepNm: float, interpMode: int, cam: CameraNoiseParams | None,
+                seed: int, outAdu=None, outPhotons=None):
+        """Records one Render (after an eager one on the capture stream: buffers made, launch choices tuned) as a
+        HIP graph (torch.cuda.CUDAGraph); Replay() then runs the whole movie again into the same outputs."""
+        import torch
+        stream = torch.cuda.Stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream):
+            if not self.Render(psfRequest, oversampling, zStepNm, interpMode, cam, seed, outAdu, outPhotons):
+                raise IscgError(self.splat.err)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            if not self.Render(psfRequest, oversampling, zStepNm, interpMode, cam, seed, outAdu, outPhotons):
+                raise IscgError(self.splat.err)
+        self.graph = g
+        self.ctx.set_stream(None)
+        return g
+
+    def Replay(self):
+        self.graph.replay()
 
     def Emitters(self):
         """The last Render's frame emitters (FRAME_EMITTER_DTYPE) and per-frame offsets, copied to the host."""

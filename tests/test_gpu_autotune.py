@@ -80,7 +80,8 @@ def test_psf_chunked_kernel_vs_oracle_every_candidate_identical(orc, shape):
 
 def test_splat_grid_every_candidate_identical(orc):
     """16 frames of 512x512 with Poisson(800) blinks, Catmull-Rom, EMCCD: the persistent grid's workgroups per CU
-    (4, 8, 16, 32) and the tile edge (32, or 16: + 64) give the same photons and ADU; frame 0 equals the oracle's."""
+    (4, 8, 16, 32), the tile edge (32, or 16: + 64) and the form (per hit, or per pixel: + 128) give the same photons
+    and ADU; frame 0 equals the oracle's."""
     req = configs.c4_psf(seed=1)
     req.update(nz=11, pupilSamples=64)
     planes = orc.psf_planes(O.psf_request(**req), normalize=True)
@@ -114,7 +115,7 @@ def test_splat_grid_every_candidate_identical(orc):
     adu, ph = render(c)
     ch = c.autotune_choices()
     key = _only_key(ch, "splat_tiles")
-    cands = [per_cu + 64 * t16 for t16 in (0, 1) for per_cu in (4, 8, 16, 32)]
+    cands = [per_cu + 64 * t16 + 128 * pix for pix in (0, 1) for t16 in (0, 1) for per_cu in (4, 8, 16, 32)]
     assert ch[key] in cands, ch
     assert np.array_equal(adu, want_adu) and np.array_equal(ph, want_ph)
     for cand in cands:

@@ -63,3 +63,18 @@ def test_c1_movie_end_to_end_vs_oracle(ctx, orc):
         hits3 += int(((span(x0) == 2) & (span(y0) == 2)).sum())
     print("blinks spanning 3 x 3 tiles:", hits3, flush=True)
     assert hits3 > 0
+
+    # the whole movie captured as one HIP graph (the benchmark's step): every replay bit-identical to the eager
+    # render, with eager work between replays
+    want_adu = torch.from_numpy(adu.view(np.int16)).cuda()
+    out = torch.zeros((50, h, w), dtype=torch.int16, device="cuda")
+    movie.Capture(req, 4, 50.0, 2, cam, 1, outAdu=out)
+    for k in range(3):
+        out.zero_()
+        movie.Replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, want_adu), f"replay {k}"
+        other = torch.empty_like(out)
+        assert movie.Render(req, 4, 50.0, 2, cam, 1, outAdu=other), movie.splat.err
+        torch.cuda.synchronize()
+        assert torch.equal(other, want_adu)
