@@ -584,9 +584,138 @@ int32_t iscg_persistent_bins(iscg_context* h, const iscg_dye_address* dyes, int3
                              int64_t binLo, int64_t binHi, int64_t* offsets, iscg_bin_blink* out, int64_t capacity,
                              int64_t* total);
 
-/* The device's jsm::sin (fn 0), cos (1), exp (2), log (3) (core/src/jsmath.h:15-22; fdlibm as V8 runs it) of n
- * values, the functions the schedules use; sin/cos for |x| <= 2^19 pi/2 (NaN beyond). */
+/* The device's jsm::sin (fn 0), cos (1), exp (2), log (3), cbrt (4) (core/src/jsmath.h:15-31; fdlibm as V8 runs
+ * it) of n values, and V8's Math.hypot of n pairs (fn 5: x holds 2n values) or triples (fn 6: 3n values), the
+ * functions the schedules and dye sites use; sin/cos for |x| <= 2^19 pi/2 (NaN beyond). */
 int32_t iscg_jsm(iscg_context* h, int32_t fn, const double* x, int64_t n, double* out);
+
+/* ---- the cell field's dye blocks and window queries --------------------------------------------------------------
+ * Replaces the block walk of isc::World (core/src/world.cpp:455-640) and the dye generation and queries it drives
+ * (GenerateDyes / GenerateNpcDyes / GenerateLaminDyes :808-895 over DyesInBlock / NpcDyesInBlock / LaminDyesInBlock,
+ * core/src/dyes.cpp:127-253; SitesInWindow :1212, EventsInWindow :1219, ContinuousInWindow :1279, Density3dInWindow
+ * :1519), for one set of kinetics per structure (no kinetics history, no clock map). The cells' geometry is the
+ * caller's (the reference's CellAssets: packed cells, microtubule centrelines, pores, lamina points); the device keeps
+ * it, builds the microtubule frames (BuildMtFrames, core/src/microtubules.cpp:701) and the dye blocks' reach data
+ * once, and each query selects the blocks that reach the window, generates their dyes on the reference's 0.1 nm
+ * lattice (World::DyeStore) and answers in the reference's order, bit for bit. */
+
+/* A cell of the field (isc::Cell, core/src/cells.h:48-75: the fields the walk reads). */
+typedef struct
+{
+   int32_t cx, cy;
+   double x, y, packRot, rOuter;
+} iscg_world_cell;
+
+/* A nuclear pore (isc::Npc, core/src/npcs.h:36-40): cell-local centre, outward normal, ring-plane axes (um), its
+ * candidate index. */
+typedef struct
+{
+   double c[3], n[3], e1[3], e2[3];
+   int32_t cand, reserved;
+} iscg_npc;
+
+/* A point of the nuclear lamina (isc::LaminPoint, core/src/npcs.h:44-48). */
+typedef struct
+{
+   double c[3], n[3];
+   int32_t cand, reserved;
+} iscg_lamin_point;
+
+/* The cells' geometry (host arrays; isc::CellAssets, core/src/world.h:58-90):
+ *   cells[nCells] in the walk's order (World::CellsInRectPtr: pack block x, then y, then the block's order);
+ *   cell c's microtubules are [mtStart[c], mtStart[c+1]) (nCells + 1 entries); microtubule m's centreline is the
+ *   points pts[3 ptStart[m] .. 3 ptStart[m+1]) (x, y, z, cell-local um) and mtIndex[m] its index in its cell (null:
+ *   its position in the cell's range);
+ *   cell c's pores npcs[npcStart[c] .. npcStart[c+1]) and lamina points lamin[laminStart[c] .. laminStart[c+1]), in
+ *   candidate order (null start arrays: none);
+ *   npcSites: NpcAttachmentPoints of the params (64 x 3, nm); npcSiteReachUm: NpcSiteReachUm. */
+typedef struct
+{
+   int32_t nCells;
+   const iscg_world_cell* cells;
+   const int32_t* mtStart;
+   const int64_t* ptStart;
+   const int32_t* mtIndex;
+   const double* pts;
+   const int32_t* npcStart;
+   const iscg_npc* npcs;
+   const int32_t* laminStart;
+   const iscg_lamin_point* lamin;
+   double npcSites[192];
+   double npcSiteReachUm;
+} iscg_world_geometry;
+
+/* A structure's label (isc::Label, core/src/dyes.h:163-176): which sites carry a dye and how it emits. mode 0
+ * dSTORM, 1 PALM, 2 DNA-PAINT, 3 WideField; preState PALM only; hasLinkage 0 takes the structure's built-in linkage
+ * (LegacyLinkage), and the linkage is normalised (NormLinkage) as World::SetLabel does. The orientation does not
+ * change the dyes or their emission and is not taken. */
+typedef struct
+{
+   double density, fluorescentFraction;
+   int32_t mode, preState;
+   iscg_kinetics kin;
+   int32_t hasLinkage, reserved;
+   double binderNm, binderFlexNm, linkerMinNm, linkerMaxNm;
+} iscg_label;
+
+/* A dye of SitesInWindow (isc::WorldDye, core/src/world.h:92-99). */
+typedef struct
+{
+   double x, y, z;
+   uint32_t id;
+   int32_t cx, cy, mtIndex, k, n, structure, reserved;
+} iscg_world_dye;
+
+/* A blink or continuous window (isc::WorldEvent, core/src/world.h:103-111, without its cache key). */
+typedef struct
+{
+   double x, y, z, tOn, tOff, brightness, aux;
+   uint32_t id;
+   uint8_t structure, state, reserved[2];
+} iscg_world_event;
+
+/* A dye of one walked block as the generators make it (isc::Dye / isc::NpcDye, core/src/dyes.h:263-300): the
+ * cell-local, unrounded position (before World::GenerateDyes' LocalToWorld and lattice), its id, the block's place
+ * in the walk, the structure, the element (microtubule index in its cell; a pore's or lamina point's index in its
+ * cell's list) and the site (k, n). */
+typedef struct
+{
+   double x, y, z;
+   uint32_t id;
+   int32_t block, structure, element, k, n, reserved;
+} iscg_block_dye;
+
+typedef struct iscg_world iscg_world;
+
+/* Keeps the geometry on the context's device (isc::World over its cell assets, core/src/world.h:152): the default
+ * labels are the reference's (density 0.7, every dye fluorescent, DNA-PAINT, isc::Kinetics' defaults, the built-in
+ * linkages). The world uses h's stream and reports errors on h; destroy it before h. */
+int32_t iscg_world_create(iscg_context* h, uint32_t seed, const iscg_world_geometry* geometry, iscg_world** out);
+int32_t iscg_world_destroy(iscg_world* w);
+/* World::SetLabel (core/src/world.cpp:1080) for structure 0 (microtubules), 1 (pores), 2 (lamina) or 3 (docking
+ * strands: no geometry here); -1 for an invalid label (as ValidateLabel, kinetics as the C ABI checks them). */
+int32_t iscg_world_set_label(iscg_world* w, int32_t structure, const iscg_label* label);
+
+/* The queries take the window as {x0, y0, x1, y1, zMin, zMax} (world um) and write their first min(*total, capacity)
+ * rows (host or device out; *total receives the count). */
+/* DyesInBlock / NpcDyesInBlock / LaminDyesInBlock (core/src/dyes.cpp:127-253) for every block the window's walk
+ * takes (World::ForEachDyeBlock's selection, core/src/world.cpp:556-589), blocks in walk order, each block's dyes in
+ * the generator's order, not filtered by the window. */
+int32_t iscg_world_block_dyes(iscg_world* w, const double window[6], iscg_block_dye* out, int64_t capacity,
+                              int64_t* total);
+/* World::SitesInWindow (core/src/world.cpp:1212). */
+int32_t iscg_world_sites(iscg_world* w, const double window[6], iscg_world_dye* out, int64_t capacity, int64_t* total);
+/* World::EventsInWindow (core/src/world.cpp:1219): the blinks overlapping [t0, t1), per dye block in walk order:
+ * dSTORM / PALM blinks by tOn (ties in dye and blink order), DNA-PAINT blinks in (dye, bin, index) order. */
+int32_t iscg_world_events(iscg_world* w, const double window[6], double t0, double t1, iscg_world_event* out,
+                          int64_t capacity, int64_t* total);
+/* World::ContinuousInWindow (core/src/world.cpp:1279): the continuous windows ending after tMin, in dye order. */
+int32_t iscg_world_continuous(iscg_world* w, const double window[6], double tMin, iscg_world_event* out,
+                              int64_t capacity, int64_t* total);
+/* World::Density3dInWindow (core/src/world.cpp:1519): counts on nx x ny x nz voxels, out[(k ny + iy) nx + ix] (host
+ * or device), structures of structureMask; *total receives the count. */
+int32_t iscg_world_density3d(iscg_world* w, const double window[6], int32_t nx, int32_t ny, int32_t nz,
+                             uint32_t structureMask, float* out, int64_t* total);
 
 #ifdef __cplusplus
 }

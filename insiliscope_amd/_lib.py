@@ -99,6 +99,60 @@ class Kinetics(C.Structure):
                                           "initialOnSec")]
 
 
+class WorldCell(C.Structure):
+    """iscg_world_cell: the fields of isc::Cell the block walk reads (core/src/cells.h:48-75)."""
+    _fields_ = [("cx", C.c_int32), ("cy", C.c_int32), ("x", C.c_double), ("y", C.c_double),
+                ("packRot", C.c_double), ("rOuter", C.c_double)]
+
+
+class Npc(C.Structure):
+    """iscg_npc = isc::Npc (core/src/npcs.h:36-40) without its surface piece."""
+    _fields_ = [("c", C.c_double * 3), ("n", C.c_double * 3), ("e1", C.c_double * 3), ("e2", C.c_double * 3),
+                ("cand", C.c_int32), ("reserved", C.c_int32)]
+
+
+class LaminPoint(C.Structure):
+    """iscg_lamin_point = isc::LaminPoint (core/src/npcs.h:44-48) without its surface piece."""
+    _fields_ = [("c", C.c_double * 3), ("n", C.c_double * 3), ("cand", C.c_int32), ("reserved", C.c_int32)]
+
+
+class WorldGeometry(C.Structure):
+    """iscg_world_geometry: the cells' assets (isc::CellAssets, core/src/world.h:58-90) as flat host arrays."""
+    _fields_ = [("nCells", C.c_int32), ("cells", C.c_void_p), ("mtStart", C.c_void_p), ("ptStart", C.c_void_p),
+                ("mtIndex", C.c_void_p), ("pts", C.c_void_p), ("npcStart", C.c_void_p), ("npcs", C.c_void_p),
+                ("laminStart", C.c_void_p), ("lamin", C.c_void_p), ("npcSites", C.c_double * 192),
+                ("npcSiteReachUm", C.c_double)]
+
+
+class Label(C.Structure):
+    """iscg_label = isc::Label's emission and linkage (core/src/dyes.h:163-176)."""
+    _fields_ = [("density", C.c_double), ("fluorescentFraction", C.c_double), ("mode", C.c_int32),
+                ("preState", C.c_int32), ("kin", Kinetics), ("hasLinkage", C.c_int32), ("reserved", C.c_int32),
+                ("binderNm", C.c_double), ("binderFlexNm", C.c_double), ("linkerMinNm", C.c_double),
+                ("linkerMaxNm", C.c_double)]
+
+
+class WorldDye(C.Structure):
+    """iscg_world_dye = isc::WorldDye (core/src/world.h:92-99)."""
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("id", C.c_uint32), ("cx", C.c_int32),
+                ("cy", C.c_int32), ("mtIndex", C.c_int32), ("k", C.c_int32), ("n", C.c_int32),
+                ("structure", C.c_int32), ("reserved", C.c_int32)]
+
+
+class BlockDye(C.Structure):
+    """iscg_block_dye: a generator's dye (isc::Dye / isc::NpcDye, core/src/dyes.h:263-300) with its walk block."""
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("id", C.c_uint32), ("block", C.c_int32),
+                ("structure", C.c_int32), ("element", C.c_int32), ("k", C.c_int32), ("n", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class WorldEvent(C.Structure):
+    """iscg_world_event = isc::WorldEvent without its cache key (core/src/world.h:103-111)."""
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("tOn", C.c_double), ("tOff", C.c_double),
+                ("brightness", C.c_double), ("aux", C.c_double), ("id", C.c_uint32), ("structure", C.c_uint8),
+                ("state", C.c_uint8), ("reserved", C.c_uint8 * 2)]
+
+
 _lib = None
 
 I32, U32, U64, D, P = C.c_int32, C.c_uint32, C.c_uint64, C.c_double, C.c_void_p
@@ -172,6 +226,14 @@ SIGNATURES = {
     "iscg_persistent_blinks": (I32, [P, P, I32, C.POINTER(Kinetics), D, D, P, P, C.c_int64, P]),
     "iscg_persistent_bins": (I32, [P, P, I32, C.POINTER(Kinetics), C.c_int64, C.c_int64, P, P, C.c_int64, P]),
     "iscg_jsm": (I32, [P, I32, P, C.c_int64, P]),
+    "iscg_world_create": (I32, [P, U32, C.POINTER(WorldGeometry), C.POINTER(P)]),
+    "iscg_world_destroy": (I32, [P]),
+    "iscg_world_set_label": (I32, [P, I32, C.POINTER(Label)]),
+    "iscg_world_block_dyes": (I32, [P, P, P, C.c_int64, P]),
+    "iscg_world_sites": (I32, [P, P, P, C.c_int64, P]),
+    "iscg_world_events": (I32, [P, P, D, D, P, C.c_int64, P]),
+    "iscg_world_continuous": (I32, [P, P, D, P, C.c_int64, P]),
+    "iscg_world_density3d": (I32, [P, P, I32, I32, I32, U32, P, P]),
 }
 
 

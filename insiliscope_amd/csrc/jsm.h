@@ -301,5 +301,90 @@ __device__ inline double sin(double x)
    }
 }
 
+// cbrt(x) as fdlibm's successor in FreeBSD msun computes it (the form V8 runs for Math.cbrt): a 5-bit estimate t
+// from the exponent (hx / 3 + B1, or via 2^54 for subnormals), a polynomial P0..P4 in r = t^3 / x to 23 bits, t
+// rounded away from zero to 23 bits, then one Newton step t + t (x / t^2 - t) / (2 t + x / t^2).
+__device__ inline double cbrt(double x)
+{
+   constexpr uint32_t B1 = 715094163, B2 = 696219795;
+   constexpr double P0 = 1.87595182427177009643, P1 = -1.88497979543377169875, P2 = 1.621429720105354466140,
+                    P3 = -0.758397934778766047437, P4 = 0.145996192886612446982;
+   int32_t hx = hi_of(x);
+   const uint32_t low = lo_of(x);
+   const uint32_t sign = static_cast<uint32_t>(hx) & 0x80000000u;
+   hx = static_cast<int32_t>(static_cast<uint32_t>(hx) ^ sign);
+   if (hx >= 0x7ff00000)
+      return x + x;
+   double t;
+   if (hx < 0x00100000)
+   {
+      if ((static_cast<uint32_t>(hx) | low) == 0)
+         return x;
+      t = of_words(0x43500000u, 0u) * x;
+      const uint32_t high = static_cast<uint32_t>(hi_of(t));
+      t = of_words(sign | ((high & 0x7fffffffu) / 3 + B2), 0u);
+   }
+   else
+   {
+      t = of_words(sign | (static_cast<uint32_t>(hx) / 3 + B1), 0u);
+   }
+   double r = (t * t) * (t / x);
+   t = t * ((P0 + r * (P1 + r * P2)) + ((r * r) * r) * (P3 + r * P4));
+   const unsigned long long bits =
+      (static_cast<unsigned long long>(__double_as_longlong(t)) + 0x80000000ull) & 0xffffffffc0000000ull;
+   t = __longlong_as_double(static_cast<long long>(bits));
+   const double s = t * t;
+   r = x / s;
+   const double w = t + t;
+   r = (r - t) / (w + r);
+   return t + t * r;
+}
+
+// V8's two-argument Math.hypot: both normalised by the larger, sqrt of the sum of squares, times it.
+__device__ inline double hypot(double x, double y)
+{
+   const double a = fabs(x), b = fabs(y);
+   if (isinf(a) || isinf(b))
+      return __builtin_inf();
+   if (isnan(a) || isnan(b))
+      return __builtin_nan("");
+   const double mx = a > b ? a : b;
+   if (mx == 0)
+      return 0;
+   const double an = a / mx, bn = b / mx;
+   return sqrt(an * an + bn * bn) * mx;
+}
+
+// V8's generic Math.hypot of three: normalised by the largest, a Kahan-compensated sum of squares in argument order.
+__device__ inline double hypot(double x, double y, double z)
+{
+   const double v[3] = {fabs(x), fabs(y), fabs(z)};
+   bool nan = false;
+   double mx = 0;
+   for (int i = 0; i < 3; ++i)
+   {
+      if (isnan(v[i]))
+         nan = true;
+      else if (v[i] > mx)
+         mx = v[i];
+   }
+   if (isinf(mx))
+      return __builtin_inf();
+   if (nan)
+      return __builtin_nan("");
+   if (mx == 0)
+      return 0;
+   double sum = 0, comp = 0;
+   for (int i = 0; i < 3; ++i)
+   {
+      const double n = v[i] / mx;
+      const double summand = n * n - comp;
+      const double pre = sum + summand;
+      comp = (pre - sum) - summand;
+      sum = pre;
+   }
+   return sqrt(sum) * mx;
+}
+
 }  // namespace jsm
 }  // namespace iscg

@@ -1473,11 +1473,21 @@ def Kinetics(activationRatePerSec: float = 0.01, onSec: float = 0.05, offSec: fl
 
 @dataclass
 class Label:
-    """The fields of isc::Label (core/src/dyes.h:166-179) a schedule reads: mode, kinetics, PALM pre state."""
+    """The fields of isc::Label (core/src/dyes.h:166-179) the schedules and the world read: mode, kinetics, PALM pre
+    state, the labelled density and fluorescent fraction, and the linkage (binderNm, binderFlexNm, linkerMinNm,
+    linkerMaxNm; None: the structure's built-in one, LegacyLinkage). Defaults are the reference's."""
 
     mode: str = "DnaPaint"
     kin: _lib.Kinetics = field(default_factory=Kinetics)
     preState: bool = False
+    density: float = 0.7
+    fluorescentFraction: float = 1.0
+    linkage: tuple | None = None
+
+    def to_c(self) -> _lib.Label:
+        lk = self.linkage if self.linkage is not None else (0.0, 0.0, 0.0, 0.0)
+        return _lib.Label(float(self.density), float(self.fluorescentFraction), LABEL_MODES[self.mode],
+                          int(self.preState), self.kin, int(self.linkage is not None), 0, *(float(v) for v in lk))
 
 
 # core/src/dyes.h:33-41
@@ -1590,11 +1600,171 @@ def PersistentBlinksInBins(dyes, kin: _lib.Kinetics, binLo: int, binHi: int, ctx
 
 
 def JsMath(fn: str, x, ctx: Context | None = None):
-    """The device's jsm::sin / cos / exp / log (core/src/jsmath.h) of each element of x (float64)."""
+    """The device's jsm::sin / cos / exp / log / cbrt (core/src/jsmath.h) of each element of x (float64), or V8's
+    Math.hypot ("hypot2" / "hypot3") of each row of x (n x 2 or n x 3)."""
     ctx = ctx or default_context()
     x = np.ascontiguousarray(x, np.float64)
-    out = np.empty_like(x)
+    fns = ("sin", "cos", "exp", "log", "cbrt", "hypot2", "hypot3")
+    k = fns.index(fn)
+    n = x.shape[0] if k >= 5 else x.size
+    out = np.empty(n) if k >= 5 else np.empty_like(x)
     ctx.follow_torch()
-    ctx.check(ctx.L.iscg_jsm(ctx.h, ("sin", "cos", "exp", "log").index(fn), x.ctypes.data, x.size,
-                             out.ctypes.data), "iscg_jsm")
+    ctx.check(ctx.L.iscg_jsm(ctx.h, k, x.ctypes.data, n, out.ctypes.data), "iscg_jsm")
     return out
+
+
+# ---- the cell field's dye blocks and window queries ----------------------------------------------------------------
+
+# isc::World's structures (core/src/world.h:54-58) and the rows of its queries (iscg_world_dye, iscg_world_event)
+STRUCTURE_MT, STRUCTURE_NPC, STRUCTURE_LAMIN, STRUCTURE_ORIGAMI = 0, 1, 2, 3
+WORLD_CELL_DTYPE = np.dtype({"names": ["cx", "cy", "x", "y", "packRot", "rOuter"],
+                             "formats": [np.int32, np.int32, np.float64, np.float64, np.float64, np.float64],
+                             "offsets": [0, 4, 8, 16, 24, 32], "itemsize": 40})
+NPC_DTYPE = np.dtype({"names": ["c", "n", "e1", "e2", "cand"],
+                      "formats": [(np.float64, 3), (np.float64, 3), (np.float64, 3), (np.float64, 3), np.int32],
+                      "offsets": [0, 24, 48, 72, 96], "itemsize": 104})
+LAMIN_DTYPE = np.dtype({"names": ["c", "n", "cand"], "formats": [(np.float64, 3), (np.float64, 3), np.int32],
+                        "offsets": [0, 24, 48], "itemsize": 56})
+WORLD_DYE_DTYPE = np.dtype({"names": ["x", "y", "z", "id", "cx", "cy", "mtIndex", "k", "n", "structure"],
+                            "formats": [np.float64] * 3 + [np.uint32] + [np.int32] * 6,
+                            "offsets": [0, 8, 16, 24, 28, 32, 36, 40, 44, 48], "itemsize": 56})
+BLOCK_DYE_DTYPE = np.dtype({"names": ["x", "y", "z", "id", "block", "structure", "element", "k", "n"],
+                            "formats": [np.float64] * 3 + [np.uint32] + [np.int32] * 5,
+                            "offsets": [0, 8, 16, 24, 28, 32, 36, 40, 44], "itemsize": 56})
+WORLD_EVENT_DTYPE = np.dtype({"names": ["x", "y", "z", "tOn", "tOff", "brightness", "aux", "id", "structure", "state"],
+                              "formats": [np.float64] * 7 + [np.uint32, np.uint8, np.uint8],
+                              "offsets": [0, 8, 16, 24, 32, 40, 48, 56, 60, 61], "itemsize": 64})
+
+
+@dataclass
+class WorldGeometry:
+    """A cell field's cells as isc::CellAssets holds them (core/src/world.h:58-90), flattened: cells in the walk's
+    order (WORLD_CELL_DTYPE), cell c's microtubules [mtStart[c], mtStart[c+1]) with microtubule m's centreline
+    pts[ptStart[m]:ptStart[m+1]] (cell-local um) and mtIndex[m] its index in the cell (None: its position), the
+    pores (NPC_DTYPE) and lamina points (LAMIN_DTYPE) of cell c at [npcStart[c], npcStart[c+1]) and [laminStart[c],
+    laminStart[c+1]) in candidate order, NpcAttachmentPoints (64 x 3 nm) and NpcSiteReachUm."""
+
+    cells: np.ndarray
+    mtStart: np.ndarray
+    ptStart: np.ndarray
+    pts: np.ndarray
+    mtIndex: np.ndarray | None = None
+    npcStart: np.ndarray | None = None
+    npcs: np.ndarray | None = None
+    laminStart: np.ndarray | None = None
+    lamin: np.ndarray | None = None
+    npcSites: np.ndarray = field(default_factory=lambda: np.zeros(192))
+    npcSiteReachUm: float = 0.0
+
+
+class World:
+    """isc::World's dye blocks and window queries (core/src/world.h:152-300) over a cell field's geometry kept on the
+    GPU: SitesInWindow, EventsInWindow, ContinuousInWindow, Density3dInWindow and DensityInWindow answer as the
+    reference's do, bit for bit and in its order, for one set of kinetics per structure (no kinetics history)."""
+
+    def __init__(self, seed: int, geometry: WorldGeometry, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        g = geometry
+        keep = []
+
+        def arr(a, dtype):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype)
+            keep.append(a)
+            return a.ctypes.data
+
+        cells = np.ascontiguousarray(g.cells, WORLD_CELL_DTYPE)
+        keep.append(cells)
+        c = _lib.WorldGeometry()
+        c.nCells = cells.size
+        c.cells = cells.ctypes.data if cells.size else None
+        c.mtStart = arr(g.mtStart, np.int32)
+        c.ptStart = arr(g.ptStart, np.int64)
+        c.mtIndex = arr(g.mtIndex, np.int32)
+        c.pts = arr(np.asarray(g.pts, np.float64).reshape(-1), np.float64)
+        c.npcStart = arr(g.npcStart, np.int32)
+        c.npcs = arr(g.npcs, NPC_DTYPE) if g.npcs is not None and len(g.npcs) else None
+        c.laminStart = arr(g.laminStart, np.int32)
+        c.lamin = arr(g.lamin, LAMIN_DTYPE) if g.lamin is not None and len(g.lamin) else None
+        c.npcSites[:] = [float(v) for v in np.asarray(g.npcSites, np.float64).reshape(-1)[:192]]
+        c.npcSiteReachUm = float(g.npcSiteReachUm)
+        h = C.c_void_p()
+        self.ctx.follow_torch()
+        self.ctx.check(self.ctx.L.iscg_world_create(self.ctx.h, int(seed) & 0xFFFFFFFF, C.byref(c), C.byref(h)),
+                       "iscg_world_create")
+        self.w = h.value
+        self._cap = {}
+
+    def close(self):
+        if getattr(self, "w", None):
+            self.ctx.L.iscg_world_destroy(self.w)
+            self.w = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def SetLabel(self, structure: int, label: Label) -> None:
+        """World::SetLabel (core/src/world.cpp:1080)."""
+        lab = label.to_c()
+        self.ctx.check(self.ctx.L.iscg_world_set_label(self.w, int(structure), C.byref(lab)), "iscg_world_set_label")
+
+    def _rows(self, fn, key, dtype, args):
+        win = np.ascontiguousarray(args[0], np.float64)
+        total = np.zeros(1, np.int64)
+        cap = self._cap.get(key, 1 << 16)
+        self.ctx.follow_torch()
+        for _ in range(2):
+            out = np.zeros(cap, dtype)
+            self.ctx.check(fn(self.w, win.ctypes.data, *args[1:], out.ctypes.data, cap, total.ctypes.data),
+                           fn.__name__)
+            if total[0] <= cap:
+                self._cap[key] = max(cap, int(total[0]))
+                return out[: total[0]]
+            cap = int(total[0])
+        raise IscgError(f"{fn.__name__}: row count changed between calls")
+
+    def BlockDyes(self, x0, y0, x1, y1, zMin, zMax):
+        """DyesInBlock / NpcDyesInBlock / LaminDyesInBlock (core/src/dyes.cpp:127-253) of every block the window's
+        walk takes: BLOCK_DYE_DTYPE rows, cell-local and unrounded, blocks in walk order."""
+        return self._rows(self.ctx.L.iscg_world_block_dyes, "blocks", BLOCK_DYE_DTYPE,
+                          ((x0, y0, x1, y1, zMin, zMax),))
+
+    def SitesInWindow(self, x0, y0, x1, y1, zMin, zMax):
+        """World::SitesInWindow (core/src/world.cpp:1212): WORLD_DYE_DTYPE rows."""
+        return self._rows(self.ctx.L.iscg_world_sites, "sites", WORLD_DYE_DTYPE, ((x0, y0, x1, y1, zMin, zMax),))
+
+    def EventsInWindow(self, x0, y0, x1, y1, zMin, zMax, t0, t1):
+        """World::EventsInWindow (core/src/world.cpp:1219): WORLD_EVENT_DTYPE rows."""
+        return self._rows(self.ctx.L.iscg_world_events, "events", WORLD_EVENT_DTYPE,
+                          ((x0, y0, x1, y1, zMin, zMax), float(t0), float(t1)))
+
+    def ContinuousInWindow(self, x0, y0, x1, y1, zMin, zMax, tMin):
+        """World::ContinuousInWindow (core/src/world.cpp:1279): WORLD_EVENT_DTYPE rows."""
+        return self._rows(self.ctx.L.iscg_world_continuous, "cont", WORLD_EVENT_DTYPE,
+                          ((x0, y0, x1, y1, zMin, zMax), float(tMin)))
+
+    def Density3dInWindow(self, x0, y0, x1, y1, zMin, zMax, nx: int, ny: int, nz: int, structureMask: int = 0xF,
+                          out=None):
+        """World::Density3dInWindow (core/src/world.cpp:1519): (total, counts[nz, ny, nx] float32). out may be a
+        device tensor of nx * ny * nz float32."""
+        win = np.array([x0, y0, x1, y1, zMin, zMax], np.float64)
+        if out is None:
+            out = np.zeros((nz, ny, nx), np.float32)
+        if _numel(out) != nx * ny * nz:
+            raise IscgError("Density3dInWindow: out must hold nx * ny * nz values")
+        p, keep = _ptr(out)
+        total = np.zeros(1, np.int64)
+        self.ctx.follow_torch()
+        self.ctx.check(self.ctx.L.iscg_world_density3d(self.w, win.ctypes.data, int(nx), int(ny), int(nz),
+                                                       int(structureMask), p, total.ctypes.data),
+                       "iscg_world_density3d")
+        return int(total[0]), out
+
+    def DensityInWindow(self, x0, y0, x1, y1, zMin, zMax, nx: int, ny: int, out=None):
+        """World::DensityInWindow (core/src/world.cpp:1510): every structure, one plane."""
+        total, out = self.Density3dInWindow(x0, y0, x1, y1, zMin, zMax, nx, ny, 1, 0xF, out)
+        return total, out.reshape(ny, nx) if isinstance(out, np.ndarray) else out

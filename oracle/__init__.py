@@ -850,6 +850,261 @@ class Reference:
         arr = (PsfRequest * len(reqs))(*reqs)
         return self.L.ref_psf_library(arr, len(reqs), _p(out, C.c_float))
 
+    # ---- the world (core/src/world.{h,cpp}) --------------------------------------------------------------------
+
+    def _world_sigs(self):
+        L = self.L
+        if getattr(self, "_world_ready", False):
+            return L
+        D, I32, U32, I64, P = C.c_double, C.c_int32, C.c_uint32, C.c_int64, C.c_void_p
+        PD, PI32, PF = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_float)
+        L.ref_world_new.argtypes = [U32, C.POINTER(C.c_char_p), PD, I32]
+        L.ref_world_new.restype = P
+        L.ref_world_free.argtypes = [P]
+        L.ref_world_set_label.argtypes = [P, I32, PD]
+        L.ref_world_set_label.restype = I32
+        L.ref_world_get_label.argtypes = [P, I32, PD]
+        L.ref_world_cells.argtypes = [P, D, D, D, D, PD, I32]
+        L.ref_world_cells.restype = I32
+        L.ref_world_cell_mts.argtypes = [P, I32, I32, PD, I64, PI32, I32, PI32]
+        L.ref_world_cell_mts.restype = I64
+        L.ref_mt_frames.argtypes = [PD, I32, PD, PD, PD, PD]
+        L.ref_world_cell_envelope.argtypes = [P, I32, I32, I32, PD, I32]
+        L.ref_world_cell_envelope.restype = I32
+        L.ref_world_npc_sites.argtypes = [P, PD, PD]
+        L.ref_world_sites.argtypes = [P, D, D, D, D, D, D, PD, I64]
+        L.ref_world_sites.restype = I64
+        L.ref_world_events.argtypes = [P, I32, D, D, D, D, D, D, D, D, PD, I64]
+        L.ref_world_events.restype = I64
+        L.ref_world_density3d.argtypes = [P, D, D, D, D, D, D, I32, I32, I32, U32, PF]
+        L.ref_world_density3d.restype = I64
+        L.ref_dyes_in_block.argtypes = [U32, I32, I32, I32, PD, I32, I32, D, D, PD, PD, I32]
+        L.ref_dyes_in_block.restype = I32
+        L.ref_jsm_geom.argtypes = [I32, PD, PD, PD, I64, PD]
+        L.ref_case_mts.argtypes = [U32, C.POINTER(C.c_char_p), PD, I32, I32, I32, PD, I64, PI32, I32, PI32]
+        L.ref_case_mts.restype = I64
+        L.ref_case_envelope.argtypes = [U32, C.POINTER(C.c_char_p), PD, I32, I32, I32, I32, PD, I32, PD, PD]
+        L.ref_case_envelope.restype = I32
+        L.ref_envelope_dyes.argtypes = [I32, U32, I32, I32, PD, I32, PI32, I32, PD, D, D, PD, PD, I32]
+        L.ref_envelope_dyes.restype = I32
+        L.ref_mt_site_geometry.argtypes = [PD, I32, D, D, D, D, D, PD]
+        L.ref_mt_protofilament.argtypes = [D, I32, PD, PD]
+        self._world_ready = True
+        return L
+
+    @staticmethod
+    def _params(params):
+        params = params or {}
+        names = (C.c_char_p * max(1, len(params)))(*[k.encode() for k in params])
+        vals = np.array([float(v) for v in params.values()] or [0.0])
+        return names, vals, len(params)
+
+    def case_mts(self, seed, params, cx, cy):
+        """BuildMicrotubulesForCell of the raw candidate cell (cx, cy): (points (n, 3), per-microtubule counts)."""
+        L = self._world_sigs()
+        names, vals, n = self._params(params)
+        nm = C.c_int32()
+        tot = L.ref_case_mts(seed & 0xFFFFFFFF, names, _p(vals, C.c_double), n, cx, cy, None, 0, None, 0,
+                             C.byref(nm))
+        if tot < 0:
+            raise ValueError("no cell there, or an unknown parameter")
+        xyz = np.zeros((max(tot, 1), 3))
+        lens = np.zeros(max(nm.value, 1), np.int32)
+        L.ref_case_mts(seed & 0xFFFFFFFF, names, _p(vals, C.c_double), n, cx, cy, _p(xyz, C.c_double), tot,
+                       _p(lens, C.c_int32), nm.value, C.byref(nm))
+        return xyz[:tot], lens[: nm.value]
+
+    def case_envelope(self, seed, params, cx, cy, which):
+        """BuildNpcsForCell (which 1: rows c, n, e1, e2, cand, piece) or BuildLaminForCell (2: c, n, cand, piece) of
+        the raw candidate cell, with NpcAttachmentPoints and NpcSiteReachUm: (rows, sites, reach)."""
+        L = self._world_sigs()
+        names, vals, n = self._params(params)
+        width = 14 if which == 1 else 8
+        sites = np.zeros(192)
+        reach = C.c_double()
+        cnt = L.ref_case_envelope(seed & 0xFFFFFFFF, names, _p(vals, C.c_double), n, cx, cy, which, None, 0,
+                                  _p(sites, C.c_double), C.byref(reach))
+        if cnt < 0:
+            raise ValueError("no cell there, or an unknown parameter")
+        rows = np.zeros((max(cnt, 1), width))
+        L.ref_case_envelope(seed & 0xFFFFFFFF, names, _p(vals, C.c_double), n, cx, cy, which, _p(rows, C.c_double),
+                            cnt, _p(sites, C.c_double), C.byref(reach))
+        return rows[:cnt], sites, reach.value
+
+    def envelope_dyes(self, which, seed, cx, cy, rows, ids, sites, density, ff, linkage):
+        """NpcDyesInBlock (which 1) or LaminDyesInBlock (2): rows pore, k, n, id, pos, site, rigid, tip."""
+        L = self._world_sigs()
+        rows = np.ascontiguousarray(rows, np.float64)
+        ids = np.ascontiguousarray(ids, np.int32)
+        sites = np.ascontiguousarray(sites, np.float64)
+        lk = np.ascontiguousarray(linkage, np.float64)
+        cap = max(1, len(ids) * (64 if which == 1 else 1))
+        out = np.zeros((cap, 16))
+        n = L.ref_envelope_dyes(which, seed & 0xFFFFFFFF, cx, cy, _p(rows, C.c_double), len(rows),
+                                _p(ids, C.c_int32), len(ids), _p(sites, C.c_double), density, ff,
+                                _p(lk, C.c_double), _p(out, C.c_double), cap)
+        return out[:n]
+
+    def mt_site_geometry(self, pts, S, theta, r1, r2, r3):
+        """MtSiteGeometry: (att, tip, dye) as 9 doubles."""
+        pts = np.ascontiguousarray(pts, np.float64)
+        out = np.zeros(9)
+        self._world_sigs().ref_mt_site_geometry(_p(pts, C.c_double), len(pts), S, theta, r1, r2, r3,
+                                                _p(out, C.c_double))
+        return out
+
+    def mt_protofilament(self, phase, k):
+        th, off = C.c_double(), C.c_double()
+        self._world_sigs().ref_mt_protofilament(phase, k, C.byref(th), C.byref(off))
+        return th.value, off.value
+
+    def world_new(self, seed: int, params: dict | None = None):
+        """A reference World (core/src/world.h:152) of seed with the default params and these overrides."""
+        L = self._world_sigs()
+        params = params or {}
+        names = (C.c_char_p * max(1, len(params)))(*[k.encode() for k in params])
+        vals = np.array([float(v) for v in params.values()] or [0.0])
+        w = L.ref_world_new(seed & 0xFFFFFFFF, names, _p(vals, C.c_double), len(params))
+        if not w:
+            raise ValueError(f"unknown world parameter in {params}")
+        return w
+
+    def world_free(self, w):
+        self._world_sigs().ref_world_free(w)
+
+    @staticmethod
+    def label_vector(density=0.7, fluorescentFraction=1.0, mode=2, kin=(0.01, 0.05, 1.0, 1.0, 0.5, 0.0),
+                     preState=0, linkage=None):
+        """A label in the C ABI's ISC_LABEL layout (core/include/insiliscope/insiliscope.h:218-278)."""
+        lk = linkage if linkage is not None else (np.nan,) * 4
+        return np.array([density, fluorescentFraction, mode, *kin, preState, 0, 90, 0, 0, 0, 0, *lk], np.float64)
+
+    def world_set_label(self, w, structure: int, vec) -> int:
+        vec = np.ascontiguousarray(vec, np.float64)
+        return self._world_sigs().ref_world_set_label(w, structure, _p(vec, C.c_double))
+
+    def world_get_label(self, w, structure: int):
+        out = np.zeros(14)
+        self._world_sigs().ref_world_get_label(w, structure, _p(out, C.c_double))
+        return out
+
+    def world_cells(self, w, x0, y0, x1, y1):
+        """World::CellsInRect: rows cx, cy, x, y, packRot, rOuter in its order."""
+        L = self._world_sigs()
+        out = np.zeros((512, 6))
+        n = L.ref_world_cells(w, x0, y0, x1, y1, _p(out, C.c_double), 512)
+        if n > 512:
+            out = np.zeros((n, 6))
+            L.ref_world_cells(w, x0, y0, x1, y1, _p(out, C.c_double), n)
+        return out[:n]
+
+    def world_cell_mts(self, w, cx, cy):
+        """The microtubules of a cell (World::Assets): (points (n, 3), per-microtubule point counts)."""
+        L = self._world_sigs()
+        nm = C.c_int32()
+        tot = L.ref_world_cell_mts(w, cx, cy, None, 0, None, 0, C.byref(nm))
+        xyz = np.zeros((max(tot, 1), 3))
+        lens = np.zeros(max(nm.value, 1), np.int32)
+        L.ref_world_cell_mts(w, cx, cy, _p(xyz, C.c_double), tot, _p(lens, C.c_int32), nm.value, C.byref(nm))
+        return xyz[:tot], lens[: nm.value]
+
+    def world_cell_envelope(self, w, which: int, cx, cy):
+        """The pores (which 1: rows c, n, e1, e2, cand) or lamina points (2: c, n, cand) of a cell."""
+        L = self._world_sigs()
+        width = 13 if which == 1 else 7
+        n = L.ref_world_cell_envelope(w, which, cx, cy, None, 0)
+        out = np.zeros((max(n, 1), width))
+        if n > 0:
+            L.ref_world_cell_envelope(w, which, cx, cy, _p(out, C.c_double), n)
+        return out[: max(n, 0)]
+
+    def world_npc_sites(self, w):
+        sites = np.zeros(192)
+        r = C.c_double()
+        self._world_sigs().ref_world_npc_sites(w, _p(sites, C.c_double), C.byref(r))
+        return sites, r.value
+
+    def world_geometry(self, w, x0, y0, x1, y1, npcs=True, lamin=True):
+        """The geometry of the cells CellsInRect returns for the rect, as the fields of the product's
+        WorldGeometry (a dict): cells, microtubule ranges and points, pores and lamina points."""
+        rows = self.world_cells(w, x0, y0, x1, y1)
+        cells = np.zeros(len(rows), [("cx", "<i4"), ("cy", "<i4"), ("x", "<f8"), ("y", "<f8"), ("packRot", "<f8"),
+                                     ("rOuter", "<f8")])
+        mtStart, ptStart, pts = [0], [0], []
+        npcStart, laminStart, npcRows, laminRows = [0], [0], [], []
+        for i, r in enumerate(rows):
+            cells[i] = (int(r[0]), int(r[1]), r[2], r[3], r[4], r[5])
+            xyz, lens = self.world_cell_mts(w, int(r[0]), int(r[1]))
+            pts.append(xyz)
+            for ln in lens:
+                ptStart.append(ptStart[-1] + int(ln))
+            mtStart.append(mtStart[-1] + len(lens))
+            pr = self.world_cell_envelope(w, 1, int(r[0]), int(r[1])) if npcs else np.zeros((0, 13))
+            lr = self.world_cell_envelope(w, 2, int(r[0]), int(r[1])) if lamin else np.zeros((0, 7))
+            npcRows.append(pr)
+            laminRows.append(lr)
+            npcStart.append(npcStart[-1] + len(pr))
+            laminStart.append(laminStart[-1] + len(lr))
+        sites, reach = self.world_npc_sites(w)
+        npcRows = np.concatenate(npcRows) if npcRows else np.zeros((0, 13))
+        laminRows = np.concatenate(laminRows) if laminRows else np.zeros((0, 7))
+        return dict(cells=cells, mtStart=np.array(mtStart, np.int32), ptStart=np.array(ptStart, np.int64),
+                    pts=np.concatenate(pts) if pts else np.zeros((0, 3)), npcStart=np.array(npcStart, np.int32),
+                    npcRows=npcRows, laminStart=np.array(laminStart, np.int32), laminRows=laminRows,
+                    npcSites=sites, npcSiteReachUm=reach)
+
+    def world_sites(self, w, win):
+        L = self._world_sigs()
+        n = L.ref_world_sites(w, *win, None, 0)
+        out = np.zeros((max(n, 1), 10))
+        L.ref_world_sites(w, *win, _p(out, C.c_double), n)
+        return out[:n]
+
+    def world_events(self, w, win, t0, t1, continuous=False):
+        """World::EventsInWindow (continuous False; t0, t1) or ContinuousInWindow (tMin = t0): rows x, y, z, tOn,
+        tOff, brightness, aux, id, structure, state."""
+        L = self._world_sigs()
+        which = 1 if continuous else 0
+        n = L.ref_world_events(w, which, *win, t0, t1, None, 0)
+        out = np.zeros((max(n, 1), 10))
+        L.ref_world_events(w, which, *win, t0, t1, _p(out, C.c_double), n)
+        return out[:n]
+
+    def world_density3d(self, w, win, nx, ny, nz, mask=0xF):
+        out = np.zeros((nz, ny, nx), np.float32)
+        tot = self._world_sigs().ref_world_density3d(w, *win, nx, ny, nz, mask, _p(out, C.c_float))
+        return tot, out
+
+    def mt_frames(self, pts):
+        """BuildMtFrames (core/src/microtubules.cpp:701): (cum, T, U, V)."""
+        pts = np.ascontiguousarray(pts, np.float64)
+        n = len(pts)
+        cum = np.zeros(n)
+        T, U, V = (np.zeros((max(n - 1, 1), 3)) for _ in range(3))
+        self._world_sigs().ref_mt_frames(_p(pts, C.c_double), n, _p(cum, C.c_double), _p(T, C.c_double),
+                                         _p(U, C.c_double), _p(V, C.c_double))
+        return cum, T[: max(n - 1, 0)], U[: max(n - 1, 0)], V[: max(n - 1, 0)]
+
+    def dyes_in_block(self, seed, cx, cy, mt_index, pts, block, density, ff, linkage=(12.0, 0.0, 2.0, 5.0)):
+        """DyesInBlock (core/src/dyes.cpp:127): rows x, y, z, k, n, id, theta (cell-local)."""
+        L = self._world_sigs()
+        pts = np.ascontiguousarray(pts, np.float64)
+        lk = np.array(linkage, np.float64)
+        out = np.zeros((4096, 7))
+        n = L.ref_dyes_in_block(seed & 0xFFFFFFFF, cx, cy, mt_index, _p(pts, C.c_double), len(pts), block, density,
+                                ff, _p(lk, C.c_double), _p(out, C.c_double), 4096)
+        return out[:n]
+
+    def jsm_geom(self, fn: str, x, y=None, z=None):
+        """jsm::cbrt (fn "cbrt"), 2-argument hypot ("hypot2") or 3-argument hypot ("hypot3") elementwise."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(x if y is None else y, np.float64)
+        z = np.ascontiguousarray(x if z is None else z, np.float64)
+        out = np.zeros_like(x)
+        self._world_sigs().ref_jsm_geom(("cbrt", "hypot2", "hypot3").index(fn), _p(x, C.c_double),
+                                        _p(y, C.c_double), _p(z, C.c_double), x.size, _p(out, C.c_double))
+        return out
+
 
 # ---- WidefieldGpuJob (Sim/WidefieldRender.h:306-340): parsing captured jobs and the CPU statement of the hosts --
 

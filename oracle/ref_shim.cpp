@@ -19,8 +19,14 @@
 #include "Parallel.h"
 #include "dyes.h"
 #include "jsmath.h"
+#include "microtubules.h"
+#include "npcs.h"
+#include "params.h"
+#include "world.h"
+#include "cells.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -1081,6 +1087,394 @@ int32_t ref_collect_emitters_full(const double* ev7, int32_t nEv, int64_t frame,
    zCounts[0] = clamped;
    zCounts[1] = total;
    return n;
+}
+
+// ---- the world's dye blocks and queries (core/src/world.{h,cpp}), for the checks of the device world ----
+
+// A reference World of seed `seed` with the default params (core/src/params.h) and n name = value overrides
+// (isc::SetParam). nullptr on an unknown name.
+void* ref_world_new(uint32_t seed, const char* const* names, const double* values, int32_t n)
+{
+   isc::Params p;
+   for (int32_t i = 0; i < n; ++i)
+      if (!isc::SetParam(p, names[i], values[i]))
+         return nullptr;
+   return new isc::World(seed, p);
+}
+
+void ref_world_free(void* w) { delete static_cast<isc::World*>(w); }
+
+// World::SetLabel from ISC_LABEL_COUNT (20) doubles in the C ABI's layout (core/include/insiliscope/insiliscope.h:
+// 218-278): density, fluorescent fraction, mode, rate, on, off, bleach, cv, initial on, pre state, orientation mode,
+// polar, azimuth, wobble, motion, off-target count, binder, flex, linker min, linker max (any NaN: the structure's
+// linkage). 0, or -1 when the world refuses it.
+int32_t ref_world_set_label(void* wp, int32_t s, const double* v)
+{
+   isc::Label l;
+   l.density = v[0];
+   l.fluorescentFraction = v[1];
+   l.mode = static_cast<isc::LabelMode>(static_cast<int>(v[2]));
+   l.kin.activationRatePerSec = v[3];
+   l.kin.onSec = v[4];
+   l.kin.offSec = v[5];
+   l.kin.bleachProb = v[6];
+   l.kin.photonCV = v[7];
+   l.kin.initialOnSec = v[8];
+   l.preState = v[9] != 0;
+   l.orientation.mode = static_cast<isc::OrientationMode>(static_cast<int>(v[10]));
+   l.orientation.polarDeg = v[11];
+   l.orientation.azimuthDeg = v[12];
+   l.orientation.wobbleDeg = v[13];
+   l.motion = static_cast<int>(v[14]);
+   l.offTargetCount = static_cast<int>(v[15]);
+   l.hasLinkage = !(std::isnan(v[16]) || std::isnan(v[17]) || std::isnan(v[18]) || std::isnan(v[19]));
+   if (l.hasLinkage)
+      l.linkage = isc::Linkage{v[16], v[17], v[18], v[19]};
+   return static_cast<isc::World*>(wp)->SetLabel(s, l) ? 0 : -1;
+}
+
+// The label the world keeps for structure s, linkage normalised: density, fraction, mode, pre state, rate, on, off,
+// bleach, cv, initial on, binder, flex, linker min, linker max (14 doubles).
+void ref_world_get_label(void* wp, int32_t s, double* out)
+{
+   const isc::Label& l = static_cast<isc::World*>(wp)->GetLabel(s);
+   const double v[14] = {l.density, l.fluorescentFraction, (double)(int)l.mode, l.preState ? 1.0 : 0.0,
+                         l.kin.activationRatePerSec, l.kin.onSec, l.kin.offSec, l.kin.bleachProb, l.kin.photonCV,
+                         l.kin.initialOnSec, l.linkage.binderNm, l.linkage.binderFlexNm, l.linkage.linkerMinNm,
+                         l.linkage.linkerMaxNm};
+   std::memcpy(out, v, sizeof v);
+}
+
+// World::CellsInRect (its order): rows cx, cy, x, y, packRot, rOuter. Returns the count (rows past cap not written).
+int32_t ref_world_cells(void* wp, double x0, double y0, double x1, double y1, double* out, int32_t cap)
+{
+   std::vector<isc::Cell> cells;
+   static_cast<isc::World*>(wp)->CellsInRect(x0, y0, x1, y1, cells);
+   for (size_t i = 0; i < cells.size() && (int32_t)i < cap; ++i)
+   {
+      const isc::Cell& c = cells[i];
+      const double r[6] = {(double)c.cx, (double)c.cy, c.x, c.y, c.packRot, c.rOuter};
+      std::memcpy(out + 6 * i, r, sizeof r);
+   }
+   return (int32_t)cells.size();
+}
+
+// The microtubules of the cell at chunk (cx, cy) (World::Assets): every point (x, y, z cell-local) up to capPts and
+// each microtubule's point count up to capMts; *nMts = the count. Returns the total point count, -1 without a cell.
+int64_t ref_world_cell_mts(void* wp, int32_t cx, int32_t cy, double* xyz, int64_t capPts, int32_t* lens,
+                           int32_t capMts, int32_t* nMts)
+{
+   isc::World& w = *static_cast<isc::World*>(wp);
+   isc::Cell c;
+   if (!w.FindCell(cx, cy, c))
+      return -1;
+   const isc::CellAssets& A = w.Assets(c);
+   int64_t total = 0;
+   for (size_t i = 0; i < A.mts.size(); ++i)
+   {
+      const std::vector<isc::Pt3>& pts = A.mts[i].pts;
+      if ((int32_t)i < capMts)
+         lens[i] = (int32_t)pts.size();
+      for (const isc::Pt3& q : pts)
+      {
+         if (total < capPts)
+         {
+            xyz[3 * total] = q.x;
+            xyz[3 * total + 1] = q.y;
+            xyz[3 * total + 2] = q.z;
+         }
+         ++total;
+      }
+   }
+   *nMts = (int32_t)A.mts.size();
+   return total;
+}
+
+// BuildMtFrames (core/src/microtubules.cpp:701) of n points: cum (n), then T, U, V (3 x (n - 1) each).
+void ref_mt_frames(const double* xyz, int32_t n, double* cum, double* T, double* U, double* V)
+{
+   std::vector<isc::Pt3> pts((size_t)n);
+   for (int32_t i = 0; i < n; ++i)
+      pts[(size_t)i] = isc::Pt3{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+   const isc::MtFrames f = isc::BuildMtFrames(pts);
+   for (size_t i = 0; i < f.cum.size(); ++i)
+      cum[i] = f.cum[i];
+   for (size_t i = 0; i < f.T.size(); ++i)
+   {
+      const isc::Pt3* v[3] = {&f.T[i], &f.U[i], &f.V[i]};
+      double* o[3] = {T, U, V};
+      for (int q = 0; q < 3; ++q)
+      {
+         o[q][3 * i] = v[q]->x;
+         o[q][3 * i + 1] = v[q]->y;
+         o[q][3 * i + 2] = v[q]->z;
+      }
+   }
+}
+
+// The pores (which 1: World::CellNpcs; rows c, n, e1, e2, cand: 13 doubles) or lamina points (which 2:
+// World::CellLamin; rows c, n, cand: 7 doubles) of the cell at chunk (cx, cy), up to cap rows. Returns the count, -1
+// without a cell.
+int32_t ref_world_cell_envelope(void* wp, int32_t which, int32_t cx, int32_t cy, double* out, int32_t cap)
+{
+   isc::World& w = *static_cast<isc::World*>(wp);
+   if (which == 1)
+   {
+      std::vector<isc::Npc> v;
+      if (!w.CellNpcs(cx, cy, v))
+         return -1;
+      for (size_t i = 0; i < v.size() && (int32_t)i < cap; ++i)
+      {
+         const isc::Npc& q = v[i];
+         const double r[13] = {q.c.x,  q.c.y,  q.c.z,  q.n.x,  q.n.y,  q.n.z,         q.e1.x,
+                               q.e1.y, q.e1.z, q.e2.x, q.e2.y, q.e2.z, (double)q.cand};
+         std::memcpy(out + 13 * i, r, sizeof r);
+      }
+      return (int32_t)v.size();
+   }
+   std::vector<isc::LaminPoint> v;
+   if (!w.CellLamin(cx, cy, v))
+      return -1;
+   for (size_t i = 0; i < v.size() && (int32_t)i < cap; ++i)
+   {
+      const isc::LaminPoint& q = v[i];
+      const double r[7] = {q.c.x, q.c.y, q.c.z, q.n.x, q.n.y, q.n.z, (double)q.cand};
+      std::memcpy(out + 7 * i, r, sizeof r);
+   }
+   return (int32_t)v.size();
+}
+
+// NpcAttachmentPoints (64 x 3 nm) and NpcSiteReachUm of the world's params.
+void ref_world_npc_sites(void* wp, double* sites, double* reachUm)
+{
+   const isc::Params& p = static_cast<isc::World*>(wp)->GetParams();
+   isc::NpcAttachmentPoints(p, sites);
+   *reachUm = isc::NpcSiteReachUm(p);
+}
+
+// World::SitesInWindow: rows x, y, z, id, cx, cy, mtIndex, k, n, structure (10 doubles). Returns the count.
+int64_t ref_world_sites(void* wp, double x0, double y0, double x1, double y1, double zMin, double zMax, double* out,
+                        int64_t cap)
+{
+   std::vector<isc::WorldDye> v;
+   static_cast<isc::World*>(wp)->SitesInWindow(x0, y0, x1, y1, zMin, zMax, v);
+   for (size_t i = 0; i < v.size() && (int64_t)i < cap; ++i)
+   {
+      const isc::WorldDye& d = v[i];
+      const double r[10] = {d.x, d.y, d.z, (double)d.id, (double)d.cx, (double)d.cy, (double)d.mtIndex,
+                            (double)d.k, (double)d.n, (double)d.structure};
+      std::memcpy(out + 10 * i, r, sizeof r);
+   }
+   return (int64_t)v.size();
+}
+
+// World::EventsInWindow (which 0; t0, t1) or World::ContinuousInWindow (which 1; tMin = t0): rows x, y, z, tOn, tOff,
+// brightness, aux, id, structure, state (10 doubles). Returns the count.
+int64_t ref_world_events(void* wp, int32_t which, double x0, double y0, double x1, double y1, double zMin,
+                         double zMax, double t0, double t1, double* out, int64_t cap)
+{
+   std::vector<isc::WorldEvent> v;
+   isc::World& w = *static_cast<isc::World*>(wp);
+   if (which == 0)
+      w.EventsInWindow(x0, y0, x1, y1, zMin, zMax, t0, t1, v);
+   else
+      w.ContinuousInWindow(x0, y0, x1, y1, zMin, zMax, t0, v);
+   for (size_t i = 0; i < v.size() && (int64_t)i < cap; ++i)
+   {
+      const isc::WorldEvent& e = v[i];
+      const double r[10] = {e.x, e.y, e.z, e.tOn, e.tOff, e.brightness, e.aux, (double)e.id, (double)e.structure,
+                            (double)e.state};
+      std::memcpy(out + 10 * i, r, sizeof r);
+   }
+   return (int64_t)v.size();
+}
+
+// World::Density3dInWindow into out[nx * ny * nz]; returns the total.
+int64_t ref_world_density3d(void* wp, double x0, double y0, double x1, double y1, double zMin, double zMax,
+                            int32_t nx, int32_t ny, int32_t nz, uint32_t mask, float* out)
+{
+   return static_cast<isc::World*>(wp)->Density3dInWindow(x0, y0, x1, y1, zMin, zMax, nx, ny, nz, mask, out);
+}
+
+// DyesInBlock (core/src/dyes.cpp:127) for one microtubule given as points: rows x, y, z, k, n, id, theta
+// (7 doubles), up to cap. Returns the count.
+int32_t ref_dyes_in_block(uint32_t seed, int32_t cx, int32_t cy, int32_t mtIndex, const double* xyz, int32_t nPts,
+                          int32_t block, double density, double fluorescentFraction, const double* linkage4,
+                          double* out, int32_t cap)
+{
+   std::vector<isc::Pt3> pts((size_t)nPts);
+   for (int32_t i = 0; i < nPts; ++i)
+      pts[(size_t)i] = isc::Pt3{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+   const isc::MtFrames fr = isc::BuildMtFrames(pts);
+   std::vector<isc::Dye> v;
+   isc::DyesInBlock(seed, cx, cy, mtIndex, pts, fr, block, density, fluorescentFraction, v,
+                    isc::Linkage{linkage4[0], linkage4[1], linkage4[2], linkage4[3]});
+   for (size_t i = 0; i < v.size() && (int32_t)i < cap; ++i)
+   {
+      const isc::Dye& d = v[i];
+      const double r[7] = {d.pos.x, d.pos.y, d.pos.z, (double)d.k, (double)d.n, (double)d.id, d.theta};
+      std::memcpy(out + 7 * i, r, sizeof r);
+   }
+   return (int32_t)v.size();
+}
+
+// jsm::cbrt and the 2- and 3-argument jsm::hypot (core/src/jsmath.h) of n inputs: fn 0 cbrt(x), 1 hypot(x, y),
+// 2 hypot(x, y, z) (x, y, z: the three input arrays).
+void ref_jsm_geom(int32_t fn, const double* x, const double* y, const double* z, int64_t n, double* out)
+{
+   for (int64_t i = 0; i < n; ++i)
+      out[i] = fn == 0 ? isc::jsm::cbrt(x[i]) : fn == 1 ? isc::jsm::hypot(x[i], y[i])
+                                                        : isc::jsm::hypot(x[i], y[i], z[i]);
+}
+
+// ---- the golden cases' raw candidate cells (tests/parity's CellsCase / NpcsCase / LaminCase inputs) ----
+
+static bool CaseCell(uint32_t seed, const char* const* names, const double* values, int32_t n, int32_t cx, int32_t cy,
+                     isc::Params& p, isc::Cell& c)
+{
+   for (int32_t i = 0; i < n; ++i)
+      if (!isc::SetParam(p, names[i], values[i]))
+         return false;
+   isc::NormalizeParams(p);
+   c = isc::RawCandidate(seed, cx, cy, p);
+   return c.present;
+}
+
+// The microtubules of the raw candidate cell (cx, cy) (BuildMicrotubulesForCell): points up to capPts, per-microtubule
+// counts up to capMts, *nMts. Returns the point count, -1 when the chunk has no cell or a param is unknown.
+int64_t ref_case_mts(uint32_t seed, const char* const* names, const double* values, int32_t n, int32_t cx, int32_t cy,
+                     double* xyz, int64_t capPts, int32_t* lens, int32_t capMts, int32_t* nMts)
+{
+   isc::Params p;
+   isc::Cell c;
+   if (!CaseCell(seed, names, values, n, cx, cy, p, c))
+      return -1;
+   const std::vector<isc::Microtubule> mts = isc::BuildMicrotubulesForCell(seed, c, p, isc::BuildMtCellGeom(c, p));
+   int64_t total = 0;
+   for (size_t i = 0; i < mts.size(); ++i)
+   {
+      if ((int32_t)i < capMts)
+         lens[i] = (int32_t)mts[i].pts.size();
+      for (const isc::Pt3& q : mts[i].pts)
+      {
+         if (total < capPts)
+         {
+            xyz[3 * total] = q.x;
+            xyz[3 * total + 1] = q.y;
+            xyz[3 * total + 2] = q.z;
+         }
+         ++total;
+      }
+   }
+   *nMts = (int32_t)mts.size();
+   return total;
+}
+
+// The pores (which 1: rows c, n, e1, e2, cand, piece: 14 doubles; BuildNpcsForCell) or lamina points (which 2: rows
+// c, n, cand, piece: 8 doubles; BuildLaminForCell) of the raw candidate cell (cx, cy), up to cap; sites / reachUm:
+// NpcAttachmentPoints and NpcSiteReachUm of the params. Returns the count, -1 as ref_case_mts.
+int32_t ref_case_envelope(uint32_t seed, const char* const* names, const double* values, int32_t n, int32_t cx,
+                          int32_t cy, int32_t which, double* out, int32_t cap, double* sites, double* reachUm)
+{
+   isc::Params p;
+   isc::Cell c;
+   if (!CaseCell(seed, names, values, n, cx, cy, p, c))
+      return -1;
+   isc::NpcAttachmentPoints(p, sites);
+   *reachUm = isc::NpcSiteReachUm(p);
+   const std::vector<isc::Npc> pores = isc::BuildNpcsForCell(seed, cx, cy, c, p);
+   if (which == 1)
+   {
+      for (size_t i = 0; i < pores.size() && (int32_t)i < cap; ++i)
+      {
+         const isc::Npc& q = pores[i];
+         const double r[14] = {q.c.x,  q.c.y,  q.c.z,  q.n.x,  q.n.y,          q.n.z,          q.e1.x,
+                               q.e1.y, q.e1.z, q.e2.x, q.e2.y, q.e2.z, (double)q.cand, (double)q.piece};
+         std::memcpy(out + 14 * i, r, sizeof r);
+      }
+      return (int32_t)pores.size();
+   }
+   const std::vector<isc::LaminPoint> pts = isc::BuildLaminForCell(seed, cx, cy, c, p, pores);
+   for (size_t i = 0; i < pts.size() && (int32_t)i < cap; ++i)
+   {
+      const isc::LaminPoint& q = pts[i];
+      const double r[8] = {q.c.x, q.c.y, q.c.z, q.n.x, q.n.y, q.n.z, (double)q.cand, (double)q.piece};
+      std::memcpy(out + 8 * i, r, sizeof r);
+   }
+   return (int32_t)pts.size();
+}
+
+// NpcDyesInBlock (which 1; rows as ref_case_envelope's, sites 64 x 3 nm) or LaminDyesInBlock (which 2) of the points
+// rows[ids[0 .. nIds)]: rows pore, k, n, id, pos, site, rigid, tip (16 doubles), up to cap. Returns the count.
+int32_t ref_envelope_dyes(int32_t which, uint32_t seed, int32_t cx, int32_t cy, const double* rows, int32_t nRows,
+                          const int32_t* ids, int32_t nIds, const double* sites, double density, double ff,
+                          const double* linkage4, double* out, int32_t cap)
+{
+   const isc::Linkage lk{linkage4[0], linkage4[1], linkage4[2], linkage4[3]};
+   const std::vector<int32_t> idv(ids, ids + nIds);
+   std::vector<isc::NpcDye> dyes;
+   std::vector<isc::LinkageParts> parts;
+   if (which == 1)
+   {
+      std::vector<isc::Npc> pores((size_t)nRows);
+      for (int32_t i = 0; i < nRows; ++i)
+      {
+         const double* r = rows + 14 * i;
+         isc::Npc& q = pores[(size_t)i];
+         q.c = {r[0], r[1], r[2]};
+         q.n = {r[3], r[4], r[5]};
+         q.e1 = {r[6], r[7], r[8]};
+         q.e2 = {r[9], r[10], r[11]};
+         q.cand = (int32_t)r[12];
+         q.piece = (int)r[13];
+      }
+      isc::NpcDyesInBlock(seed, cx, cy, pores, idv, sites, density, ff, dyes, lk, &parts);
+   }
+   else
+   {
+      std::vector<isc::LaminPoint> pts((size_t)nRows);
+      for (int32_t i = 0; i < nRows; ++i)
+      {
+         const double* r = rows + 8 * i;
+         isc::LaminPoint& q = pts[(size_t)i];
+         q.c = {r[0], r[1], r[2]};
+         q.n = {r[3], r[4], r[5]};
+         q.cand = (int32_t)r[6];
+         q.piece = (int)r[7];
+      }
+      isc::LaminDyesInBlock(seed, cx, cy, pts, idv, density, ff, dyes, lk, &parts);
+   }
+   for (size_t i = 0; i < dyes.size() && (int32_t)i < cap; ++i)
+   {
+      const isc::NpcDye& e = dyes[i];
+      const isc::LinkageParts& P = parts[i];
+      const double r[16] = {(double)e.pore, (double)e.k, (double)e.n, (double)e.id, e.pos.x,   e.pos.y,
+                            e.pos.z,        P.site.x,    P.site.y,    P.site.z,     P.rigid.x, P.rigid.y,
+                            P.rigid.z,      P.tip.x,     P.tip.y,     P.tip.z};
+      std::memcpy(out + 16 * i, r, sizeof r);
+   }
+   return (int32_t)dyes.size();
+}
+
+// MtSiteGeometry (core/src/dyes.cpp:120) on a microtubule's points at arc S (um), azimuth theta and linker draws
+// r1..r3: att, tip, dye (9 doubles).
+void ref_mt_site_geometry(const double* xyz, int32_t nPts, double S, double theta, double r1, double r2, double r3,
+                          double* out)
+{
+   std::vector<isc::Pt3> pts((size_t)nPts);
+   for (int32_t i = 0; i < nPts; ++i)
+      pts[(size_t)i] = isc::Pt3{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+   const isc::MtFrames fr = isc::BuildMtFrames(pts);
+   const isc::SiteGeom g = isc::MtSiteGeometry(pts, fr, isc::MtSegmentAt(fr, S), S, theta, r1, r2, r3);
+   const double r[9] = {g.att.x, g.att.y, g.att.z, g.tip.x, g.tip.y, g.tip.z, g.dye.x, g.dye.y, g.dye.z};
+   std::memcpy(out, r, sizeof r);
+}
+
+// MtProtofilamentTheta and MtProtofilamentOffsetNm (core/src/dyes.cpp:29-34).
+void ref_mt_protofilament(double phase, int32_t k, double* theta, double* offNm)
+{
+   *theta = isc::MtProtofilamentTheta(phase, k);
+   *offNm = isc::MtProtofilamentOffsetNm(k);
 }
 
 } // extern "C"

@@ -33,6 +33,14 @@ Outputs (inputs and expected outputs only):
   wf_drift_ref.npz   WidefieldDriftFrames (RenderShiftedImages) frames of a spectra-keeping scene under seeded drift
                      trajectories, with the focus series' kept spectra and geometry
 
+  world_golden.txt   lines of spec/golden/cases.txt and ref.txt (the reference's frozen goldens) for the dye sites: the
+                     params sets, the npcs / lamin / cells m08 cases, their npc, npd, lam, lmd lines (the npc, npd and
+                     lam lines cut to their first samples), the mtp / mtl lines of every 16th microtubule of m08
+  world_inputs.npz   the inputs those lines were made from, from the compiled reference: the first 40 pores and 200
+                     lamina points of each case cell (BuildNpcsForCell / BuildLaminForCell of the raw candidate), the
+                     pore attachment sites, and the points of microtubules 0, 16, .., 112 of m08 with the digests of
+                     their DyesInBlock dyes under two labels
+
 `python tests/golden/make_golden.py roi` writes only the named fixtures.
 """
 from __future__ import annotations
@@ -448,6 +456,101 @@ def wf_drift(R):
     np.savez_compressed(os.path.join(HERE, "wf_drift_ref.npz"), **out)
 
 
+def _cut(line, header, per, keep):
+    t = line.split()
+    return " ".join(t[: header + per * keep])
+
+
+WORLD_MT_IDS = list(range(0, 128, 16))
+# (density, fluorescent fraction, linkage binder, flex, linker min, max) of the microtubule digests
+WORLD_MT_LABELS = [(0.7, 0.9, (12.0, 0.0, 2.0, 5.0)), (1.0, 1.0, (10.0, 4.0, 1.0, 6.0))]
+
+
+def world_mt_digest(rows):
+    """sha256 of a DyesInBlock row set: x, y, z (float64) then k, n, id (int64), row-major."""
+    import hashlib
+    a = np.ascontiguousarray(rows[:, :3], np.float64)
+    b = np.ascontiguousarray(rows[:, 3:6], np.float64).astype(np.int64)
+    return hashlib.sha256(a.tobytes() + b.tobytes()).hexdigest()
+
+
+def world(R):
+    cases = open(os.path.join(REF, "spec/golden/cases.txt")).read().splitlines()
+    keep = [ln for ln in cases if ln.startswith("params ") or ln.startswith("npcs ") or ln.startswith("lamin ")
+            or ln.startswith("cells m08 ")]
+    out = list(keep)
+    with open(os.path.join(REF, "spec/golden/ref.txt")) as f:
+        for line in f:
+            line = line.rstrip("\n")
+            t = line.split(" ", 5)
+            kind = t[0]
+            if kind == "npc":
+                out.append(_cut(line, 18, 14, 2))
+            elif kind == "npd":
+                out.append(_cut(line, 9, 16, 30))
+            elif kind == "lam":
+                out.append(_cut(line, 13, 8, 9))
+            elif kind == "lmd":
+                out.append(line)
+            elif kind in ("mtp", "mtl") and t[1] == "m08" and int(t[4]) % 16 == 0:
+                out.append(line)
+    with open(os.path.join(HERE, "world_golden.txt"), "w") as f:
+        f.write("\n".join(out) + "\n")
+    params = {}
+    for ln in keep:
+        if ln.startswith("params "):
+            t = ln.split()
+            params[t[1]] = {kv.split("=")[0]: float(kv.split("=")[1]) for kv in t[2:]}
+    arrays = {}
+    for line in out:
+        t = line.split()
+        if t[0] not in ("npc", "lam"):
+            continue
+        case = next(c for c in keep if c.split()[1] == t[1])
+        c = case.split()
+        seed, pname, cx, cy = int(c[2]), c[3], int(t[2]), int(t[3])
+        which = 1 if t[0] == "npc" else 2
+        rows, sites, reach = R.case_envelope(seed, params[pname], cx, cy, which)
+        arrays[f"{t[1]}_rows"] = rows[: 40 if which == 1 else 200]
+        arrays[f"{t[1]}_meta"] = np.array([seed, cx, cy, which], np.int64)
+        arrays[f"{t[1]}_sites"] = sites
+        arrays[f"{t[1]}_reach"] = np.array([reach])
+    # microtubules of cells m08 (seed 1249, defaults): the cell of the mtp lines
+    m08 = next(c for c in keep if c.startswith("cells m08")).split()
+    mtp = [ln.split() for ln in out if ln.startswith("mtp m08 ")]
+    cx, cy = int(mtp[0][2]), int(mtp[0][3])
+    seed = int(m08[2])
+    xyz, lens = R.case_mts(seed, params[m08[3]], cx, cy)
+    starts = np.concatenate([[0], np.cumsum(lens)])
+    pts, counts = [], []
+    for i in WORLD_MT_IDS:
+        pts.append(xyz[starts[i]:starts[i + 1]])
+        counts.append(lens[i])
+    arrays["mt_meta"] = np.array([seed, cx, cy], np.int64)
+    arrays["mt_ids"] = np.array(WORLD_MT_IDS, np.int32)
+    arrays["mt_pts"] = np.concatenate(pts)
+    arrays["mt_counts"] = np.array(counts, np.int64)
+    digests, blocks = [], []
+    for li, (dens, ff, lk) in enumerate(WORLD_MT_LABELS):
+        for j, i in enumerate(WORLD_MT_IDS):
+            p = pts[j]
+            rows, per = [], []
+            total = R.mt_frames(p)[0][-1]
+            for b in range(int(np.ceil(total / 1.0))):
+                r = R.dyes_in_block(seed, cx, cy, i, p, b, dens, ff, lk)
+                rows.append(r)
+                per.append(len(r))
+            rows = np.concatenate(rows) if rows else np.zeros((0, 7))
+            digests.append(world_mt_digest(rows))
+            blocks.append(np.array(per, np.int64))
+            if li == 0 and j == 0:
+                arrays["mt_first_rows"] = rows[:64]
+    arrays["mt_digests"] = np.array(digests)
+    arrays["mt_block_counts"] = np.concatenate(blocks)
+    arrays["mt_block_starts"] = np.concatenate([[0], np.cumsum([len(b) for b in blocks])])
+    np.savez_compressed(os.path.join(HERE, "world_inputs.npz"), **arrays)
+
+
 def main():
     R = O.Reference()
     only = set(sys.argv[1:])
@@ -470,6 +573,7 @@ def main():
     pupil_samples(R)
     psf_big(R)
     wf_drift(R)
+    world(R)
     print("fixtures written to", HERE)
 
 

@@ -44,6 +44,12 @@ def test_struct_layouts():
     assert C.sizeof(_lib.Camera) == 56
     assert C.sizeof(_lib.PsfRequest) == C.sizeof(O.PsfRequest)
     assert host.FRAME_EMITTER_DTYPE.itemsize == 32
+    for dt, st in ((host.WORLD_CELL_DTYPE, _lib.WorldCell), (host.NPC_DTYPE, _lib.Npc),
+                   (host.LAMIN_DTYPE, _lib.LaminPoint), (host.WORLD_DYE_DTYPE, _lib.WorldDye),
+                   (host.BLOCK_DYE_DTYPE, _lib.BlockDye), (host.WORLD_EVENT_DTYPE, _lib.WorldEvent)):
+        assert dt.itemsize == C.sizeof(st), st
+        for name in dt.names:
+            assert dt.fields[name][1] == getattr(st, name).offset, (st, name)
     assert host.SPLAT_EMITTER_DTYPE.itemsize == 64
 
 
@@ -57,7 +63,9 @@ def test_struct_layouts_match_header(tmp_path):
         pytest.skip("gcc not found")
     pairs = {"iscg_camera": _lib.Camera, "iscg_kernel_view": _lib.KernelView, "iscg_wf_desc": _lib.WfDesc,
              "iscg_wf_dep": _lib.WfDep, "iscg_wf_job": _lib.WfJob, "iscg_roi_model": _lib.RoiModel,
-             "iscg_psf_request": _lib.PsfRequest}
+             "iscg_psf_request": _lib.PsfRequest, "iscg_world_cell": _lib.WorldCell, "iscg_npc": _lib.Npc,
+             "iscg_lamin_point": _lib.LaminPoint, "iscg_world_geometry": _lib.WorldGeometry,
+             "iscg_label": _lib.Label, "iscg_world_dye": _lib.WorldDye, "iscg_block_dye": _lib.BlockDye, "iscg_world_event": _lib.WorldEvent}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "iscg.h"', "int main(void) {"]
     for cname, py in pairs.items():
         lines.append(f'printf("{cname} <sizeof> %zu\\n", sizeof({cname}));')
