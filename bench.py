@@ -646,6 +646,59 @@ def cpu_dyes(n=200000):
             "sample": f"isc::LabelSchedule of {n} dyes ({total[0]} blinks) in one thread, {reps}x in {el:.2f} s"}
 
 
+WORLD_WINDOW = (0.0, 0.0, 25.6, 25.6, 0.0, 3.2)
+
+
+def run_world(ctx, steps, warmup):
+    """World::Density3dInWindow on the device over a synthetic cell field the size of the seed-1 field's 25.6 um
+    window (5 cells, 2200 microtubules, 296,606 centreline points, 6300 pores; configs.world_field), microtubules and
+    pores labelled (DNA-PAINT, density 0.7): one step = the walk's selection, every walked block's dyes generated
+    (DyesInBlock / NpcDyesInBlock, LocalToWorld, the 0.1 nm lattice) and binned into the widefield workload's
+    256 x 256 x 64 density (100 x 100 x 50 nm voxels), written to HBM."""
+    import torch
+    from insiliscope_amd import configs, host
+    g = host.WorldGeometry.from_rows(**configs.world_field(1))
+    W = host.World(1, g, ctx)
+    out = torch.zeros((64, 256, 256), dtype=torch.float32, device="cuda")
+    n = [0]
+
+    def step(_=0):
+        n[0], _o = W.Density3dInWindow(*WORLD_WINDOW, 256, 256, 64, 0xF, out=out)
+
+    for _ in range(warmup):
+        step()
+    with Timer(ctx) as t:
+        for _ in range(steps):
+            step()
+    ms = t.ms / steps
+    prof = profiled(ctx, step, 3)
+    kernels = {k: round(v[1] / max(v[0], 1) * 1000, 2) for k, v in prof.items() if k.startswith("world_")}
+    W.close()
+    return {"value": n[0] * 1000.0 / ms, "unit": "dyes/s", "ms_per_step": ms, "dyes_per_step": n[0],
+            "kernels_us": kernels,
+            "config": {"workload": "Density3dInWindow of a 25.6 um synthetic cell field (5 cells, 2200 microtubules, "
+                                   "6300 pores; DNA-PAINT 0.7), 256x256x64 voxels, dyes generated every query"}}
+
+
+def cpu_world():
+    import oracle as O
+    from insiliscope_amd import configs
+    ref = O.load_reference()
+    if ref is None:
+        return None
+    f = configs.world_field(1)
+    cells = np.stack([f["cells"][k] for k in ("cx", "cy", "x", "y", "packRot")], 1)[:1]
+    lk = (12.0, 0.0, 2.0, 5.0)
+    res = []
+    reps, el = timed_repeat(lambda: res.append(ref.mt_density(1, cells, f["mtStart"][:2], f["ptStart"], f["pts"], 0.7,
+                                                              1.0, lk, WORLD_WINDOW, 256, 256, 64)[0]),
+                            CPU_MIN_S_SECONDARY)
+    return {"value": round(reps * res[0] / el, 1), "unit": "dyes/s", "cores": 1, "kind": "reference",
+            "sample": f"DyesInBlock + LocalToWorld + lattice + binning (World::GenerateDyes / Density3dInWindow) "
+                      f"over the microtubules of the first cell ({res[0]} dyes in the window) in one thread, "
+                      f"{reps}x in {el:.2f} s"}
+
+
 def measure_peaks():
     """The rates this box attains, beside the datasheet peaks the rooflines use: a 1 GiB device-to-device copy
     (read + write bytes / time) and square GEMMs through torch (hipBLASLt / rocBLAS) in FP32, FP16 and BF16."""
@@ -684,7 +737,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="wf", choices=["wf", "smlm", "psf", "dyes", "adjoint", "c1"])
+    ap.add_argument("--workload", default="wf", choices=["wf", "smlm", "psf", "dyes", "adjoint", "c1", "world"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -730,6 +783,10 @@ def main():
     elif args.workload == "dyes":
         r = run_dyes(ctx, args.steps, args.warmup)
         config, unit, dtype = r["config"], "dyes/s", "f64"
+    elif args.workload == "world":
+        r = run_world(ctx, args.steps, args.warmup)
+        config, unit, dtype = r["config"], "dyes/s", "f64"
+        r["roofline"] = None
     elif args.workload == "c1":
         r = run_c1(ctx, args.steps, args.warmup)
         r.pop("_events")
@@ -789,6 +846,16 @@ def main():
                 c["speedup_vs_cpu"] = round(c["value"] / b["value"], 1)
         except Exception as e:
             secondary["adjoint_c5"] = {"error": str(e)}
+        try:
+            wr = run_world(ctx, 10, 2)
+            secondary["world_density"] = wr
+            if not args.no_cpu:
+                c = cpu_world()
+                if c:
+                    wr["cpu_baseline"] = c
+                    wr["speedup_vs_cpu"] = round(wr["value"] / c["value"], 1)
+        except Exception as e:
+            secondary["world_density"] = {"error": str(e)}
         try:
             p = run_psf(ctx, 3, 1)
             secondary["psf_library_c3"] = p

@@ -230,3 +230,56 @@ def c2_density(seed=1, nx=256, ny=256, nz=64, occupancy=0.03):
     second = u[..., 2] < 0.3
     w = np.where(second, 0.5, 1.0)
     return np.where(on, count * w, 0.0).astype(np.float32)
+
+
+def world_field(seed=1, n_cells=5, mts_per_cell=440, pores_per_cell=1260, field_um=25.6):
+    """A synthetic cell field the size of the seed-1 field over a 25.6 um window (5 cells, about 2200 microtubules
+    of 70-200 points at 0.12 um steps, 6300 pores on ellipsoidal nuclei), as WorldGeometry fields: the shapes the
+    dye-site generators take, drawn from a seeded generator (the cells' own growth rules are not restated)."""
+    rng = np.random.default_rng(seed)
+    centres = np.array([(4.0, 5.0), (14.0, 20.0), (22.0, 6.0), (8.0, 18.0), (20.0, 14.0)])[:n_cells]
+    centres = centres * field_um / 25.6
+    cells = np.zeros(n_cells, [("cx", "<i4"), ("cy", "<i4"), ("x", "<f8"), ("y", "<f8"), ("packRot", "<f8"),
+                               ("rOuter", "<f8")])
+    mt_start, pt_start, pts = [0], [0], []
+    npc_start, npcs = [0], []
+    for c in range(n_cells):
+        cells[c] = (c - 2, c % 2, centres[c, 0], centres[c, 1], rng.uniform(-0.6, 0.6), 15.0)
+        for _ in range(mts_per_cell):
+            n = int(rng.integers(70, 200))
+            a0 = rng.uniform(0, 2 * np.pi)
+            ang = a0 + np.cumsum(rng.normal(0, 0.08, n))
+            r0 = 4.5
+            step = 0.12
+            x = r0 * np.cos(a0) + np.concatenate([[0], np.cumsum(step * np.cos(ang[1:]))])
+            y = r0 * np.sin(a0) + np.concatenate([[0], np.cumsum(step * np.sin(ang[1:]))])
+            z = np.clip(1.8 - np.linspace(0, 1.4, n) + np.cumsum(rng.normal(0, 0.01, n)), 0.2, 3.0)
+            pts.append(np.stack([x, y, z], 1))
+            pt_start.append(pt_start[-1] + n)
+        mt_start.append(mt_start[-1] + mts_per_cell)
+        # pores on the nucleus ellipsoid (semi-axes 5.5, 4.5, 1.3 um about (0, 0, 1.7)), outward normals
+        u = rng.uniform(-1, 1, pores_per_cell)
+        phi = rng.uniform(0, 2 * np.pi, pores_per_cell)
+        s = np.sqrt(1 - u * u)
+        ax = np.array([5.5, 4.5, 1.3])
+        cpos = np.stack([ax[0] * s * np.cos(phi), ax[1] * s * np.sin(phi), 1.7 + ax[2] * u], 1)
+        nrm = (cpos - [0, 0, 1.7]) / ax ** 2
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        ref = np.where(np.abs(nrm[:, 2:3]) < 0.9, [[0.0, 0.0, 1.0]], [[1.0, 0.0, 0.0]])
+        e1 = np.cross(nrm, ref)
+        e1 /= np.linalg.norm(e1, axis=1, keepdims=True)
+        e2 = np.cross(nrm, e1)
+        rows = np.zeros((pores_per_cell, 13))
+        rows[:, 0:3], rows[:, 3:6], rows[:, 6:9], rows[:, 9:12] = cpos, nrm, e1, e2
+        rows[:, 12] = 2 * np.arange(pores_per_cell)
+        npcs.append(rows)
+        npc_start.append(npc_start[-1] + pores_per_cell)
+    # 64 attachment sites: 2 rings at +-25 nm along the axis, 8 corners at 45 degrees, 4-point arcs of 20 nm spread
+    k = np.arange(64)
+    ring, corner, arc = k // 32, (k % 32) // 4, k % 4
+    ang = corner * np.pi / 4 + (arc - 1.5) * np.radians(45) / 2 * 0.2
+    sites = np.stack([53 * np.cos(ang), 53 * np.sin(ang), np.where(ring == 0, -25.0, 25.0)], 1)
+    return dict(cells=cells, mtStart=np.array(mt_start, np.int32), ptStart=np.array(pt_start, np.int64),
+                pts=np.concatenate(pts), npcStart=np.array(npc_start, np.int32), npcRows=np.concatenate(npcs),
+                laminStart=np.zeros(n_cells + 1, np.int32), laminRows=np.zeros((0, 7)),
+                npcSites=sites.reshape(-1), npcSiteReachUm=float(np.hypot(63.0, 25.0) * 1e-3))

@@ -1656,6 +1656,32 @@ class WorldGeometry:
     npcSites: np.ndarray = field(default_factory=lambda: np.zeros(192))
     npcSiteReachUm: float = 0.0
 
+    @classmethod
+    def from_rows(cls, cells, mtStart, ptStart, pts, npcStart=None, npcRows=None, laminStart=None, laminRows=None,
+                  npcSites=None, npcSiteReachUm=0.0, mtIndex=None):
+        """From row arrays: cells with fields cx, cy, x, y, packRot, rOuter; pores as rows c, n, e1, e2, cand (13
+        values); lamina points as rows c, n, cand (7 values)."""
+        c = np.zeros(len(cells), WORLD_CELL_DTYPE)
+        for k in WORLD_CELL_DTYPE.names:
+            c[k] = cells[k]
+        g = cls(cells=c, mtStart=np.asarray(mtStart, np.int32), ptStart=np.asarray(ptStart, np.int64),
+                pts=np.asarray(pts, np.float64).reshape(-1, 3), mtIndex=mtIndex)
+        if npcRows is not None and len(npcRows):
+            r = np.asarray(npcRows, np.float64)
+            a = np.zeros(len(r), NPC_DTYPE)
+            a["c"], a["n"], a["e1"], a["e2"] = r[:, 0:3], r[:, 3:6], r[:, 6:9], r[:, 9:12]
+            a["cand"] = r[:, 12].astype(np.int32)
+            g.npcs, g.npcStart = a, np.asarray(npcStart, np.int32)
+        if laminRows is not None and len(laminRows):
+            r = np.asarray(laminRows, np.float64)
+            a = np.zeros(len(r), LAMIN_DTYPE)
+            a["c"], a["n"], a["cand"] = r[:, 0:3], r[:, 3:6], r[:, 6].astype(np.int32)
+            g.lamin, g.laminStart = a, np.asarray(laminStart, np.int32)
+        if npcSites is not None:
+            g.npcSites = np.asarray(npcSites, np.float64)
+        g.npcSiteReachUm = float(npcSiteReachUm)
+        return g
+
 
 class World:
     """isc::World's dye blocks and window queries (core/src/world.h:152-300) over a cell field's geometry kept on the

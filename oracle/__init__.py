@@ -889,6 +889,8 @@ class Reference:
         L.ref_envelope_dyes.restype = I32
         L.ref_mt_site_geometry.argtypes = [PD, I32, D, D, D, D, D, PD]
         L.ref_mt_protofilament.argtypes = [D, I32, PD, PD]
+        L.ref_mt_density.argtypes = [U32, PD, I32, PI32, C.POINTER(C.c_int64), PD, D, D, PD, PD, I32, I32, I32, PF]
+        L.ref_mt_density.restype = I64
         self._world_ready = True
         return L
 
@@ -952,6 +954,22 @@ class Reference:
         self._world_sigs().ref_mt_site_geometry(_p(pts, C.c_double), len(pts), S, theta, r1, r2, r3,
                                                 _p(out, C.c_double))
         return out
+
+    def mt_density(self, seed, cells, mtStart, ptStart, pts, density, ff, linkage, win, nx, ny, nz):
+        """GenerateDyes + Density3dInWindow's binning over given microtubules in one thread (cells: rows cx, cy, x,
+        y, packRot): (dyes counted, counts)."""
+        L = self._world_sigs()
+        cells = np.ascontiguousarray(cells, np.float64)
+        mtStart = np.ascontiguousarray(mtStart, np.int32)
+        ptStart = np.ascontiguousarray(ptStart, np.int64)
+        pts = np.ascontiguousarray(pts, np.float64)
+        lk = np.ascontiguousarray(linkage, np.float64)
+        w = np.ascontiguousarray(win, np.float64)
+        out = np.zeros((nz, ny, nx), np.float32)
+        n = L.ref_mt_density(seed & 0xFFFFFFFF, _p(cells, C.c_double), len(cells), _p(mtStart, C.c_int32),
+                             ptStart.ctypes.data_as(C.POINTER(C.c_int64)), _p(pts, C.c_double), density, ff,
+                             _p(lk, C.c_double), _p(w, C.c_double), nx, ny, nz, _p(out, C.c_float))
+        return n, out
 
     def mt_protofilament(self, phase, k):
         th, off = C.c_double(), C.c_double()

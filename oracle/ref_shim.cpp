@@ -1477,4 +1477,65 @@ void ref_mt_protofilament(double phase, int32_t k, double* theta, double* offNm)
    *offNm = isc::MtProtofilamentOffsetNm(k);
 }
 
+// World::GenerateDyes + Density3dInWindow's binning (core/src/world.cpp:808-838, 1519-1543) over caller-given
+// microtubules, one thread: per cell (rows cx, cy, x, y, packRot), per microtubule (points xyz[3 ptStart[m] ..)),
+// BuildMtFrames once, DyesInBlock for each 1 um block with the label (density, fraction, linkage), LocalToWorld, the
+// dye store's 0.1 nm lattice, and the window's voxel counts (out, nx * ny * nz, added to). Returns the dyes counted.
+int64_t ref_mt_density(uint32_t seed, const double* cells, int32_t nCells, const int32_t* mtStart,
+                       const int64_t* ptStart, const double* xyz, double density, double ff, const double* linkage4,
+                       const double* win, int32_t nx, int32_t ny, int32_t nz, float* out)
+{
+   const isc::Linkage lk{linkage4[0], linkage4[1], linkage4[2], linkage4[3]};
+   const double sx = nx / (win[2] - win[0]), sy = ny / (win[3] - win[1]);
+   const double sz = nz > 1 ? nz / (win[5] - win[4]) : 0.0;
+   auto lattice = [](double um) {
+      const int64_t g = (int64_t)std::floor(um * 10000.0 - 0.3819660112501051 + 0.5);
+      return ((double)g + 0.3819660112501051) / 10000.0;
+   };
+   int64_t total = 0;
+   std::vector<isc::Dye> dyes;
+   for (int32_t c = 0; c < nCells; ++c)
+   {
+      const double* cr = cells + 5 * c;
+      isc::Cell cell;
+      cell.cx = (int32_t)cr[0];
+      cell.cy = (int32_t)cr[1];
+      cell.x = cr[2];
+      cell.y = cr[3];
+      cell.packRot = cr[4];
+      for (int32_t m = mtStart[c]; m < mtStart[c + 1]; ++m)
+      {
+         std::vector<isc::Pt3> pts((size_t)(ptStart[m + 1] - ptStart[m]));
+         for (size_t i = 0; i < pts.size(); ++i)
+         {
+            const double* q = xyz + 3 * (ptStart[m] + (int64_t)i);
+            pts[i] = isc::Pt3{q[0], q[1], q[2]};
+         }
+         if (pts.size() < 2)
+            continue;
+         const isc::MtFrames fr = isc::BuildMtFrames(pts);
+         const int nBlocks = (int)std::ceil(fr.Length() / isc::DYE_BLOCK_UM);
+         for (int b = 0; b < nBlocks; ++b)
+         {
+            dyes.clear();
+            isc::DyesInBlock(seed, cell.cx, cell.cy, m - mtStart[c], pts, fr, b, density, ff, dyes, lk);
+            for (const isc::Dye& d : dyes)
+            {
+               double wx, wy;
+               isc::LocalToWorld(cell, d.pos.x, d.pos.y, wx, wy);
+               const double x = lattice(wx), y = lattice(wy), z = lattice(d.pos.z);
+               if (!(z >= win[4] && z < win[5] && x >= win[0] && x < win[2] && y >= win[1] && y < win[3]))
+                  continue;
+               const int ix = std::min(nx - 1, (int)std::floor((x - win[0]) * sx));
+               const int iy = std::min(ny - 1, (int)std::floor((y - win[1]) * sy));
+               const int iz = nz > 1 ? std::min(nz - 1, (int)std::floor((z - win[4]) * sz)) : 0;
+               out[((size_t)iz * ny + iy) * nx + ix] += 1;
+               ++total;
+            }
+         }
+      }
+   }
+   return total;
+}
+
 } // extern "C"

@@ -197,21 +197,7 @@ FIELD = (0.0, 0.0, 25.6, 25.6)   # the widefield workload's 256 x 256 px field a
 
 
 def geometry_from(gd):
-    npcs = np.zeros(len(gd["npcRows"]), host.NPC_DTYPE)
-    r = gd["npcRows"]
-    if len(r):
-        npcs["c"], npcs["n"], npcs["e1"], npcs["e2"] = r[:, 0:3], r[:, 3:6], r[:, 6:9], r[:, 9:12]
-        npcs["cand"] = r[:, 12].astype(np.int32)
-    lam = np.zeros(len(gd["laminRows"]), host.LAMIN_DTYPE)
-    r = gd["laminRows"]
-    if len(r):
-        lam["c"], lam["n"], lam["cand"] = r[:, 0:3], r[:, 3:6], r[:, 6].astype(np.int32)
-    cells = np.zeros(len(gd["cells"]), host.WORLD_CELL_DTYPE)
-    for k in ("cx", "cy", "x", "y", "packRot", "rOuter"):
-        cells[k] = gd["cells"][k]
-    return host.WorldGeometry(cells=cells, mtStart=gd["mtStart"], ptStart=gd["ptStart"], pts=gd["pts"],
-                              npcStart=gd["npcStart"], npcs=npcs, laminStart=gd["laminStart"], lamin=lam,
-                              npcSites=gd["npcSites"], npcSiteReachUm=gd["npcSiteReachUm"])
+    return host.WorldGeometry.from_rows(**gd)
 
 
 @pytest.fixture(scope="module")
