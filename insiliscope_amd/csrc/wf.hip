@@ -190,8 +190,9 @@ __global__ __launch_bounds__(zconv_cols<N>() * N / 8) void zconv_kernel(const fl
 //      [row0, row0 + rows) stored from registers (h = 0) or added to (h = 1, the same lane wrote them).
 // Within a wave the lanes run in lockstep and its LDS operations complete in order, so a line that one wave owns
 // needs only a compiler fence (wave_sync) between the read and the write phases of a stage.
-// Columns 0 and NX/2 are real in (y, z) (Hermitian-symmetric rows of the x transform), so one workgroup takes both
-// as u = a + ib: after the (y, z) transform U(k) = A(k) + iB(k) with A(-k) = conj A(k), B likewise, so
+// Columns 0 and NX/2 are real in (y, z) (Hermitian-symmetric rows of the x transform), so the rows passes carry them
+// as one column u = a + ib (column 0) and one workgroup takes it: after the (y, z) transform U(k) = A(k) + iB(k) with
+// A(-k) = conj A(k), B likewise, so
 //   A Ghat0 + i B GhatN = U(k) P(k) + conj(U(-k)) Q(k),  P = (Ghat0 + GhatN) / 2,  Q = (Ghat0 - GhatN) / 2,
 // and the inverse gives a' + ib'. ky and -ky have the same parity; the packed workgroup's chunk j holds
 // m = 32 j + p at even positions 2p and the m of -ky at odd ones (128, i.e. ky = 256, in place of the duplicate 0),
@@ -335,6 +336,9 @@ __device__ __forceinline__ void line_fft128(float2* L, int q, float2 (&v)[8], co
    }
 }
 
+// A spectrum line's slot in LDS: 128 values and 8 more, so the 4 lines a wave holds start on different banks.
+constexpr int kGLine = 136;
+
 // The half-slab line m of chunk j at position s (0..63); the packed workgroup pairs ky with -ky (same parity h).
 __device__ __forceinline__ int slab_m(bool packed, int h, int j, int s)
 {
@@ -348,8 +352,8 @@ __device__ __forceinline__ int slab_m(bool packed, int h, int j, int s)
    return mp == 0 ? 128 : 256 - mp;
 }
 
-// The work of one workgroup (PACKED: columns 0 and NH together). Two bodies, so that the packed one's extra
-// exchanges do not raise the register allocation of the others.
+// The work of one workgroup (PACKED: columns 0 and NH, packed as column 0). Two bodies, so that the packed one's
+// extra exchanges do not raise the register allocation of the others.
 template <bool PACKED>
 __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny, int nzIn, size_t inCol,
                                           float2* __restrict__ out, int row0, int rows, int nOut, size_t outCol,
@@ -363,10 +367,8 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
    const int tid0 = threadIdx.x;
    const int c = packed ? 0 : (int)blockIdx.x;
    const float2* src = in + (size_t)c * inCol;
-   const float2* srcN = in + (size_t)NH * inCol;
    const float2* G = ghat + (size_t)c * NY * NZ;
    float2* dst = out + (size_t)c * outCol;
-   float2* dstN = out + (size_t)NH * outCol;
 // the lane ids, opaque to the compiler at each batch and chunk so that it recomputes addresses there instead of
 // keeping every unrolled iteration's addresses live (which would spill the register slab)
 #define SLAB_IDS                   \
@@ -400,7 +402,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
             if (z < nzIn && y < ny)
             {
                const size_t o = (size_t)z * ny + y;
-               x = packed ? make_float2(src[o].x, srcN[o].x) : src[o];
+               x = src[o];
             }
             v[r] = h ? cmulf(x, twy[y]) : x;
          }
@@ -419,72 +421,82 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
       for (int j = 0; j < NJ; ++j)
       {
          SLAB_IDS;
-         const int pos = 4 * w + (lane >> 4), q = lane & 15;
+         const int pos = 4 * w + (lane >> 4), q = lane & 15, sl = lane >> 4;
          const int ky = 2 * slab_m(packed, h, j, pos) + h;
          float2* Z = buf + pos * LSZ;
-         // the spectrum, in flight during the forward transform
+         // the spectrum, in flight during the forward transform: in registers, or (the packed workgroup, whose
+         // partner lines exchange it) the wave's 4 lines straight into LDS (one 1 KB line per load, lane i's 16 bytes
+         // at 16 i), holding no registers
          float2 g[8];
+         float2* gw = gx + w * (4 * kGLine);
+         if constexpr (packed)
+         {
 #pragma unroll
-         for (int m = 0; m < 8; ++m)
-            g[m] = G[(size_t)ky * NZ + q + 16 * m];
+            for (int t = 0; t < 4; ++t)
+            {
+               const int kyt = 2 * slab_m(packed, h, j, 4 * w + t) + h;
+               __builtin_amdgcn_global_load_lds(
+                  static_cast<const void*>(G + (size_t)kyt * NZ + 2 * lane),
+                  reinterpret_cast<__attribute__((address_space(3))) void*>(
+                     reinterpret_cast<uintptr_t>(gw + t * kGLine)),
+                  16, 0, 0);
+            }
+         }
+         else
+         {
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+               g[m] = G[(size_t)ky * NZ + q + 16 * m];
+         }
+         const float2* gl = gw + sl * kGLine;
          float2 v[8];
 #pragma unroll
          for (int r = 0; r < 8; ++r)
             v[r] = r < NI ? R[j][r < NI ? r : 0] : make_float2(0.0f, 0.0f);
          line_fft128<false, false>(Z, q, v, twz);
          // v[m] = U at kz = q + 16 m
-         if (packed)
+         if constexpr (packed)
          {
-            // the partner line (-ky) is the neighbouring position (itself for ky = 0 and 256)
-            const bool self = ky == 0 || ky == NY / 2, plus = ky < NY / 2;
+            // the spectrum lines have landed once the wave's loads are done
+            __builtin_amdgcn_s_waitcnt(0);
+            asm volatile("" ::: "memory");
+            wave_sync();
+            // the partner line (-ky) is the neighbouring position (itself for ky = 0 and 256). Per line type:
+            //   0 < ky < 256:  P = G(k),           Q = G'(k)            (G: this line's values, G': the partner's)
+            //   256 < ky:      P = conj G'(-k),    Q = conj G(-k)
+            //   ky = 0, 256:   P = G(k), Q = G(-k) for kz < 64; P = conj G(-k), Q = conj G(k) above;
+            //                  the real values at kz = 0 and 64 paired in slots 0 (P) and 64 (Q)
+            // (branch-free: the lines of one wave are of different types)
+            const bool self = ky == 0 || ky == NY / 2, plus = !self && ky < NY / 2, minus = !self && !plus;
             const int pp = self ? pos : (pos ^ 1);
             const float2* Zp = buf + pp * LSZ;
-            float2* Gown = gx + pos * LSZ;
-            const float2* Gp = gx + pp * LSZ;
-            wave_sync();
+            const float2* Gp = gw + (self ? sl : (sl ^ 1)) * kGLine;
+            const float2* pLine = minus ? Gp : gl;
+            const float2* qLine = plus ? Gp : gl;
 #pragma unroll
             for (int m = 0; m < 8; ++m)
-            {
                Z[sp(q + 16 * m)] = v[m];
-               Gown[sp(q + 16 * m)] = g[m];
-            }
             wave_sync();
 #pragma unroll
             for (int m = 0; m < 8; ++m)
             {
                const int kz = q + 16 * m, nkz = (NZ - kz) & (NZ - 1);
                const float2 up = Zp[sp(nkz)];
-               float2 P, Q;
-               if (self)
+               const bool low = m < 4;
+               const bool pAtK = low ? !minus : plus, qAtK = low ? plus : !minus;
+               // the real slots (kz 0 and 64 of a self-paired line): P from slot 0, Q from slot 64
+               const bool real = (m == 0 || m == 4) && self && q == 0;
+               const bool cj = !real && (low ? minus : !plus);
+               float2 P = pLine[real ? 0 : (pAtK ? kz : nkz)], Q = qLine[real ? NZ / 2 : (qAtK ? kz : nkz)];
+               if (cj)
                {
-                  if (q == 0 && (m == 0 || m == 4))
-                  {
-                     const float2 p = Gown[0], qq = Gown[sp(NZ / 2)];
-                     P = make_float2(m == 0 ? p.x : p.y, 0.0f);
-                     Q = make_float2(m == 0 ? qq.x : qq.y, 0.0f);
-                  }
-                  else if (kz < NZ / 2)
-                  {
-                     P = g[m];
-                     Q = Gown[sp(nkz)];
-                  }
-                  else
-                  {
-                     const float2 p = Gown[sp(nkz)];
-                     P = make_float2(p.x, -p.y);
-                     Q = make_float2(g[m].x, -g[m].y);
-                  }
+                  P.y = -P.y;
+                  Q.y = -Q.y;
                }
-               else if (plus)
+               if (real)
                {
-                  P = g[m];
-                  Q = Gp[sp(kz)];
-               }
-               else
-               {
-                  const float2 p = Gp[sp(nkz)], qq = Gown[sp(nkz)];
-                  P = make_float2(p.x, -p.y);
-                  Q = make_float2(qq.x, -qq.y);
+                  P = make_float2(m == 0 ? P.x : P.y, 0.0f);
+                  Q = make_float2(m == 0 ? Q.x : Q.y, 0.0f);
                }
                v[m] = cmaf(make_float2(up.x, -up.y), Q, cmulf(v[m], P));
                // one k at a time: the scheduler would otherwise issue every iteration's LDS reads at once
@@ -521,7 +533,6 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
          __syncthreads();
          const int fl = 2 * w + (lane >> 5), q = lane & 31, f = 32 * b + fl;
          float2* d = dst + (size_t)f * rows;
-         float2* dN = dstN + (size_t)f * rows;
          // h = 1 adds to the rows this lane stored for h = 0: their loads overlap the transform (the first 256)
          constexpr int KP = 8;
          float2 prev[KP];
@@ -531,7 +542,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
             const int yo = q + 32 * k;
             prev[k] = make_float2(0.0f, 0.0f);
             if (h && f < nOut && yo < rows)
-               prev[k] = packed ? make_float2(d[yo].x, dN[yo].x) : d[yo];
+               prev[k] = d[yo];
          }
          float2 v[8];
          float2* Lf = buf + fl * LSY;
@@ -548,13 +559,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
                   const float2 tw = twy[y];
                   val = cmaf(val, make_float2(tw.x, -tw.y), p);
                }
-               if (packed)
-               {
-                  d[yo] = make_float2(val.x, 0.0f);
-                  dN[yo] = make_float2(val.y, 0.0f);
-               }
-               else
-                  d[yo] = val;
+               d[yo] = val;
             };
 #pragma unroll
             for (int k = 0; k < KP; ++k)
@@ -562,7 +567,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
                   put(q + 32 * k, prev[k]);
 #pragma unroll 1
             for (int yo = q + 32 * KP; yo < rows; yo += 32)
-               put(yo, h ? (packed ? make_float2(d[yo].x, dN[yo].x) : d[yo]) : make_float2(0.0f, 0.0f));
+               put(yo, h ? d[yo] : make_float2(0.0f, 0.0f));
          }
          __syncthreads();
       }
@@ -580,7 +585,7 @@ __global__ __launch_bounds__(1024) void slab_kernel(const float2* __restrict__ i
    constexpr int LSY = 273, LSZ = 137;
    constexpr int BUF = 32 * LSY > 64 * LSZ ? 32 * LSY : 64 * LSZ;
    __shared__ float2 buf[BUF];
-   __shared__ float2 gx[64 * LSZ];  // the packed workgroup's spectrum lines, exchanged between partners
+   __shared__ float2 gx[64 * kGLine];  // each wave's 4 spectrum lines of the current chunk
    __shared__ float2 twy[NY];
    __shared__ float2 twz[NZ];
    for (int i = threadIdx.x; i < NY; i += 1024)

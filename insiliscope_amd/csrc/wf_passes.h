@@ -189,6 +189,8 @@ __global__ __launch_bounds__(256) void rows_c2r_kernel(const float2* __restrict_
 // Passes A and E for the fused slab pass (wf.hip), whose half spectra are column-major: column k's rows are
 // contiguous, element (row, k) at k * colStride + row (colStride even). kCmThreads threads, kCmThreads / (N / 8) lines
 // per block (N = 512: 8 complex lines = 16 rows, 128-byte column segments); rows r and r + 1 move as one 16-byte pair.
+// The columns 0 and N/2 of a real row are real; they travel packed as one complex column, a + ib in column 0 (the
+// slab pass transforms the pair as one), and column N/2 is neither written nor read.
 constexpr int kCmThreads = 256;
 template <int N>
 __global__ __launch_bounds__(kCmThreads) void rows_r2c_cm_kernel(const float* __restrict__ in, size_t inRowStride,
@@ -220,9 +222,18 @@ __global__ __launch_bounds__(kCmThreads) void rows_r2c_cm_kernel(const float* __
          const int k = i / LINES, l = i % LINES, pr = p0 + l;
          if (pr >= pairs)
             continue;
+         if (k == N / 2)
+            continue;
          const float2 z = buf[l * LS + k], zc = buf[l * LS + ((N - k) & (N - 1))];
-         const float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
-         const float2 B = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+         float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+         float2 B = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+         if (k == 0)
+         {
+            // columns 0 and N/2, both real: z[0] = a0 + i b0, z[N/2] = aN + i bN
+            const float2 zN = buf[l * LS + N / 2];
+            A = make_float2(z.x, zN.x);
+            B = make_float2(z.y, zN.y);
+         }
          float2* o = out + (size_t)k * colStride + 2 * pr;
          if (2 * pr + 1 < nRows)
             *reinterpret_cast<float4*>(o) = make_float4(A.x, A.y, B.x, B.y);
@@ -250,7 +261,8 @@ __global__ __launch_bounds__(kCmThreads) void rows_c2r_cm_kernel(const float2* _
          float2 A = make_float2(0.0f, 0.0f), B = make_float2(0.0f, 0.0f);
          if (pr < pairs)
          {
-            const float2* s = in + (size_t)k * colStride + 2 * pr;
+            // column N/2 is the imaginary part of the packed column 0
+            const float2* s = in + (size_t)(k == N / 2 ? 0 : k) * colStride + 2 * pr;
             if (2 * pr + 1 < nRows)
             {
                const float4 ab = *reinterpret_cast<const float4*>(s);
@@ -259,6 +271,11 @@ __global__ __launch_bounds__(kCmThreads) void rows_c2r_cm_kernel(const float2* _
             }
             else
                A = s[0];
+            if (k == N / 2)
+            {
+               A.x = A.y;
+               B.x = B.y;
+            }
          }
          if (k == 0 || k == N / 2)
          {
