@@ -421,86 +421,81 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
       for (int j = 0; j < NJ; ++j)
       {
          SLAB_IDS;
-         const int pos = 4 * w + (lane >> 4), q = lane & 15, sl = lane >> 4;
+         const int pos = 4 * w + (lane >> 4), q = lane & 15;
          const int ky = 2 * slab_m(packed, h, j, pos) + h;
          float2* Z = buf + pos * LSZ;
          // the spectrum, in flight during the forward transform: in registers, or (the packed workgroup, whose
          // partner lines exchange it) the wave's 4 lines straight into LDS (one 1 KB line per load, lane i's 16 bytes
          // at 16 i), holding no registers
          float2 g[8];
-         float2* gw = gx + w * (4 * kGLine);
-         if constexpr (packed)
-         {
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
-            {
-               const int kyt = 2 * slab_m(packed, h, j, 4 * w + t) + h;
-               __builtin_amdgcn_global_load_lds(
-                  static_cast<const void*>(G + (size_t)kyt * NZ + 2 * lane),
-                  reinterpret_cast<__attribute__((address_space(3))) void*>(
-                     reinterpret_cast<uintptr_t>(gw + t * kGLine)),
-                  16, 0, 0);
-            }
-         }
-         else
-         {
-#pragma unroll
-            for (int m = 0; m < 8; ++m)
-               g[m] = G[(size_t)ky * NZ + q + 16 * m];
-         }
-         const float2* gl = gw + sl * kGLine;
+         for (int m = 0; m < 8; ++m)
+            g[m] = G[(size_t)ky * NZ + q + 16 * m];
+         // the packed workgroup's pairs (ky, -ky < 0) sit in neighbouring 16-lane rows; the -ky row transforms the
+         // conjugate of its line, so that lane (q, m) of both rows holds U(ky, kz) and W(kz) = conj U(-ky, -kz)
+         const bool self = packed && (ky == 0 || ky == NY / 2), minus = packed && !self && ky > NY / 2;
          float2 v[8];
 #pragma unroll
          for (int r = 0; r < 8; ++r)
+         {
             v[r] = r < NI ? R[j][r < NI ? r : 0] : make_float2(0.0f, 0.0f);
+            if (minus)
+               v[r].y = -v[r].y;
+         }
          line_fft128<false, false>(Z, q, v, twz);
          // v[m] = U at kz = q + 16 m
          if constexpr (packed)
          {
-            // the spectrum lines have landed once the wave's loads are done
-            __builtin_amdgcn_s_waitcnt(0);
-            asm volatile("" ::: "memory");
-            wave_sync();
-            // the partner line (-ky) is the neighbouring position (itself for ky = 0 and 256). Per line type:
-            //   0 < ky < 256:  P = G(k),           Q = G'(k)            (G: this line's values, G': the partner's)
-            //   256 < ky:      P = conj G'(-k),    Q = conj G(-k)
-            //   ky = 0, 256:   P = G(k), Q = G(-k) for kz < 64; P = conj G(-k), Q = conj G(k) above;
-            //                  the real values at kz = 0 and 64 paired in slots 0 (P) and 64 (Q)
-            // (branch-free: the lines of one wave are of different types)
-            const bool self = ky == 0 || ky == NY / 2, plus = !self && ky < NY / 2, minus = !self && !plus;
-            const int pp = self ? pos : (pos ^ 1);
-            const float2* Zp = buf + pp * LSZ;
-            const float2* Gp = gw + (self ? sl : (sl ^ 1)) * kGLine;
-            const float2* pLine = minus ? Gp : gl;
-            const float2* qLine = plus ? Gp : gl;
+            // V(k) = U(k) P(k) + conj(U(-k)) Q(k) with P, Q Hermitian, stored P(ky, kz) on the line ky and Q(ky, kz)
+            // on the line -ky, both at kz. With W = conj U(-ky, -kz) from the conjugated transform of the -ky row:
+            //   row ky:   V(ky, kz) = U P + W Q;   row -ky: V(-ky, -kz) = conj(W P + U Q),
+            // so both rows form X = own P + other Q from the partner row's values at the same lane (v_permlane16_swap
+            // trades the rows of a pair), and the -ky row's inverse of X, conjugated, is its line. The self-paired
+            // lines ky = 0 and 256 (two rows of one wave) take U(-k) and their P, Q through LDS instead: P = G(k),
+            // Q = G(-k) for kz < 64, P = conj G(-k), Q = conj G(k) above, the real values at kz = 0 and 64 paired in
+            // slots 0 (P) and 64 (Q).
+            const bool oddRow = (lane >> 4) & 1;
+            auto swp = [&](float x) {
+               const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+               return __uint_as_float(oddRow ? r[0] : r[1]);
+            };
+            float2* gs = gx + pos * kGLine;
+            if (self)
+            {
 #pragma unroll
-            for (int m = 0; m < 8; ++m)
-               Z[sp(q + 16 * m)] = v[m];
+               for (int m = 0; m < 8; ++m)
+               {
+                  Z[sp(q + 16 * m)] = v[m];
+                  gs[q + 16 * m] = g[m];
+               }
+            }
             wave_sync();
 #pragma unroll
             for (int m = 0; m < 8; ++m)
             {
-               const int kz = q + 16 * m, nkz = (NZ - kz) & (NZ - 1);
-               const float2 up = Zp[sp(nkz)];
-               const bool low = m < 4;
-               const bool pAtK = low ? !minus : plus, qAtK = low ? plus : !minus;
-               // the real slots (kz 0 and 64 of a self-paired line): P from slot 0, Q from slot 64
-               const bool real = (m == 0 || m == 4) && self && q == 0;
-               const bool cj = !real && (low ? minus : !plus);
-               float2 P = pLine[real ? 0 : (pAtK ? kz : nkz)], Q = qLine[real ? NZ / 2 : (qAtK ? kz : nkz)];
-               if (cj)
+               const float2 ov = make_float2(swp(v[m].x), swp(v[m].y));
+               const float2 og = make_float2(swp(g[m].x), swp(g[m].y));
+               const float2 P = minus ? og : g[m], Q = minus ? g[m] : og;
+               float2 x = cmaf(ov, Q, cmulf(v[m], P));
+               if (self)
                {
-                  P.y = -P.y;
-                  Q.y = -Q.y;
+                  const int kz = q + 16 * m, nkz = (NZ - kz) & (NZ - 1);
+                  const float2 up = Z[sp(nkz)];
+                  const bool low = m < 4, real = (m == 0 || m == 4) && q == 0;
+                  float2 P2 = gs[real ? 0 : (low ? kz : nkz)], Q2 = gs[real ? NZ / 2 : (low ? nkz : kz)];
+                  if (!low && !real)
+                  {
+                     P2.y = -P2.y;
+                     Q2.y = -Q2.y;
+                  }
+                  if (real)
+                  {
+                     P2 = make_float2(m == 0 ? P2.x : P2.y, 0.0f);
+                     Q2 = make_float2(m == 0 ? Q2.x : Q2.y, 0.0f);
+                  }
+                  x = cmaf(make_float2(up.x, -up.y), Q2, cmulf(v[m], P2));
                }
-               if (real)
-               {
-                  P = make_float2(m == 0 ? P.x : P.y, 0.0f);
-                  Q = make_float2(m == 0 ? Q.x : Q.y, 0.0f);
-               }
-               v[m] = cmaf(make_float2(up.x, -up.y), Q, cmulf(v[m], P));
-               // one k at a time: the scheduler would otherwise issue every iteration's LDS reads at once
-               __builtin_amdgcn_sched_barrier(0);
+               v[m] = x;
             }
          }
          else
@@ -513,7 +508,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
          line_fft128<true, true>(Z, q, v, twz);
 #pragma unroll
          for (int i = 0; i < NI; ++i)
-            R[j][i] = v[i];
+            R[j][i] = minus ? make_float2(v[i].x, -v[i].y) : v[i];
          wave_sync();
       }
       __syncthreads();
@@ -585,7 +580,7 @@ __global__ __launch_bounds__(1024) void slab_kernel(const float2* __restrict__ i
    constexpr int LSY = 273, LSZ = 137;
    constexpr int BUF = 32 * LSY > 64 * LSZ ? 32 * LSY : 64 * LSZ;
    __shared__ float2 buf[BUF];
-   __shared__ float2 gx[64 * kGLine];  // each wave's 4 spectrum lines of the current chunk
+   __shared__ float2 gx[4 * kGLine];  // the self-paired lines' spectrum values (positions 0 and 1 of the packed workgroup)
    __shared__ float2 twy[NY];
    __shared__ float2 twz[NZ];
    for (int i = threadIdx.x; i < NY; i += 1024)
