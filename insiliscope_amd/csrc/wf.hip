@@ -380,7 +380,23 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
    for (int h = 0; h < 2; ++h)
    {
       float2 R[NJ][NI];
-      // 1. y forward, z lines 32 b + 2 w + (lane >> 5)
+      // 1. y forward, z lines 32 b + 2 w + (lane >> 5); the second batch's rows are loaded before the first
+      // batch's transform, so their latency overlaps it
+      float2 xb[2][8];
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+      {
+         SLAB_IDS;
+         const int z = 32 * b + 2 * w + (lane >> 5), q = lane & 31;
+#pragma unroll
+         for (int r = 0; r < 8; ++r)
+         {
+            const int y = q + 32 * r;  // < 256: ny <= 256 on this path
+            xb[b][r] = make_float2(0.0f, 0.0f);
+            if (z < nzIn && y < ny)
+               xb[b][r] = src[(size_t)z * ny + y];
+         }
+      }
 #pragma unroll
       for (int b = 0; b < 2; ++b)
       {
@@ -392,20 +408,11 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
                R[j][2 * b] = R[j][2 * b + 1] = make_float2(0.0f, 0.0f);
             continue;
          }
-         const int zl = 2 * w + (lane >> 5), q = lane & 31, z = 32 * b + zl;
+         const int zl = 2 * w + (lane >> 5), q = lane & 31;
          float2 v[8];
 #pragma unroll
          for (int r = 0; r < 8; ++r)
-         {
-            const int y = q + 32 * r;  // < 256: ny <= 256 on this path
-            float2 x = make_float2(0.0f, 0.0f);
-            if (z < nzIn && y < ny)
-            {
-               const size_t o = (size_t)z * ny + y;
-               x = src[o];
-            }
-            v[r] = h ? cmulf(x, twy[y]) : x;
-         }
+            v[r] = h ? cmulf(xb[b][r], twy[q + 32 * r]) : xb[b][r];
          line_fft256<false, true, false>(buf + zl * LSY, q, v, twy);
          __syncthreads();
 #pragma unroll
