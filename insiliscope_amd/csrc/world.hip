@@ -559,18 +559,72 @@ __global__ __launch_bounds__(kThreads) void block_count_kernel(Geo g, QueryArgs 
    }
 }
 
-// RAW false: the dyes as the world keeps them (world coordinates on the lattice) with their block; RAW true: the
-// generators' own output (cell-local, unrounded) as iscg_block_dye rows, those below capacity.
-template <bool RAW>
+// What a density query bins into (Density3dInWindow's grid and scales).
+struct DensityArgs
+{
+   float* out;
+   unsigned long long* total;
+   int32_t nx, ny, nz;
+   uint32_t mask;
+   double sx, sy, sz;
+};
+
+constexpr int kFillRecords = 0, kFillRaw = 1, kFillDensity = 2;
+// Voxels of a block's local counts in LDS (a box around its midpoint holding every dye it can make).
+constexpr int kLocalBins = 4096;
+
+// MODE kFillRecords: the dyes as the world keeps them (world coordinates on the lattice) with their block;
+// kFillRaw: the generators' own output (cell-local, unrounded) as iscg_block_dye rows, those below capacity;
+// kFillDensity: no records, each dye in the window counted into its voxel (Density3dInWindow), through LDS counts
+// over a box around the block's midpoint that every dye of the block falls in (its reach), added to the grid once
+// per voxel (integer counts: the same in any order).
+template <int MODE>
 __global__ __launch_bounds__(kThreads) void block_fill_kernel(Geo g, QueryArgs q, const int32_t* __restrict__ sel,
                                                               const int64_t* __restrict__ dyeStart,
                                                               DyeRec* __restrict__ dyes, int32_t* __restrict__ dyeBlock,
-                                                              iscg_block_dye* __restrict__ raw, int64_t capacity)
+                                                              iscg_block_dye* __restrict__ raw, int64_t capacity,
+                                                              DensityArgs den)
 {
    const int s = blockIdx.x;
    const BlockView v = block_view(g, q, sel[s]);
-   const int32_t cellIdx = g.cands[sel[s]].cell;
+   const Cand cand = g.cands[sel[s]];
+   const int32_t cellIdx = cand.cell;
    const iscg_world_cell cell = g.cells[cellIdx];
+   // the density box: voxels [b0, b0 + bn) per axis (x, y, z), used when it fits kLocalBins
+   __shared__ int box[7];
+   __shared__ float bins[MODE == kFillDensity ? kLocalBins : 1];
+   long long counted = 0;
+   bool binThis = false;
+   if constexpr (MODE == kFillDensity)
+   {
+      binThis = ((den.mask >> v.kind) & 1u) != 0;
+      if (threadIdx.x == 0)
+      {
+         const double rad = v.kind == 0 ? q.blockReach : v.kind == 1 ? q.npcRb : q.laminRb;
+         const Window& w = q.win;
+         auto range = [&](double c, double lo, double sc, int n, int& b0, int& bn) {
+            const double f0 = fmax(-2.0, fmin((double)n + 1, floor((c - rad - lo) * sc) - 1));
+            const double f1 = fmax(-2.0, fmin((double)n + 1, floor((c + rad - lo) * sc) + 1));
+            const int i0 = max(0, (int)f0), i1 = min(n - 1, (int)f1);
+            b0 = i0;
+            bn = i1 >= i0 ? i1 - i0 + 1 : 0;
+         };
+         range(cand.wx, w.x0, den.sx, den.nx, box[0], box[3]);
+         range(cand.wy, w.y0, den.sy, den.ny, box[1], box[4]);
+         if (den.nz > 1)
+            range(cand.z, w.zMin, den.sz, den.nz, box[2], box[5]);
+         else
+         {
+            box[2] = 0;
+            box[5] = 1;
+         }
+         const long long nb = (long long)box[3] * box[4] * box[5];
+         box[6] = nb <= kLocalBins ? (int)nb : -1;
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < max(box[6], 0); i += kThreads)
+         bins[i] = 0.0f;
+   }
    __shared__ double sc[16], ss[16];  // per protofilament (cos, sin theta), or per pore corner
    __shared__ double rotc, rots;
    __shared__ int rotated;
@@ -598,16 +652,17 @@ __global__ __launch_bounds__(kThreads) void block_fill_kernel(Geo g, QueryArgs q
    }
    __syncthreads();
    const LabelDev& L = q.lab[v.kind];
-   long long base = dyeStart[s];
+   long long base = MODE == kFillDensity ? 0 : dyeStart[s];
    for (int j0 = 0; j0 < v.nCand; j0 += kThreads)
    {
       const int j = j0 + threadIdx.x;
       Site st{};
       if (j < v.nCand)
          st = site_of(g, q, v, j);
-      long long tot;
-      const long long ex = block_exclusive<kThreads / 64>(st.on ? 1 : 0, tot);
-      if (st.on)
+      long long tot = 0, ex = 0;
+      if constexpr (MODE != kFillDensity)
+         ex = block_exclusive<kThreads / 64>(st.on ? 1 : 0, tot);
+      if (st.on && (MODE != kFillDensity || binThis))
       {
          const Address at = Address::of(v.h1, (uint32_t)st.k, (uint32_t)st.n);
          const double r1 = at.draw(CH_LINK_U), r2 = at.draw(CH_LINK_PHI), r3 = at.draw(CH_LINK_R);
@@ -650,7 +705,7 @@ __global__ __launch_bounds__(kThreads) void block_fill_kernel(Geo g, QueryArgs q
             pos = linked_dye(Pt{pq.c[0], pq.c[1], pq.c[2]}, a, 0, L, at, r1, r2, r3);
          }
          const long long d = base + ex;
-         if constexpr (RAW)
+         if constexpr (MODE == kFillRaw)
          {
             if (d < capacity)
             {
@@ -674,12 +729,50 @@ __global__ __launch_bounds__(kThreads) void block_fill_kernel(Geo g, QueryArgs q
             wx = cell.x + pos.x * rotc - pos.y * rots;
             wy = cell.y + pos.x * rots + pos.y * rotc;
          }
-         dyes[d] = DyeRec{on_lattice(wx), on_lattice(wy), on_lattice(pos.z), at.word(CH_LABEL),
+         const DyeRec rec{on_lattice(wx), on_lattice(wy), on_lattice(pos.z), at.word(CH_LABEL),
                           ((uint32_t)st.k << 24) | ((uint32_t)st.n & 0xFFFFFFu)};
-         dyeBlock[d] = s;
+         if constexpr (MODE == kFillRecords)
+         {
+            dyes[d] = rec;
+            dyeBlock[d] = s;
+         }
+         else
+         {
+            const Window& w = q.win;
+            if (rec.z >= w.zMin && rec.z < w.zMax && rec.x >= w.x0 && rec.x < w.x1 && rec.y >= w.y0 && rec.y < w.y1)
+            {
+               const int ix = min(den.nx - 1, (int)floor((rec.x - w.x0) * den.sx));
+               const int iy = min(den.ny - 1, (int)floor((rec.y - w.y0) * den.sy));
+               const int iz = den.nz > 1 ? min(den.nz - 1, (int)floor((rec.z - w.zMin) * den.sz)) : 0;
+               const int lx = ix - box[0], ly = iy - box[1], lz = iz - box[2];
+               if (box[6] > 0 && lx >= 0 && lx < box[3] && ly >= 0 && ly < box[4] && lz >= 0 && lz < box[5])
+                  atomicAdd(bins + (lz * box[4] + ly) * box[3] + lx, 1.0f);
+               else
+                  atomicAdd(den.out + ((size_t)iz * den.ny + iy) * den.nx + ix, 1.0f);
+               ++counted;
+            }
+         }
          }
       }
       base += tot;
+   }
+   if constexpr (MODE == kFillDensity)
+   {
+      __syncthreads();
+      const int nb = box[6];
+      for (int i = threadIdx.x; i < nb; i += kThreads)
+      {
+         const float c = bins[i];
+         if (c != 0.0f)
+         {
+            const int lx = i % box[3], ly = (i / box[3]) % box[4], lz = i / (box[3] * box[4]);
+            atomicAdd(den.out + ((size_t)(box[2] + lz) * den.ny + (box[1] + ly)) * den.nx + (box[0] + lx), c);
+         }
+      }
+      long long tot;
+      (void)block_exclusive<kThreads / 64>(counted, tot);
+      if (threadIdx.x == 0 && tot)
+         atomicAdd(den.total, (unsigned long long)tot);
    }
 }
 
@@ -726,35 +819,6 @@ __global__ __launch_bounds__(kThreads) void zero_kernel(float* __restrict__ out,
       out[i] = 0.0f;
    if (blockIdx.x == 0 && threadIdx.x == 0)
       *total = 0;
-}
-
-// Density3dInWindow's binning (core/src/world.cpp:1519-1543): integer counts, the same in any order.
-__global__ __launch_bounds__(kThreads) void density_kernel(const DyeRec* __restrict__ dyes,
-                                                           const int32_t* __restrict__ dyeBlock,
-                                                           const BlockMeta* __restrict__ meta, int64_t n, Window w,
-                                                           uint32_t mask, int32_t nx, int32_t ny, int32_t nz,
-                                                           double sx, double sy, double sz, float* __restrict__ out,
-                                                           unsigned long long* __restrict__ total)
-{
-   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-   long long c = 0;
-   if (i < n)
-   {
-      const DyeRec d = dyes[i];
-      const int s = meta[dyeBlock[i]].structure;
-      if (((mask >> s) & 1u) && in_window(d, w))
-      {
-         const int ix = min(nx - 1, (int)floor((d.x - w.x0) * sx));
-         const int iy = min(ny - 1, (int)floor((d.y - w.y0) * sy));
-         const int iz = nz > 1 ? min(nz - 1, (int)floor((d.z - w.zMin) * sz)) : 0;
-         atomicAdd(out + ((size_t)iz * ny + iy) * nx + ix, 1.0f);
-         c = 1;
-      }
-   }
-   long long tot;
-   (void)block_exclusive<kThreads / 64>(c, tot);
-   if (threadIdx.x == 0 && tot)
-      atomicAdd(total, (unsigned long long)tot);
 }
 
 // The in-window dyes' addresses and structures, compacted in dye order (winIdx: their dye index).
@@ -1243,9 +1307,8 @@ bool valid_window(const double* win)
    return true;
 }
 
-// The dye blocks of a query: select, count, scan, fill. *nSel, *nDyes (host).
-// raw: only select and count (iscg_world_block_dyes writes the generators' rows itself).
-bool make_dyes(iscg_world* w, const QueryArgs& q, int64_t& nSel, int64_t& nDyes, bool raw = false)
+// The candidate blocks the window's walk takes, in walk order (w->sel[0 .. nSel)).
+bool select_blocks(iscg_world* w, const QueryArgs& q, int64_t& nSel)
 {
    hipStream_t st = w->h->stream;
    const int64_t n = w->nCands;
@@ -1264,6 +1327,16 @@ bool make_dyes(iscg_world* w, const QueryArgs& q, int64_t& nSel, int64_t& nDyes,
    if (n)
       hipLaunchKernelGGL(compact_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, w->flag.as<int32_t>(),
                          w->off.as<int64_t>(), n, w->sel.as<int32_t>());
+   return wok(w, hipGetLastError(), "world selection");
+}
+
+// The dye blocks of a query: select, count, scan, fill. *nSel, *nDyes (host). raw: only select and count
+// (iscg_world_block_dyes writes the generators' rows itself).
+bool make_dyes(iscg_world* w, const QueryArgs& q, int64_t& nSel, int64_t& nDyes, bool raw = false)
+{
+   hipStream_t st = w->h->stream;
+   if (!select_blocks(w, q, nSel))
+      return false;
    if (!reserve<int32_t>(w, w->counts, nSel, "world block counts") ||
        !reserve<BlockMeta>(w, w->meta, nSel, "world block meta") ||
        !reserve<int64_t>(w, w->dyeStart, nSel + 1, "world dye starts"))
@@ -1288,9 +1361,9 @@ bool make_dyes(iscg_world* w, const QueryArgs& q, int64_t& nSel, int64_t& nDyes,
    if (nSel && !raw)
    {
       ProfScope ps(w->h, "world_block_fill");
-      hipLaunchKernelGGL(block_fill_kernel<false>, dim3((unsigned)nSel), dim3(kThreads), 0, st, g, q,
+      hipLaunchKernelGGL(block_fill_kernel<kFillRecords>, dim3((unsigned)nSel), dim3(kThreads), 0, st, g, q,
                          w->sel.as<int32_t>(), w->dyeStart.as<int64_t>(), w->dyes.as<DyeRec>(),
-                         w->dyeBlock.as<int32_t>(), nullptr, 0);
+                         w->dyeBlock.as<int32_t>(), nullptr, 0, DensityArgs{});
    }
    return wok(w, hipGetLastError(), "world dye blocks");
 }
@@ -1585,8 +1658,8 @@ int32_t iscg_world_block_dyes(iscg_world* w, const double window[6], iscg_block_
    if (!dout)
       return -1;
    const Geo g = geo_of(w);
-   hipLaunchKernelGGL(block_fill_kernel<true>, dim3((unsigned)nSel), dim3(kThreads), 0, w->h->stream, g, q,
-                      w->sel.as<int32_t>(), w->dyeStart.as<int64_t>(), nullptr, nullptr, dout, nOut);
+   hipLaunchKernelGGL(block_fill_kernel<kFillRaw>, dim3((unsigned)nSel), dim3(kThreads), 0, w->h->stream, g, q,
+                      w->sel.as<int32_t>(), w->dyeStart.as<int64_t>(), nullptr, nullptr, dout, nOut, DensityArgs{});
    if (!wok(w, hipGetLastError(), "iscg_world_block_dyes"))
       return -1;
    return finish_out(w->h, out, dout, (size_t)nOut * sizeof(iscg_block_dye)) ? 0 : -1;
@@ -1641,18 +1714,25 @@ int32_t iscg_world_density3d(iscg_world* w, const double window[6], int32_t nx, 
    if (structureMask)
    {
       const QueryArgs q = query_args(w, window);
-      int64_t nSel = 0, nDyes = 0;
-      if (!make_dyes(w, q, nSel, nDyes))
+      int64_t nSel = 0;
+      if (!select_blocks(w, q, nSel))
          return -1;
       // Density3dInWindow's scales (core/src/world.cpp:1527-1529)
-      const double sx = nx / (window[2] - window[0]), sy = ny / (window[3] - window[1]);
-      const double sz = nz > 1 ? nz / (window[5] - window[4]) : 0.0;
-      if (nDyes)
+      DensityArgs den{};
+      den.out = dout;
+      den.total = w->total.as<unsigned long long>();
+      den.nx = nx;
+      den.ny = ny;
+      den.nz = nz;
+      den.mask = structureMask;
+      den.sx = nx / (window[2] - window[0]);
+      den.sy = ny / (window[3] - window[1]);
+      den.sz = nz > 1 ? nz / (window[5] - window[4]) : 0.0;
+      if (nSel)
       {
          ProfScope ps(w->h, "world_density");
-         hipLaunchKernelGGL(density_kernel, dim3(blocks_for(nDyes)), dim3(kThreads), 0, st, w->dyes.as<DyeRec>(),
-                            w->dyeBlock.as<int32_t>(), w->meta.as<BlockMeta>(), nDyes, q.win, structureMask, nx, ny,
-                            nz, sx, sy, sz, dout, w->total.as<unsigned long long>());
+         hipLaunchKernelGGL(block_fill_kernel<kFillDensity>, dim3((unsigned)nSel), dim3(kThreads), 0, st, geo_of(w), q,
+                            w->sel.as<int32_t>(), nullptr, nullptr, nullptr, nullptr, 0, den);
       }
    }
    if (!wok(w, hipGetLastError(), "iscg_world_density3d") ||
