@@ -297,21 +297,31 @@ __global__ __launch_bounds__(kThreads) void mids_kernel(Cand* __restrict__ cands
    c.z = lz;
 }
 
-// The farthest pore (which 0) or lamina point (1) centre from the cell centre in xy (EnvelopeBlocks' reachXY), one
-// lane per cell.
+// The farthest pore or lamina point centre from the cell centre in xy (EnvelopeBlocks' reachXY), one workgroup per
+// cell (a maximum: the same in any order).
 __global__ __launch_bounds__(kThreads) void env_reach_kernel(const int32_t* __restrict__ start, const Pt* __restrict__ c0,
                                                              int strideDoubles, int32_t nCells, double* __restrict__ out)
 {
-   const int cidx = blockIdx.x * kThreads + threadIdx.x;
-   if (cidx >= nCells)
-      return;
+   __shared__ double part[kThreads / 64];
+   const int cidx = blockIdx.x;
    double r = 0;
-   for (int i = start[cidx]; i < start[cidx + 1]; ++i)
+   for (int i = start[cidx] + threadIdx.x; i < start[cidx + 1]; i += kThreads)
    {
       const double* q = reinterpret_cast<const double*>(c0) + (size_t)i * strideDoubles;
       r = dmax(r, jsm::hypot(q[0], q[1]));
    }
-   out[cidx] = r;
+   for (int d = 32; d > 0; d >>= 1)
+      r = dmax(r, __shfl_xor(r, d, 64));
+   if ((threadIdx.x & 63) == 0)
+      part[threadIdx.x >> 6] = r;
+   __syncthreads();
+   if (threadIdx.x == 0)
+   {
+      double m = 0;
+      for (int w = 0; w < kThreads / 64; ++w)
+         m = dmax(m, part[w]);
+      out[cidx] = m;
+   }
 }
 
 // ---- scans -------------------------------------------------------------------------------------------------------
@@ -1605,11 +1615,11 @@ int32_t iscg_world_create(iscg_context* h, uint32_t seed, const iscg_world_geome
                          w->cum.as<double>(), w->T.as<Pt>(), w->npcB.as<EnvBlock>(), w->lamB.as<EnvBlock>());
    if (nCells)
    {
-      hipLaunchKernelGGL(env_reach_kernel, dim3(blocks_for(nCells)), dim3(kThreads), 0, st,
-                         w->npcStart.as<int32_t>(), reinterpret_cast<const Pt*>(w->npcs.as<iscg_npc>()),
-                         (int)(sizeof(iscg_npc) / sizeof(double)), nCells, w->npcReach.as<double>());
-      hipLaunchKernelGGL(env_reach_kernel, dim3(blocks_for(nCells)), dim3(kThreads), 0, st,
-                         w->laminStart.as<int32_t>(), reinterpret_cast<const Pt*>(w->lamin.as<iscg_lamin_point>()),
+      hipLaunchKernelGGL(env_reach_kernel, dim3(nCells), dim3(kThreads), 0, st, w->npcStart.as<int32_t>(),
+                         reinterpret_cast<const Pt*>(w->npcs.as<iscg_npc>()), (int)(sizeof(iscg_npc) / sizeof(double)),
+                         nCells, w->npcReach.as<double>());
+      hipLaunchKernelGGL(env_reach_kernel, dim3(nCells), dim3(kThreads), 0, st, w->laminStart.as<int32_t>(),
+                         reinterpret_cast<const Pt*>(w->lamin.as<iscg_lamin_point>()),
                          (int)(sizeof(iscg_lamin_point) / sizeof(double)), nCells, w->laminReach.as<double>());
    }
    if (!wok(w, hipGetLastError(), "world blocks") || !wok(w, hipStreamSynchronize(st), "world blocks"))
