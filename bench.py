@@ -595,15 +595,15 @@ def run_dyes(ctx, steps, warmup, n=1 << 21):
     total = torch.zeros(1, dtype=torch.int64, device="cuda")
     ctx.follow_torch()
     L = ctx.L
-    ctx.check(L.iscg_label_blinks(ctx.h, dd.data_ptr(), n, 0, host.C.byref(kin), DYE_T_MAX, offsets.data_ptr(), None,
-                                  0, total.data_ptr()), "iscg_label_blinks")
+    ctx.check(L.iscg_label_blinks(ctx.h, dd.data_ptr(), n, 0, host.C.byref(kin), None, DYE_T_MAX, offsets.data_ptr(),
+                                  None, 0, total.data_ptr()), "iscg_label_blinks")
     torch.cuda.synchronize()
     cap = int(total.item())
     out = torch.empty(cap * 3, dtype=torch.float64, device="cuda")
 
     def step(_=0):
-        ctx.check(L.iscg_label_blinks(ctx.h, dd.data_ptr(), n, 0, host.C.byref(kin), DYE_T_MAX, offsets.data_ptr(),
-                                      out.data_ptr(), cap, total.data_ptr()), "iscg_label_blinks")
+        ctx.check(L.iscg_label_blinks(ctx.h, dd.data_ptr(), n, 0, host.C.byref(kin), None, DYE_T_MAX,
+                                      offsets.data_ptr(), out.data_ptr(), cap, total.data_ptr()), "iscg_label_blinks")
 
     for _ in range(warmup):
         step()

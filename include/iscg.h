@@ -534,6 +534,18 @@ typedef struct
    double activationRatePerSec, onSec, offSec, bleachProb, photonCV, initialOnSec;
 } iscg_kinetics;
 
+/* isc::KineticsHistory (core/src/dyes.h:138-149): n segments of constant kinetics, segment i from tStart[i] (tStart[0]
+ * = 0, then strictly increasing, all finite) to tStart[i+1], the last for ever (host arrays of n, at most 65536). The
+ * schedule calls take it as LabelSchedule and PersistentBlinks take theirs: NULL or n = 0 none (the kinetics argument
+ * from t = 0), n = 1 kin[0] from t = 0, n >= 2 the draws spent over the segments. A call given two or more segments
+ * synchronizes the context's stream once, after copying them to the device. */
+typedef struct
+{
+   int32_t n, reserved;
+   const double* tStart;
+   const iscg_kinetics* kin;
+} iscg_kinetics_history;
+
 /* isc::Blink (core/src/dyes.h:197) and isc::BinBlink (:200). */
 typedef struct
 {
@@ -560,29 +572,31 @@ typedef struct
  * capacity >= *total (the values are a pure function of the addresses) gives all of them. */
 
 /* Replaces isc::LabelSchedule's blinks (core/src/dyes.cpp:440-471, dyes.h:235) for nDyes dyes of one label (mode:
- * 0 dSTORM, 1 PALM, 2 DNA-PAINT, 3 WideField = isc::LabelMode; one set of kinetics, no history): each dye's blinks
+ * 0 dSTORM, 1 PALM, 2 DNA-PAINT, 3 WideField = isc::LabelMode; kin, or hist as above): each dye's blinks
  * in time order, stopping at the first starting at or after tMax, bit-identical to the reference's. */
 int32_t iscg_label_blinks(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, int32_t mode,
-                          const iscg_kinetics* kin, double tMax, int64_t* offsets, iscg_blink* out, int64_t capacity,
-                          int64_t* total);
+                          const iscg_kinetics* kin, const iscg_kinetics_history* hist, double tMax, int64_t* offsets,
+                          iscg_blink* out, int64_t capacity, int64_t* total);
 
-/* Replaces isc::LabelSchedule's continuous windows (core/src/dyes.cpp:447-477): at most one per dye for one set of
- * kinetics; has[i] = 1 and out[i] set when dye i has one, else has[i] = 0 (out[i] untouched). */
+/* Replaces isc::LabelSchedule's continuous windows (core/src/dyes.cpp:447-477): at most one per dye; has[i] = 1 and
+ * out[i] set when dye i has one, else has[i] = 0 (out[i] untouched). */
 int32_t iscg_label_cont(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, int32_t mode, int32_t preState,
-                        const iscg_kinetics* kin, iscg_cont_window* out, uint8_t* has);
+                        const iscg_kinetics* kin, const iscg_kinetics_history* hist, iscg_cont_window* out,
+                        uint8_t* has);
 
 /* Replaces isc::PersistentBlinks (core/src/dyes.cpp:636-646, dyes.h:250): each persistent (DNA-PAINT) dye's blinks
- * overlapping the window (t0, t1), by time bin then index, one set of kinetics. Both persistent calls take at most
- * 2^20 time bins per dye and an activation rate of at most 1e5 per bin (else -1). */
+ * overlapping the window (t0, t1), by time bin then index (under a history, a bin's later segments' blinks after its
+ * own, numbered j | i << 24). Both persistent calls take at most 2^20 time bins per dye and an activation rate of at
+ * most 1e5 per bin in every segment (else -1). */
 int32_t iscg_persistent_blinks(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, const iscg_kinetics* kin,
-                               double t0, double t1, int64_t* offsets, iscg_blink* out, int64_t capacity,
-                               int64_t* total);
+                               const iscg_kinetics_history* hist, double t0, double t1, int64_t* offsets,
+                               iscg_blink* out, int64_t capacity, int64_t* total);
 
 /* Replaces isc::PersistentBlinksInBins (core/src/dyes.cpp:648-655, dyes.h:256): every blink of bins [binLo, binHi]
  * with its bin and index, unfiltered. */
 int32_t iscg_persistent_bins(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, const iscg_kinetics* kin,
-                             int64_t binLo, int64_t binHi, int64_t* offsets, iscg_bin_blink* out, int64_t capacity,
-                             int64_t* total);
+                             const iscg_kinetics_history* hist, int64_t binLo, int64_t binHi, int64_t* offsets,
+                             iscg_bin_blink* out, int64_t capacity, int64_t* total);
 
 /* The device's jsm::sin (fn 0), cos (1), exp (2), log (3), cbrt (4) (core/src/jsmath.h:15-31; fdlibm as V8 runs
  * it) of n values, and V8's Math.hypot of n pairs (fn 5: x holds 2n values) or triples (fn 6: 3n values), the
@@ -593,7 +607,7 @@ int32_t iscg_jsm(iscg_context* h, int32_t fn, const double* x, int64_t n, double
  * Replaces the block walk of isc::World (core/src/world.cpp:455-640) and the dye generation and queries it drives
  * (GenerateDyes / GenerateNpcDyes / GenerateLaminDyes :808-895 over DyesInBlock / NpcDyesInBlock / LaminDyesInBlock,
  * core/src/dyes.cpp:127-253; SitesInWindow :1212, EventsInWindow :1219, ContinuousInWindow :1279, Density3dInWindow
- * :1519), for one set of kinetics per structure (no kinetics history, no clock map). The cells' geometry is the
+ * :1519), for one set of kinetics or one kinetics history per structure (no clock map). The cells' geometry is the
  * caller's (the reference's CellAssets: packed cells, microtubule centrelines, pores, lamina points); the device keeps
  * it, builds the microtubule frames (BuildMtFrames, core/src/microtubules.cpp:701) and the dye blocks' reach data
  * once, and each query selects the blocks that reach the window, generates their dyes on the reference's 0.1 nm
@@ -695,6 +709,11 @@ int32_t iscg_world_destroy(iscg_world* w);
 /* World::SetLabel (core/src/world.cpp:1080) for structure 0 (microtubules), 1 (pores), 2 (lamina) or 3 (docking
  * strands: no geometry here); -1 for an invalid label (as ValidateLabel, kinetics as the C ABI checks them). */
 int32_t iscg_world_set_label(iscg_world* w, int32_t structure, const iscg_label* label);
+/* World::SetKineticsHistory (core/src/world.cpp:933-977): n segments starting at tStart[0..n) (0, then finite and
+ * increasing), structure s's kinetics in segment i at kin[i * 4 + s]; n = 0 clears it. The events and continuous
+ * windows of a structure then follow its history (HistoryOf) as LabelSchedule and PersistentBlinksInBins do under
+ * it; a label's own kinetics apply again once the history is cleared. -1 (nothing changes) on bad input. */
+int32_t iscg_world_set_history(iscg_world* w, const double* tStart, const iscg_kinetics* kin, int32_t n);
 
 /* The queries take the window as {x0, y0, x1, y1, zMin, zMax} (world um) and write their first min(*total, capacity)
  * rows (host or device out; *total receives the count). */

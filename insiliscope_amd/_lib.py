@@ -99,6 +99,11 @@ class Kinetics(C.Structure):
                                           "initialOnSec")]
 
 
+class KineticsHistory(C.Structure):
+    _fields_ = [("n", C.c_int32), ("reserved", C.c_int32), ("tStart", C.POINTER(C.c_double)),
+                ("kin", C.POINTER(Kinetics))]
+
+
 class WorldCell(C.Structure):
     """iscg_world_cell: the fields of isc::Cell the block walk reads (core/src/cells.h:48-75)."""
     _fields_ = [("cx", C.c_int32), ("cy", C.c_int32), ("x", C.c_double), ("y", C.c_double),
@@ -221,14 +226,18 @@ SIGNATURES = {
     "iscg_roi_backward": (I32, [P, C.POINTER(RoiModel), C.POINTER(D), P, I32, P, P, C.POINTER(D), C.POINTER(D)]),
     "iscg_wf_conv3d_plan": (I32, [P, C.POINTER(WfDesc), P, I32, I32, I32, D, I32]),
     "iscg_wf_conv3d": (I32, [P, P, C.POINTER(Camera), P, D, U32, U32, U32, P, P]),
-    "iscg_label_blinks": (I32, [P, P, I32, I32, C.POINTER(Kinetics), D, P, P, C.c_int64, P]),
-    "iscg_label_cont": (I32, [P, P, I32, I32, I32, C.POINTER(Kinetics), P, P]),
-    "iscg_persistent_blinks": (I32, [P, P, I32, C.POINTER(Kinetics), D, D, P, P, C.c_int64, P]),
-    "iscg_persistent_bins": (I32, [P, P, I32, C.POINTER(Kinetics), C.c_int64, C.c_int64, P, P, C.c_int64, P]),
+    "iscg_label_blinks": (I32, [P, P, I32, I32, C.POINTER(Kinetics), C.POINTER(KineticsHistory), D, P, P, C.c_int64,
+                                P]),
+    "iscg_label_cont": (I32, [P, P, I32, I32, I32, C.POINTER(Kinetics), C.POINTER(KineticsHistory), P, P]),
+    "iscg_persistent_blinks": (I32, [P, P, I32, C.POINTER(Kinetics), C.POINTER(KineticsHistory), D, D, P, P,
+                                     C.c_int64, P]),
+    "iscg_persistent_bins": (I32, [P, P, I32, C.POINTER(Kinetics), C.POINTER(KineticsHistory), C.c_int64, C.c_int64,
+                                   P, P, C.c_int64, P]),
     "iscg_jsm": (I32, [P, I32, P, C.c_int64, P]),
     "iscg_world_create": (I32, [P, U32, C.POINTER(WorldGeometry), C.POINTER(P)]),
     "iscg_world_destroy": (I32, [P]),
     "iscg_world_set_label": (I32, [P, I32, C.POINTER(Label)]),
+    "iscg_world_set_history": (I32, [P, P, P, I32]),
     "iscg_world_block_dyes": (I32, [P, P, P, C.c_int64, P]),
     "iscg_world_sites": (I32, [P, P, P, C.c_int64, P]),
     "iscg_world_events": (I32, [P, P, D, D, P, C.c_int64, P]),

@@ -1,6 +1,6 @@
 // dyes.hip -- dye schedules on the device: LabelSchedule's blinks and continuous windows and the persistent
-// (DNA-PAINT) blinks of a batch of dye addresses (core/src/dyes.cpp:254-657, one set of kinetics), bit-identical to
-// the reference's. Each value is a pure function of (h1, k, n): the draws are Unit(Pcg4d(h1, k, n, channel)) (the
+// (DNA-PAINT) blinks of a batch of dye addresses (core/src/dyes.cpp:254-657, one set of kinetics or a
+// KineticsHistory), bit-identical to the reference's. Each value is a pure function of (h1, k, n): the draws are Unit(Pcg4d(h1, k, n, channel)) (the
 // channels of core/src/dyes.h:45-57) and the FP64 arithmetic is the reference's, in its order, with the jsm::
 // functions restated in jsm.h.
 //
@@ -13,7 +13,9 @@
 // The work is FP64 VALU (a log per draw, a log, cos and exp per brightness) and the output is written once.
 #include "dye_gen.h"
 
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 using namespace iscg;
 using namespace iscg::dyegen;
@@ -95,41 +97,20 @@ __global__ __launch_bounds__(kDyeThreads) void dye_fill_kernel(Gen g, const iscg
    (void)wave_schedules<true>(g, dyes, begin, end, counts, base, wgBegin, out, capacity);
 }
 
-// LabelSchedule's continuous windows (core/src/dyes.cpp:447-477), one set of kinetics: at most one per dye.
+// LabelSchedule's continuous windows (core/src/dyes.cpp:447-477): at most one per dye.
 __global__ __launch_bounds__(kDyeThreads) void label_cont_kernel(const iscg_dye_address* __restrict__ dyes,
                                                                   int32_t nDyes, int32_t mode, int32_t preState,
-                                                                  iscg_kinetics kin, iscg_cont_window* __restrict__ out,
+                                                                  iscg_kinetics kin, HistDev hist,
+                                                                  iscg_cont_window* __restrict__ out,
                                                                   uint8_t* __restrict__ has)
 {
    const int i = blockIdx.x * kDyeThreads + threadIdx.x;
    if (i >= nDyes)
       return;
    const iscg_dye_address d = dyes[i];
-   const uint32_t h1 = d.h1, k = static_cast<uint32_t>(d.k), n = static_cast<uint32_t>(d.n);
-   uint8_t one = 0;
    iscg_cont_window w{};
-   if (mode == 3)
-   {
-      w = iscg_cont_window{0.0, __builtin_inf(), 3, -jsm::log(draw(h1, k, n, CH_AUX))};
-      one = 1;
-   }
-   else if (mode == 0 && kin.initialOnSec > 0)
-   {
-      const double shift = -jsm::log(draw(h1, k, n, CH_INIT_ON)) * kin.initialOnSec;
-      if (shift != 0)
-      {
-         w = iscg_cont_window{0.0, shift, 2, 0.0};
-         one = 1;
-      }
-   }
-   else if (mode == 1 && preState)
-   {
-      const double tOff = kin.activationRatePerSec > 0
-                             ? -jsm::log(draw(h1, k, n, CH_ACT)) / kin.activationRatePerSec
-                             : __builtin_inf();
-      w = iscg_cont_window{0.0, tOff, 1, -jsm::log(draw(h1, k, n, CH_AUX))};
-      one = 1;
-   }
+   const bool one = cont_window(mode, preState, kin, hist,
+                                Address::of(d.h1, static_cast<uint32_t>(d.k), static_cast<uint32_t>(d.n)), w);
    if (one)
       out[i] = w;
    has[i] = one;
@@ -210,12 +191,65 @@ int32_t run_schedule(iscg_context* h, const char* what, const Gen& g, const iscg
 
 bool bad_kinetics(const iscg_kinetics* kin) { return kin == nullptr; }
 
+constexpr int32_t kMaxSegments = 1 << 16;
+
+// A history argument as LabelSchedule / PersistentGen read theirs: none (nullptr or n = 0) keeps kin; one segment
+// replaces kin by its kinetics; two or more are packed into stage[4] (dev), the stream synchronized so the packing
+// may go. False (and the error set) for a malformed history.
+bool take_history(iscg_context* h, const char* what, const iscg_kinetics_history* hist, iscg_kinetics& kin,
+                  HistDev& dev)
+{
+   dev = HistDev{0, nullptr};
+   if (!hist || hist->n == 0)
+      return true;
+   if (hist->n < 0 || hist->n > kMaxSegments || !hist->tStart || !hist->kin)
+   {
+      fail(h, std::string(what) + ": bad kinetics history");
+      return false;
+   }
+   for (int32_t i = 0; i < hist->n; ++i)
+      if (!std::isfinite(hist->tStart[i]) || (i == 0 ? hist->tStart[0] != 0 : !(hist->tStart[i] > hist->tStart[i - 1])))
+      {
+         fail(h, std::string(what) + ": a history starts at 0 and its segment starts increase");
+         return false;
+      }
+   if (hist->n == 1)
+   {
+      kin = hist->kin[0];
+      return true;
+   }
+   std::vector<HistSeg> segs((size_t)hist->n);
+   for (int32_t i = 0; i < hist->n; ++i)
+      segs[(size_t)i] = HistSeg{hist->tStart[i], hist->kin[i]};
+   const void* p = stage_in(h, 4, segs.data(), segs.size() * sizeof(HistSeg));
+   if (!p || !ok(h, hipStreamSynchronize(h->stream), what))
+      return false;
+   dev = HistDev{hist->n, static_cast<const HistSeg*>(p)};
+   return true;
+}
+
+// The largest activation rate and ON time of kin or of the history's segments (PersistentMaxOn, core/src/dyes.cpp:628).
+void persist_extremes(const iscg_kinetics& kin, const iscg_kinetics_history* hist, double& rate, double& onSec)
+{
+   rate = kin.activationRatePerSec;
+   onSec = kin.onSec;
+   if (!hist || hist->n <= 0)
+      return;
+   rate = 0;
+   onSec = 0;
+   for (int32_t i = 0; i < hist->n; ++i)
+   {
+      rate = std::max(rate, hist->kin[i].activationRatePerSec);
+      onSec = std::max(onSec, hist->kin[i].onSec);
+   }
+}
+
 // Every lane's loop over bins and candidate blinks must end within a launch: at most kMaxBins bins per dye and a
 // per-bin mean count of at most kMaxPerBin (the reference's callers ask for a few bins at a few blinks each).
 constexpr long long kMaxBins = 1 << 20;
 constexpr double kMaxPerBin = 1e5;
 
-bool persist_bounded(iscg_context* h, const char* what, const iscg_kinetics* kin, long long b0, long long b1)
+bool persist_bounded(iscg_context* h, const char* what, double rate, long long b0, long long b1)
 {
    const long long lo = b0 > 0 ? b0 : 0;
    if (b1 >= lo && b1 - lo >= kMaxBins)
@@ -223,7 +257,7 @@ bool persist_bounded(iscg_context* h, const char* what, const iscg_kinetics* kin
       fail(h, std::string(what) + ": more than 2^20 time bins per dye");
       return false;
    }
-   if (kin->activationRatePerSec * kBinSec > kMaxPerBin)
+   if (!(rate * kBinSec <= kMaxPerBin) && rate > 0)
    {
       fail(h, std::string(what) + ": activation rate above 1e5 blinks per bin");
       return false;
@@ -236,61 +270,97 @@ bool persist_bounded(iscg_context* h, const char* what, const iscg_kinetics* kin
 extern "C" {
 
 int32_t iscg_label_blinks(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, int32_t mode,
-                          const iscg_kinetics* kin, double tMax, int64_t* offsets, iscg_blink* out, int64_t capacity,
-                          int64_t* total)
+                          const iscg_kinetics* kin, const iscg_kinetics_history* hist, double tMax, int64_t* offsets,
+                          iscg_blink* out, int64_t capacity, int64_t* total)
 {
    if (begin(h))
       return -1;
    if (bad_kinetics(kin) || mode < 0 || mode > 3 || (nDyes > 0 && !dyes))
       return fail(h, "iscg_label_blinks: bad argument");
-   const LabelGen g{mode, *kin, tMax, -__builtin_inf()};
+   iscg_kinetics k = *kin;
+   HistDev hd;
+   if (!take_history(h, "iscg_label_blinks", hist, k, hd))
+      return -1;
+   if (hd.n >= 2)
+   {
+      const LabelHistGen g{mode, hd, tMax, -__builtin_inf(), __builtin_inf()};
+      return run_schedule(h, "iscg_label_blinks", g, dyes, nDyes, offsets, out, capacity, total);
+   }
+   const LabelGen g{mode, k, tMax, -__builtin_inf(), __builtin_inf()};
    return run_schedule(h, "iscg_label_blinks", g, dyes, nDyes, offsets, out, capacity, total);
 }
 
 int32_t iscg_persistent_blinks(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, const iscg_kinetics* kin,
-                               double t0, double t1, int64_t* offsets, iscg_blink* out, int64_t capacity,
-                               int64_t* total)
+                               const iscg_kinetics_history* hist, double t0, double t1, int64_t* offsets,
+                               iscg_blink* out, int64_t capacity, int64_t* total)
 {
    if (begin(h))
       return -1;
    if (bad_kinetics(kin) || (nDyes > 0 && !dyes))
       return fail(h, "iscg_persistent_blinks: bad argument");
+   iscg_kinetics k = *kin;
+   HistDev hd;
+   if (!take_history(h, "iscg_persistent_blinks", hist, k, hd))
+      return -1;
+   double rate, onSec;
+   persist_extremes(*kin, hist, rate, onSec);
    // PersistentBlinks (core/src/dyes.cpp:636-646): no window, no blinks; bins from the lookback of the longest ON.
    long long b0 = 0, b1 = -1;
    if (t1 > t0)
    {
-      const double maxOn = kOnCap * kin->onSec;
+      const double maxOn = kOnCap * onSec;
       const long long f = (long long)std::floor((t0 - maxOn) / kBinSec);
       b0 = f > 0 ? f : 0;
       b1 = (long long)std::floor(t1 / kBinSec);
    }
-   if (!persist_bounded(h, "iscg_persistent_blinks", kin, b0, b1))
+   if (!persist_bounded(h, "iscg_persistent_blinks", rate, b0, b1))
       return -1;
-   const PersistGen<true, iscg_blink> g{*kin, t0, t1, b0, b1};
+   if (hd.n >= 2)
+   {
+      const PersistHistGen<true, iscg_blink> g{hd, t0, t1, b0, b1};
+      return run_schedule(h, "iscg_persistent_blinks", g, dyes, nDyes, offsets, out, capacity, total);
+   }
+   const PersistGen<true, iscg_blink> g{k, t0, t1, b0, b1};
    return run_schedule(h, "iscg_persistent_blinks", g, dyes, nDyes, offsets, out, capacity, total);
 }
 
 int32_t iscg_persistent_bins(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, const iscg_kinetics* kin,
-                             int64_t binLo, int64_t binHi, int64_t* offsets, iscg_bin_blink* out, int64_t capacity,
-                             int64_t* total)
+                             const iscg_kinetics_history* hist, int64_t binLo, int64_t binHi, int64_t* offsets,
+                             iscg_bin_blink* out, int64_t capacity, int64_t* total)
 {
    if (begin(h))
       return -1;
    if (bad_kinetics(kin) || (nDyes > 0 && !dyes))
       return fail(h, "iscg_persistent_bins: bad argument");
-   if (!persist_bounded(h, "iscg_persistent_bins", kin, binLo, binHi))
+   iscg_kinetics k = *kin;
+   HistDev hd;
+   if (!take_history(h, "iscg_persistent_bins", hist, k, hd))
       return -1;
-   const PersistGen<false, iscg_bin_blink> g{*kin, 0.0, 0.0, (long long)binLo, (long long)binHi};
+   double rate, onSec;
+   persist_extremes(*kin, hist, rate, onSec);
+   if (!persist_bounded(h, "iscg_persistent_bins", rate, binLo, binHi))
+      return -1;
+   if (hd.n >= 2)
+   {
+      const PersistHistGen<false, iscg_bin_blink> g{hd, 0.0, 0.0, (long long)binLo, (long long)binHi};
+      return run_schedule(h, "iscg_persistent_bins", g, dyes, nDyes, offsets, out, capacity, total);
+   }
+   const PersistGen<false, iscg_bin_blink> g{k, 0.0, 0.0, (long long)binLo, (long long)binHi};
    return run_schedule(h, "iscg_persistent_bins", g, dyes, nDyes, offsets, out, capacity, total);
 }
 
 int32_t iscg_label_cont(iscg_context* h, const iscg_dye_address* dyes, int32_t nDyes, int32_t mode, int32_t preState,
-                        const iscg_kinetics* kin, iscg_cont_window* out, uint8_t* has)
+                        const iscg_kinetics* kin, const iscg_kinetics_history* hist, iscg_cont_window* out,
+                        uint8_t* has)
 {
    if (begin(h))
       return -1;
    if (bad_kinetics(kin) || mode < 0 || mode > 3 || nDyes < 0 || (nDyes > 0 && (!dyes || !out || !has)))
       return fail(h, "iscg_label_cont: bad argument");
+   iscg_kinetics k = *kin;
+   HistDev hd;
+   if (!take_history(h, "iscg_label_cont", hist, k, hd))
+      return -1;
    if (nDyes == 0)
       return 0;
    const iscg_dye_address* dd = (const iscg_dye_address*)stage_in(h, 0, dyes, (size_t)nDyes * sizeof(*dyes));
@@ -302,7 +372,7 @@ int32_t iscg_label_cont(iscg_context* h, const iscg_dye_address* dyes, int32_t n
    {
       ProfScope ps(h, "dye_cont");
       hipLaunchKernelGGL(label_cont_kernel, dim3((nDyes + kDyeThreads - 1) / kDyeThreads), dim3(kDyeThreads), 0,
-                         h->stream, dd, nDyes, mode, preState, *kin, dout, dhas);
+                         h->stream, dd, nDyes, mode, preState, k, hd, dout, dhas);
    }
    if (!ok(h, hipGetLastError(), "iscg_label_cont launch"))
       return -1;

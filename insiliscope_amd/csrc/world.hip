@@ -121,6 +121,7 @@ struct QueryArgs
    double mtReach, blockReach, npcRb, laminRb;
    double npcSiteReach;
    LabelDev lab[3];
+   HistDev hist[3];  // each structure's kinetics history of two or more segments (n = 0: the label's kinetics)
 };
 
 // ---- device helpers ----------------------------------------------------------------------------------------------
@@ -1080,8 +1081,8 @@ __global__ __launch_bounds__(kThreads) void gather_events_kernel(const int32_t* 
    out[e] = r;
 }
 
-// LabelSchedule's continuous window of each in-window dye (core/src/dyes.cpp:440-478) ending after tMin: 1 if it has
-// one (the window in win[i]).
+// LabelSchedule's continuous window of each in-window dye (core/src/dyes.cpp:440-478, under the structure's history)
+// ending after tMin: 1 if it has one (the window in win[i]).
 __global__ __launch_bounds__(kThreads) void cont_kernel(const iscg_dye_address* __restrict__ addr,
                                                         const uint8_t* __restrict__ structure, int32_t n, QueryArgs q,
                                                         double tMin, iscg_cont_window* __restrict__ win,
@@ -1091,35 +1092,14 @@ __global__ __launch_bounds__(kThreads) void cont_kernel(const iscg_dye_address* 
    if (i >= n)
       return;
    const int st = structure[i];
-   int32_t one = 0;
+   bool one = false;
    iscg_cont_window w{};
    if (st < 3)
    {
       const LabelDev& L = q.lab[st];
       const iscg_dye_address d = addr[i];
-      const Address at = Address::of(d.h1, (uint32_t)d.k, (uint32_t)d.n);
-      if (L.mode == 3)
-      {
-         w = iscg_cont_window{0.0, __builtin_inf(), 3, -jsm::log(at.draw(CH_AUX))};
-         one = 1;
-      }
-      else if (L.mode == 0 && L.kin.initialOnSec > 0)
-      {
-         const double shift = -jsm::log(at.draw(CH_INIT_ON)) * L.kin.initialOnSec;
-         if (shift != 0)
-         {
-            w = iscg_cont_window{0.0, shift, 2, 0.0};
-            one = 1;
-         }
-      }
-      else if (L.mode == 1 && L.preState)
-      {
-         const double tOff = L.kin.activationRatePerSec > 0
-                                ? -jsm::log(at.draw(CH_ACT)) / L.kin.activationRatePerSec
-                                : __builtin_inf();
-         w = iscg_cont_window{0.0, tOff, 1, -jsm::log(at.draw(CH_AUX))};
-         one = 1;
-      }
+      one = cont_window(L.mode, L.preState, L.kin, q.hist[st],
+                        Address::of(d.h1, (uint32_t)d.k, (uint32_t)d.n), w);
    }
    one = one && w.tOff > tMin;
    win[i] = w;
@@ -1154,6 +1134,7 @@ __global__ __launch_bounds__(kThreads) void cont_rows_kernel(const iscg_cont_win
 }
 
 using PersistWin = PersistGen<true, iscg_blink>;
+using PersistHistWin = PersistHistGen<true, iscg_blink>;
 
 unsigned blocks_for(int64_t n, int per = kThreads) { return (unsigned)((n + per - 1) / per); }
 
@@ -1171,6 +1152,11 @@ struct iscg_world
    double npcSiteReach = 0;
    iscg_label labels[kStructures] = {};
    double protoOff[kProto] = {};
+   // World::SetKineticsHistory: the segment starts and each structure's kinetics per segment (host copies), the
+   // segments of two or more on the device
+   std::vector<double> histT;
+   std::vector<iscg_kinetics> histKin[kStructures];
+   DevBuf histSeg[kStructures];
    // geometry
    DevBuf cells, mts, pts, cum, T, U, V, cands, npcB, lamB, npcIds, lamIds, npcs, lamin, npcSites, npcStart,
       laminStart, npcReach, laminReach;
@@ -1263,7 +1249,14 @@ QueryArgs query_args(const iscg_world* w, const double* win)
    q.win = Window{win[0], win[1], win[2], win[3], win[4], win[5]};
    q.seed = w->seed;
    for (int s = 0; s < 3; ++s)
+   {
       q.lab[s] = label_dev(s, w->labels[s]);
+      // HistoryOf(s): one segment is its kinetics from t = 0, two or more the walk over them
+      if (w->histT.size() == 1)
+         q.lab[s].kin = w->histKin[s][0];
+      q.hist[s] = w->histT.size() >= 2 ? HistDev{(int32_t)w->histT.size(), w->histSeg[s].as<HistSeg>()}
+                                       : HistDev{0, nullptr};
+   }
    double lk[3][4];
    for (int s = 0; s < 3; ++s)
       norm_linkage(s, w->labels[s], lk[s]);
@@ -1431,14 +1424,30 @@ bool valid_label(const iscg_label& l)
    return true;
 }
 
-// The DNA-PAINT bin range of a query (World::PersistFor, core/src/world.cpp:728-740; no history).
-bool persist_range(const iscg_kinetics& kin, double t0, double t1, long long& b0, long long& b1)
+// The DNA-PAINT bin range of a query and the largest rate it draws at (World::PersistFor, core/src/world.cpp:728-740,
+// with PersistentMaxOn, core/src/dyes.cpp:628-634): the label's kinetics, or those of every segment of a history.
+bool persist_range(const iscg_kinetics& kin, const std::vector<iscg_kinetics>* hist, double t0, double t1,
+                   long long& b0, long long& b1, double& maxRate)
 {
-   const double maxOn = 20.0 * kin.onSec;  // PersistentMaxOn
+   double maxOn = 20.0 * kin.onSec;
+   bool any = kin.activationRatePerSec > 0;
+   maxRate = kin.activationRatePerSec;
+   if (hist && !hist->empty())
+   {
+      maxOn = 0;
+      any = false;
+      maxRate = 0;
+      for (const iscg_kinetics& q : *hist)
+      {
+         maxOn = std::max(maxOn, 20.0 * q.onSec);
+         any = any || q.activationRatePerSec > 0;
+         maxRate = std::max(maxRate, q.activationRatePerSec);
+      }
+   }
    const long long f = (long long)std::floor((t0 - maxOn) / 1.0);
    b0 = f > 0 ? f : 0;
    b1 = (long long)std::floor(t1 / 1.0);
-   return kin.activationRatePerSec > 0 && t1 > t0;
+   return any && t1 > t0;
 }
 
 }  // namespace
@@ -1649,6 +1658,45 @@ int32_t iscg_world_set_label(iscg_world* w, int32_t structure, const iscg_label*
    return 0;
 }
 
+int32_t iscg_world_set_history(iscg_world* w, const double* tStart, const iscg_kinetics* kin, int32_t n)
+{
+   if (!w)
+      return -1;
+   if (begin(w->h))
+      return -1;
+   if (n < 0 || n > (1 << 16) || (n > 0 && (!tStart || !kin)))
+      return wfail(w, "iscg_world_set_history: bad argument");
+   // World::SetKineticsHistory's checks (core/src/world.cpp:933-945)
+   for (int32_t i = 0; i < n; ++i)
+      if (!std::isfinite(tStart[i]) || (i == 0 ? tStart[i] != 0 : !(tStart[i] > tStart[i - 1])))
+         return wfail(w, "iscg_world_set_history: segment starts must be 0, then finite and increasing");
+   for (int64_t i = 0; i < (int64_t)n * kStructures; ++i)
+   {
+      iscg_label l = w->labels[0];
+      l.kin = kin[i];
+      if (!valid_label(l))
+         return wfail(w, "iscg_world_set_history: bad kinetics");
+   }
+   std::vector<HistSeg> segs((size_t)n);
+   for (int s = 0; s < kStructures; ++s)
+   {
+      w->histKin[s].resize((size_t)n);
+      for (int32_t i = 0; i < n; ++i)
+      {
+         w->histKin[s][(size_t)i] = kin[(size_t)i * kStructures + (size_t)s];
+         segs[(size_t)i] = HistSeg{tStart[i], w->histKin[s][(size_t)i]};
+      }
+      if (n >= 2 && (!upload(w, w->histSeg[s], segs.data(), segs.size(), "iscg_world_set_history") ||
+                     !wok(w, hipStreamSynchronize(w->h->stream), "iscg_world_set_history")))
+      {
+         w->histT.clear();
+         return -1;
+      }
+   }
+   w->histT.assign(tStart, tStart + n);
+   return 0;
+}
+
 int32_t iscg_world_block_dyes(iscg_world* w, const double window[6], iscg_block_dye* out, int64_t capacity,
                               int64_t* total)
 {
@@ -1805,11 +1853,15 @@ int32_t iscg_world_events(iscg_world* w, const double window[6], double t0, doub
    for (int s = 0; s < 3; ++s)
       if (q.lab[s].mode == 2)
       {
-         persist[s] = persist_range(q.lab[s].kin, t0, t1, pb0[s], pb1[s]);
+         double maxRate = 0;
+         persist[s] = persist_range(w->labels[s].kin, w->histT.empty() ? nullptr : &w->histKin[s], t0, t1, pb0[s],
+                                    pb1[s], maxRate);
          const long long lo = pb0[s] > 0 ? pb0[s] : 0;
-         if (persist[s] && (pb1[s] - lo >= (1ll << 20) || q.lab[s].kin.activationRatePerSec > 1e5))
+         if (persist[s] && (pb1[s] - lo >= (1ll << 20) || maxRate > 1e5))
             return wfail(w, "iscg_world_events: more than 2^20 time bins or 1e5 blinks per bin for DNA-PAINT");
       }
+   // World::Schedule's horizon (twice the query's end); the blinks starting at or after t1 are not wanted
+   const double horizon = 2 * t1;
    int64_t nSel = 0, nDyes = 0, nWin = 0, nEv = 0;
    if (!make_dyes(w, q, nSel, nDyes) || !window_dyes(w, q, nDyes, 0xFu, nWin))
       return -1;
@@ -1829,11 +1881,24 @@ int32_t iscg_world_events(iscg_world* w, const double window[6], double t0, doub
    {
       const OfStructure sel{w->structure.as<uint8_t>(), (uint8_t)s};
       ProfScope ps(w->h, "world_schedule_count");
-      if (q.lab[s].mode == 0 || q.lab[s].mode == 1)
+      const bool multi = q.hist[s].n >= 2;
+      if ((q.lab[s].mode == 0 || q.lab[s].mode == 1) && multi)
       {
-         const LabelGen g{q.lab[s].mode, q.lab[s].kin, t1, t0};
+         const LabelHistGen g{q.lab[s].mode, q.hist[s], horizon, t0, t1};
+         hipLaunchKernelGGL(world_count_kernel<LabelHistGen>, dim3(grid), dim3(kDyeThreads), 0, st, g,
+                            w->addr.as<iscg_dye_address>(), nw, w->evCounts.as<int32_t>(), sel);
+      }
+      else if (q.lab[s].mode == 0 || q.lab[s].mode == 1)
+      {
+         const LabelGen g{q.lab[s].mode, q.lab[s].kin, horizon, t0, t1};
          hipLaunchKernelGGL(world_count_kernel<LabelGen>, dim3(grid), dim3(kDyeThreads), 0, st, g,
                             w->addr.as<iscg_dye_address>(), nw, w->evCounts.as<int32_t>(), sel);
+      }
+      else if (q.lab[s].mode == 2 && persist[s] && multi)
+      {
+         const PersistHistWin g{q.hist[s], t0, t1, pb0[s], pb1[s]};
+         hipLaunchKernelGGL(world_count_kernel<PersistHistWin>, dim3(grid), dim3(kDyeThreads), 0, st,
+                            g, w->addr.as<iscg_dye_address>(), nw, w->evCounts.as<int32_t>(), sel);
       }
       else if (q.lab[s].mode == 2 && persist[s])
       {
@@ -1855,10 +1920,25 @@ int32_t iscg_world_events(iscg_world* w, const double window[6], double t0, doub
    {
       const OfStructure sel{w->structure.as<uint8_t>(), (uint8_t)s};
       ProfScope ps(w->h, "world_schedule_fill");
-      if (q.lab[s].mode == 0 || q.lab[s].mode == 1)
+      const bool multi = q.hist[s].n >= 2;
+      if ((q.lab[s].mode == 0 || q.lab[s].mode == 1) && multi)
       {
-         const LabelGen g{q.lab[s].mode, q.lab[s].kin, t1, t0};
+         const LabelHistGen g{q.lab[s].mode, q.hist[s], horizon, t0, t1};
+         hipLaunchKernelGGL(world_fill_kernel<LabelHistGen>, dim3(grid), dim3(kDyeThreads), 0, st, g,
+                            w->addr.as<iscg_dye_address>(), nw, w->evCounts.as<int32_t>(), w->evOff.as<int64_t>(),
+                            w->blinks.as<iscg_blink>(), nEv, sel);
+      }
+      else if (q.lab[s].mode == 0 || q.lab[s].mode == 1)
+      {
+         const LabelGen g{q.lab[s].mode, q.lab[s].kin, horizon, t0, t1};
          hipLaunchKernelGGL(world_fill_kernel<LabelGen>, dim3(grid), dim3(kDyeThreads), 0, st, g,
+                            w->addr.as<iscg_dye_address>(), nw, w->evCounts.as<int32_t>(), w->evOff.as<int64_t>(),
+                            w->blinks.as<iscg_blink>(), nEv, sel);
+      }
+      else if (q.lab[s].mode == 2 && persist[s] && multi)
+      {
+         const PersistHistWin g{q.hist[s], t0, t1, pb0[s], pb1[s]};
+         hipLaunchKernelGGL(world_fill_kernel<PersistHistWin>, dim3(grid), dim3(kDyeThreads), 0, st, g,
                             w->addr.as<iscg_dye_address>(), nw, w->evCounts.as<int32_t>(), w->evOff.as<int64_t>(),
                             w->blinks.as<iscg_blink>(), nEv, sel);
       }

@@ -1471,6 +1471,27 @@ def Kinetics(activationRatePerSec: float = 0.01, onSec: float = 0.05, offSec: fl
     return _lib.Kinetics(activationRatePerSec, onSec, offSec, bleachProb, photonCV, initialOnSec)
 
 
+class KineticsHistory:
+    """isc::KineticsHistory (core/src/dyes.h:138-149): segment i of constant kinetics kin[i] from tStart[i] (tStart[0]
+    = 0, then increasing) to tStart[i+1], the last for ever. The schedule calls take it where the reference's take
+    theirs (LabelSchedule, PersistentBlinks, PersistentBlinksInBins: hist=...)."""
+
+    def __init__(self, tStart, kin):
+        self.tStart = np.ascontiguousarray(tStart, np.float64)
+        kin = list(kin)
+        if len(kin) != self.tStart.size:
+            raise IscgError("KineticsHistory: one set of kinetics per segment")
+        self.kin = (_lib.Kinetics * max(1, len(kin)))(*kin)
+        self._c = _lib.KineticsHistory(len(kin), 0, self.tStart.ctypes.data_as(C.POINTER(C.c_double)), self.kin)
+
+    def ref(self):
+        return C.byref(self._c)
+
+
+def _hist(hist):
+    return None if hist is None else hist.ref()
+
+
 @dataclass
 class Label:
     """The fields of isc::Label (core/src/dyes.h:166-179) the schedules and the world read: mode, kinetics, PALM pre
@@ -1550,21 +1571,22 @@ def _records(ctx, fn, args, n, dtype, capacity):
     raise IscgError("dye schedule: record count changed between calls")
 
 
-def LabelSchedule(dyes, label: Label, tMax: float = float("inf"), ctx: Context | None = None, capacity=None):
+def LabelSchedule(dyes, label: Label, tMax: float = float("inf"), ctx: Context | None = None, capacity=None,
+                  hist: KineticsHistory | None = None):
     """isc::LabelSchedule's blinks (core/src/dyes.cpp:440-471) for a batch of dyes of one label on the GPU: (offsets,
     blinks) with dye i's blinks at blinks[offsets[i]:offsets[i+1]] (BLINK_DTYPE), in time order, bit-identical to
-    the reference's. One set of kinetics (no KineticsHistory)."""
+    the reference's; under hist, the label's kinetics per segment of the dye clock (DyeScheduleHistory)."""
     ctx = ctx or default_context()
     d, n = _dyes(dyes)
     p, keep = _ptr(d)
     ctx.follow_torch()
-    return _records(ctx, ctx.L.iscg_label_blinks, (p, n, LABEL_MODES[label.mode], C.byref(label.kin), float(tMax)), n,
-                    BLINK_DTYPE, capacity)
+    return _records(ctx, ctx.L.iscg_label_blinks, (p, n, LABEL_MODES[label.mode], C.byref(label.kin), _hist(hist),
+                                                   float(tMax)), n, BLINK_DTYPE, capacity)
 
 
-def LabelContWindows(dyes, label: Label, ctx: Context | None = None):
+def LabelContWindows(dyes, label: Label, ctx: Context | None = None, hist: KineticsHistory | None = None):
     """isc::LabelSchedule's continuous windows (core/src/dyes.cpp:447-477): (has, windows) per dye, CONT_WINDOW_DTYPE
-    (at most one per dye for one set of kinetics)."""
+    (at most one per dye)."""
     ctx = ctx or default_context()
     d, n = _dyes(dyes)
     p, keep = _ptr(d)
@@ -1572,30 +1594,31 @@ def LabelContWindows(dyes, label: Label, ctx: Context | None = None):
     out = np.zeros(n, CONT_WINDOW_DTYPE)
     ctx.follow_torch()
     ctx.check(ctx.L.iscg_label_cont(ctx.h, p, n, LABEL_MODES[label.mode], int(label.preState), C.byref(label.kin),
-                                    out.ctypes.data, has.ctypes.data), "iscg_label_cont")
+                                    _hist(hist), out.ctypes.data, has.ctypes.data), "iscg_label_cont")
     return has.astype(bool), out
 
 
-def PersistentBlinks(dyes, kin: _lib.Kinetics, t0: float, t1: float, ctx: Context | None = None, capacity=None):
+def PersistentBlinks(dyes, kin: _lib.Kinetics, t0: float, t1: float, ctx: Context | None = None, capacity=None,
+                     hist: KineticsHistory | None = None):
     """isc::PersistentBlinks (core/src/dyes.cpp:636-646) of a batch of persistent (DNA-PAINT) dyes: the blinks
     overlapping (t0, t1), as (offsets, blinks)."""
     ctx = ctx or default_context()
     d, n = _dyes(dyes)
     p, keep = _ptr(d)
     ctx.follow_torch()
-    return _records(ctx, ctx.L.iscg_persistent_blinks, (p, n, C.byref(kin), float(t0), float(t1)), n, BLINK_DTYPE,
-                    capacity)
+    return _records(ctx, ctx.L.iscg_persistent_blinks, (p, n, C.byref(kin), _hist(hist), float(t0), float(t1)), n,
+                    BLINK_DTYPE, capacity)
 
 
 def PersistentBlinksInBins(dyes, kin: _lib.Kinetics, binLo: int, binHi: int, ctx: Context | None = None,
-                           capacity=None):
+                           capacity=None, hist: KineticsHistory | None = None):
     """isc::PersistentBlinksInBins (core/src/dyes.cpp:648-655): every blink of bins [binLo, binHi] with its bin and
     index (BIN_BLINK_DTYPE), as (offsets, blinks)."""
     ctx = ctx or default_context()
     d, n = _dyes(dyes)
     p, keep = _ptr(d)
     ctx.follow_torch()
-    return _records(ctx, ctx.L.iscg_persistent_bins, (p, n, C.byref(kin), int(binLo), int(binHi)), n,
+    return _records(ctx, ctx.L.iscg_persistent_bins, (p, n, C.byref(kin), _hist(hist), int(binLo), int(binHi)), n,
                     BIN_BLINK_DTYPE, capacity)
 
 
@@ -1686,7 +1709,7 @@ class WorldGeometry:
 class World:
     """isc::World's dye blocks and window queries (core/src/world.h:152-300) over a cell field's geometry kept on the
     GPU: SitesInWindow, EventsInWindow, ContinuousInWindow, Density3dInWindow and DensityInWindow answer as the
-    reference's do, bit for bit and in its order, for one set of kinetics per structure (no kinetics history)."""
+    reference's do, bit for bit and in its order, for one set of kinetics or one kinetics history per structure."""
 
     def __init__(self, seed: int, geometry: WorldGeometry, ctx: Context | None = None):
         self.ctx = ctx or default_context()
@@ -1737,6 +1760,16 @@ class World:
         """World::SetLabel (core/src/world.cpp:1080)."""
         lab = label.to_c()
         self.ctx.check(self.ctx.L.iscg_world_set_label(self.w, int(structure), C.byref(lab)), "iscg_world_set_label")
+
+    def SetKineticsHistory(self, tStart, kin) -> None:
+        """World::SetKineticsHistory (core/src/world.cpp:933): segment starts tStart (0, then increasing) and
+        kin[i * 4 + s], structure s's kinetics in segment i; empty clears it."""
+        t = np.ascontiguousarray(tStart, np.float64)
+        k = (_lib.Kinetics * max(1, len(kin)))(*kin)
+        if len(kin) != 4 * t.size:
+            raise IscgError("SetKineticsHistory: 4 kinetics per segment")
+        self.ctx.check(self.ctx.L.iscg_world_set_history(self.w, t.ctypes.data, C.addressof(k), int(t.size)),
+                       "iscg_world_set_history")
 
     def _rows(self, fn, key, dtype, args):
         win = np.ascontiguousarray(args[0], np.float64)

@@ -539,10 +539,10 @@ class Reference:
         L.ref_dye_h1.argtypes = [I32, U32, I32, I32, I32]
         L.ref_dye_h1.restype = U32
         PK = C.POINTER(Kinetics)
-        L.ref_label_blinks.argtypes = [C.c_void_p, I32, I32, PK, D, PI32, PD, I64]
+        L.ref_label_blinks.argtypes = [C.c_void_p, I32, I32, PK, D, PI32, PD, I64, I32, PD, PK]
         L.ref_label_blinks.restype = I64
-        L.ref_label_cont.argtypes = [C.c_void_p, I32, I32, I32, PK, PI32, PD]
-        L.ref_persistent_blinks.argtypes = [C.c_void_p, I32, PK, I32, D, D, I64, I64, PI32, PD, I64]
+        L.ref_label_cont.argtypes = [C.c_void_p, I32, I32, I32, PK, PI32, PD, I32, PD, PK]
+        L.ref_persistent_blinks.argtypes = [C.c_void_p, I32, PK, I32, D, D, I64, I64, PI32, PD, I64, I32, PD, PK]
         L.ref_persistent_blinks.restype = I64
 
     def psf_planes(self, req, size, nz):
@@ -755,34 +755,50 @@ class Reference:
         self.L.ref_jsm(JSM_FUNCS.index(fn), _p(x, C.c_double), x.size, _p(out, C.c_double))
         return out
 
-    def label_blinks(self, dyes, mode, kin, t_max):
-        """LabelSchedule (core/src/dyes.cpp:440) blinks per dye: (counts, blinks total x 3)."""
+    @staticmethod
+    def _hist(hist):
+        """(n, tStart, kinetics array) of a history (tStart, [Kinetics]) or None."""
+        if hist is None:
+            return 0, None, None
+        t = np.ascontiguousarray(hist[0], np.float64)
+        k = (Kinetics * len(hist[1]))(*hist[1])
+        return len(hist[1]), t, k
+
+    def label_blinks(self, dyes, mode, kin, t_max, hist=None):
+        """LabelSchedule (core/src/dyes.cpp:440) blinks per dye, under hist = (tStart, [Kinetics]) if given:
+        (counts, blinks total x 3)."""
         d = np.ascontiguousarray(dyes, np.uint32).reshape(-1, 3)
         counts = np.zeros(len(d), np.int32)
+        n, t, k = self._hist(hist)
+        h = (n, None if t is None else _p(t, C.c_double), k)
         total = self.L.ref_label_blinks(d.ctypes.data, len(d), mode, C.byref(kin), t_max, _p(counts, C.c_int32),
-                                        None, 0)
+                                        None, 0, *h)
         out = np.zeros((max(total, 1), 3))
         self.L.ref_label_blinks(d.ctypes.data, len(d), mode, C.byref(kin), t_max, _p(counts, C.c_int32),
-                                _p(out, C.c_double), total)
+                                _p(out, C.c_double), total, *h)
         return counts, out[:total]
 
-    def label_cont(self, dyes, mode, pre_state, kin):
+    def label_cont(self, dyes, mode, pre_state, kin, hist=None):
         d = np.ascontiguousarray(dyes, np.uint32).reshape(-1, 3)
         has = np.zeros(len(d), np.int32)
         w = np.zeros((len(d), 4))
+        n, t, k = self._hist(hist)
         self.L.ref_label_cont(d.ctypes.data, len(d), mode, int(pre_state), C.byref(kin), _p(has, C.c_int32),
-                              _p(w, C.c_double))
+                              _p(w, C.c_double), n, None if t is None else _p(t, C.c_double), k)
         return has, w
 
-    def persistent_blinks(self, dyes, kin, t0=0.0, t1=0.0, bins=None):
-        """PersistentBlinks (window) or PersistentBlinksInBins (bins = (lo, hi)): (counts, blinks total x 5)."""
+    def persistent_blinks(self, dyes, kin, t0=0.0, t1=0.0, bins=None, hist=None):
+        """PersistentBlinks (window) or PersistentBlinksInBins (bins = (lo, hi)), under hist if given: (counts,
+        blinks total x 5)."""
         d = np.ascontiguousarray(dyes, np.uint32).reshape(-1, 3)
         counts = np.zeros(len(d), np.int32)
         lo, hi = bins if bins is not None else (0, 0)
         args = (d.ctypes.data, len(d), C.byref(kin), int(bins is not None), t0, t1, lo, hi, _p(counts, C.c_int32))
-        total = self.L.ref_persistent_blinks(*args, None, 0)
+        n, t, k = self._hist(hist)
+        h = (n, None if t is None else _p(t, C.c_double), k)
+        total = self.L.ref_persistent_blinks(*args, None, 0, *h)
         out = np.zeros((max(total, 1), 5))
-        self.L.ref_persistent_blinks(*args, _p(out, C.c_double), total)
+        self.L.ref_persistent_blinks(*args, _p(out, C.c_double), total, *h)
         return counts, out[:total]
 
     def noise_setup(self, seed, w, h, offset, offset_std, gain, gain_std, read, read_std, sensor_h=0, row_y0=0):
@@ -863,6 +879,8 @@ class Reference:
         L.ref_world_free.argtypes = [P]
         L.ref_world_set_label.argtypes = [P, I32, PD]
         L.ref_world_set_label.restype = I32
+        L.ref_world_set_history.argtypes = [P, PD, C.POINTER(Kinetics), I32]
+        L.ref_world_set_history.restype = I32
         L.ref_world_get_label.argtypes = [P, I32, PD]
         L.ref_world_cells.argtypes = [P, D, D, D, D, PD, I32]
         L.ref_world_cells.restype = I32
@@ -1000,6 +1018,12 @@ class Reference:
     def world_set_label(self, w, structure: int, vec) -> int:
         vec = np.ascontiguousarray(vec, np.float64)
         return self._world_sigs().ref_world_set_label(w, structure, _p(vec, C.c_double))
+
+    def world_set_history(self, w, tStart, kin) -> int:
+        """World::SetKineticsHistory (core/src/world.cpp:933): kin[i * 4 + s] per segment i and structure s."""
+        t = np.ascontiguousarray(tStart, np.float64)
+        k = (Kinetics * max(1, len(kin)))(*kin)
+        return self._world_sigs().ref_world_set_history(w, _p(t, C.c_double), k, len(t))
 
     def world_get_label(self, w, structure: int):
         out = np.zeros(14)

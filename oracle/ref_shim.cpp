@@ -592,18 +592,48 @@ static isc::Label MakeLabel(int32_t mode, int32_t preState, const orc_kinetics* 
    return l;
 }
 
-// LabelSchedule (core/src/dyes.cpp:440) for nDyes addresses (h1, k, n as 3 x uint32), blinks only: counts[i] per
-// dye, out = every blink (tOn, tOff, brightness) in dye order, up to cap. Returns the total.
+static isc::Kinetics KinOf(const orc_kinetics& k)
+{
+   isc::Kinetics q;
+   q.activationRatePerSec = k.activationRatePerSec;
+   q.onSec = k.onSec;
+   q.offSec = k.offSec;
+   q.bleachProb = k.bleachProb;
+   q.photonCV = k.photonCV;
+   q.initialOnSec = k.initialOnSec;
+   return q;
+}
+
+// A KineticsHistory of nSeg segments (tStart[i], segKin[i]); nullptr for nSeg 0 (none).
+static const isc::KineticsHistory* HistOf(int32_t nSeg, const double* tStart, const orc_kinetics* segKin,
+                                          isc::KineticsHistory& h)
+{
+   if (nSeg <= 0)
+      return nullptr;
+   h.tStart.assign(tStart, tStart + nSeg);
+   h.kin.clear();
+   for (int32_t i = 0; i < nSeg; ++i)
+      h.kin.push_back(KinOf(segKin[i]));
+   return &h;
+}
+
+// LabelSchedule (core/src/dyes.cpp:440) for nDyes addresses (h1, k, n as 3 x uint32), blinks only, under the history
+// of nSeg segments (0: none): counts[i] per dye, out = every blink (tOn, tOff, brightness) in dye order, up to cap.
+// Returns the total.
 int64_t ref_label_blinks(const uint32_t* dyes, int32_t nDyes, int32_t mode, const orc_kinetics* kin, double tMax,
-                         int32_t* counts, double* out, int64_t cap)
+                         int32_t* counts, double* out, int64_t cap, int32_t nSeg, const double* tStart,
+                         const orc_kinetics* segKin)
 {
    const isc::Label l = MakeLabel(mode, 0, kin);
+   isc::KineticsHistory hh;
+   const isc::KineticsHistory* hist = HistOf(nSeg, tStart, segKin, hh);
    std::vector<isc::Blink> b;
    int64_t total = 0;
    for (int32_t i = 0; i < nDyes; ++i)
    {
       b.clear();
-      isc::LabelSchedule(dyes[3 * i], (int32_t)dyes[3 * i + 1], (int32_t)dyes[3 * i + 2], l, &b, nullptr, tMax);
+      isc::LabelSchedule(dyes[3 * i], (int32_t)dyes[3 * i + 1], (int32_t)dyes[3 * i + 2], l, &b, nullptr, tMax,
+                         hist);
       counts[i] = (int32_t)b.size();
       for (const isc::Blink& x : b)
       {
@@ -619,16 +649,20 @@ int64_t ref_label_blinks(const uint32_t* dyes, int32_t nDyes, int32_t mode, cons
    return total;
 }
 
-// LabelSchedule's continuous windows for nDyes addresses: has[i] windows (0 or 1), w = 4 doubles per dye.
+// LabelSchedule's continuous windows for nDyes addresses (history as ref_label_blinks'): has[i] windows (0 or 1),
+// w = 4 doubles per dye.
 void ref_label_cont(const uint32_t* dyes, int32_t nDyes, int32_t mode, int32_t preState, const orc_kinetics* kin,
-                    int32_t* has, double* w)
+                    int32_t* has, double* w, int32_t nSeg, const double* tStart, const orc_kinetics* segKin)
 {
    const isc::Label l = MakeLabel(mode, preState, kin);
+   isc::KineticsHistory hh;
+   const isc::KineticsHistory* hist = HistOf(nSeg, tStart, segKin, hh);
    std::vector<isc::ContWindow> c;
    for (int32_t i = 0; i < nDyes; ++i)
    {
       c.clear();
-      isc::LabelSchedule(dyes[3 * i], (int32_t)dyes[3 * i + 1], (int32_t)dyes[3 * i + 2], l, nullptr, &c);
+      isc::LabelSchedule(dyes[3 * i], (int32_t)dyes[3 * i + 1], (int32_t)dyes[3 * i + 2], l, nullptr, &c, INFINITY,
+                         hist);
       has[i] = (int32_t)c.size();
       if (!c.empty())
       {
@@ -642,11 +676,14 @@ void ref_label_cont(const uint32_t* dyes, int32_t nDyes, int32_t mode, int32_t p
 
 // PersistentBlinks (bins 0: window t0, t1) or PersistentBlinksInBins (bins [binLo, binHi]) (core/src/dyes.cpp:636,
 // :648) for nDyes addresses: counts per dye, out = 5 doubles per blink (tOn, tOff, brightness, bin, j; bin and j 0
-// for the window form) up to cap. Returns the total.
+// for the window form) up to cap, under the history of nSeg segments (0: none). Returns the total.
 int64_t ref_persistent_blinks(const uint32_t* dyes, int32_t nDyes, const orc_kinetics* kin, int32_t bins, double t0,
-                              double t1, int64_t binLo, int64_t binHi, int32_t* counts, double* out, int64_t cap)
+                              double t1, int64_t binLo, int64_t binHi, int32_t* counts, double* out, int64_t cap,
+                              int32_t nSeg, const double* tStart, const orc_kinetics* segKin)
 {
    const isc::Label l = MakeLabel(2, 0, kin);
+   isc::KineticsHistory hh;
+   const isc::KineticsHistory* hist = HistOf(nSeg, tStart, segKin, hh);
    int64_t total = 0;
    for (int32_t i = 0; i < nDyes; ++i)
    {
@@ -656,14 +693,14 @@ int64_t ref_persistent_blinks(const uint32_t* dyes, int32_t nDyes, const orc_kin
       if (bins)
       {
          std::vector<isc::BinBlink> b;
-         isc::PersistentBlinksInBins(h1, k, n, l.kin, (long)binLo, (long)binHi, b);
+         isc::PersistentBlinksInBins(h1, k, n, l.kin, (long)binLo, (long)binHi, b, hist);
          for (const isc::BinBlink& x : b)
             rec.insert(rec.end(), { x.tOn, x.tOff, x.brightness, (double)x.bin, (double)x.j });
       }
       else
       {
          std::vector<isc::Blink> b;
-         isc::PersistentBlinks(h1, k, n, l.kin, t0, t1, b);
+         isc::PersistentBlinks(h1, k, n, l.kin, t0, t1, b, hist);
          for (const isc::Blink& x : b)
             rec.insert(rec.end(), { x.tOn, x.tOff, x.brightness, 0.0, 0.0 });
       }
@@ -1131,6 +1168,17 @@ int32_t ref_world_set_label(void* wp, int32_t s, const double* v)
    if (l.hasLinkage)
       l.linkage = isc::Linkage{v[16], v[17], v[18], v[19]};
    return static_cast<isc::World*>(wp)->SetLabel(s, l) ? 0 : -1;
+}
+
+// World::SetKineticsHistory (core/src/world.cpp:933): n segment starts, kin[i * 4 + s] structure s's kinetics in
+// segment i. 0, or -1 when the world refuses it.
+int32_t ref_world_set_history(void* wp, const double* tStart, const orc_kinetics* kin, int32_t n)
+{
+   std::vector<double> t(tStart, tStart + n);
+   std::vector<isc::Kinetics> k;
+   for (int32_t i = 0; i < 4 * n; ++i)
+      k.push_back(KinOf(kin[i]));
+   return static_cast<isc::World*>(wp)->SetKineticsHistory(t, k) ? 0 : -1;
 }
 
 // The label the world keeps for structure s, linkage normalised: density, fraction, mode, pre state, rate, on, off,

@@ -317,6 +317,39 @@ DYE_PERSIST_CASES = [
 ]
 
 
+# Kinetics histories (KineticsHistory, core/src/dyes.h:138-149): (name, mode, preState, tStart, [kinetics per
+# segment], tMax) of the LabelSchedule cases, (name, tStart, [kinetics], t0, t1, bins) of the persistent ones.
+_HA = dict(activationRatePerSec=0.4, onSec=0.1, offSec=1.0, bleachProb=0.3, photonCV=0.5, initialOnSec=0.4)
+_HB = dict(activationRatePerSec=3.0, onSec=0.02, offSec=0.2, bleachProb=0.05, photonCV=0.0, initialOnSec=0.0)
+_HC = dict(activationRatePerSec=0.1, onSec=0.3, offSec=4.0, bleachProb=0.5, photonCV=0.8, initialOnSec=2.0)
+DYE_HIST_LABEL_CASES = [
+    ("dstorm_3seg", 0, 0, [0.0, 2.5, 6.0], [_HA, _HB, _HC], 60.0),
+    ("dstorm_init_crosses", 0, 0, [0.0, 0.05, 0.3], [dict(_HA, initialOnSec=2.0), _HC, dict(_HB, initialOnSec=0.5)],
+     40.0),
+    ("dstorm_init_stops", 0, 0, [0.0, 0.2], [dict(_HA, initialOnSec=1.0), _HB], 30.0),
+    ("dstorm_rate0_first", 0, 0, [0.0, 5.0], [dict(_HA, activationRatePerSec=0.0), dict(_HB, bleachProb=0.01)], 12.0),
+    ("palm_pre_2seg", 1, 1, [0.0, 10.0], [dict(_HA, activationRatePerSec=0.05, photonCV=0.0),
+                                          dict(_HA, activationRatePerSec=1.5, onSec=0.5)], 200.0),
+    ("palm_pre_never", 1, 1, [0.0, 1.0], [dict(_HB, activationRatePerSec=0.0), dict(_HB, activationRatePerSec=0.0)],
+     100.0),
+    ("palm_many", 1, 0, [0.37 * i for i in range(12)],
+     [dict(_HB if i % 3 else _HC, activationRatePerSec=(0.0, 0.8, 5.0)[i % 3], bleachProb=0.02) for i in range(12)],
+     6.0),
+    ("paint", 2, 0, [0.0, 1.0], [_HA, _HB], 100.0),
+    ("widefield", 3, 0, [0.0, 1.0], [_HA, _HB], 100.0),
+]
+DYE_HIST_PERSIST_CASES = [
+    ("win_3seg", [0.0, 2.3, 5.0], [dict(activationRatePerSec=2.0), dict(activationRatePerSec=0.5, onSec=0.4),
+                                   dict(activationRatePerSec=8.0, onSec=0.01, photonCV=0.0)], 1.0, 7.5, None),
+    ("bins_split", [0.0, 0.25, 0.5, 0.75, 2.0],
+     [dict(activationRatePerSec=3.0), dict(activationRatePerSec=0.0), dict(activationRatePerSec=40.0),
+      dict(activationRatePerSec=1.0, photonCV=0.0), dict(activationRatePerSec=40.0, onSec=0.01)], 0.0, 0.0, (0, 2)),
+    ("win_late", [0.0, 100.5], [dict(activationRatePerSec=0.7, onSec=0.3), dict(activationRatePerSec=0.2, onSec=2.0)],
+     95.0, 110.0, None),
+    ("bins_one_seg", [0.0], [dict(activationRatePerSec=0.9)], 0.0, 0.0, (2, 6)),
+]
+
+
 def dye_addresses(seed, n):
     rng = np.random.default_rng(seed)
     d = np.zeros((n, 3), np.uint32)
@@ -372,6 +405,29 @@ def dyes(R):
         out["jsm_" + fn + "_x"] = x
         out["jsm_" + fn + "_y"] = R.jsm(fn, x)
     np.savez_compressed(os.path.join(HERE, "dyes_ref.npz"), **out)
+
+
+def dyes_hist(R):
+    """The schedules under kinetics histories, from the compiled reference (LabelSchedule's blinks and continuous
+    windows, PersistentBlinks / PersistentBlinksInBins with a KineticsHistory*)."""
+    out = {}
+    d = dye_addresses(12, 200)
+    out["dyes"] = d
+    for name, mode, pre, t_start, kws, t_max in DYE_HIST_LABEL_CASES:
+        hist = (t_start, [O.kinetics(**k) for k in kws])
+        kin = O.kinetics(activationRatePerSec=0.01)
+        c, b = R.label_blinks(d, mode, kin, t_max, hist=hist)
+        out["label_" + name + "_counts"] = c
+        out["label_" + name + "_blinks"] = b
+        has, w = R.label_cont(d, mode, pre, kin, hist=hist)
+        out["cont_" + name + "_has"] = has
+        out["cont_" + name + "_w"] = w * (has[:, None] != 0)
+    for name, t_start, kws, t0, t1, bins in DYE_HIST_PERSIST_CASES:
+        hist = (t_start, [O.kinetics(**k) for k in kws])
+        c, b = R.persistent_blinks(d[:80], O.kinetics(activationRatePerSec=0.01), t0, t1, bins, hist=hist)
+        out["persist_" + name + "_counts"] = c
+        out["persist_" + name + "_blinks"] = b
+    np.savez_compressed(os.path.join(HERE, "dyes_hist_ref.npz"), **out)
 
 
 ROI_PIN_CASES = {  # name: (oversampling, kernelHalfWidthPx, sampleDepthNm, sampleIndex)
@@ -569,6 +625,7 @@ def main():
     wf(R, P)
     binned(R, P)
     dyes(R)
+    dyes_hist(R)
     roi(R)
     pupil_samples(R)
     psf_big(R)

@@ -354,3 +354,57 @@ def test_queries_on_device_tensors(field):
     torch.cuda.synchronize()
     tot_h, host_out = W.Density3dInWindow(*win, 256, 256, 64, 0xF)
     assert tot == tot_h and np.array_equal(out.cpu().numpy(), host_out)
+
+
+# A kinetics history for every structure (World::SetKineticsHistory): three segments; per segment the kinetics of
+# structures 0..3 (rate, on, off, bleach, cv, initial on). Segment 0 carries initial ON times, so dSTORM's walk of it
+# crosses a change; segment 1 switches PALM pores and DNA-PAINT on harder; segment 2 slows everything.
+HIST_T = [0.0, 0.8, 2.6]
+HIST_KIN = [
+    [(0.5, 0.05, 1.0, 0.3, 0.5, 1.2), (0.2, 0.1, 0.5, 0.5, 0.4, 0.0), (0.3, 0.2, 1.0, 1.0, 0.5, 0.0),
+     (0.01, 0.05, 1.0, 1.0, 0.5, 0.0)],
+    [(4.0, 0.02, 0.1, 0.05, 0.0, 0.0), (3.0, 0.05, 0.2, 0.1, 0.8, 0.0), (6.0, 0.03, 1.0, 1.0, 0.0, 0.0),
+     (0.01, 0.05, 1.0, 1.0, 0.5, 0.0)],
+    [(0.1, 0.3, 3.0, 0.5, 1.0, 2.0), (0.05, 0.4, 2.0, 0.9, 0.2, 0.0), (0.7, 0.5, 1.0, 1.0, 0.3, 0.0),
+     (0.01, 0.05, 1.0, 1.0, 0.5, 0.0)],
+]
+
+
+@needs_ref
+@pytest.mark.parametrize("label", ["mixed", "defaults", "long"])
+def test_events_under_kinetics_history_vs_reference(field, label):
+    """EventsInWindow and ContinuousInWindow under World::SetKineticsHistory (HistoryOf per structure: dSTORM /
+    PALM through DyeScheduleHistory, DNA-PAINT through PersistentGen's pieces and the lookback of the longest ON of
+    any segment) against the compiled reference's World; then cleared, the labels' own kinetics again."""
+    R, w, W = field
+    set_labels(R, w, W, label)
+    flat = [k for seg in HIST_KIN for k in seg]
+    assert R.world_set_history(w, HIST_T, [O.Kinetics(*k) for k in flat]) == 0
+    W.SetKineticsHistory(HIST_T, [host.Kinetics(*k) for k in flat])
+    try:
+        for win in WINDOWS[:2]:
+            for t0, t1 in [(0.0, 2.0), (2.2, 3.1), (0.75, 0.85)]:
+                want = R.world_events(w, win, t0, t1)
+                got = event_matrix(W.EventsInWindow(*win, t0, t1))
+                print(label, win, (t0, t1), "history events", len(want), flush=True)
+                assert same_rows(got, want), (label, win, t0, t1)
+            want = R.world_events(w, win, 0.1, 0.0, continuous=True)
+            got = event_matrix(W.ContinuousInWindow(*win, 0.1))
+            assert same_rows(got, want), (label, win, "continuous")
+    finally:
+        assert R.world_set_history(w, [], []) == 0
+        W.SetKineticsHistory([], [])
+    want = R.world_events(w, WINDOWS[0], 0.0, 1.0)
+    assert same_rows(event_matrix(W.EventsInWindow(*WINDOWS[0], 0.0, 1.0)), want)
+
+
+def test_set_history_refuses_bad_input(ctx):
+    W = host.World(3, one_cell_geometry(0, 0), ctx)
+    good = [host.Kinetics()] * 8
+    for t in ([0.5, 1.0], [0.0, 0.0], [0.0, float("inf")]):
+        with pytest.raises(host.IscgError):
+            W.SetKineticsHistory(t, good)
+    with pytest.raises(host.IscgError):
+        W.SetKineticsHistory([0.0, 1.0], [host.Kinetics(onSec=0.0)] + good[1:])
+    W.SetKineticsHistory([0.0, 1.0], good)
+    W.close()

@@ -65,7 +65,9 @@ def test_struct_layouts_match_header(tmp_path):
              "iscg_wf_dep": _lib.WfDep, "iscg_wf_job": _lib.WfJob, "iscg_roi_model": _lib.RoiModel,
              "iscg_psf_request": _lib.PsfRequest, "iscg_world_cell": _lib.WorldCell, "iscg_npc": _lib.Npc,
              "iscg_lamin_point": _lib.LaminPoint, "iscg_world_geometry": _lib.WorldGeometry,
-             "iscg_label": _lib.Label, "iscg_world_dye": _lib.WorldDye, "iscg_block_dye": _lib.BlockDye, "iscg_world_event": _lib.WorldEvent}
+             "iscg_label": _lib.Label, "iscg_world_dye": _lib.WorldDye, "iscg_block_dye": _lib.BlockDye,
+             "iscg_world_event": _lib.WorldEvent, "iscg_kinetics": _lib.Kinetics,
+             "iscg_kinetics_history": _lib.KineticsHistory}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "iscg.h"', "int main(void) {"]
     for cname, py in pairs.items():
         lines.append(f'printf("{cname} <sizeof> %zu\\n", sizeof({cname}));')
@@ -85,6 +87,19 @@ def test_struct_layouts_match_header(tmp_path):
         assert got[(cname, "<sizeof>")] == C.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_host_dye_arguments():
+    """The host mirrors of isc::Label and isc::KineticsHistory build their C arguments without a GPU: the label's
+    fields in order, the history's segment count and arrays (n = 0 for none)."""
+    lab = host.Label("PALM", host.Kinetics(0.5, 0.1), True, density=0.4)
+    c = lab.to_c()
+    assert (c.mode, c.preState, c.density, c.kin.activationRatePerSec, c.kin.onSec) == (1, 1, 0.4, 0.5, 0.1)
+    h = host.KineticsHistory([0.0, 2.5, 6.0], [host.Kinetics(1.0), host.Kinetics(2.0), host.Kinetics(3.0)])
+    assert h._c.n == 3 and h._c.tStart[1] == 2.5 and h._c.kin[2].activationRatePerSec == 3.0
+    assert host.KineticsHistory([], [])._c.n == 0
+    with pytest.raises(host.IscgError):
+        host.KineticsHistory([0.0, 1.0], [host.Kinetics()])
 
 
 def splat_fixture(golden):
