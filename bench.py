@@ -640,7 +640,8 @@ def cpu_dyes(n=200000):
     counts = np.zeros(n, np.int32)
     total = []
     reps, el = timed_repeat(lambda: total.append(ref.L.ref_label_blinks(
-        d.ctypes.data, n, 0, C.byref(kin), DYE_T_MAX, counts.ctypes.data_as(C.POINTER(C.c_int32)), None, 0)),
+        d.ctypes.data, n, 0, C.byref(kin), DYE_T_MAX, counts.ctypes.data_as(C.POINTER(C.c_int32)), None, 0, 0, None,
+        None)),
         CPU_MIN_S_SECONDARY)
     return {"value": round(reps * n / el, 1), "unit": "dyes/s", "cores": 1, "kind": "reference",
             "sample": f"isc::LabelSchedule of {n} dyes ({total[0]} blinks) in one thread, {reps}x in {el:.2f} s"}
