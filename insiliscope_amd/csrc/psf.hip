@@ -173,11 +173,53 @@ __device__ __forceinline__ void split_store(float v, _Float16* hi, _Float16* lo)
    *lo = (_Float16)(v - (float)h);
 }
 
+// Splits v[0..2N) into FP16 hi = v rounded and lo = (v - hi) rounded, two values per conversion.
+template <int N, class H>
+__device__ __forceinline__ void split_pairs(const float* v, H& hi, H& lo)
+{
+   typedef float f2 __attribute__((ext_vector_type(2)));
+   typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+   for (int p = 0; p < N; ++p)
+   {
+      const f2 x = {v[2 * p], v[2 * p + 1]};
+      const h2 h = __builtin_convertvector(x, h2);
+      const h2 l = __builtin_convertvector(x - __builtin_convertvector(h, f2), h2);
+      hi[2 * p] = h.x;
+      hi[2 * p + 1] = h.y;
+      lo[2 * p] = l.x;
+      lo[2 * p + 1] = l.y;
+   }
+}
+
 // Wraps an FP64 angle to [-pi, pi] in FP64, then FP32 sincos of the remainder.
 __device__ __forceinline__ void wrapped_sincos(double ang, float* s, float* c)
 {
    const double r = ang - 6.283185307179586476925 * rint(ang * 0.15915494309189533577);
    sincosf((float)r, s, c);
+}
+
+// The same values without a branch (so it schedules beside matrix-core work): the angle reduced in FP64 to
+// y = ang - q pi/2 (q = rint(ang 2/pi), |y| <= pi/4), the Taylor polynomials of sin (to y^9) and cos (to y^10) in
+// FP32 by Horner with FMA (truncation below 2e-9 on |y| <= pi/4, under the FP32 rounding of the result), then the
+// quadrant q mod 4 swaps and signs them.
+__device__ __forceinline__ void quadrant_sincos(double ang, float* s, float* c)
+{
+   const double q = rint(ang * 0.63661977236758134308);
+   const float y = (float)(ang - q * 1.57079632679489661923), y2 = y * y;
+   float ps = fmaf(y2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+   ps = fmaf(y2, ps, 1.0f / 120.0f);
+   ps = fmaf(y2, ps, -1.0f / 6.0f);
+   const float sp = fmaf(y * y2, ps, y);
+   float pc = fmaf(y2, -1.0f / 3628800.0f, 1.0f / 40320.0f);
+   pc = fmaf(y2, pc, -1.0f / 720.0f);
+   pc = fmaf(y2, pc, 1.0f / 24.0f);
+   pc = fmaf(y2, pc, -0.5f);
+   const float cp = fmaf(y2, pc, 1.0f);
+   const int qi = (int)(long long)q & 3;  // two's complement: q = -1 -> 3
+   const float sa = (qi & 1) ? cp : sp, ca = (qi & 1) ? sp : cp;
+   *s = (qi & 2) ? -sa : sa;
+   *c = ((qi + 1) & 2) ? -ca : ca;
 }
 
 // W planes per request: [4][Pp][Mp] halves (re hi, re lo, im hi, im lo); zero beyond P rows / M columns.
@@ -676,13 +718,16 @@ __global__ __launch_bounds__(kPsfThreads, WLDS ? 2 : 4) void psf_small_kernel(co
 //      fragments are reused across the tiles, each W fragment read once;
 //   2. mid, split and transposed, in LDS (midT[x][n], rows x < P only); E[y][x] = sum_n W[y][n] mid[n][x] for the
 //      wave's E tiles over all MP (K).
-// Two barriers per plane: after step 0 (the factors are in LDS and every wave has left the previous plane's step 2,
-// so midT may be rewritten) and after step 1. LDS holds W and midT for the P valid rows only (rows >= P read as
-// zeros): 2 x 4 x 66 x 136 halves + 2145 factors at MP 128 (157 KB), one workgroup per CU. Workgroups per request
+// Two barriers per plane: after step 1 (midT complete, every wave done reading the plane's factors) and after step 2
+// (the next plane's factors, made during step 2 beside the products, are in LDS, and every wave has left step 2, so
+// midT may be rewritten). The first plane's factors are made before the loop. LDS holds W and midT for the P valid rows only (the padded rows >= P of
+// a tile read row P - 1 and their results are not stored): 2 x 4 x 66 x 136 halves + 2145 factors at MP 128
+// (157 KB), one workgroup per CU. The products keep the real part's two terms in two accumulators (re re - im im at
+// the end), so no fragment is negated, and each lane's defocus B values are read once per workgroup. Workgroups per request
 // (zg) and their XCD placement as in psf_small_kernel. The pupil samples differ from the FP64-phase kernels' by
 // FP32 rounding (one complex product more), so this form is not a tuning candidate beside them.
 constexpr int kRbRows = 66;
-template <int MP>
+template <int MP, int CTF>  // CTF: the tiles per side when known at compile time (5: planes padded to 80), else 0
 __global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfReqDev* __restrict__ reqs,
                                                                      const float2* __restrict__ P0,
                                                                      const double* __restrict__ B,
@@ -702,7 +747,7 @@ __global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfR
    static_assert(NRB * G == kPsfWaves, "row blocks must divide the waves");
    __shared__ __attribute__((aligned(16))) _Float16 wl[4][kRbRows][PS];
    __shared__ __attribute__((aligned(16))) _Float16 midT[4][kRbRows][PS];
-   __shared__ float2 defocus[ND];
+   __shared__ float2 defocus[ND + 1];  // + a spare slot the lanes past the last entry write
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int b = (int)blockIdx.x, slot = b / kXcds;
    const int r = (slot / zg) * kXcds + b % kXcds;
@@ -710,14 +755,13 @@ __global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfR
       return;  // the last XCD row's spare workgroups
    const PsfReqDev& q = reqs[r];
    const int P = q.size;
-   const int CT = Pp / 16, TE = CT * CT;
+   const int CT = CTF ? CTF : Pp / 16, TE = CT * CT;
    const size_t wPer = (size_t)Pp * MP;
    const _Float16* Wr = W + (size_t)r * 4 * wPer;
    const float2* P0r = P0 + (size_t)r * MP * MP;
    const double* Br = B + (size_t)r * MP * MP;
    const int c0 = q.M / 2, nd = (c0 + 1) * (c0 + 2) / 2;  // M <= MP: c0 <= HM, nd <= ND
    const int lr = lane & 15, lk = 8 * (lane >> 4);
-   const half8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
    for (int i = tid; i < 4 * P * (MP / 8); i += kPsfThreads)
    {
       const int part = i / (P * (MP / 8)), rem = i % (P * (MP / 8)), row = rem / (MP / 8), c8 = rem % (MP / 8);
@@ -729,11 +773,15 @@ __global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfR
    // this lane's pupil row n = 16 rb + lr: its offset from the centre (rows past the M grid are zero in P0 and take
    // any in-range factor)
    const int ai = min(abs(16 * rb + lr - c0), c0);
-   for (int zi = z0; zi < z1; ++zi)
+   // the defocus entries e = tid + kPsfThreads u of this lane: hi (hi + 1) / 2 + lo = e, their B read once here
+   constexpr int NDT = (ND + kPsfThreads - 1) / kPsfThreads;
+   double bdef[NDT];
+#pragma unroll
+   for (int u = 0; u < NDT; ++u)
    {
-      // 0. the plane's defocus factors; entry e = hi (hi + 1) / 2 + lo, read at B[c0 - hi][c0 - lo]
-      const double t = tz[(size_t)r * nz + zi];
-      for (int e = tid; e < nd; e += kPsfThreads)
+      const int e = tid + kPsfThreads * u;
+      bdef[u] = 0.0;
+      if (e < nd)
       {
          int hi = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
          while (hi * (hi + 1) / 2 > e)
@@ -741,101 +789,145 @@ __global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfR
          while ((hi + 1) * (hi + 2) / 2 <= e)
             ++hi;
          const int lo = e - hi * (hi + 1) / 2;
-         float sn, c;
-         wrapped_sincos(t * Br[(size_t)(c0 - hi) * MP + (c0 - lo)], &sn, &c);
-         defocus[e] = make_float2(c, sn);
+         bdef[u] = Br[(size_t)(c0 - hi) * MP + (c0 - lo)];
       }
-      __syncthreads();
-      // 1. this wave's pupil rows n = 16 rb + lr, columns m = 32 s + lk + j, as split A fragments
-      half8 prh[KS], prl[KS], pih[KS], pil[KS];
+   }
+   // entry u of this lane's defocus factors for the plane at t (the lanes past the last entry write the spare slot)
+   auto defocus_entry = [&](int u, double t) {
+      const int e = tid + kPsfThreads * u;
+      float sn, c;
+      quadrant_sincos(t * bdef[u], &sn, &c);
+      defocus[e < nd ? e : ND] = make_float2(c, sn);
+   };
+   // 0. the first plane's defocus factors (each later plane's are made during the previous plane's step 2)
+   if (z0 < z1)
+   {
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks)
-      {
+      for (int u = 0; u < NDT; ++u)
+         defocus_entry(u, tz[(size_t)r * nz + z0]);
+   }
+   __syncthreads();  // W staged, the first factors in LDS
+   for (int zi = z0; zi < z1; ++zi)
+   {
+      // 1. this wave's pupil rows n = 16 rb + lr, columns m = 32 ks + lk + j, as split A fragments, one k-step at a
+      // time; mid[n][x] = sum_m Pup[n][m] W[x][m] over the wave's column tiles, k-steps outermost so the next step's
+      // pupil (VALU, LDS reads) can be scheduled beside this step's products. The real part is (re re) - (im im)
+      // from two accumulators, so no fragment is negated; columns x >= P read row P - 1 (their columns are not
+      // stored).
+      auto pupil = [&](int ks, half8& rh, half8& rl, half8& ih, half8& il) {
          const float4* src = reinterpret_cast<const float4*>(P0r + (size_t)(16 * rb + lr) * MP + 32 * ks + lk);
          float4 v[4];
 #pragma unroll
          for (int u = 0; u < 4; ++u)
             v[u] = src[u];
+         float2 d[8];
+#pragma unroll
+         for (int j = 0; j < 8; ++j)
+         {
+            const int aj = min(abs(32 * ks + lk + j - c0), c0);
+            const int hi = max(ai, aj), lo = min(ai, aj);
+            d[j] = defocus[hi * (hi + 1) / 2 + lo];
+         }
+         float cr[8], ci[8];
 #pragma unroll
          for (int j = 0; j < 8; ++j)
          {
             const float pc = (j & 1) ? v[j >> 1].z : v[j >> 1].x, ps = (j & 1) ? v[j >> 1].w : v[j >> 1].y;
-            const int aj = min(abs(32 * ks + lk + j - c0), c0);
-            const int hi = max(ai, aj), lo = min(ai, aj);
-            const float2 d = defocus[hi * (hi + 1) / 2 + lo];
-            const float c = pc * d.x - ps * d.y, sn = pc * d.y + ps * d.x;
-            const _Float16 ch = (_Float16)c, sh = (_Float16)sn;
-            prh[ks][j] = ch;
-            prl[ks][j] = (_Float16)(c - (float)ch);
-            pih[ks][j] = sh;
-            pil[ks][j] = (_Float16)(sn - (float)sh);
+            cr[j] = fmaf(pc, d[j].x, -(ps * d[j].y));
+            ci[j] = fmaf(pc, d[j].y, ps * d[j].x);
+         }
+         split_pairs<4>(cr, rh, rl);
+         split_pairs<4>(ci, ih, il);
+      };
+      float4v mRp[PM], mRn[PM], mI[PM];
+#pragma unroll
+      for (int s = 0; s < PM; ++s)
+         mRp[s] = mRn[s] = mI[s] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+      half8 prh, prl, pih, pil;
+      pupil(0, prh, prl, pih, pil);
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+      {
+         half8 nrh, nrl, nih, nil;
+         if (ks + 1 < KS)
+            pupil(ks + 1, nrh, nrl, nih, nil);
+#pragma unroll
+         for (int s = 0; s < PM; ++s)
+         {
+            const int ct = g0 + G * s;
+            if (ct < CT)
+            {
+               const int xr = min(16 * ct + lr, P - 1);
+               const half8 brh = ld8(&wl[0][xr][32 * ks + lk]), brl = ld8(&wl[1][xr][32 * ks + lk]);
+               const half8 bih = ld8(&wl[2][xr][32 * ks + lk]), bil = ld8(&wl[3][xr][32 * ks + lk]);
+               mRp[s] = mfma3(prh, prl, brh, brl, mRp[s]);
+               mRn[s] = mfma3(pih, pil, bih, bil, mRn[s]);
+               mI[s] = mfma3(prh, prl, bih, bil, mI[s]);
+               mI[s] = mfma3(pih, pil, brh, brl, mI[s]);
+            }
+         }
+         if (ks + 1 < KS)
+         {
+            prh = nrh;
+            prl = nrl;
+            pih = nih;
+            pil = nil;
          }
       }
-#pragma unroll 1
+      // D rows n = 16 rb + 4 (lane >> 4) + i, column x -> midT[.][x][n .. n + 3]
+#pragma unroll
       for (int s = 0; s < PM; ++s)
       {
-         const int ct = g0 + G * s;
-         if (ct >= CT)
-            continue;
-         const int x = 16 * ct + lr;
-         float4v mR = {0.0f, 0.0f, 0.0f, 0.0f}, mI = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-         for (int ks = 0; ks < KS; ++ks)
-         {
-            const bool in = x < P;
-            const half8 brh = in ? ld8(&wl[0][x][32 * ks + lk]) : zero8, brl = in ? ld8(&wl[1][x][32 * ks + lk]) : zero8;
-            const half8 bih = in ? ld8(&wl[2][x][32 * ks + lk]) : zero8, bil = in ? ld8(&wl[3][x][32 * ks + lk]) : zero8;
-            mR = mfma3(prh[ks], prl[ks], brh, brl, mR);
-            mR = mfma3(-pih[ks], -pil[ks], bih, bil, mR);
-            mI = mfma3(prh[ks], prl[ks], bih, bil, mI);
-            mI = mfma3(pih[ks], pil[ks], brh, brl, mI);
-         }
-         // D rows n = 16 rb + 4 (lane >> 4) + i, column x -> midT[.][x][n .. n + 3]
-         if (x < P)
+         const int x = 16 * (g0 + G * s) + lr;
+         if (g0 + G * s < CT && x < P)
          {
             const int n = 16 * rb + 4 * (lane >> 4);
             typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+            const float4v mR = mRp[s] - mRn[s];
+            const float vr[4] = {mR[0], mR[1], mR[2], mR[3]}, vi[4] = {mI[s][0], mI[s][1], mI[s][2], mI[s][3]};
             half4 rh, rl, ih, il;
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-            {
-               const _Float16 a = (_Float16)mR[i], bb = (_Float16)mI[i];
-               rh[i] = a;
-               rl[i] = (_Float16)(mR[i] - (float)a);
-               ih[i] = bb;
-               il[i] = (_Float16)(mI[i] - (float)bb);
-            }
+            split_pairs<2>(vr, rh, rl);
+            split_pairs<2>(vi, ih, il);
             *(half4*)&midT[0][x][n] = rh;
             *(half4*)&midT[1][x][n] = rl;
             *(half4*)&midT[2][x][n] = ih;
             *(half4*)&midT[3][x][n] = il;
          }
       }
-      __syncthreads();
-      // 2. E tiles: E[y][x] = sum_n W[y][n] mid[n][x]
+      __syncthreads();  // midT complete; every wave's reads of this plane's defocus factors done
+      // 2. E tiles: E[y][x] = sum_n W[y][n] mid[n][x]; rows and columns >= P read row P - 1 and are not stored. The
+      // next plane's defocus factors (VALU) go in beside each tile's products (matrix cores) in one basic block
       float* plane = out + ((size_t)r * nz + zi) * P * P;
-#pragma unroll 1
+      const double tn = zi + 1 < z1 ? tz[(size_t)r * nz + zi + 1] : 0.0;  // past the last plane: unused factors
+#pragma unroll
       for (int s = 0; s < PE; ++s)
       {
          const int te = wave + kPsfWaves * s;
          if (te >= TE)
+         {
+            if (s < NDT)
+               defocus_entry(s, tn);
             continue;
-         const int ry = te / CT, cx = te % CT, y = 16 * ry + lr, x = 16 * cx + lr;
-         float4v eR = {0.0f, 0.0f, 0.0f, 0.0f}, eI = {0.0f, 0.0f, 0.0f, 0.0f};
+         }
+         const int ry = te / CT, cx = te % CT;
+         const int yr = min(16 * ry + lr, P - 1), xr = min(16 * cx + lr, P - 1);
+         float4v eRp = {0.0f, 0.0f, 0.0f, 0.0f}, eRn = eRp, eI = eRp;
 #pragma unroll
          for (int ks = 0; ks < KS; ++ks)
          {
-            const bool iy = y < P, ix = x < P;
             const int k = 32 * ks + lk;
-            const half8 arh = iy ? ld8(&wl[0][y][k]) : zero8, arl = iy ? ld8(&wl[1][y][k]) : zero8;
-            const half8 aih = iy ? ld8(&wl[2][y][k]) : zero8, ail = iy ? ld8(&wl[3][y][k]) : zero8;
-            const half8 brh = ix ? ld8(&midT[0][x][k]) : zero8, brl = ix ? ld8(&midT[1][x][k]) : zero8;
-            const half8 bih = ix ? ld8(&midT[2][x][k]) : zero8, bil = ix ? ld8(&midT[3][x][k]) : zero8;
-            eR = mfma3(arh, arl, brh, brl, eR);
-            eR = mfma3(-aih, -ail, bih, bil, eR);
+            const half8 arh = ld8(&wl[0][yr][k]), arl = ld8(&wl[1][yr][k]);
+            const half8 aih = ld8(&wl[2][yr][k]), ail = ld8(&wl[3][yr][k]);
+            const half8 brh = ld8(&midT[0][xr][k]), brl = ld8(&midT[1][xr][k]);
+            const half8 bih = ld8(&midT[2][xr][k]), bil = ld8(&midT[3][xr][k]);
+            eRp = mfma3(arh, arl, brh, brl, eRp);
+            eRn = mfma3(aih, ail, bih, bil, eRn);
             eI = mfma3(arh, arl, bih, bil, eI);
             eI = mfma3(aih, ail, brh, brl, eI);
          }
+         if (s < NDT)
+            defocus_entry(s, tn);
+         const float4v eR = eRp - eRn;
          const int xo = 16 * cx + lr;
 #pragma unroll
          for (int i = 0; i < 4; ++i)
@@ -845,7 +937,10 @@ __global__ __launch_bounds__(kPsfThreads, 1) void psf_small_rb_kernel(const PsfR
                plane[(size_t)yo * P + xo] = eR[i] * eR[i] + eI[i] * eI[i];
          }
       }
-      // (the next plane's first barrier orders these midT reads before its step 1 rewrites midT)
+#pragma unroll
+      for (int u = PE; u < NDT; ++u)
+         defocus_entry(u, tn);
+      __syncthreads();  // the next factors in LDS; every wave's midT reads done before step 1 rewrites midT
    }
 }
 
@@ -1002,10 +1097,31 @@ static bool psf_planes_launch(iscg_context* h, const PsfBatch& s, int choice, fl
       if constexpr (MP == 32 || MP == 64 || MP == 128)
          if (rb)
          {
-            hipLaunchKernelGGL((psf_small_rb_kernel<MP>), dim3((nReq + kXcds - 1) / kXcds * kXcds * zg),
-                               dim3(kPsfThreads), 0, h->stream, s.reqs.as<const PsfReqDev>(),
-                               s.tabP.as<const float2>(), s.tabB.as<const double>(), s.W.as<const _Float16>(),
-                               s.tz.as<const double>(), nz, Pp, nReq, zg, out);
+            const dim3 grid((nReq + kXcds - 1) / kXcds * kXcds * zg);
+            // the tile loops resolved at compile time: every tile count at M 128 (its register budget is the
+            // tightest), the 5 x 5 tiles of planes padded to 80 at the smaller pupils
+            auto launch = [&](auto ctf) {
+               hipLaunchKernelGGL((psf_small_rb_kernel<MP, decltype(ctf)::value>), grid, dim3(kPsfThreads), 0,
+                                  h->stream, s.reqs.as<const PsfReqDev>(), s.tabP.as<const float2>(),
+                                  s.tabB.as<const double>(), s.W.as<const _Float16>(), s.tz.as<const double>(), nz,
+                                  Pp, nReq, zg, out);
+            };
+            const int ct = Pp / 16;
+            if (ct == 5)
+               launch(std::integral_constant<int, 5>{});
+            else if constexpr (MP == 128)
+            {
+               if (ct == 4)
+                  launch(std::integral_constant<int, 4>{});
+               else if (ct == 3)
+                  launch(std::integral_constant<int, 3>{});
+               else if (ct == 2)
+                  launch(std::integral_constant<int, 2>{});
+               else
+                  launch(std::integral_constant<int, 1>{});
+            }
+            else
+               launch(std::integral_constant<int, 0>{});
             return;
          }
       if (Pp <= kSmallPp)
