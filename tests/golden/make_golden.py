@@ -430,6 +430,29 @@ def dyes_hist(R):
     np.savez_compressed(os.path.join(HERE, "dyes_hist_ref.npz"), **out)
 
 
+# A window of the real seed-1 cell field (laminDensity 20, every structure DNA-PAINT at 0.7): 6.4 x 6.4 um at 100 nm,
+# 64 planes of 50 nm; the foci of the end-to-end check, 8 of them from 1.525 um in dye-plane steps.
+WORLD_WF_WINDOW = (9.6, 9.6, 16.0, 16.0, 0.0, 3.2)
+WORLD_WF_FOCI = (1.525, 0.05, 8)
+
+
+def world_wf(R):
+    """The end-to-end check of the cell field through the mean field: the compiled reference's World density of the
+    window (Density3dInWindow, 64 x 64 x 64) and its WidefieldScene focus series of that density with the small PSF
+    stack of psf_ref.npz (65^2 x 9 planes, os 4, 50 nm); the density as its nonzero voxels."""
+    P = np.load(os.path.join(HERE, "psf_ref.npz"))
+    w = R.world_new(1, {"laminDensity": 20})
+    for s in range(3):
+        assert R.world_set_label(w, s, R.label_vector()) == 0
+    total, dens = R.world_density3d(w, WORLD_WF_WINDOW, 64, 64, 64, 0xF)
+    R.world_free(w)
+    f0, df, nf = WORLD_WF_FOCI
+    imgs = R.wf_focus_series(dens, 50.0, P["small_planes"], 4, 50.0, f0 + df * np.arange(nf))
+    idx = np.flatnonzero(dens)
+    np.savez_compressed(os.path.join(HERE, "world_wf_ref.npz"), total=np.int64(total), shape=np.array(dens.shape),
+                        idx=idx.astype(np.int32), val=dens.reshape(-1)[idx], images=imgs)
+
+
 ROI_PIN_CASES = {  # name: (oversampling, kernelHalfWidthPx, sampleDepthNm, sampleIndex)
     "os4": (4, 8, 0.0, 1.33),
     "os4_depth": (4, 8, 3000.0, 1.33),
@@ -631,6 +654,7 @@ def main():
     psf_big(R)
     wf_drift(R)
     world(R)
+    world_wf(R)
     print("fixtures written to", HERE)
 
 

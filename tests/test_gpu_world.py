@@ -408,3 +408,63 @@ def test_set_history_refuses_bad_input(ctx):
         W.SetKineticsHistory([0.0, 1.0], [host.Kinetics(onSec=0.0)] + good[1:])
     W.SetKineticsHistory([0.0, 1.0], good)
     W.close()
+
+
+# ---- the cell field through the mean field, end to end ---------------------------------------------------------
+
+def _world_wf():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mg", os.path.join(GOLDEN, "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    return mg.WORLD_WF_WINDOW, mg.WORLD_WF_FOCI
+
+
+WF_TOL = 1e-5  # relative L2 per focus image: the FP32 3D FFT path against the reference's FP32 per-focus path
+
+
+def _focus_series_vs_fixture(ctx, golden, dens, fft):
+    G = golden("world_wf_ref.npz")
+    planes = golden("psf_ref.npz")["small_planes"]
+    f0, df, nf = _world_wf()[1]
+    cache = host.PsfKernelCache(planes, oversampling=4, zStepNm=50.0)
+    wf = host.WidefieldStack(ctx)
+    assert wf.Plan(cache, 64, 64, 64, df, 0, f0, nf, fft=fft), wf.err
+    img = np.zeros(wf.image_shape, np.float32)
+    assert wf.FocusSeries(dens, outImages=img), wf.err
+    want = G["images"]
+    errs = [float(np.sqrt(((img[f].astype(np.float64) - want[f]) ** 2).sum() / (want[f].astype(np.float64) ** 2).sum()))
+            for f in range(nf)]
+    print("cell-field window focus series, fft", fft, "rel L2 max %.3g" % max(errs), flush=True)
+    assert max(errs) <= WF_TOL, errs
+
+
+@pytest.mark.parametrize("fft", [(0, 0, 0), (128, 512, 128)])
+def test_cell_field_window_focus_series_vs_reference(ctx, golden, fft):
+    """The committed window of the real seed-1 cell field (tests/golden/world_wf_ref.npz: the compiled reference's
+    Density3dInWindow of a 6.4 um square, 64 planes, every structure DNA-PAINT at 0.7) through the device mean field:
+    the reference's WidefieldScene focus series of that density (8 foci, the small 65^2 x 9-plane PSF), per focus
+    within 1e-5 relative L2, on the five-pass and the slab paths."""
+    G = golden("world_wf_ref.npz")
+    dens = np.zeros(int(np.prod(G["shape"])), np.float32)
+    dens[G["idx"]] = G["val"]
+    _focus_series_vs_fixture(ctx, golden, dens.reshape(tuple(G["shape"])), fft)
+
+
+@needs_ref
+def test_cell_field_density_on_device_into_the_mean_field(field, golden):
+    """The same window generated on the device (World::Density3dInWindow into a device tensor) is the committed
+    density bit for bit, and that tensor, left in HBM, goes straight into the focus series."""
+    import torch
+    R, w, W = field
+    set_labels(R, w, W, "defaults")
+    G = golden("world_wf_ref.npz")
+    win = _world_wf()[0]
+    out = torch.zeros((64, 64, 64), dtype=torch.float32, device="cuda")
+    total, out = W.Density3dInWindow(*win, 64, 64, 64, 0xF, out=out)
+    torch.cuda.synchronize()
+    want = np.zeros(int(np.prod(G["shape"])), np.float32)
+    want[G["idx"]] = G["val"]
+    assert total == int(G["total"])
+    assert np.array_equal(out.cpu().numpy().reshape(-1).view(np.uint32), want.view(np.uint32))
+    _focus_series_vs_fixture(W.ctx, golden, out, (128, 512, 128))

@@ -192,3 +192,52 @@ def test_frames_closed_forms():
     assert np.allclose(np.linalg.norm(U, axis=1), 1, atol=1e-14)
     assert np.abs((T * U).sum(axis=1)).max() < 1e-14
     assert np.allclose(V, np.cross(T, U), atol=1e-15)
+
+
+def _mg():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mg", os.path.join(GOLDEN, "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    return mg
+
+
+def test_cell_field_window_images_are_its_density_convolved(orc):
+    """The end-to-end fixture's reference images (tests/golden/world_wf_ref.npz) are the oracle's FP64 direct
+    per-focus convolution of its density with the resampled small PSF (PlanFocus's blend and clamp), to FP32
+    rounding: the fixture pairs a density with its focus series, whatever produced either."""
+    mg = _mg()
+    G = np.load(os.path.join(GOLDEN, "world_wf_ref.npz"))
+    P = np.load(os.path.join(GOLDEN, "psf_ref.npz"))
+    dens = np.zeros(int(np.prod(G["shape"])), np.float32)
+    dens[G["idx"]] = G["val"]
+    dens = dens.reshape(tuple(G["shape"]))
+    assert int(G["total"]) == int(dens.sum())
+    f0, df, nf = mg.WORLD_WF_FOCI
+    R = orc.wf_radius(65, 4, 1)
+    K = np.stack([orc.wf_kernel(p, 4, 1, R) for p in P["small_planes"]])
+    want = np.maximum(orc.wf_direct(dens, df, 0, K, 50.0, f0 + df * np.arange(nf)), 0.0)
+    for f in range(nf):
+        e = np.sqrt(((G["images"][f] - want[f]) ** 2).sum() / (want[f] ** 2).sum())
+        assert e <= 1e-5, (f, e)
+
+
+@needs_ref
+def test_cell_field_window_fixture_is_the_references():
+    """The committed window is the compiled reference's World density of the real seed-1 field and its
+    WidefieldScene focus series, as written."""
+    mg = _mg()
+    R = O.Reference()
+    w = R.world_new(1, {"laminDensity": 20})
+    for s in range(3):
+        assert R.world_set_label(w, s, R.label_vector()) == 0
+    total, dens = R.world_density3d(w, mg.WORLD_WF_WINDOW, 64, 64, 64, 0xF)
+    R.world_free(w)
+    G = np.load(os.path.join(GOLDEN, "world_wf_ref.npz"))
+    assert total == int(G["total"])
+    idx = np.flatnonzero(dens)
+    assert np.array_equal(idx, G["idx"]) and np.array_equal(dens.reshape(-1)[idx], G["val"])
+    P = np.load(os.path.join(GOLDEN, "psf_ref.npz"))
+    f0, df, nf = mg.WORLD_WF_FOCI
+    imgs = R.wf_focus_series(dens, 50.0, P["small_planes"], 4, 50.0, f0 + df * np.arange(nf))
+    assert np.array_equal(imgs.view(np.uint32), G["images"].view(np.uint32))
