@@ -1133,6 +1133,37 @@ __global__ __launch_bounds__(kThreads) void cont_rows_kernel(const iscg_cont_win
    out[off[i]] = r;
 }
 
+// The most blinks of one selected block (block s's blinks are evOff[winOff[dyeStart[s]]] .. of s + 1): the longest
+// block decides the merge passes. One workgroup; the maximum is the same in any order.
+__global__ __launch_bounds__(1024) void longest_block_kernel(const int64_t* __restrict__ dyeStart,
+                                                             const int64_t* __restrict__ winOff,
+                                                             const int64_t* __restrict__ evOff, int64_t nSel,
+                                                             int64_t* __restrict__ out)
+{
+   __shared__ long long part[16];
+   long long m = 0;
+   for (int64_t s = threadIdx.x; s < nSel; s += 1024)
+   {
+      const long long n = evOff[winOff[dyeStart[s + 1]]] - evOff[winOff[dyeStart[s]]];
+      m = n > m ? n : m;
+   }
+   for (int d = 32; d > 0; d >>= 1)
+   {
+      const long long o = __shfl_xor(m, d, 64);
+      m = o > m ? o : m;
+   }
+   if ((threadIdx.x & 63) == 0)
+      part[threadIdx.x >> 6] = m;
+   __syncthreads();
+   if (threadIdx.x == 0)
+   {
+      long long r = 0;
+      for (int i = 0; i < 16; ++i)
+         r = part[i] > r ? part[i] : r;
+      *out = r;
+   }
+}
+
 using PersistWin = PersistGen<true, iscg_blink>;
 using PersistHistWin = PersistHistGen<true, iscg_blink>;
 
@@ -1962,17 +1993,16 @@ int32_t iscg_world_events(iscg_world* w, const double window[6], double t0, doub
                          w->dyeStart.as<int64_t>(), w->winOff.as<int64_t>(), w->evOff.as<int64_t>(),
                          w->blinks.as<iscg_blink>(), w->perm.as<int32_t>());
    }
-   // the longest block (host copies of the offsets it needs: per selected block, its event range)
-   std::vector<int64_t> dyeStart((size_t)nSel + 1), winOff((size_t)nDyes + 1), evOff((size_t)nWin + 1);
-   if (!wok(w, hipMemcpyAsync(dyeStart.data(), w->dyeStart.p, dyeStart.size() * 8, hipMemcpyDeviceToHost, st), "ev") ||
-       !wok(w, hipMemcpyAsync(winOff.data(), w->winOff.p, winOff.size() * 8, hipMemcpyDeviceToHost, st), "ev") ||
-       !wok(w, hipMemcpyAsync(evOff.data(), w->evOff.p, evOff.size() * 8, hipMemcpyDeviceToHost, st), "ev") ||
+   // the longest block, reduced on the device (one value back, not the offsets)
+   if (!reserve<int64_t>(w, w->total, 1, "iscg_world_events"))
+      return -1;
+   hipLaunchKernelGGL(longest_block_kernel, dim3(1), dim3(1024), 0, st, w->dyeStart.as<int64_t>(),
+                      w->winOff.as<int64_t>(), w->evOff.as<int64_t>(), nSel, w->total.as<int64_t>());
+   int64_t longest = 0;
+   if (!wok(w, hipGetLastError(), "iscg_world_events") ||
+       !wok(w, hipMemcpyAsync(&longest, w->total.p, sizeof longest, hipMemcpyDeviceToHost, st), "iscg_world_events") ||
        !wok(w, hipStreamSynchronize(st), "iscg_world_events"))
       return -1;
-   int64_t longest = 0;
-   for (int64_t s = 0; s < nSel; ++s)
-      longest = std::max(longest, evOff[(size_t)winOff[(size_t)dyeStart[(size_t)s + 1]]] -
-                                     evOff[(size_t)winOff[(size_t)dyeStart[(size_t)s]]]);
    int32_t* cur = w->perm.as<int32_t>();
    if (longest > kSortMax)
    {
