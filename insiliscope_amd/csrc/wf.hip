@@ -174,20 +174,13 @@ __global__ __launch_bounds__(zconv_cols<N>() * N / 8) void zconv_kernel(const fl
 
 // ---- fused passes B + C + D (NY = 512, NZ = 128) -------------------------------------------------------------
 //
-// One workgroup per column kx of the half spectrum does the whole (y, z) work of that column: its slab never
-// leaves the chip. The column-major passes A and E around it make every global access a contiguous segment: in
-// [kx][z][y < ny], Ghat [kx][ky][kz], out [kx][f][rows]. 1024 lanes; w = tid >> 6 (wave), lane = tid & 63.
+// One workgroup per column kx of the half spectrum does the whole (y, z) work of that column: its 64 x 512 slab
+// never leaves the chip, so the column's input rows are read once and its output rows written once. The
+// column-major passes A and E around it make every global access a contiguous segment: in [kx][z][y < ny], Ghat
+// [kx][ky][kz], out [kx][f][rows]. 1024 lanes; w = tid >> 6 (wave), lane = tid & 63.
 // The y transform is split by the parity h of ky (decimation in frequency): with x' = (x[y] + (-1)^h x[y + 256])
 // e^{-2 pi i h y / 512}, Y[2m + h] = FFT256(x')[m], and the inverse is out[y] = e[y mod 256] + e^{2 pi i y / 512}
-// o[y mod 256] with e, o the 256-point inverses of the even and odd halves. Each half is a 64 x 256 slab held in
-// registers, R[j][i] = S_h[z = (lane & 15) + 16 i][m = m(j, 4 w + (lane >> 4))], 16 values per lane:
-//   1. 2 batches of 32 z lines: two lines per wave (32 lanes each) transformed along y on their own (loads straight
-//      into the first radix-8 stage), then one workgroup-wide transpose through LDS fills R[.][i];
-//   2. 4 chunks of 64 m lines, 4 lines per wave, 16 lanes per line: FFT along z (first stage from registers, the
-//      upper 64 planes zero), times Ghat (loaded before the transform), inverse, the first 64 planes back to R - all
-//      inside the wave, no workgroup barrier;
-//   3. 2 batches of 32 foci: transpose R into LDS, two foci per wave transformed back along y, rows
-//      [row0, row0 + rows) stored from registers (h = 0) or added to (h = 1, the same lane wrote them).
+// o[y mod 256] with e, o the 256-point inverses of the even and odd halves. Both halves stay in registers (slab_body).
 // Within a wave the lanes run in lockstep and its LDS operations complete in order, so a line that one wave owns
 // needs only a compiler fence (wave_sync) between the read and the write phases of a stage.
 // Columns 0 and NX/2 are real in (y, z) (Hermitian-symmetric rows of the x transform), so the rows passes carry them
@@ -352,253 +345,313 @@ __device__ __forceinline__ int slab_m(bool packed, int h, int j, int s)
    return mp == 0 ? 128 : 256 - mp;
 }
 
-// The work of one workgroup (PACKED: columns 0 and NH, packed as column 0). Two bodies, so that the packed one's
-// extra exchanges do not raise the register allocation of the others.
-template <bool PACKED>
+// One line of the z convolution (16 lanes, q = lane & 15): its 64 planes x[i] at z = q + 16 i in, FFT along z (the
+// upper 64 planes zero), times the kernel spectrum gl[kz] (the line's 128 values, staged in LDS), inverse, the first
+// 64 planes back to x. after_g runs once the line's spectrum has been read from gl. Z: the line's LDS scratch,
+// gs: the self-paired lines' spectrum slot (packed workgroup only).
+template <bool PACKED, typename F>
+__device__ __forceinline__ void zconv_line(float2 (&x)[4], int ky, const float2* gl, F&& after_g, float2* Z,
+                                           float2* gs, int lane, const float2* twz)
+{
+   constexpr int NY = 512, NZ = 128, NI = 4;
+   constexpr bool packed = PACKED;
+   const int q = lane & 15;
+   // the packed workgroup's pairs (ky, -ky < 0) sit in neighbouring 16-lane rows; the -ky row transforms the
+   // conjugate of its line, so that lane (q, m) of both rows holds U(ky, kz) and W(kz) = conj U(-ky, -kz)
+   const bool self = packed && (ky == 0 || ky == NY / 2), minus = packed && !self && ky > NY / 2;
+   float2 v[8];
+#pragma unroll
+   for (int r = 0; r < 8; ++r)
+   {
+      v[r] = r < NI ? x[r < NI ? r : 0] : make_float2(0.0f, 0.0f);
+      if (minus)
+         v[r].y = -v[r].y;
+   }
+   line_fft128<false, false>(Z, q, v, twz);
+   // v[m] = U at kz = q + 16 m; the spectrum's loads into gl have landed once the wave's memory count is zero
+   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+   float2 g[8];
+#pragma unroll
+   for (int m = 0; m < 8; ++m)
+      g[m] = gl[q + 16 * m];
+   // the reads have returned before the next chunk's spectrum may land in the same slots
+   asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+   after_g();
+   if constexpr (packed)
+   {
+      // V(k) = U(k) P(k) + conj(U(-k)) Q(k) with P, Q Hermitian, stored P(ky, kz) on the line ky and Q(ky, kz)
+      // on the line -ky, both at kz. With W = conj U(-ky, -kz) from the conjugated transform of the -ky row:
+      //   row ky:   V(ky, kz) = U P + W Q;   row -ky: V(-ky, -kz) = conj(W P + U Q),
+      // so both rows form X = own P + other Q from the partner row's values at the same lane (v_permlane16_swap
+      // trades the rows of a pair), and the -ky row's inverse of X, conjugated, is its line. The self-paired
+      // lines ky = 0 and 256 (two rows of one wave) take U(-k) and their P, Q through LDS instead: P = G(k),
+      // Q = G(-k) for kz < 64, P = conj G(-k), Q = conj G(k) above, the real values at kz = 0 and 64 paired in
+      // slots 0 (P) and 64 (Q).
+      const bool oddRow = (lane >> 4) & 1;
+      auto swp = [&](float a) {
+         const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(a), false, false);
+         return __uint_as_float(oddRow ? r[0] : r[1]);
+      };
+      if (self)
+      {
+#pragma unroll
+         for (int m = 0; m < 8; ++m)
+         {
+            Z[sp(q + 16 * m)] = v[m];
+            gs[q + 16 * m] = g[m];
+         }
+      }
+      wave_sync();
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+      {
+         const float2 ov = make_float2(swp(v[m].x), swp(v[m].y));
+         const float2 og = make_float2(swp(g[m].x), swp(g[m].y));
+         const float2 P = minus ? og : g[m], Q = minus ? g[m] : og;
+         float2 y = cmaf(ov, Q, cmulf(v[m], P));
+         if (self)
+         {
+            const int kz = q + 16 * m, nkz = (NZ - kz) & (NZ - 1);
+            const float2 up = Z[sp(nkz)];
+            const bool low = m < 4, real = (m == 0 || m == 4) && q == 0;
+            float2 P2 = gs[real ? 0 : (low ? kz : nkz)], Q2 = gs[real ? NZ / 2 : (low ? nkz : kz)];
+            if (!low && !real)
+            {
+               P2.y = -P2.y;
+               Q2.y = -Q2.y;
+            }
+            if (real)
+            {
+               P2 = make_float2(m == 0 ? P2.x : P2.y, 0.0f);
+               Q2 = make_float2(m == 0 ? Q2.x : Q2.y, 0.0f);
+            }
+            y = cmaf(make_float2(up.x, -up.y), Q2, cmulf(v[m], P2));
+         }
+         v[m] = y;
+      }
+   }
+   else
+   {
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+         v[m] = cmulf(v[m], g[m]);
+   }
+   wave_sync();
+   line_fft128<true, true>(Z, q, v, twz);
+#pragma unroll
+   for (int i = 0; i < NI; ++i)
+      x[i] = minus ? make_float2(v[i].x, -v[i].y) : v[i];
+   wave_sync();
+}
+
+// The whole slab at once (both parities): the 64 x 512 slab of a column stays in registers, R[j][i] = S_h[z =
+// (lane & 15) + 16 i][m] for line g = CL j + 4 w + (lane >> 4) of the 512 (CL = 4 x waves lines per chunk; parity
+// h = g / 256, m = slab_m's line g % 64 of the parity's 64-line chunk (g % 256) / 64). A wave transforms both
+// halves of a z line (lanes 0..31 the even half, 32..63 the odd, twisted one) from a single read of the line, and
+// both halves of a focus, which it combines into the output rows: the input column is read once and every output row
+// written once.
+//   1. 4 passes of 16 z lines along y, each followed by one workgroup-wide transpose into R[.][p];
+//   2. the chunks along z (zconv_line), 4 lines per wave, the next chunk's kernel spectrum in flight into LDS;
+//   3. 4 passes of 16 foci: transpose R[.][p] into LDS, the foci back along y (both halves), the rows
+//      [row0, row0 + rows) = e[y mod 256] + e^{2 pi i y / 512} o[y mod 256] stored once.
+template <int THREADS, bool PACKED>
 __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny, int nzIn, size_t inCol,
-                                          float2* __restrict__ out, int row0, int rows, int nOut, size_t outCol,
-                                          const float2* __restrict__ ghat, int NH, float2* buf, float2* gx,
-                                          const float2* twy, const float2* twz)
+                                               float2* __restrict__ out, int row0, int rows, int nOut, size_t outCol,
+                                               const float2* __restrict__ ghat, float2* buf, float2* gx,
+                                               float2* gst, const float2* twy, const float2* twz)
 {
    constexpr int NY = 512, NZ = 128;
-   constexpr int NJ = 4, NI = 4;  // chunks of 64 lines, batches of 16 planes (64 held)
-   constexpr int LSY = 273, LSZ = 137;  // odd strides >= sp(256), sp(128)
+   constexpr int WAVES = THREADS / 64, LPW = 16 / WAVES;  // z lines (foci) per wave and pass
+   constexpr int CL = 4 * WAVES, NJ = 512 / CL, NI = 4;   // lines per chunk, chunks, passes of 16 planes
+   constexpr int LSY = 273, LSZ = 137;                    // odd strides >= sp(256), sp(128)
    constexpr bool packed = PACKED;
    const int tid0 = threadIdx.x;
    const int c = packed ? 0 : (int)blockIdx.x;
    const float2* src = in + (size_t)c * inCol;
    const float2* G = ghat + (size_t)c * NY * NZ;
    float2* dst = out + (size_t)c * outCol;
-// the lane ids, opaque to the compiler at each batch and chunk so that it recomputes addresses there instead of
-// keeping every unrolled iteration's addresses live (which would spill the register slab)
+// the lane ids, opaque to the compiler at each pass and chunk so that it recomputes addresses there instead of
+// keeping every unrolled iteration's addresses live
 #define SLAB_IDS                   \
    int tid = tid0;                 \
    asm volatile("" : "+v"(tid));   \
    const int lane = tid & 63, w = tid >> 6
-
-#pragma unroll 1
-   for (int h = 0; h < 2; ++h)
-   {
-      float2 R[NJ][NI];
-      // 1. y forward, z lines 32 b + 2 w + (lane >> 5); the second batch's rows are loaded before the first
-      // batch's transform, so their latency overlaps it
-      float2 xb[2][8];
+   // chunk j's line at position s: its parity and half-slab index
+   auto mline = [](int j, int s) {
+      const int g = CL * j + s;
+      return slab_m(packed, g >> 8, (g >> 6) & 3, g & 63);
+   };
+   auto kyof = [&](int j, int s) { return 2 * mline(j, s) + ((CL * j) >> 8); };
+   // the line zl of pass p: z = 16 p + zl, element y = (lane & 31) + 32 r (ny <= 256 on this path)
+   auto load_line = [&](int p, int zl, int lane, float2 (&xv)[8]) {
+      const int z = 16 * p + zl, q = lane & 31;
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int r = 0; r < 8; ++r)
       {
-         SLAB_IDS;
-         const int z = 32 * b + 2 * w + (lane >> 5), q = lane & 31;
-#pragma unroll
-         for (int r = 0; r < 8; ++r)
-         {
-            const int y = q + 32 * r;  // < 256: ny <= 256 on this path
-            xb[b][r] = make_float2(0.0f, 0.0f);
-            if (z < nzIn && y < ny)
-               xb[b][r] = src[(size_t)z * ny + y];
-         }
+         const int y = q + 32 * r;
+         xv[r] = make_float2(0.0f, 0.0f);
+         if (z < nzIn && y < ny)
+            xv[r] = src[(size_t)z * ny + y];
       }
+   };
+   // line (zl, h) of a pass in LDS: zl + 16 h
+   auto yline = [&](int zl, int h) { return buf + (zl + 16 * h) * LSY; };
+   // the kernel spectrum of chunk j's lines 4 w .. 4 w + 3 (1 KB each, contiguous in HBM) straight into the wave's
+   // slots of gst, one 16-byte piece per lane and line, no registers held; read after s_waitcnt vmcnt(0)
+   auto stage_g = [&](int j, int w, int lane) {
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int i = 0; i < 4; ++i)
       {
-         SLAB_IDS;
-         if (32 * b >= nzIn)
-         {
+         const float4* s4 = reinterpret_cast<const float4*>(G + (size_t)kyof(j, 4 * w + i) * NZ) + lane;
+         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)s4,
+                                          (__attribute__((address_space(3))) void*)(gst + (4 * w + i) * NZ), 16, 0,
+                                          0);
+      }
+   };
+
+   float2 R[NJ][NI];
+   // 1. y forward; the next pass's lines are loaded before this pass's transforms, chunk 0's spectrum before all
+   float2 xn[LPW][8];
+   {
+      SLAB_IDS;
+      stage_g(0, w, lane);
 #pragma unroll
-            for (int j = 0; j < NJ; ++j)
-               R[j][2 * b] = R[j][2 * b + 1] = make_float2(0.0f, 0.0f);
-            continue;
-         }
-         const int zl = 2 * w + (lane >> 5), q = lane & 31;
-         float2 v[8];
+      for (int t = 0; t < LPW; ++t)
+         load_line(0, LPW * w + t, lane, xn[t]);
+   }
+#pragma unroll
+   for (int p = 0; p < NI; ++p)
+   {
+      SLAB_IDS;
+      float2 v[LPW][8];
+#pragma unroll
+      for (int t = 0; t < LPW; ++t)
 #pragma unroll
          for (int r = 0; r < 8; ++r)
-            v[r] = h ? cmulf(xb[b][r], twy[q + 32 * r]) : xb[b][r];
-         line_fft256<false, true, false>(buf + zl * LSY, q, v, twy);
-         __syncthreads();
+            v[t][r] = xn[t][r];
+      if (p + 1 < NI)
+      {
+#pragma unroll
+         for (int t = 0; t < LPW; ++t)
+            load_line(p + 1, LPW * w + t, lane, xn[t]);
+      }
+      if (16 * p >= nzIn)
+      {
 #pragma unroll
          for (int j = 0; j < NJ; ++j)
-#pragma unroll
-            for (int t = 0; t < 2; ++t)
-               R[j][2 * b + t] = buf[((lane & 15) + 16 * t) * LSY + sp(slab_m(packed, h, j, 4 * w + (lane >> 4)))];
-         __syncthreads();
+            R[j][p] = make_float2(0.0f, 0.0f);
+         continue;
       }
-
-      // 2. z convolution, lines 4 w .. 4 w + 3 of each chunk per wave
+      const int h = lane >> 5, q = lane & 31;
 #pragma unroll
-      for (int j = 0; j < NJ; ++j)
+      for (int t = 0; t < LPW; ++t)
       {
-         SLAB_IDS;
-         const int pos = 4 * w + (lane >> 4), q = lane & 15;
-         const int ky = 2 * slab_m(packed, h, j, pos) + h;
-         float2* Z = buf + pos * LSZ;
-         // the spectrum, in flight during the forward transform: in registers, or (the packed workgroup, whose
-         // partner lines exchange it) the wave's 4 lines straight into LDS (one 1 KB line per load, lane i's 16 bytes
-         // at 16 i), holding no registers
-         float2 g[8];
-#pragma unroll
-         for (int m = 0; m < 8; ++m)
-            g[m] = G[(size_t)ky * NZ + q + 16 * m];
-         // the packed workgroup's pairs (ky, -ky < 0) sit in neighbouring 16-lane rows; the -ky row transforms the
-         // conjugate of its line, so that lane (q, m) of both rows holds U(ky, kz) and W(kz) = conj U(-ky, -kz)
-         const bool self = packed && (ky == 0 || ky == NY / 2), minus = packed && !self && ky > NY / 2;
-         float2 v[8];
 #pragma unroll
          for (int r = 0; r < 8; ++r)
-         {
-            v[r] = r < NI ? R[j][r < NI ? r : 0] : make_float2(0.0f, 0.0f);
-            if (minus)
-               v[r].y = -v[r].y;
-         }
-         line_fft128<false, false>(Z, q, v, twz);
-         // v[m] = U at kz = q + 16 m
-         if constexpr (packed)
-         {
-            // V(k) = U(k) P(k) + conj(U(-k)) Q(k) with P, Q Hermitian, stored P(ky, kz) on the line ky and Q(ky, kz)
-            // on the line -ky, both at kz. With W = conj U(-ky, -kz) from the conjugated transform of the -ky row:
-            //   row ky:   V(ky, kz) = U P + W Q;   row -ky: V(-ky, -kz) = conj(W P + U Q),
-            // so both rows form X = own P + other Q from the partner row's values at the same lane (v_permlane16_swap
-            // trades the rows of a pair), and the -ky row's inverse of X, conjugated, is its line. The self-paired
-            // lines ky = 0 and 256 (two rows of one wave) take U(-k) and their P, Q through LDS instead: P = G(k),
-            // Q = G(-k) for kz < 64, P = conj G(-k), Q = conj G(k) above, the real values at kz = 0 and 64 paired in
-            // slots 0 (P) and 64 (Q).
-            const bool oddRow = (lane >> 4) & 1;
-            auto swp = [&](float x) {
-               const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
-               return __uint_as_float(oddRow ? r[0] : r[1]);
-            };
-            float2* gs = gx + pos * kGLine;
-            if (self)
-            {
-#pragma unroll
-               for (int m = 0; m < 8; ++m)
-               {
-                  Z[sp(q + 16 * m)] = v[m];
-                  gs[q + 16 * m] = g[m];
-               }
-            }
-            wave_sync();
-#pragma unroll
-            for (int m = 0; m < 8; ++m)
-            {
-               const float2 ov = make_float2(swp(v[m].x), swp(v[m].y));
-               const float2 og = make_float2(swp(g[m].x), swp(g[m].y));
-               const float2 P = minus ? og : g[m], Q = minus ? g[m] : og;
-               float2 x = cmaf(ov, Q, cmulf(v[m], P));
-               if (self)
-               {
-                  const int kz = q + 16 * m, nkz = (NZ - kz) & (NZ - 1);
-                  const float2 up = Z[sp(nkz)];
-                  const bool low = m < 4, real = (m == 0 || m == 4) && q == 0;
-                  float2 P2 = gs[real ? 0 : (low ? kz : nkz)], Q2 = gs[real ? NZ / 2 : (low ? nkz : kz)];
-                  if (!low && !real)
-                  {
-                     P2.y = -P2.y;
-                     Q2.y = -Q2.y;
-                  }
-                  if (real)
-                  {
-                     P2 = make_float2(m == 0 ? P2.x : P2.y, 0.0f);
-                     Q2 = make_float2(m == 0 ? Q2.x : Q2.y, 0.0f);
-                  }
-                  x = cmaf(make_float2(up.x, -up.y), Q2, cmulf(v[m], P2));
-               }
-               v[m] = x;
-            }
-         }
-         else
-         {
-#pragma unroll
-            for (int m = 0; m < 8; ++m)
-               v[m] = cmulf(v[m], g[m]);
-         }
-         wave_sync();
-         line_fft128<true, true>(Z, q, v, twz);
-#pragma unroll
-         for (int i = 0; i < NI; ++i)
-            R[j][i] = minus ? make_float2(v[i].x, -v[i].y) : v[i];
-         wave_sync();
+            v[t][r] = cmulf(v[t][r], h ? twy[q + 32 * r] : make_float2(1.0f, 0.0f));
+         line_fft256<false, true, false>(yline(LPW * w + t, h), q, v[t], twy);
       }
       __syncthreads();
-
-      // 3. y inverse, foci 32 b + 2 w + (lane >> 5); rows [row0, row0 + rows)
-#pragma unroll
-      for (int b = 0; b < 2; ++b)
       {
-         SLAB_IDS;
-         if (32 * b >= nOut)
-            break;
+         const int s = 4 * w + (lane >> 4), zl = lane & 15;
 #pragma unroll
          for (int j = 0; j < NJ; ++j)
+            R[j][p] = yline(zl, (CL * j) >> 8)[sp(mline(j, s))];
+      }
+      __syncthreads();
+   }
+
+   // 2. z convolution, lines 4 w .. 4 w + 3 of each chunk per wave; the chunk's kernel spectrum is in gst
 #pragma unroll
-            for (int t = 0; t < 2; ++t)
-               buf[((lane & 15) + 16 * t) * LSY + sp(slab_m(packed, h, j, 4 * w + (lane >> 4)))] = R[j][2 * b + t];
-         __syncthreads();
-         const int fl = 2 * w + (lane >> 5), q = lane & 31, f = 32 * b + fl;
-         float2* d = dst + (size_t)f * rows;
-         // h = 1 adds to the rows this lane stored for h = 0: their loads overlap the transform (the first 256)
-         constexpr int KP = 8;
-         float2 prev[KP];
+   for (int j = 0; j < NJ; ++j)
+   {
+      SLAB_IDS;
+      const int pos = 4 * w + (lane >> 4);
+      zconv_line<packed>(
+          R[j], kyof(j, pos), gst + pos * NZ,
+          [&] {
+             if (j + 1 < NJ)
+                stage_g(j + 1, w, lane);
+          },
+          buf + pos * LSZ, gx + pos * kGLine, lane, twz);
+      // the chunk's values are formed here: without this the compiler sinks their last butterfly into the
+      // (conditional) passes of stage 3 and keeps its operands, three times the registers, live until then
 #pragma unroll
-         for (int k = 0; k < KP; ++k)
-         {
-            const int yo = q + 32 * k;
-            prev[k] = make_float2(0.0f, 0.0f);
-            if (h && f < nOut && yo < rows)
-               prev[k] = d[yo];
-         }
+      for (int i = 0; i < NI; ++i)
+         asm volatile("" : "+v"(R[j][i].x), "+v"(R[j][i].y));
+   }
+   __syncthreads();
+
+   // 3. y inverse, foci 16 p + LPW w + t per wave; rows [row0, row0 + rows)
+#pragma unroll
+   for (int p = 0; p < NI; ++p)
+   {
+      SLAB_IDS;
+      if (16 * p >= nOut)
+         break;
+      {
+         const int s = 4 * w + (lane >> 4), fl = lane & 15;
+#pragma unroll
+         for (int j = 0; j < NJ; ++j)
+            yline(fl, (CL * j) >> 8)[sp(mline(j, s))] = R[j][p];
+      }
+      __syncthreads();
+      const int h = lane >> 5, q = lane & 31;
+#pragma unroll
+      for (int t = 0; t < LPW; ++t)
+      {
          float2 v[8];
-         float2* Lf = buf + fl * LSY;
-         line_fft256<true, false, false>(Lf, q, v, twy);
-         wave_sync();
+         line_fft256<true, false, false>(yline(LPW * w + t, h), q, v, twy);
+      }
+      wave_sync();
+#pragma unroll
+      for (int t = 0; t < LPW; ++t)
+      {
+         const int f = 16 * p + LPW * w + t;
          if (f < nOut)
          {
-            // output row y = row0 + yo takes element y mod 256 of this half (twisted by e^{2 pi i y / 512} for h = 1)
-            auto put = [&](int yo, float2 p) {
-               const int y = row0 + yo;
-               float2 val = Lf[sp(y & 255)];
-               if (h)
-               {
-                  const float2 tw = twy[y];
-                  val = cmaf(val, make_float2(tw.x, -tw.y), p);
-               }
-               d[yo] = val;
-            };
-#pragma unroll
-            for (int k = 0; k < KP; ++k)
-               if (q + 32 * k < rows)
-                  put(q + 32 * k, prev[k]);
-#pragma unroll 1
-            for (int yo = q + 32 * KP; yo < rows; yo += 32)
-               put(yo, h ? d[yo] : make_float2(0.0f, 0.0f));
+            float2* d = dst + (size_t)f * rows;
+            const float2* E = yline(LPW * w + t, 0);
+            const float2* O = yline(LPW * w + t, 1);
+#pragma unroll 4
+            for (int yo = lane; yo < rows; yo += 64)
+            {
+               const int y = row0 + yo, m = y & 255;
+               const float2 tw = twy[y];
+               d[yo] = cmaf(O[sp(m)], make_float2(tw.x, -tw.y), E[sp(m)]);
+            }
          }
-         __syncthreads();
       }
+      __syncthreads();
    }
 #undef SLAB_IDS
 }
 
-template <int NY, int NZ>
-__global__ __launch_bounds__(1024) void slab_kernel(const float2* __restrict__ in, int ny, int nzIn, size_t inCol,
-                                                    float2* __restrict__ out, int row0, int rows, int nOut,
-                                                    size_t outCol, const float2* __restrict__ ghat, int NH,
-                                                    const float2* __restrict__ twY, const float2* __restrict__ twZ)
+template <int NY, int NZ, int THREADS>
+__global__ __launch_bounds__(THREADS) void slab_kernel(const float2* __restrict__ in, int ny, int nzIn,
+                                                            size_t inCol, float2* __restrict__ out, int row0,
+                                                            int rows, int nOut, size_t outCol,
+                                                            const float2* __restrict__ ghat,
+                                                            const float2* __restrict__ twY,
+                                                            const float2* __restrict__ twZ)
 {
    static_assert(NY == 512 && NZ == 128, "the slab pass is written for a 512 x 128 (y, z) slab");
-   constexpr int LSY = 273, LSZ = 137;
-   constexpr int BUF = 32 * LSY > 64 * LSZ ? 32 * LSY : 64 * LSZ;
+   constexpr int LSY = 273, LSZ = 137, CL = THREADS / 16;
+   constexpr int BUF = 32 * LSY > CL * LSZ ? 32 * LSY : CL * LSZ;
    __shared__ float2 buf[BUF];
    __shared__ float2 gx[4 * kGLine];  // the self-paired lines' spectrum values (positions 0 and 1 of the packed workgroup)
+   __shared__ __attribute__((aligned(16))) float2 gst[CL * NZ];  // one chunk's kernel spectrum: CL lines of 1 KB
    __shared__ float2 twy[NY];
    __shared__ float2 twz[NZ];
-   for (int i = threadIdx.x; i < NY; i += 1024)
+   for (int i = threadIdx.x; i < NY; i += THREADS)
       twy[i] = twY[i];
-   for (int i = threadIdx.x; i < NZ; i += 1024)
+   for (int i = threadIdx.x; i < NZ; i += THREADS)
       twz[i] = twZ[i];
    __syncthreads();
    if (blockIdx.x == 0)
-      slab_body<true>(in, ny, nzIn, inCol, out, row0, rows, nOut, outCol, ghat, NH, buf, gx, twy, twz);
+      slab_body<THREADS, true>(in, ny, nzIn, inCol, out, row0, rows, nOut, outCol, ghat, buf, gx, gst, twy, twz);
    else
-      slab_body<false>(in, ny, nzIn, inCol, out, row0, rows, nOut, outCol, ghat, NH, buf, gx, twy, twz);
+      slab_body<THREADS, false>(in, ny, nzIn, inCol, out, row0, rows, nOut, outCol, ghat, buf, gx, gst, twy, twz);
 }
 
 // The kernel spectrum for the slab pass: out[c][ky][kz] = ghat[kz][ky][c] for columns 1 .. NH - 1; column 0 holds
@@ -872,10 +925,9 @@ int32_t iscg_wf_conv3d(iscg_context* h, const float* density, const iscg_camera*
       }
       {
          ProfScope ps(h, "wf_slab");
-         hipLaunchKernelGGL((slab_kernel<512, 128>), dim3(d.NX / 2), dim3(1024), 0, h->stream,
+         hipLaunchKernelGGL((slab_kernel<512, 128, 1024>), dim3(d.NX / 2), dim3(1024), 0, h->stream,
                             (const float2*)x1, (int)d.ny, (int)d.nzD, s.col1, x2, (int)d.fovY0, (int)d.ch, (int)d.nF,
-                            s.col2, s.ghat.as<const float2>(), (int)(d.NX / 2), s.twY.as<const float2>(),
-                            s.twZ.as<const float2>());
+                            s.col2, s.ghat.as<const float2>(), s.twY.as<const float2>(), s.twZ.as<const float2>());
       }
       {
          ProfScope ps(h, "wf_rows_c2r_noise");
