@@ -184,15 +184,12 @@ __global__ __launch_bounds__(zconv_cols<N>() * N / 8) void zconv_kernel(const fl
 // Within a wave the lanes run in lockstep and its LDS operations complete in order, so a line that one wave owns
 // needs only a compiler fence (wave_sync) between the read and the write phases of a stage.
 // Columns 0 and NX/2 are real in (y, z) (Hermitian-symmetric rows of the x transform), so the rows passes carry them
-// as one column u = a + ib (column 0) and one workgroup takes it: after the (y, z) transform U(k) = A(k) + iB(k) with
-// A(-k) = conj A(k), B likewise, so
-//   A Ghat0 + i B GhatN = U(k) P(k) + conj(U(-k)) Q(k),  P = (Ghat0 + GhatN) / 2,  Q = (Ghat0 - GhatN) / 2,
-// and the inverse gives a' + ib'. ky and -ky have the same parity; the packed workgroup's chunk j holds
-// m = 32 j + p at even positions 2p and the m of -ky at odd ones (128, i.e. ky = 256, in place of the duplicate 0),
-// so k and -k sit on neighbouring lines of one wave. P and Q are Hermitian too, so column 0 of the spectrum holds
-// one value per k, as the others do: P(kz, ky) on the lines 0 < ky < 256, Q(kz, ky) on the line -ky, and
-// the self-paired lines ky = 0 and 256 hold P(kz) for 0 < kz < 64 and Q(kz) at 128 - kz, with the real values at
-// kz = 0 and 64 paired in slots 0 (P) and 64 (Q). This leaves NX/2 workgroups: one per CU at NX = 512.
+// as one column u = a + ib (column 0) and one workgroup takes it. After its y transform, U(ky) = A(ky) + iB(ky) with
+// A(-ky) = conj A(ky), B likewise, so the pair (ky, -ky) gives A(ky) and B(ky), which it holds on the lines ky and
+// -ky (pair_columns): each line is then an ordinary z convolution, with Ghat0(ky) on the line ky and GhatN(ky) on
+// the line -ky, and the inverse y transform takes U'(ky) = A' + iB', U'(-ky) = conj A' + i conj B' back to a' + ib'.
+// The lines ky = 0 and 256 pair with themselves: A and B are real there, and their z convolution takes U(-kz)
+// (zconv_line). This leaves NX/2 workgroups: one per CU at NX = 512.
 
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 
@@ -332,18 +329,6 @@ __device__ __forceinline__ void line_fft128(float2* L, int q, float2 (&v)[8], co
 // A spectrum line's slot in LDS: 128 values and 8 more, so the 4 lines a wave holds start on different banks.
 constexpr int kGLine = 136;
 
-// The half-slab line m of chunk j at position s (0..63); the packed workgroup pairs ky with -ky (same parity h).
-__device__ __forceinline__ int slab_m(bool packed, int h, int j, int s)
-{
-   if (!packed)
-      return 64 * j + s;
-   const int mp = 32 * j + (s >> 1);
-   if (!(s & 1))
-      return mp;
-   if (h)
-      return 255 - mp;
-   return mp == 0 ? 128 : 256 - mp;
-}
 
 // One line of the z convolution (16 lanes, q = lane & 15): its 64 planes x[i] at z = q + 16 i in, FFT along z (the
 // upper 64 planes zero), times the kernel spectrum gl[kz] (the line's 128 values, staged in LDS), inverse, the first
@@ -354,19 +339,11 @@ __device__ __forceinline__ void zconv_line(float2 (&x)[4], int ky, const float2*
                                            float2* gs, int lane, const float2* twz)
 {
    constexpr int NY = 512, NZ = 128, NI = 4;
-   constexpr bool packed = PACKED;
    const int q = lane & 15;
-   // the packed workgroup's pairs (ky, -ky < 0) sit in neighbouring 16-lane rows; the -ky row transforms the
-   // conjugate of its line, so that lane (q, m) of both rows holds U(ky, kz) and W(kz) = conj U(-ky, -kz)
-   const bool self = packed && (ky == 0 || ky == NY / 2), minus = packed && !self && ky > NY / 2;
    float2 v[8];
 #pragma unroll
    for (int r = 0; r < 8; ++r)
-   {
       v[r] = r < NI ? x[r < NI ? r : 0] : make_float2(0.0f, 0.0f);
-      if (minus)
-         v[r].y = -v[r].y;
-   }
    line_fft128<false, false>(Z, q, v, twz);
    // v[m] = U at kz = q + 16 m; the spectrum's loads into gl have landed once the wave's memory count is zero
    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -377,21 +354,14 @@ __device__ __forceinline__ void zconv_line(float2 (&x)[4], int ky, const float2*
    // the reads have returned before the next chunk's spectrum may land in the same slots
    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
    after_g();
-   if constexpr (packed)
+   if constexpr (PACKED)
    {
-      // V(k) = U(k) P(k) + conj(U(-k)) Q(k) with P, Q Hermitian, stored P(ky, kz) on the line ky and Q(ky, kz)
-      // on the line -ky, both at kz. With W = conj U(-ky, -kz) from the conjugated transform of the -ky row:
-      //   row ky:   V(ky, kz) = U P + W Q;   row -ky: V(-ky, -kz) = conj(W P + U Q),
-      // so both rows form X = own P + other Q from the partner row's values at the same lane (v_permlane16_swap
-      // trades the rows of a pair), and the -ky row's inverse of X, conjugated, is its line. The self-paired
-      // lines ky = 0 and 256 (two rows of one wave) take U(-k) and their P, Q through LDS instead: P = G(k),
-      // Q = G(-k) for kz < 64, P = conj G(-k), Q = conj G(k) above, the real values at kz = 0 and 64 paired in
-      // slots 0 (P) and 64 (Q).
-      const bool oddRow = (lane >> 4) & 1;
-      auto swp = [&](float a) {
-         const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(a), false, false);
-         return __uint_as_float(oddRow ? r[0] : r[1]);
-      };
+      // The packed workgroup's lines ky = 0 and 256 hold U = A + iB of the two real columns (A, B real in z, so
+      // Hermitian in kz), which its unpacking leaves as they are: A Ghat0 + i B GhatN = U(k) P(k) + conj(U(-k)) Q(k)
+      // with P = (Ghat0 + GhatN) / 2, Q = (Ghat0 - GhatN) / 2, both Hermitian in kz; U(-k) comes through LDS, and the
+      // line's spectrum slots hold P(kz) for 0 < kz < 64 and Q(kz) at 128 - kz, the real values at kz = 0 and 64
+      // paired in slots 0 (P) and 64 (Q).
+      const bool self = ky == 0 || ky == NY / 2;
       if (self)
       {
 #pragma unroll
@@ -405,10 +375,7 @@ __device__ __forceinline__ void zconv_line(float2 (&x)[4], int ky, const float2*
 #pragma unroll
       for (int m = 0; m < 8; ++m)
       {
-         const float2 ov = make_float2(swp(v[m].x), swp(v[m].y));
-         const float2 og = make_float2(swp(g[m].x), swp(g[m].y));
-         const float2 P = minus ? og : g[m], Q = minus ? g[m] : og;
-         float2 y = cmaf(ov, Q, cmulf(v[m], P));
+         float2 y = cmulf(v[m], g[m]);
          if (self)
          {
             const int kz = q + 16 * m, nkz = (NZ - kz) & (NZ - 1);
@@ -440,13 +407,42 @@ __device__ __forceinline__ void zconv_line(float2 (&x)[4], int ky, const float2*
    line_fft128<true, true>(Z, q, v, twz);
 #pragma unroll
    for (int i = 0; i < NI; ++i)
-      x[i] = minus ? make_float2(v[i].x, -v[i].y) : v[i];
+      x[i] = v[i];
    wave_sync();
+}
+
+// The packed workgroup's half line h (lanes of one half of a wave, q = lane & 31) in LDS: the pairs (ky, -ky) of
+// parity h, half-line indices m and 256 - m (h = 0; ky = 0 and 256 pair with themselves) or 255 - m (h = 1).
+// UNPACK: U = A + iB of the real columns a (kx = 0) and b (kx = NX/2) -> A(ky) on the line ky, B(ky) on the line -ky,
+// A = (U(ky) + conj U(-ky)) / 2, B = (U(ky) - conj U(-ky)) / 2i; else the inverse, U(ky) = A + iB, U(-ky) =
+// conj A + i conj B. Each lane owns whole pairs.
+template <bool UNPACK>
+__device__ __forceinline__ void pair_columns(float2* L, int h, int q)
+{
+#pragma unroll
+   for (int k = 0; k < 4; ++k)
+   {
+      const int m = q + 32 * k;
+      if (h == 0 && m == 0)
+         continue;
+      const int mp = (h ? 255 : 256) - m;
+      const float2 o = L[sp(m)], p = L[sp(mp)];
+      if (UNPACK)
+      {
+         L[sp(m)] = make_float2(0.5f * (o.x + p.x), 0.5f * (o.y - p.y));
+         L[sp(mp)] = make_float2(0.5f * (o.y + p.y), 0.5f * (p.x - o.x));
+      }
+      else
+      {
+         L[sp(m)] = make_float2(o.x - p.y, o.y + p.x);
+         L[sp(mp)] = make_float2(o.x + p.y, p.x - o.y);
+      }
+   }
 }
 
 // The whole slab at once (both parities): the 64 x 512 slab of a column stays in registers, R[j][i] = S_h[z =
 // (lane & 15) + 16 i][m] for line g = CL j + 4 w + (lane >> 4) of the 512 (CL = 4 x waves lines per chunk; parity
-// h = g / 256, m = slab_m's line g % 64 of the parity's 64-line chunk (g % 256) / 64). A wave transforms both
+// h = g / 256, half-line index m = g % 256, ky = 2 m + h). A wave transforms both
 // halves of a z line (lanes 0..31 the even half, 32..63 the odd, twisted one) from a single read of the line, and
 // both halves of a focus, which it combines into the output rows: the input column is read once and every output row
 // written once.
@@ -476,11 +472,8 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
    int tid = tid0;                 \
    asm volatile("" : "+v"(tid));   \
    const int lane = tid & 63, w = tid >> 6
-   // chunk j's line at position s: its parity and half-slab index
-   auto mline = [](int j, int s) {
-      const int g = CL * j + s;
-      return slab_m(packed, g >> 8, (g >> 6) & 3, g & 63);
-   };
+   // chunk j's line at position s: line g = CL j + s of the slab, parity g / 256, half-line index g % 256
+   auto mline = [](int j, int s) { return (CL * j + s) & 255; };
    auto kyof = [&](int j, int s) { return 2 * mline(j, s) + ((CL * j) >> 8); };
    // the line zl of pass p: z = 16 p + zl, element y = (lane & 31) + 32 r (ny <= 256 on this path)
    auto load_line = [&](int p, int zl, int lane, float2 (&xv)[8]) {
@@ -550,6 +543,11 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
          for (int r = 0; r < 8; ++r)
             v[t][r] = cmulf(v[t][r], h ? twy[q + 32 * r] : make_float2(1.0f, 0.0f));
          line_fft256<false, true, false>(yline(LPW * w + t, h), q, v[t], twy);
+         if constexpr (packed)
+         {
+            wave_sync();
+            pair_columns<true>(yline(LPW * w + t, h), h, q);
+         }
       }
       __syncthreads();
       {
@@ -600,6 +598,11 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
 #pragma unroll
       for (int t = 0; t < LPW; ++t)
       {
+         if constexpr (packed)
+         {
+            pair_columns<false>(yline(LPW * w + t, h), h, q);
+            wave_sync();
+         }
          float2 v[8];
          line_fft256<true, false, false>(yline(LPW * w + t, h), q, v, twy);
       }
@@ -655,8 +658,10 @@ __global__ __launch_bounds__(THREADS) void slab_kernel(const float2* __restrict_
 }
 
 // The kernel spectrum for the slab pass: out[c][ky][kz] = ghat[kz][ky][c] for columns 1 .. NH - 1; column 0 holds
-// the packed workgroup's P = (Ghat0 + GhatNH) / 2 and Q = (Ghat0 - GhatNH) / 2, one value per k (P(kz, ky) on the
-// lines 0 < ky < NY/2, Q(kz, ky) on the line NY - ky, both folded into the self-paired lines ky = 0 and NY/2).
+// Ghat0(kz, ky) on the lines 0 < ky < NY/2 and GhatNH(kz, NY - ky) on the lines NY/2 < ky < NY (the packed
+// workgroup's A and B lines), and on the self-paired lines ky = 0 and NY/2 P = (Ghat0 + GhatNH) / 2 and
+// Q = (Ghat0 - GhatNH) / 2, one value per kz: P(kz) for 0 < kz < NZ/2, Q(NZ - kz) at kz > NZ/2, the real values at
+// kz = 0 and NZ/2 paired in slots 0 (P) and NZ/2 (Q).
 __global__ void ghat_columns_kernel(const float2* __restrict__ ghat, int NZ, int NY, int S, int NH,
                                     float2* __restrict__ out)
 {
@@ -670,31 +675,29 @@ __global__ void ghat_columns_kernel(const float2* __restrict__ ghat, int NZ, int
          out[idx] = ghat[((size_t)kz * NY + ky) * S + c];
          continue;
       }
-      // P(kz, ky) and Q(kz, ky) of column 0
-      auto P = [&](int z, int y) {
-         const float2* s = ghat + ((size_t)z * NY + y) * S;
+      if (ky != 0 && ky != NY / 2)
+      {
+         out[idx] = ky < NY / 2 ? ghat[((size_t)kz * NY + ky) * S] : ghat[((size_t)kz * NY + (NY - ky)) * S + NH];
+         continue;
+      }
+      // P(kz) and Q(kz) of the self-paired line ky
+      auto P = [&](int z) {
+         const float2* s = ghat + ((size_t)z * NY + ky) * S;
          return make_float2(0.5f * (s[0].x + s[NH].x), 0.5f * (s[0].y + s[NH].y));
       };
-      auto Q = [&](int z, int y) {
-         const float2* s = ghat + ((size_t)z * NY + y) * S;
+      auto Q = [&](int z) {
+         const float2* s = ghat + ((size_t)z * NY + ky) * S;
          return make_float2(0.5f * (s[0].x - s[NH].x), 0.5f * (s[0].y - s[NH].y));
       };
       float2 v;
-      if (ky == 0 || ky == NY / 2)
-      {
-         if (kz == 0)
-            v = make_float2(P(0, ky).x, P(NZ / 2, ky).x);
-         else if (kz == NZ / 2)
-            v = make_float2(Q(0, ky).x, Q(NZ / 2, ky).x);
-         else if (kz < NZ / 2)
-            v = P(kz, ky);
-         else
-            v = Q(NZ - kz, ky);
-      }
-      else if (ky < NY / 2)
-         v = P(kz, ky);
+      if (kz == 0)
+         v = make_float2(P(0).x, P(NZ / 2).x);
+      else if (kz == NZ / 2)
+         v = make_float2(Q(0).x, Q(NZ / 2).x);
+      else if (kz < NZ / 2)
+         v = P(kz);
       else
-         v = Q(kz, NY - ky);
+         v = Q(NZ - kz);
       out[idx] = v;
    }
 }
