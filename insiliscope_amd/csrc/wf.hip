@@ -186,7 +186,7 @@ __global__ __launch_bounds__(zconv_cols<N>() * N / 8) void zconv_kernel(const fl
 // Columns 0 and NX/2 are real in (y, z) (Hermitian-symmetric rows of the x transform), so the rows passes carry them
 // as one column u = a + ib (column 0) and one workgroup takes it. After its y transform, U(ky) = A(ky) + iB(ky) with
 // A(-ky) = conj A(ky), B likewise, so the pair (ky, -ky) gives A(ky) and B(ky), which it holds on the lines ky and
-// -ky (pair_columns): each line is then an ordinary z convolution, with Ghat0(ky) on the line ky and GhatN(ky) on
+// -ky (unpack_at): each line is then an ordinary z convolution, with Ghat0(ky) on the line ky and GhatN(ky) on
 // the line -ky, and the inverse y transform takes U'(ky) = A' + iB', U'(-ky) = conj A' + i conj B' back to a' + ib'.
 // The lines ky = 0 and 256 pair with themselves: A and B are real there, and their z convolution takes U(-kz)
 // (zconv_line). This leaves NX/2 workgroups: one per CU at NX = 512.
@@ -411,70 +411,76 @@ __device__ __forceinline__ void zconv_line(float2 (&x)[4], int ky, const float2*
    wave_sync();
 }
 
-// The packed workgroup's half line h (lanes of one half of a wave, q = lane & 31) in LDS: the pairs (ky, -ky) of
-// parity h, half-line indices m and 256 - m (h = 0; ky = 0 and 256 pair with themselves) or 255 - m (h = 1).
-// UNPACK: U = A + iB of the real columns a (kx = 0) and b (kx = NX/2) -> A(ky) on the line ky, B(ky) on the line -ky,
-// A = (U(ky) + conj U(-ky)) / 2, B = (U(ky) - conj U(-ky)) / 2i; else the inverse, U(ky) = A + iB, U(-ky) =
-// conj A + i conj B. Each lane owns whole pairs.
-template <bool UNPACK>
-__device__ __forceinline__ void pair_columns(float2* L, int h, int q)
+// The packed workgroup's columns a (kx = 0) and b (kx = NX/2), real in (y, z), travel as U = A + iB after the y
+// transform. Element m of half line L (parity h) pairs with mp = 256 - m (h = 0; m = 0 and 128, i.e. ky = 0 and
+// 256, pair with themselves) or 255 - m (h = 1); m < 128 is the line +ky, the other the line -ky.
+// unpack_at: A(ky) = (U(ky) + conj U(-ky)) / 2 on the line +ky, B(ky) = (U(ky) - conj U(-ky)) / 2i on the line -ky.
+__device__ __forceinline__ float2 unpack_at(const float2* L, int h, int m)
 {
-#pragma unroll
-   for (int k = 0; k < 4; ++k)
-   {
-      const int m = q + 32 * k;
-      if (h == 0 && m == 0)
-         continue;
-      const int mp = (h ? 255 : 256) - m;
-      const float2 o = L[sp(m)], p = L[sp(mp)];
-      if (UNPACK)
-      {
-         L[sp(m)] = make_float2(0.5f * (o.x + p.x), 0.5f * (o.y - p.y));
-         L[sp(mp)] = make_float2(0.5f * (o.y + p.y), 0.5f * (p.x - o.x));
-      }
-      else
-      {
-         L[sp(m)] = make_float2(o.x - p.y, o.y + p.x);
-         L[sp(mp)] = make_float2(o.x + p.y, p.x - o.y);
-      }
-   }
+   const bool self = h == 0 && (m & 127) == 0;
+   const float2 o = L[sp(m)], p = L[sp(self ? m : (h ? 255 : 256) - m)];
+   if (self)
+      return o;
+   return m < 128 ? make_float2(0.5f * (o.x + p.x), 0.5f * (o.y - p.y))
+                  : make_float2(0.5f * (p.y + o.y), 0.5f * (o.x - p.x));
+}
+// repack_at: the inverse, U(ky) = A + iB on the line +ky, U(-ky) = conj A + i conj B on the line -ky.
+__device__ __forceinline__ float2 repack_at(const float2* L, int h, int m)
+{
+   const bool self = h == 0 && (m & 127) == 0;
+   const float2 o = L[sp(m)], p = L[sp(self ? m : (h ? 255 : 256) - m)];
+   if (self)
+      return o;
+   return m < 128 ? make_float2(o.x - p.y, o.y + p.x) : make_float2(p.x + o.y, o.x - p.y);
 }
 
 // The whole slab at once (both parities): the 64 x 512 slab of a column stays in registers, R[j][i] = S_h[z =
-// (lane & 15) + 16 i][m] for line g = CL j + 4 w + (lane >> 4) of the 512 (CL = 4 x waves lines per chunk; parity
-// h = g / 256, half-line index m = g % 256, ky = 2 m + h). A wave transforms both
-// halves of a z line (lanes 0..31 the even half, 32..63 the odd, twisted one) from a single read of the line, and
-// both halves of a focus, which it combines into the output rows: the input column is read once and every output row
-// written once.
-//   1. 4 passes of 16 z lines along y, each followed by one workgroup-wide transpose into R[.][p];
-//   2. the chunks along z (zconv_line), 4 lines per wave, the next chunk's kernel spectrum in flight into LDS;
-//   3. 4 passes of 16 foci: transpose R[.][p] into LDS, the foci back along y (both halves), the rows
-//      [row0, row0 + rows) = e[y mod 256] + e^{2 pi i y / 512} o[y mod 256] stored once.
-template <int THREADS, bool PACKED>
+// (lane & 15) + 16 i][m] for line g = 64 j + 4 w + (lane >> 4) of the 512 (parity h = g / 256, half-line index
+// m = g % 256, ky = 2 m + h). A wave transforms both halves of a z line (lanes 0..31 the even half, 32..63 the odd,
+// twisted one) from a single read of the line, and both halves of a focus, which it combines into the output rows:
+// the input column is read once and every output row written once.
+//   1. 4 passes of 16 z lines along y (one per wave), each into its own LDS buffer (two, alternating) followed by a
+//      workgroup barrier and the transpose into R[.][p]: one barrier per pass;
+//   2. 8 chunks of 64 lines along z (zconv_line), 4 lines per wave, the next chunk's kernel spectrum in flight into
+//      LDS (gst);
+//   3. 4 passes of 16 foci: transpose R[.][p] into the pass's buffer, barrier, one focus per wave back along y (both
+//      halves), the rows [row0, row0 + rows) = e[y mod 256] + e^{2 pi i y / 512} o[y mod 256] stored once.
+// LDS: U holds the two y buffers (32 half lines each) in stages 1 and 3, and gst (first 64 KB) and the z lines'
+// scratch (after it) in stage 2; gst overlaps only buffer 0, so chunk 0's spectrum starts loading once pass 2's
+// transposes are done.
+constexpr int kSlabLSY = 273, kSlabLSZ = 137;          // odd strides >= sp(256), sp(128)
+constexpr int kSlabYB = 32 * kSlabLSY;                 // one y buffer: 16 lines x 2 halves
+constexpr int kSlabGst = 64 * 128;                     // one chunk's kernel spectrum: 64 lines of 1 KB
+constexpr int kSlabU = 2 * kSlabYB;
+static_assert(kSlabGst + 64 * kSlabLSZ <= kSlabU, "stage 2's LDS fits the y buffers");
+static_assert(kSlabGst <= kSlabYB, "gst overlaps only the first y buffer");
+
+template <bool PACKED>
 __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny, int nzIn, size_t inCol,
-                                               float2* __restrict__ out, int row0, int rows, int nOut, size_t outCol,
-                                               const float2* __restrict__ ghat, float2* buf, float2* gx,
-                                               float2* gst, const float2* twy, const float2* twz)
+                                          float2* __restrict__ out, int row0, int rows, int nOut, size_t outCol,
+                                          const float2* __restrict__ ghat, float2* U, float2* gx, const float2* twy,
+                                          const float2* twz)
 {
    constexpr int NY = 512, NZ = 128;
-   constexpr int WAVES = THREADS / 64, LPW = 16 / WAVES;  // z lines (foci) per wave and pass
-   constexpr int CL = 4 * WAVES, NJ = 512 / CL, NI = 4;   // lines per chunk, chunks, passes of 16 planes
-   constexpr int LSY = 273, LSZ = 137;                    // odd strides >= sp(256), sp(128)
+   constexpr int NJ = 8, NI = 4;  // chunks of 64 lines, passes of 16 planes
+   constexpr int LSY = kSlabLSY, LSZ = kSlabLSZ;
    constexpr bool packed = PACKED;
    const int tid0 = threadIdx.x;
    const int c = packed ? 0 : (int)blockIdx.x;
    const float2* src = in + (size_t)c * inCol;
    const float2* G = ghat + (size_t)c * NY * NZ;
    float2* dst = out + (size_t)c * outCol;
+   float2* gst = U;                // stage 2: the chunk's kernel spectrum
+   float2* zscr = U + kSlabGst;    // stage 2: the z lines' scratch
 // the lane ids, opaque to the compiler at each pass and chunk so that it recomputes addresses there instead of
 // keeping every unrolled iteration's addresses live
 #define SLAB_IDS                   \
    int tid = tid0;                 \
    asm volatile("" : "+v"(tid));   \
    const int lane = tid & 63, w = tid >> 6
-   // chunk j's line at position s: line g = CL j + s of the slab, parity g / 256, half-line index g % 256
-   auto mline = [](int j, int s) { return (CL * j + s) & 255; };
-   auto kyof = [&](int j, int s) { return 2 * mline(j, s) + ((CL * j) >> 8); };
+   // chunk j's line at position s: line g = 64 j + s of the slab, parity g / 256, half-line index g % 256
+   auto mline = [](int j, int s) { return (64 * j + s) & 255; };
+   auto kyof = [&](int j, int s) { return 2 * mline(j, s) + (j >> 2); };
    // the line zl of pass p: z = 16 p + zl, element y = (lane & 31) + 32 r (ny <= 256 on this path)
    auto load_line = [&](int p, int zl, int lane, float2 (&xv)[8]) {
       const int z = 16 * p + zl, q = lane & 31;
@@ -487,8 +493,8 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
             xv[r] = src[(size_t)z * ny + y];
       }
    };
-   // line (zl, h) of a pass in LDS: zl + 16 h
-   auto yline = [&](int zl, int h) { return buf + (zl + 16 * h) * LSY; };
+   // line (zl, h) of pass p's buffer: zl + 16 h
+   auto yline = [&](int p, int zl, int h) { return U + (p & 1) * kSlabYB + (zl + 16 * h) * LSY; };
    // the kernel spectrum of chunk j's lines 4 w .. 4 w + 3 (1 KB each, contiguous in HBM) straight into the wave's
    // slots of gst, one 16-byte piece per lane and line, no registers held; read after s_waitcnt vmcnt(0)
    auto stage_g = [&](int j, int w, int lane) {
@@ -503,61 +509,50 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
    };
 
    float2 R[NJ][NI];
-   // 1. y forward; the next pass's lines are loaded before this pass's transforms, chunk 0's spectrum before all
-   float2 xn[LPW][8];
+   // 1. y forward; the next pass's line is loaded before this pass's transform
+   float2 xn[8];
    {
       SLAB_IDS;
-      stage_g(0, w, lane);
-#pragma unroll
-      for (int t = 0; t < LPW; ++t)
-         load_line(0, LPW * w + t, lane, xn[t]);
+      load_line(0, w, lane, xn);
    }
 #pragma unroll
    for (int p = 0; p < NI; ++p)
    {
       SLAB_IDS;
-      float2 v[LPW][8];
+      float2 v[8];
 #pragma unroll
-      for (int t = 0; t < LPW; ++t)
-#pragma unroll
-         for (int r = 0; r < 8; ++r)
-            v[t][r] = xn[t][r];
+      for (int r = 0; r < 8; ++r)
+         v[r] = xn[r];
       if (p + 1 < NI)
+         load_line(p + 1, w, lane, xn);
+      if (16 * p < nzIn)
       {
-#pragma unroll
-         for (int t = 0; t < LPW; ++t)
-            load_line(p + 1, LPW * w + t, lane, xn[t]);
-      }
-      if (16 * p >= nzIn)
-      {
-#pragma unroll
-         for (int j = 0; j < NJ; ++j)
-            R[j][p] = make_float2(0.0f, 0.0f);
-         continue;
-      }
-      const int h = lane >> 5, q = lane & 31;
-#pragma unroll
-      for (int t = 0; t < LPW; ++t)
-      {
+         const int h = lane >> 5, q = lane & 31;
 #pragma unroll
          for (int r = 0; r < 8; ++r)
-            v[t][r] = cmulf(v[t][r], h ? twy[q + 32 * r] : make_float2(1.0f, 0.0f));
-         line_fft256<false, true, false>(yline(LPW * w + t, h), q, v[t], twy);
-         if constexpr (packed)
+            v[r] = cmulf(v[r], h ? twy[q + 32 * r] : make_float2(1.0f, 0.0f));
+         line_fft256<false, true, false>(yline(p, w, h), q, v, twy);
+      }
+      __syncthreads();
+      // buffer 0 is free once every wave has passed this barrier of pass 3 (its last readers were pass 2's
+      // transposes): chunk 0's spectrum goes into gst, inside it
+      if (p == NI - 1)
+         stage_g(0, w, lane);
+      const int s = 4 * w + (lane >> 4), zl = lane & 15;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+      {
+         if (16 * p >= nzIn)
+            R[j][p] = make_float2(0.0f, 0.0f);
+         else
          {
-            wave_sync();
-            pair_columns<true>(yline(LPW * w + t, h), h, q);
+            const float2* L = yline(p, zl, j >> 2);
+            R[j][p] = packed ? unpack_at(L, j >> 2, mline(j, s)) : L[sp(mline(j, s))];
          }
       }
-      __syncthreads();
-      {
-         const int s = 4 * w + (lane >> 4), zl = lane & 15;
-#pragma unroll
-         for (int j = 0; j < NJ; ++j)
-            R[j][p] = yline(zl, (CL * j) >> 8)[sp(mline(j, s))];
-      }
-      __syncthreads();
    }
+   // the z scratch overlaps buffer 1, which pass 3's transposes read
+   __syncthreads();
 
    // 2. z convolution, lines 4 w .. 4 w + 3 of each chunk per wave; the chunk's kernel spectrum is in gst
 #pragma unroll
@@ -571,7 +566,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
              if (j + 1 < NJ)
                 stage_g(j + 1, w, lane);
           },
-          buf + pos * LSZ, gx + pos * kGLine, lane, twz);
+          zscr + pos * LSZ, gx + pos * kGLine, lane, twz);
       // the chunk's values are formed here: without this the compiler sinks their last butterfly into the
       // (conditional) passes of stage 3 and keeps its operands, three times the registers, live until then
 #pragma unroll
@@ -580,7 +575,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
    }
    __syncthreads();
 
-   // 3. y inverse, foci 16 p + LPW w + t per wave; rows [row0, row0 + rows)
+   // 3. y inverse, focus 16 p + w per wave; rows [row0, row0 + rows)
 #pragma unroll
    for (int p = 0; p < NI; ++p)
    {
@@ -591,70 +586,58 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
          const int s = 4 * w + (lane >> 4), fl = lane & 15;
 #pragma unroll
          for (int j = 0; j < NJ; ++j)
-            yline(fl, (CL * j) >> 8)[sp(mline(j, s))] = R[j][p];
+            yline(p, fl, j >> 2)[sp(mline(j, s))] = R[j][p];
       }
       __syncthreads();
-      const int h = lane >> 5, q = lane & 31;
-#pragma unroll
-      for (int t = 0; t < LPW; ++t)
+      const int h = lane >> 5, q = lane & 31, f = 16 * p + w;
+      float2 v[8];
+      if constexpr (packed)
       {
-         if constexpr (packed)
-         {
-            pair_columns<false>(yline(LPW * w + t, h), h, q);
-            wave_sync();
-         }
-         float2 v[8];
-         line_fft256<true, false, false>(yline(LPW * w + t, h), q, v, twy);
+#pragma unroll
+         for (int r = 0; r < 8; ++r)
+            v[r] = repack_at(yline(p, w, h), h, q + 32 * r);
+         line_fft256<true, true, false>(yline(p, w, h), q, v, twy);
       }
+      else
+         line_fft256<true, false, false>(yline(p, w, h), q, v, twy);
       wave_sync();
-#pragma unroll
-      for (int t = 0; t < LPW; ++t)
+      if (f < nOut)
       {
-         const int f = 16 * p + LPW * w + t;
-         if (f < nOut)
-         {
-            float2* d = dst + (size_t)f * rows;
-            const float2* E = yline(LPW * w + t, 0);
-            const float2* O = yline(LPW * w + t, 1);
+         float2* d = dst + (size_t)f * rows;
+         const float2* E = yline(p, w, 0);
+         const float2* O = yline(p, w, 1);
 #pragma unroll 4
-            for (int yo = lane; yo < rows; yo += 64)
-            {
-               const int y = row0 + yo, m = y & 255;
-               const float2 tw = twy[y];
-               d[yo] = cmaf(O[sp(m)], make_float2(tw.x, -tw.y), E[sp(m)]);
-            }
+         for (int yo = lane; yo < rows; yo += 64)
+         {
+            const int y = row0 + yo, m = y & 255;
+            const float2 tw = twy[y];
+            d[yo] = cmaf(O[sp(m)], make_float2(tw.x, -tw.y), E[sp(m)]);
          }
       }
-      __syncthreads();
    }
 #undef SLAB_IDS
 }
 
-template <int NY, int NZ, int THREADS>
-__global__ __launch_bounds__(THREADS) void slab_kernel(const float2* __restrict__ in, int ny, int nzIn,
-                                                            size_t inCol, float2* __restrict__ out, int row0,
-                                                            int rows, int nOut, size_t outCol,
-                                                            const float2* __restrict__ ghat,
-                                                            const float2* __restrict__ twY,
-                                                            const float2* __restrict__ twZ)
+template <int NY, int NZ>
+__global__ __launch_bounds__(1024) void slab_kernel(const float2* __restrict__ in, int ny, int nzIn, size_t inCol,
+                                                    float2* __restrict__ out, int row0, int rows, int nOut,
+                                                    size_t outCol, const float2* __restrict__ ghat,
+                                                    const float2* __restrict__ twY, const float2* __restrict__ twZ)
 {
    static_assert(NY == 512 && NZ == 128, "the slab pass is written for a 512 x 128 (y, z) slab");
-   constexpr int LSY = 273, LSZ = 137, CL = THREADS / 16;
-   constexpr int BUF = 32 * LSY > CL * LSZ ? 32 * LSY : CL * LSZ;
-   __shared__ float2 buf[BUF];
-   __shared__ float2 gx[4 * kGLine];  // the self-paired lines' spectrum values (positions 0 and 1 of the packed workgroup)
-   __shared__ __attribute__((aligned(16))) float2 gst[CL * NZ];  // one chunk's kernel spectrum: CL lines of 1 KB
+   __shared__ __attribute__((aligned(16))) float2 U[kSlabU];
+   __shared__ float2 gx[4 * kGLine];  // the self-paired lines' spectrum values (position 0 of the packed workgroup)
    __shared__ float2 twy[NY];
    __shared__ float2 twz[NZ];
-   for (int i = threadIdx.x; i < NY; i += THREADS)
+   for (int i = threadIdx.x; i < NY; i += 1024)
       twy[i] = twY[i];
-   for (int i = threadIdx.x; i < NZ; i += THREADS)
+   for (int i = threadIdx.x; i < NZ; i += 1024)
       twz[i] = twZ[i];
    __syncthreads();
    if (blockIdx.x == 0)
-      slab_body<THREADS, true>(in, ny, nzIn, inCol, out, row0, rows, nOut, outCol, ghat, buf, gx, gst, twy, twz);
+      slab_body<true>(in, ny, nzIn, inCol, out, row0, rows, nOut, outCol, ghat, U, gx, twy, twz);
    else
-      slab_body<THREADS, false>(in, ny, nzIn, inCol, out, row0, rows, nOut, outCol, ghat, buf, gx, gst, twy, twz);
+      slab_body<false>(in, ny, nzIn, inCol, out, row0, rows, nOut, outCol, ghat, U, gx, twy, twz);
 }
 
 // The kernel spectrum for the slab pass: out[c][ky][kz] = ghat[kz][ky][c] for columns 1 .. NH - 1; column 0 holds
@@ -928,7 +911,7 @@ int32_t iscg_wf_conv3d(iscg_context* h, const float* density, const iscg_camera*
       }
       {
          ProfScope ps(h, "wf_slab");
-         hipLaunchKernelGGL((slab_kernel<512, 128, 1024>), dim3(d.NX / 2), dim3(1024), 0, h->stream,
+         hipLaunchKernelGGL((slab_kernel<512, 128>), dim3(d.NX / 2), dim3(1024), 0, h->stream,
                             (const float2*)x1, (int)d.ny, (int)d.nzD, s.col1, x2, (int)d.fovY0, (int)d.ch, (int)d.nF,
                             s.col2, s.ghat.as<const float2>(), s.twY.as<const float2>(), s.twZ.as<const float2>());
       }
