@@ -481,16 +481,17 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
    // chunk j's line at position s: line g = 64 j + s of the slab, parity g / 256, half-line index g % 256
    auto mline = [](int j, int s) { return (64 * j + s) & 255; };
    auto kyof = [&](int j, int s) { return 2 * mline(j, s) + (j >> 2); };
-   // the line zl of pass p: z = 16 p + zl, element y = (lane & 31) + 32 r (ny <= 256 on this path)
+   // the line zl of pass p: z = 16 p + zl, element y = (lane & 31) + 32 r (ny <= 256 on this path); every lane
+   // loads (element 0 where it is outside the rows) and keeps the value or zero, with no branch
    auto load_line = [&](int p, int zl, int lane, float2 (&xv)[8]) {
       const int z = 16 * p + zl, q = lane & 31;
 #pragma unroll
       for (int r = 0; r < 8; ++r)
       {
          const int y = q + 32 * r;
-         xv[r] = make_float2(0.0f, 0.0f);
-         if (z < nzIn && y < ny)
-            xv[r] = src[(size_t)z * ny + y];
+         const bool in = z < nzIn && y < ny;
+         const float2 x = src[in ? (uint32_t)(z * ny + y) : 0u];
+         xv[r] = in ? x : make_float2(0.0f, 0.0f);
       }
    };
    // line (zl, h) of pass p's buffer: zl + 16 h
@@ -501,7 +502,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
 #pragma unroll
       for (int i = 0; i < 4; ++i)
       {
-         const float4* s4 = reinterpret_cast<const float4*>(G + (size_t)kyof(j, 4 * w + i) * NZ) + lane;
+         const float4* s4 = reinterpret_cast<const float4*>(G) + (uint32_t)(kyof(j, 4 * w + i) * (NZ / 2) + lane);
          __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)s4,
                                           (__attribute__((address_space(3))) void*)(gst + (4 * w + i) * NZ), 16, 0,
                                           0);
@@ -539,16 +540,25 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
       if (p == NI - 1)
          stage_g(0, w, lane);
       const int s = 4 * w + (lane >> 4), zl = lane & 15;
-#pragma unroll
-      for (int j = 0; j < NJ; ++j)
+      if (16 * p >= nzIn)
       {
-         if (16 * p >= nzIn)
+#pragma unroll
+         for (int j = 0; j < NJ; ++j)
             R[j][p] = make_float2(0.0f, 0.0f);
-         else
-         {
-            const float2* L = yline(p, zl, j >> 2);
-            R[j][p] = packed ? unpack_at(L, j >> 2, mline(j, s)) : L[sp(mline(j, s))];
-         }
+      }
+      else if constexpr (packed)
+      {
+#pragma unroll
+         for (int j = 0; j < NJ; ++j)
+            R[j][p] = unpack_at(yline(p, zl, j >> 2), j >> 2, mline(j, s));
+      }
+      else
+      {
+         // element m = 64 (j % 4) + s of line (zl, j / 4) sits at sp(m) = 68 (j % 4) + sp(s): one address per pass
+         const float2* L = yline(p, zl, 0) + sp(s);
+#pragma unroll
+         for (int j = 0; j < NJ; ++j)
+            R[j][p] = L[(j >> 2) * 16 * LSY + 68 * (j & 3)];
       }
    }
    // the z scratch overlaps buffer 1, which pass 3's transposes read
@@ -584,9 +594,10 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
          break;
       {
          const int s = 4 * w + (lane >> 4), fl = lane & 15;
+         float2* L = yline(p, fl, 0) + sp(s);
 #pragma unroll
          for (int j = 0; j < NJ; ++j)
-            yline(p, fl, j >> 2)[sp(mline(j, s))] = R[j][p];
+            L[(j >> 2) * 16 * LSY + 68 * (j & 3)] = R[j][p];
       }
       __syncthreads();
       const int h = lane >> 5, q = lane & 31, f = 16 * p + w;
@@ -603,7 +614,6 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
       wave_sync();
       if (f < nOut)
       {
-         float2* d = dst + (size_t)f * rows;
          const float2* E = yline(p, w, 0);
          const float2* O = yline(p, w, 1);
 #pragma unroll 4
@@ -611,7 +621,7 @@ __device__ __forceinline__ void slab_body(const float2* __restrict__ in, int ny,
          {
             const int y = row0 + yo, m = y & 255;
             const float2 tw = twy[y];
-            d[yo] = cmaf(O[sp(m)], make_float2(tw.x, -tw.y), E[sp(m)]);
+            dst[(uint32_t)(f * rows + yo)] = cmaf(O[sp(m)], make_float2(tw.x, -tw.y), E[sp(m)]);
          }
       }
    }
