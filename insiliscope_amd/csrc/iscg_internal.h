@@ -357,17 +357,31 @@ struct ProfScope
 // see (u1 and u2 each take the 2^23 values (j + 0.5) 2^-23; the Poisson mean is (float)l with l in (0, 60], every
 // float up to 60) against the FP64 functions of the exact chain, and the fast path is used only when
 //   |g_fast - g| <= max|dr| (1 + max|dc|) + 5.77 max|dc| + 1e-15 <= kGaussErr / 2   (|r| <= sqrt(-2 log 2^-24) = 5.768)
-//   |L_fast / L - 1| <= max rel(expf) + 3.6e-6 <= kExpRel / 2         (|l - (float)l| <= 60 * 2^-24 = 3.58e-6)
+//   |L_fast / L - 1| <= max rel(exp_neg_fast) + 3.6e-6 <= kExpRel / 2  (|l - (float)l| <= 60 * 2^-24 = 3.58e-6)
 // The margins below use the full kGaussErr and kExpRel, twice the verified bounds.
 constexpr double kGaussErr = 1e-5;
-constexpr double kExpRel = 1.6e-5;
+constexpr double kExpRel = 2e-5;
 constexpr double kRMax = 5.77;
+
+// The FP32 transcendentals of the fast path are the hardware's instructions: log2, square root, cosine of turns and
+// exp2. Their accuracy is not assumed: noise_verify_device measures these very functions over every input.
+__device__ __forceinline__ float gauss_r_fast(float u1)  // sqrt(-2 ln u1)
+{
+   return __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+}
+__device__ __forceinline__ float gauss_c_fast(float u2)  // cos(2 pi u2)
+{
+   return __builtin_amdgcn_cosf(u2);
+}
+__device__ __forceinline__ float exp_neg_fast(float l)  // e^-l
+{
+   return __builtin_amdgcn_exp2f(-1.4426950408889634f * l);
+}
 
 __device__ __forceinline__ double gauss_fast(CounterRng& u)
 {
    const float u1 = (float)u.uniform(), u2 = (float)u.uniform();
-   const float r = sqrtf(-2.0f * logf(u1));
-   return (double)r * (double)cospif(2.0f * u2);  // exact product of two floats
+   return (double)gauss_r_fast(u1) * (double)gauss_c_fast(u2);  // exact product of two floats
 }
 
 // Poisson(l) with the filtered threshold; ok = false when a decision fell inside the error interval.
@@ -384,7 +398,7 @@ __device__ __forceinline__ double counter_poisson_fast(double l, CounterRng& u, 
    }
    if (!(l > 0.0))
       return 0.0;
-   const double L = (double)expf(-(float)l);
+   const double L = (double)exp_neg_fast((float)l);
    const double tol = L * kExpRel;
    double last, prev;
    const int n = knuth_fresh(L, u, last, prev);

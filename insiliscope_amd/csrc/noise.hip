@@ -161,8 +161,8 @@ __global__ __launch_bounds__(256) void verify_gauss_kernel(unsigned long long* o
    {
       const double u = ((double)j + 0.5) * 1.1920928955078125e-7;
       const float uf = (float)u;
-      const double dr = fabs((double)sqrtf(-2.0f * logf(uf)) - sqrt(-2.0 * log(u)));
-      const double dc = fabs((double)cospif(2.0f * uf) - cos(6.283185307179586 * u));
+      const double dr = fabs((double)gauss_r_fast(uf) - sqrt(-2.0 * log(u)));
+      const double dc = fabs((double)gauss_c_fast(uf) - cos(6.283185307179586 * u));
       mr = !(dr <= mr) ? dr : mr;  // NaN propagates
       mc = !(dc <= mc) ? dc : mc;
    }
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void verify_gauss_kernel(unsigned long long* o
    }
 }
 
-// out[2] = max |expf(-f) / exp(-f) - 1| over every float f in [0, 60].
+// out[2] = max |exp_neg_fast(f) / exp(-f) - 1| over every float f in [0, 60].
 __global__ __launch_bounds__(256) void verify_exp_kernel(unsigned long long* out)
 {
    constexpr uint32_t kLast = 0x42700000u;  // 60.0f
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(256) void verify_exp_kernel(unsigned long long* out
    {
       const float f = __uint_as_float(b);
       const double e = exp(-(double)f);
-      const double rel = fabs((double)expf(-f) - e) / e;
+      const double rel = fabs((double)exp_neg_fast(f) - e) / e;
       m = !(rel <= m) ? rel : m;
    }
    m = wave_max(m);
