@@ -229,7 +229,7 @@ def cpu_wf(planes):
 
 # VALU wave instructions of one splat_tiles_kernel<32> launch over C4's 2000 frames (SQ_INSTS_VALU per dispatch,
 # rocprofv3 --pmc on bench.py --workload smlm), and the VALU issue peak in G wave-instructions/s
-SPLAT_C4_VALU_INSTS = 14_856_556_313
+SPLAT_C4_VALU_INSTS = 14_033_300_000  # mean of 21 dispatches (they vary by about 2e6)
 VALU_ISSUE_PEAK_G = 256 * 2.4
 
 
