@@ -137,6 +137,32 @@ def test_wf_slab_full_size_vs_five_pass(ctx, orc, golden):
     assert rel_l2(img[0], want) < TOL
 
 
+@pytest.mark.parametrize("nzD,nF,ny,nx,fov", [(37, 45, 200, 180, (7, 11, 160, 150)), (17, 64, 256, 256, (0, 0, 0, 0)),
+                                             (64, 33, 129, 256, (0, 100, 256, 29))])
+def test_wf_slab_partial_passes_vs_five_pass(ctx, golden, nzD, nF, ny, nx, fov):
+    """The slab path with plane and focus counts that are not multiples of its 16-line passes (the last pass partly
+    empty, later passes skipped), fewer than 256 dye rows and FOV crops, against the five-pass path on a grid with NZ
+    256 (the same linear convolution): relative L2 per focus <= 1e-5."""
+    P = golden("psf_ref.npz")
+    planes = P["small_planes"]
+    rng = np.random.default_rng(nzD * 1000 + nF)
+    dens = np.where(rng.random((nzD, ny, nx)) < 0.03, 1.0 + np.floor(rng.random((nzD, ny, nx)) * 3), 0.0)
+    dens = dens.astype(np.float32)
+    cache = host.PsfKernelCache(planes, oversampling=4, zStepNm=50.0)
+    print("nzD", nzD, "nF", nF, "ny", ny, "nx", nx, "fov", fov, flush=True)
+    imgs = []
+    for fft in ((512, 512, 128), (512, 512, 256)):
+        wf = host.WidefieldStack(ctx)
+        assert wf.Plan(cache, nx, ny, nzD, 0.05, -5, 0.05 * -3 + 0.01, nF, fov=fov, fft=fft), wf.err
+        img = np.zeros(wf.image_shape, np.float32)
+        assert wf.FocusSeries(dens, outImages=img), wf.err
+        imgs.append(img)
+    assert imgs[0].shape[0] == nF
+    for f in range(nF):
+        e = rel_l2(imgs[0][f], imgs[1][f])
+        assert e <= TOL, f"focus {f}: rel L2 {e:.3g}"
+
+
 def fp64_fft_stack(orc, dens, K, dz, k0, foci, z_step_nm, grid):
     """The focus stack in FP64 by another route than the oracle's direct sum: out[f] = sum_k D[k] (*) H[f - k] as
     one zero-padded 3D FFT convolution in float64 (scipy.fft), H[m] the PlanFocus blend (the oracle's weights) of
