@@ -7,7 +7,9 @@ Default workload (`wf`): the widefield mean field of a 256 x 256 x 64 dye densit
 convolved with a 127-plane Gibson-Lanni PSF (257 x 257 at 100 nm, 50 nm apart; NA 1.45, ni 1.518, ns 1.33, 670 nm,
 Noll 5-15 +-0.05 waves) by a zero-padded 512 x 512 x 128 FP32 R2C/C2R 3D FFT, sampled at the 64 dye-plane foci, then
 sCMOS noise (read 1.5 e-, gain 2 e-/ADU, offset 100) -> 64 uint16 frames. A step is one such focus stack; the metric
-is frames (focus images) per second with the density already in HBM.
+is frames (focus images) per second with the density already in HBM. By default 1000 stacks are timed after 100
+warm-up stacks (about 0.15 s of GPU work), the sustained rate: the first few milliseconds of work on an idle GPU run
+slower (20 stacks after 3 read about 7 % lower).
 
 Secondary lines in the same JSON: the SMLM DNA-PAINT movie (512 x 512, Poisson(800) blinks/frame, Catmull-Rom splat
 from a 65 x 65 x 81 table, EMCCD noise) in frames/s, and the PSF library (512 PSFs x 41 planes of 65 x 65, M 128) in
@@ -736,8 +738,8 @@ def measure_peaks():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="wf", choices=["wf", "smlm", "psf", "dyes", "adjoint", "c1", "world"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
